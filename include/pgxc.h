@@ -1,0 +1,151 @@
+/*
+ * pgxc.h -- C ABI of the MI355X exchange-correlation grid integration.
+ *
+ * Drop-in boundary for the body of ParaGauss's calc_xcks_nl / calc_xcks
+ * (modules/xc_hamiltonian.f90:1280-1580 and :812-1036).  Everything around
+ * those routines stays in the Fortran SCF driver: xc_build's allocation and
+ * zeroing (modules/xc_hamiltonian.f90:1038-1116), xc_sndrcv (:769-798),
+ * xc_pseudo2d_sym, mixing and the checkpoint records.  The Fortran side binds
+ * these entry points with ISO_C_BINDING (paragauss-gpl_amd/fortran/pgxc_binding.f90).
+ *
+ * All arrays are plain host pointers in the reference's Fortran layouts
+ * (column-major, 0-based here).  Every entry point returns PGXC_OK (0) or a
+ * non-zero status and never aborts; pgxc_last_error() describes the failure,
+ * which the Fortran caller routes to error_handler (error_handler.f90), as
+ * ASSERT/ABORT do in the reference (include/def.h:10-18).
+ *
+ * Threading: one context = one HIP stream on the current device.  A context is
+ * not re-entrant; distinct contexts share no state.
+ */
+#ifndef PGXC_H
+#define PGXC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PGXC_OK 0
+#define PGXC_ERR_ARG 1         /* invalid argument or shape                   */
+#define PGXC_ERR_UNSUPPORTED 2 /* functional / spin / l not supported          */
+#define PGXC_ERR_HIP 3         /* HIP runtime error (message says which call)  */
+#define PGXC_ERR_NOMEM 4       /* device allocation failed                    */
+#define PGXC_ERR_STATE 5       /* call out of order (e.g. build before density) */
+
+/* Functional terms: bit (1 << e) for each enabled xc_cntrl enumerator e
+ * (modules/xc_cntrl.f90:61-104).  Terms are summed in enum order with
+ * frac = 1 (modules/xc_func.f90:284-304).  XC="PBE" selects all four
+ * (modules/xc_cntrl.f90:652-670). */
+#define PGXC_XC_XALPHA (1u << 1)
+#define PGXC_XC_PWLDAC (1u << 3)
+#define PGXC_XC_PBEX (1u << 10)
+#define PGXC_XC_PBEC (1u << 18)
+#define PGXC_XC_LDA (PGXC_XC_XALPHA | PGXC_XC_PWLDAC)
+#define PGXC_XC_PBE (PGXC_XC_XALPHA | PGXC_XC_PWLDAC | PGXC_XC_PBEX | PGXC_XC_PBEC)
+
+/* Arithmetic of the two contractions (Phi P and Phi^T W).  Basis values, the
+ * functional, per-point state, E_xc/charge and V accumulation are FP64 in
+ * every mode. */
+#define PGXC_PREC_FP64 0   /* FP64 MFMA                                     */
+#define PGXC_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation                   */
+#define PGXC_PREC_BF16X3 2 /* split-BF16 (hi*hi + hi*lo + lo*hi), FP32 acc.  */
+
+/* density modes for pgxc_set_density / pgxc_set_orbitals */
+#define PGXC_DENSITY_FULL 0        /* next build integrates P itself          */
+#define PGXC_DENSITY_INCREMENTAL 1 /* next build integrates dP = P - P_prev   */
+
+typedef struct pgxc_ctx pgxc_ctx;
+
+/*
+ * Create a context on the current HIP device.
+ *
+ * Basis (C1, the order of modules/orbitalprojection_module.f90:30-50):
+ *   shell_info[5*s .. 5*s+4] = atom, l, n_exp, n_unc, n_contr for shell s
+ *     (one shell per (unique atom, l), as unique_atom_basis_type,
+ *      modules/unique_atom_module.f90:143-168).  A shell yields
+ *      (2l+1)*(n_unc+n_contr) functions, m outer, radial function inner
+ *      (modules/orbital_module.f90:2509-2610).
+ *   exps, norms: concatenated per shell (n_exp each); norms as
+ *      unique_atom_basis_norm_ob (modules/unique_atom_module.f90:426-444).
+ *   contr: concatenated per shell, n_exp x n_contr column-major each.
+ *   atom_xyz: 3 doubles per atom (bohr).
+ * Grid: grid is npts x 4 column-major (x, y, z, w), the layout of grdpts
+ *   (modules/grid_module.f90:194,3093).  Batches are contiguous veclen-point
+ *   slices in grid order (more_grid, modules/grid_module.f90:1133-1181);
+ *   veclen = machineparameters_veclen (default 128).
+ * Replaces: the setup part of xc_build (xc_allocate/orbital_setup,
+ *   modules/xc_hamiltonian.f90:239-283) plus the device upload of grid/basis.
+ */
+int pgxc_create(pgxc_ctx **ctx, int32_t n_atoms, const double *atom_xyz,
+                int32_t n_shells, const int32_t *shell_info, const double *exps,
+                const double *norms, const double *contr, const double *grid,
+                int64_t npts, int32_t veclen);
+
+/*
+ * Density matrix P (nbas x nbas, column-major, symmetric), total density for
+ * ispin = 1.  Equivalent to eigvec_occ / occ_num_occ
+ * (modules/occupied_levels_module.f90:112-117) through P = C diag(occ) C^T.
+ * mode = PGXC_DENSITY_INCREMENTAL makes the next build integrate dP against
+ * the state left by the previous build on this context.
+ */
+int pgxc_set_density(pgxc_ctx *ctx, int32_t ispin, int32_t nbas, const double *P,
+                     int32_t mode);
+
+/* Occupied orbitals C (nbas x nocc, column-major) and occupations occ(nocc),
+ * exactly the arguments density_calc_nl reads
+ * (modules/density_calc_module.f90:1490-1540); P is formed on the device. */
+int pgxc_set_orbitals(pgxc_ctx *ctx, int32_t ispin, int32_t nbas, int32_t nocc,
+                      const double *C, const double *occ, int32_t mode);
+
+/*
+ * One XC build over all batches.  Replaces the body of calc_xcks_nl (GGA) /
+ * calc_xcks (LDA).  ham_packed has nbas*(nbas+1)/2 doubles and receives
+ * ham_xc_arr ADDED INTO (the reference accumulates into the zeroed array,
+ * modules/xc_hamiltonian.f90:1561-1567); packing (j = 1..n, k = 1..j).
+ * exc and charge are also added into (exc_int, charge_int, :1408,1423).
+ * Any of ham_packed / exc / charge may be NULL to leave the result on the
+ * device (pgxc_get_result fetches it).
+ */
+int pgxc_build(pgxc_ctx *ctx, uint32_t functional, int32_t precision,
+               double *ham_packed, double *exc, double *charge);
+
+/* Copy the last build's results (same semantics as pgxc_build's outputs). */
+int pgxc_get_result(pgxc_ctx *ctx, double *ham_packed, double *exc, double *charge);
+
+/* Number of basis functions of the context (dims(1) in C1). */
+int64_t pgxc_nbas(const pgxc_ctx *ctx);
+
+/* Work counters of the last build: [0] points, [1] batches, [2] algorithmic
+ * flops (4 B n^2 + 16 B n summed over batches, n = screened functions),
+ * [3] exp evaluations, [4] HBM bytes of the algorithmic model. */
+int pgxc_last_stats(const pgxc_ctx *ctx, double *stats5);
+
+/* Device timing of the last build's kernels in milliseconds, by stage:
+ * [0] basis, [1] density contraction, [2] functional, [3] potential
+ * contraction, [4] reductions/packing, [5] whole build. */
+int pgxc_last_timing(const pgxc_ctx *ctx, double *ms6);
+
+const char *pgxc_last_error(const pgxc_ctx *ctx);
+
+/*
+ * Grid generation: Becke partition of atom-centred one-centre weights.
+ * Replaces becke0 (modules/grid_module.f90:2442-2497) with the step function
+ * mu3 = p(p(p(mu))) (autodiff/becke_step_func.f90:84-124), atomic-size
+ * adjustment off (art = 0, modules/grid_module.f90:2344-2347).
+ *   pts: npts x 3 column-major; owner[p]: atom hosting point p;
+ *   w_atomic[p]: its one-centre weight w_ang * w_rad * 4 pi
+ *   (modules/grid_module.f90:426-433); w_out[p] = w_atomic[p] * P_owner / sum_i P_i.
+ * Context-free; runs on the current device.
+ */
+int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64_t npts,
+                       const double *pts, const int32_t *owner, const double *w_atomic,
+                       double *w_out);
+
+int pgxc_destroy(pgxc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PGXC_H */

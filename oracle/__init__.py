@@ -1,0 +1,202 @@
+"""ctypes view of the CPU FP64 oracle (oracle/liboracle.so) and of the
+reference build (oracle/_ref/libpgref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+Reference lines each oracle function follows are listed in pgxc_oracle.c.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+XC_XALPHA, XC_PWLDAC, XC_PBEX, XC_PBEC = 1, 3, 10, 18
+MASK_PBE = (1 << XC_XALPHA) | (1 << XC_PWLDAC) | (1 << XC_PBEX) | (1 << XC_PBEC)
+MASK_LDA = (1 << XC_XALPHA) | (1 << XC_PWLDAC)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run make -f oracle/Makefile")
+        L = ctypes.CDLL(path)
+        L.or_sh_point.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.or_solid_harmonics.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.or_prim_norm.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.or_prim_norm.restype = ctypes.c_double
+        L.or_nbas.argtypes = [ctypes.c_int, _ip]
+        L.or_nbas.restype = ctypes.c_int
+        L.or_basis_eval.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _dp,
+                                    ctypes.c_int, _dp, ctypes.c_long, _dp, _dp]
+        for name in ("or_xalpha", "or_pw92"):
+            getattr(L, name).argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        for name in ("or_pbex", "or_pbec"):
+            getattr(L, name).argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.or_functional.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.or_functional.restype = ctypes.c_int
+        L.or_xc_build.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _dp,
+                                  _dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                  ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
+        L.or_xc_build.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def ref_lib():
+    """The compiled reference modules, or None where they were not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libpgref.so")
+        if not os.path.exists(path):
+            return None
+        L = ctypes.CDLL(path, mode=os.RTLD_LAZY)
+        L.pgref_pw92.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.pgref_pbec.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.pgref_solid_harmonics.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+        _REF = L
+    return _REF
+
+
+# ---------------------------------------------------------------- functionals
+def _term1(name, rho):
+    rho = _c(rho)
+    f, fr = np.zeros_like(rho), np.zeros_like(rho)
+    getattr(lib(), name)(rho.size, _d(rho), _d(f), _d(fr))
+    return f, fr
+
+
+def _term2(name, rho, gam):
+    rho, gam = _c(rho), _c(gam)
+    f, fr, fg = np.zeros_like(rho), np.zeros_like(rho), np.zeros_like(rho)
+    getattr(lib(), name)(rho.size, _d(rho), _d(gam), _d(f), _d(fr), _d(fg))
+    return f, fr, fg
+
+
+def xalpha(rho):
+    return _term1("or_xalpha", rho)
+
+
+def pw92(rho):
+    return _term1("or_pw92", rho)
+
+
+def pbex(rho, gam):
+    return _term2("or_pbex", rho, gam)
+
+
+def pbec(rho, gam):
+    return _term2("or_pbec", rho, gam)
+
+
+def functional(mask, rho, gam):
+    rho, gam = _c(rho), _c(gam)
+    f, fr, fg = np.zeros_like(rho), np.zeros_like(rho), np.zeros_like(rho)
+    rc = lib().or_functional(mask, rho.size, _d(rho), _d(gam), _d(f), _d(fr), _d(fg))
+    if rc != 0:
+        raise ValueError("or_functional: unsupported functional mask %#x (%d)" % (mask, rc))
+    return f, fr, fg
+
+
+def ref_pw92(rho):
+    rho = _c(rho)
+    f, fr = np.zeros_like(rho), np.zeros_like(rho)
+    ref_lib().pgref_pw92(rho.size, _d(rho), _d(f), _d(fr))
+    return f, fr
+
+
+def ref_pbec(rho, gam):
+    rho, gam = _c(rho), _c(gam)
+    f, fr, fg = np.zeros_like(rho), np.zeros_like(rho), np.zeros_like(rho)
+    ref_lib().pgref_pbec(rho.size, _d(rho), _d(gam), _d(f), _d(fr), _d(fg))
+    return f, fr, fg
+
+
+# ------------------------------------------------------------ solid harmonics
+def solid_harmonics(lmax, rel_xyz):
+    """rel_xyz (npts,3) -> C (nlm,npts), G (nlm,3,npts)."""
+    xyz = np.asfortranarray(rel_xyz, dtype=np.float64)
+    npts = xyz.shape[0]
+    nlm = (lmax + 1) ** 2
+    C = np.zeros((nlm, npts))
+    G = np.zeros((nlm, 3, npts))
+    lib().or_solid_harmonics(lmax, npts, xyz.ctypes.data_as(_dp), _d(C), _d(G))
+    return C, G
+
+
+def ref_solid_harmonics(lmax, pts, center):
+    pts = np.asfortranarray(pts, dtype=np.float64)
+    npts = pts.shape[0]
+    nlm = (lmax + 1) ** 2
+    C = np.zeros((nlm, npts))          # Fortran c(npts, nlm)
+    G = np.zeros((nlm, 3, npts))       # Fortran g(npts, 3, nlm)
+    ctr = _c(center)
+    ref_lib().pgref_solid_harmonics(npts, lmax, pts.ctypes.data_as(_dp), _d(ctr), _d(C), _d(G))
+    return C, G
+
+
+def prim_norm(L, alpha):
+    return lib().or_prim_norm(L, alpha)
+
+
+# ----------------------------------------------------------------- basis, XC
+def _basis_args(basis):
+    info = _c(basis.shell_info, np.int32)
+    return (basis.n_atoms, _c(basis.atom_xyz), basis.n_shells, info,
+            _c(basis.exps), _c(basis.norms), _c(basis.contr))
+
+
+def basis_eval(basis, pts):
+    """pts (npts,3) -> phi (nbas,npts), grad (nbas,3,npts)."""
+    na, axyz, ns, info, ex, nm, ct = _basis_args(basis)
+    P = np.asfortranarray(pts, dtype=np.float64)
+    npts = P.shape[0]
+    n = basis.nbas
+    phi = np.zeros((n, npts))
+    grad = np.zeros((n, 3, npts))
+    lib().or_basis_eval(na, _d(axyz), ns, info.ctypes.data_as(_ip), _d(ex), _d(nm), _d(ct),
+                        npts, P.ctypes.data_as(_dp), npts, _d(phi), _d(grad))
+    return phi, grad
+
+
+def xc_build(basis, grid, mask, P=None, C=None, occ=None, veclen=128,
+             batch_range=(0, -1), nthreads=1):
+    """Reference-order FP64 XC build.  grid: (npts,4) x,y,z,w.
+    Returns (ham_packed, exc, charge)."""
+    na, axyz, ns, info, ex, nm, ct = _basis_args(basis)
+    G = np.asfortranarray(grid, dtype=np.float64)
+    npts = G.shape[0]
+    n = basis.nbas
+    if P is not None:
+        dform, dens, nocc, oc = 0, np.asfortranarray(P, dtype=np.float64), 0, np.zeros(1)
+    else:
+        dform, dens = 1, np.asfortranarray(C, dtype=np.float64)
+        nocc, oc = dens.shape[1], _c(occ)
+    ham = np.zeros(n * (n + 1) // 2)
+    e = np.zeros(1)
+    q = np.zeros(1)
+    rc = lib().or_xc_build(na, _d(axyz), ns, info.ctypes.data_as(_ip), _d(ex), _d(nm), _d(ct),
+                           G.ctypes.data_as(_dp), npts, veclen, mask, dform,
+                           dens.ctypes.data_as(_dp), nocc, _d(oc), _d(ham), _d(e), _d(q),
+                           batch_range[0], batch_range[1], nthreads)
+    if rc != 0:
+        raise ValueError("or_xc_build failed: %d" % rc)
+    return ham, float(e[0]), float(q[0])

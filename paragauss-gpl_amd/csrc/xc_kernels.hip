@@ -1,0 +1,810 @@
+// HIP kernels of the XC build for gfx950 (CDNA4, wave64).
+//
+//   k_basis       basis values and gradients on a chunk of grid points
+//                 (ParaGauss modules/orbital_module.f90:1742-1932, 2421-2619;
+//                  solid harmonics modules/solid_harmonics_module.f90:369-568)
+//   k_density     X = Phi D on MFMA, fused reduction rho = sum Phi X,
+//                 grad rho = 2 sum dPhi X  (modules/density_calc_module.f90:1478-1577)
+//   k_functional  Slater + PW92 + PBE-x + PBE-c, FP64, closed-form derivatives
+//                 (modules/xc_func.f90:187-305 and the term modules)
+//   k_wmat        W = a Phi + b . grad Phi  (modules/xc_hamiltonian.f90:1429-1483)
+//   k_potential   H = W^T Phi on MFMA, split over points  (:1528-1531)
+//   k_pack        V = H + H^T into the packed lower triangle (:1561-1567)
+#include "pgxc_internal.h"
+
+namespace pgxc {
+
+// ------------------------------------------------------------------ helpers
+// Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
+// contiguous range of tile ids so that tiles sharing an operand panel share
+// that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  const int per = nblk >> 3;
+  if (bid >= (per << 3)) return bid;
+  return (bid & 7) * per + (bid >> 3);
+}
+
+template <typename T>
+struct MF;
+template <>
+struct MF<double> {
+  typedef d4 acc;
+  __device__ static __forceinline__ acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MF<float> {
+  typedef f4 acc;
+  __device__ static __forceinline__ acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// 16x16x4 MFMA operand/result placement (wave64):
+//   A(i,k): lane = i + 16 k      B(k,j): lane = j + 16 k
+//   D(i,j): lane = j + 16 (i / 4), element i % 4
+__device__ __forceinline__ int mf_row(int lane, int r) { return ((lane >> 4) << 2) + r; }
+__device__ __forceinline__ int mf_col(int lane) { return lane & 15; }
+
+// Block tile BM x BN over K in steps of BK; 4 waves arranged 2 x 2, wave tile
+// (BM/2) x (BN/2) made of 16x16 MFMA blocks.  Two LDS stages, next stage
+// prefetched into registers while the current one feeds the MFMAs.
+//   AKC: A(m,k) = A[m*sA + k] (k contiguous) else A[m + k*sA]
+//   BKC: B(k,n) = B[n*sB + k] (k contiguous) else B[k*sB + n]
+template <typename T, int BM, int BN, int BK, bool AKC, bool BKC>
+__device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
+                                          const T *__restrict__ B, long sB, int ksteps,
+                                          T *As, T *Bs,
+                                          typename MF<T>::acc (&acc)[BM / 32][BN / 32]) {
+  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+  constexpr int LA = BM * BK / kThreads, LB = BN * BK / kThreads;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  T ra[LA], rb[LB];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = typename MF<T>::acc{0, 0, 0, 0};
+
+  auto load = [&](int kt) {
+    const long kb = (long)kt * BK;
+#pragma unroll
+    for (int e = 0; e < LA; ++e) {
+      const int idx = tid + kThreads * e;
+      if (AKC) {
+        const int mm = idx / BK, kk = idx % BK;
+        ra[e] = A[(long)mm * sA + kb + kk];
+      } else {
+        const int mm = idx % BM, kk = idx / BM;
+        ra[e] = A[mm + (kb + kk) * sA];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < LB; ++e) {
+      const int idx = tid + kThreads * e;
+      if (BKC) {
+        const int nn = idx / BK, kk = idx % BK;
+        rb[e] = B[(long)nn * sB + kb + kk];
+      } else {
+        const int nn = idx % BN, kk = idx / BN;
+        rb[e] = B[(kb + kk) * sB + nn];
+      }
+    }
+  };
+  auto store = [&](int buf) {
+    T *as = As + buf * BK * BM, *bs = Bs + buf * BK * BN;
+#pragma unroll
+    for (int e = 0; e < LA; ++e) {
+      const int idx = tid + kThreads * e;
+      const int mm = AKC ? idx / BK : idx % BM, kk = AKC ? idx % BK : idx / BM;
+      as[kk * BM + mm] = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < LB; ++e) {
+      const int idx = tid + kThreads * e;
+      const int nn = BKC ? idx / BK : idx % BN, kk = BKC ? idx % BK : idx / BN;
+      bs[kk * BN + nn] = rb[e];
+    }
+  };
+
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < ksteps; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < ksteps) load(kt + 1);
+    const T *as = As + cur * BK * BM, *bs = Bs + cur * BK * BN;
+#pragma unroll
+    for (int k4 = 0; k4 < BK / 4; ++k4) {
+      const int kr = k4 * 4 + (lane >> 4);
+      T a[TM], b[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) a[i] = as[kr * BM + wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) b[j] = bs[kr * BN + wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = MF<T>::mma(a[i], b[j], acc[i][j]);
+    }
+    if (kt + 1 < ksteps) store(cur ^ 1);
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------- solid harmonics
+// Racah-normalised real solid harmonics up to L and their gradients, by the
+// reference's recursion in l (m = 1 is z-like, then (2m, 2m+1) pairs, top
+// pair by the (x, y) rotation); gradients by differentiating the recursion.
+template <int L>
+__device__ __forceinline__ void sh_eval(double x, double y, double z, double *C, double *Cx,
+                                        double *Cy, double *Cz) {
+  const double r2 = x * x + y * y + z * z;
+  C[0] = 1.0;
+  Cx[0] = Cy[0] = Cz[0] = 0.0;
+  if constexpr (L >= 1) {
+    C[1] = z; C[2] = x; C[3] = y;
+    Cx[1] = 0.0; Cy[1] = 0.0; Cz[1] = 1.0;
+    Cx[2] = 1.0; Cy[2] = 0.0; Cz[2] = 0.0;
+    Cx[3] = 0.0; Cy[3] = 1.0; Cz[3] = 0.0;
+  }
+  const double dr2x = 2.0 * x, dr2y = 2.0 * y, dr2z = 2.0 * z;
+#pragma unroll
+  for (int l = 2; l <= L; ++l) {
+    const int o = l * l, o1 = (l - 1) * (l - 1), o2 = (l - 2) * (l - 2);
+    const double lr = l, l1r = l - 1, ll1r = 2 * l - 1;
+    C[o] = (ll1r * z * C[o1] - l1r * r2 * C[o2]) / lr;
+    Cx[o] = (ll1r * z * Cx[o1] - l1r * (dr2x * C[o2] + r2 * Cx[o2])) / lr;
+    Cy[o] = (ll1r * z * Cy[o1] - l1r * (dr2y * C[o2] + r2 * Cy[o2])) / lr;
+    Cz[o] = (ll1r * (C[o1] + z * Cz[o1]) - l1r * (dr2z * C[o2] + r2 * Cz[o2])) / lr;
+#pragma unroll
+    for (int im = 1; im <= l - 2; ++im) {
+      const double sim = (double)(im * im);
+      const double rlm = 1.0 / sqrt(lr * lr - sim);
+      const double rl1m = sqrt(l1r * l1r - sim);
+      const double a = ll1r * z * rlm, b = rl1m * r2 * rlm;
+      const double daz = ll1r * rlm, db = rl1m * rlm;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int m = 2 * im + k - 1;
+        C[o + m] = a * C[o1 + m] - b * C[o2 + m];
+        Cx[o + m] = a * Cx[o1 + m] - b * Cx[o2 + m] - db * dr2x * C[o2 + m];
+        Cy[o + m] = a * Cy[o1 + m] - b * Cy[o2 + m] - db * dr2y * C[o2 + m];
+        Cz[o + m] = a * Cz[o1 + m] + daz * C[o1 + m] - b * Cz[o2 + m] - db * dr2z * C[o2 + m];
+      }
+    }
+    {
+      const double sim = (double)((l - 1) * (l - 1));
+      const double rlm = 1.0 / sqrt(lr * lr - sim);
+      const double a = ll1r * z * rlm, daz = ll1r * rlm;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int m = 2 * l - 3 + k;
+        C[o + m] = a * C[o1 + m];
+        Cx[o + m] = a * Cx[o1 + m];
+        Cy[o + m] = a * Cy[o1 + m];
+        Cz[o + m] = a * Cz[o1 + m] + daz * C[o1 + m];
+      }
+    }
+    {
+      const double rll = sqrt(ll1r / (2.0 * lr));
+      const int i2 = o1 + 2 * l - 3, i1 = o1 + 2 * l - 2;
+      C[o + 2 * l - 1] = (x * C[i2] - y * C[i1]) * rll;
+      C[o + 2 * l] = (y * C[i2] + x * C[i1]) * rll;
+      Cx[o + 2 * l - 1] = (C[i2] + x * Cx[i2] - y * Cx[i1]) * rll;
+      Cy[o + 2 * l - 1] = (x * Cy[i2] - C[i1] - y * Cy[i1]) * rll;
+      Cz[o + 2 * l - 1] = (x * Cz[i2] - y * Cz[i1]) * rll;
+      Cx[o + 2 * l] = (y * Cx[i2] + C[i1] + x * Cx[i1]) * rll;
+      Cy[o + 2 * l] = (C[i2] + y * Cy[i2] + x * Cy[i1]) * rll;
+      Cz[o + 2 * l] = (y * Cz[i2] + x * Cz[i1]) * rll;
+    }
+  }
+}
+
+// ------------------------------------------------------------------- basis
+template <int L, bool GRAD, typename T>
+__device__ __forceinline__ void put_fn(long f0, int nf, double r0, double r1, double x,
+                                       double y, double z, const double *C, const double *Cx,
+                                       const double *Cy, const double *Cz, T *phi, long Mc,
+                                       long cs, int p) {
+  constexpr int o = L * L;
+#pragma unroll
+  for (int m = 0; m < 2 * L + 1; ++m) {
+    const long q = (f0 + (long)m * nf) * Mc + p;
+    const double sv = C[o + m];
+    phi[q] = (T)(sv * r0);
+    if (GRAD) {
+      const double hv = sv * r1;
+      phi[cs + q] = (T)(x * hv + Cx[o + m] * r0);
+      phi[2 * cs + q] = (T)(y * hv + Cy[o + m] * r0);
+      phi[3 * cs + q] = (T)(z * hv + Cz[o + m] * r0);
+    }
+  }
+}
+
+template <int L, bool GRAD, typename T>
+__device__ __forceinline__ void shell_emit(const ShellD &sh, const double *__restrict__ exps,
+                                           const double *__restrict__ norms,
+                                           const double *__restrict__ contr, double x, double y,
+                                           double z, double r2, const double *C,
+                                           const double *Cx, const double *Cy,
+                                           const double *Cz, T *phi, long Mc, long cs, int p) {
+  const int ne = sh.n_exp, nu = sh.n_unc, nc = sh.n_contr, nf = nu + nc;
+  double rc0[kMaxContr], rc1[kMaxContr];
+#pragma unroll
+  for (int k = 0; k < kMaxContr; ++k) rc0[k] = rc1[k] = 0.0;
+  for (int e = 0; e < ne; ++e) {
+    const double alpha = exps[sh.exp_off + e];
+    const double ar2 = alpha * r2;
+    double r0 = 0.0, r1 = 0.0;
+    if (ar2 < 50.0) {  // primitive cut-off of radial1()
+      r0 = norms[sh.exp_off + e] * exp(-ar2);
+      r1 = (-2.0 * alpha) * r0;
+    }
+    if (e < nu) put_fn<L, GRAD>(sh.func_off + e, nf, r0, r1, x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p);
+#pragma unroll
+    for (int k = 0; k < kMaxContr; ++k)
+      if (k < nc) {
+        const double c = contr[sh.contr_off + k * ne + e];
+        rc0[k] += r0 * c;
+        rc1[k] += r1 * c;
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < kMaxContr; ++k)
+    if (k < nc)
+      put_fn<L, GRAD>(sh.func_off + nu + k, nf, rc0[k], rc1[k], x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p);
+}
+
+template <int LA, bool GRAD, typename T>
+__device__ __forceinline__ void basis_atom(const AtomD &at, const ShellD *__restrict__ shells,
+                                           const double *exps, const double *norms,
+                                           const double *contr, double x, double y, double z,
+                                           T *phi, long Mc, long cs, int p) {
+  constexpr int NLM = (LA + 1) * (LA + 1);
+  double C[NLM], Cx[NLM], Cy[NLM], Cz[NLM];
+  sh_eval<LA>(x, y, z, C, Cx, Cy, Cz);
+  const double r2 = x * x + y * y + z * z;
+  for (int s = at.shell_begin; s < at.shell_end; ++s) {
+    const ShellD sh = shells[s];
+    switch (sh.l) {
+      case 0: shell_emit<0, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p); break;
+      case 1: if constexpr (LA >= 1) shell_emit<1, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p); break;
+      case 2: if constexpr (LA >= 2) shell_emit<2, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p); break;
+      case 3: if constexpr (LA >= 3) shell_emit<3, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p); break;
+      case 4: if constexpr (LA >= 4) shell_emit<4, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p); break;
+    }
+  }
+}
+
+// grid: (Mc / 256, n_atoms); one thread per (point, atom)
+template <bool GRAD, typename T>
+__global__ __launch_bounds__(kThreads) void k_basis(const double *__restrict__ gx,
+                                                    const double *__restrict__ gy,
+                                                    const double *__restrict__ gz, long p0,
+                                                    const AtomD *__restrict__ atoms,
+                                                    const ShellD *__restrict__ shells,
+                                                    const double *__restrict__ exps,
+                                                    const double *__restrict__ norms,
+                                                    const double *__restrict__ contr, T *phi,
+                                                    long Mc, long NP) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  const AtomD at = atoms[blockIdx.y];
+  const double x = gx[p0 + p] - at.x, y = gy[p0 + p] - at.y, z = gz[p0 + p] - at.z;
+  const long cs = NP * Mc;
+  switch (at.lmax) {
+    case 0: basis_atom<0, GRAD>(at, shells, exps, norms, contr, x, y, z, phi, Mc, cs, p); break;
+    case 1: basis_atom<1, GRAD>(at, shells, exps, norms, contr, x, y, z, phi, Mc, cs, p); break;
+    case 2: basis_atom<2, GRAD>(at, shells, exps, norms, contr, x, y, z, phi, Mc, cs, p); break;
+    case 3: basis_atom<3, GRAD>(at, shells, exps, norms, contr, x, y, z, phi, Mc, cs, p); break;
+    default: basis_atom<4, GRAD>(at, shells, exps, norms, contr, x, y, z, phi, Mc, cs, p); break;
+  }
+}
+
+template <typename T>
+void launch_basis(hipStream_t s, bool grad, const double *gx, const double *gy,
+                  const double *gz, long p0, long Mc, int n_atoms, const AtomD *atoms,
+                  const ShellD *shells, const double *exps, const double *norms,
+                  const double *contr, T *phi, long NP) {
+  dim3 grid((unsigned)(Mc / kThreads), (unsigned)n_atoms);
+  if (grad)
+    hipLaunchKernelGGL((k_basis<true, T>), grid, dim3(kThreads), 0, s, gx, gy, gz, p0, atoms,
+                       shells, exps, norms, contr, phi, Mc, NP);
+  else
+    hipLaunchKernelGGL((k_basis<false, T>), grid, dim3(kThreads), 0, s, gx, gy, gz, p0, atoms,
+                       shells, exps, norms, contr, phi, Mc, NP);
+}
+template void launch_basis<double>(hipStream_t, bool, const double *, const double *,
+                                   const double *, long, long, int, const AtomD *,
+                                   const ShellD *, const double *, const double *,
+                                   const double *, double *, long);
+template void launch_basis<float>(hipStream_t, bool, const double *, const double *,
+                                  const double *, long, long, int, const AtomD *,
+                                  const ShellD *, const double *, const double *,
+                                  const double *, float *, long);
+
+// ----------------------------------------------------------------- density
+constexpr int kDBM = 128, kDBN = 128, kDBK = 16;
+int density_tiles(int NP) { return NP / kDBN; }
+
+// One workgroup: 128 points x 128 functions of X = Phi D, then the partial
+// contractions of that column tile for each point.
+template <typename T, bool GRAD>
+__global__ __launch_bounds__(kThreads) void k_density(const T *__restrict__ phi,
+                                                      const T *__restrict__ D, long Mc, int NP,
+                                                      double *__restrict__ part, int ntn) {
+  constexpr int BM = kDBM, BN = kDBN, BK = kDBK, WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  __shared__ T As[2 * BK * BM];
+  __shared__ T Bs[2 * BK * BN];
+  __shared__ double red[2][4][BM];
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tm = tile / ntn, tn = tile % ntn;
+  const long m0 = (long)tm * BM;
+  const int n0 = tn * BN;
+  typename MF<T>::acc acc[TM][TN];
+  gemm_tile<T, BM, BN, BK, false, false>(phi + m0, Mc, D + n0, NP, NP / BK, As, Bs, acc);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const long cs = (long)NP * Mc;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = wm * WM + i * 16 + mf_row(lane, r);
+      const long p = m0 + row;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const long n = n0 + wn * WN + j * 16 + mf_col(lane);
+        const double xv = (double)acc[i][j][r];
+        const long q = n * Mc + p;
+        s0 += (double)phi[q] * xv;
+        if (GRAD) {
+          s1 += 2.0 * (double)phi[cs + q] * xv;
+          s2 += 2.0 * (double)phi[2 * cs + q] * xv;
+          s3 += 2.0 * (double)phi[3 * cs + q] * xv;
+        }
+      }
+#pragma unroll
+      for (int off = 8; off >= 1; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        if (GRAD) {
+          s1 += __shfl_xor(s1, off);
+          s2 += __shfl_xor(s2, off);
+          s3 += __shfl_xor(s3, off);
+        }
+      }
+      if ((lane & 15) == 0) {
+        red[wn][0][row] = s0;
+        if (GRAD) {
+          red[wn][1][row] = s1;
+          red[wn][2][row] = s2;
+          red[wn][3][row] = s3;
+        }
+      }
+    }
+  __syncthreads();
+  constexpr int NC = GRAD ? 4 : 1;
+  for (int idx = tid; idx < NC * BM; idx += kThreads) {
+    const int c = idx / BM, row = idx % BM;
+    part[((long)tn * 4 + c) * Mc + m0 + row] = red[0][c][row] + red[1][c][row];
+  }
+}
+
+template <typename T>
+void launch_density(hipStream_t s, bool grad, const T *phi, const T *D, long Mc, int NP,
+                    double *part) {
+  const int ntn = NP / kDBN, ntm = (int)(Mc / kDBM);
+  dim3 grid((unsigned)(ntm * ntn));
+  if (grad)
+    hipLaunchKernelGGL((k_density<T, true>), grid, dim3(kThreads), 0, s, phi, D, Mc, NP, part, ntn);
+  else
+    hipLaunchKernelGGL((k_density<T, false>), grid, dim3(kThreads), 0, s, phi, D, Mc, NP, part, ntn);
+}
+template void launch_density<double>(hipStream_t, bool, const double *, const double *, long,
+                                     int, double *);
+template void launch_density<float>(hipStream_t, bool, const float *, const float *, long, int,
+                                    double *);
+
+// -------------------------------------------------------------- functional
+// Closed-shell terms; each returns F, dF/drho, dF/dgamma added into (f, fr, fg).
+namespace xcf {
+constexpr double PI = 3.14159265358979324;
+constexpr double DEPS = 1.0e-30;
+
+// Slater exchange: -(3/4)(3/pi)^(1/3) rho^(4/3), zero for rho <= 0
+__device__ __forceinline__ void xalpha(double rho, double &f, double &fr) {
+  const double xa = -(3.0 / 4.0) * cbrt(3.0 / PI);
+  if (rho > 0.0) {
+    const double c = cbrt(rho);
+    f += xa * (rho * c);
+    fr += xa * ((4.0 / 3.0) * c);
+  }
+}
+
+// PW92 paramagnetic G(rs) and dG/drs
+__device__ __forceinline__ void pw92_g(double rs, double &E, double &dE) {
+  const double a = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382,
+               b4 = 0.49294;
+  const double s = sqrt(rs);
+  const double Q0 = -(2.0 * a * (1.0 + a1 * rs));
+  const double Q1 = 2.0 * a * (b1 * s + b2 * rs + b3 * rs * s + b4 * rs * rs);
+  const double Q2 = log(1.0 + 1.0 / Q1);
+  const double dQ1 = 2.0 * a * (b1 * 0.5 / s + b2 + 1.5 * b3 * s + 2.0 * b4 * rs);
+  const double dQ2 = (-dQ1 / (Q1 * Q1)) / (1.0 + 1.0 / Q1);
+  E = Q0 * Q2;
+  dE = (-2.0 * a * a1) * Q2 + Q0 * dQ2;
+}
+
+// n * eps_c^PW92(n, zeta = 0) with the n < DEPS clamp; value and dn
+__device__ __forceinline__ void pw92_unpol(double n, double &F, double &dF) {
+  const double crs = cbrt(3.0 / (4.0 * PI));
+  double nn = n;
+  bool fixed = false;
+  if (nn < DEPS) { nn = DEPS; fixed = true; }
+  const double rs = crs / cbrt(nn);
+  double E, dE;
+  pw92_g(rs, E, dE);
+  F = nn * E;
+  dF = fixed ? 0.0 : E + nn * dE * (-rs / (3.0 * nn));
+}
+
+__device__ __forceinline__ void pwldac(double rho, double &f, double &fr) {
+  double F, dF;
+  pw92_unpol(0.5 * rho + 0.5 * rho, F, dF);
+  f += F;
+  fr += dF;
+}
+
+// PBE exchange excess e_x G(s^2), G = kappa mu s2 / (kappa + mu s2)
+__device__ __forceinline__ void pbex(double rho, double gam, double &f, double &fr,
+                                     double &fg) {
+  const double xa = -(3.0 / 4.0) * cbrt(3.0 / PI);
+  const double kappa = 0.804, mu = 0.066725 * (PI * PI / 3.0);
+  const double s2_fact = 1.0 / (4.0 * cbrt(3.0 * PI * PI) * cbrt(3.0 * PI * PI));
+  double ex = 0.0, dex = 0.0;
+  if (rho > 0.0) {
+    const double c = cbrt(rho);
+    ex = xa * (rho * c);
+    dex = xa * ((4.0 / 3.0) * c);
+  }
+  double s2 = 0.0, s2r = 0.0, s2g = 0.0;
+  if (rho >= 1.0e-20) {
+    const double c = cbrt(rho);
+    s2g = s2_fact / (rho * rho * c * c);
+    s2 = gam * s2g;
+    s2r = -(8.0 / 3.0) * s2 / rho;
+  }
+  const double den = kappa + mu * s2;
+  const double G = kappa * mu * s2 / den;
+  const double dG = (kappa * kappa * mu) / (den * den);
+  f += ex * G;
+  fr += dex * G + ex * dG * s2r;
+  fg += ex * dG * s2g;
+}
+
+// PBE correlation gradient term n H(n, gamma), zeta = 0 (phi = 1)
+__device__ __forceinline__ void pbec(double rho, double g2, double &f, double &fr,
+                                     double &fg) {
+  const double beta = 0.066725, gc = 0.03109069086965489503494086371;
+  const double bg = beta / gc;
+  const double K = PI / (16.0 * cbrt(3.0 * PI * PI));
+  const double nv = 0.5 * rho + 0.5 * rho;
+  if (nv <= 0.0) return;
+  double Fl, dFl;
+  pw92_unpol(nv, Fl, dFl);
+  const double El = Fl / nv, dEl = dFl / nv - Fl / (nv * nv);
+  // phi = (za^(2/3) + zb^(2/3)) / 2^(1/3) = 1 for za = zb = 1/2
+  const double fi3 = 1.0;
+  const double Ex = exp(-El / (gc * fi3));
+  const double dd = Ex - 1.0 + DEPS;
+  const double A = bg / dd;
+  const double dA = (bg / (dd * dd)) * Ex * (dEl / (gc * fi3));
+  const double c = cbrt(nv);
+  const double n73 = nv * nv * c;
+  const double D = n73 + DEPS;
+  const double t2 = g2 / D * K;
+  const double dt2n = -(g2 * K / (D * D)) * (7.0 / 3.0) * (nv * c);
+  const double dt2g = K / D;
+  const double x = A * t2;
+  const double dxn = dA * t2 + A * dt2n, dxg = A * dt2g;
+  const double q = 1.0 + x + x * x;
+  const double rat = (1.0 + x) / q;
+  const double drat = -x * (2.0 + x) / (q * q);
+  const double arg = 1.0 + bg * t2 * rat;
+  const double H = gc * fi3 * log(arg);
+  const double pre = gc * fi3 / arg * bg;
+  f += nv * H;
+  fr += H + nv * pre * (dt2n * rat + t2 * drat * dxn);
+  fg += nv * pre * (dt2g * rat + t2 * drat * dxg);
+}
+}  // namespace xcf
+
+long functional_blocks(long Mc) { return Mc / kThreads; }
+
+template <bool GGA>
+__global__ __launch_bounds__(kThreads) void k_functional(
+    const double *__restrict__ part, int ntiles, long Mc, const double *__restrict__ w, long p0,
+    uint32_t mask, int incremental, double *__restrict__ state, long npad,
+    double *__restrict__ coef, double *__restrict__ epart, long blk_off) {
+  __shared__ double se[kThreads], sq[kThreads];
+  const int tid = threadIdx.x;
+  const long p = (long)blockIdx.x * kThreads + tid, gp = p0 + p;
+  double rho = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  for (int t = 0; t < ntiles; ++t) {
+    rho += part[((long)t * 4 + 0) * Mc + p];
+    if (GGA) {
+      g0 += part[((long)t * 4 + 1) * Mc + p];
+      g1 += part[((long)t * 4 + 2) * Mc + p];
+      g2 += part[((long)t * 4 + 3) * Mc + p];
+    }
+  }
+  if (incremental) {
+    rho = state[S_RHO * npad + gp] + rho;
+    if (GGA) {
+      g0 = state[S_GX * npad + gp] + g0;
+      g1 = state[S_GY * npad + gp] + g1;
+      g2 = state[S_GZ * npad + gp] + g2;
+    }
+  }
+  const double gam = g0 * g0 + g1 * g1 + g2 * g2;
+  double f = 0.0, fr = 0.0, fg = 0.0;
+  if (mask & (1u << 1)) xcf::xalpha(rho, f, fr);
+  if (mask & (1u << 3)) xcf::pwldac(rho, f, fr);
+  if (GGA) {
+    if (mask & (1u << 10)) xcf::pbex(rho, gam, f, fr, fg);
+    if (mask & (1u << 18)) xcf::pbec(rho, gam, f, fr, fg);
+  }
+  const double ww = w[gp];
+  const double a = 0.5 * fr * ww;
+  const double bq = ww * 2.0 * fg;
+  const double b0 = bq * g0, b1 = bq * g1, b2 = bq * g2;
+  if (incremental) {
+    coef[0 * Mc + p] = a - state[S_A * npad + gp];
+    if (GGA) {
+      coef[1 * Mc + p] = b0 - state[S_BX * npad + gp];
+      coef[2 * Mc + p] = b1 - state[S_BY * npad + gp];
+      coef[3 * Mc + p] = b2 - state[S_BZ * npad + gp];
+    }
+  } else {
+    coef[0 * Mc + p] = a;
+    if (GGA) {
+      coef[1 * Mc + p] = b0;
+      coef[2 * Mc + p] = b1;
+      coef[3 * Mc + p] = b2;
+    }
+  }
+  state[S_RHO * npad + gp] = rho;
+  state[S_A * npad + gp] = a;
+  if (GGA) {
+    state[S_GX * npad + gp] = g0;
+    state[S_GY * npad + gp] = g1;
+    state[S_GZ * npad + gp] = g2;
+    state[S_BX * npad + gp] = b0;
+    state[S_BY * npad + gp] = b1;
+    state[S_BZ * npad + gp] = b2;
+  }
+  se[tid] = f * ww;
+  sq[tid] = rho * ww;
+  __syncthreads();
+  for (int st = kThreads / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      se[tid] += se[tid + st];
+      sq[tid] += sq[tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    epart[2 * (blk_off + blockIdx.x) + 0] = se[0];
+    epart[2 * (blk_off + blockIdx.x) + 1] = sq[0];
+  }
+}
+
+void launch_functional(hipStream_t s, bool gga, bool incremental, uint32_t mask,
+                       const double *part, int ntiles, long Mc, const double *w, long p0,
+                       double *state, long npad, double *coef, double *epart, long blk_off) {
+  dim3 grid((unsigned)(Mc / kThreads));
+  if (gga)
+    hipLaunchKernelGGL((k_functional<true>), grid, dim3(kThreads), 0, s, part, ntiles, Mc, w, p0,
+                       mask, (int)incremental, state, npad, coef, epart, blk_off);
+  else
+    hipLaunchKernelGGL((k_functional<false>), grid, dim3(kThreads), 0, s, part, ntiles, Mc, w, p0,
+                       mask, (int)incremental, state, npad, coef, epart, blk_off);
+}
+
+// -------------------------------------------------------------------- W
+template <typename T, bool GRAD>
+__global__ __launch_bounds__(kThreads) void k_wmat(const T *__restrict__ phi,
+                                                   const double *__restrict__ coef, long Mc,
+                                                   long NP, T *__restrict__ W) {
+  const long p = (long)blockIdx.x * kThreads + threadIdx.x;
+  const long f = blockIdx.y;
+  const long q = f * Mc + p, cs = NP * Mc;
+  double v = coef[p] * (double)phi[q];
+  if (GRAD)
+    v = v + coef[Mc + p] * (double)phi[cs + q] + coef[2 * Mc + p] * (double)phi[2 * cs + q] +
+        coef[3 * Mc + p] * (double)phi[3 * cs + q];
+  W[q] = (T)v;
+}
+
+template <typename T>
+void launch_wmat(hipStream_t s, bool grad, const T *phi, const double *coef, long Mc, int NP,
+                 T *W) {
+  dim3 grid((unsigned)(Mc / kThreads), (unsigned)NP);
+  if (grad)
+    hipLaunchKernelGGL((k_wmat<T, true>), grid, dim3(kThreads), 0, s, phi, coef, Mc, (long)NP, W);
+  else
+    hipLaunchKernelGGL((k_wmat<T, false>), grid, dim3(kThreads), 0, s, phi, coef, Mc, (long)NP, W);
+}
+template void launch_wmat<double>(hipStream_t, bool, const double *, const double *, long, int,
+                                  double *);
+template void launch_wmat<float>(hipStream_t, bool, const float *, const double *, long, int,
+                                 float *);
+
+// --------------------------------------------------------------- potential
+constexpr int kPBM = 128, kPBN = 128, kPBK = 16;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_potential(const T *__restrict__ W,
+                                                        const T *__restrict__ phi, long Mc,
+                                                        int NP, int ksteps, double *Hpart,
+                                                        int ntn, int first) {
+  constexpr int BM = kPBM, BN = kPBN, BK = kPBK, WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  __shared__ T As[2 * BK * BM];
+  __shared__ T Bs[2 * BK * BN];
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int i0 = (tile / ntn) * BM, j0 = (tile % ntn) * BN;
+  const long kb = (long)blockIdx.y * ksteps * BK;
+  typename MF<T>::acc acc[TM][TN];
+  gemm_tile<T, BM, BN, BK, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb, Mc,
+                                       ksteps, As, Bs, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  double *H = Hpart + (long)blockIdx.y * NP * NP;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long row = i0 + wm * WM + i * 16 + mf_row(lane, r);
+        const long col = j0 + wn * WN + j * 16 + mf_col(lane);
+        const double v = (double)acc[i][j][r];
+        const long q = row * NP + col;
+        H[q] = first ? v : H[q] + v;
+      }
+}
+
+template <typename T>
+void launch_potential(hipStream_t s, const T *W, const T *phi, long Mc, int NP, int nsplit,
+                      double *Hpart, bool first) {
+  const int ntn = NP / kPBN, ntm = NP / kPBM;
+  const int ksteps = (int)(Mc / kPBK) / nsplit;
+  dim3 grid((unsigned)(ntm * ntn), (unsigned)nsplit);
+  hipLaunchKernelGGL((k_potential<T>), grid, dim3(kThreads), 0, s, W, phi, Mc, NP, ksteps,
+                     Hpart, ntn, (int)first);
+}
+template void launch_potential<double>(hipStream_t, const double *, const double *, long, int,
+                                       int, double *, bool);
+template void launch_potential<float>(hipStream_t, const float *, const float *, long, int, int,
+                                      double *, bool);
+
+// -------------------------------------------------------------------- pack
+__global__ __launch_bounds__(kThreads) void k_pack(const double *__restrict__ Hpart, int nsplit,
+                                                   long NP, long n, const double *base,
+                                                   double *__restrict__ V) {
+  const long idx = (long)blockIdx.x * kThreads + threadIdx.x;
+  const long npk = n * (n + 1) / 2;
+  if (idx >= npk) return;
+  long j = (long)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+  while (j * (j + 1) / 2 > idx) --j;
+  while ((j + 1) * (j + 2) / 2 <= idx) ++j;
+  const long k = idx - j * (j + 1) / 2;
+  double hjk = 0.0, hkj = 0.0;
+  for (int s = 0; s < nsplit; ++s) {
+    hjk += Hpart[(long)s * NP * NP + j * NP + k];
+    hkj += Hpart[(long)s * NP * NP + k * NP + j];
+  }
+  const double v = hjk + hkj;
+  V[idx] = base ? base[idx] + v : v;
+}
+
+void launch_pack(hipStream_t s, const double *Hpart, int nsplit, int NP, int n,
+                 const double *base, double *Vpacked) {
+  const long npk = (long)n * (n + 1) / 2;
+  dim3 grid((unsigned)((npk + kThreads - 1) / kThreads));
+  hipLaunchKernelGGL(k_pack, grid, dim3(kThreads), 0, s, Hpart, nsplit, (long)NP, (long)n, base,
+                     Vpacked);
+}
+
+// ------------------------------------------------------------ small kernels
+__global__ __launch_bounds__(kThreads) void k_form_density(const double *__restrict__ C,
+                                                           const double *__restrict__ occ, int n,
+                                                           int nocc, int NP, double *P) {
+  const int j = blockIdx.x * kThreads + threadIdx.x, i = blockIdx.y;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < nocc; ++k) s += C[i + (long)k * n] * occ[k] * C[j + (long)k * n];
+  P[(long)i * NP + j] = s;
+}
+
+void launch_form_density(hipStream_t s, const double *C, const double *occ, int n, int nocc,
+                         int NP, double *P) {
+  dim3 grid((unsigned)((n + kThreads - 1) / kThreads), (unsigned)n);
+  hipLaunchKernelGGL(k_form_density, grid, dim3(kThreads), 0, s, C, occ, n, nocc, NP, P);
+}
+
+__global__ void k_sub(const double *a, const double *b, long n, double *out) {
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) out[i] = a[i] - b[i];
+}
+void launch_sub(hipStream_t s, const double *a, const double *b, long n, double *out) {
+  hipLaunchKernelGGL(k_sub, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                     a, b, n, out);
+}
+
+template <typename T>
+__global__ void k_cast(const double *in, long n, T *out) {
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) out[i] = (T)in[i];
+}
+template <typename T>
+void launch_cast(hipStream_t s, const double *in, long n, T *out) {
+  hipLaunchKernelGGL((k_cast<T>), dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads),
+                     0, s, in, n, out);
+}
+template void launch_cast<double>(hipStream_t, const double *, long, double *);
+template void launch_cast<float>(hipStream_t, const double *, long, float *);
+
+}  // namespace pgxc
+
+// ------------------------------------------------------------------ Becke
+namespace pgxc {
+// Becke partition weight of the owning atom (J. Chem. Phys. 88, 2547 (1988)),
+// as becke0 (ParaGauss modules/grid_module.f90:2442-2497) with the step
+// function mu3 = p(p(p(mu))), p(m) = m (3 - m^2) / 2
+// (autodiff/becke_step_func.f90:84-124), no atomic-size adjustment.
+__device__ __forceinline__ double becke_step(double mu) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) mu = mu * (3.0 - mu * mu) / 2.0;
+  return mu;
+}
+
+__global__ __launch_bounds__(kThreads) void k_becke(int na, const double *__restrict__ axyz,
+                                                    const double *__restrict__ inv_dist,
+                                                    long npts, const double *__restrict__ pts,
+                                                    const int *__restrict__ owner,
+                                                    const double *__restrict__ wat,
+                                                    double *__restrict__ wout) {
+  const long p = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= npts) return;
+  const double x = pts[p], y = pts[npts + p], z = pts[2 * npts + p];
+  const int own = owner[p];
+  double sum = 0.0, pown = 0.0;
+  for (int i = 0; i < na; ++i) {
+    const double dxi = axyz[3 * i] - x, dyi = axyz[3 * i + 1] - y, dzi = axyz[3 * i + 2] - z;
+    const double ri = sqrt(dxi * dxi + dyi * dyi + dzi * dzi);
+    double pi = 1.0;
+    for (int k = 0; k < na && pi != 0.0; ++k) {
+      if (k == i) continue;
+      const double dxk = axyz[3 * k] - x, dyk = axyz[3 * k + 1] - y, dzk = axyz[3 * k + 2] - z;
+      const double rk = sqrt(dxk * dxk + dyk * dyk + dzk * dzk);
+      const double mu = (ri - rk) * inv_dist[(long)i * na + k];
+      pi *= (1.0 - becke_step(mu)) / 2.0;
+    }
+    sum += pi;
+    if (i == own) pown = pi;
+  }
+  wout[p] = wat[p] * (pown / sum);
+}
+
+void launch_becke(hipStream_t s, int na, const double *axyz, const double *inv_dist, long npts,
+                  const double *pts, const int *owner, const double *wat, double *wout) {
+  hipLaunchKernelGGL(k_becke, dim3((unsigned)((npts + kThreads - 1) / kThreads)), dim3(kThreads),
+                     0, s, na, axyz, inv_dist, npts, pts, owner, wat, wout);
+}
+}  // namespace pgxc

@@ -1,0 +1,108 @@
+!
+! pgxc_binding.f90 -- ISO_C_BINDING interface to include/pgxc.h.
+!
+! The Fortran SCF driver keeps its call interface: xc_build (ParaGauss
+! modules/xc_hamiltonian.f90:1038) still calls calc_xcks_nl(ham_xc_arr)
+! (:1130), whose body becomes a call of xc_gpu_build (xc_gpu.f90).  The
+! enumerators below are the bit positions of xc_cntrl's bind(c) enum
+! (modules/xc_cntrl.f90:61-104), so a functional mask is built from is_on().
+!
+module pgxc_binding
+  use iso_c_binding, only: c_int, c_int32_t, c_int64_t, c_uint32_t, c_double, c_ptr, c_char
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: PGXC_OK = 0, PGXC_ERR_ARG = 1, &
+       PGXC_ERR_UNSUPPORTED = 2, PGXC_ERR_HIP = 3, PGXC_ERR_NOMEM = 4, PGXC_ERR_STATE = 5
+
+  ! xc_cntrl enumerators used by XC="PBE" (modules/xc_cntrl.f90:652-670)
+  integer, parameter, public :: xc_XAlpha = 1, xc_PWLDAc = 3, xc_PBEx = 10, xc_PBEc = 18
+
+  integer(c_int32_t), parameter, public :: PGXC_PREC_FP64 = 0, PGXC_PREC_FP32 = 1, &
+       PGXC_PREC_BF16X3 = 2
+  integer(c_int32_t), parameter, public :: PGXC_DENSITY_FULL = 0, PGXC_DENSITY_INCREMENTAL = 1
+
+  public :: pgxc_create, pgxc_set_density, pgxc_set_orbitals, pgxc_build, pgxc_get_result
+  public :: pgxc_nbas, pgxc_last_error, pgxc_destroy, pgxc_last_timing, pgxc_last_stats
+  public :: pgxc_becke_weights
+
+  interface
+     integer(c_int) function pgxc_create(ctx, n_atoms, atom_xyz, n_shells, shell_info, &
+          exps, norms, contr, grid, npts, veclen) bind(C, name="pgxc_create")
+       import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+       type(c_ptr), intent(out) :: ctx
+       integer(c_int32_t), value :: n_atoms, n_shells, veclen
+       real(c_double), intent(in) :: atom_xyz(*), exps(*), norms(*), contr(*), grid(*)
+       integer(c_int32_t), intent(in) :: shell_info(*)
+       integer(c_int64_t), value :: npts
+     end function pgxc_create
+
+     integer(c_int) function pgxc_set_density(ctx, ispin, nbas, P, mode) &
+          bind(C, name="pgxc_set_density")
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ispin, nbas, mode
+       real(c_double), intent(in) :: P(*)
+     end function pgxc_set_density
+
+     integer(c_int) function pgxc_set_orbitals(ctx, ispin, nbas, nocc, C, occ, mode) &
+          bind(C, name="pgxc_set_orbitals")
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ispin, nbas, nocc, mode
+       real(c_double), intent(in) :: C(*), occ(*)
+     end function pgxc_set_orbitals
+
+     integer(c_int) function pgxc_build(ctx, functional, precision, ham, exc, charge) &
+          bind(C, name="pgxc_build")
+       import :: c_int, c_int32_t, c_uint32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_uint32_t), value :: functional
+       integer(c_int32_t), value :: precision
+       real(c_double), intent(inout) :: ham(*), exc, charge
+     end function pgxc_build
+
+     integer(c_int) function pgxc_get_result(ctx, ham, exc, charge) bind(C, name="pgxc_get_result")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: ham(*), exc, charge
+     end function pgxc_get_result
+
+     integer(c_int64_t) function pgxc_nbas(ctx) bind(C, name="pgxc_nbas")
+       import :: c_int64_t, c_ptr
+       type(c_ptr), value :: ctx
+     end function pgxc_nbas
+
+     integer(c_int) function pgxc_last_timing(ctx, ms6) bind(C, name="pgxc_last_timing")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ms6(6)
+     end function pgxc_last_timing
+
+     integer(c_int) function pgxc_last_stats(ctx, s5) bind(C, name="pgxc_last_stats")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: s5(5)
+     end function pgxc_last_stats
+
+     type(c_ptr) function pgxc_last_error(ctx) bind(C, name="pgxc_last_error")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end function pgxc_last_error
+
+     integer(c_int) function pgxc_destroy(ctx) bind(C, name="pgxc_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function pgxc_destroy
+
+     integer(c_int) function pgxc_becke_weights(n_atoms, atom_xyz, npts, pts, owner, &
+          w_atomic, w_out) bind(C, name="pgxc_becke_weights")
+       import :: c_int, c_int32_t, c_int64_t, c_double
+       integer(c_int32_t), value :: n_atoms
+       integer(c_int64_t), value :: npts
+       real(c_double), intent(in) :: atom_xyz(*), pts(*), w_atomic(*)
+       integer(c_int32_t), intent(in) :: owner(*)
+       real(c_double), intent(out) :: w_out(*)
+     end function pgxc_becke_weights
+  end interface
+end module pgxc_binding

@@ -1,0 +1,58 @@
+!
+! pgxc_driver -- standalone Fortran caller of the library, in the shape the
+! ParaGauss SCF driver calls it (setup once, then orbitals + build).
+!
+! usage: pgxc_driver <input.bin> <output.bin>
+! input  (stream, native endianness): int32 n_atoms, n_shells, nbas, nocc,
+!        veclen, precision, mask; int64 npts; real64 atom_xyz(3,n_atoms);
+!        int32 shell_info(5,n_shells); int32 n_exp_total, n_contr_total;
+!        real64 exps(n_exp_total), norms(n_exp_total), contr(n_contr_total),
+!        grdpts(npts,4), eigvec_occ(nbas,nocc), occ_num_occ(nocc)
+! output: real64 exc_int, charge_int, ham_xc_arr(nbas*(nbas+1)/2)
+!
+program pgxc_driver
+  use iso_c_binding
+  use xc_gpu
+  implicit none
+  character(len=1024) :: fin, fout
+  integer(c_int32_t) :: n_atoms, n_shells, nbas, nocc, veclen, precision, ne, nc
+  integer(c_uint32_t) :: mask
+  integer(c_int64_t) :: npts
+  real(c_double), allocatable :: atom_xyz(:, :), exps(:), norms(:), contr(:), grdpts(:, :)
+  real(c_double), allocatable :: eigvec_occ(:, :), occ(:), ham(:)
+  integer(c_int32_t), allocatable :: shell_info(:, :)
+  real(c_double) :: exc_int, charge_int
+  integer :: u
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(u) n_atoms, n_shells, nbas, nocc, veclen, precision, mask
+  read(u) npts
+  allocate(atom_xyz(3, n_atoms), shell_info(5, n_shells))
+  read(u) atom_xyz
+  read(u) shell_info
+  read(u) ne, nc
+  allocate(exps(ne), norms(ne), contr(max(nc, 1)), grdpts(npts, 4), eigvec_occ(nbas, nocc), occ(nocc))
+  read(u) exps, norms
+  if (nc > 0) read(u) contr(1:nc)
+  read(u) grdpts, eigvec_occ, occ
+  close(u)
+
+  ! xc_setup
+  call xc_gpu_setup(atom_xyz, shell_info, exps, norms, contr, grdpts, int(veclen))
+  ! xc_build: zero the accumulators (modules/xc_hamiltonian.f90:1101-1116),
+  ! then the calc_xcks_nl body
+  allocate(ham(nbas * (nbas + 1) / 2))
+  ham = 0.0_c_double
+  exc_int = 0.0_c_double
+  charge_int = 0.0_c_double
+  call xc_gpu_set_orbitals(eigvec_occ, occ)
+  call xc_gpu_build(mask, precision, ham, exc_int, charge_int)
+  call xc_gpu_close()
+
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(u) exc_int, charge_int, ham
+  close(u)
+  write(*, '(a, es24.16, a, es24.16)') 'exc_int = ', exc_int, '  charge_int = ', charge_int
+end program pgxc_driver

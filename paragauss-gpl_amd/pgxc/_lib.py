@@ -1,0 +1,62 @@
+"""Loader of libpgxc.so, the HIP implementation behind include/pgxc.h.
+
+There is no CPU fallback: if the library is missing, or no HIP device is
+present when a build is requested, the calls raise.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libpgxc.so")
+
+PGXC_OK = 0
+ERRORS = {1: "PGXC_ERR_ARG", 2: "PGXC_ERR_UNSUPPORTED", 3: "PGXC_ERR_HIP",
+          4: "PGXC_ERR_NOMEM", 5: "PGXC_ERR_STATE"}
+
+# header symbols, checked by the tests against include/pgxc.h
+EXPORTS = ("pgxc_create", "pgxc_set_density", "pgxc_set_orbitals", "pgxc_build",
+           "pgxc_get_result", "pgxc_nbas", "pgxc_last_stats", "pgxc_last_timing",
+           "pgxc_last_error", "pgxc_destroy", "pgxc_becke_weights")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class PgxcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERRORS.get(code, "status %d" % code), msg))
+        self.code = code
+
+
+def load():
+    """Return the ctypes handle; raise if libpgxc.so has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libpgxc.so not built (expected at %s); run "
+                           "__graft_entry__.build() or make -C paragauss-gpl_amd" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    L.pgxc_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int32, _dp, ctypes.c_int32, _ip,
+                              _dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int32]
+    L.pgxc_set_density.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32]
+    L.pgxc_set_orbitals.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp,
+                                    _dp, ctypes.c_int32]
+    L.pgxc_build.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int32, _dp, _dp, _dp]
+    L.pgxc_get_result.argtypes = [_vp, _dp, _dp, _dp]
+    L.pgxc_nbas.argtypes = [_vp]
+    L.pgxc_nbas.restype = ctypes.c_int64
+    L.pgxc_last_stats.argtypes = [_vp, _dp]
+    L.pgxc_last_timing.argtypes = [_vp, _dp]
+    L.pgxc_last_error.argtypes = [_vp]
+    L.pgxc_last_error.restype = ctypes.c_char_p
+    L.pgxc_destroy.argtypes = [_vp]
+    L.pgxc_becke_weights.argtypes = [ctypes.c_int32, _dp, ctypes.c_int64, _dp, _ip, _dp, _dp]
+    L.pgxc_becke_weights.restype = ctypes.c_int
+    for name in EXPORTS:
+        if name not in ("pgxc_nbas", "pgxc_last_error"):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L

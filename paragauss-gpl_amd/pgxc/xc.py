@@ -1,0 +1,144 @@
+"""Host-side mirror of the reference's XC-build interface over the C ABI.
+
+XCBuilder follows the life cycle of ParaGauss's xc_hamiltonian module:
+    xc_setup  -> XCBuilder(basis, grid)            (pgxc_create)
+    density   -> set_density(P) / set_orbitals(C, occ)
+    xc_build  -> build(functional) -> (ham_xc_arr, exc_int, charge_int)
+    xc_close  -> close()
+ham_xc_arr is the packed lower triangle (j = 1..n, k = 1..j) of
+modules/xc_hamiltonian.f90:1561-1567; exc/charge are exc_int and charge_int
+(:1408,1423).  Errors raise PgxcError with the library's message, the role
+error_handler plays in the reference.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+XALPHA, PWLDAC, PBEX, PBEC = 1 << 1, 1 << 3, 1 << 10, 1 << 18
+PBE = XALPHA | PWLDAC | PBEX | PBEC
+LDA = XALPHA | PWLDAC
+FUNCTIONALS = {"PBE": PBE, "LDA": LDA, "XALPHA": XALPHA}
+
+FP64, FP32, BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
+FULL, INCREMENTAL = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class XCBuilder:
+    def __init__(self, basis, grid, veclen=128):
+        self._L = _lib.load()
+        self.basis = basis
+        self.nbas = basis.nbas
+        g = np.asfortranarray(grid, dtype=np.float64)
+        if g.ndim != 2 or g.shape[1] != 4:
+            raise ValueError("grid must be (npts, 4): x, y, z, w")
+        self.npts = g.shape[0]
+        self._keep = [np.ascontiguousarray(basis.atom_xyz, dtype=np.float64),
+                      np.ascontiguousarray(basis.shell_info, dtype=np.int32),
+                      np.ascontiguousarray(basis.exps), np.ascontiguousarray(basis.norms),
+                      np.ascontiguousarray(basis.contr), g]
+        axyz, info, ex, nm, ct, g = self._keep
+        h = ctypes.c_void_p()
+        rc = self._L.pgxc_create(ctypes.byref(h), basis.n_atoms, _ptr(axyz), basis.n_shells,
+                                 info.ctypes.data_as(_ip), _ptr(ex), _ptr(nm), _ptr(ct), _ptr(g),
+                                 self.npts, veclen)
+        self._h = h
+        self._check(rc)
+
+    def _check(self, rc):
+        if rc != _lib.PGXC_OK:
+            msg = self._L.pgxc_last_error(self._h).decode()
+            raise _lib.PgxcError(rc, msg)
+
+    def set_density(self, P, incremental=False):
+        P = np.asfortranarray(P, dtype=np.float64)
+        self._check(self._L.pgxc_set_density(self._h, 1, self.nbas, _ptr(P),
+                                             INCREMENTAL if incremental else FULL))
+
+    def set_orbitals(self, C, occ, incremental=False):
+        C = np.asfortranarray(C, dtype=np.float64)
+        occ = np.ascontiguousarray(occ, dtype=np.float64)
+        self._check(self._L.pgxc_set_orbitals(self._h, 1, self.nbas, C.shape[1], _ptr(C),
+                                              _ptr(occ), INCREMENTAL if incremental else FULL))
+
+    def build(self, functional=PBE, precision=FP64, fetch=True):
+        if isinstance(functional, str):
+            functional = FUNCTIONALS[functional]
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        if not fetch:
+            self._check(self._L.pgxc_build(self._h, functional, precision, None, None, None))
+            return None
+        ham = np.zeros(self.nbas * (self.nbas + 1) // 2)
+        e = np.zeros(1)
+        q = np.zeros(1)
+        self._check(self._L.pgxc_build(self._h, functional, precision, _ptr(ham), _ptr(e), _ptr(q)))
+        return ham, float(e[0]), float(q[0])
+
+    def result(self):
+        ham = np.zeros(self.nbas * (self.nbas + 1) // 2)
+        e = np.zeros(1)
+        q = np.zeros(1)
+        self._check(self._L.pgxc_get_result(self._h, _ptr(ham), _ptr(e), _ptr(q)))
+        return ham, float(e[0]), float(q[0])
+
+    def stats(self):
+        s = np.zeros(5)
+        self._check(self._L.pgxc_last_stats(self._h, _ptr(s)))
+        return dict(points=s[0], batches=s[1], flops=s[2], exps=s[3], bytes=s[4])
+
+    def timing(self):
+        t = np.zeros(6)
+        self._check(self._L.pgxc_last_timing(self._h, _ptr(t)))
+        return dict(basis=t[0], density=t[1], functional=t[2], potential=t[3], pack=t[4],
+                    total=t[5])
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.pgxc_destroy(self._h)
+        self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def unpack(ham_packed, n):
+    """Packed lower triangle -> symmetric n x n."""
+    V = np.zeros((n, n))
+    j, k = np.tril_indices(n)
+    V[j, k] = ham_packed
+    V[k, j] = ham_packed
+    return V
+
+
+def becke_weights(atom_xyz, pts, owner, w_atomic):
+    """Becke-partitioned weights on the GPU (pgxc_becke_weights)."""
+    L = _lib.load()
+    axyz = np.ascontiguousarray(atom_xyz, dtype=np.float64)
+    P = np.asfortranarray(pts, dtype=np.float64)
+    own = np.ascontiguousarray(owner, dtype=np.int32)
+    wa = np.ascontiguousarray(w_atomic, dtype=np.float64)
+    out = np.zeros(P.shape[0])
+    rc = L.pgxc_becke_weights(axyz.shape[0], _ptr(axyz), P.shape[0], _ptr(P),
+                              own.ctypes.data_as(_ip), _ptr(wa), _ptr(out))
+    if rc != 0:
+        raise _lib.PgxcError(rc, "pgxc_becke_weights failed")
+    return out

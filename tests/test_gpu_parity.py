@@ -1,0 +1,170 @@
+"""GPU parity: the HIP build (through the C ABI) against the FP64 oracle on
+identical inputs.
+
+Tolerances (north-star bar for the plain FP64 path):
+  |dE_xc| <= 1e-9 Eh, max |dV_xc| <= 1e-10, |dcharge| <= 1e-9.
+Mixed precision (FP32 MFMA contractions) states its own bounds below.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from pgxc import XCBuilder, basis, grid, systems
+from pgxc.xc import FP32, FP64, LDA, PBE
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def h2o_case():
+    els, xyz = systems.h2o()
+    b = basis.seeded_basis(els, xyz, seed=1, shell_set="small")
+    g = grid.molecular_grid(els, xyz, nrad=75, nang=302)
+    C, occ = systems.random_orbitals(b.nbas, 5, seed=2)
+    return b, g, C, occ
+
+
+_ORACLE = {}
+
+
+def oracle_result(key, b, g, mask, C=None, occ=None, P=None):
+    if key not in _ORACLE:
+        _ORACLE[key] = oracle.xc_build(b, g, mask, C=C, occ=occ, P=P, veclen=128, nthreads=NT)
+    return _ORACLE[key]
+
+
+@pytest.fixture(scope="module")
+def h2o():
+    return h2o_case()
+
+
+def check(res, ref, e_tol=1e-9, v_tol=1e-10, q_tol=1e-9):
+    ham, e, q = res
+    hr, er, qr = ref
+    dv = np.max(np.abs(ham - hr))
+    print("dE=%.3e dV=%.3e dQ=%.3e" % (e - er, dv, q - qr), flush=True)
+    assert abs(e - er) <= e_tol
+    assert dv <= v_tol
+    assert abs(q - qr) <= q_tol
+
+
+def test_h2o_pbe_fp64(h2o):
+    b, g, C, occ = h2o
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        res = xc.build(PBE, FP64)
+    check(res, oracle_result("h2o_pbe", b, g, oracle.MASK_PBE, C=C, occ=occ))
+
+
+def test_h2o_lda_fp64(h2o):
+    b, g, C, occ = h2o
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        res = xc.build(LDA, FP64)
+    check(res, oracle_result("h2o_lda", b, g, oracle.MASK_LDA, C=C, occ=occ))
+
+
+def test_density_matrix_input_and_determinism(h2o):
+    b, g, C, occ = h2o
+    P = (C * occ) @ C.T
+    with XCBuilder(b, g) as xc:
+        xc.set_density(P)
+        r1 = xc.build(PBE, FP64)
+        r2 = xc.build(PBE, FP64)
+    assert np.array_equal(r1[0], r2[0]) and r1[1] == r2[1] and r1[2] == r2[2]
+    check(r1, oracle_result("h2o_pbe", b, g, oracle.MASK_PBE, C=C, occ=occ))
+
+
+def test_results_accumulate_into_outputs(h2o):
+    b, g, C, occ = h2o
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        ham, e, q = xc.build(PBE, FP64)
+        h2, e2, q2 = xc.result()
+    assert np.array_equal(ham, h2) and e == e2 and q == q2
+
+
+def test_h2o_fp32_full_build(h2o):
+    # FP32 MFMA contractions of the full density: error bound stated, not FP64
+    b, g, C, occ = h2o
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        res = xc.build(PBE, FP32)
+    ref = oracle_result("h2o_pbe", b, g, oracle.MASK_PBE, C=C, occ=occ)
+    check(res, ref, e_tol=1e-5, v_tol=1e-5, q_tol=1e-5)
+
+
+def test_incremental_fp64_equals_full(h2o):
+    b, g, C, occ = h2o
+    C1 = systems.perturbed_orbitals(C, 1e-2, seed=9)
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        xc.build(PBE, FP64)
+        xc.set_orbitals(C1, occ, incremental=True)
+        inc = xc.build(PBE, FP64)
+    check(inc, oracle_result("h2o_pbe_c1", b, g, oracle.MASK_PBE, C=C1, occ=occ))
+
+
+@pytest.mark.parametrize("norm", [1e-2, 1e-4, 1e-6])
+def test_incremental_fp32_error_tracks_dP(h2o, norm):
+    b, g, C, occ = h2o
+    C1 = systems.perturbed_orbitals(C, norm, seed=10)
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        xc.build(PBE, FP64)
+        xc.set_orbitals(C1, occ, incremental=True)
+        inc = xc.build(PBE, FP32)
+    ref = oracle.xc_build(b, g, oracle.MASK_PBE, C=C1, occ=occ, nthreads=NT)
+    # low-precision error proportional to ||dP||, plus the FP64 floor
+    check(inc, ref, e_tol=1e-9 + 1e-3 * norm, v_tol=1e-10 + 1e-3 * norm, q_tol=1e-9 + 1e-3 * norm)
+
+
+def test_water_cluster_medium_basis_fp64():
+    els, xyz = systems.water_cluster(4, seed=3)
+    b = basis.seeded_basis(els, xyz, seed=4, shell_set="medium")
+    g = grid.molecular_grid(els, xyz, nrad=40, nang=302, partition="gpu")
+    C, occ = systems.random_orbitals(b.nbas, 20, seed=5)
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        res = xc.build(PBE, FP64)
+    ref = oracle.xc_build(b, g, oracle.MASK_PBE, C=C, occ=occ, nthreads=NT)
+    check(res, ref)
+
+
+def test_becke_gpu_matches_numpy():
+    els, xyz = systems.water_cluster(2, seed=1)
+    g_np = grid.molecular_grid(els, xyz, nrad=30, nang=302, partition="numpy")
+    g_gpu = grid.molecular_grid(els, xyz, nrad=30, nang=302, partition="gpu")
+    assert np.allclose(g_np[:, 3], g_gpu[:, 3], rtol=1e-12, atol=1e-300)
+    # partition of unity: the weights of all atomic grids integrate 1 like the atomic ones
+    r2 = np.sum((g_gpu[:, :3] - xyz[0]) ** 2, axis=1)
+    assert np.sum(g_gpu[:, 3] * (2 / np.pi) ** 1.5 * np.exp(-2 * r2)) == pytest.approx(1.0, abs=1e-6)
+
+
+def test_fortran_driver_matches(h2o, tmp_path):
+    b, g, C, occ = h2o
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([b.n_atoms, b.n_shells, b.nbas, C.shape[1], 128, FP64, PBE], dtype=np.int32).tofile(f)
+        np.array([g.shape[0]], dtype=np.int64).tofile(f)
+        np.ascontiguousarray(b.atom_xyz).tofile(f)
+        np.ascontiguousarray(b.shell_info, dtype=np.int32).tofile(f)
+        np.array([len(b.exps), len(b.contr)], dtype=np.int32).tofile(f)
+        b.exps.tofile(f)
+        b.norms.tofile(f)
+        np.ascontiguousarray(b.contr).tofile(f)
+        np.asfortranarray(g).T.copy().tofile(f)
+        np.asfortranarray(C).T.copy().tofile(f)
+        np.ascontiguousarray(occ).tofile(f)
+    exe = os.path.join(ROOT, "paragauss-gpl_amd", "fortran", "pgxc_driver")
+    out = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=300)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    raw = np.fromfile(fout, dtype=np.float64)
+    e, q, ham = raw[0], raw[1], raw[2:]
+    check((ham, e, q), oracle_result("h2o_pbe", b, g, oracle.MASK_PBE, C=C, occ=occ))
