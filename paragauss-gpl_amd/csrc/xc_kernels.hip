@@ -26,12 +26,18 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
 
 template <typename T>
 struct MF;
+// 16x16x4 MFMA operand/result placement (wave64), both types:
+//   A(i,k): lane = i + 16 k      B(k,j): lane = j + 16 k
+// Result element r of lane l (measured on gfx950):
+//   f64: D(i,j) with i = l / 16 + 4 r, j = l % 16
+//   f32: D(i,j) with i = 4 (l / 16) + r, j = l % 16
 template <>
 struct MF<double> {
   typedef d4 acc;
   __device__ static __forceinline__ acc mma(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
 struct MF<float> {
@@ -39,12 +45,8 @@ struct MF<float> {
   __device__ static __forceinline__ acc mma(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static __forceinline__ int row(int lane, int r) { return ((lane >> 4) << 2) + r; }
 };
-
-// 16x16x4 MFMA operand/result placement (wave64):
-//   A(i,k): lane = i + 16 k      B(k,j): lane = j + 16 k
-//   D(i,j): lane = j + 16 (i / 4), element i % 4
-__device__ __forceinline__ int mf_row(int lane, int r) { return ((lane >> 4) << 2) + r; }
 __device__ __forceinline__ int mf_col(int lane) { return lane & 15; }
 
 // Block tile BM x BN over K in steps of BK; 4 waves arranged 2 x 2, wave tile
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const T *__restrict__ phi,
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = wm * WM + i * 16 + mf_row(lane, r);
+      const int row = wm * WM + i * 16 + MF<T>::row(lane, r);
       const long p = m0 + row;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
@@ -671,7 +673,7 @@ __global__ __launch_bounds__(kThreads) void k_potential(const T *__restrict__ W,
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long row = i0 + wm * WM + i * 16 + mf_row(lane, r);
+        const long row = i0 + wm * WM + i * 16 + MF<T>::row(lane, r);
         const long col = j0 + wn * WN + j * 16 + mf_col(lane);
         const double v = (double)acc[i][j][r];
         const long q = row * NP + col;
