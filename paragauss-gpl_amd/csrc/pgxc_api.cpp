@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pgxc.h"
@@ -26,7 +27,7 @@ struct pgxc_ctx {
   // sizes
   int n_atoms = 0, n_shells = 0, nbas = 0, NP = 0, veclen = 128;
   long npts = 0, npad = 0, Mc = 0;
-  int nchunks = 0, nsplit = 1, ntiles = 0;
+  int nchunks = 0, nsplit = 1, ntiles = 0, tf = 128;
   // device: geometry and basis
   double *grid = nullptr;  // [4][npad] x, y, z, w
   AtomD *atoms = nullptr;
@@ -164,8 +165,15 @@ int pgxc_create(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int32_t
     if (!(exps[e] > 0.0) || !std::isfinite(norms[e]))
       return fail(c, PGXC_ERR_ARG, "pgxc_create: exponents must be positive and norms finite");
   c->nbas = (int)foff;
+  // function tile: the one of 128 / 96 / 64 that pads nbas least (larger on ties)
+  c->tf = 128;
   c->NP = (int)round_up(c->nbas, 128);
-  c->ntiles = density_tiles(c->NP);
+  for (int t : {96, 64})
+    if (round_up(c->nbas, t) < c->NP) {
+      c->NP = (int)round_up(c->nbas, t);
+      c->tf = t;
+    }
+  c->ntiles = density_tiles(c->NP, c->tf);
 
   // ---- chunking of the grid ----------------------------------------------
   // phi chunk = 4 x NP x Mc doubles, kept under ~2 GiB
@@ -176,9 +184,9 @@ int pgxc_create(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int32_t
   c->npad = c->Mc * c->nchunks;
   // split of the point range in the potential contraction: enough
   // workgroups for 256 CUs, each split a whole number of 16-point steps
-  const int ntile_pot = (c->NP / 128) * (c->NP / 128);
+  const int ntile_pot = (c->NP / c->tf) * (c->NP / c->tf);
   c->nsplit = 1;
-  while (c->nsplit < 16 && ntile_pot * c->nsplit < 512 && (c->Mc / 16) % (c->nsplit * 2) == 0)
+  while (c->nsplit < 16 && ntile_pot * c->nsplit < 512 && (c->Mc / 32) % (c->nsplit * 2) == 0)
     c->nsplit *= 2;
   c->nblk_total = c->npad / kThreads;
 
@@ -303,7 +311,7 @@ static int ensure_work(pgxc_ctx *c, size_t elem) {
 
 template <typename T>
 static int run_chunks(pgxc_ctx *c, bool gga, bool incremental, uint32_t functional,
-                      const T *D) {
+                      const T *D, bool bf16x3) {
   hipStream_t s = c->stream;
   T *phi = (T *)c->phi, *W = (T *)c->W;
   float ms[4] = {0, 0, 0, 0};
@@ -313,14 +321,28 @@ static int run_chunks(pgxc_ctx *c, bool gga, bool incremental, uint32_t function
     launch_basis<T>(s, gga, c->grid, c->grid + c->npad, c->grid + 2 * c->npad, p0, c->Mc,
                     c->n_atoms, c->atoms, c->shells, c->exps, c->norms, c->contr, phi, c->NP);
     HIPCHK(c, hipEventRecord(c->ev[1], s));
-    launch_density<T>(s, gga, phi, D, c->Mc, c->NP, c->part);
+    if constexpr (std::is_same<T, float>::value) {
+      if (bf16x3)
+        launch_density_bf16x3(s, gga, phi, D, c->Mc, c->NP, c->tf, c->part);
+      else
+        launch_density<T>(s, gga, phi, D, c->Mc, c->NP, c->tf, c->part);
+    } else {
+      launch_density<T>(s, gga, phi, D, c->Mc, c->NP, c->tf, c->part);
+    }
     HIPCHK(c, hipEventRecord(c->ev[2], s));
     launch_functional(s, gga, incremental, functional, c->part, c->ntiles, c->Mc,
                       c->grid + 3 * c->npad, p0, c->state, c->npad, c->coef, c->epart,
                       p0 / kThreads);
     launch_wmat<T>(s, gga, phi, c->coef, c->Mc, c->NP, W);
     HIPCHK(c, hipEventRecord(c->ev[3], s));
-    launch_potential<T>(s, W, phi, c->Mc, c->NP, c->nsplit, c->Hpart, ch == 0);
+    if constexpr (std::is_same<T, float>::value) {
+      if (bf16x3)
+        launch_potential_bf16x3(s, W, phi, c->Mc, c->NP, c->tf, c->nsplit, c->Hpart, ch == 0);
+      else
+        launch_potential<T>(s, W, phi, c->Mc, c->NP, c->tf, c->nsplit, c->Hpart, ch == 0);
+    } else {
+      launch_potential<T>(s, W, phi, c->Mc, c->NP, c->tf, c->nsplit, c->Hpart, ch == 0);
+    }
     HIPCHK(c, hipEventRecord(c->ev[4], s));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventSynchronize(c->ev[4]));
@@ -343,7 +365,7 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
     return fail(c, PGXC_ERR_UNSUPPORTED, "functional mask has terms other than XAlpha, PWLDAc, PBEx, PBEc");
   if ((functional & PGXC_XC_PBEC) && !(functional & PGXC_XC_PWLDAC))
     return fail(c, PGXC_ERR_ARG, "PBEc needs the PW92 LDA correlation it is built on (PWLDAc)");
-  if (precision != PGXC_PREC_FP64 && precision != PGXC_PREC_FP32)
+  if (precision != PGXC_PREC_FP64 && precision != PGXC_PREC_FP32 && precision != PGXC_PREC_BF16X3)
     return fail(c, PGXC_ERR_UNSUPPORTED, "precision mode " + std::to_string(precision) + " not available");
   if (!c->have_density) return fail(c, PGXC_ERR_STATE, "pgxc_build before pgxc_set_density/orbitals");
   const bool gga = (functional & (PGXC_XC_PBEX | PGXC_XC_PBEC)) != 0;
@@ -365,12 +387,14 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
   int rc;
   if (precision == PGXC_PREC_FP64) {
     if ((rc = ensure_work(c, 8))) return rc;
-    if ((rc = run_chunks<double>(c, gga, incremental, functional, c->Dop))) return rc;
+    if ((rc = run_chunks<double>(c, gga, incremental, functional, c->Dop, false))) return rc;
   } else {
     if ((rc = dalloc(c, &c->Dlow, np2))) return rc;
     launch_cast<float>(s, c->Dop, np2, c->Dlow);
     if ((rc = ensure_work(c, 4))) return rc;
-    if ((rc = run_chunks<float>(c, gga, incremental, functional, c->Dlow))) return rc;
+    if ((rc = run_chunks<float>(c, gga, incremental, functional, c->Dlow,
+                                 precision == PGXC_PREC_BF16X3)))
+      return rc;
   }
   HIPCHK(c, hipEventRecord(c->ev[5], s));
   launch_pack(s, c->Hpart, c->nsplit, c->NP, c->nbas, incremental ? c->Vstate : nullptr, c->Vout);

@@ -38,10 +38,11 @@ void launch_basis(hipStream_t s, bool grad, const double *gx, const double *gy,
 
 // X = Phi D (D = P or dP, NP x NP); per-point partial sums over each
 // 128-function column tile: part[t][c][p], c = rho, 2 X.dPhi/dx, ...
+// tf = function tile (64, 96 or 128), NP a multiple of it
 template <typename T>
-void launch_density(hipStream_t s, bool grad, const T *phi, const T *D, long Mc, int NP,
+void launch_density(hipStream_t s, bool grad, const T *phi, const T *D, long Mc, int NP, int tf,
                     double *part);
-int density_tiles(int NP);
+int density_tiles(int NP, int tf);
 
 // Functional on points of one chunk: sums the partials (in tile order), adds
 // the stored state when incremental, evaluates f, f_rho, f_gamma, forms the
@@ -60,14 +61,14 @@ void launch_wmat(hipStream_t s, bool grad, const T *phi, const double *coef, lon
 
 // Hpart[sl][i][j] (+)= sum_{p in split sl} W[i][p] phi0[j][p]
 template <typename T>
-void launch_potential(hipStream_t s, const T *W, const T *phi, long Mc, int NP,
+void launch_potential(hipStream_t s, const T *W, const T *phi, long Mc, int NP, int tf,
                       int nsplit, double *Hpart, bool first);
 
 // split-BF16 GEMMs read FP32 operands and split while staging
 void launch_density_bf16x3(hipStream_t s, bool grad, const float *phi, const float *D,
-                           long Mc, int NP, double *part);
+                           long Mc, int NP, int tf, double *part);
 void launch_potential_bf16x3(hipStream_t s, const float *W, const float *phi, long Mc,
-                             int NP, int nsplit, double *Hpart, bool first);
+                             int NP, int tf, int nsplit, double *Hpart, bool first);
 
 // V (packed, n(n+1)/2) = base + sum_sl (H[sl] + H[sl]^T) restricted to n x n
 void launch_pack(hipStream_t s, const double *Hpart, int nsplit, int NP, int n,

@@ -11,43 +11,9 @@
 //   k_potential   H = W^T Phi on MFMA, split over points  (:1528-1531)
 //   k_pack        V = H + H^T into the packed lower triangle (:1561-1567)
 #include "pgxc_internal.h"
+#include "xc_device.h"
 
 namespace pgxc {
-
-// ------------------------------------------------------------------ helpers
-// Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
-// contiguous range of tile ids so that tiles sharing an operand panel share
-// that XCD's L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
-  const int per = nblk >> 3;
-  if (bid >= (per << 3)) return bid;
-  return (bid & 7) * per + (bid >> 3);
-}
-
-template <typename T>
-struct MF;
-// 16x16x4 MFMA operand/result placement (wave64), both types:
-//   A(i,k): lane = i + 16 k      B(k,j): lane = j + 16 k
-// Result element r of lane l (measured on gfx950):
-//   f64: D(i,j) with i = l / 16 + 4 r, j = l % 16
-//   f32: D(i,j) with i = 4 (l / 16) + r, j = l % 16
-template <>
-struct MF<double> {
-  typedef d4 acc;
-  __device__ static __forceinline__ acc mma(double a, double b, acc c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  __device__ static __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
-};
-template <>
-struct MF<float> {
-  typedef f4 acc;
-  __device__ static __forceinline__ acc mma(float a, float b, acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  __device__ static __forceinline__ int row(int lane, int r) { return ((lane >> 4) << 2) + r; }
-};
-__device__ __forceinline__ int mf_col(int lane) { return lane & 15; }
 
 // Block tile BM x BN over K in steps of BK; 4 waves arranged 2 x 2, wave tile
 // (BM/2) x (BN/2) made of 16x16 MFMA blocks.  Two LDS stages, next stage
@@ -327,16 +293,16 @@ template void launch_basis<float>(hipStream_t, bool, const double *, const doubl
                                   const double *, float *, long);
 
 // ----------------------------------------------------------------- density
-constexpr int kDBM = 128, kDBN = 128, kDBK = 16;
-int density_tiles(int NP) { return NP / kDBN; }
+constexpr int kDBM = 128, kDBK = 16;
+int density_tiles(int NP, int tf) { return NP / tf; }
 
 // One workgroup: 128 points x 128 functions of X = Phi D, then the partial
 // contractions of that column tile for each point.
-template <typename T, bool GRAD>
+template <typename T, int BN, bool GRAD>
 __global__ __launch_bounds__(kThreads) void k_density(const T *__restrict__ phi,
                                                       const T *__restrict__ D, long Mc, int NP,
                                                       double *__restrict__ part, int ntn) {
-  constexpr int BM = kDBM, BN = kDBN, BK = kDBK, WM = BM / 2, WN = BN / 2;
+  constexpr int BM = kDBM, BK = kDBK, WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
   __shared__ T As[2 * BK * BM];
   __shared__ T Bs[2 * BK * BN];
@@ -348,68 +314,33 @@ __global__ __launch_bounds__(kThreads) void k_density(const T *__restrict__ phi,
   typename MF<T>::acc acc[TM][TN];
   gemm_tile<T, BM, BN, BK, false, false>(phi + m0, Mc, D + n0, NP, NP / BK, As, Bs, acc);
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const long cs = (long)NP * Mc;
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = wm * WM + i * 16 + MF<T>::row(lane, r);
-      const long p = m0 + row;
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        const long n = n0 + wn * WN + j * 16 + mf_col(lane);
-        const double xv = (double)acc[i][j][r];
-        const long q = n * Mc + p;
-        s0 += (double)phi[q] * xv;
-        if (GRAD) {
-          s1 += 2.0 * (double)phi[cs + q] * xv;
-          s2 += 2.0 * (double)phi[2 * cs + q] * xv;
-          s3 += 2.0 * (double)phi[3 * cs + q] * xv;
-        }
-      }
-#pragma unroll
-      for (int off = 8; off >= 1; off >>= 1) {
-        s0 += __shfl_xor(s0, off);
-        if (GRAD) {
-          s1 += __shfl_xor(s1, off);
-          s2 += __shfl_xor(s2, off);
-          s3 += __shfl_xor(s3, off);
-        }
-      }
-      if ((lane & 15) == 0) {
-        red[wn][0][row] = s0;
-        if (GRAD) {
-          red[wn][1][row] = s1;
-          red[wn][2][row] = s2;
-          red[wn][3][row] = s3;
-        }
-      }
-    }
-  __syncthreads();
-  constexpr int NC = GRAD ? 4 : 1;
-  for (int idx = tid; idx < NC * BM; idx += kThreads) {
-    const int c = idx / BM, row = idx % BM;
-    part[((long)tn * 4 + c) * Mc + m0 + row] = red[0][c][row] + red[1][c][row];
-  }
+  density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, Mc, NP, m0, n0, tn, part, red);
+}
+
+template <typename T, int BN>
+static void density_bn(hipStream_t s, bool grad, const T *phi, const T *D, long Mc, int NP,
+                       double *part) {
+  const int ntn = NP / BN, ntm = (int)(Mc / kDBM);
+  dim3 grid((unsigned)(ntm * ntn));
+  if (grad)
+    hipLaunchKernelGGL((k_density<T, BN, true>), grid, dim3(kThreads), 0, s, phi, D, Mc, NP, part, ntn);
+  else
+    hipLaunchKernelGGL((k_density<T, BN, false>), grid, dim3(kThreads), 0, s, phi, D, Mc, NP, part, ntn);
 }
 
 template <typename T>
-void launch_density(hipStream_t s, bool grad, const T *phi, const T *D, long Mc, int NP,
+void launch_density(hipStream_t s, bool grad, const T *phi, const T *D, long Mc, int NP, int tf,
                     double *part) {
-  const int ntn = NP / kDBN, ntm = (int)(Mc / kDBM);
-  dim3 grid((unsigned)(ntm * ntn));
-  if (grad)
-    hipLaunchKernelGGL((k_density<T, true>), grid, dim3(kThreads), 0, s, phi, D, Mc, NP, part, ntn);
-  else
-    hipLaunchKernelGGL((k_density<T, false>), grid, dim3(kThreads), 0, s, phi, D, Mc, NP, part, ntn);
+  switch (tf) {
+    case 64: density_bn<T, 64>(s, grad, phi, D, Mc, NP, part); break;
+    case 96: density_bn<T, 96>(s, grad, phi, D, Mc, NP, part); break;
+    default: density_bn<T, 128>(s, grad, phi, D, Mc, NP, part); break;
+  }
 }
 template void launch_density<double>(hipStream_t, bool, const double *, const double *, long,
-                                     int, double *);
+                                     int, int, double *);
 template void launch_density<float>(hipStream_t, bool, const float *, const float *, long, int,
-                                    double *);
+                                    int, double *);
 
 // -------------------------------------------------------------- functional
 // Closed-shell terms; each returns F, dF/drho, dF/dgamma added into (f, fr, fg).
@@ -647,14 +578,14 @@ template void launch_wmat<float>(hipStream_t, bool, const float *, const double 
                                  float *);
 
 // --------------------------------------------------------------- potential
-constexpr int kPBM = 128, kPBN = 128, kPBK = 16;
+constexpr int kPBK = 16;
 
-template <typename T>
+template <typename T, int BT>
 __global__ __launch_bounds__(kThreads) void k_potential(const T *__restrict__ W,
                                                         const T *__restrict__ phi, long Mc,
                                                         int NP, int ksteps, double *Hpart,
                                                         int ntn, int first) {
-  constexpr int BM = kPBM, BN = kPBN, BK = kPBK, WM = BM / 2, WN = BN / 2;
+  constexpr int BM = BT, BN = BT, BK = kPBK, WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
   __shared__ T As[2 * BK * BM];
   __shared__ T Bs[2 * BK * BN];
@@ -664,36 +595,32 @@ __global__ __launch_bounds__(kThreads) void k_potential(const T *__restrict__ W,
   typename MF<T>::acc acc[TM][TN];
   gemm_tile<T, BM, BN, BK, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb, Mc,
                                        ksteps, As, Bs, acc);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  double *H = Hpart + (long)blockIdx.y * NP * NP;
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long row = i0 + wm * WM + i * 16 + MF<T>::row(lane, r);
-        const long col = j0 + wn * WN + j * 16 + mf_col(lane);
-        const double v = (double)acc[i][j][r];
-        const long q = row * NP + col;
-        H[q] = first ? v : H[q] + v;
-      }
+  potential_epilogue<T, typename MF<T>::acc, BM, BN>(acc, Hpart + (long)blockIdx.y * NP * NP, NP, i0, j0, first);
+}
+
+template <typename T, int BT>
+static void potential_bt(hipStream_t s, const T *W, const T *phi, long Mc, int NP, int nsplit,
+                         double *Hpart, bool first) {
+  const int nt = NP / BT;
+  const int ksteps = (int)(Mc / kPBK) / nsplit;
+  dim3 grid((unsigned)(nt * nt), (unsigned)nsplit);
+  hipLaunchKernelGGL((k_potential<T, BT>), grid, dim3(kThreads), 0, s, W, phi, Mc, NP, ksteps,
+                     Hpart, nt, (int)first);
 }
 
 template <typename T>
-void launch_potential(hipStream_t s, const T *W, const T *phi, long Mc, int NP, int nsplit,
-                      double *Hpart, bool first) {
-  const int ntn = NP / kPBN, ntm = NP / kPBM;
-  const int ksteps = (int)(Mc / kPBK) / nsplit;
-  dim3 grid((unsigned)(ntm * ntn), (unsigned)nsplit);
-  hipLaunchKernelGGL((k_potential<T>), grid, dim3(kThreads), 0, s, W, phi, Mc, NP, ksteps,
-                     Hpart, ntn, (int)first);
+void launch_potential(hipStream_t s, const T *W, const T *phi, long Mc, int NP, int tf,
+                      int nsplit, double *Hpart, bool first) {
+  switch (tf) {
+    case 64: potential_bt<T, 64>(s, W, phi, Mc, NP, nsplit, Hpart, first); break;
+    case 96: potential_bt<T, 96>(s, W, phi, Mc, NP, nsplit, Hpart, first); break;
+    default: potential_bt<T, 128>(s, W, phi, Mc, NP, nsplit, Hpart, first); break;
+  }
 }
 template void launch_potential<double>(hipStream_t, const double *, const double *, long, int,
-                                       int, double *, bool);
+                                       int, int, double *, bool);
 template void launch_potential<float>(hipStream_t, const float *, const float *, long, int, int,
-                                      double *, bool);
+                                      int, double *, bool);
 
 // -------------------------------------------------------------------- pack
 __global__ __launch_bounds__(kThreads) void k_pack(const double *__restrict__ Hpart, int nsplit,

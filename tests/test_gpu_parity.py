@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from pgxc import XCBuilder, basis, grid, systems
-from pgxc.xc import FP32, FP64, LDA, PBE
+from pgxc.xc import BF16X3, FP32, FP64, LDA, PBE
 
 pytestmark = pytest.mark.gpu
 
@@ -89,12 +89,13 @@ def test_results_accumulate_into_outputs(h2o):
     assert np.array_equal(ham, h2) and e == e2 and q == q2
 
 
-def test_h2o_fp32_full_build(h2o):
-    # FP32 MFMA contractions of the full density: error bound stated, not FP64
+@pytest.mark.parametrize("prec", [FP32, BF16X3])
+def test_h2o_mixed_full_build(h2o, prec):
+    # low-precision contractions of the full density: error bound stated, not FP64
     b, g, C, occ = h2o
     with XCBuilder(b, g) as xc:
         xc.set_orbitals(C, occ)
-        res = xc.build(PBE, FP32)
+        res = xc.build(PBE, prec)
     ref = oracle_result("h2o_pbe", b, g, oracle.MASK_PBE, C=C, occ=occ)
     check(res, ref, e_tol=1e-5, v_tol=1e-5, q_tol=1e-5)
 
@@ -110,16 +111,17 @@ def test_incremental_fp64_equals_full(h2o):
     check(inc, oracle_result("h2o_pbe_c1", b, g, oracle.MASK_PBE, C=C1, occ=occ))
 
 
+@pytest.mark.parametrize("prec", [FP32, BF16X3])
 @pytest.mark.parametrize("norm", [1e-2, 1e-4, 1e-6])
-def test_incremental_fp32_error_tracks_dP(h2o, norm):
+def test_incremental_mixed_error_tracks_dP(h2o, norm, prec):
     b, g, C, occ = h2o
     C1 = systems.perturbed_orbitals(C, norm, seed=10)
     with XCBuilder(b, g) as xc:
         xc.set_orbitals(C, occ)
         xc.build(PBE, FP64)
         xc.set_orbitals(C1, occ, incremental=True)
-        inc = xc.build(PBE, FP32)
-    ref = oracle.xc_build(b, g, oracle.MASK_PBE, C=C1, occ=occ, nthreads=NT)
+        inc = xc.build(PBE, prec)
+    ref = oracle_result(("h2o_pbe_inc", norm), b, g, oracle.MASK_PBE, C=C1, occ=occ)
     # low-precision error proportional to ||dP||, plus the FP64 floor
     check(inc, ref, e_tol=1e-9 + 1e-3 * norm, v_tol=1e-10 + 1e-3 * norm, q_tol=1e-9 + 1e-3 * norm)
 
