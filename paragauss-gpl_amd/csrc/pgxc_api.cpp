@@ -40,6 +40,7 @@ struct pgxc_ctx {
   void *phi = nullptr;
   void *W = nullptr;
   size_t phi_bytes = 0, w_bytes = 0;
+  size_t phi_elem = 0;  // element size the padding rows of phi were last cleared for
   double *part = nullptr, *coef = nullptr, *Hpart = nullptr;
   // device: state and results
   double *state = nullptr;  // [S_NSLOT][npad]
@@ -295,9 +296,15 @@ static int ensure_work(pgxc_ctx *c, size_t elem) {
     c->phi = nullptr;
     c->phi_bytes = 0;
     HIPCHK(c, hipMalloc(&c->phi, need_phi));
-    // function rows nbas..NP-1 are never written and must read as zero
-    HIPCHK(c, hipMemset(c->phi, 0, need_phi));
     c->phi_bytes = need_phi;
+    c->phi_elem = 0;
+  }
+  // function rows nbas..NP-1 are never written and must read as zero in the
+  // element type of this build (a buffer last used at another precision
+  // holds that precision's bytes there)
+  if (c->phi_elem != elem) {
+    HIPCHK(c, hipMemsetAsync(c->phi, 0, c->phi_bytes, c->stream));
+    c->phi_elem = elem;
   }
   if (c->w_bytes < need_w) {
     dfree(c->W);

@@ -74,7 +74,7 @@ def molecular_grid(elements, atom_xyz, nrad=75, nang=302, partition="auto",
     own = np.concatenate(own)
     if partition == "auto":
         partition = "numpy" if pts.shape[0] * len(elements) ** 2 < 5e7 else "gpu"
-    if len(elements) == 1:
+    if len(elements) == 1 or partition == "none":
         w = wat
     elif partition == "numpy":
         w = becke_numpy(atom_xyz, pts, own, wat)

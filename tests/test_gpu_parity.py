@@ -3,7 +3,10 @@ identical inputs.
 
 Tolerances (north-star bar for the plain FP64 path):
   |dE_xc| <= 1e-9 Eh, max |dV_xc| <= 1e-10, |dcharge| <= 1e-9.
-Mixed precision (FP32 MFMA contractions) states its own bounds below.
+Mixed precision: a contraction in FP32 carries a relative error ~1e-6 after
+accumulation, split-BF16 ~2^-16 per product (a_lo b_lo dropped, lo rounded).
+Full mixed builds are bounded relative to |E_xc|, max|V| and the charge;
+incremental mixed builds by the FP64 floor plus a multiple of ||dP||_F.
 """
 import os
 import subprocess
@@ -97,7 +100,9 @@ def test_h2o_mixed_full_build(h2o, prec):
         xc.set_orbitals(C, occ)
         res = xc.build(PBE, prec)
     ref = oracle_result("h2o_pbe", b, g, oracle.MASK_PBE, C=C, occ=occ)
-    check(res, ref, e_tol=1e-5, v_tol=1e-5, q_tol=1e-5)
+    rel = 2e-6 if prec == FP32 else 5e-5
+    vmax = np.max(np.abs(ref[0]))
+    check(res, ref, e_tol=rel * abs(ref[1]), v_tol=10 * rel * vmax, q_tol=rel * abs(ref[2]))
 
 
 def test_incremental_fp64_equals_full(h2o):
@@ -123,7 +128,8 @@ def test_incremental_mixed_error_tracks_dP(h2o, norm, prec):
         inc = xc.build(PBE, prec)
     ref = oracle_result(("h2o_pbe_inc", norm), b, g, oracle.MASK_PBE, C=C1, occ=occ)
     # low-precision error proportional to ||dP||, plus the FP64 floor
-    check(inc, ref, e_tol=1e-9 + 1e-3 * norm, v_tol=1e-10 + 1e-3 * norm, q_tol=1e-9 + 1e-3 * norm)
+    k = 1e-3 if prec == FP32 else 5e-3
+    check(inc, ref, e_tol=1e-9 + k * norm, v_tol=1e-10 + k * norm, q_tol=1e-9 + k * norm)
 
 
 def test_water_cluster_medium_basis_fp64():
@@ -142,10 +148,13 @@ def test_becke_gpu_matches_numpy():
     els, xyz = systems.water_cluster(2, seed=1)
     g_np = grid.molecular_grid(els, xyz, nrad=30, nang=302, partition="numpy")
     g_gpu = grid.molecular_grid(els, xyz, nrad=30, nang=302, partition="gpu")
-    assert np.allclose(g_np[:, 3], g_gpu[:, 3], rtol=1e-12, atol=1e-300)
-    # partition of unity: the weights of all atomic grids integrate 1 like the atomic ones
+    w_at = grid.molecular_grid(els, xyz, nrad=30, nang=302, partition="none")[:, 3]
+    # the partition factor P_owner / sum P lies in [0, 1]; compare it absolutely
+    assert np.max(np.abs(g_np[:, 3] - g_gpu[:, 3]) / w_at) < 1e-12
+    # partition of unity: the partitioned grid integrates a normalised Gaussian
+    # to the quadrature error of this coarse (30 radial shells) grid
     r2 = np.sum((g_gpu[:, :3] - xyz[0]) ** 2, axis=1)
-    assert np.sum(g_gpu[:, 3] * (2 / np.pi) ** 1.5 * np.exp(-2 * r2)) == pytest.approx(1.0, abs=1e-6)
+    assert np.sum(g_gpu[:, 3] * (2 / np.pi) ** 1.5 * np.exp(-2 * r2)) == pytest.approx(1.0, abs=1e-4)
 
 
 def test_fortran_driver_matches(h2o, tmp_path):
