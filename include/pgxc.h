@@ -57,6 +57,16 @@ extern "C" {
 #define PGXC_DENSITY_INCREMENTAL 1 /* next build integrates dP = P - P_prev   */
 
 typedef struct pgxc_ctx pgxc_ctx;
+typedef struct pgxc_batch pgxc_batch;
+
+/* Layout options of a context (pgxc_create_opt).  tile: function tile of the
+ * MFMA contractions, 64, 96 or 128 (0: the one padding nbas least).
+ * chunk_points: largest number of grid points held in one work chunk, a
+ * multiple of 256 (0: as many as keep the FP64 basis values under 2 GiB). */
+typedef struct pgxc_options {
+  int32_t tile;
+  int64_t chunk_points;
+} pgxc_options;
 
 /*
  * Create a context on the current HIP device.
@@ -82,6 +92,18 @@ int pgxc_create(pgxc_ctx **ctx, int32_t n_atoms, const double *atom_xyz,
                 int32_t n_shells, const int32_t *shell_info, const double *exps,
                 const double *norms, const double *contr, const double *grid,
                 int64_t npts, int32_t veclen);
+
+/* pgxc_create with layout options (opt may be NULL).  No counterpart in the
+ * reference: the batch path (pgxc_batch_*) uses it to size many contexts. */
+int pgxc_create_opt(pgxc_ctx **ctx, int32_t n_atoms, const double *atom_xyz,
+                    int32_t n_shells, const int32_t *shell_info, const double *exps,
+                    const double *norms, const double *contr, const double *grid,
+                    int64_t npts, int32_t veclen, const pgxc_options *opt);
+
+/* Layout chosen for a context: [0] padded nbas, [1] function tile,
+ * [2] points per chunk, [3] chunks, [4] point splits of the potential
+ * contraction, [5] padded number of points. */
+int pgxc_layout(const pgxc_ctx *ctx, int64_t *out6);
 
 /*
  * Density matrix P (nbas x nbas, column-major, symmetric), total density for
@@ -144,6 +166,34 @@ int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64_t npts,
                        double *w_out);
 
 int pgxc_destroy(pgxc_ctx *ctx);
+
+/*
+ * Batches: the XC builds of many molecules (contexts, each with its own
+ * basis, grid and density) run together, one launch per stage over the work
+ * items of all of them, optionally replayed from a captured hipGraph.  In the
+ * reference every molecule is a separate ParaGauss run; the batch replaces that
+ * loop of xc_build calls.  Each context keeps its own results
+ * (pgxc_get_result) and incremental state, exactly as after pgxc_build.
+ * Results do not depend on the schedule: every work item writes its own
+ * partial sums, reduced in a fixed order.
+ */
+int pgxc_batch_create(pgxc_batch **batch, int32_t n, pgxc_ctx *const *ctxs);
+
+/* Schedule priority per molecule (n doubles, e.g. predicted cost; larger runs
+ * first); NULL restores the natural order. */
+int pgxc_batch_set_priority(pgxc_batch *batch, const double *cost);
+
+/* One build of every molecule; use_graph = 1 captures the launch sequence
+ * into a hipGraph on first use and replays it afterwards. */
+int pgxc_batch_build(pgxc_batch *batch, uint32_t functional, int32_t precision,
+                     int32_t use_graph);
+
+/* Device time of the last batch build (ms). */
+int pgxc_batch_last_ms(const pgxc_batch *batch, double *ms);
+
+const char *pgxc_batch_last_error(const pgxc_batch *batch);
+
+int pgxc_batch_destroy(pgxc_batch *batch);
 
 #ifdef __cplusplus
 }

@@ -1,16 +1,19 @@
 // C-ABI layer of the MI355X XC build (declared in include/pgxc.h).
 //
-// Orchestration of one build, chunk by chunk over the grid (a chunk is a
-// whole number of reference batches, so the sums are those of calc_xcks_nl,
-// ParaGauss modules/xc_hamiltonian.f90:1348-1575, regrouped):
-//   basis -> density contraction -> functional -> W -> potential contraction
-// followed by one symmetrisation into the packed lower triangle.
+// A build is a plan: for every super-step, the units (grid chunks) of every
+// molecule taking part, and per stage the list of work items over those units.
+// Stage kernels walk their lists persistently; one molecule is a batch of one.
+// Per super-step:  basis -> density contraction -> functional -> W ->
+// potential contraction; then one packing into the lower triangle per molecule.
+// Chunks are whole numbers of reference batches, so the sums are those of
+// calc_xcks_nl (ParaGauss modules/xc_hamiltonian.f90:1348-1575), regrouped.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -20,33 +23,43 @@
 
 using namespace pgxc;
 
+namespace {
+
+struct Plan;
+
+long round_up(long a, long b) { return (a + b - 1) / b * b; }
+
+void dfree(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
 struct pgxc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
   // sizes
-  int n_atoms = 0, n_shells = 0, nbas = 0, NP = 0, veclen = 128;
+  int n_atoms = 0, n_shells = 0, nbas = 0, NP = 0, veclen = 128, tf = 128;
   long npts = 0, npad = 0, Mc = 0;
-  int nchunks = 0, nsplit = 1, ntiles = 0, tf = 128;
+  int nchunks = 0, nsplit = 1, ntiles = 0;
+  long nblk_total = 0;
   // device: geometry and basis
   double *grid = nullptr;  // [4][npad] x, y, z, w
   AtomD *atoms = nullptr;
   ShellD *shells = nullptr;
   double *exps = nullptr, *norms = nullptr, *contr = nullptr;
-  // device: density
-  double *P = nullptr, *Pref = nullptr, *Dop = nullptr;  // NP x NP
-  float *Dlow = nullptr;
-  // device: work buffers (sized for the largest precision used so far)
-  void *phi = nullptr;
-  void *W = nullptr;
-  size_t phi_bytes = 0, w_bytes = 0;
-  size_t phi_elem = 0;  // element size the padding rows of phi were last cleared for
+  // device: density (NP x NP); D holds the operand in the build's precision
+  double *P = nullptr, *Pref = nullptr;
+  void *D = nullptr;
+  // device: work buffers of one chunk
+  void *phi = nullptr, *W = nullptr;
+  size_t phi_bytes = 0, w_bytes = 0, phi_elem = 0;
   double *part = nullptr, *coef = nullptr, *Hpart = nullptr;
   // device: state and results
-  double *state = nullptr;  // [S_NSLOT][npad]
+  double *state = nullptr;                    // [S_NSLOT][npad]
   double *Vstate = nullptr, *Vout = nullptr;  // packed
   double *epart = nullptr;
-  long nblk_total = 0;
   // host
   int have_density = 0, density_mode = 0, have_state = 0;
   uint32_t state_functional = 0;
@@ -54,6 +67,8 @@ struct pgxc_ctx {
   double timing[6] = {0, 0, 0, 0, 0, 0};
   double stats[5] = {0, 0, 0, 0, 0};
   std::vector<hipEvent_t> ev;
+  Plan *plan = nullptr;
+  double exps_per_point = 0;
 };
 
 namespace {
@@ -71,8 +86,6 @@ int fail(pgxc_ctx *c, int code, const std::string &msg) {
                   std::string(#call) + ": " + hipGetErrorString(e_));                  \
   } while (0)
 
-long round_up(long a, long b) { return (a + b - 1) / b * b; }
-
 template <typename T>
 int dalloc(pgxc_ctx *c, T **p, size_t count) {
   if (*p) return PGXC_OK;
@@ -84,13 +97,367 @@ int dalloc(pgxc_ctx *c, T **p, size_t count) {
   return PGXC_OK;
 }
 
-void dfree(void *p) {
-  if (p) (void)hipFree(p);
+const uint32_t kSupported = PGXC_XC_PBE;
+const int kTiles[3] = {64, 96, 128};
+
+// ------------------------------------------------------------------- plans
+struct StepPlan {
+  int nunits = 0;
+  StageList basis, func, wmat;
+  StageList dens[3], pot[3];
+};
+
+struct Plan {
+  uint32_t functional = 0;
+  int precision = 0;
+  std::vector<int> incr;  // per molecule
+  long order_version = -1;
+  std::vector<StepPlan> steps;
+  PackD *d_pack = nullptr;
+  long *d_pack_base = nullptr, *d_prep_base = nullptr;
+  long pack_total = 0, prep_total = 0;
+  int nmol = 0;
+  char *blob = nullptr;  // device memory holding every list
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_src = nullptr;
+  ~Plan() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (graph_src) (void)hipGraphDestroy(graph_src);
+    dfree(blob);
+  }
+};
+
+void free_plan(Plan *&p) {
+  delete p;
+  p = nullptr;
 }
 
-const uint32_t kSupported = PGXC_XC_PBE;
+// Schedule of one stage list: a unit's items stay together, units in
+// descending priority (largest predicted cost first).  Position L of the
+// persistent walk runs on XCD L % 8, so contiguous runs of the sequence are
+// spread to keep neighbouring items on one XCD.  Without priorities the kernels
+// use the natural XCD-blocked order.
+void make_order(const std::vector<long> &base, const std::vector<double> &prio,
+                std::vector<int> &order) {
+  order.clear();
+  if (prio.empty()) return;
+  const int nu = (int)base.size() - 1;
+  std::vector<int> us(nu);
+  std::iota(us.begin(), us.end(), 0);
+  std::stable_sort(us.begin(), us.end(), [&](int a, int b) { return prio[a] > prio[b]; });
+  std::vector<int> seq;
+  seq.reserve(base[nu]);
+  for (int u : us)
+    for (long i = base[u]; i < base[u + 1]; ++i) seq.push_back((int)i);
+  const long total = base[nu];
+  order.resize(total);
+  const long per = total / 8;
+  for (long L = 0; L < total; ++L) {
+    long r = L;
+    if (L < per * 8) r = (L % 8) * per + L / 8;
+    order[L] = seq[r];
+  }
+}
+
+enum ItemKind { K_BASIS, K_FUNC, K_WMAT, K_DENS, K_POT };
+
+long unit_items(const UnitD &U, ItemKind k, int tf) {
+  switch (k) {
+    case K_BASIS: return items_basis(U.Mc, U.n_atoms);
+    case K_FUNC: return items_functional(U.Mc);
+    case K_WMAT: return items_wmat(U.Mc, U.NP);
+    case K_DENS: return items_density(U.Mc, U.NP, tf);
+    default: return items_potential(U.NP, tf, U.nsplit);
+  }
+}
 
 }  // namespace
+
+// Build (or reuse) the plan of a set of molecules for one configuration.
+static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int precision,
+                    const std::vector<int> &incr, const std::vector<double> &unit_prio,
+                    long order_version, Plan *&plan, pgxc_ctx *errc) {
+  if (plan && plan->functional == functional && plan->precision == precision &&
+      plan->incr == incr && plan->order_version == order_version)
+    return PGXC_OK;
+  free_plan(plan);
+  plan = new Plan();
+  plan->functional = functional;
+  plan->precision = precision;
+  plan->incr = incr;
+  plan->order_version = order_version;
+  const int nm = (int)mols.size();
+  plan->nmol = nm;
+  int nsteps = 0;
+  for (auto *c : mols) nsteps = std::max(nsteps, c->nchunks);
+
+  // host image of everything that goes to the device
+  std::vector<char> img;
+  auto put = [&](const void *src, size_t bytes) {
+    const size_t off = (size_t)round_up((long)img.size(), 256);
+    img.resize(off + std::max<size_t>(bytes, 1));
+    if (bytes) memcpy(img.data() + off, src, bytes);
+    return off;
+  };
+  struct Pending {
+    size_t units_off = 0, base_off = 0, order_off = 0;
+    int nu = 0;
+    long total = 0;
+    bool has_order = false;
+  };
+  auto stage = [&](const std::vector<UnitD> &us, const std::vector<int> &um, ItemKind kind,
+                   int tf) {
+    Pending P;
+    P.nu = (int)us.size();
+    std::vector<long> base(us.size() + 1, 0);
+    for (size_t i = 0; i < us.size(); ++i) base[i + 1] = base[i] + unit_items(us[i], kind, tf);
+    P.total = base.back();
+    P.units_off = put(us.data(), sizeof(UnitD) * us.size());
+    P.base_off = put(base.data(), sizeof(long) * base.size());
+    std::vector<double> pr;
+    if (!unit_prio.empty())
+      for (int m : um) pr.push_back(unit_prio[m]);
+    std::vector<int> order;
+    make_order(base, pr, order);
+    P.has_order = !order.empty();
+    if (P.has_order) P.order_off = put(order.data(), sizeof(int) * order.size());
+    return P;
+  };
+  std::vector<std::vector<Pending>> pend(nsteps);  // basis, func, wmat, dens x3, pot x3
+  for (int k = 0; k < nsteps; ++k) {
+    std::vector<UnitD> units;
+    std::vector<int> umol;
+    for (int m = 0; m < nm; ++m) {
+      pgxc_ctx *c = mols[m];
+      if (k >= c->nchunks) continue;
+      const long p0 = (long)k * c->Mc;
+      UnitD U;
+      memset(&U, 0, sizeof(U));
+      U.gx = c->grid + p0;
+      U.gy = c->grid + c->npad + p0;
+      U.gz = c->grid + 2 * c->npad + p0;
+      U.gw = c->grid + 3 * c->npad + p0;
+      U.atoms = c->atoms;
+      U.shells = c->shells;
+      U.exps = c->exps;
+      U.norms = c->norms;
+      U.contr = c->contr;
+      U.phi = c->phi;
+      U.W = c->W;
+      U.D = c->D;
+      U.part = c->part;
+      U.coef = c->coef;
+      U.state = c->state + p0;
+      U.epart = c->epart + 2 * (p0 / kThreads);
+      U.Hpart = c->Hpart;
+      U.Mc = c->Mc;
+      U.npad = c->npad;
+      U.n_atoms = c->n_atoms;
+      U.NP = c->NP;
+      U.ntiles = c->ntiles;
+      U.nsplit = c->nsplit;
+      U.first = (k == 0);
+      U.incremental = incr[m];
+      U.mask = functional;
+      units.push_back(U);
+      umol.push_back(m);
+    }
+    pend[k].push_back(stage(units, umol, K_BASIS, 0));
+    pend[k].push_back(stage(units, umol, K_FUNC, 0));
+    pend[k].push_back(stage(units, umol, K_WMAT, 0));
+    for (int kind = 0; kind < 2; ++kind)
+      for (int t = 0; t < 3; ++t) {
+        std::vector<UnitD> sub;
+        std::vector<int> subm;
+        for (size_t i = 0; i < units.size(); ++i)
+          if (mols[umol[i]]->tf == kTiles[t]) {
+            sub.push_back(units[i]);
+            subm.push_back(umol[i]);
+          }
+        pend[k].push_back(stage(sub, subm, kind == 0 ? K_DENS : K_POT, kTiles[t]));
+      }
+  }
+  // per-molecule operand preparation and packing
+  std::vector<PackD> packs(nm);
+  std::vector<long> pbase(nm + 1, 0), qbase(nm + 1, 0);
+  for (int m = 0; m < nm; ++m) {
+    pgxc_ctx *c = mols[m];
+    PackD &Q = packs[m];
+    Q.Hpart = c->Hpart;
+    Q.Vout = c->Vout;
+    Q.Vstate = c->Vstate;
+    Q.P = c->P;
+    Q.Pref = c->Pref;
+    Q.D = c->D;
+    Q.n = c->nbas;
+    Q.NP = c->NP;
+    Q.nsplit = c->nsplit;
+    Q.incremental = incr[m];
+    const long npk = (long)c->nbas * (c->nbas + 1) / 2;
+    pbase[m + 1] = pbase[m] + (npk + kThreads - 1) / kThreads;
+    qbase[m + 1] = qbase[m] + ((long)c->NP * c->NP + kThreads - 1) / kThreads;
+  }
+  const size_t pack_off = put(packs.data(), sizeof(PackD) * nm);
+  const size_t pb_off = put(pbase.data(), sizeof(long) * pbase.size());
+  const size_t qb_off = put(qbase.data(), sizeof(long) * qbase.size());
+  HIPCHK(errc, hipMalloc((void **)&plan->blob, std::max<size_t>(img.size(), 256)));
+  HIPCHK(errc, hipMemcpy(plan->blob, img.data(), img.size(), hipMemcpyHostToDevice));
+  char *B = plan->blob;
+  auto mk = [&](const Pending &P) {
+    StageList L;
+    L.units = (const UnitD *)(B + P.units_off);
+    L.nu = P.nu;
+    L.base = (const long *)(B + P.base_off);
+    L.order = P.has_order ? (const int *)(B + P.order_off) : nullptr;
+    L.total = P.total;
+    return L;
+  };
+  plan->steps.resize(nsteps);
+  for (int k = 0; k < nsteps; ++k) {
+    StepPlan &S = plan->steps[k];
+    S.basis = mk(pend[k][0]);
+    S.func = mk(pend[k][1]);
+    S.wmat = mk(pend[k][2]);
+    for (int t = 0; t < 3; ++t) S.dens[t] = mk(pend[k][3 + t]);
+    for (int t = 0; t < 3; ++t) S.pot[t] = mk(pend[k][6 + t]);
+    S.nunits = pend[k][0].nu;
+  }
+  plan->d_pack = (PackD *)(B + pack_off);
+  plan->d_pack_base = (long *)(B + pb_off);
+  plan->d_prep_base = (long *)(B + qb_off);
+  plan->pack_total = pbase[nm];
+  plan->prep_total = qbase[nm];
+  return PGXC_OK;
+}
+
+// Launch a plan on stream s.  With ev (8 events) each super-step is timed by
+// stage (basis, density contraction, functional + W, potential contraction)
+// into stage_ms, and ev[5]..ev[6] bracket the packing.
+template <typename T>
+static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hipEvent_t> *ev,
+                        float *stage_ms) {
+  const bool x3 = pl.precision == PGXC_PREC_BF16X3;
+  launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total,
+              !std::is_same<T, double>::value, false);
+  for (const StepPlan &S : pl.steps) {
+    if (ev) (void)hipEventRecord((*ev)[0], s);
+    launch_basis<T>(s, gga, S.basis);
+    if (ev) (void)hipEventRecord((*ev)[1], s);
+    for (int t = 0; t < 3; ++t) {
+      if constexpr (std::is_same<T, float>::value) {
+        if (x3) {
+          launch_density_bf16x3(s, gga, kTiles[t], S.dens[t]);
+          continue;
+        }
+      }
+      launch_density<T>(s, gga, kTiles[t], S.dens[t]);
+    }
+    if (ev) (void)hipEventRecord((*ev)[2], s);
+    launch_functional(s, gga, S.func);
+    launch_wmat<T>(s, gga, S.wmat);
+    if (ev) (void)hipEventRecord((*ev)[3], s);
+    for (int t = 0; t < 3; ++t) {
+      if constexpr (std::is_same<T, float>::value) {
+        if (x3) {
+          launch_potential_bf16x3(s, kTiles[t], S.pot[t]);
+          continue;
+        }
+      }
+      launch_potential<T>(s, kTiles[t], S.pot[t]);
+    }
+    if (ev) {
+      (void)hipEventRecord((*ev)[4], s);
+      (void)hipEventSynchronize((*ev)[4]);
+      for (int k = 0; k < 4; ++k) {
+        float t = 0;
+        (void)hipEventElapsedTime(&t, (*ev)[k], (*ev)[k + 1]);
+        stage_ms[k] += t;
+      }
+    }
+  }
+  if (ev) (void)hipEventRecord((*ev)[5], s);
+  launch_pack(s, pl.d_pack, pl.nmol, pl.d_pack_base, pl.pack_total);
+  launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total, false, true);
+  if (ev) (void)hipEventRecord((*ev)[6], s);
+}
+
+static int ensure_work(pgxc_ctx *c, size_t elem, hipStream_t s) {
+  const size_t need_phi = 4 * (size_t)c->NP * c->Mc * elem;
+  const size_t need_w = (size_t)c->NP * c->Mc * elem;
+  bool moved = false;
+  if (c->phi_bytes < need_phi) {
+    dfree(c->phi);
+    c->phi = nullptr;
+    c->phi_bytes = 0;
+    HIPCHK(c, hipMalloc(&c->phi, need_phi));
+    c->phi_bytes = need_phi;
+    c->phi_elem = 0;
+    moved = true;
+  }
+  // function rows nbas..NP-1 are never written and must read as zero in the
+  // element type of this build (a buffer last used at another precision
+  // holds that precision's bytes there)
+  if (c->phi_elem != elem) {
+    HIPCHK(c, hipMemsetAsync(c->phi, 0, c->phi_bytes, s));
+    c->phi_elem = elem;
+  }
+  if (c->w_bytes < need_w) {
+    dfree(c->W);
+    c->W = nullptr;
+    c->w_bytes = 0;
+    HIPCHK(c, hipMalloc(&c->W, need_w));
+    c->w_bytes = need_w;
+    moved = true;
+  }
+  if (moved) free_plan(c->plan);  // its unit descriptors hold the old pointers
+  return PGXC_OK;
+}
+
+static int validate_build(pgxc_ctx *c, uint32_t functional, int32_t precision, int &incr) {
+  if (!c) return PGXC_ERR_ARG;
+  if (functional == 0 || (functional & ~kSupported))
+    return fail(c, PGXC_ERR_UNSUPPORTED, "functional mask has terms other than XAlpha, PWLDAc, PBEx, PBEc");
+  if ((functional & PGXC_XC_PBEC) && !(functional & PGXC_XC_PWLDAC))
+    return fail(c, PGXC_ERR_ARG, "PBEc needs the PW92 LDA correlation it is built on (PWLDAc)");
+  if (precision != PGXC_PREC_FP64 && precision != PGXC_PREC_FP32 && precision != PGXC_PREC_BF16X3)
+    return fail(c, PGXC_ERR_UNSUPPORTED, "precision mode " + std::to_string(precision) + " not available");
+  if (!c->have_density) return fail(c, PGXC_ERR_STATE, "pgxc_build before pgxc_set_density/orbitals");
+  incr = 0;
+  if (c->density_mode == PGXC_DENSITY_INCREMENTAL) {
+    if (!c->have_state)
+      return fail(c, PGXC_ERR_STATE, "incremental density but no previous build on this context");
+    if (c->state_functional != functional)
+      return fail(c, PGXC_ERR_STATE, "incremental build with a functional different from the previous build");
+    incr = 1;
+  }
+  return PGXC_OK;
+}
+
+static void set_stats(pgxc_ctx *c) {
+  const double B = (double)c->npts, n = (double)c->nbas;
+  c->stats[0] = B;
+  c->stats[1] = std::ceil(B / c->veclen);
+  c->stats[2] = 4.0 * B * n * n + 16.0 * B * n;  // dense-equivalent (no screening)
+  c->stats[3] = B * c->exps_per_point;
+  c->stats[4] = 32.0 * B + 16.0 * n * n;
+}
+
+static int collect(pgxc_ctx *c, hipStream_t s, uint32_t functional) {
+  std::vector<double> ep(2 * (size_t)c->nblk_total);
+  HIPCHK(c, hipMemcpyAsync(ep.data(), c->epart, ep.size() * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  double e = 0.0, q = 0.0;
+  for (long b = 0; b < c->nblk_total; ++b) {
+    e += ep[2 * b];
+    q += ep[2 * b + 1];
+  }
+  c->last_exc = e;
+  c->last_charge = q;
+  c->have_state = 1;
+  c->state_functional = functional;
+  set_stats(c);
+  return PGXC_OK;
+}
 
 extern "C" {
 
@@ -98,9 +465,10 @@ const char *pgxc_last_error(const pgxc_ctx *ctx) { return ctx ? ctx->err.c_str()
 
 int64_t pgxc_nbas(const pgxc_ctx *ctx) { return ctx ? ctx->nbas : -1; }
 
-int pgxc_create(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int32_t n_shells,
-                const int32_t *shell_info, const double *exps, const double *norms,
-                const double *contr, const double *grid, int64_t npts, int32_t veclen) {
+int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int32_t n_shells,
+                    const int32_t *shell_info, const double *exps, const double *norms,
+                    const double *contr, const double *grid, int64_t npts, int32_t veclen,
+                    const pgxc_options *opt) {
   if (!out) return PGXC_ERR_ARG;
   *out = nullptr;
   pgxc_ctx *c = new pgxc_ctx();
@@ -108,6 +476,10 @@ int pgxc_create(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int32_t
   if (n_atoms <= 0 || n_shells <= 0 || !atom_xyz || !shell_info || !exps || !norms ||
       (!contr && n_shells > 0) || !grid || npts <= 0 || veclen <= 0)
     return fail(c, PGXC_ERR_ARG, "pgxc_create: null pointer or non-positive size");
+  if (opt && opt->tile != 0 && opt->tile != 64 && opt->tile != 96 && opt->tile != 128)
+    return fail(c, PGXC_ERR_ARG, "pgxc_create: tile must be 0, 64, 96 or 128");
+  if (opt && (opt->chunk_points < 0 || opt->chunk_points % 256 != 0))
+    return fail(c, PGXC_ERR_ARG, "pgxc_create: chunk_points must be a non-negative multiple of 256");
   c->n_atoms = n_atoms;
   c->n_shells = n_shells;
   c->npts = npts;
@@ -166,25 +538,32 @@ int pgxc_create(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int32_t
     if (!(exps[e] > 0.0) || !std::isfinite(norms[e]))
       return fail(c, PGXC_ERR_ARG, "pgxc_create: exponents must be positive and norms finite");
   c->nbas = (int)foff;
-  // function tile: the one of 128 / 96 / 64 that pads nbas least (larger on ties)
-  c->tf = 128;
-  c->NP = (int)round_up(c->nbas, 128);
-  for (int t : {96, 64})
-    if (round_up(c->nbas, t) < c->NP) {
-      c->NP = (int)round_up(c->nbas, t);
-      c->tf = t;
-    }
-  c->ntiles = density_tiles(c->NP, c->tf);
+  c->exps_per_point = (double)eoff;
+  // function tile: requested, or the one of 128 / 96 / 64 that pads nbas least
+  if (opt && opt->tile) {
+    c->tf = opt->tile;
+    c->NP = (int)round_up(c->nbas, c->tf);
+  } else {
+    c->tf = 128;
+    c->NP = (int)round_up(c->nbas, 128);
+    for (int t : {96, 64})
+      if (round_up(c->nbas, t) < c->NP) {
+        c->NP = (int)round_up(c->nbas, t);
+        c->tf = t;
+      }
+  }
+  c->ntiles = c->NP / c->tf;
 
   // ---- chunking of the grid ----------------------------------------------
-  // phi chunk = 4 x NP x Mc doubles, kept under ~2 GiB
-  const long cap = std::max<long>(256, ((1L << 31) / (4L * c->NP * 8)) / 256 * 256);
-  const long mc_max = std::min<long>(cap, 65536);
+  // phi chunk = 4 x NP x Mc doubles, kept under ~2 GiB unless requested
+  long mc_max = std::max<long>(256, ((1L << 31) / (4L * c->NP * 8)) / 256 * 256);
+  mc_max = std::min<long>(mc_max, 65536);
+  if (opt && opt->chunk_points) mc_max = opt->chunk_points;
   c->nchunks = (int)((npts + mc_max - 1) / mc_max);
   c->Mc = round_up((npts + c->nchunks - 1) / c->nchunks, 256);
   c->npad = c->Mc * c->nchunks;
-  // split of the point range in the potential contraction: enough
-  // workgroups for 256 CUs, each split a whole number of 16-point steps
+  // split of the point range in the potential contraction: enough items
+  // for 256 CUs, each split a whole number of 32-point steps
   const int ntile_pot = (c->NP / c->tf) * (c->NP / c->tf);
   c->nsplit = 1;
   while (c->nsplit < 16 && ntile_pot * c->nsplit < 512 && (c->Mc / 32) % (c->nsplit * 2) == 0)
@@ -214,7 +593,7 @@ int pgxc_create(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int32_t
   const size_t np2 = (size_t)c->NP * c->NP;
   if ((rc = dalloc(c, &c->P, np2))) return rc;
   if ((rc = dalloc(c, &c->Pref, np2))) return rc;
-  if ((rc = dalloc(c, &c->Dop, np2))) return rc;
+  HIPCHK(c, hipMalloc(&c->D, np2 * 8));
   HIPCHK(c, hipMemset(c->P, 0, np2 * 8));
   HIPCHK(c, hipMemset(c->Pref, 0, np2 * 8));
   if ((rc = dalloc(c, &c->state, (size_t)S_NSLOT * c->npad))) return rc;
@@ -228,8 +607,26 @@ int pgxc_create(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int32_t
   if ((rc = dalloc(c, &c->Hpart, (size_t)c->nsplit * np2))) return rc;
   HIPCHK(c, hipMemset(c->part, 0, (size_t)c->ntiles * 4 * c->Mc * 8));
   HIPCHK(c, hipMemset(c->coef, 0, 4 * (size_t)c->Mc * 8));
-  c->ev.resize(12);
+  c->ev.resize(8);
   for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  return PGXC_OK;
+}
+
+int pgxc_create(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int32_t n_shells,
+                const int32_t *shell_info, const double *exps, const double *norms,
+                const double *contr, const double *grid, int64_t npts, int32_t veclen) {
+  return pgxc_create_opt(out, n_atoms, atom_xyz, n_shells, shell_info, exps, norms, contr, grid,
+                         npts, veclen, nullptr);
+}
+
+int pgxc_layout(const pgxc_ctx *c, int64_t *out6) {
+  if (!c || !out6) return PGXC_ERR_ARG;
+  out6[0] = c->NP;
+  out6[1] = c->tf;
+  out6[2] = c->Mc;
+  out6[3] = c->nchunks;
+  out6[4] = c->nsplit;
+  out6[5] = c->npad;
   return PGXC_OK;
 }
 
@@ -248,7 +645,8 @@ static int check_density_args(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32_t 
   return PGXC_OK;
 }
 
-extern "C" int pgxc_set_density(pgxc_ctx *c, int32_t ispin, int32_t nbas, const double *P, int32_t mode) {
+extern "C" int pgxc_set_density(pgxc_ctx *c, int32_t ispin, int32_t nbas, const double *P,
+                                int32_t mode) {
   int rc = check_density_args(c, ispin, nbas, mode);
   if (rc) return rc;
   if (!P) return fail(c, PGXC_ERR_ARG, "pgxc_set_density: P is null");
@@ -263,8 +661,8 @@ extern "C" int pgxc_set_density(pgxc_ctx *c, int32_t ispin, int32_t nbas, const 
   return PGXC_OK;
 }
 
-extern "C" int pgxc_set_orbitals(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32_t nocc, const double *C,
-                      const double *occ, int32_t mode) {
+extern "C" int pgxc_set_orbitals(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32_t nocc,
+                                 const double *C, const double *occ, int32_t mode) {
   int rc = check_density_args(c, ispin, nbas, mode);
   if (rc) return rc;
   if (nocc < 0 || (nocc > 0 && (!C || !occ)))
@@ -288,152 +686,35 @@ extern "C" int pgxc_set_orbitals(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32
   return PGXC_OK;
 }
 
-static int ensure_work(pgxc_ctx *c, size_t elem) {
-  const size_t need_phi = 4 * (size_t)c->NP * c->Mc * elem;
-  const size_t need_w = (size_t)c->NP * c->Mc * elem;
-  if (c->phi_bytes < need_phi) {
-    dfree(c->phi);
-    c->phi = nullptr;
-    c->phi_bytes = 0;
-    HIPCHK(c, hipMalloc(&c->phi, need_phi));
-    c->phi_bytes = need_phi;
-    c->phi_elem = 0;
-  }
-  // function rows nbas..NP-1 are never written and must read as zero in the
-  // element type of this build (a buffer last used at another precision
-  // holds that precision's bytes there)
-  if (c->phi_elem != elem) {
-    HIPCHK(c, hipMemsetAsync(c->phi, 0, c->phi_bytes, c->stream));
-    c->phi_elem = elem;
-  }
-  if (c->w_bytes < need_w) {
-    dfree(c->W);
-    c->W = nullptr;
-    c->w_bytes = 0;
-    HIPCHK(c, hipMalloc(&c->W, need_w));
-    c->w_bytes = need_w;
-  }
-  return PGXC_OK;
-}
-
-template <typename T>
-static int run_chunks(pgxc_ctx *c, bool gga, bool incremental, uint32_t functional,
-                      const T *D, bool bf16x3) {
-  hipStream_t s = c->stream;
-  T *phi = (T *)c->phi, *W = (T *)c->W;
-  float ms[4] = {0, 0, 0, 0};
-  for (int ch = 0; ch < c->nchunks; ++ch) {
-    const long p0 = (long)ch * c->Mc;
-    HIPCHK(c, hipEventRecord(c->ev[0], s));
-    launch_basis<T>(s, gga, c->grid, c->grid + c->npad, c->grid + 2 * c->npad, p0, c->Mc,
-                    c->n_atoms, c->atoms, c->shells, c->exps, c->norms, c->contr, phi, c->NP);
-    HIPCHK(c, hipEventRecord(c->ev[1], s));
-    if constexpr (std::is_same<T, float>::value) {
-      if (bf16x3)
-        launch_density_bf16x3(s, gga, phi, D, c->Mc, c->NP, c->tf, c->part);
-      else
-        launch_density<T>(s, gga, phi, D, c->Mc, c->NP, c->tf, c->part);
-    } else {
-      launch_density<T>(s, gga, phi, D, c->Mc, c->NP, c->tf, c->part);
-    }
-    HIPCHK(c, hipEventRecord(c->ev[2], s));
-    launch_functional(s, gga, incremental, functional, c->part, c->ntiles, c->Mc,
-                      c->grid + 3 * c->npad, p0, c->state, c->npad, c->coef, c->epart,
-                      p0 / kThreads);
-    launch_wmat<T>(s, gga, phi, c->coef, c->Mc, c->NP, W);
-    HIPCHK(c, hipEventRecord(c->ev[3], s));
-    if constexpr (std::is_same<T, float>::value) {
-      if (bf16x3)
-        launch_potential_bf16x3(s, W, phi, c->Mc, c->NP, c->tf, c->nsplit, c->Hpart, ch == 0);
-      else
-        launch_potential<T>(s, W, phi, c->Mc, c->NP, c->tf, c->nsplit, c->Hpart, ch == 0);
-    } else {
-      launch_potential<T>(s, W, phi, c->Mc, c->NP, c->tf, c->nsplit, c->Hpart, ch == 0);
-    }
-    HIPCHK(c, hipEventRecord(c->ev[4], s));
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventSynchronize(c->ev[4]));
-    for (int k = 0; k < 4; ++k) {
-      float t = 0;
-      HIPCHK(c, hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]));
-      ms[k] += t;
-    }
-  }
-  for (int k = 0; k < 4; ++k) c->timing[k] = ms[k];
-  return PGXC_OK;
-}
-
 extern "C" {
 
 int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham, double *exc,
                double *charge) {
-  if (!c) return PGXC_ERR_ARG;
-  if (functional == 0 || (functional & ~kSupported))
-    return fail(c, PGXC_ERR_UNSUPPORTED, "functional mask has terms other than XAlpha, PWLDAc, PBEx, PBEc");
-  if ((functional & PGXC_XC_PBEC) && !(functional & PGXC_XC_PWLDAC))
-    return fail(c, PGXC_ERR_ARG, "PBEc needs the PW92 LDA correlation it is built on (PWLDAc)");
-  if (precision != PGXC_PREC_FP64 && precision != PGXC_PREC_FP32 && precision != PGXC_PREC_BF16X3)
-    return fail(c, PGXC_ERR_UNSUPPORTED, "precision mode " + std::to_string(precision) + " not available");
-  if (!c->have_density) return fail(c, PGXC_ERR_STATE, "pgxc_build before pgxc_set_density/orbitals");
+  int incr = 0;
+  int rc = validate_build(c, functional, precision, incr);
+  if (rc) return rc;
   const bool gga = (functional & (PGXC_XC_PBEX | PGXC_XC_PBEC)) != 0;
-  bool incremental = false;
-  if (c->density_mode == PGXC_DENSITY_INCREMENTAL) {
-    if (!c->have_state)
-      return fail(c, PGXC_ERR_STATE, "incremental density but no previous build on this context");
-    if (c->state_functional != functional)
-      return fail(c, PGXC_ERR_STATE, "incremental build with a functional different from the previous build");
-    incremental = true;
-  }
   hipStream_t s = c->stream;
-  const long np2 = (long)c->NP * c->NP;
-  HIPCHK(c, hipEventRecord(c->ev[10], s));
-  if (incremental)
-    launch_sub(s, c->P, c->Pref, np2, c->Dop);
+  const size_t elem = precision == PGXC_PREC_FP64 ? 8 : 4;
+  if ((rc = ensure_work(c, elem, s))) return rc;
+  std::vector<pgxc_ctx *> mols{c};
+  std::vector<int> incrv{incr};
+  if ((rc = get_plan(mols, functional, precision, incrv, {}, -1, c->plan, c))) return rc;
+  float ms[4] = {0, 0, 0, 0};
+  HIPCHK(c, hipEventRecord(c->ev[7], s));
+  if (elem == 8)
+    launch_plan<double>(s, *c->plan, gga, &c->ev, ms);
   else
-    HIPCHK(c, hipMemcpyAsync(c->Dop, c->P, np2 * 8, hipMemcpyDeviceToDevice, s));
-  int rc;
-  if (precision == PGXC_PREC_FP64) {
-    if ((rc = ensure_work(c, 8))) return rc;
-    if ((rc = run_chunks<double>(c, gga, incremental, functional, c->Dop, false))) return rc;
-  } else {
-    if ((rc = dalloc(c, &c->Dlow, np2))) return rc;
-    launch_cast<float>(s, c->Dop, np2, c->Dlow);
-    if ((rc = ensure_work(c, 4))) return rc;
-    if ((rc = run_chunks<float>(c, gga, incremental, functional, c->Dlow,
-                                 precision == PGXC_PREC_BF16X3)))
-      return rc;
-  }
-  HIPCHK(c, hipEventRecord(c->ev[5], s));
-  launch_pack(s, c->Hpart, c->nsplit, c->NP, c->nbas, incremental ? c->Vstate : nullptr, c->Vout);
-  const size_t npk = (size_t)c->nbas * (c->nbas + 1) / 2;
-  HIPCHK(c, hipMemcpyAsync(c->Vstate, c->Vout, npk * 8, hipMemcpyDeviceToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->Pref, c->P, np2 * 8, hipMemcpyDeviceToDevice, s));
-  HIPCHK(c, hipEventRecord(c->ev[6], s));
+    launch_plan<float>(s, *c->plan, gga, &c->ev, ms);
   HIPCHK(c, hipGetLastError());
-  std::vector<double> ep(2 * (size_t)c->nblk_total);
-  HIPCHK(c, hipMemcpyAsync(ep.data(), c->epart, ep.size() * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
+  for (int k = 0; k < 4; ++k) c->timing[k] = ms[k];
   float t = 0;
   HIPCHK(c, hipEventElapsedTime(&t, c->ev[5], c->ev[6]));
   c->timing[4] = t;
-  HIPCHK(c, hipEventElapsedTime(&t, c->ev[10], c->ev[6]));
+  HIPCHK(c, hipEventElapsedTime(&t, c->ev[7], c->ev[6]));
   c->timing[5] = t;
-  double e = 0.0, q = 0.0;
-  for (long b = 0; b < c->nblk_total; ++b) {
-    e += ep[2 * b];
-    q += ep[2 * b + 1];
-  }
-  c->last_exc = e;
-  c->last_charge = q;
-  c->have_state = 1;
-  c->state_functional = functional;
-  // dense-equivalent work model (no screening on this path)
-  const double B = (double)c->npts, n = (double)c->nbas;
-  c->stats[0] = B;
-  c->stats[1] = std::ceil(B / c->veclen);
-  c->stats[2] = 4.0 * B * n * n + 16.0 * B * n;
-  c->stats[3] = 0;
-  c->stats[4] = 32.0 * B + 16.0 * n * n;
+  if ((rc = collect(c, s, functional))) return rc;
   return pgxc_get_result(c, ham, exc, charge);
 }
 
@@ -466,8 +747,9 @@ int pgxc_last_timing(const pgxc_ctx *c, double *ms6) {
 int pgxc_destroy(pgxc_ctx *c) {
   if (!c) return PGXC_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_plan(c->plan);
   void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
-                  c->Dop, c->Dlow, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
+                  c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart};
   for (void *b : bufs) dfree(b);
   for (auto &e : c->ev)
@@ -479,10 +761,166 @@ int pgxc_destroy(pgxc_ctx *c) {
 
 }  // extern "C"
 
-extern "C" int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64_t npts,
-                                  const double *pts, const int32_t *owner,
-                                  const double *w_atomic, double *w_out) {
-  if (n_atoms <= 0 || npts < 0 || !atom_xyz || (npts > 0 && (!pts || !owner || !w_atomic || !w_out)))
+// ------------------------------------------------------------------ batches
+struct pgxc_batch {
+  std::vector<pgxc_ctx *> mols;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Plan *plan = nullptr;
+  std::vector<double> prio;
+  long order_version = -1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double last_ms = 0;
+};
+
+extern "C" {
+
+const char *pgxc_batch_last_error(const pgxc_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int pgxc_batch_create(pgxc_batch **out, int32_t n, pgxc_ctx *const *ctxs) {
+  if (!out) return PGXC_ERR_ARG;
+  *out = nullptr;
+  if (n <= 0 || !ctxs) return PGXC_ERR_ARG;
+  pgxc_batch *b = new pgxc_batch();
+  *out = b;
+  int dev = -1;
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) {
+      b->err = "pgxc_batch_create: null context " + std::to_string(i);
+      return PGXC_ERR_ARG;
+    }
+    if (dev >= 0 && ctxs[i]->device != dev) {
+      b->err = "pgxc_batch_create: contexts on different devices";
+      return PGXC_ERR_ARG;
+    }
+    dev = ctxs[i]->device;
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) {
+        b->err = "pgxc_batch_create: context listed twice";
+        return PGXC_ERR_ARG;
+      }
+    b->mols.push_back(ctxs[i]);
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
+    b->err = "pgxc_batch_create: stream/event creation failed";
+    return PGXC_ERR_HIP;
+  }
+  return PGXC_OK;
+}
+
+int pgxc_batch_set_priority(pgxc_batch *b, const double *cost) {
+  if (!b) return PGXC_ERR_ARG;
+  if (cost)
+    b->prio.assign(cost, cost + b->mols.size());
+  else
+    b->prio.clear();
+  ++b->order_version;
+  return PGXC_OK;
+}
+
+int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int32_t use_graph) {
+  if (!b) return PGXC_ERR_ARG;
+  const int nm = (int)b->mols.size();
+  std::vector<int> incr(nm, 0);
+  for (int m = 0; m < nm; ++m) {
+    int rc = validate_build(b->mols[m], functional, precision, incr[m]);
+    if (rc) {
+      b->err = "molecule " + std::to_string(m) + ": " + b->mols[m]->err;
+      return rc;
+    }
+  }
+  const bool gga = (functional & (PGXC_XC_PBEX | PGXC_XC_PBEC)) != 0;
+  const size_t elem = precision == PGXC_PREC_FP64 ? 8 : 4;
+  hipStream_t s = b->stream;
+  bool moved = false;
+  for (int m = 0; m < nm; ++m) {
+    pgxc_ctx *c = b->mols[m];
+    const void *phi0 = c->phi, *w0 = c->W;
+    int rc = ensure_work(c, elem, s);
+    if (rc) {
+      b->err = "molecule " + std::to_string(m) + ": " + c->err;
+      return rc;
+    }
+    moved = moved || phi0 != c->phi || w0 != c->W;
+  }
+  if (moved) free_plan(b->plan);
+  int rc = get_plan(b->mols, functional, precision, incr, b->prio, b->order_version, b->plan,
+                    b->mols[0]);
+  if (rc) {
+    b->err = b->mols[0]->err;
+    return rc;
+  }
+  Plan &pl = *b->plan;
+  if (use_graph && !pl.graph) {
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      if (elem == 8)
+        launch_plan<double>(s, pl, gga, nullptr, nullptr);
+      else
+        launch_plan<float>(s, pl, gga, nullptr, nullptr);
+      e = hipStreamEndCapture(s, &pl.graph_src);
+      if (e == hipSuccess) e = hipGraphInstantiate(&pl.graph, pl.graph_src, nullptr, nullptr, 0);
+    }
+    if (e != hipSuccess) {
+      b->err = std::string("hipGraph capture: ") + hipGetErrorString(e);
+      return PGXC_ERR_HIP;
+    }
+  }
+  if (hipEventRecord(b->e0, s) != hipSuccess) return PGXC_ERR_HIP;
+  if (use_graph) {
+    if (hipGraphLaunch(pl.graph, s) != hipSuccess) {
+      b->err = "hipGraphLaunch failed";
+      return PGXC_ERR_HIP;
+    }
+  } else if (elem == 8) {
+    launch_plan<double>(s, pl, gga, nullptr, nullptr);
+  } else {
+    launch_plan<float>(s, pl, gga, nullptr, nullptr);
+  }
+  (void)hipEventRecord(b->e1, s);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    b->err = std::string("batch build: ") + hipGetErrorString(e);
+    return PGXC_ERR_HIP;
+  }
+  float t = 0;
+  (void)hipEventElapsedTime(&t, b->e0, b->e1);
+  b->last_ms = t;
+  for (int m = 0; m < nm; ++m) {
+    pgxc_ctx *c = b->mols[m];
+    rc = collect(c, s, functional);
+    if (rc) {
+      b->err = c->err;
+      return rc;
+    }
+    c->timing[5] = t;
+  }
+  return PGXC_OK;
+}
+
+int pgxc_batch_last_ms(const pgxc_batch *b, double *ms) {
+  if (!b || !ms) return PGXC_ERR_ARG;
+  *ms = b->last_ms;
+  return PGXC_OK;
+}
+
+int pgxc_batch_destroy(pgxc_batch *b) {
+  if (!b) return PGXC_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  free_plan(b->plan);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return PGXC_OK;
+}
+
+int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64_t npts, const double *pts,
+                       const int32_t *owner, const double *w_atomic, double *w_out) {
+  if (n_atoms <= 0 || npts < 0 || !atom_xyz ||
+      (npts > 0 && (!pts || !owner || !w_atomic || !w_out)))
     return PGXC_ERR_ARG;
   if (npts == 0) return PGXC_OK;
   for (int64_t p = 0; p < npts; ++p)
@@ -491,8 +929,9 @@ extern "C" int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64
   for (int i = 0; i < n_atoms; ++i)
     for (int k = 0; k < n_atoms; ++k) {
       if (i == k) continue;
-      const double dx = atom_xyz[3 * i] - atom_xyz[3 * k], dy = atom_xyz[3 * i + 1] - atom_xyz[3 * k + 1],
-                   dz = atom_xyz[3 * i + 2] - atom_xyz[3 * k + 2];
+      const double dx = atom_xyz[3 * i] - atom_xyz[3 * k];
+      const double dy = atom_xyz[3 * i + 1] - atom_xyz[3 * k + 1];
+      const double dz = atom_xyz[3 * i + 2] - atom_xyz[3 * k + 2];
       const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
       if (!(d > 0.0)) return PGXC_ERR_ARG;  // coincident centres
       inv[(size_t)i * n_atoms + k] = 1.0 / d;
@@ -502,7 +941,8 @@ extern "C" int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64
   hipStream_t s = nullptr;
   int rc = PGXC_OK;
   auto chk = [&](hipError_t e) {
-    if (e != hipSuccess && rc == PGXC_OK) rc = e == hipErrorOutOfMemory ? PGXC_ERR_NOMEM : PGXC_ERR_HIP;
+    if (e != hipSuccess && rc == PGXC_OK)
+      rc = e == hipErrorOutOfMemory ? PGXC_ERR_NOMEM : PGXC_ERR_HIP;
   };
   chk(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   chk(hipMalloc((void **)&d_a, 24 * (size_t)n_atoms));
@@ -524,7 +964,14 @@ extern "C" int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64
     chk(hipMemcpyAsync(w_out, d_o, 8 * (size_t)npts, hipMemcpyDeviceToHost, s));
     chk(hipStreamSynchronize(s));
   }
-  dfree(d_a); dfree(d_inv); dfree(d_p); dfree(d_w); dfree(d_o); dfree(d_own);
+  dfree(d_a);
+  dfree(d_inv);
+  dfree(d_p);
+  dfree(d_w);
+  dfree(d_o);
+  dfree(d_own);
   if (s) (void)hipStreamDestroy(s);
   return rc;
 }
+
+}  // extern "C"

@@ -27,61 +27,76 @@ struct AtomD {
 // Per-point state kept across builds (incremental mode), SoA of length npad.
 enum StateSlot { S_RHO = 0, S_GX, S_GY, S_GZ, S_A, S_BX, S_BY, S_BZ, S_NSLOT };
 
-// ---- launchers (kernels live in xc_kernels.hip) -------------------------
-// Basis values (+ gradients) of every function on points [p0, p0+Mc) into
-// phi[c][f][p] (c = value, d/dx, d/dy, d/dz), f < nbas; rows nbas..NP-1 untouched.
+// ---- work lists ----------------------------------------------------------
+// A unit is one grid chunk of one molecule: Mc points (multiple of 256) with
+// the molecule's basis, its density operand and the chunk's work buffers.
+struct UnitD {
+  const double *gx, *gy, *gz, *gw;  // grid of the unit's points (offset to its first point)
+  const AtomD *atoms;
+  const ShellD *shells;
+  const double *exps, *norms, *contr;
+  void *phi;        // [4][NP][Mc] values, d/dx, d/dy, d/dz (T)
+  void *W;          // [NP][Mc] (T)
+  const void *D;    // [NP][NP] density operand (T)
+  double *part;     // [NP/tf][4][Mc] density partials
+  double *coef;     // [4][Mc] potential coefficients a, b
+  double *state;    // [S_NSLOT][npad] molecule state, offset to the unit's first point
+  double *epart;    // [Mc/256][2] block sums of w f and w rho (offset to the unit)
+  double *Hpart;    // [nsplit][NP][NP] potential partials of the molecule
+  long Mc, npad;
+  int n_atoms, NP, ntiles, nsplit, first, incremental;
+  uint32_t mask;
+  int pad;
+};
+
+// Per-molecule descriptors of the operand preparation and the final packing.
+struct PackD {
+  const double *Hpart;
+  double *Vout, *Vstate;
+  const double *P;
+  double *Pref;
+  void *D;
+  long n, NP;
+  int nsplit, incremental;
+};
+
+// One launch's items: prefix offsets base[0..nu] over units, optional order.
+struct StageList {
+  const UnitD *units;
+  int nu;
+  const long *base;
+  const int *order;
+  long total;
+};
+
+// ---- launchers (kernels live in xc_kernels.hip / xc_bf16x3.hip) -----------
 template <typename T>
-void launch_basis(hipStream_t s, bool grad, const double *gx, const double *gy,
-                  const double *gz, long p0, long Mc, int n_atoms, const AtomD *atoms,
-                  const ShellD *shells, const double *exps, const double *norms,
-                  const double *contr, T *phi, long NP);
-
-// X = Phi D (D = P or dP, NP x NP); per-point partial sums over each
-// 128-function column tile: part[t][c][p], c = rho, 2 X.dPhi/dx, ...
-// tf = function tile (64, 96 or 128), NP a multiple of it
+void launch_basis(hipStream_t s, bool grad, const StageList &L);
+// tf = function tile (64, 96 or 128) shared by every unit of the list
 template <typename T>
-void launch_density(hipStream_t s, bool grad, const T *phi, const T *D, long Mc, int NP, int tf,
-                    double *part);
-int density_tiles(int NP, int tf);
-
-// Functional on points of one chunk: sums the partials (in tile order), adds
-// the stored state when incremental, evaluates f, f_rho, f_gamma, forms the
-// potential coefficients, updates the state, writes block partials of
-// (sum w f, sum w rho) to epart[blk_off + b].
-void launch_functional(hipStream_t s, bool gga, bool incremental, uint32_t mask,
-                       const double *part, int ntiles, long Mc, const double *w,
-                       long p0, double *state, long npad, double *coef,
-                       double *epart, long blk_off);
-long functional_blocks(long Mc);
-
-// W[f][p] = a phi0 + b . grad phi (T precision)
+void launch_density(hipStream_t s, bool grad, int tf, const StageList &L);
+void launch_functional(hipStream_t s, bool gga, const StageList &L);
 template <typename T>
-void launch_wmat(hipStream_t s, bool grad, const T *phi, const double *coef, long Mc,
-                 int NP, T *W);
-
-// Hpart[sl][i][j] (+)= sum_{p in split sl} W[i][p] phi0[j][p]
+void launch_wmat(hipStream_t s, bool grad, const StageList &L);
 template <typename T>
-void launch_potential(hipStream_t s, const T *W, const T *phi, long Mc, int NP, int tf,
-                      int nsplit, double *Hpart, bool first);
+void launch_potential(hipStream_t s, int tf, const StageList &L);
+void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L);
+void launch_potential_bf16x3(hipStream_t s, int tf, const StageList &L);
+void launch_pack(hipStream_t s, const PackD *mols, int nm, const long *base, long total);
+void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, long total,
+                 bool to_float, bool finish);
 
-// split-BF16 GEMMs read FP32 operands and split while staging
-void launch_density_bf16x3(hipStream_t s, bool grad, const float *phi, const float *D,
-                           long Mc, int NP, int tf, double *part);
-void launch_potential_bf16x3(hipStream_t s, const float *W, const float *phi, long Mc,
-                             int NP, int tf, int nsplit, double *Hpart, bool first);
-
-// V (packed, n(n+1)/2) = base + sum_sl (H[sl] + H[sl]^T) restricted to n x n
-void launch_pack(hipStream_t s, const double *Hpart, int nsplit, int NP, int n,
-                 const double *base, double *Vpacked);
+// item counts per unit, shared by the planner and the kernels' decoding
+inline long items_basis(long Mc, int n_atoms) { return (Mc / kThreads) * n_atoms; }
+inline long items_density(long Mc, int NP, int tf) { return (Mc / 128) * (NP / tf); }
+inline long items_functional(long Mc) { return Mc / kThreads; }
+inline long items_wmat(long Mc, int NP) { return (Mc / kThreads) * (NP / 16); }
+inline long items_potential(int NP, int tf, int nsplit) { return (long)(NP / tf) * (NP / tf) * nsplit; }
 
 // P = C diag(occ) C^T into an NP x NP padded array
 void launch_form_density(hipStream_t s, const double *C, const double *occ, int n,
                          int nocc, int NP, double *P);
 
-// out = a - b (NP*NP), and a cast of a double array to T
-void launch_sub(hipStream_t s, const double *a, const double *b, long n, double *out);
-template <typename T>
-void launch_cast(hipStream_t s, const double *in, long n, T *out);
 
 // Becke partition weights of points owned by atoms (grid generation)
 void launch_becke(hipStream_t s, int na, const double *axyz, const double *inv_dist, long npts,

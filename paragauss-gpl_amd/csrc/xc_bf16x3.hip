@@ -142,75 +142,99 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, lo
   }
 }
 
+__device__ __forceinline__ int find_unit_x3(const long *__restrict__ base, int nu, long it) {
+  int lo = 0, hi = nu - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= it) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ long item_at_x3(const int *__restrict__ order, long L, long total) {
+  return order ? (long)order[L] : (long)xcd_remap((int)L, (int)total);
+}
+
+// item = (128-point tile, function tile), as k_density
 template <int BN, bool GRAD>
-__global__ __launch_bounds__(kThreads) void k_density_bf16x3(const float *__restrict__ phi,
-                                                             const float *__restrict__ D, long Mc,
-                                                             int NP, double *__restrict__ part,
-                                                             int ntn) {
+__global__ __launch_bounds__(kThreads) void k_density_bf16x3(const UnitD *__restrict__ units, int nu,
+                                                             const long *__restrict__ base,
+                                                             const int *__restrict__ order,
+                                                             long total) {
   constexpr int BM = kXBM;
   __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BM, BN>::n];
   __shared__ double red[2][4][BM];
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int tm = tile / ntn, tn = tile % ntn;
-  const long m0 = (long)tm * BM;
-  const int n0 = tn * BN;
-  f4 acc[BM / 32][BN / 32];
-  gemm_tile_bf16x3<BM, BN, false, false>(phi + m0, Mc, D + n0, NP, NP / kXBK, lds, acc);
-  density_epilogue<float, f4, BM, BN, GRAD>(acc, phi, Mc, NP, m0, n0, tn, part, red);
-}
-
-template <int BN>
-static void density_x3(hipStream_t s, bool grad, const float *phi, const float *D, long Mc, int NP,
-                       double *part) {
-  const int ntn = NP / BN, ntm = (int)(Mc / kXBM);
-  dim3 grid((unsigned)(ntm * ntn));
-  if (grad)
-    hipLaunchKernelGGL((k_density_bf16x3<BN, true>), grid, dim3(kThreads), 0, s, phi, D, Mc, NP, part, ntn);
-  else
-    hipLaunchKernelGGL((k_density_bf16x3<BN, false>), grid, dim3(kThreads), 0, s, phi, D, Mc, NP, part, ntn);
-}
-
-void launch_density_bf16x3(hipStream_t s, bool grad, const float *phi, const float *D, long Mc,
-                           int NP, int tf, double *part) {
-  switch (tf) {
-    case 64: density_x3<64>(s, grad, phi, D, Mc, NP, part); break;
-    case 96: density_x3<96>(s, grad, phi, D, Mc, NP, part); break;
-    default: density_x3<128>(s, grad, phi, D, Mc, NP, part); break;
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const long it = item_at_x3(order, L, total);
+    const int u = find_unit_x3(base, nu, it);
+    const UnitD &U = units[u];
+    const long local = it - base[u];
+    const int ntn = U.NP / BN;
+    const long m0 = (local / ntn) * BM;
+    const int tn = (int)(local % ntn), n0 = tn * BN;
+    const float *phi = (const float *)U.phi, *D = (const float *)U.D;
+    f4 acc[BM / 32][BN / 32];
+    gemm_tile_bf16x3<BM, BN, false, false>(phi + m0, U.Mc, D + n0, U.NP, U.NP / kXBK, lds, acc);
+    density_epilogue<float, f4, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn, U.part, red);
   }
 }
 
-template <int BT>
-__global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const float *__restrict__ W,
-                                                               const float *__restrict__ phi,
-                                                               long Mc, int NP, int ksteps,
-                                                               double *Hpart, int ntn, int first) {
-  constexpr int BM = BT, BN = BT;
-  __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BM, BN>::n];
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int i0 = (tile / ntn) * BM, j0 = (tile % ntn) * BN;
-  const long kb = (long)blockIdx.y * ksteps * kXBK;
-  f4 acc[BM / 32][BN / 32];
-  gemm_tile_bf16x3<BM, BN, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb, Mc,
-                                       ksteps, lds, acc);
-  potential_epilogue<float, f4, BM, BN>(acc, Hpart + (long)blockIdx.y * NP * NP, NP, i0, j0, first);
+static unsigned grid_x3(long total) {
+  const long cap = 256L * 2;
+  return (unsigned)(total < cap ? (total > 0 ? total : 1) : cap);
 }
 
-template <int BT>
-static void potential_x3(hipStream_t s, const float *W, const float *phi, long Mc, int NP,
-                         int nsplit, double *Hpart, bool first) {
-  const int nt = NP / BT;
-  const int ksteps = (int)(Mc / kXBK) / nsplit;
-  dim3 grid((unsigned)(nt * nt), (unsigned)nsplit);
-  hipLaunchKernelGGL((k_potential_bf16x3<BT>), grid, dim3(kThreads), 0, s, W, phi, Mc, NP, ksteps,
-                     Hpart, nt, (int)first);
+template <int BN>
+static void density_x3(hipStream_t s, bool grad, const StageList &L) {
+  dim3 g(grid_x3(L.total));
+  if (grad)
+    hipLaunchKernelGGL((k_density_bf16x3<BN, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  else
+    hipLaunchKernelGGL((k_density_bf16x3<BN, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
 }
 
-void launch_potential_bf16x3(hipStream_t s, const float *W, const float *phi, long Mc, int NP,
-                             int tf, int nsplit, double *Hpart, bool first) {
+void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L) {
+  if (!L.total) return;
   switch (tf) {
-    case 64: potential_x3<64>(s, W, phi, Mc, NP, nsplit, Hpart, first); break;
-    case 96: potential_x3<96>(s, W, phi, Mc, NP, nsplit, Hpart, first); break;
-    default: potential_x3<128>(s, W, phi, Mc, NP, nsplit, Hpart, first); break;
+    case 64: density_x3<64>(s, grad, L); break;
+    case 96: density_x3<96>(s, grad, L); break;
+    default: density_x3<128>(s, grad, L); break;
+  }
+}
+
+// item = (point split, function tile pair), as k_potential
+template <int BT>
+__global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const UnitD *__restrict__ units, int nu,
+                                                               const long *__restrict__ base,
+                                                               const int *__restrict__ order,
+                                                               long total) {
+  __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BT, BT>::n];
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const long it = item_at_x3(order, L, total);
+    const int u = find_unit_x3(base, nu, it);
+    const UnitD &U = units[u];
+    const long local = it - base[u];
+    const int nt = U.NP / BT, nt2 = nt * nt;
+    const int sl = (int)(local / nt2), tile = (int)(local % nt2);
+    const int i0 = (tile / nt) * BT, j0 = (tile % nt) * BT;
+    const int ksteps = (int)(U.Mc / kXBK) / U.nsplit;
+    const long kb = (long)sl * ksteps * kXBK, Mc = U.Mc;
+    const float *W = (const float *)U.W, *phi = (const float *)U.phi;
+    f4 acc[BT / 32][BT / 32];
+    gemm_tile_bf16x3<BT, BT, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb, Mc,
+                                         ksteps, lds, acc);
+    potential_epilogue<float, f4, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP, U.NP, i0, j0,
+                                          U.first);
+  }
+}
+
+void launch_potential_bf16x3(hipStream_t s, int tf, const StageList &L) {
+  if (!L.total) return;
+  dim3 g(grid_x3(L.total));
+  switch (tf) {
+    case 64: hipLaunchKernelGGL((k_potential_bf16x3<64>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
+    case 96: hipLaunchKernelGGL((k_potential_bf16x3<96>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
+    default: hipLaunchKernelGGL((k_potential_bf16x3<128>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
   }
 }
 

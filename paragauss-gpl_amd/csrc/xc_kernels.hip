@@ -1,11 +1,12 @@
-// HIP kernels of the XC build for gfx950 (CDNA4, wave64).
+// HIP kernels of the XC build for gfx950 (CDNA4, wave64).  All stage kernels
+// run over work lists of units (a unit = one grid chunk of one molecule), so one
+// launch per stage serves a single molecule or a batch of many.
 //
-//   k_basis       basis values and gradients on a chunk of grid points
-//                 (ParaGauss modules/orbital_module.f90:1742-1932, 2421-2619;
-//                  solid harmonics modules/solid_harmonics_module.f90:369-568)
-//   k_density     X = Phi D on MFMA, fused reduction rho = sum Phi X,
-//                 grad rho = 2 sum dPhi X  (modules/density_calc_module.f90:1478-1577)
-//   k_functional  Slater + PW92 + PBE-x + PBE-c, FP64, closed-form derivatives
+//   k_basis       basis values and gradients  (ParaGauss modules/orbital_module.f90:1742-1932,
+//                 2421-2619; solid harmonics modules/solid_harmonics_module.f90:369-568)
+//   k_density     X = Phi D on MFMA, fused rho = sum Phi X, grad rho = 2 sum dPhi X
+//                 (modules/density_calc_module.f90:1478-1577)
+//   k_functional  Slater + PW92 + PBE-x + PBE-c in FP64, closed-form derivatives
 //                 (modules/xc_func.f90:187-305 and the term modules)
 //   k_wmat        W = a Phi + b . grad Phi  (modules/xc_hamiltonian.f90:1429-1483)
 //   k_potential   H = W^T Phi on MFMA, split over points  (:1528-1531)
@@ -246,102 +247,6 @@ __device__ __forceinline__ void basis_atom(const AtomD &at, const ShellD *__rest
   }
 }
 
-// grid: (Mc / 256, n_atoms); one thread per (point, atom)
-template <bool GRAD, typename T>
-__global__ __launch_bounds__(kThreads) void k_basis(const double *__restrict__ gx,
-                                                    const double *__restrict__ gy,
-                                                    const double *__restrict__ gz, long p0,
-                                                    const AtomD *__restrict__ atoms,
-                                                    const ShellD *__restrict__ shells,
-                                                    const double *__restrict__ exps,
-                                                    const double *__restrict__ norms,
-                                                    const double *__restrict__ contr, T *phi,
-                                                    long Mc, long NP) {
-  const int p = blockIdx.x * kThreads + threadIdx.x;
-  const AtomD at = atoms[blockIdx.y];
-  const double x = gx[p0 + p] - at.x, y = gy[p0 + p] - at.y, z = gz[p0 + p] - at.z;
-  const long cs = NP * Mc;
-  switch (at.lmax) {
-    case 0: basis_atom<0, GRAD>(at, shells, exps, norms, contr, x, y, z, phi, Mc, cs, p); break;
-    case 1: basis_atom<1, GRAD>(at, shells, exps, norms, contr, x, y, z, phi, Mc, cs, p); break;
-    case 2: basis_atom<2, GRAD>(at, shells, exps, norms, contr, x, y, z, phi, Mc, cs, p); break;
-    case 3: basis_atom<3, GRAD>(at, shells, exps, norms, contr, x, y, z, phi, Mc, cs, p); break;
-    default: basis_atom<4, GRAD>(at, shells, exps, norms, contr, x, y, z, phi, Mc, cs, p); break;
-  }
-}
-
-template <typename T>
-void launch_basis(hipStream_t s, bool grad, const double *gx, const double *gy,
-                  const double *gz, long p0, long Mc, int n_atoms, const AtomD *atoms,
-                  const ShellD *shells, const double *exps, const double *norms,
-                  const double *contr, T *phi, long NP) {
-  dim3 grid((unsigned)(Mc / kThreads), (unsigned)n_atoms);
-  if (grad)
-    hipLaunchKernelGGL((k_basis<true, T>), grid, dim3(kThreads), 0, s, gx, gy, gz, p0, atoms,
-                       shells, exps, norms, contr, phi, Mc, NP);
-  else
-    hipLaunchKernelGGL((k_basis<false, T>), grid, dim3(kThreads), 0, s, gx, gy, gz, p0, atoms,
-                       shells, exps, norms, contr, phi, Mc, NP);
-}
-template void launch_basis<double>(hipStream_t, bool, const double *, const double *,
-                                   const double *, long, long, int, const AtomD *,
-                                   const ShellD *, const double *, const double *,
-                                   const double *, double *, long);
-template void launch_basis<float>(hipStream_t, bool, const double *, const double *,
-                                  const double *, long, long, int, const AtomD *,
-                                  const ShellD *, const double *, const double *,
-                                  const double *, float *, long);
-
-// ----------------------------------------------------------------- density
-constexpr int kDBM = 128, kDBK = 16;
-int density_tiles(int NP, int tf) { return NP / tf; }
-
-// One workgroup: 128 points x 128 functions of X = Phi D, then the partial
-// contractions of that column tile for each point.
-template <typename T, int BN, bool GRAD>
-__global__ __launch_bounds__(kThreads) void k_density(const T *__restrict__ phi,
-                                                      const T *__restrict__ D, long Mc, int NP,
-                                                      double *__restrict__ part, int ntn) {
-  constexpr int BM = kDBM, BK = kDBK, WM = BM / 2, WN = BN / 2;
-  constexpr int TM = WM / 16, TN = WN / 16;
-  __shared__ T As[2 * BK * BM];
-  __shared__ T Bs[2 * BK * BN];
-  __shared__ double red[2][4][BM];
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int tm = tile / ntn, tn = tile % ntn;
-  const long m0 = (long)tm * BM;
-  const int n0 = tn * BN;
-  typename MF<T>::acc acc[TM][TN];
-  gemm_tile<T, BM, BN, BK, false, false>(phi + m0, Mc, D + n0, NP, NP / BK, As, Bs, acc);
-
-  density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, Mc, NP, m0, n0, tn, part, red);
-}
-
-template <typename T, int BN>
-static void density_bn(hipStream_t s, bool grad, const T *phi, const T *D, long Mc, int NP,
-                       double *part) {
-  const int ntn = NP / BN, ntm = (int)(Mc / kDBM);
-  dim3 grid((unsigned)(ntm * ntn));
-  if (grad)
-    hipLaunchKernelGGL((k_density<T, BN, true>), grid, dim3(kThreads), 0, s, phi, D, Mc, NP, part, ntn);
-  else
-    hipLaunchKernelGGL((k_density<T, BN, false>), grid, dim3(kThreads), 0, s, phi, D, Mc, NP, part, ntn);
-}
-
-template <typename T>
-void launch_density(hipStream_t s, bool grad, const T *phi, const T *D, long Mc, int NP, int tf,
-                    double *part) {
-  switch (tf) {
-    case 64: density_bn<T, 64>(s, grad, phi, D, Mc, NP, part); break;
-    case 96: density_bn<T, 96>(s, grad, phi, D, Mc, NP, part); break;
-    default: density_bn<T, 128>(s, grad, phi, D, Mc, NP, part); break;
-  }
-}
-template void launch_density<double>(hipStream_t, bool, const double *, const double *, long,
-                                     int, int, double *);
-template void launch_density<float>(hipStream_t, bool, const float *, const float *, long, int,
-                                    int, double *);
-
 // -------------------------------------------------------------- functional
 // Closed-shell terms; each returns F, dF/drho, dF/dgamma added into (f, fr, fg).
 namespace xcf {
@@ -456,199 +361,372 @@ __device__ __forceinline__ void pbec(double rho, double g2, double &f, double &f
 }
 }  // namespace xcf
 
-long functional_blocks(long Mc) { return Mc / kThreads; }
+// ============================================================ work lists
+// Every stage kernel is persistent: a grid of at most a few waves of
+// workgroups walks the stage's item list (item = unit-local tile), in the
+// order given by `order` (cost-model schedule) or, without one, in an
+// XCD-blocked order so that neighbouring tiles share an XCD's L2.  Items write
+// disjoint outputs, so the order never changes a result bit.
 
+__device__ __forceinline__ int find_unit(const long *__restrict__ base, int nu, long it) {
+  int lo = 0, hi = nu - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= it) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ long item_at(const int *__restrict__ order, long L, long total) {
+  return order ? (long)order[L] : (long)xcd_remap((int)L, (int)total);
+}
+
+// ------------------------------------------------------------------- basis
+// item = (256-point block, atom) of a unit; one thread per (point, atom)
+template <bool GRAD, typename T>
+__global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ units, int nu,
+                                                    const long *__restrict__ base,
+                                                    const int *__restrict__ order, long total) {
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const long it = item_at(order, L, total);
+    const int u = find_unit(base, nu, it);
+    const UnitD &U = units[u];
+    const long local = it - base[u];
+    const int atom = (int)(local % U.n_atoms);
+    const long p = (local / U.n_atoms) * kThreads + threadIdx.x;
+    const AtomD at = U.atoms[atom];
+    const double x = U.gx[p] - at.x, y = U.gy[p] - at.y, z = U.gz[p] - at.z;
+    T *phi = (T *)U.phi;
+    const long cs = (long)U.NP * U.Mc;
+    switch (at.lmax) {
+      case 0: basis_atom<0, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p); break;
+      case 1: basis_atom<1, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p); break;
+      case 2: basis_atom<2, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p); break;
+      case 3: basis_atom<3, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p); break;
+      default: basis_atom<4, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p); break;
+    }
+  }
+}
+
+// ----------------------------------------------------------------- density
+constexpr int kDBM = 128, kDBK = 16;
+
+// item = (128-point tile, function tile) of X = Phi D, then the tile's
+// partial contractions per point
+template <typename T, int BN, bool GRAD>
+__global__ __launch_bounds__(kThreads) void k_density(const UnitD *__restrict__ units, int nu,
+                                                      const long *__restrict__ base,
+                                                      const int *__restrict__ order, long total) {
+  constexpr int BM = kDBM, BK = kDBK, TM = BM / 32, TN = BN / 32;
+  __shared__ T As[2 * BK * BM];
+  __shared__ T Bs[2 * BK * BN];
+  __shared__ double red[2][4][BM];
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const long it = item_at(order, L, total);
+    const int u = find_unit(base, nu, it);
+    const UnitD &U = units[u];
+    const long local = it - base[u];
+    const int ntn = U.NP / BN;
+    const long m0 = (local / ntn) * BM;
+    const int tn = (int)(local % ntn), n0 = tn * BN;
+    const T *phi = (const T *)U.phi, *D = (const T *)U.D;
+    typename MF<T>::acc acc[TM][TN];
+    gemm_tile<T, BM, BN, BK, false, false>(phi + m0, U.Mc, D + n0, U.NP, U.NP / BK, As, Bs, acc);
+    density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn,
+                                                          U.part, red);
+  }
+}
+
+// item = 256-point block of a unit
 template <bool GGA>
-__global__ __launch_bounds__(kThreads) void k_functional(
-    const double *__restrict__ part, int ntiles, long Mc, const double *__restrict__ w, long p0,
-    uint32_t mask, int incremental, double *__restrict__ state, long npad,
-    double *__restrict__ coef, double *__restrict__ epart, long blk_off) {
+__global__ __launch_bounds__(kThreads) void k_functional(const UnitD *__restrict__ units, int nu,
+                                                         const long *__restrict__ base,
+                                                         const int *__restrict__ order,
+                                                         long total) {
   __shared__ double se[kThreads], sq[kThreads];
   const int tid = threadIdx.x;
-  const long p = (long)blockIdx.x * kThreads + tid, gp = p0 + p;
-  double rho = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
-  for (int t = 0; t < ntiles; ++t) {
-    rho += part[((long)t * 4 + 0) * Mc + p];
-    if (GGA) {
-      g0 += part[((long)t * 4 + 1) * Mc + p];
-      g1 += part[((long)t * 4 + 2) * Mc + p];
-      g2 += part[((long)t * 4 + 3) * Mc + p];
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const long it = item_at(order, L, total);
+    const int u = find_unit(base, nu, it);
+    const UnitD &U = units[u];
+    const long blk = it - base[u];
+    const long p = blk * kThreads + tid, Mc = U.Mc, npad = U.npad;
+    const double *part = U.part;
+    double *state = U.state, *coef = U.coef;  // state is offset to the unit's first point
+    double rho = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    const int ntiles = U.ntiles;
+    for (int t = 0; t < ntiles; ++t) {
+      rho += part[((long)t * 4 + 0) * Mc + p];
+      if (GGA) {
+        g0 += part[((long)t * 4 + 1) * Mc + p];
+        g1 += part[((long)t * 4 + 2) * Mc + p];
+        g2 += part[((long)t * 4 + 3) * Mc + p];
+      }
     }
-  }
-  if (incremental) {
-    rho = state[S_RHO * npad + gp] + rho;
-    if (GGA) {
-      g0 = state[S_GX * npad + gp] + g0;
-      g1 = state[S_GY * npad + gp] + g1;
-      g2 = state[S_GZ * npad + gp] + g2;
+    if (U.incremental) {
+      rho = state[S_RHO * npad + p] + rho;
+      if (GGA) {
+        g0 = state[S_GX * npad + p] + g0;
+        g1 = state[S_GY * npad + p] + g1;
+        g2 = state[S_GZ * npad + p] + g2;
+      }
     }
-  }
-  const double gam = g0 * g0 + g1 * g1 + g2 * g2;
-  double f = 0.0, fr = 0.0, fg = 0.0;
-  if (mask & (1u << 1)) xcf::xalpha(rho, f, fr);
-  if (mask & (1u << 3)) xcf::pwldac(rho, f, fr);
-  if (GGA) {
-    if (mask & (1u << 10)) xcf::pbex(rho, gam, f, fr, fg);
-    if (mask & (1u << 18)) xcf::pbec(rho, gam, f, fr, fg);
-  }
-  const double ww = w[gp];
-  const double a = 0.5 * fr * ww;
-  const double bq = ww * 2.0 * fg;
-  const double b0 = bq * g0, b1 = bq * g1, b2 = bq * g2;
-  if (incremental) {
-    coef[0 * Mc + p] = a - state[S_A * npad + gp];
+    const double gam = g0 * g0 + g1 * g1 + g2 * g2;
+    double f = 0.0, fr = 0.0, fg = 0.0;
+    const uint32_t mask = U.mask;
+    if (mask & (1u << 1)) xcf::xalpha(rho, f, fr);
+    if (mask & (1u << 3)) xcf::pwldac(rho, f, fr);
     if (GGA) {
-      coef[1 * Mc + p] = b0 - state[S_BX * npad + gp];
-      coef[2 * Mc + p] = b1 - state[S_BY * npad + gp];
-      coef[3 * Mc + p] = b2 - state[S_BZ * npad + gp];
+      if (mask & (1u << 10)) xcf::pbex(rho, gam, f, fr, fg);
+      if (mask & (1u << 18)) xcf::pbec(rho, gam, f, fr, fg);
     }
-  } else {
-    coef[0 * Mc + p] = a;
+    const double ww = U.gw[p];
+    const double a = 0.5 * fr * ww;
+    const double bq = ww * 2.0 * fg;
+    const double b0 = bq * g0, b1 = bq * g1, b2 = bq * g2;
+    if (U.incremental) {
+      coef[0 * Mc + p] = a - state[S_A * npad + p];
+      if (GGA) {
+        coef[1 * Mc + p] = b0 - state[S_BX * npad + p];
+        coef[2 * Mc + p] = b1 - state[S_BY * npad + p];
+        coef[3 * Mc + p] = b2 - state[S_BZ * npad + p];
+      }
+    } else {
+      coef[0 * Mc + p] = a;
+      if (GGA) {
+        coef[1 * Mc + p] = b0;
+        coef[2 * Mc + p] = b1;
+        coef[3 * Mc + p] = b2;
+      }
+    }
+    state[S_RHO * npad + p] = rho;
+    state[S_A * npad + p] = a;
     if (GGA) {
-      coef[1 * Mc + p] = b0;
-      coef[2 * Mc + p] = b1;
-      coef[3 * Mc + p] = b2;
+      state[S_GX * npad + p] = g0;
+      state[S_GY * npad + p] = g1;
+      state[S_GZ * npad + p] = g2;
+      state[S_BX * npad + p] = b0;
+      state[S_BY * npad + p] = b1;
+      state[S_BZ * npad + p] = b2;
     }
-  }
-  state[S_RHO * npad + gp] = rho;
-  state[S_A * npad + gp] = a;
-  if (GGA) {
-    state[S_GX * npad + gp] = g0;
-    state[S_GY * npad + gp] = g1;
-    state[S_GZ * npad + gp] = g2;
-    state[S_BX * npad + gp] = b0;
-    state[S_BY * npad + gp] = b1;
-    state[S_BZ * npad + gp] = b2;
-  }
-  se[tid] = f * ww;
-  sq[tid] = rho * ww;
-  __syncthreads();
-  for (int st = kThreads / 2; st > 0; st >>= 1) {
-    if (tid < st) {
-      se[tid] += se[tid + st];
-      sq[tid] += sq[tid + st];
+    se[tid] = f * ww;
+    sq[tid] = rho * ww;
+    __syncthreads();
+    for (int st = kThreads / 2; st > 0; st >>= 1) {
+      if (tid < st) {
+        se[tid] += se[tid + st];
+        sq[tid] += sq[tid + st];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      U.epart[2 * blk + 0] = se[0];
+      U.epart[2 * blk + 1] = sq[0];
     }
     __syncthreads();
   }
-  if (tid == 0) {
-    epart[2 * (blk_off + blockIdx.x) + 0] = se[0];
-    epart[2 * (blk_off + blockIdx.x) + 1] = sq[0];
-  }
-}
-
-void launch_functional(hipStream_t s, bool gga, bool incremental, uint32_t mask,
-                       const double *part, int ntiles, long Mc, const double *w, long p0,
-                       double *state, long npad, double *coef, double *epart, long blk_off) {
-  dim3 grid((unsigned)(Mc / kThreads));
-  if (gga)
-    hipLaunchKernelGGL((k_functional<true>), grid, dim3(kThreads), 0, s, part, ntiles, Mc, w, p0,
-                       mask, (int)incremental, state, npad, coef, epart, blk_off);
-  else
-    hipLaunchKernelGGL((k_functional<false>), grid, dim3(kThreads), 0, s, part, ntiles, Mc, w, p0,
-                       mask, (int)incremental, state, npad, coef, epart, blk_off);
 }
 
 // -------------------------------------------------------------------- W
+// item = (256-point block, 16-function group): W = a Phi + b . grad Phi
 template <typename T, bool GRAD>
-__global__ __launch_bounds__(kThreads) void k_wmat(const T *__restrict__ phi,
-                                                   const double *__restrict__ coef, long Mc,
-                                                   long NP, T *__restrict__ W) {
-  const long p = (long)blockIdx.x * kThreads + threadIdx.x;
-  const long f = blockIdx.y;
-  const long q = f * Mc + p, cs = NP * Mc;
-  double v = coef[p] * (double)phi[q];
-  if (GRAD)
-    v = v + coef[Mc + p] * (double)phi[cs + q] + coef[2 * Mc + p] * (double)phi[2 * cs + q] +
-        coef[3 * Mc + p] * (double)phi[3 * cs + q];
-  W[q] = (T)v;
+__global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ units, int nu,
+                                                   const long *__restrict__ base,
+                                                   const int *__restrict__ order, long total) {
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const long it = item_at(order, L, total);
+    const int u = find_unit(base, nu, it);
+    const UnitD &U = units[u];
+    const long local = it - base[u];
+    const int ng = U.NP / 16;
+    const long p = (local / ng) * kThreads + threadIdx.x;
+    const long f0 = (local % ng) * 16;
+    const long Mc = U.Mc, cs = (long)U.NP * Mc;
+    const T *phi = (const T *)U.phi;
+    T *W = (T *)U.W;
+    const double a = U.coef[p];
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    if (GRAD) {
+      b0 = U.coef[Mc + p];
+      b1 = U.coef[2 * Mc + p];
+      b2 = U.coef[3 * Mc + p];
+    }
+#pragma unroll 4
+    for (int k = 0; k < 16; ++k) {
+      const long q = (f0 + k) * Mc + p;
+      double v = a * (double)phi[q];
+      if (GRAD)
+        v = v + b0 * (double)phi[cs + q] + b1 * (double)phi[2 * cs + q] + b2 * (double)phi[3 * cs + q];
+      W[q] = (T)v;
+    }
+  }
 }
-
-template <typename T>
-void launch_wmat(hipStream_t s, bool grad, const T *phi, const double *coef, long Mc, int NP,
-                 T *W) {
-  dim3 grid((unsigned)(Mc / kThreads), (unsigned)NP);
-  if (grad)
-    hipLaunchKernelGGL((k_wmat<T, true>), grid, dim3(kThreads), 0, s, phi, coef, Mc, (long)NP, W);
-  else
-    hipLaunchKernelGGL((k_wmat<T, false>), grid, dim3(kThreads), 0, s, phi, coef, Mc, (long)NP, W);
-}
-template void launch_wmat<double>(hipStream_t, bool, const double *, const double *, long, int,
-                                  double *);
-template void launch_wmat<float>(hipStream_t, bool, const float *, const double *, long, int,
-                                 float *);
 
 // --------------------------------------------------------------- potential
 constexpr int kPBK = 16;
 
+// item = (point split, function tile pair) of H[split] (+)= W^T Phi
 template <typename T, int BT>
-__global__ __launch_bounds__(kThreads) void k_potential(const T *__restrict__ W,
-                                                        const T *__restrict__ phi, long Mc,
-                                                        int NP, int ksteps, double *Hpart,
-                                                        int ntn, int first) {
-  constexpr int BM = BT, BN = BT, BK = kPBK, WM = BM / 2, WN = BN / 2;
-  constexpr int TM = WM / 16, TN = WN / 16;
-  __shared__ T As[2 * BK * BM];
-  __shared__ T Bs[2 * BK * BN];
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int i0 = (tile / ntn) * BM, j0 = (tile % ntn) * BN;
-  const long kb = (long)blockIdx.y * ksteps * BK;
-  typename MF<T>::acc acc[TM][TN];
-  gemm_tile<T, BM, BN, BK, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb, Mc,
-                                       ksteps, As, Bs, acc);
-  potential_epilogue<T, typename MF<T>::acc, BM, BN>(acc, Hpart + (long)blockIdx.y * NP * NP, NP, i0, j0, first);
+__global__ __launch_bounds__(kThreads) void k_potential(const UnitD *__restrict__ units, int nu,
+                                                        const long *__restrict__ base,
+                                                        const int *__restrict__ order,
+                                                        long total) {
+  constexpr int BK = kPBK, TT = BT / 32;
+  __shared__ T As[2 * BK * BT];
+  __shared__ T Bs[2 * BK * BT];
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const long it = item_at(order, L, total);
+    const int u = find_unit(base, nu, it);
+    const UnitD &U = units[u];
+    const long local = it - base[u];
+    const int nt = U.NP / BT, nt2 = nt * nt;
+    const int sl = (int)(local / nt2), tile = (int)(local % nt2);
+    const int i0 = (tile / nt) * BT, j0 = (tile % nt) * BT;
+    const int ksteps = (int)(U.Mc / BK) / U.nsplit;
+    const long kb = (long)sl * ksteps * BK, Mc = U.Mc;
+    const T *W = (const T *)U.W, *phi = (const T *)U.phi;
+    typename MF<T>::acc acc[TT][TT];
+    gemm_tile<T, BT, BT, BK, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb, Mc,
+                                         ksteps, As, Bs, acc);
+    potential_epilogue<T, typename MF<T>::acc, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP,
+                                                       U.NP, i0, j0, U.first);
+  }
 }
 
-template <typename T, int BT>
-static void potential_bt(hipStream_t s, const T *W, const T *phi, long Mc, int NP, int nsplit,
-                         double *Hpart, bool first) {
-  const int nt = NP / BT;
-  const int ksteps = (int)(Mc / kPBK) / nsplit;
-  dim3 grid((unsigned)(nt * nt), (unsigned)nsplit);
-  hipLaunchKernelGGL((k_potential<T, BT>), grid, dim3(kThreads), 0, s, W, phi, Mc, NP, ksteps,
-                     Hpart, nt, (int)first);
+// -------------------------------------------------------------------- pack
+// item = 256 packed elements of one molecule: V = base + sum_split (H + H^T);
+// then the molecule's state for the next incremental build: V_state = V.
+__global__ __launch_bounds__(kThreads) void k_pack(const PackD *__restrict__ mols, int nm,
+                                                   const long *__restrict__ base, long total) {
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const int m = find_unit(base, nm, L);
+    const PackD &M = mols[m];
+    const long idx = (L - base[m]) * kThreads + threadIdx.x;
+    const long n = M.n, NP = M.NP, npk = n * (n + 1) / 2;
+    if (idx >= npk) continue;
+    long j = (long)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+    while (j * (j + 1) / 2 > idx) --j;
+    while ((j + 1) * (j + 2) / 2 <= idx) ++j;
+    const long k = idx - j * (j + 1) / 2;
+    double hjk = 0.0, hkj = 0.0;
+    for (int s = 0; s < M.nsplit; ++s) {
+      hjk += M.Hpart[(long)s * NP * NP + j * NP + k];
+      hkj += M.Hpart[(long)s * NP * NP + k * NP + j];
+    }
+    const double v = hjk + hkj;
+    const double out = M.incremental ? M.Vstate[idx] + v : v;
+    M.Vout[idx] = out;
+    M.Vstate[idx] = out;
+  }
+}
+
+// Density operand of one molecule: D = P - P_ref (incremental) or P, in the
+// contraction precision; afterwards P_ref = P.  item = 256 elements.
+__global__ __launch_bounds__(kThreads) void k_prep(const PackD *__restrict__ mols, int nm,
+                                                   const long *__restrict__ base, long total,
+                                                   int to_float, int finish) {
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const int m = find_unit(base, nm, L);
+    const PackD &M = mols[m];
+    const long i = (L - base[m]) * kThreads + threadIdx.x;
+    if (i >= (long)M.NP * M.NP) continue;
+    if (finish) {
+      M.Pref[i] = M.P[i];
+    } else {
+      const double d = M.incremental ? M.P[i] - M.Pref[i] : M.P[i];
+      if (to_float)
+        ((float *)M.D)[i] = (float)d;
+      else
+        ((double *)M.D)[i] = d;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- launches
+static unsigned persistent_grid(long total, int per_cu) {
+  const long cap = 256L * per_cu;
+  return (unsigned)(total < cap ? (total > 0 ? total : 1) : cap);
 }
 
 template <typename T>
-void launch_potential(hipStream_t s, const T *W, const T *phi, long Mc, int NP, int tf,
-                      int nsplit, double *Hpart, bool first) {
+void launch_basis(hipStream_t s, bool grad, const StageList &L) {
+  if (!L.total) return;
+  dim3 g(persistent_grid(L.total, 8));
+  if (grad)
+    hipLaunchKernelGGL((k_basis<true, T>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  else
+    hipLaunchKernelGGL((k_basis<false, T>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
+template <typename T, int BN>
+static void density_bn(hipStream_t s, bool grad, const StageList &L) {
+  dim3 g(persistent_grid(L.total, 2));
+  if (grad)
+    hipLaunchKernelGGL((k_density<T, BN, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  else
+    hipLaunchKernelGGL((k_density<T, BN, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
+template <typename T>
+void launch_density(hipStream_t s, bool grad, int tf, const StageList &L) {
+  if (!L.total) return;
   switch (tf) {
-    case 64: potential_bt<T, 64>(s, W, phi, Mc, NP, nsplit, Hpart, first); break;
-    case 96: potential_bt<T, 96>(s, W, phi, Mc, NP, nsplit, Hpart, first); break;
-    default: potential_bt<T, 128>(s, W, phi, Mc, NP, nsplit, Hpart, first); break;
+    case 64: density_bn<T, 64>(s, grad, L); break;
+    case 96: density_bn<T, 96>(s, grad, L); break;
+    default: density_bn<T, 128>(s, grad, L); break;
   }
 }
-template void launch_potential<double>(hipStream_t, const double *, const double *, long, int,
-                                       int, int, double *, bool);
-template void launch_potential<float>(hipStream_t, const float *, const float *, long, int, int,
-                                      int, double *, bool);
 
-// -------------------------------------------------------------------- pack
-__global__ __launch_bounds__(kThreads) void k_pack(const double *__restrict__ Hpart, int nsplit,
-                                                   long NP, long n, const double *base,
-                                                   double *__restrict__ V) {
-  const long idx = (long)blockIdx.x * kThreads + threadIdx.x;
-  const long npk = n * (n + 1) / 2;
-  if (idx >= npk) return;
-  long j = (long)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
-  while (j * (j + 1) / 2 > idx) --j;
-  while ((j + 1) * (j + 2) / 2 <= idx) ++j;
-  const long k = idx - j * (j + 1) / 2;
-  double hjk = 0.0, hkj = 0.0;
-  for (int s = 0; s < nsplit; ++s) {
-    hjk += Hpart[(long)s * NP * NP + j * NP + k];
-    hkj += Hpart[(long)s * NP * NP + k * NP + j];
+void launch_functional(hipStream_t s, bool gga, const StageList &L) {
+  if (!L.total) return;
+  dim3 g(persistent_grid(L.total, 8));
+  if (gga)
+    hipLaunchKernelGGL((k_functional<true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  else
+    hipLaunchKernelGGL((k_functional<false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
+template <typename T>
+void launch_wmat(hipStream_t s, bool grad, const StageList &L) {
+  if (!L.total) return;
+  dim3 g(persistent_grid(L.total, 8));
+  if (grad)
+    hipLaunchKernelGGL((k_wmat<T, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  else
+    hipLaunchKernelGGL((k_wmat<T, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
+template <typename T>
+void launch_potential(hipStream_t s, int tf, const StageList &L) {
+  if (!L.total) return;
+  dim3 g(persistent_grid(L.total, 2));
+  switch (tf) {
+    case 64: hipLaunchKernelGGL((k_potential<T, 64>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
+    case 96: hipLaunchKernelGGL((k_potential<T, 96>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
+    default: hipLaunchKernelGGL((k_potential<T, 128>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
   }
-  const double v = hjk + hkj;
-  V[idx] = base ? base[idx] + v : v;
 }
 
-void launch_pack(hipStream_t s, const double *Hpart, int nsplit, int NP, int n,
-                 const double *base, double *Vpacked) {
-  const long npk = (long)n * (n + 1) / 2;
-  dim3 grid((unsigned)((npk + kThreads - 1) / kThreads));
-  hipLaunchKernelGGL(k_pack, grid, dim3(kThreads), 0, s, Hpart, nsplit, (long)NP, (long)n, base,
-                     Vpacked);
+void launch_pack(hipStream_t s, const PackD *mols, int nm, const long *base, long total) {
+  if (!total) return;
+  hipLaunchKernelGGL(k_pack, dim3(persistent_grid(total, 8)), dim3(kThreads), 0, s, mols, nm, base, total);
 }
+
+void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, long total,
+                 bool to_float, bool finish) {
+  if (!total) return;
+  hipLaunchKernelGGL(k_prep, dim3(persistent_grid(total, 8)), dim3(kThreads), 0, s, mols, nm, base,
+                     total, (int)to_float, (int)finish);
+}
+
+#define PGXC_INST(T)                                                                  \
+  template void launch_basis<T>(hipStream_t, bool, const StageList &);               \
+  template void launch_density<T>(hipStream_t, bool, int, const StageList &);        \
+  template void launch_wmat<T>(hipStream_t, bool, const StageList &);                \
+  template void launch_potential<T>(hipStream_t, int, const StageList &);
+PGXC_INST(double)
+PGXC_INST(float)
+#undef PGXC_INST
 
 // ------------------------------------------------------------ small kernels
 __global__ __launch_bounds__(kThreads) void k_form_density(const double *__restrict__ C,
@@ -667,27 +745,7 @@ void launch_form_density(hipStream_t s, const double *C, const double *occ, int 
   hipLaunchKernelGGL(k_form_density, grid, dim3(kThreads), 0, s, C, occ, n, nocc, NP, P);
 }
 
-__global__ void k_sub(const double *a, const double *b, long n, double *out) {
-  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
-  if (i < n) out[i] = a[i] - b[i];
-}
-void launch_sub(hipStream_t s, const double *a, const double *b, long n, double *out) {
-  hipLaunchKernelGGL(k_sub, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
-                     a, b, n, out);
-}
 
-template <typename T>
-__global__ void k_cast(const double *in, long n, T *out) {
-  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
-  if (i < n) out[i] = (T)in[i];
-}
-template <typename T>
-void launch_cast(hipStream_t s, const double *in, long n, T *out) {
-  hipLaunchKernelGGL((k_cast<T>), dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads),
-                     0, s, in, n, out);
-}
-template void launch_cast<double>(hipStream_t, const double *, long, double *);
-template void launch_cast<float>(hipStream_t, const double *, long, float *);
 
 }  // namespace pgxc
 

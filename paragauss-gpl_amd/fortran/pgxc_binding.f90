@@ -24,7 +24,14 @@ module pgxc_binding
 
   public :: pgxc_create, pgxc_set_density, pgxc_set_orbitals, pgxc_build, pgxc_get_result
   public :: pgxc_nbas, pgxc_last_error, pgxc_destroy, pgxc_last_timing, pgxc_last_stats
-  public :: pgxc_becke_weights
+  public :: pgxc_becke_weights, pgxc_create_opt, pgxc_layout, pgxc_options
+  public :: pgxc_batch_create, pgxc_batch_set_priority, pgxc_batch_build, pgxc_batch_last_ms
+  public :: pgxc_batch_last_error, pgxc_batch_destroy
+
+  type, bind(C) :: pgxc_options
+     integer(c_int32_t) :: tile
+     integer(c_int64_t) :: chunk_points
+  end type pgxc_options
 
   interface
      integer(c_int) function pgxc_create(ctx, n_atoms, atom_xyz, n_shells, shell_info, &
@@ -104,5 +111,60 @@ module pgxc_binding
        integer(c_int32_t), intent(in) :: owner(*)
        real(c_double), intent(out) :: w_out(*)
      end function pgxc_becke_weights
+
+     integer(c_int) function pgxc_create_opt(ctx, n_atoms, atom_xyz, n_shells, shell_info, &
+          exps, norms, contr, grid, npts, veclen, opt) bind(C, name="pgxc_create_opt")
+       import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr, pgxc_options
+       type(c_ptr), intent(out) :: ctx
+       integer(c_int32_t), value :: n_atoms, n_shells, veclen
+       real(c_double), intent(in) :: atom_xyz(*), exps(*), norms(*), contr(*), grid(*)
+       integer(c_int32_t), intent(in) :: shell_info(*)
+       integer(c_int64_t), value :: npts
+       type(pgxc_options), intent(in) :: opt
+     end function pgxc_create_opt
+
+     integer(c_int) function pgxc_layout(ctx, out6) bind(C, name="pgxc_layout")
+       import :: c_int, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: out6(6)
+     end function pgxc_layout
+
+     integer(c_int) function pgxc_batch_create(batch, n, ctxs) bind(C, name="pgxc_batch_create")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), intent(out) :: batch
+       integer(c_int32_t), value :: n
+       type(c_ptr), intent(in) :: ctxs(*)
+     end function pgxc_batch_create
+
+     integer(c_int) function pgxc_batch_set_priority(batch, cost) &
+          bind(C, name="pgxc_batch_set_priority")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: batch
+       real(c_double), intent(in) :: cost(*)
+     end function pgxc_batch_set_priority
+
+     integer(c_int) function pgxc_batch_build(batch, functional, precision, use_graph) &
+          bind(C, name="pgxc_batch_build")
+       import :: c_int, c_int32_t, c_uint32_t, c_ptr
+       type(c_ptr), value :: batch
+       integer(c_uint32_t), value :: functional
+       integer(c_int32_t), value :: precision, use_graph
+     end function pgxc_batch_build
+
+     integer(c_int) function pgxc_batch_last_ms(batch, ms) bind(C, name="pgxc_batch_last_ms")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: batch
+       real(c_double), intent(out) :: ms
+     end function pgxc_batch_last_ms
+
+     type(c_ptr) function pgxc_batch_last_error(batch) bind(C, name="pgxc_batch_last_error")
+       import :: c_ptr
+       type(c_ptr), value :: batch
+     end function pgxc_batch_last_error
+
+     integer(c_int) function pgxc_batch_destroy(batch) bind(C, name="pgxc_batch_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+     end function pgxc_batch_destroy
   end interface
 end module pgxc_binding

@@ -6,5 +6,5 @@ densities), grid (Pople radial x Lebedev x Becke), xc (XCBuilder over the
 C ABI), lebedev (302-point rule).
 """
 from .basis import Basis, Shell, legacy_norm, seeded_basis  # noqa: F401
-from .xc import (BF16X3, FP32, FP64, LDA, PBE, XCBuilder, becke_weights,  # noqa: F401
-                 unpack)
+from .xc import (BF16X3, FP32, FP64, LDA, PBE, Batch, XCBuilder,  # noqa: F401
+                 becke_weights, unpack)

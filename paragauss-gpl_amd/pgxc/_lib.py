@@ -14,9 +14,15 @@ ERRORS = {1: "PGXC_ERR_ARG", 2: "PGXC_ERR_UNSUPPORTED", 3: "PGXC_ERR_HIP",
           4: "PGXC_ERR_NOMEM", 5: "PGXC_ERR_STATE"}
 
 # header symbols, checked by the tests against include/pgxc.h
-EXPORTS = ("pgxc_create", "pgxc_set_density", "pgxc_set_orbitals", "pgxc_build",
-           "pgxc_get_result", "pgxc_nbas", "pgxc_last_stats", "pgxc_last_timing",
-           "pgxc_last_error", "pgxc_destroy", "pgxc_becke_weights")
+EXPORTS = ("pgxc_create", "pgxc_create_opt", "pgxc_layout", "pgxc_set_density",
+           "pgxc_set_orbitals", "pgxc_build", "pgxc_get_result", "pgxc_nbas", "pgxc_last_stats",
+           "pgxc_last_timing", "pgxc_last_error", "pgxc_destroy", "pgxc_becke_weights",
+           "pgxc_batch_create", "pgxc_batch_set_priority", "pgxc_batch_build",
+           "pgxc_batch_last_ms", "pgxc_batch_last_error", "pgxc_batch_destroy")
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int32), ("chunk_points", ctypes.c_int64)]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -41,6 +47,15 @@ def load():
     L = ctypes.CDLL(LIB_PATH)
     L.pgxc_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int32, _dp, ctypes.c_int32, _ip,
                               _dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int32]
+    L.pgxc_create_opt.argtypes = L.pgxc_create.argtypes + [ctypes.POINTER(Options)]
+    L.pgxc_layout.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64)]
+    L.pgxc_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.POINTER(_vp)]
+    L.pgxc_batch_set_priority.argtypes = [_vp, _dp]
+    L.pgxc_batch_build.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32]
+    L.pgxc_batch_last_ms.argtypes = [_vp, _dp]
+    L.pgxc_batch_last_error.argtypes = [_vp]
+    L.pgxc_batch_last_error.restype = ctypes.c_char_p
+    L.pgxc_batch_destroy.argtypes = [_vp]
     L.pgxc_set_density.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_int32]
     L.pgxc_set_orbitals.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp,
                                     _dp, ctypes.c_int32]
@@ -56,7 +71,7 @@ def load():
     L.pgxc_becke_weights.argtypes = [ctypes.c_int32, _dp, ctypes.c_int64, _dp, _ip, _dp, _dp]
     L.pgxc_becke_weights.restype = ctypes.c_int
     for name in EXPORTS:
-        if name not in ("pgxc_nbas", "pgxc_last_error"):
+        if name not in ("pgxc_nbas", "pgxc_last_error", "pgxc_batch_last_error"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

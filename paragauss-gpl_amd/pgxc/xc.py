@@ -34,7 +34,10 @@ def _ptr(a):
 
 
 class XCBuilder:
-    def __init__(self, basis, grid, veclen=128):
+    """tile: function tile of the contractions (64/96/128, 0 = least padding);
+    chunk_points: points per work chunk (multiple of 256, 0 = default)."""
+
+    def __init__(self, basis, grid, veclen=128, tile=0, chunk_points=0):
         self._L = _lib.load()
         self.basis = basis
         self.nbas = basis.nbas
@@ -48,9 +51,10 @@ class XCBuilder:
                       np.ascontiguousarray(basis.contr), g]
         axyz, info, ex, nm, ct, g = self._keep
         h = ctypes.c_void_p()
-        rc = self._L.pgxc_create(ctypes.byref(h), basis.n_atoms, _ptr(axyz), basis.n_shells,
-                                 info.ctypes.data_as(_ip), _ptr(ex), _ptr(nm), _ptr(ct), _ptr(g),
-                                 self.npts, veclen)
+        opt = _lib.Options(tile, chunk_points)
+        rc = self._L.pgxc_create_opt(ctypes.byref(h), basis.n_atoms, _ptr(axyz), basis.n_shells,
+                                     info.ctypes.data_as(_ip), _ptr(ex), _ptr(nm), _ptr(ct),
+                                     _ptr(g), self.npts, veclen, ctypes.byref(opt))
         self._h = h
         self._check(rc)
 
@@ -91,6 +95,12 @@ class XCBuilder:
         self._check(self._L.pgxc_get_result(self._h, _ptr(ham), _ptr(e), _ptr(q)))
         return ham, float(e[0]), float(q[0])
 
+    def layout(self):
+        out = (ctypes.c_int64 * 6)()
+        self._check(self._L.pgxc_layout(self._h, out))
+        return dict(NP=out[0], tile=out[1], chunk_points=out[2], chunks=out[3], nsplit=out[4],
+                    npad=out[5])
+
     def stats(self):
         s = np.zeros(5)
         self._check(self._L.pgxc_last_stats(self._h, _ptr(s)))
@@ -105,6 +115,65 @@ class XCBuilder:
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self._L.pgxc_destroy(self._h)
+        self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Batch:
+    """Many molecules built together (pgxc_batch_*): one launch per stage over
+    the work items of all of them, optionally replayed from a hipGraph.
+    Results stay in each XCBuilder (builder.result())."""
+
+    def __init__(self, builders):
+        self._L = _lib.load()
+        self.builders = list(builders)
+        arr = (ctypes.c_void_p * len(self.builders))(*[b._h.value for b in self.builders])
+        h = ctypes.c_void_p()
+        rc = self._L.pgxc_batch_create(ctypes.byref(h), len(self.builders), arr)
+        self._h = h
+        self._check(rc)
+
+    def _check(self, rc):
+        if rc != _lib.PGXC_OK:
+            raise _lib.PgxcError(rc, self._L.pgxc_batch_last_error(self._h).decode())
+
+    def set_priority(self, cost):
+        if cost is None:
+            self._check(self._L.pgxc_batch_set_priority(self._h, None))
+        else:
+            c = np.ascontiguousarray(cost, dtype=np.float64)
+            assert c.size == len(self.builders)
+            self._check(self._L.pgxc_batch_set_priority(self._h, _ptr(c)))
+
+    def build(self, functional=PBE, precision=FP64, graph=False):
+        if isinstance(functional, str):
+            functional = FUNCTIONALS[functional]
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        self._check(self._L.pgxc_batch_build(self._h, functional, precision, int(bool(graph))))
+
+    def last_ms(self):
+        t = np.zeros(1)
+        self._check(self._L.pgxc_batch_last_ms(self._h, _ptr(t)))
+        return float(t[0])
+
+    def results(self):
+        return [b.result() for b in self.builders]
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.pgxc_batch_destroy(self._h)
         self._h = ctypes.c_void_p()
 
     def __del__(self):
