@@ -4,13 +4,16 @@
 Workload: the (H2O)32 cluster (96 atoms, 1056 basis functions, ~1.9 M
 Becke-partitioned grid points, 75 radial x 302 Lebedev per atom, 160
 occupied orbitals, PBE).  A step is one full XC build over the whole grid.
-  value      : mixed-precision build (FP32 MFMA contractions) run as an
-               incremental difference-density build, ||dP||_F = 1e-4,
-               inputs resident in HBM; device time of the build.
+  value      : mixed-precision build run as an incremental difference-density
+               build (each step a new density, ||dP||_F = 1e-4 from the last),
+               inputs resident in HBM; device time of the build.  The
+               fastest of the FP32-MFMA and split-BF16 forms whose E_xc stays
+               within 1e-6 Eh of the FP64 build is reported.
   fp64_*     : the plain FP64 GPU build of the same workload.
-  dE_vs_fp64 : E_xc of the mixed build minus E_xc of the FP64 build of the
-               same density (the FP64 build matches the CPU oracle to 1e-9 Eh
-               in tests/test_gpu_parity.py).
+  dE_vs_fp64 : E_xc after the chain of mixed incremental builds minus E_xc of
+               a full FP64 build of the final density (the FP64 build
+               matches the CPU oracle to 1e-9 Eh in tests/test_gpu_parity.py);
+               dV_vs_fp64 the same for max |V_xc|.
 Prints one JSON line.
 """
 import argparse
@@ -49,8 +52,17 @@ def workload(name):
     g = grid.molecular_grid(els, xyz, nrad=nrad, nang=302, partition="gpu")
     log("grid: %d points for %d atoms in %.1f s" % (g.shape[0], len(els), time.time() - t))
     C, occ = systems.random_orbitals(b.nbas, nocc, seed=13)
-    C1 = systems.perturbed_orbitals(C, 1e-4, seed=14)
-    return els, b, g, C, C1, occ
+    return els, b, g, C, occ
+
+
+def density_chain(C, n, norm=1e-4, seed=14):
+    """n orbital sets, each a perturbation of the previous with ||dP||_F = norm."""
+    from pgxc import systems
+    out, cur = [], C
+    for k in range(n):
+        cur = systems.perturbed_orbitals(cur, norm, seed=seed + k)
+        out.append(cur)
+    return out
 
 
 def cpu_baseline(b, g, C, occ, budget_s):
@@ -62,7 +74,7 @@ def cpu_baseline(b, g, C, occ, budget_s):
     t = time.time()
     oracle.xc_build(b, g, oracle.MASK_PBE, C=C, occ=occ, veclen=veclen, batch_range=(0, nt),
                     nthreads=nt)
-    per = (time.time() - t) / nt
+    per = time.time() - t  # wall time of one batch per thread
     nsamp = int(min(nb, max(nt, budget_s / max(per, 1e-6) * nt)))
     lo = (nb - nsamp) // 2  # middle of the grid: a mix of inner and outer shells
     t = time.time()
@@ -81,22 +93,37 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="h2o32")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16x3"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(the multi-GPU sharding is a design note only)" % args.gpus)
         sys.exit(2)
 
+    import numpy as np
     from pgxc import XCBuilder
     from pgxc.xc import FP64, PBE, PRECISIONS
 
-    els, b, g, C, C1, occ = workload(args.workload)
+    els, b, g, C, occ = workload(args.workload)
     npts, n = g.shape[0], b.nbas
-    mixed = PRECISIONS[args.precision]
-    xc = XCBuilder(b, g)
+    precs = ["fp32", "bf16x3"] if args.precision == "auto" else [args.precision]
+    tuned = None
+    if not args.no_tune:
+        from pgxc import autotune
+        t0 = time.time()
+        tuned = autotune.tune(b, g, C, occ, mixed=PRECISIONS["fp32" if precs == ["fp32"] else "bf16x3"],
+                              log=log)
+        log("autotune: %s in %.1f s" % ({k: v for k, v in tuned.items() if k != "table"},
+                                         time.time() - t0))
+        xc = XCBuilder(b, g, tile=tuned["tile"],
+                       variants=(tuned["variant_fp64"], tuned["variant_fp32"]))
+    else:
+        xc = XCBuilder(b, g)
+    layout = xc.layout()
+    log("layout:", layout)
 
     # ---- plain FP64 full builds
     xc.set_orbitals(C, occ)
@@ -108,41 +135,43 @@ def main():
         tm = xc.timing()
         t64.append(tm["total"])
         st64.append(tm)
-    _, e64_c, _ = xc.result()
     log("fp64: %.2f ms/build  stages %s" % (sum(t64) / len(t64), st64[-1]))
 
-    # ---- mixed-precision incremental builds, ||dP||_F = 1e-4 per step
-    dens = [C1, C]
-    for i in range(args.warmup):
-        xc.set_orbitals(dens[i % 2], occ, incremental=True)
-        xc.build(PBE, mixed, fetch=False)
-    tmx, stmx = [], []
-    wall0 = time.time()
-    last = None
-    for i in range(args.steps):
-        last = dens[(args.warmup + i) % 2]
-        xc.set_orbitals(last, occ, incremental=True)
-        xc.build(PBE, mixed, fetch=False)
-        tm = xc.timing()
-        tmx.append(tm["total"])
-        stmx.append(tm)
-    wall = time.time() - wall0
-    _, e_mx, _ = xc.result()
-    # FP64 full build of the same final density for the accuracy figure
-    xc.set_orbitals(last, occ)
-    xc.build(PBE, FP64, fetch=False)
-    _, e_ref, _ = xc.result()
+    # ---- mixed-precision incremental builds along a chain of densities
+    chain = density_chain(C, args.warmup + args.steps)
+    runs = {}
+    for prec in precs:
+        xc.set_orbitals(C, occ)
+        xc.build(PBE, FP64, fetch=False)  # FP64 state the chain starts from
+        times, stages = [], []
+        for i, Ck in enumerate(chain):
+            xc.set_orbitals(Ck, occ, incremental=True)
+            xc.build(PBE, PRECISIONS[prec], fetch=False)
+            if i >= args.warmup:
+                tm = xc.timing()
+                times.append(tm["total"])
+                stages.append(tm)
+        v_mx, e_mx, _ = xc.result()
+        xc.set_orbitals(chain[-1], occ)
+        v_ref, e_ref, _ = xc.build(PBE, FP64)
+        runs[prec] = dict(ms=sum(times) / len(times), st=stages[-1], dE=e_mx - e_ref,
+                          dV=float(np.max(np.abs(v_mx - v_ref))), E=e_ref)
+        log("%s: %.2f ms/build dE=%.3e dV=%.3e stages %s" % (prec, runs[prec]["ms"], runs[prec]["dE"],
+                                                            runs[prec]["dV"], stages[-1]))
     xc.close()
+    ok = [p for p in precs if abs(runs[p]["dE"]) <= 1e-6] or precs
+    best = min(ok, key=lambda p: runs[p]["ms"])
+    R = runs[best]
 
     ms64 = sum(t64) / len(t64)
-    msmx = sum(tmx) / len(tmx)
+    msmx = R["ms"]
     # dominant kernel of the mixed build and its roofline (dense-equivalent
     # algorithmic flops: 2 npts n^2 per contraction; no screening on this path)
-    st = stmx[-1]
+    st = R["st"]
     dom = "density" if st["density"] >= st["potential"] else "potential"
     flops = 2.0 * npts * n * n
     achieved = flops / (st[dom] * 1e-3) / 1e12
-    peak = PEAK["fp32" if args.precision == "fp32" else "bf16"]
+    peak = PEAK["fp32"] if best == "fp32" else PEAK["bf16"] / 3.0
     out = {
         "metric": "XC-build grid pts/s, mixed vs FP64 GPU at |dE_xc|<=1e-6 Eh; %MFMA/HBM peak",
         "value": npts / (msmx * 1e-3),
@@ -154,25 +183,31 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+" + args.precision,
+        "dtype": "f64+" + best,
         "data": "synthetic",
         "config": {"workload": "(H2O)32 PBE XC build, %d basis functions, %d grid points, "
-                               "%d occupied, incremental dP (||dP||_F=1e-4), %s contractions"
-                               % (n, npts, C.shape[1], args.precision),
+                               "%d occupied, incremental dP (||dP||_F=1e-4 per step), %s contractions"
+                               % (n, npts, C.shape[1], best),
                    "global_batch": npts, "seq_len": 128, "parallelism": "single GPU"},
         "fp64_pts_per_s": npts / (ms64 * 1e-3),
         "fp64_ms_per_step": ms64,
         "speedup_vs_fp64": ms64 / msmx,
-        "dE_vs_fp64": e_mx - e_ref,
-        "E_xc": e_ref,
-        "wall_pts_per_s_incl_pcie": npts * args.steps / wall,
-        "stages_ms": {k: round(v, 3) for k, v in st.items()},
-        "fp64_stages_ms": {k: round(v, 3) for k, v in st64[-1].items()},
-        "roofline": {"bound": "mfma", "kernel": "k_" + dom, "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None},
+        "dE_vs_fp64": R["dE"],
+        "dV_vs_fp64": R["dV"],
+        "E_xc": R["E"],
+        "mixed_forms": {p: {"ms_per_step": runs[p]["ms"], "pts_per_s": npts / (runs[p]["ms"] * 1e-3),
+                            "dE_vs_fp64": runs[p]["dE"], "dV_vs_fp64": runs[p]["dV"]} for p in precs},
+        "layout": {k: int(v) for k, v in layout.items()},
+        "stages_ms": {k: round(float(v), 3) for k, v in st.items()},
+        "fp64_stages_ms": {k: round(float(v), 3) for k, v in st64[-1].items()},
+        "roofline": {"bound": "mfma", "kernel": "k_" + dom + ("_bf16x3" if best == "bf16x3" else ""),
+                     "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s" + (" (FP32-equivalent: BF16 peak / 3 passes)" if best == "bf16x3" else ""),
+                     "frac": achieved / peak, "traffic": None},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(b, g, C, occ, args.cpu_seconds)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -62,10 +62,16 @@ typedef struct pgxc_batch pgxc_batch;
 /* Layout options of a context (pgxc_create_opt).  tile: function tile of the
  * MFMA contractions, 64, 96 or 128 (0: the one padding nbas least).
  * chunk_points: largest number of grid points held in one work chunk, a
- * multiple of 256 (0: as many as keep the FP64 basis values under 2 GiB). */
+ * multiple of 256 (0: as many as keep the FP64 basis values under 2 GiB).
+ * variant_fp64 / variant_fp32: contraction kernel variant for FP64 and FP32
+ * builds, 0 (K step 16, two LDS stages), 1 (K step 32, one LDS stage),
+ * 2 (K step 32, two LDS stages); -1 keeps the default.  pgxc/autotune.py picks
+ * tile and variants by timing them. */
 typedef struct pgxc_options {
   int32_t tile;
   int64_t chunk_points;
+  int32_t variant_fp64;
+  int32_t variant_fp32;
 } pgxc_options;
 
 /*
@@ -102,8 +108,8 @@ int pgxc_create_opt(pgxc_ctx **ctx, int32_t n_atoms, const double *atom_xyz,
 
 /* Layout chosen for a context: [0] padded nbas, [1] function tile,
  * [2] points per chunk, [3] chunks, [4] point splits of the potential
- * contraction, [5] padded number of points. */
-int pgxc_layout(const pgxc_ctx *ctx, int64_t *out6);
+ * contraction, [5] padded number of points, [6] FP64 variant, [7] FP32 variant. */
+int pgxc_layout(const pgxc_ctx *ctx, int64_t *out8);
 
 /*
  * Density matrix P (nbas x nbas, column-major, symmetric), total density for

@@ -41,6 +41,7 @@ struct pgxc_ctx {
   std::string err;
   // sizes
   int n_atoms = 0, n_shells = 0, nbas = 0, NP = 0, veclen = 128, tf = 128;
+  int var[2] = {0, 1};  // contraction variant for FP64 and for FP32 builds
   long npts = 0, npad = 0, Mc = 0;
   int nchunks = 0, nsplit = 1, ntiles = 0;
   long nblk_total = 0;
@@ -101,10 +102,13 @@ const uint32_t kSupported = PGXC_XC_PBE;
 const int kTiles[3] = {64, 96, 128};
 
 // ------------------------------------------------------------------- plans
+// density / potential lists per (function tile, variant): index 3 * t + v
+constexpr int kLists = 3 * kNumVariants;
+
 struct StepPlan {
   int nunits = 0;
   StageList basis, func, wmat;
-  StageList dens[3], pot[3];
+  StageList dens[kLists], pot[kLists];
 };
 
 struct Plan {
@@ -223,7 +227,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     if (P.has_order) P.order_off = put(order.data(), sizeof(int) * order.size());
     return P;
   };
-  std::vector<std::vector<Pending>> pend(nsteps);  // basis, func, wmat, dens x3, pot x3
+  std::vector<std::vector<Pending>> pend(nsteps);  // basis, func, wmat, dens, pot
   for (int k = 0; k < nsteps; ++k) {
     std::vector<UnitD> units;
     std::vector<int> umol;
@@ -265,15 +269,19 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     pend[k].push_back(stage(units, umol, K_BASIS, 0));
     pend[k].push_back(stage(units, umol, K_FUNC, 0));
     pend[k].push_back(stage(units, umol, K_WMAT, 0));
+    const int cls = precision == PGXC_PREC_FP64 ? 0 : 1;
     for (int kind = 0; kind < 2; ++kind)
-      for (int t = 0; t < 3; ++t) {
+      for (int li = 0; li < kLists; ++li) {
+        const int t = li / kNumVariants, v = li % kNumVariants;
         std::vector<UnitD> sub;
         std::vector<int> subm;
-        for (size_t i = 0; i < units.size(); ++i)
-          if (mols[umol[i]]->tf == kTiles[t]) {
+        for (size_t i = 0; i < units.size(); ++i) {
+          const pgxc_ctx *c = mols[umol[i]];
+          if (c->tf == kTiles[t] && c->var[cls] == v) {
             sub.push_back(units[i]);
             subm.push_back(umol[i]);
           }
+        }
         pend[k].push_back(stage(sub, subm, kind == 0 ? K_DENS : K_POT, kTiles[t]));
       }
   }
@@ -318,8 +326,8 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     S.basis = mk(pend[k][0]);
     S.func = mk(pend[k][1]);
     S.wmat = mk(pend[k][2]);
-    for (int t = 0; t < 3; ++t) S.dens[t] = mk(pend[k][3 + t]);
-    for (int t = 0; t < 3; ++t) S.pot[t] = mk(pend[k][6 + t]);
+    for (int li = 0; li < kLists; ++li) S.dens[li] = mk(pend[k][3 + li]);
+    for (int li = 0; li < kLists; ++li) S.pot[li] = mk(pend[k][3 + kLists + li]);
     S.nunits = pend[k][0].nu;
   }
   plan->d_pack = (PackD *)(B + pack_off);
@@ -343,27 +351,29 @@ static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hip
     if (ev) (void)hipEventRecord((*ev)[0], s);
     launch_basis<T>(s, gga, S.basis);
     if (ev) (void)hipEventRecord((*ev)[1], s);
-    for (int t = 0; t < 3; ++t) {
+    for (int li = 0; li < kLists; ++li) {
+      const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;
       if constexpr (std::is_same<T, float>::value) {
         if (x3) {
-          launch_density_bf16x3(s, gga, kTiles[t], S.dens[t]);
+          launch_density_bf16x3(s, gga, tf, S.dens[li]);
           continue;
         }
       }
-      launch_density<T>(s, gga, kTiles[t], S.dens[t]);
+      launch_density<T>(s, gga, tf, v, S.dens[li]);
     }
     if (ev) (void)hipEventRecord((*ev)[2], s);
     launch_functional(s, gga, S.func);
     launch_wmat<T>(s, gga, S.wmat);
     if (ev) (void)hipEventRecord((*ev)[3], s);
-    for (int t = 0; t < 3; ++t) {
+    for (int li = 0; li < kLists; ++li) {
+      const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;
       if constexpr (std::is_same<T, float>::value) {
         if (x3) {
-          launch_potential_bf16x3(s, kTiles[t], S.pot[t]);
+          launch_potential_bf16x3(s, tf, S.pot[li]);
           continue;
         }
       }
-      launch_potential<T>(s, kTiles[t], S.pot[t]);
+      launch_potential<T>(s, tf, v, S.pot[li]);
     }
     if (ev) {
       (void)hipEventRecord((*ev)[4], s);
@@ -480,6 +490,12 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
     return fail(c, PGXC_ERR_ARG, "pgxc_create: tile must be 0, 64, 96 or 128");
   if (opt && (opt->chunk_points < 0 || opt->chunk_points % 256 != 0))
     return fail(c, PGXC_ERR_ARG, "pgxc_create: chunk_points must be a non-negative multiple of 256");
+  if (opt && (opt->variant_fp64 < -1 || opt->variant_fp64 >= kNumVariants || opt->variant_fp32 < -1 ||
+              opt->variant_fp32 >= kNumVariants))
+    return fail(c, PGXC_ERR_ARG, "pgxc_create: variant must be -1 (default) or 0.." +
+                                     std::to_string(kNumVariants - 1));
+  if (opt && opt->variant_fp64 >= 0) c->var[0] = opt->variant_fp64;
+  if (opt && opt->variant_fp32 >= 0) c->var[1] = opt->variant_fp32;
   c->n_atoms = n_atoms;
   c->n_shells = n_shells;
   c->npts = npts;
@@ -566,7 +582,8 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   // for 256 CUs, each split a whole number of 32-point steps
   const int ntile_pot = (c->NP / c->tf) * (c->NP / c->tf);
   c->nsplit = 1;
-  while (c->nsplit < 16 && ntile_pot * c->nsplit < 512 && (c->Mc / 32) % (c->nsplit * 2) == 0)
+  while (c->nsplit < 16 && ntile_pot * c->nsplit < 512 &&
+         (c->Mc / kSplitStep) % (c->nsplit * 2) == 0)
     c->nsplit *= 2;
   c->nblk_total = c->npad / kThreads;
 
@@ -619,14 +636,16 @@ int pgxc_create(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int32_t
                          npts, veclen, nullptr);
 }
 
-int pgxc_layout(const pgxc_ctx *c, int64_t *out6) {
-  if (!c || !out6) return PGXC_ERR_ARG;
-  out6[0] = c->NP;
-  out6[1] = c->tf;
-  out6[2] = c->Mc;
-  out6[3] = c->nchunks;
-  out6[4] = c->nsplit;
-  out6[5] = c->npad;
+int pgxc_layout(const pgxc_ctx *c, int64_t *out8) {
+  if (!c || !out8) return PGXC_ERR_ARG;
+  out8[0] = c->NP;
+  out8[1] = c->tf;
+  out8[2] = c->Mc;
+  out8[3] = c->nchunks;
+  out8[4] = c->nsplit;
+  out8[5] = c->npad;
+  out8[6] = c->var[0];
+  out8[7] = c->var[1];
   return PGXC_OK;
 }
 

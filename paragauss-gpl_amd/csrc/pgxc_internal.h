@@ -72,14 +72,16 @@ struct StageList {
 // ---- launchers (kernels live in xc_kernels.hip / xc_bf16x3.hip) -----------
 template <typename T>
 void launch_basis(hipStream_t s, bool grad, const StageList &L);
-// tf = function tile (64, 96 or 128) shared by every unit of the list
+// tf = function tile (64, 96 or 128) shared by every unit of the list;
+// var = contraction variant: 0 (K step 16, 2 LDS stages), 1 (32, 1), 2 (32, 2)
+constexpr int kNumVariants = 3;
 template <typename T>
-void launch_density(hipStream_t s, bool grad, int tf, const StageList &L);
+void launch_density(hipStream_t s, bool grad, int tf, int var, const StageList &L);
 void launch_functional(hipStream_t s, bool gga, const StageList &L);
 template <typename T>
 void launch_wmat(hipStream_t s, bool grad, const StageList &L);
 template <typename T>
-void launch_potential(hipStream_t s, int tf, const StageList &L);
+void launch_potential(hipStream_t s, int tf, int var, const StageList &L);
 void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L);
 void launch_potential_bf16x3(hipStream_t s, int tf, const StageList &L);
 void launch_pack(hipStream_t s, const PackD *mols, int nm, const long *base, long total);
@@ -92,6 +94,8 @@ inline long items_density(long Mc, int NP, int tf) { return (Mc / 128) * (NP / t
 inline long items_functional(long Mc) { return Mc / kThreads; }
 inline long items_wmat(long Mc, int NP) { return (Mc / kThreads) * (NP / 16); }
 inline long items_potential(int NP, int tf, int nsplit) { return (long)(NP / tf) * (NP / tf) * nsplit; }
+// points per K step of the potential contraction across all variants
+constexpr int kSplitStep = 32;
 
 // P = C diag(occ) C^T into an NP x NP padded array
 void launch_form_density(hipStream_t s, const double *C, const double *occ, int n,

@@ -16,21 +16,49 @@
 
 namespace pgxc {
 
+// Widest vector (elements, <= 16 bytes) that splits a BM x BK operand tile
+// evenly over the workgroup and along its contiguous dimension.
+constexpr int pick_vec(int elems, int contig, int maxv) {
+  for (int v = maxv; v > 1; v /= 2)
+    if (elems % (kThreads * v) == 0 && contig % v == 0) return v;
+  return 1;
+}
+
+template <typename T, int N>
+struct VecT {
+  typedef T type __attribute__((ext_vector_type(N)));
+  __device__ static __forceinline__ T get(const type &v, int t) { return v[t]; }
+};
+template <typename T>
+struct VecT<T, 1> {
+  typedef T type;
+  __device__ static __forceinline__ T get(const T &v, int) { return v; }
+};
+
 // Block tile BM x BN over K in steps of BK; 4 waves arranged 2 x 2, wave tile
-// (BM/2) x (BN/2) made of 16x16 MFMA blocks.  Two LDS stages, next stage
-// prefetched into registers while the current one feeds the MFMAs.
+// (BM/2) x (BN/2) made of 16x16 MFMA blocks.  ST = 2: two LDS stages, the next
+// slab prefetched into registers while the current one feeds the MFMAs (one
+// barrier per step).  ST = 1: one LDS stage, the next slab's global loads are
+// in flight during the MFMAs (two barriers per step, half the LDS).
 //   AKC: A(m,k) = A[m*sA + k] (k contiguous) else A[m + k*sA]
 //   BKC: B(k,n) = B[n*sB + k] (k contiguous) else B[k*sB + n]
-template <typename T, int BM, int BN, int BK, bool AKC, bool BKC>
+template <typename T, int BM, int BN, int BK, int ST, bool AKC, bool BKC>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
                                           const T *__restrict__ B, long sB, int ksteps,
                                           T *As, T *Bs,
                                           typename MF<T>::acc (&acc)[BM / 32][BN / 32]) {
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
-  constexpr int LA = BM * BK / kThreads, LB = BN * BK / kThreads;
+  constexpr int MAXV = 16 / (int)sizeof(T);
+  constexpr int VA = pick_vec(BM * BK, AKC ? BK : BM, MAXV);
+  constexpr int VB = pick_vec(BN * BK, BKC ? BK : BN, MAXV);
+  constexpr int LA = BM * BK / (kThreads * VA), LB = BN * BK / (kThreads * VB);
+  static_assert(LA * kThreads * VA == BM * BK && LB * kThreads * VB == BN * BK, "tile split");
+  typedef VecT<T, VA> VA_t;
+  typedef VecT<T, VB> VB_t;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  T ra[LA], rb[LB];
+  typename VA_t::type ra[LA];
+  typename VB_t::type rb[LB];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -42,22 +70,22 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
     for (int e = 0; e < LA; ++e) {
       const int idx = tid + kThreads * e;
       if (AKC) {
-        const int mm = idx / BK, kk = idx % BK;
-        ra[e] = A[(long)mm * sA + kb + kk];
+        const int mm = idx / (BK / VA), kv = idx % (BK / VA);
+        ra[e] = *(const typename VA_t::type *)(A + (long)mm * sA + kb + kv * VA);
       } else {
-        const int mm = idx % BM, kk = idx / BM;
-        ra[e] = A[mm + (kb + kk) * sA];
+        const int mv = idx % (BM / VA), kk = idx / (BM / VA);
+        ra[e] = *(const typename VA_t::type *)(A + mv * VA + (kb + kk) * sA);
       }
     }
 #pragma unroll
     for (int e = 0; e < LB; ++e) {
       const int idx = tid + kThreads * e;
       if (BKC) {
-        const int nn = idx / BK, kk = idx % BK;
-        rb[e] = B[(long)nn * sB + kb + kk];
+        const int nn = idx / (BK / VB), kv = idx % (BK / VB);
+        rb[e] = *(const typename VB_t::type *)(B + (long)nn * sB + kb + kv * VB);
       } else {
-        const int nn = idx % BN, kk = idx / BN;
-        rb[e] = B[(kb + kk) * sB + nn];
+        const int nv = idx % (BN / VB), kk = idx / (BN / VB);
+        rb[e] = *(const typename VB_t::type *)(B + (kb + kk) * sB + nv * VB);
       }
     }
   };
@@ -66,24 +94,28 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
 #pragma unroll
     for (int e = 0; e < LA; ++e) {
       const int idx = tid + kThreads * e;
-      const int mm = AKC ? idx / BK : idx % BM, kk = AKC ? idx % BK : idx / BM;
-      as[kk * BM + mm] = ra[e];
+#pragma unroll
+      for (int t = 0; t < VA; ++t) {
+        int mm, kk;
+        if (AKC) { mm = idx / (BK / VA); kk = (idx % (BK / VA)) * VA + t; }
+        else { mm = (idx % (BM / VA)) * VA + t; kk = idx / (BM / VA); }
+        as[kk * BM + mm] = VA_t::get(ra[e], t);
+      }
     }
 #pragma unroll
     for (int e = 0; e < LB; ++e) {
       const int idx = tid + kThreads * e;
-      const int nn = BKC ? idx / BK : idx % BN, kk = BKC ? idx % BK : idx / BN;
-      bs[kk * BN + nn] = rb[e];
+#pragma unroll
+      for (int t = 0; t < VB; ++t) {
+        int nn, kk;
+        if (BKC) { nn = idx / (BK / VB); kk = (idx % (BK / VB)) * VB + t; }
+        else { nn = (idx % (BN / VB)) * VB + t; kk = idx / (BN / VB); }
+        bs[kk * BN + nn] = VB_t::get(rb[e], t);
+      }
     }
   };
-
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int kt = 0; kt < ksteps; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < ksteps) load(kt + 1);
-    const T *as = As + cur * BK * BM, *bs = Bs + cur * BK * BN;
+  auto compute = [&](int buf) {
+    const T *as = As + buf * BK * BM, *bs = Bs + buf * BK * BN;
 #pragma unroll
     for (int k4 = 0; k4 < BK / 4; ++k4) {
       const int kr = k4 * 4 + (lane >> 4);
@@ -97,8 +129,28 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = MF<T>::mma(a[i], b[j], acc[i][j]);
     }
-    if (kt + 1 < ksteps) store(cur ^ 1);
+  };
+
+  if constexpr (ST == 2) {
+    load(0);
+    store(0);
     __syncthreads();
+    for (int kt = 0; kt < ksteps; ++kt) {
+      const int cur = kt & 1;
+      if (kt + 1 < ksteps) load(kt + 1);
+      compute(cur);
+      if (kt + 1 < ksteps) store(cur ^ 1);
+      __syncthreads();
+    }
+  } else {
+    load(0);
+    for (int kt = 0; kt < ksteps; ++kt) {
+      store(0);
+      __syncthreads();
+      if (kt + 1 < ksteps) load(kt + 1);
+      compute(0);
+      __syncthreads();
+    }
   }
 }
 
@@ -409,17 +461,23 @@ __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ un
 }
 
 // ----------------------------------------------------------------- density
-constexpr int kDBM = 128, kDBK = 16;
+constexpr int kDBM = 128;
+
+// K step and LDS stages of the contraction variants the layout can select
+template <int V> struct Var;
+template <> struct Var<0> { static constexpr int BK = 16, ST = 2; };
+template <> struct Var<1> { static constexpr int BK = 32, ST = 1; };
+template <> struct Var<2> { static constexpr int BK = 32, ST = 2; };
 
 // item = (128-point tile, function tile) of X = Phi D, then the tile's
 // partial contractions per point
-template <typename T, int BN, bool GRAD>
+template <typename T, int BN, int VAR, bool GRAD>
 __global__ __launch_bounds__(kThreads) void k_density(const UnitD *__restrict__ units, int nu,
                                                       const long *__restrict__ base,
                                                       const int *__restrict__ order, long total) {
-  constexpr int BM = kDBM, BK = kDBK, TM = BM / 32, TN = BN / 32;
-  __shared__ T As[2 * BK * BM];
-  __shared__ T Bs[2 * BK * BN];
+  constexpr int BM = kDBM, BK = Var<VAR>::BK, ST = Var<VAR>::ST, TM = BM / 32, TN = BN / 32;
+  __shared__ T As[ST * BK * BM];
+  __shared__ T Bs[ST * BK * BN];
   __shared__ double red[2][4][BM];
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const long it = item_at(order, L, total);
@@ -431,7 +489,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const UnitD *__restrict__ 
     const int tn = (int)(local % ntn), n0 = tn * BN;
     const T *phi = (const T *)U.phi, *D = (const T *)U.D;
     typename MF<T>::acc acc[TM][TN];
-    gemm_tile<T, BM, BN, BK, false, false>(phi + m0, U.Mc, D + n0, U.NP, U.NP / BK, As, Bs, acc);
+    gemm_tile<T, BM, BN, BK, ST, false, false>(phi + m0, U.Mc, D + n0, U.NP, U.NP / BK, As, Bs, acc);
     density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn,
                                                           U.part, red);
   }
@@ -563,17 +621,15 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
 }
 
 // --------------------------------------------------------------- potential
-constexpr int kPBK = 16;
-
 // item = (point split, function tile pair) of H[split] (+)= W^T Phi
-template <typename T, int BT>
+template <typename T, int BT, int VAR>
 __global__ __launch_bounds__(kThreads) void k_potential(const UnitD *__restrict__ units, int nu,
                                                         const long *__restrict__ base,
                                                         const int *__restrict__ order,
                                                         long total) {
-  constexpr int BK = kPBK, TT = BT / 32;
-  __shared__ T As[2 * BK * BT];
-  __shared__ T Bs[2 * BK * BT];
+  constexpr int BK = Var<VAR>::BK, ST = Var<VAR>::ST, TT = BT / 32;
+  __shared__ T As[ST * BK * BT];
+  __shared__ T Bs[ST * BK * BT];
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const long it = item_at(order, L, total);
     const int u = find_unit(base, nu, it);
@@ -586,8 +642,8 @@ __global__ __launch_bounds__(kThreads) void k_potential(const UnitD *__restrict_
     const long kb = (long)sl * ksteps * BK, Mc = U.Mc;
     const T *W = (const T *)U.W, *phi = (const T *)U.phi;
     typename MF<T>::acc acc[TT][TT];
-    gemm_tile<T, BT, BT, BK, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb, Mc,
-                                         ksteps, As, Bs, acc);
+    gemm_tile<T, BT, BT, BK, ST, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb,
+                                             Mc, ksteps, As, Bs, acc);
     potential_epilogue<T, typename MF<T>::acc, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP,
                                                        U.NP, i0, j0, U.first);
   }
@@ -658,22 +714,31 @@ void launch_basis(hipStream_t s, bool grad, const StageList &L) {
     hipLaunchKernelGGL((k_basis<false, T>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
 }
 
-template <typename T, int BN>
+template <typename T, int BN, int VAR>
 static void density_bn(hipStream_t s, bool grad, const StageList &L) {
   dim3 g(persistent_grid(L.total, 2));
   if (grad)
-    hipLaunchKernelGGL((k_density<T, BN, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+    hipLaunchKernelGGL((k_density<T, BN, VAR, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
   else
-    hipLaunchKernelGGL((k_density<T, BN, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+    hipLaunchKernelGGL((k_density<T, BN, VAR, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
+template <typename T, int VAR>
+static void density_var(hipStream_t s, bool grad, int tf, const StageList &L) {
+  switch (tf) {
+    case 64: density_bn<T, 64, VAR>(s, grad, L); break;
+    case 96: density_bn<T, 96, VAR>(s, grad, L); break;
+    default: density_bn<T, 128, VAR>(s, grad, L); break;
+  }
 }
 
 template <typename T>
-void launch_density(hipStream_t s, bool grad, int tf, const StageList &L) {
+void launch_density(hipStream_t s, bool grad, int tf, int var, const StageList &L) {
   if (!L.total) return;
-  switch (tf) {
-    case 64: density_bn<T, 64>(s, grad, L); break;
-    case 96: density_bn<T, 96>(s, grad, L); break;
-    default: density_bn<T, 128>(s, grad, L); break;
+  switch (var) {
+    case 1: density_var<T, 1>(s, grad, tf, L); break;
+    case 2: density_var<T, 2>(s, grad, tf, L); break;
+    default: density_var<T, 0>(s, grad, tf, L); break;
   }
 }
 
@@ -696,14 +761,23 @@ void launch_wmat(hipStream_t s, bool grad, const StageList &L) {
     hipLaunchKernelGGL((k_wmat<T, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
 }
 
-template <typename T>
-void launch_potential(hipStream_t s, int tf, const StageList &L) {
-  if (!L.total) return;
+template <typename T, int VAR>
+static void potential_var(hipStream_t s, int tf, const StageList &L) {
   dim3 g(persistent_grid(L.total, 2));
   switch (tf) {
-    case 64: hipLaunchKernelGGL((k_potential<T, 64>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
-    case 96: hipLaunchKernelGGL((k_potential<T, 96>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
-    default: hipLaunchKernelGGL((k_potential<T, 128>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
+    case 64: hipLaunchKernelGGL((k_potential<T, 64, VAR>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
+    case 96: hipLaunchKernelGGL((k_potential<T, 96, VAR>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
+    default: hipLaunchKernelGGL((k_potential<T, 128, VAR>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
+  }
+}
+
+template <typename T>
+void launch_potential(hipStream_t s, int tf, int var, const StageList &L) {
+  if (!L.total) return;
+  switch (var) {
+    case 1: potential_var<T, 1>(s, tf, L); break;
+    case 2: potential_var<T, 2>(s, tf, L); break;
+    default: potential_var<T, 0>(s, tf, L); break;
   }
 }
 
@@ -721,9 +795,9 @@ void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, lon
 
 #define PGXC_INST(T)                                                                  \
   template void launch_basis<T>(hipStream_t, bool, const StageList &);               \
-  template void launch_density<T>(hipStream_t, bool, int, const StageList &);        \
+  template void launch_density<T>(hipStream_t, bool, int, int, const StageList &);   \
   template void launch_wmat<T>(hipStream_t, bool, const StageList &);                \
-  template void launch_potential<T>(hipStream_t, int, const StageList &);
+  template void launch_potential<T>(hipStream_t, int, int, const StageList &);
 PGXC_INST(double)
 PGXC_INST(float)
 #undef PGXC_INST

@@ -31,6 +31,8 @@ module pgxc_binding
   type, bind(C) :: pgxc_options
      integer(c_int32_t) :: tile
      integer(c_int64_t) :: chunk_points
+     integer(c_int32_t) :: variant_fp64
+     integer(c_int32_t) :: variant_fp32
   end type pgxc_options
 
   interface
@@ -123,10 +125,10 @@ module pgxc_binding
        type(pgxc_options), intent(in) :: opt
      end function pgxc_create_opt
 
-     integer(c_int) function pgxc_layout(ctx, out6) bind(C, name="pgxc_layout")
+     integer(c_int) function pgxc_layout(ctx, out8) bind(C, name="pgxc_layout")
        import :: c_int, c_int64_t, c_ptr
        type(c_ptr), value :: ctx
-       integer(c_int64_t), intent(out) :: out6(6)
+       integer(c_int64_t), intent(out) :: out8(8)
      end function pgxc_layout
 
      integer(c_int) function pgxc_batch_create(batch, n, ctxs) bind(C, name="pgxc_batch_create")

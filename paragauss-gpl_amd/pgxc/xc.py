@@ -35,9 +35,10 @@ def _ptr(a):
 
 class XCBuilder:
     """tile: function tile of the contractions (64/96/128, 0 = least padding);
-    chunk_points: points per work chunk (multiple of 256, 0 = default)."""
+    chunk_points: points per work chunk (multiple of 256, 0 = default);
+    variants: (FP64, FP32) contraction kernel variants (-1 = default)."""
 
-    def __init__(self, basis, grid, veclen=128, tile=0, chunk_points=0):
+    def __init__(self, basis, grid, veclen=128, tile=0, chunk_points=0, variants=(-1, -1)):
         self._L = _lib.load()
         self.basis = basis
         self.nbas = basis.nbas
@@ -51,7 +52,7 @@ class XCBuilder:
                       np.ascontiguousarray(basis.contr), g]
         axyz, info, ex, nm, ct, g = self._keep
         h = ctypes.c_void_p()
-        opt = _lib.Options(tile, chunk_points)
+        opt = _lib.Options(tile, chunk_points, variants[0], variants[1])
         rc = self._L.pgxc_create_opt(ctypes.byref(h), basis.n_atoms, _ptr(axyz), basis.n_shells,
                                      info.ctypes.data_as(_ip), _ptr(ex), _ptr(nm), _ptr(ct),
                                      _ptr(g), self.npts, veclen, ctypes.byref(opt))
@@ -96,10 +97,10 @@ class XCBuilder:
         return ham, float(e[0]), float(q[0])
 
     def layout(self):
-        out = (ctypes.c_int64 * 6)()
+        out = (ctypes.c_int64 * 8)()
         self._check(self._L.pgxc_layout(self._h, out))
         return dict(NP=out[0], tile=out[1], chunk_points=out[2], chunks=out[3], nsplit=out[4],
-                    npad=out[5])
+                    npad=out[5], variant_fp64=out[6], variant_fp32=out[7])
 
     def stats(self):
         s = np.zeros(5)
