@@ -34,6 +34,71 @@ struct XStage {
   static constexpr int n = 2 * BM * kXBKP + 2 * BN * kXBKP;
 };
 
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// Staging of one operand tile of R rows x 32 K into the split LDS planes
+// [row][k] (k contiguous, rows padded to kXBKP).  Work unit: a 4 x 4 block
+// (4 rows x 4 k) per thread, read as four 16-byte loads along the contiguous
+// dimension (KC: k contiguous, else rows contiguous, then transposed in
+// registers) and written as 8-byte hi / lo vectors.
+template <int R, bool KC>
+struct XOperand {
+  static constexpr int NB = R * kXBK / 16;              // 4 x 4 blocks
+  static constexpr int PER = (NB + kThreads - 1) / kThreads;
+  float4_ v[PER][4];
+
+  __device__ __forceinline__ void block_rc(int blk, int &r4, int &k4) const {
+    if (KC) {
+      k4 = blk % (kXBK / 4);
+      r4 = blk / (kXBK / 4);
+    } else {
+      r4 = blk % (R / 4);
+      k4 = blk / (R / 4);
+    }
+  }
+  // X(r,k) = X[r*s + k] if KC else X[r + k*s]; kb = first k of the slab
+  __device__ __forceinline__ void load(const float *__restrict__ X, long s, long kb) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int blk = threadIdx.x + kThreads * e;
+      if (NB % kThreads != 0 && blk >= NB) break;
+      int r4, k4;
+      block_rc(blk, r4, k4);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (KC)
+          v[e][i] = *(const float4_ *)(X + (long)(4 * r4 + i) * s + kb + 4 * k4);
+        else
+          v[e][i] = *(const float4_ *)(X + 4 * r4 + (kb + 4 * k4 + i) * s);
+      }
+    }
+  }
+  __device__ __forceinline__ void store(__bf16 *hi, __bf16 *lo) const {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int blk = threadIdx.x + kThreads * e;
+      if (NB % kThreads != 0 && blk >= NB) break;
+      int r4, k4;
+      block_rc(blk, r4, k4);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        bf16x4 h, l;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const float x = KC ? v[e][i][t] : v[e][t][i];
+          __bf16 hh, ll;
+          split_bf16(x, hh, ll);
+          h[t] = hh;
+          l[t] = ll;
+        }
+        const int off = (4 * r4 + i) * kXBKP + 4 * k4;
+        *(bf16x4 *)(hi + off) = h;
+        *(bf16x4 *)(lo + off) = l;
+      }
+    }
+  }
+};
+
 // AKC: A(m,k) = A[m*sA + k] (k contiguous) else A[m + k*sA]; same for B(k,n)
 template <int BM, int BN, bool AKC, bool BKC>
 __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, long sA,
@@ -42,10 +107,10 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, lo
                                                  f4 (&acc)[BM / 32][BN / 32]) {
   constexpr int BK = kXBK, BKP = kXBKP, WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int SA = BM * BKP, SB = BN * BKP, STAGE = 2 * SA + 2 * SB;
-  constexpr int LA = BM * BK / 4 / kThreads, LB = BN * BK / 4 / kThreads;  // float4 per thread
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  float4_ ra[LA], rb[LB];
+  XOperand<BM, AKC> ra;
+  XOperand<BN, BKC> rb;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -53,59 +118,13 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, lo
 
   auto load = [&](int kt) {
     const long kb = (long)kt * BK;
-#pragma unroll
-    for (int e = 0; e < LA; ++e) {
-      const int idx = tid + kThreads * e;
-      if (AKC) {
-        const int mm = idx / (BK / 4), k4 = idx % (BK / 4);
-        ra[e] = *(const float4_ *)(A + (long)mm * sA + kb + 4 * k4);
-      } else {
-        const int m4 = idx % (BM / 4), kk = idx / (BM / 4);
-        ra[e] = *(const float4_ *)(A + 4 * m4 + (kb + kk) * sA);
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < LB; ++e) {
-      const int idx = tid + kThreads * e;
-      if (BKC) {
-        const int nn = idx / (BK / 4), k4 = idx % (BK / 4);
-        rb[e] = *(const float4_ *)(B + (long)nn * sB + kb + 4 * k4);
-      } else {
-        const int n4 = idx % (BN / 4), kk = idx / (BN / 4);
-        rb[e] = *(const float4_ *)(B + (kb + kk) * sB + 4 * n4);
-      }
-    }
+    ra.load(A, sA, kb);
+    rb.load(B, sB, kb);
   };
   auto store = [&](int buf) {
     __bf16 *ah = lds + buf * STAGE, *al = ah + SA, *bh = al + SA, *bl = bh + SB;
-#pragma unroll
-    for (int e = 0; e < LA; ++e) {
-      const int idx = tid + kThreads * e;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        int m, k;
-        if (AKC) { m = idx / (BK / 4); k = 4 * (idx % (BK / 4)) + t; }
-        else { m = 4 * (idx % (BM / 4)) + t; k = idx / (BM / 4); }
-        __bf16 h, l;
-        split_bf16(ra[e][t], h, l);
-        ah[m * BKP + k] = h;
-        al[m * BKP + k] = l;
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < LB; ++e) {
-      const int idx = tid + kThreads * e;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        int n, k;
-        if (BKC) { n = idx / (BK / 4); k = 4 * (idx % (BK / 4)) + t; }
-        else { n = 4 * (idx % (BN / 4)) + t; k = idx / (BN / 4); }
-        __bf16 h, l;
-        split_bf16(rb[e][t], h, l);
-        bh[n * BKP + k] = h;
-        bl[n * BKP + k] = l;
-      }
-    }
+    ra.store(ah, al);
+    rb.store(bh, bl);
   };
 
   load(0);

@@ -158,34 +158,33 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
 // Racah-normalised real solid harmonics up to L and their gradients, by the
 // reference's recursion in l (m = 1 is z-like, then (2m, 2m+1) pairs, top
 // pair by the (x, y) rotation); gradients by differentiating the recursion.
-template <int L>
-__device__ __forceinline__ void sh_eval(double x, double y, double z, double *C, double *Cx,
-                                        double *Cy, double *Cz) {
-  const double r2 = x * x + y * y + z * z;
-  C[0] = 1.0;
-  Cx[0] = Cy[0] = Cz[0] = 0.0;
+template <int L, typename R>
+__device__ __forceinline__ void sh_eval(R x, R y, R z, R *C, R *Cx, R *Cy, R *Cz) {
+  const R r2 = x * x + y * y + z * z;
+  C[0] = R(1);
+  Cx[0] = Cy[0] = Cz[0] = R(0);
   if constexpr (L >= 1) {
     C[1] = z; C[2] = x; C[3] = y;
-    Cx[1] = 0.0; Cy[1] = 0.0; Cz[1] = 1.0;
-    Cx[2] = 1.0; Cy[2] = 0.0; Cz[2] = 0.0;
-    Cx[3] = 0.0; Cy[3] = 1.0; Cz[3] = 0.0;
+    Cx[1] = R(0); Cy[1] = R(0); Cz[1] = R(1);
+    Cx[2] = R(1); Cy[2] = R(0); Cz[2] = R(0);
+    Cx[3] = R(0); Cy[3] = R(1); Cz[3] = R(0);
   }
-  const double dr2x = 2.0 * x, dr2y = 2.0 * y, dr2z = 2.0 * z;
+  const R dr2x = R(2) * x, dr2y = R(2) * y, dr2z = R(2) * z;
 #pragma unroll
   for (int l = 2; l <= L; ++l) {
     const int o = l * l, o1 = (l - 1) * (l - 1), o2 = (l - 2) * (l - 2);
-    const double lr = l, l1r = l - 1, ll1r = 2 * l - 1;
+    const R lr = l, l1r = l - 1, ll1r = 2 * l - 1;
     C[o] = (ll1r * z * C[o1] - l1r * r2 * C[o2]) / lr;
     Cx[o] = (ll1r * z * Cx[o1] - l1r * (dr2x * C[o2] + r2 * Cx[o2])) / lr;
     Cy[o] = (ll1r * z * Cy[o1] - l1r * (dr2y * C[o2] + r2 * Cy[o2])) / lr;
     Cz[o] = (ll1r * (C[o1] + z * Cz[o1]) - l1r * (dr2z * C[o2] + r2 * Cz[o2])) / lr;
 #pragma unroll
     for (int im = 1; im <= l - 2; ++im) {
-      const double sim = (double)(im * im);
-      const double rlm = 1.0 / sqrt(lr * lr - sim);
-      const double rl1m = sqrt(l1r * l1r - sim);
-      const double a = ll1r * z * rlm, b = rl1m * r2 * rlm;
-      const double daz = ll1r * rlm, db = rl1m * rlm;
+      const R sim = (R)(im * im);
+      const R rlm = R(1) / sqrt(lr * lr - sim);
+      const R rl1m = sqrt(l1r * l1r - sim);
+      const R a = ll1r * z * rlm, b = rl1m * r2 * rlm;
+      const R daz = ll1r * rlm, db = rl1m * rlm;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         const int m = 2 * im + k - 1;
@@ -196,9 +195,9 @@ __device__ __forceinline__ void sh_eval(double x, double y, double z, double *C,
       }
     }
     {
-      const double sim = (double)((l - 1) * (l - 1));
-      const double rlm = 1.0 / sqrt(lr * lr - sim);
-      const double a = ll1r * z * rlm, daz = ll1r * rlm;
+      const R sim = (R)((l - 1) * (l - 1));
+      const R rlm = R(1) / sqrt(lr * lr - sim);
+      const R a = ll1r * z * rlm, daz = ll1r * rlm;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         const int m = 2 * l - 3 + k;
@@ -209,7 +208,7 @@ __device__ __forceinline__ void sh_eval(double x, double y, double z, double *C,
       }
     }
     {
-      const double rll = sqrt(ll1r / (2.0 * lr));
+      const R rll = sqrt(ll1r / (R(2) * lr));
       const int i2 = o1 + 2 * l - 3, i1 = o1 + 2 * l - 2;
       C[o + 2 * l - 1] = (x * C[i2] - y * C[i1]) * rll;
       C[o + 2 * l] = (y * C[i2] + x * C[i1]) * rll;
@@ -224,22 +223,23 @@ __device__ __forceinline__ void sh_eval(double x, double y, double z, double *C,
 }
 
 // ------------------------------------------------------------------- basis
+// Evaluation in the output precision T: FP64 for the plain build, FP32 for
+// the mixed builds (whose low-precision error is proportional to ||dP||).
 template <int L, bool GRAD, typename T>
-__device__ __forceinline__ void put_fn(long f0, int nf, double r0, double r1, double x,
-                                       double y, double z, const double *C, const double *Cx,
-                                       const double *Cy, const double *Cz, T *phi, long Mc,
+__device__ __forceinline__ void put_fn(long f0, int nf, T r0, T r1, T x, T y, T z, const T *C,
+                                       const T *Cx, const T *Cy, const T *Cz, T *phi, long Mc,
                                        long cs, int p) {
   constexpr int o = L * L;
 #pragma unroll
   for (int m = 0; m < 2 * L + 1; ++m) {
     const long q = (f0 + (long)m * nf) * Mc + p;
-    const double sv = C[o + m];
-    phi[q] = (T)(sv * r0);
+    const T sv = C[o + m];
+    phi[q] = sv * r0;
     if (GRAD) {
-      const double hv = sv * r1;
-      phi[cs + q] = (T)(x * hv + Cx[o + m] * r0);
-      phi[2 * cs + q] = (T)(y * hv + Cy[o + m] * r0);
-      phi[3 * cs + q] = (T)(z * hv + Cz[o + m] * r0);
+      const T hv = sv * r1;
+      phi[cs + q] = x * hv + Cx[o + m] * r0;
+      phi[2 * cs + q] = y * hv + Cy[o + m] * r0;
+      phi[3 * cs + q] = z * hv + Cz[o + m] * r0;
     }
   }
 }
@@ -247,27 +247,26 @@ __device__ __forceinline__ void put_fn(long f0, int nf, double r0, double r1, do
 template <int L, bool GRAD, typename T>
 __device__ __forceinline__ void shell_emit(const ShellD &sh, const double *__restrict__ exps,
                                            const double *__restrict__ norms,
-                                           const double *__restrict__ contr, double x, double y,
-                                           double z, double r2, const double *C,
-                                           const double *Cx, const double *Cy,
-                                           const double *Cz, T *phi, long Mc, long cs, int p) {
+                                           const double *__restrict__ contr, T x, T y, T z, T r2,
+                                           const T *C, const T *Cx, const T *Cy, const T *Cz,
+                                           T *phi, long Mc, long cs, int p) {
   const int ne = sh.n_exp, nu = sh.n_unc, nc = sh.n_contr, nf = nu + nc;
-  double rc0[kMaxContr], rc1[kMaxContr];
+  T rc0[kMaxContr], rc1[kMaxContr];
 #pragma unroll
-  for (int k = 0; k < kMaxContr; ++k) rc0[k] = rc1[k] = 0.0;
+  for (int k = 0; k < kMaxContr; ++k) rc0[k] = rc1[k] = T(0);
   for (int e = 0; e < ne; ++e) {
-    const double alpha = exps[sh.exp_off + e];
-    const double ar2 = alpha * r2;
-    double r0 = 0.0, r1 = 0.0;
-    if (ar2 < 50.0) {  // primitive cut-off of radial1()
-      r0 = norms[sh.exp_off + e] * exp(-ar2);
-      r1 = (-2.0 * alpha) * r0;
+    const T alpha = (T)exps[sh.exp_off + e];
+    const T ar2 = alpha * r2;
+    T r0 = T(0), r1 = T(0);
+    if (ar2 < T(50)) {  // primitive cut-off of radial1()
+      r0 = (T)norms[sh.exp_off + e] * exp(-ar2);
+      r1 = (T(-2) * alpha) * r0;
     }
     if (e < nu) put_fn<L, GRAD>(sh.func_off + e, nf, r0, r1, x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p);
 #pragma unroll
     for (int k = 0; k < kMaxContr; ++k)
       if (k < nc) {
-        const double c = contr[sh.contr_off + k * ne + e];
+        const T c = (T)contr[sh.contr_off + k * ne + e];
         rc0[k] += r0 * c;
         rc1[k] += r1 * c;
       }
@@ -281,12 +280,12 @@ __device__ __forceinline__ void shell_emit(const ShellD &sh, const double *__res
 template <int LA, bool GRAD, typename T>
 __device__ __forceinline__ void basis_atom(const AtomD &at, const ShellD *__restrict__ shells,
                                            const double *exps, const double *norms,
-                                           const double *contr, double x, double y, double z,
-                                           T *phi, long Mc, long cs, int p) {
+                                           const double *contr, T x, T y, T z, T *phi, long Mc,
+                                           long cs, int p) {
   constexpr int NLM = (LA + 1) * (LA + 1);
-  double C[NLM], Cx[NLM], Cy[NLM], Cz[NLM];
-  sh_eval<LA>(x, y, z, C, Cx, Cy, Cz);
-  const double r2 = x * x + y * y + z * z;
+  T C[NLM], Cx[NLM], Cy[NLM], Cz[NLM];
+  sh_eval<LA, T>(x, y, z, C, Cx, Cy, Cz);
+  const T r2 = x * x + y * y + z * z;
   for (int s = at.shell_begin; s < at.shell_end; ++s) {
     const ShellD sh = shells[s];
     switch (sh.l) {
@@ -447,7 +446,8 @@ __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ un
     const int atom = (int)(local % U.n_atoms);
     const long p = (local / U.n_atoms) * kThreads + threadIdx.x;
     const AtomD at = U.atoms[atom];
-    const double x = U.gx[p] - at.x, y = U.gy[p] - at.y, z = U.gz[p] - at.z;
+    // centre offsets in FP64, then the evaluation precision
+    const T x = (T)(U.gx[p] - at.x), y = (T)(U.gy[p] - at.y), z = (T)(U.gz[p] - at.z);
     T *phi = (T *)U.phi;
     const long cs = (long)U.NP * U.Mc;
     switch (at.lmax) {
