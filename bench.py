@@ -65,6 +65,39 @@ def density_chain(C, n, norm=1e-4, seed=14):
     return out
 
 
+def precision_errors(xc, C, occ, precs, norms=(1e-2, 1e-4, 1e-6)):
+    """|dE_xc| and max|dV| of one incremental mixed build per ||dP||_F, each
+    against the FP64 full build of the same density."""
+    import numpy as np
+    from pgxc.xc import FP64, PBE, PRECISIONS
+    out = {}
+    for k, norm in enumerate(norms):
+        from pgxc import systems
+        C1 = systems.perturbed_orbitals(C, norm, seed=100 + k)
+        xc.set_orbitals(C1, occ)
+        v_ref, e_ref, _ = xc.build(PBE, FP64)
+        for prec in precs:
+            xc.set_orbitals(C, occ)
+            xc.build(PBE, FP64, fetch=False)
+            xc.set_orbitals(C1, occ, incremental=True)
+            v, e, _ = xc.build(PBE, PRECISIONS[prec])
+            out.setdefault(prec, {})["%.0e" % norm] = {
+                "dE": e - e_ref, "dV": float(np.max(np.abs(v - v_ref)))}
+    return out
+
+
+def copy_calibration(mib):
+    """A plain streaming kernel of known size (y = -x over mib MiB of FP32) for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import torch
+    x = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    del x, y
+
+
 def cpu_baseline(b, g, C, occ, budget_s):
     import oracle
     nt = min(16, os.cpu_count() or 1)
@@ -97,6 +130,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a streaming kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -105,6 +140,8 @@ def main():
 
     import numpy as np
     from pgxc import XCBuilder
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     from pgxc.xc import FP64, PBE, PRECISIONS
 
     els, b, g, C, occ = workload(args.workload)
@@ -158,6 +195,8 @@ def main():
                           dV=float(np.max(np.abs(v_mx - v_ref))), E=e_ref)
         log("%s: %.2f ms/build dE=%.3e dV=%.3e stages %s" % (prec, runs[prec]["ms"], runs[prec]["dE"],
                                                             runs[prec]["dV"], stages[-1]))
+    perr = precision_errors(xc, C, occ, precs)
+    log("precision errors per ||dP||_F:", perr)
     xc.close()
     ok = [p for p in precs if abs(runs[p]["dE"]) <= 1e-6] or precs
     best = min(ok, key=lambda p: runs[p]["ms"])
@@ -197,6 +236,7 @@ def main():
         "E_xc": R["E"],
         "mixed_forms": {p: {"ms_per_step": runs[p]["ms"], "pts_per_s": npts / (runs[p]["ms"] * 1e-3),
                             "dE_vs_fp64": runs[p]["dE"], "dV_vs_fp64": runs[p]["dV"]} for p in precs},
+        "precision_errors": perr,
         "layout": {k: int(v) for k, v in layout.items()},
         "stages_ms": {k: round(float(v), 3) for k, v in st.items()},
         "fp64_stages_ms": {k: round(float(v), 3) for k, v in st64[-1].items()},

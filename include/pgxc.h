@@ -66,12 +66,17 @@ typedef struct pgxc_batch pgxc_batch;
  * variant_fp64 / variant_fp32: contraction kernel variant for FP64 and FP32
  * builds, 0 (K step 16, two LDS stages), 1 (K step 32, one LDS stage),
  * 2 (K step 32, two LDS stages); -1 keeps the default.  pgxc/autotune.py picks
- * tile and variants by timing them. */
+ * tile and variants by timing them.  no_screen: 1 runs the contractions over
+ * every basis function and point; 0 (default) skips the 32-function x
+ * 32-point blocks on which the radial cut-off alpha r^2 >= 50
+ * (modules/orbital_module.f90:1839-1855) makes every basis value exactly zero,
+ * which leaves every result bit unchanged. */
 typedef struct pgxc_options {
   int32_t tile;
   int64_t chunk_points;
   int32_t variant_fp64;
   int32_t variant_fp32;
+  int32_t no_screen;
 } pgxc_options;
 
 /*
@@ -88,9 +93,12 @@ typedef struct pgxc_options {
  *   contr: concatenated per shell, n_exp x n_contr column-major each.
  *   atom_xyz: 3 doubles per atom (bohr).
  * Grid: grid is npts x 4 column-major (x, y, z, w), the layout of grdpts
- *   (modules/grid_module.f90:194,3093).  Batches are contiguous veclen-point
- *   slices in grid order (more_grid, modules/grid_module.f90:1133-1181);
- *   veclen = machineparameters_veclen (default 128).
+ *   (modules/grid_module.f90:194,3093).  The reference integrates contiguous
+ *   veclen-point batches in grid order (more_grid,
+ *   modules/grid_module.f90:1133-1181; veclen = machineparameters_veclen,
+ *   default 128); the library regroups the points along a space-filling curve
+ *   so that its point tiles are compact in space.  Every result is a sum over
+ *   points, so only the rounding of the sums depends on the grouping.
  * Replaces: the setup part of xc_build (xc_allocate/orbital_setup,
  *   modules/xc_hamiltonian.f90:239-283) plus the device upload of grid/basis.
  */

@@ -70,6 +70,14 @@ struct pgxc_ctx {
   std::vector<hipEvent_t> ev;
   Plan *plan = nullptr;
   double exps_per_point = 0;
+  // screening tables over the padded grid (see launch_screen)
+  int screen = 0, nfb = 0;
+  long gstride = 0;
+  uint16_t *klist = nullptr;
+  int *kcount = nullptr, *nbt = nullptr;
+  uint8_t *gsig = nullptr;
+  ScreenShell *sshells = nullptr;
+  double flops_screened = 0, flops_dense = 0;  // algorithmic work of one build
 };
 
 namespace {
@@ -263,6 +271,14 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.first = (k == 0);
       U.incremental = incr[m];
       U.mask = functional;
+      U.screen = c->screen;
+      U.nfb = c->nfb;
+      U.gstride = c->gstride;
+      if (c->screen) {
+        U.klist = c->klist + (p0 / kScreenTile) * c->nfb;
+        U.kcount = c->kcount + p0 / kScreenTile;
+        U.gsig = c->gsig + p0 / kGroup;
+      }
       units.push_back(U);
       umol.push_back(m);
     }
@@ -338,19 +354,25 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   return PGXC_OK;
 }
 
-// Launch a plan on stream s.  With ev (8 events) each super-step is timed by
-// stage (basis, density contraction, functional + W, potential contraction)
-// into stage_ms, and ev[5]..ev[6] bracket the packing.
+// Events of a timed build: kEvStart / kEvPack / kEvEnd bracket the build and
+// its packing; step k of the plan records five more from kEvStep + 5 k on
+// (before basis, after basis, after the density contraction, after the
+// functional + W, after the potential contraction).  Nothing waits on them
+// until the build has ended, so timing adds no host synchronisation.
+constexpr int kEvStart = 0, kEvPack = 1, kEvEnd = 2, kEvStep = 3;
+
+// Launch a plan on stream s, recording the events above when ev is given.
 template <typename T>
-static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hipEvent_t> *ev,
-                        float *stage_ms) {
+static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hipEvent_t> *ev) {
   const bool x3 = pl.precision == PGXC_PREC_BF16X3;
   launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total,
               !std::is_same<T, double>::value, false);
-  for (const StepPlan &S : pl.steps) {
-    if (ev) (void)hipEventRecord((*ev)[0], s);
+  for (size_t k = 0; k < pl.steps.size(); ++k) {
+    const StepPlan &S = pl.steps[k];
+    hipEvent_t *E = ev ? ev->data() + kEvStep + 5 * k : nullptr;
+    if (E) (void)hipEventRecord(E[0], s);
     launch_basis<T>(s, gga, S.basis);
-    if (ev) (void)hipEventRecord((*ev)[1], s);
+    if (E) (void)hipEventRecord(E[1], s);
     for (int li = 0; li < kLists; ++li) {
       const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;
       if constexpr (std::is_same<T, float>::value) {
@@ -361,10 +383,10 @@ static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hip
       }
       launch_density<T>(s, gga, tf, v, S.dens[li]);
     }
-    if (ev) (void)hipEventRecord((*ev)[2], s);
+    if (E) (void)hipEventRecord(E[2], s);
     launch_functional(s, gga, S.func);
     launch_wmat<T>(s, gga, S.wmat);
-    if (ev) (void)hipEventRecord((*ev)[3], s);
+    if (E) (void)hipEventRecord(E[3], s);
     for (int li = 0; li < kLists; ++li) {
       const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;
       if constexpr (std::is_same<T, float>::value) {
@@ -375,20 +397,12 @@ static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hip
       }
       launch_potential<T>(s, tf, v, S.pot[li]);
     }
-    if (ev) {
-      (void)hipEventRecord((*ev)[4], s);
-      (void)hipEventSynchronize((*ev)[4]);
-      for (int k = 0; k < 4; ++k) {
-        float t = 0;
-        (void)hipEventElapsedTime(&t, (*ev)[k], (*ev)[k + 1]);
-        stage_ms[k] += t;
-      }
-    }
+    if (E) (void)hipEventRecord(E[4], s);
   }
-  if (ev) (void)hipEventRecord((*ev)[5], s);
+  if (ev) (void)hipEventRecord((*ev)[kEvPack], s);
   launch_pack(s, pl.d_pack, pl.nmol, pl.d_pack_base, pl.pack_total);
   launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total, false, true);
-  if (ev) (void)hipEventRecord((*ev)[6], s);
+  if (ev) (void)hipEventRecord((*ev)[kEvEnd], s);
 }
 
 static int ensure_work(pgxc_ctx *c, size_t elem, hipStream_t s) {
@@ -447,7 +461,9 @@ static void set_stats(pgxc_ctx *c) {
   const double B = (double)c->npts, n = (double)c->nbas;
   c->stats[0] = B;
   c->stats[1] = std::ceil(B / c->veclen);
-  c->stats[2] = 4.0 * B * n * n + 16.0 * B * n;  // dense-equivalent (no screening)
+  // sum over 128-point tiles of 4 B n_b^2 + 16 B n_b, n_b = functions of the
+  // shells not cut off on the tile (dense: n_b = nbas)
+  c->stats[2] = c->screen ? c->flops_screened : c->flops_dense;
   c->stats[3] = B * c->exps_per_point;
   c->stats[4] = 32.0 * B + 16.0 * n * n;
 }
@@ -466,6 +482,77 @@ static int collect(pgxc_ctx *c, hipStream_t s, uint32_t functional) {
   c->have_state = 1;
   c->state_functional = functional;
   set_stats(c);
+  return PGXC_OK;
+}
+
+// Order of the grid points along a Morton curve over their bounding box
+// (10 bits per axis), ties in input order.
+static void morton_order(const double *grid, long npts, std::vector<long> &perm) {
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = hi[k] = grid[(size_t)k * npts];
+    for (long p = 1; p < npts; ++p) {
+      lo[k] = std::min(lo[k], grid[(size_t)k * npts + p]);
+      hi[k] = std::max(hi[k], grid[(size_t)k * npts + p]);
+    }
+  }
+  std::vector<std::pair<uint32_t, long>> key((size_t)npts);
+  for (long p = 0; p < npts; ++p) {
+    uint32_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+      const double span = hi[k] - lo[k];
+      const double f = span > 0 ? (grid[(size_t)k * npts + p] - lo[k]) / span : 0.0;
+      const uint32_t q = (uint32_t)std::min(1023.0, std::max(0.0, f * 1023.0));
+      for (int b = 0; b < 10; ++b) code |= ((q >> b) & 1u) << (3 * b + k);
+    }
+    key[(size_t)p] = {code, p};
+  }
+  std::sort(key.begin(), key.end());
+  perm.resize((size_t)npts);
+  for (long p = 0; p < npts; ++p) perm[(size_t)p] = key[(size_t)p].second;
+}
+
+// Screening tables of the padded grid (kernel k_screen) and the algorithmic
+// work of one build with and without them.
+static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double *exps, bool on) {
+  const long ntp = c->npad / kScreenTile;
+  const double n = (double)c->nbas;
+  c->flops_dense = 4.0 * c->npts * n * n + 16.0 * c->npts * n;
+  c->flops_screened = c->flops_dense;
+  c->nfb = c->NP / kGroup;
+  c->gstride = c->npad / kGroup;
+  c->screen = on && c->nfb <= kMaxBlocks && c->Mc / c->nsplit / kGroup <= kMaxSplitGroups;
+  if (!c->screen) return PGXC_OK;
+  std::vector<ScreenShell> ss(sh.size());
+  for (size_t i = 0; i < sh.size(); ++i) {
+    double amin = exps[sh[i].exp_off];
+    for (int e = 1; e < sh[i].n_exp; ++e) amin = std::min(amin, exps[sh[i].exp_off + e]);
+    ss[i].amin = amin;
+    ss[i].atom = sh[i].atom;
+    ss[i].func_off = sh[i].func_off;
+    ss[i].nfunc = (2 * sh[i].l + 1) * (sh[i].n_unc + sh[i].n_contr);
+    ss[i].pad = 0;
+  }
+  int rc;
+  if ((rc = dalloc(c, &c->sshells, ss.size()))) return rc;
+  if ((rc = dalloc(c, &c->klist, (size_t)ntp * c->nfb))) return rc;
+  if ((rc = dalloc(c, &c->kcount, ntp))) return rc;
+  if ((rc = dalloc(c, &c->nbt, ntp))) return rc;
+  if ((rc = dalloc(c, &c->gsig, (size_t)c->ntiles * c->gstride))) return rc;
+  HIPCHK(c, hipMemcpy(c->sshells, ss.data(), sizeof(ScreenShell) * ss.size(), hipMemcpyHostToDevice));
+  launch_screen(c->stream, ntp, c->grid, c->grid + c->npad, c->grid + 2 * c->npad, c->n_atoms,
+                c->atoms, c->n_shells, c->sshells, c->nfb, c->tf, c->ntiles, c->gstride, c->klist,
+                c->kcount, c->nbt, c->gsig);
+  HIPCHK(c, hipGetLastError());
+  std::vector<int> nb((size_t)ntp);
+  HIPCHK(c, hipMemcpyAsync(nb.data(), c->nbt, sizeof(int) * ntp, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double f = 0.0;
+  for (long t = 0; t < ntp; ++t) {
+    const double B = (double)std::max<long>(0, std::min<long>(kScreenTile, c->npts - t * kScreenTile));
+    f += 4.0 * B * nb[(size_t)t] * nb[(size_t)t] + 16.0 * B * nb[(size_t)t];
+  }
+  c->flops_screened = f;
   return PGXC_OK;
 }
 
@@ -494,6 +581,8 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
               opt->variant_fp32 >= kNumVariants))
     return fail(c, PGXC_ERR_ARG, "pgxc_create: variant must be -1 (default) or 0.." +
                                      std::to_string(kNumVariants - 1));
+  if (opt && (opt->no_screen < 0 || opt->no_screen > 1))
+    return fail(c, PGXC_ERR_ARG, "pgxc_create: no_screen must be 0 or 1");
   if (opt && opt->variant_fp64 >= 0) c->var[0] = opt->variant_fp64;
   if (opt && opt->variant_fp32 >= 0) c->var[1] = opt->variant_fp32;
   c->n_atoms = n_atoms;
@@ -596,17 +685,21 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   if ((rc = dalloc(c, &c->exps, eoff))) return rc;
   if ((rc = dalloc(c, &c->norms, eoff))) return rc;
   if ((rc = dalloc(c, &c->contr, std::max<long>(coff, 1)))) return rc;
-  // grid with padding points far away and zero weight
+  // grid in space-filling-curve order, padded with far points of zero weight
+  std::vector<long> perm;
+  morton_order(grid, npts, perm);
   std::vector<double> g(4 * (size_t)c->npad);
   for (int k = 0; k < 4; ++k)
     for (long p = 0; p < c->npad; ++p)
-      g[(size_t)k * c->npad + p] = p < npts ? grid[(size_t)k * npts + p] : (k < 3 ? kFar : 0.0);
+      g[(size_t)k * c->npad + p] =
+          p < npts ? grid[(size_t)k * npts + perm[p]] : (k < 3 ? kFar : 0.0);
   HIPCHK(c, hipMemcpy(c->grid, g.data(), g.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->atoms, at.data(), sizeof(AtomD) * n_atoms, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->shells, sh.data(), sizeof(ShellD) * n_shells, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->exps, exps, 8 * eoff, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->norms, norms, 8 * eoff, hipMemcpyHostToDevice));
   if (coff > 0) HIPCHK(c, hipMemcpy(c->contr, contr, 8 * coff, hipMemcpyHostToDevice));
+  if ((rc = setup_screen(c, sh, exps, !(opt && opt->no_screen)))) return rc;
   const size_t np2 = (size_t)c->NP * c->NP;
   if ((rc = dalloc(c, &c->P, np2))) return rc;
   if ((rc = dalloc(c, &c->Pref, np2))) return rc;
@@ -624,8 +717,6 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   if ((rc = dalloc(c, &c->Hpart, (size_t)c->nsplit * np2))) return rc;
   HIPCHK(c, hipMemset(c->part, 0, (size_t)c->ntiles * 4 * c->Mc * 8));
   HIPCHK(c, hipMemset(c->coef, 0, 4 * (size_t)c->Mc * 8));
-  c->ev.resize(8);
-  for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
   return PGXC_OK;
 }
 
@@ -719,19 +810,31 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
   std::vector<pgxc_ctx *> mols{c};
   std::vector<int> incrv{incr};
   if ((rc = get_plan(mols, functional, precision, incrv, {}, -1, c->plan, c))) return rc;
-  float ms[4] = {0, 0, 0, 0};
-  HIPCHK(c, hipEventRecord(c->ev[7], s));
+  const size_t nev = kEvStep + 5 * c->plan->steps.size();
+  while (c->ev.size() < nev) {
+    hipEvent_t e = nullptr;
+    HIPCHK(c, hipEventCreate(&e));
+    c->ev.push_back(e);
+  }
+  HIPCHK(c, hipEventRecord(c->ev[kEvStart], s));
   if (elem == 8)
-    launch_plan<double>(s, *c->plan, gga, &c->ev, ms);
+    launch_plan<double>(s, *c->plan, gga, &c->ev);
   else
-    launch_plan<float>(s, *c->plan, gga, &c->ev, ms);
+    launch_plan<float>(s, *c->plan, gga, &c->ev);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(s));
-  for (int k = 0; k < 4; ++k) c->timing[k] = ms[k];
+  for (int k = 0; k < 4; ++k) c->timing[k] = 0.0;
   float t = 0;
-  HIPCHK(c, hipEventElapsedTime(&t, c->ev[5], c->ev[6]));
+  for (size_t k = 0; k < c->plan->steps.size(); ++k) {
+    const hipEvent_t *E = c->ev.data() + kEvStep + 5 * k;
+    for (int j = 0; j < 4; ++j) {
+      HIPCHK(c, hipEventElapsedTime(&t, E[j], E[j + 1]));
+      c->timing[j] += t;
+    }
+  }
+  HIPCHK(c, hipEventElapsedTime(&t, c->ev[kEvPack], c->ev[kEvEnd]));
   c->timing[4] = t;
-  HIPCHK(c, hipEventElapsedTime(&t, c->ev[7], c->ev[6]));
+  HIPCHK(c, hipEventElapsedTime(&t, c->ev[kEvStart], c->ev[kEvEnd]));
   c->timing[5] = t;
   if ((rc = collect(c, s, functional))) return rc;
   return pgxc_get_result(c, ham, exc, charge);
@@ -769,7 +872,8 @@ int pgxc_destroy(pgxc_ctx *c) {
   free_plan(c->plan);
   void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
-                  c->Vstate, c->Vout, c->epart};
+                  c->Vstate, c->Vout, c->epart, c->klist, c->kcount, c->nbt, c->gsig,
+                  c->sshells};
   for (void *b : bufs) dfree(b);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -875,9 +979,9 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       if (elem == 8)
-        launch_plan<double>(s, pl, gga, nullptr, nullptr);
+        launch_plan<double>(s, pl, gga, nullptr);
       else
-        launch_plan<float>(s, pl, gga, nullptr, nullptr);
+        launch_plan<float>(s, pl, gga, nullptr);
       e = hipStreamEndCapture(s, &pl.graph_src);
       if (e == hipSuccess) e = hipGraphInstantiate(&pl.graph, pl.graph_src, nullptr, nullptr, 0);
     }
@@ -893,9 +997,9 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
       return PGXC_ERR_HIP;
     }
   } else if (elem == 8) {
-    launch_plan<double>(s, pl, gga, nullptr, nullptr);
+    launch_plan<double>(s, pl, gga, nullptr);
   } else {
-    launch_plan<float>(s, pl, gga, nullptr, nullptr);
+    launch_plan<float>(s, pl, gga, nullptr);
   }
   (void)hipEventRecord(b->e1, s);
   hipError_t e = hipGetLastError();

@@ -43,10 +43,18 @@ struct UnitD {
   double *state;    // [S_NSLOT][npad] molecule state, offset to the unit's first point
   double *epart;    // [Mc/256][2] block sums of w f and w rho (offset to the unit)
   double *Hpart;    // [nsplit][NP][NP] potential partials of the molecule
-  long Mc, npad;
+  // screening (screen != 0): for each 128-point tile of the unit, the count
+  // and ascending list of 32-function blocks that can be nonzero on it
+  // (klist row stride nfb); for each function tile, a flag per 32-point group
+  // of the molecule's grid telling whether the tile can be nonzero there
+  // (gsig row stride gstride, offset to the unit's first group)
+  const uint16_t *klist;
+  const int *kcount;
+  const uint8_t *gsig;
+  long Mc, npad, gstride;
   int n_atoms, NP, ntiles, nsplit, first, incremental;
   uint32_t mask;
-  int pad;
+  int screen, nfb, pad;
 };
 
 // Per-molecule descriptors of the operand preparation and the final packing.
@@ -96,6 +104,30 @@ inline long items_wmat(long Mc, int NP) { return (Mc / kThreads) * (NP / 16); }
 inline long items_potential(int NP, int tf, int nsplit) { return (long)(NP / tf) * (NP / tf) * nsplit; }
 // points per K step of the potential contraction across all variants
 constexpr int kSplitStep = 32;
+
+// ---- screening ------------------------------------------------------------
+// A radial factor is exactly zero where alpha r^2 >= 50 (the cut-off of
+// ParaGauss's radial, modules/orbital_module.f90:1839-1855), so a shell is
+// zero on a set of points when alpha_min r^2 >= 50 at every one of them.
+// Granularity: 32-point groups x 32-function blocks; the contractions skip
+// K steps whose operand block is zero on the whole step.  Skipped terms are
+// exact zeros, so screened and dense builds give the same bits.
+constexpr int kGroup = 32;
+constexpr int kScreenTile = 128;       // point tile of the density contraction
+constexpr int kMaxBlocks = 256;        // function blocks per molecule (NP <= 8192)
+constexpr int kMaxSplitGroups = 2048;  // point groups per potential split
+constexpr double kCutoff = 50.0 * (1.0 + 1.0e-6);  // margin: a superset in every precision
+struct ScreenShell {
+  double amin;                 // smallest exponent of the shell
+  int atom, func_off, nfunc, pad;
+};
+// per 128-point tile: kcount, klist[nfb], and the number of basis functions
+// of shells that can be nonzero on it (n_b); per function tile of width tf and
+// 32-point group: gsig
+void launch_screen(hipStream_t s, long ntiles_pts, const double *gx, const double *gy,
+                   const double *gz, int n_atoms, const AtomD *atoms, int n_shells,
+                   const ScreenShell *shells, int nfb, int tf, int ntile_f, long gstride,
+                   uint16_t *klist, int *kcount, int *nb, uint8_t *gsig);
 
 // P = C diag(occ) C^T into an NP x NP padded array
 void launch_form_density(hipStream_t s, const double *C, const double *occ, int n,

@@ -47,14 +47,13 @@ struct XOperand {
   static constexpr int PER = (NB + kThreads - 1) / kThreads;
   float4_ v[PER][4];
 
+  // Consecutive threads take consecutive k blocks of one row block for both
+  // layouts: their 8-byte LDS writes land on distinct banks (rows are 20
+  // dwords apart), and each group of 8 lanes still reads whole 128-byte lines
+  // (a row run when k is contiguous, a column run of 4 k columns otherwise).
   __device__ __forceinline__ void block_rc(int blk, int &r4, int &k4) const {
-    if (KC) {
-      k4 = blk % (kXBK / 4);
-      r4 = blk / (kXBK / 4);
-    } else {
-      r4 = blk % (R / 4);
-      k4 = blk / (R / 4);
-    }
+    k4 = blk % (kXBK / 4);
+    r4 = blk / (kXBK / 4);
   }
   // X(r,k) = X[r*s + k] if KC else X[r + k*s]; kb = first k of the slab
   __device__ __forceinline__ void load(const float *__restrict__ X, long s, long kb) {
@@ -99,11 +98,12 @@ struct XOperand {
   }
 };
 
-// AKC: A(m,k) = A[m*sA + k] (k contiguous) else A[m + k*sA]; same for B(k,n)
-template <int BM, int BN, bool AKC, bool BKC>
+// AKC: A(m,k) = A[m*sA + k] (k contiguous) else A[m + k*sA]; same for B(k,n).
+// K step kt starts at k = kmap(kt).
+template <int BM, int BN, bool AKC, bool BKC, class KMap>
 __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, long sA,
                                                  const float *__restrict__ B, long sB,
-                                                 int ksteps, __bf16 *lds,
+                                                 int ksteps, KMap kmap, __bf16 *lds,
                                                  f4 (&acc)[BM / 32][BN / 32]) {
   constexpr int BK = kXBK, BKP = kXBKP, WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int SA = BM * BKP, SB = BN * BKP, STAGE = 2 * SA + 2 * SB;
@@ -116,8 +116,9 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, lo
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
+  if (ksteps <= 0) return;
   auto load = [&](int kt) {
-    const long kb = (long)kt * BK;
+    const long kb = kmap(kt);
     ra.load(A, sA, kb);
     rb.load(B, sB, kb);
   };
@@ -183,6 +184,8 @@ __global__ __launch_bounds__(kThreads) void k_density_bf16x3(const UnitD *__rest
   constexpr int BM = kXBM;
   __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BM, BN>::n];
   __shared__ double red[2][4][BM];
+  __shared__ uint16_t kl[kMaxBlocks];
+  static_assert(kXBK == kGroup, "one K step per 32-function block");
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const long it = item_at_x3(order, L, total);
     const int u = find_unit_x3(base, nu, it);
@@ -192,8 +195,19 @@ __global__ __launch_bounds__(kThreads) void k_density_bf16x3(const UnitD *__rest
     const long m0 = (local / ntn) * BM;
     const int tn = (int)(local % ntn), n0 = tn * BN;
     const float *phi = (const float *)U.phi, *D = (const float *)U.D;
+    bool scr = U.screen != 0;
+    int ksteps = U.NP / kXBK;
+    if (scr) {
+      ksteps = density_screen(U, m0, tn, kl);
+      if (ksteps < 0) {
+        density_zero<BM, GRAD>(U.part, U.Mc, m0, tn);
+        continue;
+      }
+      scr = ksteps < U.nfb;  // a complete list is the dense walk
+    }
+    auto kmap = [&](int kt) -> long { return scr ? (long)kl[kt] * kGroup : (long)kt * kXBK; };
     f4 acc[BM / 32][BN / 32];
-    gemm_tile_bf16x3<BM, BN, false, false>(phi + m0, U.Mc, D + n0, U.NP, U.NP / kXBK, lds, acc);
+    gemm_tile_bf16x3<BM, BN, false, false>(phi + m0, U.Mc, D + n0, U.NP, ksteps, kmap, lds, acc);
     density_epilogue<float, f4, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn, U.part, red);
   }
 }
@@ -228,6 +242,8 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const UnitD *__re
                                                                const int *__restrict__ order,
                                                                long total) {
   __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BT, BT>::n];
+  __shared__ uint16_t gl[kMaxSplitGroups];
+  __shared__ int wtot[4];
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const long it = item_at_x3(order, L, total);
     const int u = find_unit_x3(base, nu, it);
@@ -236,12 +252,16 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const UnitD *__re
     const int nt = U.NP / BT, nt2 = nt * nt;
     const int sl = (int)(local / nt2), tile = (int)(local % nt2);
     const int i0 = (tile / nt) * BT, j0 = (tile % nt) * BT;
-    const int ksteps = (int)(U.Mc / kXBK) / U.nsplit;
-    const long kb = (long)sl * ksteps * kXBK, Mc = U.Mc;
+    const int kdense = (int)(U.Mc / kXBK) / U.nsplit;
+    const long kb = (long)sl * kdense * kXBK, Mc = U.Mc;
+    const int ksteps =
+        U.screen ? potential_screen(U, kb, kdense * kXBK, i0 / BT, j0 / BT, gl, wtot) : kdense;
+    const bool scr = ksteps < kdense;  // a complete list is the dense walk
+    auto kmap = [&](int kt) -> long { return scr ? (long)gl[kt] * kGroup : (long)kt * kXBK; };
     const float *W = (const float *)U.W, *phi = (const float *)U.phi;
     f4 acc[BT / 32][BT / 32];
     gemm_tile_bf16x3<BT, BT, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb, Mc,
-                                         ksteps, lds, acc);
+                                         ksteps, kmap, lds, acc);
     potential_epilogue<float, f4, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP, U.NP, i0, j0,
                                           U.first);
   }

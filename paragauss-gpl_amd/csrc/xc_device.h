@@ -29,6 +29,7 @@ struct MF<double> {
   __device__ static __forceinline__ acc mma(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static constexpr bool kQuadRows = false;
   __device__ static __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
@@ -37,14 +38,81 @@ struct MF<float> {
   __device__ static __forceinline__ acc mma(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static constexpr bool kQuadRows = true;  // a lane's four results are consecutive rows
   __device__ static __forceinline__ int row(int lane, int r) { return ((lane >> 4) << 2) + r; }
 };
 __device__ __forceinline__ int mf_col(int lane) { return lane & 15; }
 
 
-// Epilogue of X = Phi D for one 128-point x 128-function tile: contributions
+// ---------------------------------------------------------------- screening
+// Order-preserving compaction over the workgroup: threads with act append
+// value to list (after `base` entries); returns the new length.  Every thread
+// of the workgroup calls it.
+__device__ __forceinline__ int block_compact(bool act, int value, uint16_t *list, int base,
+                                             int *wtot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(act);
+  const int before = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wtot[wave] = __popcll(m);
+  __syncthreads();
+  int off = base;
+  for (int w = 0; w < wave; ++w) off += wtot[w];
+  if (act) list[off + before] = (uint16_t)value;
+  const int total = base + wtot[0] + wtot[1] + wtot[2] + wtot[3];
+  __syncthreads();
+  return total;
+}
+
+// Density item (128-point tile at m0, function tile tn): -1 when the function
+// tile is zero on the whole point tile; otherwise the number of 32-function
+// blocks that can be nonzero on the point tile, whose ascending list is
+// copied to kl.
+__device__ __forceinline__ int density_screen(const UnitD &U, long m0, int tn, uint16_t *kl) {
+  const long tile = m0 / kScreenTile;
+  const uint8_t *gs = U.gsig + (long)tn * U.gstride + tile * (kScreenTile / kGroup);
+  const bool active = (gs[0] | gs[1] | gs[2] | gs[3]) != 0;
+  const int cnt = U.kcount[tile];
+  if (active)
+    for (int i = threadIdx.x; i < cnt; i += kThreads) kl[i] = U.klist[tile * U.nfb + i];
+  __syncthreads();
+  return active ? cnt : -1;
+}
+
+// Density contributions of an item whose function tile is zero on its points.
+template <int BM, bool GRAD>
+__device__ __forceinline__ void density_zero(double *__restrict__ part, long Mc, long m0, int tn) {
+  constexpr int NC = GRAD ? 4 : 1;
+  for (int idx = threadIdx.x; idx < NC * BM; idx += kThreads) {
+    const int c = idx / BM, row = idx % BM;
+    part[((long)tn * 4 + c) * Mc + m0 + row] = 0.0;
+  }
+}
+
+// Potential item (split starting at point kb of the unit, function tiles ti,
+// tj): the groups of the split on which both tiles can be nonzero, relative to
+// the split start, into gl.  Returns the count.
+__device__ __forceinline__ int potential_screen(const UnitD &U, long kb, int npts_split, int ti,
+                                                int tj, uint16_t *gl, int *wtot) {
+  const int G = npts_split / kGroup;
+  const long g0 = kb / kGroup;
+  const uint8_t *si = U.gsig + (long)ti * U.gstride + g0;
+  const uint8_t *sj = U.gsig + (long)tj * U.gstride + g0;
+  int cnt = 0;
+  for (int b = 0; b < G; b += kThreads) {
+    const int g = b + (int)threadIdx.x;
+    const bool act = g < G && si[g] != 0 && sj[g] != 0;
+    cnt = block_compact(act, g, gl, cnt, wtot);
+  }
+  return cnt;
+}
+
+// Epilogue of X = Phi D for one point tile x function tile: contributions
 // rho += sum_n Phi(p,n) X(p,n), grad rho += 2 sum_n dPhi(p,n) X(p,n), summed
-// over the tile's columns in a fixed order and written to part[tn][c][p].
+// over the tile's columns in a fixed order (j, then the 16-lane butterfly)
+// and written to part[tn][c][p].  With the FP32 result placement a lane holds
+// four consecutive points of one column, read as one 16-byte load per
+// component; the FP64 placement reads point by point.  Both orders of the
+// sums are the same.
 template <typename T, typename ACC, int BM, int BN, bool GRAD>
 __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 32],
                                                  const T *__restrict__ phi, long Mc, int NP,
@@ -52,48 +120,59 @@ __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 
                                                  double *__restrict__ part,
                                                  double (&red)[2][4][BM]) {
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+  constexpr int NC = GRAD ? 4 : 1;
+  constexpr bool QUAD = MF<T>::kQuadRows;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const long cs = (long)NP * Mc;
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int i = 0; i < TM; ++i) {
+    double s[NC][4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = wm * WM + i * 16 + MF<T>::row(lane, r);
-      const long p = m0 + row;
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        const long n = n0 + wn * WN + j * 16 + mf_col(lane);
+      for (int r = 0; r < 4; ++r) s[c][r] = 0.0;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const long n = n0 + wn * WN + j * 16 + mf_col(lane);
+      T v[NC][4];
+      if constexpr (QUAD) {
+        const long q = n * Mc + m0 + wm * WM + i * 16 + MF<T>::row(lane, 0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const f4 t = *(const f4 *)(phi + c * cs + q);
+          v[c][0] = t.x; v[c][1] = t.y; v[c][2] = t.z; v[c][3] = t.w;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long q = n * Mc + m0 + wm * WM + i * 16 + MF<T>::row(lane, r);
+#pragma unroll
+          for (int c = 0; c < NC; ++c) v[c][r] = phi[c * cs + q];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
         const double xv = (double)acc[i][j][r];
-        const long q = n * Mc + p;
-        s0 += (double)phi[q] * xv;
-        if (GRAD) {
-          s1 += 2.0 * (double)phi[cs + q] * xv;
-          s2 += 2.0 * (double)phi[2 * cs + q] * xv;
-          s3 += 2.0 * (double)phi[3 * cs + q] * xv;
-        }
-      }
+        s[0][r] += (double)v[0][r] * xv;
 #pragma unroll
-      for (int off = 8; off >= 1; off >>= 1) {
-        s0 += __shfl_xor(s0, off);
-        if (GRAD) {
-          s1 += __shfl_xor(s1, off);
-          s2 += __shfl_xor(s2, off);
-          s3 += __shfl_xor(s3, off);
-        }
-      }
-      if ((lane & 15) == 0) {
-        red[wn][0][row] = s0;
-        if (GRAD) {
-          red[wn][1][row] = s1;
-          red[wn][2][row] = s2;
-          red[wn][3][row] = s3;
-        }
+        for (int c = 1; c < NC; ++c) s[c][r] += 2.0 * (double)v[c][r] * xv;
       }
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int off = 8; off >= 1; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s[c][r] += __shfl_xor(s[c][r], off);
+      if ((lane & 15) == 0) {
+        const int row = wm * WM + i * 16 + MF<T>::row(lane, r);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) red[wn][c][row] = s[c][r];
+      }
+    }
+  }
   __syncthreads();
-  constexpr int NC = GRAD ? 4 : 1;
   for (int idx = tid; idx < NC * BM; idx += kThreads) {
     const int c = idx / BM, row = idx % BM;
     part[((long)tn * 4 + c) * Mc + m0 + row] = red[0][c][row] + red[1][c][row];

@@ -42,11 +42,31 @@ struct VecT<T, 1> {
 // in flight during the MFMAs (two barriers per step, half the LDS).
 //   AKC: A(m,k) = A[m*sA + k] (k contiguous) else A[m + k*sA]
 //   BKC: B(k,n) = B[n*sB + k] (k contiguous) else B[k*sB + n]
-template <typename T, int BM, int BN, int BK, int ST, bool AKC, bool BKC>
+// LDS layout follows the operands.  Row-contiguous operands (both !KC) are
+// kept [k][row]: the staging writes whole vectors and an MFMA operand is one
+// element per lane.  k-contiguous operands (both KC) are kept [row][k] with
+// rows padded by 16 bytes: the staging again writes the loaded vectors as
+// they are, and each lane reads E = 16 / sizeof(T) consecutive k with one
+// 16-byte read that feeds E MFMAs (MFMA e of a k chunk takes k = 4 E c + E g
+// + e in lane group g, the same for A and B).  Rows 16 bytes apart put the
+// 16-byte reads of 16 lanes on distinct banks.
+template <typename T>
+constexpr int kc_elems() { return 16 / (int)sizeof(T); }
+template <typename T, int BK>
+constexpr int kc_row() { return BK + kc_elems<T>(); }
+template <typename T, int R, int BK, int ST, bool KC>
+constexpr int lds_elems() { return ST * R * (KC ? kc_row<T, BK>() : BK); }
+
+// K step kt starts at k = kmap(kt) (a dense walk or a screened list).
+template <typename T, int BM, int BN, int BK, int ST, bool AKC, bool BKC, class KMap>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
                                           const T *__restrict__ B, long sB, int ksteps,
-                                          T *As, T *Bs,
+                                          KMap kmap, T *As, T *Bs,
                                           typename MF<T>::acc (&acc)[BM / 32][BN / 32]) {
+  static_assert(AKC == BKC, "operands share one LDS layout");
+  constexpr bool KC = AKC;
+  constexpr int E = kc_elems<T>(), BKP = kc_row<T, BK>();
+  static_assert(!KC || BK % (4 * E) == 0, "k chunk");
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int MAXV = 16 / (int)sizeof(T);
   constexpr int VA = pick_vec(BM * BK, AKC ? BK : BM, MAXV);
@@ -64,8 +84,9 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = typename MF<T>::acc{0, 0, 0, 0};
 
+  if (ksteps <= 0) return;
   auto load = [&](int kt) {
-    const long kb = (long)kt * BK;
+    const long kb = kmap(kt);
 #pragma unroll
     for (int e = 0; e < LA; ++e) {
       const int idx = tid + kThreads * e;
@@ -90,6 +111,22 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
     }
   };
   auto store = [&](int buf) {
+    if constexpr (KC) {
+      T *as = As + buf * BM * BKP, *bs = Bs + buf * BN * BKP;
+#pragma unroll
+      for (int e = 0; e < LA; ++e) {
+        const int idx = tid + kThreads * e;
+        const int mm = idx / (BK / VA), kv = idx % (BK / VA);
+        *(typename VA_t::type *)(as + mm * BKP + kv * VA) = ra[e];
+      }
+#pragma unroll
+      for (int e = 0; e < LB; ++e) {
+        const int idx = tid + kThreads * e;
+        const int nn = idx / (BK / VB), kv = idx % (BK / VB);
+        *(typename VB_t::type *)(bs + nn * BKP + kv * VB) = rb[e];
+      }
+      return;
+    }
     T *as = As + buf * BK * BM, *bs = Bs + buf * BK * BN;
 #pragma unroll
     for (int e = 0; e < LA; ++e) {
@@ -115,6 +152,28 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
     }
   };
   auto compute = [&](int buf) {
+    if constexpr (KC) {
+      typedef typename VecT<T, E>::type VE;
+      const T *as = As + buf * BM * BKP, *bs = Bs + buf * BN * BKP;
+      const int ko = E * (lane >> 4);
+#pragma unroll
+      for (int c = 0; c < BK / (4 * E); ++c) {
+        VE a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+          a[i] = *(const VE *)(as + (wm * WM + i * 16 + (lane & 15)) * BKP + 4 * E * c + ko);
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          b[j] = *(const VE *)(bs + (wn * WN + j * 16 + (lane & 15)) * BKP + 4 * E * c + ko);
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] = MF<T>::mma(a[i][e], b[j][e], acc[i][j]);
+      }
+      return;
+    }
     const T *as = As + buf * BK * BM, *bs = Bs + buf * BK * BN;
 #pragma unroll
     for (int k4 = 0; k4 < BK / 4; ++k4) {
@@ -476,9 +535,11 @@ __global__ __launch_bounds__(kThreads) void k_density(const UnitD *__restrict__ 
                                                       const long *__restrict__ base,
                                                       const int *__restrict__ order, long total) {
   constexpr int BM = kDBM, BK = Var<VAR>::BK, ST = Var<VAR>::ST, TM = BM / 32, TN = BN / 32;
+  constexpr int R = kGroup / BK;  // K steps per 32-function block
   __shared__ T As[ST * BK * BM];
   __shared__ T Bs[ST * BK * BN];
   __shared__ double red[2][4][BM];
+  __shared__ uint16_t kl[kMaxBlocks];
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const long it = item_at(order, L, total);
     const int u = find_unit(base, nu, it);
@@ -488,8 +549,23 @@ __global__ __launch_bounds__(kThreads) void k_density(const UnitD *__restrict__ 
     const long m0 = (local / ntn) * BM;
     const int tn = (int)(local % ntn), n0 = tn * BN;
     const T *phi = (const T *)U.phi, *D = (const T *)U.D;
+    bool scr = U.screen != 0;
+    int ksteps = U.NP / BK;
+    if (scr) {
+      const int cnt = density_screen(U, m0, tn, kl);
+      if (cnt < 0) {
+        density_zero<BM, GRAD>(U.part, U.Mc, m0, tn);
+        continue;
+      }
+      scr = cnt < U.nfb;  // a complete list is the dense walk
+      ksteps = cnt * R;
+    }
+    auto kmap = [&](int kt) -> long {
+      return scr ? (long)kl[kt / R] * kGroup + (kt % R) * BK : (long)kt * BK;
+    };
     typename MF<T>::acc acc[TM][TN];
-    gemm_tile<T, BM, BN, BK, ST, false, false>(phi + m0, U.Mc, D + n0, U.NP, U.NP / BK, As, Bs, acc);
+    gemm_tile<T, BM, BN, BK, ST, false, false>(phi + m0, U.Mc, D + n0, U.NP, ksteps, kmap, As, Bs,
+                                               acc);
     density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn,
                                                           U.part, red);
   }
@@ -628,8 +704,11 @@ __global__ __launch_bounds__(kThreads) void k_potential(const UnitD *__restrict_
                                                         const int *__restrict__ order,
                                                         long total) {
   constexpr int BK = Var<VAR>::BK, ST = Var<VAR>::ST, TT = BT / 32;
-  __shared__ T As[ST * BK * BT];
-  __shared__ T Bs[ST * BK * BT];
+  constexpr int R = kGroup / BK;  // K steps per 32-point group
+  __shared__ __attribute__((aligned(16))) T As[lds_elems<T, BT, BK, ST, true>()];
+  __shared__ __attribute__((aligned(16))) T Bs[lds_elems<T, BT, BK, ST, true>()];
+  __shared__ uint16_t gl[kMaxSplitGroups];
+  __shared__ int wtot[4];
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const long it = item_at(order, L, total);
     const int u = find_unit(base, nu, it);
@@ -638,12 +717,18 @@ __global__ __launch_bounds__(kThreads) void k_potential(const UnitD *__restrict_
     const int nt = U.NP / BT, nt2 = nt * nt;
     const int sl = (int)(local / nt2), tile = (int)(local % nt2);
     const int i0 = (tile / nt) * BT, j0 = (tile % nt) * BT;
-    const int ksteps = (int)(U.Mc / BK) / U.nsplit;
-    const long kb = (long)sl * ksteps * BK, Mc = U.Mc;
+    const int kdense = (int)(U.Mc / BK) / U.nsplit;
+    const long kb = (long)sl * kdense * BK, Mc = U.Mc;
+    const int ksteps = U.screen ? potential_screen(U, kb, kdense * BK, i0 / BT, j0 / BT, gl, wtot) * R
+                                : kdense;
+    const bool scr = ksteps < kdense;  // a complete list is the dense walk
+    auto kmap = [&](int kt) -> long {
+      return scr ? (long)gl[kt / R] * kGroup + (kt % R) * BK : (long)kt * BK;
+    };
     const T *W = (const T *)U.W, *phi = (const T *)U.phi;
     typename MF<T>::acc acc[TT][TT];
     gemm_tile<T, BT, BT, BK, ST, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb,
-                                             Mc, ksteps, As, Bs, acc);
+                                             Mc, ksteps, kmap, As, Bs, acc);
     potential_epilogue<T, typename MF<T>::acc, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP,
                                                        U.NP, i0, j0, U.first);
   }
@@ -801,6 +886,91 @@ void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, lon
 PGXC_INST(double)
 PGXC_INST(float)
 #undef PGXC_INST
+
+// ---------------------------------------------------------------- screening
+// item = one 128-point tile (four 32-point groups).  The smallest r^2 of each
+// group to each atom (FP64, reduced over the group's 32 lanes), then per
+// shell and group alpha_min r^2_min < cut-off marks the shell's function
+// blocks.  Outputs: the tile's ascending block list and count, n_b (functions
+// of the shells marked on the tile) and, per function tile, a flag per group.
+__global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *__restrict__ gx,
+                                                     const double *__restrict__ gy,
+                                                     const double *__restrict__ gz, int n_atoms,
+                                                     const AtomD *__restrict__ atoms, int n_shells,
+                                                     const ScreenShell *__restrict__ shells,
+                                                     int nfb, int tf, int ntile_f, long gstride,
+                                                     uint16_t *klist, int *kcount, int *nb,
+                                                     uint8_t *gsig) {
+  constexpr int AC = 512;  // atoms per pass
+  __shared__ double r2g[4][AC];
+  __shared__ unsigned bits[4][kMaxBlocks / 32];
+  __shared__ int nbs;
+  const int tid = threadIdx.x;
+  for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int i = tid; i < 4 * (kMaxBlocks / 32); i += kThreads) (&bits[0][0])[i] = 0u;
+    if (tid == 0) nbs = 0;
+    const long p = t * kScreenTile + (tid & 127);
+    const double x = gx[p], y = gy[p], z = gz[p];
+    for (int a0 = 0; a0 < n_atoms; a0 += AC) {
+      const int na = min(AC, n_atoms - a0);
+      __syncthreads();
+      // threads 0..127 take even atoms, 128..255 odd ones (whole waves each)
+      for (int a = tid >> 7; a < na; a += 2) {
+        const AtomD A = atoms[a0 + a];
+        const double dx = x - A.x, dy = y - A.y, dz = z - A.z;
+        double r2 = dx * dx + dy * dy + dz * dz;
+#pragma unroll
+        for (int off = 16; off >= 1; off >>= 1) r2 = fmin(r2, __shfl_xor(r2, off));
+        if ((tid & 31) == 0) r2g[(tid & 127) >> 5][a] = r2;
+      }
+      __syncthreads();
+      for (int si = tid; si < n_shells; si += kThreads) {
+        const ScreenShell S = shells[si];
+        if (S.atom < a0 || S.atom >= a0 + na) continue;
+        bool any = false;
+        const int b0 = S.func_off / kGroup, b1 = (S.func_off + S.nfunc - 1) / kGroup;
+        for (int q = 0; q < 4; ++q) {
+          if (S.amin * r2g[q][S.atom - a0] < kCutoff) {
+            any = true;
+            for (int b = b0; b <= b1; ++b) atomicOr(&bits[q][b >> 5], 1u << (b & 31));
+          }
+        }
+        if (any) atomicAdd(&nbs, S.nfunc);
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int cnt = 0;
+      for (int w = 0; w < (nfb + 31) / 32; ++w) {
+        unsigned m = bits[0][w] | bits[1][w] | bits[2][w] | bits[3][w];
+        while (m) {
+          klist[t * nfb + cnt++] = (uint16_t)(w * 32 + __builtin_ctz(m));
+          m &= m - 1;
+        }
+      }
+      kcount[t] = cnt;
+      nb[t] = nbs;
+    }
+    const int bpt = tf / kGroup;
+    for (int i = tid; i < 4 * ntile_f; i += kThreads) {
+      const int q = i & 3, ft = i >> 2;
+      unsigned f = 0;
+      for (int b = ft * bpt; b < (ft + 1) * bpt; ++b) f |= (bits[q][b >> 5] >> (b & 31)) & 1u;
+      gsig[(long)ft * gstride + t * 4 + q] = (uint8_t)f;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_screen(hipStream_t s, long ntiles_pts, const double *gx, const double *gy,
+                   const double *gz, int n_atoms, const AtomD *atoms, int n_shells,
+                   const ScreenShell *shells, int nfb, int tf, int ntile_f, long gstride,
+                   uint16_t *klist, int *kcount, int *nb, uint8_t *gsig) {
+  if (ntiles_pts <= 0) return;
+  hipLaunchKernelGGL(k_screen, dim3(persistent_grid(ntiles_pts, 8)), dim3(kThreads), 0, s,
+                     ntiles_pts, gx, gy, gz, n_atoms, atoms, n_shells, shells, nfb, tf, ntile_f,
+                     gstride, klist, kcount, nb, gsig);
+}
 
 // ------------------------------------------------------------ small kernels
 __global__ __launch_bounds__(kThreads) void k_form_density(const double *__restrict__ C,

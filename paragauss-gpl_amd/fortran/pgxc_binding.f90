@@ -33,6 +33,7 @@ module pgxc_binding
      integer(c_int64_t) :: chunk_points
      integer(c_int32_t) :: variant_fp64
      integer(c_int32_t) :: variant_fp32
+     integer(c_int32_t) :: no_screen
   end type pgxc_options
 
   interface

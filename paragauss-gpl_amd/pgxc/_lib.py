@@ -23,7 +23,8 @@ EXPORTS = ("pgxc_create", "pgxc_create_opt", "pgxc_layout", "pgxc_set_density",
 
 class Options(ctypes.Structure):
     _fields_ = [("tile", ctypes.c_int32), ("chunk_points", ctypes.c_int64),
-                ("variant_fp64", ctypes.c_int32), ("variant_fp32", ctypes.c_int32)]
+                ("variant_fp64", ctypes.c_int32), ("variant_fp32", ctypes.c_int32),
+                ("no_screen", ctypes.c_int32)]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)

@@ -1,0 +1,125 @@
+"""Screening: the contractions skip 32-function x 32-point blocks on which
+the radial cut-off (alpha r^2 >= 50, ParaGauss modules/orbital_module.f90:
+1839-1855) makes every basis value exactly zero.  The skipped terms are exact
+zeros, so screened builds must reproduce dense builds bit for bit in every
+precision, and the FP64 oracle (which applies the same cut-off per point)
+to the FP64 bar.  The algorithmic work reported by pgxc_last_stats is checked
+against n_b counted here from the same cut-off rule."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from pgxc import XCBuilder, basis, grid, systems
+from pgxc.xc import BF16X3, FP32, FP64, PBE
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+
+
+def separated_waters(gap_bohr=40.0):
+    """Two waters gap_bohr apart: most basis blocks are cut off on most of
+    the other molecule's points."""
+    e1, x1 = systems.h2o()
+    e2, x2 = systems.h2o(center=(gap_bohr, 0.0, 0.0))
+    els = list(e1) + list(e2)
+    xyz = np.vstack([x1, x2])
+    b = basis.seeded_basis(els, xyz, seed=21, shell_set="small")
+    g = grid.molecular_grid(els, xyz, nrad=50, nang=302)
+    C, occ = systems.random_orbitals(b.nbas, 10, seed=22)
+    return els, xyz, b, g, C, occ
+
+
+@pytest.fixture(scope="module")
+def dimer():
+    return separated_waters()
+
+
+def morton_perm(P):
+    """The library's point order (Morton curve over the bounding box, 10 bits
+    per axis, ties in input order)."""
+    lo, hi = P.min(0), P.max(0)
+    span = np.where(hi > lo, hi - lo, 1.0)
+    q = np.clip(np.floor((P - lo) / span * 1023.0), 0, 1023).astype(np.uint64)
+    code = np.zeros(len(P), dtype=np.uint64)
+    for bit in range(10):
+        for k in range(3):
+            code |= ((q[:, k] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + k)
+    return np.argsort(code, kind="stable")
+
+
+def screened_flops(b, g):
+    """sum over 128-point tiles of 4 B n_b^2 + 16 B n_b, n_b = functions of
+    the shells with alpha_min r^2 < 50 somewhere in a 32-point group of the tile."""
+    P = g[morton_perm(g[:, :3]), :3]
+    n = len(P)
+    f = 0.0
+    amin = np.array([s.exps.min() for s in b.shells])
+    nfun = np.array([s.nfunc for s in b.shells])
+    atom = np.array([s.atom for s in b.shells])
+    for t0 in range(0, n, 128):
+        T = P[t0:t0 + 128]
+        B = len(T)
+        sig = np.zeros(len(b.shells), bool)
+        for q0 in range(0, B, 32):
+            Q = T[q0:q0 + 32]
+            r2 = ((Q[:, None, :] - b.atom_xyz[None, :, :]) ** 2).sum(-1).min(0)
+            sig |= amin * r2[atom] < 50.0 * (1 + 1e-6)
+        nb = nfun[sig].sum()
+        f += 4.0 * B * nb * nb + 16.0 * B * nb
+    return f
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32, BF16X3])
+def test_screened_equals_dense_bitwise(dimer, prec):
+    _, _, b, g, C, occ = dimer
+    out = []
+    for screen in (True, False):
+        with XCBuilder(b, g, screen=screen) as xc:
+            xc.set_orbitals(C, occ)
+            out.append(xc.build(PBE, prec))
+    (h1, e1, q1), (h2, e2, q2) = out
+    assert np.array_equal(h1, h2)
+    assert e1 == e2 and q1 == q2
+
+
+def test_screened_incremental_equals_dense(dimer):
+    _, _, b, g, C, occ = dimer
+    C2 = systems.perturbed_orbitals(C, 1e-3, seed=23)
+    out = []
+    for screen in (True, False):
+        with XCBuilder(b, g, screen=screen) as xc:
+            xc.set_orbitals(C, occ)
+            xc.build(PBE, FP64, fetch=False)
+            xc.set_orbitals(C2, occ, incremental=True)
+            out.append(xc.build(PBE, BF16X3))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+
+
+def test_screened_fp64_matches_oracle(dimer):
+    _, _, b, g, C, occ = dimer
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        ham, e, q = xc.build(PBE, FP64)
+    hr, er, qr = oracle.xc_build(b, g, oracle.MASK_PBE, C=C, occ=occ, veclen=128, nthreads=NT)
+    print("dE=%.3e dV=%.3e" % (e - er, np.max(np.abs(ham - hr))), flush=True)
+    assert abs(e - er) <= 1e-9
+    assert np.max(np.abs(ham - hr)) <= 1e-10
+    assert abs(q - qr) <= 1e-9
+
+
+def test_screened_work_counts(dimer):
+    _, _, b, g, C, occ = dimer
+    n, npts = b.nbas, g.shape[0]
+    dense = 4.0 * npts * n * n + 16.0 * npts * n
+    expect = screened_flops(b, g)
+    for screen, ref in ((True, expect), (False, dense)):
+        with XCBuilder(b, g, screen=screen) as xc:
+            xc.set_orbitals(C, occ)
+            xc.build(PBE, FP64, fetch=False)
+            got = xc.stats()["flops"]
+        print("screen=%s flops %.6e expected %.6e" % (screen, got, ref), flush=True)
+        assert abs(got - ref) <= 2e-3 * ref  # r^2 rounding may flip a shell at the cut-off
+    assert expect < 0.75 * dense  # the cut-off removes a real share of the work here
