@@ -45,9 +45,10 @@ extern "C" {
 #define PGXC_XC_LDA (PGXC_XC_XALPHA | PGXC_XC_PWLDAC)
 #define PGXC_XC_PBE (PGXC_XC_XALPHA | PGXC_XC_PWLDAC | PGXC_XC_PBEX | PGXC_XC_PBEC)
 
-/* Arithmetic of the two contractions (Phi P and Phi^T W).  Basis values, the
- * functional, per-point state, E_xc/charge and V accumulation are FP64 in
- * every mode. */
+/* Arithmetic of the two contractions (Phi P and Phi^T W).  The functional,
+ * per-point state, E_xc/charge and V accumulation are FP64 in every mode;
+ * basis values are evaluated and stored in FP64 for FP64 builds and in FP32
+ * for both mixed modes (centre offsets formed in FP64 first). */
 #define PGXC_PREC_FP64 0   /* FP64 MFMA                                     */
 #define PGXC_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation                   */
 #define PGXC_PREC_BF16X3 2 /* split-BF16 (hi*hi + hi*lo + lo*hi), FP32 acc.  */
@@ -56,8 +57,22 @@ extern "C" {
 #define PGXC_DENSITY_FULL 0        /* next build integrates P itself          */
 #define PGXC_DENSITY_INCREMENTAL 1 /* next build integrates dP = P - P_prev   */
 
+/* Schedules of the two contraction stages (pgxc_set_schedule,
+ * pgxc_batch_set_schedule).  They change which workgroup computes which work
+ * item and when, never a result bit. */
+#define PGXC_SCHED_STATIC 0   /* fixed round-robin walk (the default)         */
+#define PGXC_SCHED_ANALYTIC 1 /* queues balanced by counted flops per item     */
+#define PGXC_SCHED_MODEL 2    /* queues balanced by the cost model's prediction */
+
+/* Features of one work item, in this order (pgxc_item_log, cost model):
+ * kind (0 density, 1 potential contraction), precision one-hot (FP64, FP32,
+ * BF16x3), function tile / 128, K length left after screening / 1024, dense
+ * K length / 1024, 1 if any block is left after screening, variant / 2. */
+#define PGXC_NFEATURES 9
+
 typedef struct pgxc_ctx pgxc_ctx;
 typedef struct pgxc_batch pgxc_batch;
+typedef struct pgxc_costmodel pgxc_costmodel;
 
 /* Layout options of a context (pgxc_create_opt).  tile: function tile of the
  * MFMA contractions, 64, 96 or 128 (0: the one padding nbas least).
@@ -208,6 +223,64 @@ int pgxc_batch_last_ms(const pgxc_batch *batch, double *ms);
 const char *pgxc_batch_last_error(const pgxc_batch *batch);
 
 int pgxc_batch_destroy(pgxc_batch *batch);
+
+/*
+ * Work distribution and the learned cost model.
+ *
+ * The reference hands each MPI rank a static contiguous share of the grid
+ * batches (distribute_jobs, dlb/dlb_common.f90:516-561, through dlb_setup /
+ * dlb_give_more, dlb/dlb.f90:203-224,290-311).  Here the work items of a
+ * contraction launch (point tile x function tile, or point split x function
+ * tile pair) are either walked round-robin (PGXC_SCHED_STATIC), or cut into
+ * one queue per XCD of about equal predicted cost and drained dynamically
+ * (PGXC_SCHED_ANALYTIC: counted flops left after screening;
+ * PGXC_SCHED_MODEL: the cost model).  The model is a 3-layer MLP
+ * (PGXC_NFEATURES -> 64 -> 64 -> 1, tanh) predicting log(item time / us),
+ * trained on the GPU from the per-item timings this library logs.
+ */
+int pgxc_set_schedule(pgxc_ctx *ctx, int32_t mode, pgxc_costmodel *model);
+int pgxc_batch_set_schedule(pgxc_batch *batch, int32_t mode, pgxc_costmodel *model);
+
+/* on = 1: later builds record, per contraction work item, its features and
+ * the device time its workgroup spent on it. */
+int pgxc_set_item_log(pgxc_ctx *ctx, int32_t on);
+int pgxc_batch_set_item_log(pgxc_batch *batch, int32_t on);
+
+/* Items of the last logged build: returns their number n (negative on
+ * error); when cap >= n, features (n x PGXC_NFEATURES, row-major) and us
+ * (n) are filled. */
+int64_t pgxc_item_log(pgxc_ctx *ctx, double *features, double *us, int64_t cap);
+int64_t pgxc_batch_item_log(pgxc_batch *batch, double *features, double *us, int64_t cap);
+
+/* Cost model with n_features inputs, Glorot-uniform weights from seed. */
+int pgxc_costmodel_create(pgxc_costmodel **model, int32_t n_features, uint64_t seed);
+
+/* Number of parameters: F*64 + 64 + 64*64 + 64 + 64 + 1, flat in the order
+ * W1[F][64], b1, W2[64][64], b2, w3, b3. */
+int32_t pgxc_costmodel_nparams(const pgxc_costmodel *model);
+
+/* Parameters (nparams) and, if norm is not NULL, the standardisation
+ * (2 F + 2: feature means, feature scales, target mean, target scale). */
+int pgxc_costmodel_params(pgxc_costmodel *model, double *params, double *norm);
+int pgxc_costmodel_set_params(pgxc_costmodel *model, const double *params, const double *norm);
+
+/* steps full-batch Adam steps (learning rate lr) on the mean squared error
+ * of the standardised targets y (n) from features X (n x F, row-major); the
+ * first training call fixes the standardisation.  loss (optional): the final
+ * loss. */
+int pgxc_costmodel_train(pgxc_costmodel *model, const double *X, const double *y, int64_t n,
+                         int32_t steps, double lr, double *loss);
+
+/* Gradient (nparams) of that loss at the current parameters, and the loss. */
+int pgxc_costmodel_gradient(pgxc_costmodel *model, const double *X, const double *y, int64_t n,
+                            double *grad, double *loss);
+
+/* Predictions (n, in the target's units) for features X (n x F). */
+int pgxc_costmodel_predict(pgxc_costmodel *model, const double *X, int64_t n, double *out);
+
+const char *pgxc_costmodel_last_error(const pgxc_costmodel *model);
+
+int pgxc_costmodel_destroy(pgxc_costmodel *model);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,13 @@ struct Plan;
 
 long round_up(long a, long b) { return (a + b - 1) / b * b; }
 
+// schedule of the contraction stages (PGXC_SCHED_*) and item logging
+struct Sched {
+  int mode = PGXC_SCHED_STATIC;
+  pgxc_costmodel *model = nullptr;
+  int log = 0;
+};
+
 void dfree(void *p) {
   if (p) (void)hipFree(p);
 }
@@ -78,6 +85,9 @@ struct pgxc_ctx {
   uint8_t *gsig = nullptr;
   ScreenShell *sshells = nullptr;
   double flops_screened = 0, flops_dense = 0;  // algorithmic work of one build
+  std::vector<int> h_kcount;     // host copies of the screening tables
+  std::vector<uint8_t> h_gsig;
+  Sched sched;
 };
 
 namespace {
@@ -124,6 +134,16 @@ struct Plan {
   int precision = 0;
   std::vector<int> incr;  // per molecule
   long order_version = -1;
+  int sched_mode = 0, log = 0;
+  const pgxc_costmodel *model = nullptr;
+  long model_version = -1;
+  int *d_ctr = nullptr;  // queue counters of every contraction list
+  long nctr = 0;
+  // item log: features of every contraction item (row-major), and per list
+  // the device log and the list's first feature row
+  std::vector<double> feat;
+  std::vector<std::pair<uint32_t *, long>> logs;  // (device ticks, items)
+  double tick_us = 0.0;
   std::vector<StepPlan> steps;
   PackD *d_pack = nullptr;
   long *d_pack_base = nullptr, *d_prep_base = nullptr;
@@ -185,12 +205,72 @@ long unit_items(const UnitD &U, ItemKind k, int tf) {
 
 }  // namespace
 
+// Features (PGXC_NFEATURES) and counted flops of the contraction items of
+// one unit, in the items' natural order.  kact: K length left after
+// screening; kdense: the dense K length.
+static void unit_item_features(const pgxc_ctx *c, int step, ItemKind kind, int precision,
+                               std::vector<double> &feat, std::vector<double> &flops) {
+  const int tf = c->tf, ntn = c->NP / tf;
+  const int var = precision == PGXC_PREC_FP64 ? c->var[0] : precision == PGXC_PREC_FP32 ? c->var[1] : 0;
+  const long p0 = (long)step * c->Mc;
+  auto push = [&](int kd, double kact, double kdense, bool active, double fl) {
+    const double f[PGXC_NFEATURES] = {(double)kd,
+                                      precision == PGXC_PREC_FP64 ? 1.0 : 0.0,
+                                      precision == PGXC_PREC_FP32 ? 1.0 : 0.0,
+                                      precision == PGXC_PREC_BF16X3 ? 1.0 : 0.0,
+                                      tf / 128.0,
+                                      kact / 1024.0,
+                                      kdense / 1024.0,
+                                      active ? 1.0 : 0.0,
+                                      var / 2.0};
+    feat.insert(feat.end(), f, f + PGXC_NFEATURES);
+    flops.push_back(fl);
+  };
+  const bool scr = c->screen != 0;
+  if (kind == K_DENS) {
+    const long ntile = c->Mc / kScreenTile;
+    for (long t = 0; t < ntile; ++t) {
+      const long T = p0 / kScreenTile + t;
+      for (int tn = 0; tn < ntn; ++tn) {
+        bool active = true;
+        double kact = c->NP;
+        if (scr) {
+          const uint8_t *g = c->h_gsig.data() + (size_t)tn * c->gstride + T * (kScreenTile / kGroup);
+          active = (g[0] | g[1] | g[2] | g[3]) != 0;
+          kact = active ? (double)c->h_kcount[(size_t)T] * kGroup : 0.0;
+        }
+        push(0, kact, c->NP, active, 2.0 * kScreenTile * tf * kact);
+      }
+    }
+  } else {
+    const long spts = c->Mc / c->nsplit, G = spts / kGroup;
+    for (int sl = 0; sl < c->nsplit; ++sl) {
+      const long g0 = p0 / kGroup + sl * G;
+      for (int ti = 0; ti < ntn; ++ti)
+        for (int tj = 0; tj < ntn; ++tj) {
+          double kact = (double)spts;
+          if (scr) {
+            const uint8_t *a = c->h_gsig.data() + (size_t)ti * c->gstride + g0;
+            const uint8_t *b = c->h_gsig.data() + (size_t)tj * c->gstride + g0;
+            long cnt = 0;
+            for (long g = 0; g < G; ++g) cnt += (a[g] & b[g]) != 0;
+            kact = (double)cnt * kGroup;
+          }
+          push(1, kact, (double)spts, kact > 0, 2.0 * tf * tf * kact);
+        }
+    }
+  }
+}
+
 // Build (or reuse) the plan of a set of molecules for one configuration.
 static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int precision,
                     const std::vector<int> &incr, const std::vector<double> &unit_prio,
-                    long order_version, Plan *&plan, pgxc_ctx *errc) {
+                    long order_version, const Sched &sc, Plan *&plan, pgxc_ctx *errc) {
+  const int smode = sc.model ? sc.mode : std::min(sc.mode, (int)PGXC_SCHED_ANALYTIC);
   if (plan && plan->functional == functional && plan->precision == precision &&
-      plan->incr == incr && plan->order_version == order_version)
+      plan->incr == incr && plan->order_version == order_version && plan->sched_mode == smode &&
+      plan->log == sc.log && plan->model == sc.model &&
+      plan->model_version == costmodel_version(sc.model))
     return PGXC_OK;
   free_plan(plan);
   plan = new Plan();
@@ -198,6 +278,18 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   plan->precision = precision;
   plan->incr = incr;
   plan->order_version = order_version;
+  plan->sched_mode = smode;
+  plan->log = sc.log;
+  plan->model = sc.model;
+  plan->model_version = costmodel_version(sc.model);
+  {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, errc->device) != hipSuccess ||
+        khz <= 0)
+      khz = 100000;
+    plan->tick_us = 1000.0 / khz;
+  }
+  const bool queues = smode != PGXC_SCHED_STATIC;
   const int nm = (int)mols.size();
   plan->nmol = nm;
   int nsteps = 0;
@@ -205,20 +297,23 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
 
   // host image of everything that goes to the device
   std::vector<char> img;
-  auto put = [&](const void *src, size_t bytes) {
+  auto put = [&](const void *src, size_t bytes) {  // src == nullptr: zeros
     const size_t off = (size_t)round_up((long)img.size(), 256);
     img.resize(off + std::max<size_t>(bytes, 1));
-    if (bytes) memcpy(img.data() + off, src, bytes);
+    if (bytes && src) memcpy(img.data() + off, src, bytes);
     return off;
   };
   struct Pending {
-    size_t units_off = 0, base_off = 0, order_off = 0;
+    size_t units_off = 0, base_off = 0, order_off = 0, qbeg_off = 0, ctr_off = 0, tlog_off = 0;
     int nu = 0;
     long total = 0;
-    bool has_order = false;
+    bool has_order = false, has_queues = false, has_log = false;
+    size_t frow = 0;  // first feature row of the list's items
   };
+  std::vector<double> flops;  // counted flops per feature row
+  size_t plan_ctr_off_tmp = 0;
   auto stage = [&](const std::vector<UnitD> &us, const std::vector<int> &um, ItemKind kind,
-                   int tf) {
+                   int tf, int step) {
     Pending P;
     P.nu = (int)us.size();
     std::vector<long> base(us.size() + 1, 0);
@@ -226,13 +321,26 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     P.total = base.back();
     P.units_off = put(us.data(), sizeof(UnitD) * us.size());
     P.base_off = put(base.data(), sizeof(long) * base.size());
-    std::vector<double> pr;
-    if (!unit_prio.empty())
-      for (int m : um) pr.push_back(unit_prio[m]);
-    std::vector<int> order;
-    make_order(base, pr, order);
-    P.has_order = !order.empty();
-    if (P.has_order) P.order_off = put(order.data(), sizeof(int) * order.size());
+    const bool contr = kind == K_DENS || kind == K_POT;
+    if (contr && (queues || sc.log) && P.total > 0) {
+      P.frow = plan->feat.size() / PGXC_NFEATURES;
+      for (int m : um) unit_item_features(mols[m], step, kind, precision, plan->feat, flops);
+    }
+    if (contr && queues && P.total > 0) {
+      P.has_queues = true;  // queue bounds filled in once every cost is known
+    } else {
+      std::vector<double> pr;
+      if (!unit_prio.empty())
+        for (int m : um) pr.push_back(unit_prio[m]);
+      std::vector<int> order;
+      make_order(base, pr, order);
+      P.has_order = !order.empty();
+      if (P.has_order) P.order_off = put(order.data(), sizeof(int) * order.size());
+    }
+    if (contr && sc.log && P.total > 0) {
+      P.has_log = true;
+      P.tlog_off = put(nullptr, sizeof(uint32_t) * P.total);
+    }
     return P;
   };
   std::vector<std::vector<Pending>> pend(nsteps);  // basis, func, wmat, dens, pot
@@ -282,9 +390,9 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       units.push_back(U);
       umol.push_back(m);
     }
-    pend[k].push_back(stage(units, umol, K_BASIS, 0));
-    pend[k].push_back(stage(units, umol, K_FUNC, 0));
-    pend[k].push_back(stage(units, umol, K_WMAT, 0));
+    pend[k].push_back(stage(units, umol, K_BASIS, 0, k));
+    pend[k].push_back(stage(units, umol, K_FUNC, 0, k));
+    pend[k].push_back(stage(units, umol, K_WMAT, 0, k));
     const int cls = precision == PGXC_PREC_FP64 ? 0 : 1;
     for (int kind = 0; kind < 2; ++kind)
       for (int li = 0; li < kLists; ++li) {
@@ -298,8 +406,49 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
             subm.push_back(umol[i]);
           }
         }
-        pend[k].push_back(stage(sub, subm, kind == 0 ? K_DENS : K_POT, kTiles[t]));
+        pend[k].push_back(stage(sub, subm, kind == 0 ? K_DENS : K_POT, kTiles[t], k));
       }
+  }
+  // queue bounds: each list's positions (natural order, so neighbouring items
+  // keep sharing operand panels) cut into kQueues runs of equal predicted cost
+  {
+    std::vector<double> cost = flops;
+    const size_t nrow = flops.size();
+    if (smode == PGXC_SCHED_MODEL && nrow) {
+      if (costmodel_features(sc.model) != PGXC_NFEATURES)
+        return fail(errc, PGXC_ERR_ARG, "cost model does not take PGXC_NFEATURES features");
+      std::vector<double> pred(nrow);
+      int rc = pgxc_costmodel_predict(sc.model, plan->feat.data(), (int64_t)nrow, pred.data());
+      if (rc) return fail(errc, rc, std::string("cost model: ") + pgxc_costmodel_last_error(sc.model));
+      for (size_t i = 0; i < nrow; ++i) cost[i] = std::exp(std::min(pred[i], 50.0));
+    } else {
+      for (size_t i = 0; i < nrow; ++i) cost[i] = flops[i] + 1.0;
+    }
+    long nq = 0;
+    for (auto &pk : pend)
+      for (auto &P : pk) nq += P.has_queues;
+    std::vector<int> zeros((size_t)std::max<long>(1, nq * kQueues), 0);
+    const size_t ctr0 = put(zeros.data(), sizeof(int) * zeros.size());
+    plan->nctr = nq * kQueues;
+    long qi = 0;
+    for (auto &pk : pend)
+      for (auto &P : pk) {
+        if (!P.has_queues) continue;
+        double tot = 0.0;
+        for (long i = 0; i < P.total; ++i) tot += cost[P.frow + i];
+        long qb[kQueues + 1];
+        qb[0] = 0;
+        double acc = 0.0;
+        int q = 1;
+        for (long i = 0; i < P.total && q < kQueues; ++i) {
+          acc += cost[P.frow + i];
+          while (q < kQueues && acc >= tot * q / kQueues) qb[q++] = i + 1;
+        }
+        while (q <= kQueues) qb[q++] = P.total;
+        P.qbeg_off = put(qb, sizeof(qb));
+        P.ctr_off = ctr0 + sizeof(int) * kQueues * qi++;
+      }
+    plan_ctr_off_tmp = ctr0;
   }
   // per-molecule operand preparation and packing
   std::vector<PackD> packs(nm);
@@ -327,13 +476,18 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   HIPCHK(errc, hipMalloc((void **)&plan->blob, std::max<size_t>(img.size(), 256)));
   HIPCHK(errc, hipMemcpy(plan->blob, img.data(), img.size(), hipMemcpyHostToDevice));
   char *B = plan->blob;
+  plan->d_ctr = (int *)(B + plan_ctr_off_tmp);
   auto mk = [&](const Pending &P) {
-    StageList L;
+    StageList L{};
     L.units = (const UnitD *)(B + P.units_off);
     L.nu = P.nu;
     L.base = (const long *)(B + P.base_off);
     L.order = P.has_order ? (const int *)(B + P.order_off) : nullptr;
     L.total = P.total;
+    L.ctr = P.has_queues ? (int *)(B + P.ctr_off) : nullptr;
+    L.qbeg = P.has_queues ? (const long *)(B + P.qbeg_off) : nullptr;
+    L.tlog = P.has_log ? (uint32_t *)(B + P.tlog_off) : nullptr;
+    if (P.has_log) plan->logs.push_back({L.tlog, P.total});
     return L;
   };
   plan->steps.resize(nsteps);
@@ -354,6 +508,32 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   return PGXC_OK;
 }
 
+// Item log of a plan's last build: features and microseconds per item.
+static int64_t read_item_log(const Plan *pl, hipStream_t s, double *features, double *us,
+                             int64_t cap) {
+  if (!pl || !pl->log) return 0;
+  int64_t n = 0;
+  for (auto &lg : pl->logs) n += lg.second;
+  if (!features || !us || cap < n) return n;
+  if (hipStreamSynchronize(s) != hipSuccess) return -PGXC_ERR_HIP;
+  int64_t row = 0;
+  std::vector<uint32_t> t;
+  for (auto &lg : pl->logs) {
+    t.resize((size_t)lg.second);
+    if (hipMemcpy(t.data(), lg.first, sizeof(uint32_t) * lg.second, hipMemcpyDeviceToHost) != hipSuccess)
+      return -PGXC_ERR_HIP;
+    for (long i = 0; i < lg.second; ++i, ++row) us[row] = t[(size_t)i] * pl->tick_us;
+  }
+  memcpy(features, pl->feat.data(), sizeof(double) * PGXC_NFEATURES * n);
+  return n;
+}
+
+static int check_sched(int32_t mode, const pgxc_costmodel *model) {
+  if (mode < PGXC_SCHED_STATIC || mode > PGXC_SCHED_MODEL) return PGXC_ERR_ARG;
+  if (mode == PGXC_SCHED_MODEL && !model) return PGXC_ERR_ARG;
+  return PGXC_OK;
+}
+
 // Events of a timed build: kEvStart / kEvPack / kEvEnd bracket the build and
 // its packing; step k of the plan records five more from kEvStep + 5 k on
 // (before basis, after basis, after the density contraction, after the
@@ -365,6 +545,7 @@ constexpr int kEvStart = 0, kEvPack = 1, kEvEnd = 2, kEvStep = 3;
 template <typename T>
 static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hipEvent_t> *ev) {
   const bool x3 = pl.precision == PGXC_PREC_BF16X3;
+  if (pl.nctr) launch_zero_ints(s, pl.d_ctr, pl.nctr);
   launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total,
               !std::is_same<T, double>::value, false);
   for (size_t k = 0; k < pl.steps.size(); ++k) {
@@ -546,6 +727,12 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   HIPCHK(c, hipGetLastError());
   std::vector<int> nb((size_t)ntp);
   HIPCHK(c, hipMemcpyAsync(nb.data(), c->nbt, sizeof(int) * ntp, hipMemcpyDeviceToHost, c->stream));
+  c->h_kcount.resize((size_t)ntp);
+  c->h_gsig.resize((size_t)c->ntiles * c->gstride);
+  HIPCHK(c, hipMemcpyAsync(c->h_kcount.data(), c->kcount, sizeof(int) * ntp, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->h_gsig.data(), c->gsig, c->h_gsig.size(), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double f = 0.0;
   for (long t = 0; t < ntp; ++t) {
@@ -809,7 +996,7 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
   if ((rc = ensure_work(c, elem, s))) return rc;
   std::vector<pgxc_ctx *> mols{c};
   std::vector<int> incrv{incr};
-  if ((rc = get_plan(mols, functional, precision, incrv, {}, -1, c->plan, c))) return rc;
+  if ((rc = get_plan(mols, functional, precision, incrv, {}, -1, c->sched, c->plan, c))) return rc;
   const size_t nev = kEvStep + 5 * c->plan->steps.size();
   while (c->ev.size() < nev) {
     hipEvent_t e = nullptr;
@@ -866,6 +1053,26 @@ int pgxc_last_timing(const pgxc_ctx *c, double *ms6) {
   return PGXC_OK;
 }
 
+int pgxc_set_schedule(pgxc_ctx *c, int32_t mode, pgxc_costmodel *model) {
+  if (!c) return PGXC_ERR_ARG;
+  if (check_sched(mode, model))
+    return fail(c, PGXC_ERR_ARG, "pgxc_set_schedule: mode 0..2, and a model for PGXC_SCHED_MODEL");
+  c->sched.mode = mode;
+  c->sched.model = mode == PGXC_SCHED_MODEL ? model : nullptr;
+  return PGXC_OK;
+}
+
+int pgxc_set_item_log(pgxc_ctx *c, int32_t on) {
+  if (!c) return PGXC_ERR_ARG;
+  c->sched.log = on ? 1 : 0;
+  return PGXC_OK;
+}
+
+int64_t pgxc_item_log(pgxc_ctx *c, double *features, double *us, int64_t cap) {
+  if (!c) return -PGXC_ERR_ARG;
+  return read_item_log(c->plan, c->stream, features, us, cap);
+}
+
 int pgxc_destroy(pgxc_ctx *c) {
   if (!c) return PGXC_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -894,6 +1101,7 @@ struct pgxc_batch {
   long order_version = -1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double last_ms = 0;
+  Sched sched;
 };
 
 extern "C" {
@@ -968,8 +1176,8 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
     moved = moved || phi0 != c->phi || w0 != c->W;
   }
   if (moved) free_plan(b->plan);
-  int rc = get_plan(b->mols, functional, precision, incr, b->prio, b->order_version, b->plan,
-                    b->mols[0]);
+  int rc = get_plan(b->mols, functional, precision, incr, b->prio, b->order_version, b->sched,
+                    b->plan, b->mols[0]);
   if (rc) {
     b->err = b->mols[0]->err;
     return rc;
@@ -1021,6 +1229,28 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
     c->timing[5] = t;
   }
   return PGXC_OK;
+}
+
+int pgxc_batch_set_schedule(pgxc_batch *b, int32_t mode, pgxc_costmodel *model) {
+  if (!b) return PGXC_ERR_ARG;
+  if (check_sched(mode, model)) {
+    b->err = "pgxc_batch_set_schedule: mode 0..2, and a model for PGXC_SCHED_MODEL";
+    return PGXC_ERR_ARG;
+  }
+  b->sched.mode = mode;
+  b->sched.model = mode == PGXC_SCHED_MODEL ? model : nullptr;
+  return PGXC_OK;
+}
+
+int pgxc_batch_set_item_log(pgxc_batch *b, int32_t on) {
+  if (!b) return PGXC_ERR_ARG;
+  b->sched.log = on ? 1 : 0;
+  return PGXC_OK;
+}
+
+int64_t pgxc_batch_item_log(pgxc_batch *b, double *features, double *us, int64_t cap) {
+  if (!b) return -PGXC_ERR_ARG;
+  return read_item_log(b->plan, b->stream, features, us, cap);
 }
 
 int pgxc_batch_last_ms(const pgxc_batch *b, double *ms) {

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct pgxc_costmodel;
+
 namespace pgxc {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -69,12 +71,21 @@ struct PackD {
 };
 
 // One launch's items: prefix offsets base[0..nu] over units, optional order.
+// Queues (ctr != nullptr, contraction stages only): positions are split into
+// kQueues contiguous ranges qbeg[q] .. qbeg[q+1]-1 of about equal predicted
+// cost; workgroup b drains queue b % kQueues (the XCD it runs on), taking the
+// next position from the counter ctr[q].  tlog (optional): per item, the
+// workgroup's wall-clock ticks spent on it.
+constexpr int kQueues = 8;
 struct StageList {
   const UnitD *units;
   int nu;
   const long *base;
   const int *order;
   long total;
+  int *ctr;
+  const long *qbeg;
+  uint32_t *tlog;
 };
 
 // ---- launchers (kernels live in xc_kernels.hip / xc_bf16x3.hip) -----------
@@ -93,6 +104,12 @@ void launch_potential(hipStream_t s, int tf, int var, const StageList &L);
 void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L);
 void launch_potential_bf16x3(hipStream_t s, int tf, const StageList &L);
 void launch_pack(hipStream_t s, const PackD *mols, int nm, const long *base, long total);
+void launch_zero_ints(hipStream_t s, int *p, long n);
+unsigned contraction_grid(const StageList &L, int per_cu);
+
+// cost model internals used by the planner (cost_model.hip)
+long costmodel_version(const pgxc_costmodel *m);
+int costmodel_features(const pgxc_costmodel *m);
 void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, long total,
                  bool to_float, bool finish);
 

@@ -105,7 +105,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, lo
                                                  const float *__restrict__ B, long sB,
                                                  int ksteps, KMap kmap, __bf16 *lds,
                                                  f4 (&acc)[BM / 32][BN / 32]) {
-  constexpr int BK = kXBK, BKP = kXBKP, WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+  constexpr int BKP = kXBKP, WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int SA = BM * BKP, SB = BN * BKP, STAGE = 2 * SA + 2 * SB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -171,26 +171,22 @@ __device__ __forceinline__ int find_unit_x3(const long *__restrict__ base, int n
   return lo;
 }
 
-__device__ __forceinline__ long item_at_x3(const int *__restrict__ order, long L, long total) {
-  return order ? (long)order[L] : (long)xcd_remap((int)L, (int)total);
-}
 
 // item = (128-point tile, function tile), as k_density
 template <int BN, bool GRAD>
-__global__ __launch_bounds__(kThreads) void k_density_bf16x3(const UnitD *__restrict__ units, int nu,
-                                                             const long *__restrict__ base,
-                                                             const int *__restrict__ order,
-                                                             long total) {
+__global__ __launch_bounds__(kThreads) void k_density_bf16x3(const StageList S) {
   constexpr int BM = kXBM;
   __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BM, BN>::n];
   __shared__ double red[2][4][BM];
   __shared__ uint16_t kl[kMaxBlocks];
+  __shared__ long slot;
   static_assert(kXBK == kGroup, "one K step per 32-function block");
-  for (long L = blockIdx.x; L < total; L += gridDim.x) {
-    const long it = item_at_x3(order, L, total);
-    const int u = find_unit_x3(base, nu, it);
-    const UnitD &U = units[u];
-    const long local = it - base[u];
+  for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
+    const uint64_t t0 = S.tlog ? wall_clock64() : 0;
+    const long it = walk_item(S, L);
+    const int u = find_unit_x3(S.base, S.nu, it);
+    const UnitD &U = S.units[u];
+    const long local = it - S.base[u];
     const int ntn = U.NP / BN;
     const long m0 = (local / ntn) * BM;
     const int tn = (int)(local % ntn), n0 = tn * BN;
@@ -201,6 +197,7 @@ __global__ __launch_bounds__(kThreads) void k_density_bf16x3(const UnitD *__rest
       ksteps = density_screen(U, m0, tn, kl);
       if (ksteps < 0) {
         density_zero<BM, GRAD>(U.part, U.Mc, m0, tn);
+        item_log(S, it, t0);
         continue;
       }
       scr = ksteps < U.nfb;  // a complete list is the dense walk
@@ -209,21 +206,18 @@ __global__ __launch_bounds__(kThreads) void k_density_bf16x3(const UnitD *__rest
     f4 acc[BM / 32][BN / 32];
     gemm_tile_bf16x3<BM, BN, false, false>(phi + m0, U.Mc, D + n0, U.NP, ksteps, kmap, lds, acc);
     density_epilogue<float, f4, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn, U.part, red);
+    item_log(S, it, t0);
   }
 }
 
-static unsigned grid_x3(long total) {
-  const long cap = 256L * 2;
-  return (unsigned)(total < cap ? (total > 0 ? total : 1) : cap);
-}
 
 template <int BN>
 static void density_x3(hipStream_t s, bool grad, const StageList &L) {
-  dim3 g(grid_x3(L.total));
+  dim3 g(contraction_grid(L, 2));
   if (grad)
-    hipLaunchKernelGGL((k_density_bf16x3<BN, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+    hipLaunchKernelGGL((k_density_bf16x3<BN, true>), g, dim3(kThreads), 0, s, L);
   else
-    hipLaunchKernelGGL((k_density_bf16x3<BN, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+    hipLaunchKernelGGL((k_density_bf16x3<BN, false>), g, dim3(kThreads), 0, s, L);
 }
 
 void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L) {
@@ -237,18 +231,17 @@ void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L)
 
 // item = (point split, function tile pair), as k_potential
 template <int BT>
-__global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const UnitD *__restrict__ units, int nu,
-                                                               const long *__restrict__ base,
-                                                               const int *__restrict__ order,
-                                                               long total) {
+__global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S) {
   __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BT, BT>::n];
   __shared__ uint16_t gl[kMaxSplitGroups];
   __shared__ int wtot[4];
-  for (long L = blockIdx.x; L < total; L += gridDim.x) {
-    const long it = item_at_x3(order, L, total);
-    const int u = find_unit_x3(base, nu, it);
-    const UnitD &U = units[u];
-    const long local = it - base[u];
+  __shared__ long slot;
+  for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
+    const uint64_t t0 = S.tlog ? wall_clock64() : 0;
+    const long it = walk_item(S, L);
+    const int u = find_unit_x3(S.base, S.nu, it);
+    const UnitD &U = S.units[u];
+    const long local = it - S.base[u];
     const int nt = U.NP / BT, nt2 = nt * nt;
     const int sl = (int)(local / nt2), tile = (int)(local % nt2);
     const int i0 = (tile / nt) * BT, j0 = (tile % nt) * BT;
@@ -264,16 +257,17 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const UnitD *__re
                                          ksteps, kmap, lds, acc);
     potential_epilogue<float, f4, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP, U.NP, i0, j0,
                                           U.first);
+    item_log(S, it, t0);
   }
 }
 
 void launch_potential_bf16x3(hipStream_t s, int tf, const StageList &L) {
   if (!L.total) return;
-  dim3 g(grid_x3(L.total));
+  dim3 g(contraction_grid(L, 2));
   switch (tf) {
-    case 64: hipLaunchKernelGGL((k_potential_bf16x3<64>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
-    case 96: hipLaunchKernelGGL((k_potential_bf16x3<96>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
-    default: hipLaunchKernelGGL((k_potential_bf16x3<128>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
+    case 64: hipLaunchKernelGGL((k_potential_bf16x3<64>), g, dim3(kThreads), 0, s, L); break;
+    case 96: hipLaunchKernelGGL((k_potential_bf16x3<96>), g, dim3(kThreads), 0, s, L); break;
+    default: hipLaunchKernelGGL((k_potential_bf16x3<128>), g, dim3(kThreads), 0, s, L); break;
   }
 }
 

@@ -16,6 +16,36 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return (bid & 7) * per + (bid >> 3);
 }
 
+// Walk of a stage list's positions.  Static: blockIdx.x, + gridDim.x, ...
+// With queues: the next position of queue blockIdx.x % kQueues, or total
+// when that queue is drained.  Every thread of the workgroup calls it.
+__device__ __forceinline__ long walk_next(const StageList &S, long prev, long *slot) {
+  if (!S.ctr) return prev < 0 ? (long)blockIdx.x : prev + (long)gridDim.x;
+  const int q = (int)(blockIdx.x % kQueues);
+  if (threadIdx.x == 0) {
+    const long p = S.qbeg[q] + (long)atomicAdd(S.ctr + q, 1);
+    *slot = p < S.qbeg[q + 1] ? p : S.total;
+  }
+  __syncthreads();
+  const long p = *slot;
+  __syncthreads();
+  return p;
+}
+
+// item at a position: the schedule's order, else (static walk) the
+// XCD-blocked order, else (queues) the natural order
+__device__ __forceinline__ long walk_item(const StageList &S, long pos) {
+  if (S.order) return (long)S.order[pos];
+  return S.ctr ? pos : (long)xcd_remap((int)pos, (int)S.total);
+}
+
+// per-item time log (wall clock ticks of the workgroup)
+__device__ __forceinline__ void item_log(const StageList &S, long it, uint64_t t0) {
+  if (!S.tlog) return;
+  __syncthreads();
+  if (threadIdx.x == 0) S.tlog[it] = (uint32_t)(wall_clock64() - t0);
+}
+
 template <typename T>
 struct MF;
 // 16x16x4 MFMA operand/result placement (wave64), both types:

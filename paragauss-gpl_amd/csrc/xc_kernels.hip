@@ -531,20 +531,20 @@ template <> struct Var<2> { static constexpr int BK = 32, ST = 2; };
 // item = (128-point tile, function tile) of X = Phi D, then the tile's
 // partial contractions per point
 template <typename T, int BN, int VAR, bool GRAD>
-__global__ __launch_bounds__(kThreads) void k_density(const UnitD *__restrict__ units, int nu,
-                                                      const long *__restrict__ base,
-                                                      const int *__restrict__ order, long total) {
+__global__ __launch_bounds__(kThreads) void k_density(const StageList S) {
   constexpr int BM = kDBM, BK = Var<VAR>::BK, ST = Var<VAR>::ST, TM = BM / 32, TN = BN / 32;
   constexpr int R = kGroup / BK;  // K steps per 32-function block
   __shared__ T As[ST * BK * BM];
   __shared__ T Bs[ST * BK * BN];
   __shared__ double red[2][4][BM];
   __shared__ uint16_t kl[kMaxBlocks];
-  for (long L = blockIdx.x; L < total; L += gridDim.x) {
-    const long it = item_at(order, L, total);
-    const int u = find_unit(base, nu, it);
-    const UnitD &U = units[u];
-    const long local = it - base[u];
+  __shared__ long slot;
+  for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
+    const uint64_t t0 = S.tlog ? wall_clock64() : 0;
+    const long it = walk_item(S, L);
+    const int u = find_unit(S.base, S.nu, it);
+    const UnitD &U = S.units[u];
+    const long local = it - S.base[u];
     const int ntn = U.NP / BN;
     const long m0 = (local / ntn) * BM;
     const int tn = (int)(local % ntn), n0 = tn * BN;
@@ -555,6 +555,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const UnitD *__restrict__ 
       const int cnt = density_screen(U, m0, tn, kl);
       if (cnt < 0) {
         density_zero<BM, GRAD>(U.part, U.Mc, m0, tn);
+        item_log(S, it, t0);
         continue;
       }
       scr = cnt < U.nfb;  // a complete list is the dense walk
@@ -568,6 +569,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const UnitD *__restrict__ 
                                                acc);
     density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn,
                                                           U.part, red);
+    item_log(S, it, t0);
   }
 }
 
@@ -699,21 +701,20 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
 // --------------------------------------------------------------- potential
 // item = (point split, function tile pair) of H[split] (+)= W^T Phi
 template <typename T, int BT, int VAR>
-__global__ __launch_bounds__(kThreads) void k_potential(const UnitD *__restrict__ units, int nu,
-                                                        const long *__restrict__ base,
-                                                        const int *__restrict__ order,
-                                                        long total) {
+__global__ __launch_bounds__(kThreads) void k_potential(const StageList S) {
   constexpr int BK = Var<VAR>::BK, ST = Var<VAR>::ST, TT = BT / 32;
   constexpr int R = kGroup / BK;  // K steps per 32-point group
   __shared__ __attribute__((aligned(16))) T As[lds_elems<T, BT, BK, ST, true>()];
   __shared__ __attribute__((aligned(16))) T Bs[lds_elems<T, BT, BK, ST, true>()];
   __shared__ uint16_t gl[kMaxSplitGroups];
   __shared__ int wtot[4];
-  for (long L = blockIdx.x; L < total; L += gridDim.x) {
-    const long it = item_at(order, L, total);
-    const int u = find_unit(base, nu, it);
-    const UnitD &U = units[u];
-    const long local = it - base[u];
+  __shared__ long slot;
+  for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
+    const uint64_t t0 = S.tlog ? wall_clock64() : 0;
+    const long it = walk_item(S, L);
+    const int u = find_unit(S.base, S.nu, it);
+    const UnitD &U = S.units[u];
+    const long local = it - S.base[u];
     const int nt = U.NP / BT, nt2 = nt * nt;
     const int sl = (int)(local / nt2), tile = (int)(local % nt2);
     const int i0 = (tile / nt) * BT, j0 = (tile % nt) * BT;
@@ -731,6 +732,7 @@ __global__ __launch_bounds__(kThreads) void k_potential(const UnitD *__restrict_
                                              Mc, ksteps, kmap, As, Bs, acc);
     potential_epilogue<T, typename MF<T>::acc, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP,
                                                        U.NP, i0, j0, U.first);
+    item_log(S, it, t0);
   }
 }
 
@@ -789,6 +791,13 @@ static unsigned persistent_grid(long total, int per_cu) {
   return (unsigned)(total < cap ? (total > 0 ? total : 1) : cap);
 }
 
+// grid of a contraction launch: with queues, whole workgroups per queue
+unsigned contraction_grid(const StageList &L, int per_cu) {
+  long g = persistent_grid(L.total, per_cu);
+  if (L.ctr) g = std::max<long>(kQueues, (g + kQueues - 1) / kQueues * kQueues);
+  return (unsigned)g;
+}
+
 template <typename T>
 void launch_basis(hipStream_t s, bool grad, const StageList &L) {
   if (!L.total) return;
@@ -801,11 +810,11 @@ void launch_basis(hipStream_t s, bool grad, const StageList &L) {
 
 template <typename T, int BN, int VAR>
 static void density_bn(hipStream_t s, bool grad, const StageList &L) {
-  dim3 g(persistent_grid(L.total, 2));
+  dim3 g(contraction_grid(L, 2));
   if (grad)
-    hipLaunchKernelGGL((k_density<T, BN, VAR, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+    hipLaunchKernelGGL((k_density<T, BN, VAR, true>), g, dim3(kThreads), 0, s, L);
   else
-    hipLaunchKernelGGL((k_density<T, BN, VAR, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+    hipLaunchKernelGGL((k_density<T, BN, VAR, false>), g, dim3(kThreads), 0, s, L);
 }
 
 template <typename T, int VAR>
@@ -848,11 +857,11 @@ void launch_wmat(hipStream_t s, bool grad, const StageList &L) {
 
 template <typename T, int VAR>
 static void potential_var(hipStream_t s, int tf, const StageList &L) {
-  dim3 g(persistent_grid(L.total, 2));
+  dim3 g(contraction_grid(L, 2));
   switch (tf) {
-    case 64: hipLaunchKernelGGL((k_potential<T, 64, VAR>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
-    case 96: hipLaunchKernelGGL((k_potential<T, 96, VAR>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
-    default: hipLaunchKernelGGL((k_potential<T, 128, VAR>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total); break;
+    case 64: hipLaunchKernelGGL((k_potential<T, 64, VAR>), g, dim3(kThreads), 0, s, L); break;
+    case 96: hipLaunchKernelGGL((k_potential<T, 96, VAR>), g, dim3(kThreads), 0, s, L); break;
+    default: hipLaunchKernelGGL((k_potential<T, 128, VAR>), g, dim3(kThreads), 0, s, L); break;
   }
 }
 
@@ -864,6 +873,17 @@ void launch_potential(hipStream_t s, int tf, int var, const StageList &L) {
     case 2: potential_var<T, 2>(s, tf, L); break;
     default: potential_var<T, 0>(s, tf, L); break;
   }
+}
+
+__global__ __launch_bounds__(kThreads) void k_zero_ints(int *p, long n) {
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads)
+    p[i] = 0;
+}
+
+void launch_zero_ints(hipStream_t s, int *p, long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_ints, dim3(persistent_grid((n + kThreads - 1) / kThreads, 1)),
+                     dim3(kThreads), 0, s, p, n);
 }
 
 void launch_pack(hipStream_t s, const PackD *mols, int nm, const long *base, long total) {

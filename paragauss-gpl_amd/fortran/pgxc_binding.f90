@@ -27,6 +27,13 @@ module pgxc_binding
   public :: pgxc_becke_weights, pgxc_create_opt, pgxc_layout, pgxc_options
   public :: pgxc_batch_create, pgxc_batch_set_priority, pgxc_batch_build, pgxc_batch_last_ms
   public :: pgxc_batch_last_error, pgxc_batch_destroy
+  public :: pgxc_set_schedule, pgxc_batch_set_schedule, pgxc_set_item_log
+  public :: pgxc_batch_set_item_log, pgxc_item_log, pgxc_batch_item_log
+  public :: pgxc_costmodel_create, pgxc_costmodel_nparams, pgxc_costmodel_params
+  public :: pgxc_costmodel_set_params, pgxc_costmodel_train, pgxc_costmodel_gradient
+  public :: pgxc_costmodel_predict, pgxc_costmodel_last_error, pgxc_costmodel_destroy
+  integer(c_int32_t), parameter, public :: PGXC_SCHED_STATIC = 0, PGXC_SCHED_ANALYTIC = 1, &
+       PGXC_SCHED_MODEL = 2, PGXC_NFEATURES = 9
 
   type, bind(C) :: pgxc_options
      integer(c_int32_t) :: tile
@@ -169,5 +176,110 @@ module pgxc_binding
        import :: c_int, c_ptr
        type(c_ptr), value :: batch
      end function pgxc_batch_destroy
+     integer(c_int) function pgxc_set_schedule(ctx, mode, model) bind(C, name="pgxc_set_schedule")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx, model
+       integer(c_int32_t), value :: mode
+     end function pgxc_set_schedule
+
+     integer(c_int) function pgxc_batch_set_schedule(batch, mode, model) &
+          bind(C, name="pgxc_batch_set_schedule")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: batch, model
+       integer(c_int32_t), value :: mode
+     end function pgxc_batch_set_schedule
+
+     integer(c_int) function pgxc_set_item_log(ctx, on) bind(C, name="pgxc_set_item_log")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: on
+     end function pgxc_set_item_log
+
+     integer(c_int) function pgxc_batch_set_item_log(batch, on) bind(C, name="pgxc_batch_set_item_log")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: batch
+       integer(c_int32_t), value :: on
+     end function pgxc_batch_set_item_log
+
+     integer(c_int64_t) function pgxc_item_log(ctx, features, us, cap) bind(C, name="pgxc_item_log")
+       import :: c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: features(*), us(*)
+       integer(c_int64_t), value :: cap
+     end function pgxc_item_log
+
+     integer(c_int64_t) function pgxc_batch_item_log(batch, features, us, cap) &
+          bind(C, name="pgxc_batch_item_log")
+       import :: c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: batch
+       real(c_double), intent(out) :: features(*), us(*)
+       integer(c_int64_t), value :: cap
+     end function pgxc_batch_item_log
+
+     integer(c_int) function pgxc_costmodel_create(model, n_features, seed) &
+          bind(C, name="pgxc_costmodel_create")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       type(c_ptr), intent(out) :: model
+       integer(c_int32_t), value :: n_features
+       integer(c_int64_t), value :: seed
+     end function pgxc_costmodel_create
+
+     integer(c_int32_t) function pgxc_costmodel_nparams(model) bind(C, name="pgxc_costmodel_nparams")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: model
+     end function pgxc_costmodel_nparams
+
+     integer(c_int) function pgxc_costmodel_params(model, params, norm) &
+          bind(C, name="pgxc_costmodel_params")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: model
+       real(c_double), intent(out) :: params(*), norm(*)
+     end function pgxc_costmodel_params
+
+     integer(c_int) function pgxc_costmodel_set_params(model, params, norm) &
+          bind(C, name="pgxc_costmodel_set_params")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: model
+       real(c_double), intent(in) :: params(*), norm(*)
+     end function pgxc_costmodel_set_params
+
+     integer(c_int) function pgxc_costmodel_train(model, x, y, n, steps, lr, loss) &
+          bind(C, name="pgxc_costmodel_train")
+       import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: model
+       real(c_double), intent(in) :: x(*), y(*)
+       integer(c_int64_t), value :: n
+       integer(c_int32_t), value :: steps
+       real(c_double), value :: lr
+       real(c_double), intent(out) :: loss
+     end function pgxc_costmodel_train
+
+     integer(c_int) function pgxc_costmodel_gradient(model, x, y, n, grad, loss) &
+          bind(C, name="pgxc_costmodel_gradient")
+       import :: c_int, c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: model
+       real(c_double), intent(in) :: x(*), y(*)
+       integer(c_int64_t), value :: n
+       real(c_double), intent(out) :: grad(*), loss
+     end function pgxc_costmodel_gradient
+
+     integer(c_int) function pgxc_costmodel_predict(model, x, n, out) &
+          bind(C, name="pgxc_costmodel_predict")
+       import :: c_int, c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: model
+       real(c_double), intent(in) :: x(*)
+       integer(c_int64_t), value :: n
+       real(c_double), intent(out) :: out(*)
+     end function pgxc_costmodel_predict
+
+     type(c_ptr) function pgxc_costmodel_last_error(model) bind(C, name="pgxc_costmodel_last_error")
+       import :: c_ptr
+       type(c_ptr), value :: model
+     end function pgxc_costmodel_last_error
+
+     integer(c_int) function pgxc_costmodel_destroy(model) bind(C, name="pgxc_costmodel_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: model
+     end function pgxc_costmodel_destroy
   end interface
 end module pgxc_binding

@@ -18,7 +18,15 @@ EXPORTS = ("pgxc_create", "pgxc_create_opt", "pgxc_layout", "pgxc_set_density",
            "pgxc_set_orbitals", "pgxc_build", "pgxc_get_result", "pgxc_nbas", "pgxc_last_stats",
            "pgxc_last_timing", "pgxc_last_error", "pgxc_destroy", "pgxc_becke_weights",
            "pgxc_batch_create", "pgxc_batch_set_priority", "pgxc_batch_build",
-           "pgxc_batch_last_ms", "pgxc_batch_last_error", "pgxc_batch_destroy")
+           "pgxc_batch_last_ms", "pgxc_batch_last_error", "pgxc_batch_destroy",
+           "pgxc_set_schedule", "pgxc_batch_set_schedule", "pgxc_set_item_log",
+           "pgxc_batch_set_item_log", "pgxc_item_log", "pgxc_batch_item_log",
+           "pgxc_costmodel_create", "pgxc_costmodel_nparams", "pgxc_costmodel_params",
+           "pgxc_costmodel_set_params", "pgxc_costmodel_train", "pgxc_costmodel_gradient",
+           "pgxc_costmodel_predict", "pgxc_costmodel_last_error", "pgxc_costmodel_destroy")
+
+NFEATURES = 9
+SCHED_STATIC, SCHED_ANALYTIC, SCHED_MODEL = 0, 1, 2
 
 
 class Options(ctypes.Structure):
@@ -72,8 +80,29 @@ def load():
     L.pgxc_destroy.argtypes = [_vp]
     L.pgxc_becke_weights.argtypes = [ctypes.c_int32, _dp, ctypes.c_int64, _dp, _ip, _dp, _dp]
     L.pgxc_becke_weights.restype = ctypes.c_int
+    L.pgxc_set_schedule.argtypes = [_vp, ctypes.c_int32, _vp]
+    L.pgxc_batch_set_schedule.argtypes = [_vp, ctypes.c_int32, _vp]
+    L.pgxc_set_item_log.argtypes = [_vp, ctypes.c_int32]
+    L.pgxc_batch_set_item_log.argtypes = [_vp, ctypes.c_int32]
+    L.pgxc_item_log.argtypes = [_vp, _dp, _dp, ctypes.c_int64]
+    L.pgxc_batch_item_log.argtypes = [_vp, _dp, _dp, ctypes.c_int64]
+    L.pgxc_costmodel_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.c_uint64]
+    L.pgxc_costmodel_nparams.argtypes = [_vp]
+    L.pgxc_costmodel_params.argtypes = [_vp, _dp, _dp]
+    L.pgxc_costmodel_set_params.argtypes = [_vp, _dp, _dp]
+    L.pgxc_costmodel_train.argtypes = [_vp, _dp, _dp, ctypes.c_int64, ctypes.c_int32,
+                                       ctypes.c_double, _dp]
+    L.pgxc_costmodel_gradient.argtypes = [_vp, _dp, _dp, ctypes.c_int64, _dp, _dp]
+    L.pgxc_costmodel_predict.argtypes = [_vp, _dp, ctypes.c_int64, _dp]
+    L.pgxc_costmodel_last_error.argtypes = [_vp]
+    L.pgxc_costmodel_last_error.restype = ctypes.c_char_p
+    L.pgxc_costmodel_destroy.argtypes = [_vp]
+    special = {"pgxc_nbas": ctypes.c_int64, "pgxc_last_error": ctypes.c_char_p,
+               "pgxc_batch_last_error": ctypes.c_char_p,
+               "pgxc_costmodel_last_error": ctypes.c_char_p,
+               "pgxc_costmodel_nparams": ctypes.c_int32, "pgxc_item_log": ctypes.c_int64,
+               "pgxc_batch_item_log": ctypes.c_int64}
     for name in EXPORTS:
-        if name not in ("pgxc_nbas", "pgxc_last_error", "pgxc_batch_last_error"):
-            getattr(L, name).restype = ctypes.c_int
+        getattr(L, name).restype = special.get(name, ctypes.c_int)
     _lib = L
     return L

@@ -115,6 +115,26 @@ class XCBuilder:
         return dict(basis=t[0], density=t[1], functional=t[2], potential=t[3], pack=t[4],
                     total=t[5])
 
+
+    def set_schedule(self, mode, model=None):
+        """mode: _lib.SCHED_STATIC / SCHED_ANALYTIC / SCHED_MODEL (with a CostModel)."""
+        self._check(self._L.pgxc_set_schedule(self._h, mode, None if model is None else model.handle))
+
+    def set_item_log(self, on=True):
+        self._check(self._L.pgxc_set_item_log(self._h, 1 if on else 0))
+
+    def item_log(self):
+        """(features (n, NFEATURES), microseconds (n)) of the last logged build."""
+        n = self._L.pgxc_item_log(self._h, None, None, 0)
+        if n < 0:
+            raise _lib.PgxcError(-n, "item log")
+        F = np.zeros((n, _lib.NFEATURES))
+        us = np.zeros(n)
+        m = self._L.pgxc_item_log(self._h, _ptr(F), _ptr(us), n)
+        if m != n:
+            raise _lib.PgxcError(3, "item log")
+        return F, us
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self._L.pgxc_destroy(self._h)
@@ -173,6 +193,26 @@ class Batch:
 
     def results(self):
         return [b.result() for b in self.builders]
+
+
+    def set_schedule(self, mode, model=None):
+        """mode: _lib.SCHED_STATIC / SCHED_ANALYTIC / SCHED_MODEL (with a CostModel)."""
+        self._check(self._L.pgxc_batch_set_schedule(self._h, mode, None if model is None else model.handle))
+
+    def set_item_log(self, on=True):
+        self._check(self._L.pgxc_batch_set_item_log(self._h, 1 if on else 0))
+
+    def item_log(self):
+        """(features (n, NFEATURES), microseconds (n)) of the last logged build."""
+        n = self._L.pgxc_batch_item_log(self._h, None, None, 0)
+        if n < 0:
+            raise _lib.PgxcError(-n, "item log")
+        F = np.zeros((n, _lib.NFEATURES))
+        us = np.zeros(n)
+        m = self._L.pgxc_batch_item_log(self._h, _ptr(F), _ptr(us), n)
+        if m != n:
+            raise _lib.PgxcError(3, "item log")
+        return F, us
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

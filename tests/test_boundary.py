@@ -17,7 +17,7 @@ FBIND = os.path.join(ROOT, "paragauss-gpl_amd", "fortran", "pgxc_binding.f90")
 
 def declared():
     src = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*(pgxc_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int32_t|int64_t|const char \*)\s*(pgxc_\w+)\s*\(", src, re.M)))
 
 
 def test_header_declares_expected_api():
