@@ -120,23 +120,214 @@ def cpu_baseline(b, g, C, occ, budget_s):
                       "occupied-orbital density as the reference, %.1f s" % (nsamp, nb, lo, dt)}
 
 
+def learned_training_set(log):
+    """Per-item timings logged by the library on the H2O, (H2O)32 FP64 and
+    (H2O)32 mixed-precision builds: features and log(us)."""
+    import numpy as np
+    from pgxc import XCBuilder, basis, grid, systems
+    from pgxc.costmodel import target
+    from pgxc.xc import BF16X3, FP32, FP64, PBE
+    Xs, ys = [], []
+
+    def run(b, g, C, occ, precs, reps, **kw):
+        with XCBuilder(b, g, **kw) as xc:
+            xc.set_orbitals(C, occ)
+            xc.set_item_log(True)
+            xc.build(PBE, FP64, fetch=False)
+            for prec in precs:
+                for _ in range(reps):
+                    if prec != FP64:
+                        xc.set_orbitals(systems.perturbed_orbitals(C, 1e-4, seed=5), occ,
+                                        incremental=True)
+                    xc.build(PBE, prec, fetch=False)
+                    F, us = xc.item_log()
+                    Xs.append(F)
+                    ys.append(target(us))
+    els, xyz = systems.h2o()
+    b = basis.seeded_basis(els, xyz, seed=1, shell_set="small")
+    g = grid.molecular_grid(els, xyz, nrad=75, nang=302)
+    C, occ = systems.random_orbitals(b.nbas, 5, seed=2)
+    run(b, g, C, occ, [FP64], 2)
+    els, b, g, C, occ = workload("h2o32")
+    for tile in (64, 96, 128):
+        run(b, g, C, occ, [FP64, FP32, BF16X3], 1, tile=tile)
+    X, y = np.vstack(Xs), np.concatenate(ys)
+    log("cost-model training set: %d items" % len(y))
+    return X, y
+
+
+def bench_learned(args):
+    """(H2O)128 + 40-atom solute: static work split vs queues balanced by
+    counted flops vs queues balanced by the trained cost model."""
+    import numpy as np
+    from pgxc import XCBuilder, _lib, basis, grid, systems
+    from pgxc.costmodel import CostModel
+    from pgxc.xc import BF16X3, FP64, PBE
+    X, y = learned_training_set(log)
+    model = CostModel(seed=3)
+    t = time.time()
+    loss = model.train(X, y, steps=args.train_steps, lr=3e-3)
+    log("cost model trained on the GPU: %d steps, loss %.4f (standardised), %.1f s"
+        % (args.train_steps, loss, time.time() - t))
+    els, xyz = systems.water_solute(128, 40, seed=41)
+    b = basis.seeded_basis(els, xyz, seed=42, shell_set="medium")
+    t = time.time()
+    g = grid.molecular_grid(els, xyz, nrad=72, nang=302, partition="gpu")
+    log("grid: %d points for %d atoms in %.1f s" % (g.shape[0], len(els), time.time() - t))
+    C, occ = systems.random_orbitals(b.nbas, 640, seed=43)
+    npts, n = g.shape[0], b.nbas
+    xc = XCBuilder(b, g)
+    xc.set_orbitals(C, occ)
+    log("layout:", xc.layout())
+    modes = [("static", _lib.SCHED_STATIC, None), ("analytic", _lib.SCHED_ANALYTIC, None),
+             ("model", _lib.SCHED_MODEL, model)]
+    res, ref = {}, {}
+    for prec_name, prec in (("fp64", FP64), ("bf16x3", BF16X3)):
+        for name, mode, m in modes:
+            xc.set_schedule(mode, m)
+            t = time.time()
+            xc.build(PBE, prec, fetch=False)  # plan (features, predictions) + warm-up
+            tplan = time.time() - t
+            ms = []
+            for _ in range(args.steps):
+                xc.build(PBE, prec, fetch=False)
+                ms.append(xc.timing()["total"])
+            out = xc.result()
+            if prec_name not in ref:
+                ref[prec_name] = out
+            same = bool(np.array_equal(out[0], ref[prec_name][0]) and out[1] == ref[prec_name][1])
+            res["%s/%s" % (prec_name, name)] = {"ms": min(ms), "pts_per_s": npts / (min(ms) * 1e-3),
+                                               "bit_identical_to_static": same,
+                                               "first_build_s": tplan}
+            log("%s %s: %.1f ms (first build %.1f s) identical=%s"
+                % (prec_name, name, min(ms), tplan, same))
+    st = xc.stats()
+    xc.close()
+    model.close()
+    best = res["bf16x3/model"]
+    out = {
+        "metric": "XC-build grid pts/s, learned work queues vs static split, (H2O)128 + 40-atom solute",
+        "value": best["pts_per_s"], "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps,
+        "higher_is_better": True, "dtype": "f64+bf16x3", "data": "synthetic",
+        "config": {"workload": "(H2O)128 + 40-atom C/H/N/O solute, %d basis functions, %d grid "
+                               "points, PBE, full builds" % (n, npts)},
+        "schedules": res,
+        "model_gain_vs_static": {p: res[p + "/static"]["ms"] / res[p + "/model"]["ms"]
+                                 for p in ("fp64", "bf16x3")},
+        "model_gain_vs_analytic": {p: res[p + "/analytic"]["ms"] / res[p + "/model"]["ms"]
+                                   for p in ("fp64", "bf16x3")},
+        "fp64_static_pts_per_s": res["fp64/static"]["pts_per_s"],
+        "screened_flops_per_build": st["flops"], "dense_flops_per_build": 4.0 * npts * n * n + 16.0 * npts * n,
+        "training_items": int(len(y)),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch64(args):
+    """64 seeded C/H/N/O molecules: per-molecule FP64 builds vs one batched,
+    hipGraph-replayed build (FP64, and split-BF16 incremental)."""
+    import numpy as np
+    from pgxc import Batch, XCBuilder, autotune, basis, grid, systems
+    from pgxc.xc import BF16X3, FP64, PBE
+    rng = np.random.default_rng(2024)
+    mols = []
+    t = time.time()
+    for i in range(64):
+        na = int(rng.integers(3, 61))
+        els, xyz = systems.random_molecule(na, seed=1000 + i)
+        b = basis.seeded_basis(els, xyz, seed=2000 + i, shell_set="small")
+        g = grid.molecular_grid(els, xyz, nrad=int(rng.integers(50, 76)),
+                                nang=int(rng.choice([194, 302])), partition="gpu")
+        C, occ = systems.random_orbitals(b.nbas, max(1, b.nbas // 5), seed=3000 + i)
+        mols.append((b, g, C, occ))
+    npts = sum(m[1].shape[0] for m in mols)
+    log("64 molecules, %d..%d basis functions, %d grid points, %.1f s"
+        % (min(m[0].nbas for m in mols), max(m[0].nbas for m in mols), npts, time.time() - t))
+    t = time.time()
+    xcs = []
+    for b, g, C, occ in mols:
+        tu = autotune.tune(b, g, C, occ, sample_points=8192, reps=1, mixed=BF16X3)
+        xc = XCBuilder(b, g, tile=tu["tile"], variants=(tu["variant_fp64"], tu["variant_fp32"]))
+        xc.set_orbitals(C, occ)
+        xcs.append(xc)
+    log("autotuned and created 64 contexts in %.1f s" % (time.time() - t))
+    # per-molecule FP64 launches
+    for xc in xcs:
+        xc.build(PBE, FP64, fetch=False)
+    per = []
+    for _ in range(args.steps):
+        t0 = time.time()
+        for xc in xcs:
+            xc.build(PBE, FP64, fetch=False)
+        per.append(time.time() - t0)
+    ref = [xc.result() for xc in xcs]
+    with Batch(xcs) as bt:
+        bt.build(PBE, FP64, graph=True)
+        b64 = []
+        for _ in range(args.steps):
+            t0 = time.time()
+            bt.build(PBE, FP64, graph=True)
+            b64.append(time.time() - t0)
+        same = all(np.array_equal(x.result()[0], r[0]) for x, r in zip(xcs, ref))
+        # mixed precision: incremental split-BF16 builds from the FP64 state
+        chains = [systems.perturbed_orbitals(m[2], 1e-4, seed=4000 + i) for i, m in enumerate(mols)]
+        for xc, Ck, m in zip(xcs, chains, mols):
+            xc.set_orbitals(Ck, m[3], incremental=True)
+        bt.build(PBE, BF16X3, graph=True)
+        mixed = [x.result() for x in xcs]
+        bx3 = []
+        for _ in range(args.steps):  # same density again: dP = 0 after the first
+            for xc, Ck, m in zip(xcs, chains, mols):
+                xc.set_orbitals(Ck, m[3], incremental=True)
+            t0 = time.time()
+            bt.build(PBE, BF16X3, graph=True)
+            bx3.append(time.time() - t0)
+    dE = []
+    for xc, Ck, m, mx in zip(xcs, chains, mols, mixed):
+        xc.set_orbitals(Ck, m[3])
+        v, e, _ = xc.build(PBE, FP64)
+        dE.append(abs(mx[1] - e))
+    for xc in xcs:
+        xc.close()
+    t_per, t_b64, t_bx3 = min(per), min(b64), min(bx3)
+    out = {
+        "metric": "XC-build grid pts/s, 64-molecule batch in one hipGraph vs per-molecule FP64 launches",
+        "value": npts / t_bx3, "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps,
+        "higher_is_better": True, "dtype": "f64+bf16x3", "data": "synthetic",
+        "config": {"workload": "64 seeded C/H/N/O molecules (3-60 atoms), %d grid points, "
+                               "50-75 x 194/302 grids, PBE" % npts},
+        "per_molecule_fp64_s": t_per, "batch_fp64_graph_s": t_b64, "batch_bf16x3_graph_s": t_bx3,
+        "batch_fp64_bit_identical": same,
+        "speedup_batch_fp64": t_per / t_b64, "speedup_batch_bf16x3": t_per / t_bx3,
+        "max_abs_dE_bf16x3_vs_fp64": max(dE),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="h2o32")
+    ap.add_argument("--workload", default="h2o32",
+                    help="h2o32 (headline), h2o, batch64 (64-molecule batch), "
+                         "learned ((H2O)128 + solute, cost-model schedules)")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16x3"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a streaming kernel over MIB MiB (counter calibration)")
+    ap.add_argument("--train-steps", type=int, default=2000, help="cost-model training steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(the multi-GPU sharding is a design note only)" % args.gpus)
         sys.exit(2)
+    if args.workload == "learned":
+        return bench_learned(args)
+    if args.workload == "batch64":
+        return bench_batch64(args)
 
     import numpy as np
     from pgxc import XCBuilder

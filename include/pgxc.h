@@ -66,8 +66,10 @@ extern "C" {
 
 /* Features of one work item, in this order (pgxc_item_log, cost model):
  * kind (0 density, 1 potential contraction), precision one-hot (FP64, FP32,
- * BF16x3), function tile / 128, K length left after screening / 1024, dense
- * K length / 1024, 1 if any block is left after screening, variant / 2. */
+ * BF16x3), function tile / 128, log2(1 + K length left after screening) / 16,
+ * log2(1 + dense K length) / 16, 1 if any block is left after screening,
+ * variant / 2.  (K: functions of the density contraction, points of the
+ * potential contraction.) */
 #define PGXC_NFEATURES 9
 
 typedef struct pgxc_ctx pgxc_ctx;

@@ -219,8 +219,8 @@ static void unit_item_features(const pgxc_ctx *c, int step, ItemKind kind, int p
                                       precision == PGXC_PREC_FP32 ? 1.0 : 0.0,
                                       precision == PGXC_PREC_BF16X3 ? 1.0 : 0.0,
                                       tf / 128.0,
-                                      kact / 1024.0,
-                                      kdense / 1024.0,
+                                      std::log2(1.0 + kact) / 16.0,
+                                      std::log2(1.0 + kdense) / 16.0,
                                       active ? 1.0 : 0.0,
                                       var / 2.0};
     feat.insert(feat.end(), f, f + PGXC_NFEATURES);

@@ -3,8 +3,8 @@
 Follows ParaGauss's grid construction (modules/grid_module.f90:357-504):
   radial   Pople/Gill grid r_i = R (i / (n+1-i))^2, w_i = 2 R^3 (n+1) i^5 / (n+1-i)^7,
            truncated at 40 bohr (RADIALGRID, :1246-1301)
-  angular  Lebedev rule (302 points, degree 29; own solution of the
-           octahedral moment equations, pgxc/lebedev.py)
+  angular  Lebedev rule (302 points, degree 29, or 194, degree 23; own
+           solutions of the octahedral moment equations, pgxc/lebedev.py)
   product  w = w_ang * w_rad * 4 pi  (:426-433)
   Becke    partition of every atomic grid (becke0, :2442-2497), on the GPU
            through pgxc_becke_weights, or in numpy for small systems

@@ -78,6 +78,20 @@ def random_molecule(natoms, seed, bond=(1.0, 1.5), min_nonbonded=2.0):
     return els, np.array(xyz) * ANG
 
 
+def water_solute(n_water, n_solute, seed, gap_a=3.0, min_dist_a=2.0):
+    """(H2O)_n_water cluster with a seeded n_solute-atom C/H/N/O molecule placed
+    beside it along +x: the solute's nearest atom gap_a beyond the cluster's
+    extent, every solute-water distance at least min_dist_a (bohr)."""
+    ew, xw = water_cluster(n_water, seed=seed)
+    es, xs = random_molecule(n_solute, seed=seed + 1)
+    xs = xs - xs.mean(axis=0)
+    shift = xw[:, 0].max() - xs[:, 0].min() + gap_a * ANG
+    xs = xs + np.array([shift, 0.0, 0.0])
+    d = np.linalg.norm(xs[:, None] - xw[None], axis=-1)
+    assert d.min() >= min_dist_a * ANG
+    return list(ew) + list(es), np.vstack([xw, xs])
+
+
 def random_orbitals(nbas, nocc, seed):
     """QR-orthonormalised N(0,1) coefficients (nbas x nocc) and occ = 1."""
     rng = np.random.default_rng(seed)

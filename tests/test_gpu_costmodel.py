@@ -168,7 +168,9 @@ def test_network_beats_counted_flops(case, trained):
     b, g, C, occ = case
     m, X, y = trained
     Xt, yt = collect_items(b, g, C, occ, reps=1)
-    flops = lambda Z: 2.0 * (np.where(Z[:, 0] == 0, 128.0, Z[:, 4] * 128) * Z[:, 4] * 128) * Z[:, 5] * 1024
+    def flops(Z):  # 2 x (tile rows) x (tile columns) x K left after screening
+        k = np.exp2(16.0 * Z[:, 5]) - 1.0
+        return 2.0 * np.where(Z[:, 0] == 0, 128.0, Z[:, 4] * 128) * Z[:, 4] * 128 * k
     A = np.column_stack([np.ones(len(X)), np.log1p(flops(X)), X[:, 1:4]])
     coef, *_ = np.linalg.lstsq(A, y, rcond=None)
     At = np.column_stack([np.ones(len(Xt)), np.log1p(flops(Xt)), Xt[:, 1:4]])
