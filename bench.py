@@ -120,7 +120,7 @@ def cpu_baseline(b, g, C, occ, budget_s):
                       "occupied-orbital density as the reference, %.1f s" % (nsamp, nb, lo, dt)}
 
 
-def learned_training_set(log):
+def learned_training_set(log, max_items=1 << 18):
     """Per-item timings logged by the library on the H2O, (H2O)32 FP64 and
     (H2O)32 mixed-precision builds: features and log(us)."""
     import numpy as np
@@ -152,6 +152,9 @@ def learned_training_set(log):
     for tile in (64, 96, 128):
         run(b, g, C, occ, [FP64, FP32, BF16X3], 1, tile=tile)
     X, y = np.vstack(Xs), np.concatenate(ys)
+    if len(y) > max_items:  # a seeded subsample keeps training to seconds
+        keep = np.sort(np.random.default_rng(17).choice(len(y), max_items, replace=False))
+        X, y = X[keep], y[keep]
     log("cost-model training set: %d items" % len(y))
     return X, y
 
@@ -169,6 +172,11 @@ def bench_learned(args):
     loss = model.train(X, y, steps=args.train_steps, lr=3e-3)
     log("cost model trained on the GPU: %d steps, loss %.4f (standardised), %.1f s"
         % (args.train_steps, loss, time.time() - t))
+    if args.save_model:
+        P, nrm = model.params()
+        np.savez(args.save_model, params=P, norm=nrm, items=len(y), steps=args.train_steps,
+                 loss=loss)
+        log("cost model saved to", args.save_model)
     els, xyz = systems.water_solute(128, 40, seed=41)
     b = basis.seeded_basis(els, xyz, seed=42, shell_set="medium")
     t = time.time()
@@ -319,6 +327,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a streaming kernel over MIB MiB (counter calibration)")
     ap.add_argument("--train-steps", type=int, default=2000, help="cost-model training steps")
+    ap.add_argument("--save-model", default="", help="learned: write the trained model here (.npz)")
+    ap.add_argument("--schedule", default="auto", choices=["auto", "static", "analytic", "model"],
+                    help="contraction work schedule (auto: the shipped cost model if present, "
+                         "else analytic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -352,6 +364,15 @@ def main():
         xc = XCBuilder(b, g)
     layout = xc.layout()
     log("layout:", layout)
+    from pgxc import _lib
+    from pgxc.costmodel import default_model
+    sched, model = args.schedule, None
+    if sched in ("auto", "model"):
+        model = default_model()
+        sched = "model" if model is not None else "analytic"
+    xc.set_schedule({"static": _lib.SCHED_STATIC, "analytic": _lib.SCHED_ANALYTIC,
+                     "model": _lib.SCHED_MODEL}[sched], model)
+    log("schedule:", sched)
 
     # ---- plain FP64 full builds
     xc.set_orbitals(C, occ)
@@ -371,10 +392,12 @@ def main():
     for prec in precs:
         xc.set_orbitals(C, occ)
         xc.build(PBE, FP64, fetch=False)  # FP64 state the chain starts from
-        times, stages = [], []
+        times, stages, walls = [], [], []
         for i, Ck in enumerate(chain):
-            xc.set_orbitals(Ck, occ, incremental=True)
-            xc.build(PBE, PRECISIONS[prec], fetch=False)
+            t0 = time.time()
+            xc.set_orbitals(Ck, occ, incremental=True)  # host orbitals -> device, P formed there
+            xc.build(PBE, PRECISIONS[prec], fetch=True)  # packed V, E_xc, charge back to the host
+            walls.append(time.time() - t0)
             if i >= args.warmup:
                 tm = xc.timing()
                 times.append(tm["total"])
@@ -383,6 +406,7 @@ def main():
         xc.set_orbitals(chain[-1], occ)
         v_ref, e_ref, _ = xc.build(PBE, FP64)
         runs[prec] = dict(ms=sum(times) / len(times), st=stages[-1], dE=e_mx - e_ref,
+                          wall_ms=1e3 * min(walls[args.warmup:] or walls),
                           dV=float(np.max(np.abs(v_mx - v_ref))), E=e_ref)
         log("%s: %.2f ms/build dE=%.3e dV=%.3e stages %s" % (prec, runs[prec]["ms"], runs[prec]["dE"],
                                                             runs[prec]["dV"], stages[-1]))
@@ -425,8 +449,10 @@ def main():
         "dE_vs_fp64": R["dE"],
         "dV_vs_fp64": R["dV"],
         "E_xc": R["E"],
+        "wall_pts_per_s_incl_host": npts / (R["wall_ms"] * 1e-3),
         "mixed_forms": {p: {"ms_per_step": runs[p]["ms"], "pts_per_s": npts / (runs[p]["ms"] * 1e-3),
                             "dE_vs_fp64": runs[p]["dE"], "dV_vs_fp64": runs[p]["dV"]} for p in precs},
+        "schedule": sched,
         "precision_errors": perr,
         "layout": {k: int(v) for k, v in layout.items()},
         "stages_ms": {k: round(float(v), 3) for k, v in st.items()},

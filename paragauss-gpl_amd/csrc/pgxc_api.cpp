@@ -85,6 +85,8 @@ struct pgxc_ctx {
   uint8_t *gsig = nullptr;
   ScreenShell *sshells = nullptr;
   double flops_screened = 0, flops_dense = 0;  // algorithmic work of one build
+  std::vector<int> fpos;          // internal position of each caller function
+  int *d_fpos = nullptr;
   std::vector<int> h_kcount;     // host copies of the screening tables
   std::vector<uint8_t> h_gsig;
   Sched sched;
@@ -462,6 +464,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     Q.P = c->P;
     Q.Pref = c->Pref;
     Q.D = c->D;
+    Q.fpos = c->d_fpos;
     Q.n = c->nbas;
     Q.NP = c->NP;
     Q.nsplit = c->nsplit;
@@ -666,31 +669,75 @@ static int collect(pgxc_ctx *c, hipStream_t s, uint32_t functional) {
   return PGXC_OK;
 }
 
-// Order of the grid points along a Morton curve over their bounding box
-// (10 bits per axis), ties in input order.
-static void morton_order(const double *grid, long npts, std::vector<long> &perm) {
+// Order of n points (coordinate k of point p at xyz[k * stride + p * step])
+// along a Morton curve over their bounding box (10 bits per axis), ties in
+// input order.
+static void morton_order(const double *xyz, long n, long stride, long step,
+                         std::vector<long> &perm) {
   double lo[3], hi[3];
   for (int k = 0; k < 3; ++k) {
-    lo[k] = hi[k] = grid[(size_t)k * npts];
-    for (long p = 1; p < npts; ++p) {
-      lo[k] = std::min(lo[k], grid[(size_t)k * npts + p]);
-      hi[k] = std::max(hi[k], grid[(size_t)k * npts + p]);
+    lo[k] = hi[k] = xyz[(size_t)k * stride];
+    for (long p = 1; p < n; ++p) {
+      lo[k] = std::min(lo[k], xyz[(size_t)k * stride + (size_t)p * step]);
+      hi[k] = std::max(hi[k], xyz[(size_t)k * stride + (size_t)p * step]);
     }
   }
-  std::vector<std::pair<uint32_t, long>> key((size_t)npts);
-  for (long p = 0; p < npts; ++p) {
+  std::vector<std::pair<uint32_t, long>> key((size_t)n);
+  for (long p = 0; p < n; ++p) {
     uint32_t code = 0;
     for (int k = 0; k < 3; ++k) {
       const double span = hi[k] - lo[k];
-      const double f = span > 0 ? (grid[(size_t)k * npts + p] - lo[k]) / span : 0.0;
+      const double f = span > 0 ? (xyz[(size_t)k * stride + (size_t)p * step] - lo[k]) / span : 0.0;
       const uint32_t q = (uint32_t)std::min(1023.0, std::max(0.0, f * 1023.0));
       for (int b = 0; b < 10; ++b) code |= ((q >> b) & 1u) << (3 * b + k);
     }
     key[(size_t)p] = {code, p};
   }
   std::sort(key.begin(), key.end());
-  perm.resize((size_t)npts);
-  for (long p = 0; p < npts; ++p) perm[(size_t)p] = key[(size_t)p].second;
+  perm.resize((size_t)n);
+  for (long p = 0; p < n; ++p) perm[(size_t)p] = key[(size_t)p].second;
+}
+
+// Internal order of the basis functions: shells by reach class (the radius
+// sqrt(50 / alpha_min) beyond which the shell is cut off; three classes split
+// at the tertiles, longest reach first), then by their atom's place on a
+// Morton curve, then in caller order.  Functions that can be nonzero on a
+// compact group of points then fill few 32-function blocks.  A shell keeps
+// its functions together and in caller order.  Sets each shell's internal
+// func_off and fpos[caller function] = internal position.
+static void function_order(std::vector<ShellD> &sh, const double *exps, const double *atom_xyz,
+                           int n_atoms, std::vector<int> &fpos) {
+  const size_t ns = sh.size();
+  std::vector<double> reach(ns);
+  for (size_t i = 0; i < ns; ++i) {
+    double amin = exps[sh[i].exp_off];
+    for (int e = 1; e < sh[i].n_exp; ++e) amin = std::min(amin, exps[sh[i].exp_off + e]);
+    reach[i] = std::sqrt(50.0 / amin);
+  }
+  std::vector<double> sorted = reach;
+  std::sort(sorted.begin(), sorted.end());
+  const double t1 = sorted[ns / 3], t2 = sorted[(2 * ns) / 3];
+  std::vector<long> aperm;
+  morton_order(atom_xyz, n_atoms, 1, 3, aperm);
+  std::vector<int> arank(n_atoms);
+  for (int r = 0; r < n_atoms; ++r) arank[aperm[r]] = r;
+  std::vector<int> idx(ns);
+  std::iota(idx.begin(), idx.end(), 0);
+  auto cls = [&](int i) { return reach[i] >= t2 ? 0 : reach[i] >= t1 ? 1 : 2; };
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+    if (cls(a) != cls(b)) return cls(a) < cls(b);
+    return arank[sh[a].atom] < arank[sh[b].atom];
+  });
+  int nbas = 0;
+  for (auto &d : sh) nbas += (2 * d.l + 1) * (d.n_unc + d.n_contr);
+  fpos.assign(nbas, 0);
+  int off = 0;
+  for (int i : idx) {
+    const int nf = (2 * sh[i].l + 1) * (sh[i].n_unc + sh[i].n_contr);
+    for (int f = 0; f < nf; ++f) fpos[sh[i].func_off + f] = off + f;
+    sh[i].func_off = off;
+    off += nf;
+  }
 }
 
 // Screening tables of the padded grid (kernel k_screen) and the algorithmic
@@ -831,6 +878,7 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
       return fail(c, PGXC_ERR_ARG, "pgxc_create: exponents must be positive and norms finite");
   c->nbas = (int)foff;
   c->exps_per_point = (double)eoff;
+  function_order(sh, exps, atom_xyz, n_atoms, c->fpos);
   // function tile: requested, or the one of 128 / 96 / 64 that pads nbas least
   if (opt && opt->tile) {
     c->tf = opt->tile;
@@ -874,7 +922,7 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   if ((rc = dalloc(c, &c->contr, std::max<long>(coff, 1)))) return rc;
   // grid in space-filling-curve order, padded with far points of zero weight
   std::vector<long> perm;
-  morton_order(grid, npts, perm);
+  morton_order(grid, npts, npts, 1, perm);
   std::vector<double> g(4 * (size_t)c->npad);
   for (int k = 0; k < 4; ++k)
     for (long p = 0; p < c->npad; ++p)
@@ -883,6 +931,8 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   HIPCHK(c, hipMemcpy(c->grid, g.data(), g.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->atoms, at.data(), sizeof(AtomD) * n_atoms, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->shells, sh.data(), sizeof(ShellD) * n_shells, hipMemcpyHostToDevice));
+  if ((rc = dalloc(c, &c->d_fpos, c->nbas))) return rc;
+  HIPCHK(c, hipMemcpy(c->d_fpos, c->fpos.data(), sizeof(int) * c->nbas, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->exps, exps, 8 * eoff, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->norms, norms, 8 * eoff, hipMemcpyHostToDevice));
   if (coff > 0) HIPCHK(c, hipMemcpy(c->contr, contr, 8 * coff, hipMemcpyHostToDevice));
@@ -949,8 +999,9 @@ extern "C" int pgxc_set_density(pgxc_ctx *c, int32_t ispin, int32_t nbas, const 
   if (!P) return fail(c, PGXC_ERR_ARG, "pgxc_set_density: P is null");
   const int n = c->nbas, NP = c->NP;
   std::vector<double> h((size_t)NP * NP, 0.0);
+  const int *pos = c->fpos.data();
   for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) h[(size_t)i * NP + j] = P[(size_t)i + (size_t)j * n];
+    for (int i = 0; i < n; ++i) h[(size_t)pos[i] * NP + pos[j]] = P[(size_t)i + (size_t)j * n];
   HIPCHK(c, hipMemcpyAsync(c->P, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_density = 1;
@@ -970,7 +1021,10 @@ extern "C" int pgxc_set_orbitals(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32
   if (nocc > 0) {
     HIPCHK(c, hipMalloc((void **)&dC, (size_t)n * nocc * 8));
     HIPCHK(c, hipMalloc((void **)&docc, (size_t)nocc * 8));
-    HIPCHK(c, hipMemcpyAsync(dC, C, (size_t)n * nocc * 8, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> Cp((size_t)n * nocc);  // rows in the internal function order
+    for (int k = 0; k < nocc; ++k)
+      for (int i = 0; i < n; ++i) Cp[(size_t)c->fpos[i] + (size_t)k * n] = C[(size_t)i + (size_t)k * n];
+    HIPCHK(c, hipMemcpy(dC, Cp.data(), (size_t)n * nocc * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpyAsync(docc, occ, (size_t)nocc * 8, hipMemcpyHostToDevice, c->stream));
     launch_form_density(c->stream, dC, docc, n, nocc, c->NP, c->P);
     HIPCHK(c, hipGetLastError());
@@ -1080,7 +1134,7 @@ int pgxc_destroy(pgxc_ctx *c) {
   void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->klist, c->kcount, c->nbt, c->gsig,
-                  c->sshells};
+                  c->sshells, c->d_fpos};
   for (void *b : bufs) dfree(b);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);

@@ -66,6 +66,7 @@ struct PackD {
   const double *P;
   double *Pref;
   void *D;
+  const int *fpos;  // internal position of each basis function (caller order)
   long n, NP;
   int nsplit, incremental;
 };

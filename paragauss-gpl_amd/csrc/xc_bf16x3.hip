@@ -99,7 +99,9 @@ struct XOperand {
 };
 
 // AKC: A(m,k) = A[m*sA + k] (k contiguous) else A[m + k*sA]; same for B(k,n).
-// K step kt starts at k = kmap(kt).
+// K step kt starts at k = kmap(kt).  Pipeline: two LDS stages and two
+// register slots; the global loads of step kt + 2 are issued before the MFMAs
+// of step kt, so each load has two steps of MFMA work to arrive in.
 template <int BM, int BN, bool AKC, bool BKC, class KMap>
 __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, long sA,
                                                  const float *__restrict__ B, long sB,
@@ -109,31 +111,25 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, lo
   constexpr int SA = BM * BKP, SB = BN * BKP, STAGE = 2 * SA + 2 * SB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  XOperand<BM, AKC> ra;
-  XOperand<BN, BKC> rb;
+  XOperand<BM, AKC> ra0, ra1;
+  XOperand<BN, BKC> rb0, rb1;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
   if (ksteps <= 0) return;
-  auto load = [&](int kt) {
+  auto load = [&](int kt, XOperand<BM, AKC> &ra, XOperand<BN, BKC> &rb) {
     const long kb = kmap(kt);
     ra.load(A, sA, kb);
     rb.load(B, sB, kb);
   };
-  auto store = [&](int buf) {
+  auto store = [&](int buf, const XOperand<BM, AKC> &ra, const XOperand<BN, BKC> &rb) {
     __bf16 *ah = lds + buf * STAGE, *al = ah + SA, *bh = al + SA, *bl = bh + SB;
     ra.store(ah, al);
     rb.store(bh, bl);
   };
-
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int kt = 0; kt < ksteps; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < ksteps) load(kt + 1);
+  auto compute = [&](int cur) {
     const __bf16 *ah = lds + cur * STAGE, *al = ah + SA, *bh = al + SA, *bl = bh + SB;
     const int ko = 8 * (lane >> 4);
     bf16x8 ahi[TM], alo[TM], bhi[TN], blo[TN];
@@ -157,8 +153,23 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, lo
         acc[i][j] = mfma_bf16(ahi[i], blo[j], acc[i][j]);
         acc[i][j] = mfma_bf16(ahi[i], bhi[j], acc[i][j]);
       }
-    if (kt + 1 < ksteps) store(cur ^ 1);
+  };
+  // step kt lives in register slot kt & 1 and LDS stage kt & 1
+  auto step = [&](int kt, XOperand<BM, AKC> &ra_cur, XOperand<BN, BKC> &rb_cur,
+                  XOperand<BM, AKC> &ra_nxt, XOperand<BN, BKC> &rb_nxt) {
+    if (kt + 2 < ksteps) load(kt + 2, ra_cur, rb_cur);  // slot of kt is free: kt is in LDS
+    compute(kt & 1);
+    if (kt + 1 < ksteps) store((kt + 1) & 1, ra_nxt, rb_nxt);
     __syncthreads();
+  };
+
+  load(0, ra0, rb0);
+  if (ksteps > 1) load(1, ra1, rb1);
+  store(0, ra0, rb0);
+  __syncthreads();
+  for (int kt = 0; kt < ksteps; kt += 2) {
+    step(kt, ra0, rb0, ra1, rb1);
+    if (kt + 1 < ksteps) step(kt + 1, ra1, rb1, ra0, rb0);
   }
 }
 

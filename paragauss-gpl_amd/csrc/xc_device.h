@@ -136,10 +136,45 @@ __device__ __forceinline__ int potential_screen(const UnitD &U, long kb, int npt
   return cnt;
 }
 
+// Sum of M per-lane values (M = 4 or 16) over the 16 lanes of each lane
+// group (the 16 columns of an MFMA result block) by recursive halving: at the
+// exchange with lane l ^ o a lane keeps half of the values it still holds
+// and sends the other half, so M = 16 values take 8 + 4 + 2 + 1 shuffles
+// instead of 4 x 16.  Returns the total of value `index` in this lane
+// (index = l & 15 for M = 16; for M = 4 the groups of 4 lanes l & ~3 share
+// index (l & 12) / 4).  The addition order is fixed.
+template <int O, int m>
+struct GroupStep {  // one exchange with lane ^ O while m values are held
+  template <int M>
+  __device__ static __forceinline__ void run(double (&v)[M], int lane, int &index) {
+    if constexpr (m > 1) {
+      const bool hi = (lane & O) != 0;
+#pragma unroll
+      for (int k = 0; k < m / 2; ++k) {
+        const double send = hi ? v[k] : v[k + m / 2];
+        const double keep = hi ? v[k + m / 2] : v[k];
+        v[k] = keep + __shfl_xor(send, O);
+      }
+      if (hi) index += m / 2;
+    } else {
+      v[0] += __shfl_xor(v[0], O);
+    }
+    if constexpr (O > 1) GroupStep<O / 2, (m > 1 ? m / 2 : 1)>::run(v, lane, index);
+  }
+};
+
+template <int M>
+__device__ __forceinline__ double group_reduce(double (&v)[M], int lane, int &index) {
+  static_assert(M == 4 || M == 16, "4 or 16 values");
+  index = 0;
+  GroupStep<8, M>::run(v, lane, index);
+  return v[0];
+}
+
 // Epilogue of X = Phi D for one point tile x function tile: contributions
 // rho += sum_n Phi(p,n) X(p,n), grad rho += 2 sum_n dPhi(p,n) X(p,n), summed
-// over the tile's columns in a fixed order (j, then the 16-lane butterfly)
-// and written to part[tn][c][p].  With the FP32 result placement a lane holds
+// over the tile's columns in a fixed order (j, then the 16-lane group
+// reduction) and written to part[tn][c][p].  With the FP32 result placement a lane holds
 // four consecutive points of one column, read as one 16-byte load per
 // component; the FP64 placement reads point by point.  Both orders of the
 // sums are the same.
@@ -189,17 +224,16 @@ __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 
         for (int c = 1; c < NC; ++c) s[c][r] += 2.0 * (double)v[c][r] * xv;
       }
     }
+    double v[NC * 4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int off = 8; off >= 1; off >>= 1)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) s[c][r] += __shfl_xor(s[c][r], off);
-      if ((lane & 15) == 0) {
-        const int row = wm * WM + i * 16 + MF<T>::row(lane, r);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) red[wn][c][row] = s[c][r];
-      }
+      for (int r = 0; r < 4; ++r) v[c * 4 + r] = s[c][r];
+    int idx;
+    const double tot = group_reduce<NC * 4>(v, lane, idx);
+    if (NC == 4 || (lane & 3) == 0) {
+      const int c = idx / 4, r = idx % 4;
+      red[wn][c][wm * WM + i * 16 + MF<T>::row(lane, r)] = tot;
     }
   }
   __syncthreads();

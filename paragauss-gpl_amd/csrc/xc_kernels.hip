@@ -737,7 +737,8 @@ __global__ __launch_bounds__(kThreads) void k_potential(const StageList S) {
 }
 
 // -------------------------------------------------------------------- pack
-// item = 256 packed elements of one molecule: V = base + sum_split (H + H^T);
+// item = 256 packed elements of one molecule (caller's function order):
+// V = base + sum_split (H + H^T), H read at the functions' internal rows;
 // then the molecule's state for the next incremental build: V_state = V.
 __global__ __launch_bounds__(kThreads) void k_pack(const PackD *__restrict__ mols, int nm,
                                                    const long *__restrict__ base, long total) {
@@ -751,10 +752,11 @@ __global__ __launch_bounds__(kThreads) void k_pack(const PackD *__restrict__ mol
     while (j * (j + 1) / 2 > idx) --j;
     while ((j + 1) * (j + 2) / 2 <= idx) ++j;
     const long k = idx - j * (j + 1) / 2;
+    const long pj = M.fpos[j], pk = M.fpos[k];  // internal rows of caller functions j, k
     double hjk = 0.0, hkj = 0.0;
     for (int s = 0; s < M.nsplit; ++s) {
-      hjk += M.Hpart[(long)s * NP * NP + j * NP + k];
-      hkj += M.Hpart[(long)s * NP * NP + k * NP + j];
+      hjk += M.Hpart[(long)s * NP * NP + pj * NP + pk];
+      hkj += M.Hpart[(long)s * NP * NP + pk * NP + pj];
     }
     const double v = hjk + hkj;
     const double out = M.incremental ? M.Vstate[idx] + v : v;

@@ -7,6 +7,7 @@ The schedules that use it (SCHED_MODEL) replace the static contiguous split
 of the reference's distribute_jobs (dlb/dlb_common.f90:516-561).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -103,3 +104,18 @@ class CostModel:
 def target(us):
     """Training target of an item time: log(us), with a floor of 0.01 us."""
     return np.log(np.maximum(np.asarray(us, dtype=np.float64), 1e-2))
+
+
+DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cost_model_mi355x.npz")
+
+
+def default_model(path=DEFAULT_PATH):
+    """The cost model shipped with the package (trained on MI355X item timings
+    of the H2O and (H2O)32 builds by `bench.py --workload learned
+    --save-model`), or None if the file is absent."""
+    if not os.path.exists(path):
+        return None
+    d = np.load(path, allow_pickle=False)
+    m = CostModel(_lib.NFEATURES, seed=0)
+    m.set_params(d["params"], d["norm"])
+    return m
