@@ -410,6 +410,7 @@ def main():
                           dV=float(np.max(np.abs(v_mx - v_ref))), E=e_ref)
         log("%s: %.2f ms/build dE=%.3e dV=%.3e stages %s" % (prec, runs[prec]["ms"], runs[prec]["dE"],
                                                             runs[prec]["dV"], stages[-1]))
+    stats = xc.stats()
     perr = precision_errors(xc, C, occ, precs)
     log("precision errors per ||dP||_F:", perr)
     xc.close()
@@ -419,12 +420,15 @@ def main():
 
     ms64 = sum(t64) / len(t64)
     msmx = R["ms"]
-    # dominant kernel of the mixed build and its roofline (dense-equivalent
-    # algorithmic flops: 2 npts n^2 per contraction; no screening on this path)
+    # dominant kernel of the mixed build and its roofline.  Algorithmic flops of
+    # one contraction: half of pgxc_last_stats' sum over 128-point tiles of
+    # 4 B n_b^2 + 16 B n_b (n_b: functions not cut off on the tile; the 16 B n_b
+    # elementwise share is ~1 %); the dense equivalent 2 npts n^2 beside it.
     st = R["st"]
     dom = "density" if st["density"] >= st["potential"] else "potential"
-    flops = 2.0 * npts * n * n
+    flops = stats["flops"] / 2.0
     achieved = flops / (st[dom] * 1e-3) / 1e12
+    achieved_dense = 2.0 * npts * n * n / (st[dom] * 1e-3) / 1e12
     peak = PEAK["fp32"] if best == "fp32" else PEAK["bf16"] / 3.0
     out = {
         "metric": "XC-build grid pts/s, mixed vs FP64 GPU at |dE_xc|<=1e-6 Eh; %MFMA/HBM peak",
@@ -460,7 +464,10 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "k_" + dom + ("_bf16x3" if best == "bf16x3" else ""),
                      "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s" + (" (FP32-equivalent: BF16 peak / 3 passes)" if best == "bf16x3" else ""),
-                     "frac": achieved / peak, "traffic": None},
+                     "frac": achieved / peak, "traffic": None,
+                     "achieved_dense_equivalent": achieved_dense,
+                     "flops_per_build_algorithmic": stats["flops"],
+                     "flops_per_build_dense": 4.0 * npts * n * n + 16.0 * npts * n},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(b, g, C, occ, args.cpu_seconds)

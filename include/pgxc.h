@@ -87,13 +87,19 @@ typedef struct pgxc_costmodel pgxc_costmodel;
  * every basis function and point; 0 (default) skips the 32-function x
  * 32-point blocks on which the radial cut-off alpha r^2 >= 50
  * (modules/orbital_module.f90:1839-1855) makes every basis value exactly zero,
- * which leaves every result bit unchanged. */
+ * which leaves every result bit unchanged.  no_basis_cache: 1 evaluates the
+ * basis values in every build; 0 (default) keeps the values of every grid
+ * chunk on the device after the first build of each precision and reuses
+ * them (the role of ParaGauss's orbital store across SCF cycles,
+ * modules/orbitalstore_module.f90:862-977), when they fit in a quarter of the
+ * device memory. */
 typedef struct pgxc_options {
   int32_t tile;
   int64_t chunk_points;
   int32_t variant_fp64;
   int32_t variant_fp32;
   int32_t no_screen;
+  int32_t no_basis_cache;
 } pgxc_options;
 
 /*

@@ -63,6 +63,14 @@ struct pgxc_ctx {
   // device: work buffers of one chunk
   void *phi = nullptr, *W = nullptr;
   size_t phi_bytes = 0, w_bytes = 0, phi_elem = 0;
+  // basis values of every chunk kept across builds, per element type
+  // ([0] FP64, [1] FP32): built once, reused while the geometry lives
+  int cache_off = 0;
+  void *phic[2] = {nullptr, nullptr};
+  int phic_valid[2] = {0, 0}, phic_failed[2] = {0, 0};
+  void *phi_base = nullptr;  // Phi of the current build: chunk k at phi_base + k * phi_stride
+  size_t phi_stride = 0;
+  int skip_basis = 0;        // the current build reads cached basis values
   double *part = nullptr, *coef = nullptr, *Hpart = nullptr;
   // device: state and results
   double *state = nullptr;                    // [S_NSLOT][npad]
@@ -83,6 +91,8 @@ struct pgxc_ctx {
   uint16_t *klist = nullptr;
   int *kcount = nullptr, *nbt = nullptr;
   uint8_t *gsig = nullptr;
+  uint32_t *gbits = nullptr;
+  int nfw = 0;
   ScreenShell *sshells = nullptr;
   double flops_screened = 0, flops_dense = 0;  // algorithmic work of one build
   std::vector<int> fpos;          // internal position of each caller function
@@ -364,7 +374,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.exps = c->exps;
       U.norms = c->norms;
       U.contr = c->contr;
-      U.phi = c->phi;
+      U.phi = (char *)c->phi_base + (size_t)k * c->phi_stride;
       U.W = c->W;
       U.D = c->D;
       U.part = c->part;
@@ -383,16 +393,27 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.mask = functional;
       U.screen = c->screen;
       U.nfb = c->nfb;
+      U.nfw = c->nfw;
       U.gstride = c->gstride;
       if (c->screen) {
         U.klist = c->klist + (p0 / kScreenTile) * c->nfb;
         U.kcount = c->kcount + p0 / kScreenTile;
         U.gsig = c->gsig + p0 / kGroup;
+        U.gbits = c->gbits + (p0 / kGroup) * c->nfw;
       }
       units.push_back(U);
       umol.push_back(m);
     }
-    pend[k].push_back(stage(units, umol, K_BASIS, 0, k));
+    {  // molecules whose basis values are cached skip the basis stage
+      std::vector<UnitD> bu;
+      std::vector<int> bm;
+      for (size_t i = 0; i < units.size(); ++i)
+        if (!mols[umol[i]]->skip_basis) {
+          bu.push_back(units[i]);
+          bm.push_back(umol[i]);
+        }
+      pend[k].push_back(stage(bu, bm, K_BASIS, 0, k));
+    }
     pend[k].push_back(stage(units, umol, K_FUNC, 0, k));
     pend[k].push_back(stage(units, umol, K_WMAT, 0, k));
     const int cls = precision == PGXC_PREC_FP64 ? 0 : 1;
@@ -593,22 +614,6 @@ static int ensure_work(pgxc_ctx *c, size_t elem, hipStream_t s) {
   const size_t need_phi = 4 * (size_t)c->NP * c->Mc * elem;
   const size_t need_w = (size_t)c->NP * c->Mc * elem;
   bool moved = false;
-  if (c->phi_bytes < need_phi) {
-    dfree(c->phi);
-    c->phi = nullptr;
-    c->phi_bytes = 0;
-    HIPCHK(c, hipMalloc(&c->phi, need_phi));
-    c->phi_bytes = need_phi;
-    c->phi_elem = 0;
-    moved = true;
-  }
-  // function rows nbas..NP-1 are never written and must read as zero in the
-  // element type of this build (a buffer last used at another precision
-  // holds that precision's bytes there)
-  if (c->phi_elem != elem) {
-    HIPCHK(c, hipMemsetAsync(c->phi, 0, c->phi_bytes, s));
-    c->phi_elem = elem;
-  }
   if (c->w_bytes < need_w) {
     dfree(c->W);
     c->W = nullptr;
@@ -617,8 +622,55 @@ static int ensure_work(pgxc_ctx *c, size_t elem, hipStream_t s) {
     c->w_bytes = need_w;
     moved = true;
   }
+  // basis cache: every chunk's values, when they take at most a quarter of
+  // the device and leave 32 GiB free
+  const int e = elem == 8 ? 0 : 1;
+  if (!c->cache_off && !c->phic[e] && !c->phic_failed[e]) {
+    const size_t need = need_phi * (size_t)c->nchunks;
+    size_t fr = 0, tot = 0;
+    c->phic_failed[e] = 1;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && need <= tot / 4 && need + (32ull << 30) <= fr &&
+        hipMalloc(&c->phic[e], need) == hipSuccess) {
+      c->phic_failed[e] = 0;
+      HIPCHK(c, hipMemsetAsync(c->phic[e], 0, need, s));  // padding rows read as zero
+      c->phic_valid[e] = 0;
+    } else {
+      c->phic[e] = nullptr;
+      (void)hipGetLastError();
+    }
+  }
+  if (!c->phic[e]) {  // one chunk buffer, rewritten by every build
+    if (c->phi_bytes < need_phi) {
+      dfree(c->phi);
+      c->phi = nullptr;
+      c->phi_bytes = 0;
+      HIPCHK(c, hipMalloc(&c->phi, need_phi));
+      c->phi_bytes = need_phi;
+      c->phi_elem = 0;
+    }
+    // function rows nbas..NP-1 are never written and must read as zero in the
+    // element type of this build (a buffer last used at another precision
+    // holds that precision's bytes there)
+    if (c->phi_elem != elem) {
+      HIPCHK(c, hipMemsetAsync(c->phi, 0, c->phi_bytes, s));
+      c->phi_elem = elem;
+    }
+  }
+  void *base = c->phic[e] ? c->phic[e] : c->phi;
+  const size_t stride = c->phic[e] ? need_phi : 0;
+  const int skip = c->phic[e] && c->phic_valid[e];
+  if (base != c->phi_base || stride != c->phi_stride || skip != c->skip_basis) moved = true;
+  c->phi_base = base;
+  c->phi_stride = stride;
+  c->skip_basis = skip;
   if (moved) free_plan(c->plan);  // its unit descriptors hold the old pointers
   return PGXC_OK;
+}
+
+// after a build: the cache of this element type now holds valid values
+static void mark_basis_cached(pgxc_ctx *c, size_t elem) {
+  const int e = elem == 8 ? 0 : 1;
+  if (c->phic[e]) c->phic_valid[e] = 1;
 }
 
 static int validate_build(pgxc_ctx *c, uint32_t functional, int32_t precision, int &incr) {
@@ -767,10 +819,12 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   if ((rc = dalloc(c, &c->kcount, ntp))) return rc;
   if ((rc = dalloc(c, &c->nbt, ntp))) return rc;
   if ((rc = dalloc(c, &c->gsig, (size_t)c->ntiles * c->gstride))) return rc;
+  c->nfw = (c->nfb + 31) / 32;
+  if ((rc = dalloc(c, &c->gbits, (size_t)c->gstride * c->nfw))) return rc;
   HIPCHK(c, hipMemcpy(c->sshells, ss.data(), sizeof(ScreenShell) * ss.size(), hipMemcpyHostToDevice));
   launch_screen(c->stream, ntp, c->grid, c->grid + c->npad, c->grid + 2 * c->npad, c->n_atoms,
                 c->atoms, c->n_shells, c->sshells, c->nfb, c->tf, c->ntiles, c->gstride, c->klist,
-                c->kcount, c->nbt, c->gsig);
+                c->kcount, c->nbt, c->gsig, c->gbits);
   HIPCHK(c, hipGetLastError());
   std::vector<int> nb((size_t)ntp);
   HIPCHK(c, hipMemcpyAsync(nb.data(), c->nbt, sizeof(int) * ntp, hipMemcpyDeviceToHost, c->stream));
@@ -815,8 +869,10 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
               opt->variant_fp32 >= kNumVariants))
     return fail(c, PGXC_ERR_ARG, "pgxc_create: variant must be -1 (default) or 0.." +
                                      std::to_string(kNumVariants - 1));
-  if (opt && (opt->no_screen < 0 || opt->no_screen > 1))
-    return fail(c, PGXC_ERR_ARG, "pgxc_create: no_screen must be 0 or 1");
+  if (opt && (opt->no_screen < 0 || opt->no_screen > 1 || opt->no_basis_cache < 0 ||
+              opt->no_basis_cache > 1))
+    return fail(c, PGXC_ERR_ARG, "pgxc_create: no_screen and no_basis_cache must be 0 or 1");
+  c->cache_off = opt ? opt->no_basis_cache : 0;
   if (opt && opt->variant_fp64 >= 0) c->var[0] = opt->variant_fp64;
   if (opt && opt->variant_fp32 >= 0) c->var[1] = opt->variant_fp32;
   c->n_atoms = n_atoms;
@@ -1077,6 +1133,7 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
   c->timing[4] = t;
   HIPCHK(c, hipEventElapsedTime(&t, c->ev[kEvStart], c->ev[kEvEnd]));
   c->timing[5] = t;
+  mark_basis_cached(c, elem);
   if ((rc = collect(c, s, functional))) return rc;
   return pgxc_get_result(c, ham, exc, charge);
 }
@@ -1134,7 +1191,7 @@ int pgxc_destroy(pgxc_ctx *c) {
   void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->klist, c->kcount, c->nbt, c->gsig,
-                  c->sshells, c->d_fpos};
+                  c->sshells, c->d_fpos, c->phic[0], c->phic[1], c->gbits};
   for (void *b : bufs) dfree(b);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1221,13 +1278,14 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
   bool moved = false;
   for (int m = 0; m < nm; ++m) {
     pgxc_ctx *c = b->mols[m];
-    const void *phi0 = c->phi, *w0 = c->W;
+    const void *phi0 = c->phi_base, *w0 = c->W;
+    const int skip0 = c->skip_basis;
     int rc = ensure_work(c, elem, s);
     if (rc) {
       b->err = "molecule " + std::to_string(m) + ": " + c->err;
       return rc;
     }
-    moved = moved || phi0 != c->phi || w0 != c->W;
+    moved = moved || phi0 != c->phi_base || w0 != c->W || skip0 != c->skip_basis;
   }
   if (moved) free_plan(b->plan);
   int rc = get_plan(b->mols, functional, precision, incr, b->prio, b->order_version, b->sched,
@@ -1281,6 +1339,7 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
       return rc;
     }
     c->timing[5] = t;
+    mark_basis_cached(c, elem);
   }
   return PGXC_OK;
 }

@@ -50,13 +50,16 @@ struct UnitD {
   // (klist row stride nfb); for each function tile, a flag per 32-point group
   // of the molecule's grid telling whether the tile can be nonzero there
   // (gsig row stride gstride, offset to the unit's first group)
+  // gbits: per 32-point group of the unit, nfw words of flags, one per
+  // 32-function block that can be nonzero on the group
   const uint16_t *klist;
   const int *kcount;
   const uint8_t *gsig;
+  const uint32_t *gbits;
   long Mc, npad, gstride;
   int n_atoms, NP, ntiles, nsplit, first, incremental;
   uint32_t mask;
-  int screen, nfb, pad;
+  int screen, nfb, nfw;
 };
 
 // Per-molecule descriptors of the operand preparation and the final packing.
@@ -145,7 +148,7 @@ struct ScreenShell {
 void launch_screen(hipStream_t s, long ntiles_pts, const double *gx, const double *gy,
                    const double *gz, int n_atoms, const AtomD *atoms, int n_shells,
                    const ScreenShell *shells, int nfb, int tf, int ntile_f, long gstride,
-                   uint16_t *klist, int *kcount, int *nb, uint8_t *gsig);
+                   uint16_t *klist, int *kcount, int *nb, uint8_t *gsig, uint32_t *gbits);
 
 // P = C diag(occ) C^T into an NP x NP padded array
 void launch_form_density(hipStream_t s, const double *C, const double *occ, int n,

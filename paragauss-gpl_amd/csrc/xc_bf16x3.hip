@@ -216,7 +216,8 @@ __global__ __launch_bounds__(kThreads) void k_density_bf16x3(const StageList S) 
     auto kmap = [&](int kt) -> long { return scr ? (long)kl[kt] * kGroup : (long)kt * kXBK; };
     f4 acc[BM / 32][BN / 32];
     gemm_tile_bf16x3<BM, BN, false, false>(phi + m0, U.Mc, D + n0, U.NP, ksteps, kmap, lds, acc);
-    density_epilogue<float, f4, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn, U.part, red);
+    density_epilogue<float, f4, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn, U.part, red,
+                                              scr ? U.gbits : nullptr, U.nfw);
     item_log(S, it, t0);
   }
 }

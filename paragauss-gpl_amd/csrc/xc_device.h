@@ -177,13 +177,15 @@ __device__ __forceinline__ double group_reduce(double (&v)[M], int lane, int &in
 // reduction) and written to part[tn][c][p].  With the FP32 result placement a lane holds
 // four consecutive points of one column, read as one 16-byte load per
 // component; the FP64 placement reads point by point.  Both orders of the
-// sums are the same.
+// sums are the same.  With gbits (screening), 16-column blocks whose basis
+// values are exactly zero on the 16 points are skipped: they add zeros.
 template <typename T, typename ACC, int BM, int BN, bool GRAD>
 __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 32],
                                                  const T *__restrict__ phi, long Mc, int NP,
                                                  long m0, int n0, int tn,
                                                  double *__restrict__ part,
-                                                 double (&red)[2][4][BM]) {
+                                                 double (&red)[2][4][BM],
+                                                 const uint32_t *__restrict__ gbits, int nfw) {
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int NC = GRAD ? 4 : 1;
   constexpr bool QUAD = MF<T>::kQuadRows;
@@ -199,6 +201,11 @@ __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 
       for (int r = 0; r < 4; ++r) s[c][r] = 0.0;
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
+      if (gbits) {  // 16 columns cut off on these 16 points: exact zeros, skip
+        const long g = (m0 + wm * WM + i * 16) / kGroup;
+        const int b = (n0 + wn * WN + j * 16) / kGroup;
+        if (!((gbits[g * nfw + (b >> 5)] >> (b & 31)) & 1u)) continue;
+      }
       const long n = n0 + wn * WN + j * 16 + mf_col(lane);
       T v[NC][4];
       if constexpr (QUAD) {

@@ -568,7 +568,8 @@ __global__ __launch_bounds__(kThreads) void k_density(const StageList S) {
     gemm_tile<T, BM, BN, BK, ST, false, false>(phi + m0, U.Mc, D + n0, U.NP, ksteps, kmap, As, Bs,
                                                acc);
     density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn,
-                                                          U.part, red);
+                                                          U.part, red, scr ? U.gbits : nullptr,
+                                                          U.nfw);
     item_log(S, it, t0);
   }
 }
@@ -680,6 +681,14 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
     const long Mc = U.Mc, cs = (long)U.NP * Mc;
     const T *phi = (const T *)U.phi;
     T *W = (T *)U.W;
+    if (U.screen) {  // a block cut off on the point's group: W = 0 without reading Phi
+      const long b = f0 / kGroup;
+      if (!((U.gbits[(p / kGroup) * U.nfw + (b >> 5)] >> (b & 31)) & 1u)) {
+#pragma unroll 4
+        for (int k = 0; k < 16; ++k) W[(f0 + k) * Mc + p] = T(0);
+        continue;
+      }
+    }
     const double a = U.coef[p];
     double b0 = 0.0, b1 = 0.0, b2 = 0.0;
     if (GRAD) {
@@ -914,7 +923,8 @@ PGXC_INST(float)
 // group to each atom (FP64, reduced over the group's 32 lanes), then per
 // shell and group alpha_min r^2_min < cut-off marks the shell's function
 // blocks.  Outputs: the tile's ascending block list and count, n_b (functions
-// of the shells marked on the tile) and, per function tile, a flag per group.
+// of the shells marked on the tile), per group its block flags (gbits) and,
+// per function tile, a flag per group.
 __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *__restrict__ gx,
                                                      const double *__restrict__ gy,
                                                      const double *__restrict__ gz, int n_atoms,
@@ -922,7 +932,7 @@ __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *
                                                      const ScreenShell *__restrict__ shells,
                                                      int nfb, int tf, int ntile_f, long gstride,
                                                      uint16_t *klist, int *kcount, int *nb,
-                                                     uint8_t *gsig) {
+                                                     uint8_t *gsig, uint32_t *gbits) {
   constexpr int AC = 512;  // atoms per pass
   __shared__ double r2g[4][AC];
   __shared__ unsigned bits[4][kMaxBlocks / 32];
@@ -973,6 +983,11 @@ __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *
       kcount[t] = cnt;
       nb[t] = nbs;
     }
+    const int nfw = (nfb + 31) / 32;
+    for (int i = tid; i < 4 * nfw; i += kThreads) {
+      const int q = i / nfw, w = i % nfw;
+      gbits[(t * 4 + q) * nfw + w] = bits[q][w];
+    }
     const int bpt = tf / kGroup;
     for (int i = tid; i < 4 * ntile_f; i += kThreads) {
       const int q = i & 3, ft = i >> 2;
@@ -987,11 +1002,11 @@ __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *
 void launch_screen(hipStream_t s, long ntiles_pts, const double *gx, const double *gy,
                    const double *gz, int n_atoms, const AtomD *atoms, int n_shells,
                    const ScreenShell *shells, int nfb, int tf, int ntile_f, long gstride,
-                   uint16_t *klist, int *kcount, int *nb, uint8_t *gsig) {
+                   uint16_t *klist, int *kcount, int *nb, uint8_t *gsig, uint32_t *gbits) {
   if (ntiles_pts <= 0) return;
   hipLaunchKernelGGL(k_screen, dim3(persistent_grid(ntiles_pts, 8)), dim3(kThreads), 0, s,
                      ntiles_pts, gx, gy, gz, n_atoms, atoms, n_shells, shells, nfb, tf, ntile_f,
-                     gstride, klist, kcount, nb, gsig);
+                     gstride, klist, kcount, nb, gsig, gbits);
 }
 
 // ------------------------------------------------------------ small kernels

@@ -41,6 +41,7 @@ module pgxc_binding
      integer(c_int32_t) :: variant_fp64
      integer(c_int32_t) :: variant_fp32
      integer(c_int32_t) :: no_screen
+     integer(c_int32_t) :: no_basis_cache
   end type pgxc_options
 
   interface

@@ -32,7 +32,7 @@ SCHED_STATIC, SCHED_ANALYTIC, SCHED_MODEL = 0, 1, 2
 class Options(ctypes.Structure):
     _fields_ = [("tile", ctypes.c_int32), ("chunk_points", ctypes.c_int64),
                 ("variant_fp64", ctypes.c_int32), ("variant_fp32", ctypes.c_int32),
-                ("no_screen", ctypes.c_int32)]
+                ("no_screen", ctypes.c_int32), ("no_basis_cache", ctypes.c_int32)]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)

@@ -37,10 +37,11 @@ class XCBuilder:
     """tile: function tile of the contractions (64/96/128, 0 = least padding);
     chunk_points: points per work chunk (multiple of 256, 0 = default);
     variants: (FP64, FP32) contraction kernel variants (-1 = default);
-    screen: skip the blocks the radial cut-off makes exactly zero."""
+    screen: skip the blocks the radial cut-off makes exactly zero;
+    basis_cache: keep every chunk's basis values on the device across builds."""
 
     def __init__(self, basis, grid, veclen=128, tile=0, chunk_points=0, variants=(-1, -1),
-                 screen=True):
+                 screen=True, basis_cache=True):
         self._L = _lib.load()
         self.basis = basis
         self.nbas = basis.nbas
@@ -54,7 +55,8 @@ class XCBuilder:
                       np.ascontiguousarray(basis.contr), g]
         axyz, info, ex, nm, ct, g = self._keep
         h = ctypes.c_void_p()
-        opt = _lib.Options(tile, chunk_points, variants[0], variants[1], 0 if screen else 1)
+        opt = _lib.Options(tile, chunk_points, variants[0], variants[1], 0 if screen else 1,
+                           0 if basis_cache else 1)
         rc = self._L.pgxc_create_opt(ctypes.byref(h), basis.n_atoms, _ptr(axyz), basis.n_shells,
                                      info.ctypes.data_as(_ip), _ptr(ex), _ptr(nm), _ptr(ct),
                                      _ptr(g), self.npts, veclen, ctypes.byref(opt))

@@ -123,3 +123,26 @@ def test_screened_work_counts(dimer):
         print("screen=%s flops %.6e expected %.6e" % (screen, got, ref), flush=True)
         assert abs(got - ref) <= 2e-3 * ref  # r^2 rounding may flip a shell at the cut-off
     assert expect < 0.75 * dense  # the cut-off removes a real share of the work here
+
+
+@pytest.mark.parametrize("prec", [FP64, BF16X3])
+def test_basis_cache_reuse_is_bitwise(dimer, prec):
+    """Builds that reuse the cached basis values equal builds that evaluate
+    them, across precision switches and incremental builds."""
+    _, _, b, g, C, occ = dimer
+    C2 = systems.perturbed_orbitals(C, 1e-3, seed=24)
+    out = {}
+    for cache in (True, False):
+        with XCBuilder(b, g, basis_cache=cache, chunk_points=8192) as xc:
+            seq, tb = [], []
+            for Ck, p, incr in ((C, FP64, False), (C, prec, False), (C2, prec, True),
+                                (C, FP64, False), (C2, prec, True)):
+                xc.set_orbitals(Ck, occ, incremental=incr)
+                seq.append(xc.build(PBE, p))
+                tb.append(xc.timing()["basis"])
+            out[cache] = seq
+            print("cache=%s basis stage ms per build: %s" % (cache, np.round(tb, 4)), flush=True)
+            if cache:  # builds 3-5 read cached values: no basis kernel runs
+                assert max(tb[2:]) < 0.5 * tb[0]
+    for (h1, e1, q1), (h2, e2, q2) in zip(out[True], out[False]):
+        assert np.array_equal(h1, h2) and e1 == e2 and q1 == q2
