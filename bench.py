@@ -120,6 +120,20 @@ def cpu_baseline(b, g, C, occ, budget_s):
                       "occupied-orbital density as the reference, %.1f s" % (nsamp, nb, lo, dt)}
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of a kernel, from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size),
+    or None when the summary does not cover it."""
+    path = os.path.join(ROOT, "profiles", "xc_build_h2o32_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_dispatch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def learned_training_set(log, max_items=1 << 18):
     """Per-item timings logged by the library on the H2O, (H2O)32 FP64 and
     (H2O)32 mixed-precision builds: features and log(us)."""
@@ -411,6 +425,20 @@ def main():
         log("%s: %.2f ms/build dE=%.3e dV=%.3e stages %s" % (prec, runs[prec]["ms"], runs[prec]["dE"],
                                                             runs[prec]["dV"], stages[-1]))
     stats = xc.stats()
+    # work the contraction kernels execute (32 x 32 screening blocks, whole
+    # tiles): counted from the item log of one more build of each mixed form
+    executed = {}
+    xc.set_item_log(True)
+    for prec in precs:
+        xc.set_orbitals(chain[-1], occ, incremental=True)
+        xc.build(PBE, PRECISIONS[prec], fetch=False)
+        F, _ = xc.item_log()
+        k = np.exp2(16.0 * F[:, 5]) - 1.0
+        rows = np.where(F[:, 0] == 0, 128.0, F[:, 4] * 128.0)
+        fl = 2.0 * rows * F[:, 4] * 128.0 * k
+        executed[prec] = {"density": float(fl[F[:, 0] == 0].sum()),
+                          "potential": float(fl[F[:, 0] == 1].sum())}
+    xc.set_item_log(False)
     perr = precision_errors(xc, C, occ, precs)
     log("precision errors per ||dP||_F:", perr)
     xc.close()
@@ -429,6 +457,14 @@ def main():
     flops = stats["flops"] / 2.0
     achieved = flops / (st[dom] * 1e-3) / 1e12
     achieved_dense = 2.0 * npts * n * n / (st[dom] * 1e-3) / 1e12
+    achieved_exec = executed[best][dom] / (st[dom] * 1e-3) / 1e12
+    tf = layout["tile"]
+    if best == "bf16x3":
+        kname = "k_%s_bf16x3<%d%s>" % (dom, tf, ",true" if dom == "density" else "")
+    else:
+        v = layout["variant_fp32"]
+        kname = "k_%s<float,%d,%d%s>" % (dom, tf, v, ",true" if dom == "density" else "")
+    traffic = profiled_traffic(kname)
     peak = PEAK["fp32"] if best == "fp32" else PEAK["bf16"] / 3.0
     out = {
         "metric": "XC-build grid pts/s, mixed vs FP64 GPU at |dE_xc|<=1e-6 Eh; %MFMA/HBM peak",
@@ -461,11 +497,15 @@ def main():
         "layout": {k: int(v) for k, v in layout.items()},
         "stages_ms": {k: round(float(v), 3) for k, v in st.items()},
         "fp64_stages_ms": {k: round(float(v), 3) for k, v in st64[-1].items()},
-        "roofline": {"bound": "mfma", "kernel": "k_" + dom + ("_bf16x3" if best == "bf16x3" else ""),
+        "roofline": {"bound": "mfma", "kernel": kname,
                      "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s" + (" (FP32-equivalent: BF16 peak / 3 passes)" if best == "bf16x3" else ""),
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
+                     "traffic_unit": "bytes per dispatch (one grid chunk), rocprofv3 FETCH_SIZE + "
+                                     "WRITE_SIZE scaled by a copy kernel",
                      "achieved_dense_equivalent": achieved_dense,
+                     "achieved_executed": achieved_exec,
+                     "flops_per_build_executed": executed[best],
                      "flops_per_build_algorithmic": stats["flops"],
                      "flops_per_build_dense": 4.0 * npts * n * n + 16.0 * npts * n},
     }
