@@ -71,6 +71,8 @@ struct pgxc_ctx {
   void *phi_base = nullptr;  // Phi of the current build: chunk k at phi_base + k * phi_stride
   size_t phi_stride = 0;
   int skip_basis = 0;        // the current build reads cached basis values
+  int split_valid = 0;       // the FP32 cache holds its split-BF16 planes too
+  int run_split = 0;         // the current build writes the split planes
   double *part = nullptr, *coef = nullptr, *Hpart = nullptr;
   // device: state and results
   double *state = nullptr;                    // [S_NSLOT][npad]
@@ -137,7 +139,7 @@ constexpr int kLists = 3 * kNumVariants;
 
 struct StepPlan {
   int nunits = 0;
-  StageList basis, func, wmat;
+  StageList basis, func, wmat, split;
   StageList dens[kLists], pot[kLists];
 };
 
@@ -203,7 +205,7 @@ void make_order(const std::vector<long> &base, const std::vector<double> &prio,
   }
 }
 
-enum ItemKind { K_BASIS, K_FUNC, K_WMAT, K_DENS, K_POT };
+enum ItemKind { K_BASIS, K_FUNC, K_WMAT, K_DENS, K_POT, K_SPLIT };
 
 long unit_items(const UnitD &U, ItemKind k, int tf) {
   switch (k) {
@@ -211,6 +213,7 @@ long unit_items(const UnitD &U, ItemKind k, int tf) {
     case K_FUNC: return items_functional(U.Mc);
     case K_WMAT: return items_wmat(U.Mc, U.NP);
     case K_DENS: return items_density(U.Mc, U.NP, tf);
+    case K_SPLIT: return items_split(U.Mc, U.NP);
     default: return items_potential(U.NP, tf, U.nsplit);
   }
 }
@@ -375,6 +378,10 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.norms = c->norms;
       U.contr = c->contr;
       U.phi = (char *)c->phi_base + (size_t)k * c->phi_stride;
+      if (precision != PGXC_PREC_FP64) {  // split planes after the four FP32 planes
+        U.phis = (char *)U.phi + 16 * (size_t)c->NP * c->Mc;
+        U.phit = (char *)U.phis + 4 * (size_t)c->NP * c->Mc;
+      }
       U.W = c->W;
       U.D = c->D;
       U.part = c->part;
@@ -431,6 +438,17 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
         }
         pend[k].push_back(stage(sub, subm, kind == 0 ? K_DENS : K_POT, kTiles[t], k));
       }
+    {  // split-BF16 operand planes of the basis values, unless already cached
+      std::vector<UnitD> su;
+      std::vector<int> sm;
+      if (precision == PGXC_PREC_BF16X3)
+        for (size_t i = 0; i < units.size(); ++i)
+          if (mols[umol[i]]->run_split) {
+            su.push_back(units[i]);
+            sm.push_back(umol[i]);
+          }
+      pend[k].push_back(stage(su, sm, K_SPLIT, 0, k));
+    }
   }
   // queue bounds: each list's positions (natural order, so neighbouring items
   // keep sharing operand panels) cut into kQueues runs of equal predicted cost
@@ -522,6 +540,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     S.wmat = mk(pend[k][2]);
     for (int li = 0; li < kLists; ++li) S.dens[li] = mk(pend[k][3 + li]);
     for (int li = 0; li < kLists; ++li) S.pot[li] = mk(pend[k][3 + kLists + li]);
+    S.split = mk(pend[k][3 + 2 * kLists]);
     S.nunits = pend[k][0].nu;
   }
   plan->d_pack = (PackD *)(B + pack_off);
@@ -571,12 +590,13 @@ static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hip
   const bool x3 = pl.precision == PGXC_PREC_BF16X3;
   if (pl.nctr) launch_zero_ints(s, pl.d_ctr, pl.nctr);
   launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total,
-              !std::is_same<T, double>::value, false);
+              x3 ? 2 : std::is_same<T, double>::value ? 0 : 1, false);
   for (size_t k = 0; k < pl.steps.size(); ++k) {
     const StepPlan &S = pl.steps[k];
     hipEvent_t *E = ev ? ev->data() + kEvStep + 5 * k : nullptr;
     if (E) (void)hipEventRecord(E[0], s);
     launch_basis<T>(s, gga, S.basis);
+    if (x3) launch_split(s, S.split);
     if (E) (void)hipEventRecord(E[1], s);
     for (int li = 0; li < kLists; ++li) {
       const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;
@@ -590,7 +610,7 @@ static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hip
     }
     if (E) (void)hipEventRecord(E[2], s);
     launch_functional(s, gga, S.func);
-    launch_wmat<T>(s, gga, S.wmat);
+    launch_wmat<T>(s, gga, x3, S.wmat);
     if (E) (void)hipEventRecord(E[3], s);
     for (int li = 0; li < kLists; ++li) {
       const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;
@@ -606,12 +626,16 @@ static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hip
   }
   if (ev) (void)hipEventRecord((*ev)[kEvPack], s);
   launch_pack(s, pl.d_pack, pl.nmol, pl.d_pack_base, pl.pack_total);
-  launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total, false, true);
+  launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total, 0, true);
   if (ev) (void)hipEventRecord((*ev)[kEvEnd], s);
 }
 
-static int ensure_work(pgxc_ctx *c, size_t elem, hipStream_t s) {
-  const size_t need_phi = 4 * (size_t)c->NP * c->Mc * elem;
+// bytes of basis values per (function, point): 4 planes, plus for FP32 the
+// split-BF16 planes in both layouts (2 x 2 x 2 bytes)
+static size_t phi_elem_bytes(size_t elem) { return elem == 8 ? 32 : 24; }
+
+static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
+  const size_t need_phi = phi_elem_bytes(elem) * (size_t)c->NP * c->Mc;
   const size_t need_w = (size_t)c->NP * c->Mc * elem;
   bool moved = false;
   if (c->w_bytes < need_w) {
@@ -659,18 +683,23 @@ static int ensure_work(pgxc_ctx *c, size_t elem, hipStream_t s) {
   void *base = c->phic[e] ? c->phic[e] : c->phi;
   const size_t stride = c->phic[e] ? need_phi : 0;
   const int skip = c->phic[e] && c->phic_valid[e];
-  if (base != c->phi_base || stride != c->phi_stride || skip != c->skip_basis) moved = true;
+  const int rs = precision == PGXC_PREC_BF16X3 && !(c->phic[1] && c->split_valid);
+  if (base != c->phi_base || stride != c->phi_stride || skip != c->skip_basis ||
+      rs != c->run_split)
+    moved = true;
   c->phi_base = base;
   c->phi_stride = stride;
   c->skip_basis = skip;
+  c->run_split = rs;
   if (moved) free_plan(c->plan);  // its unit descriptors hold the old pointers
   return PGXC_OK;
 }
 
 // after a build: the cache of this element type now holds valid values
-static void mark_basis_cached(pgxc_ctx *c, size_t elem) {
+static void mark_basis_cached(pgxc_ctx *c, size_t elem, int precision) {
   const int e = elem == 8 ? 0 : 1;
   if (c->phic[e]) c->phic_valid[e] = 1;
+  if (c->phic[1] && precision == PGXC_PREC_BF16X3) c->split_valid = 1;
 }
 
 static int validate_build(pgxc_ctx *c, uint32_t functional, int32_t precision, int &incr) {
@@ -1103,7 +1132,7 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
   const bool gga = (functional & (PGXC_XC_PBEX | PGXC_XC_PBEC)) != 0;
   hipStream_t s = c->stream;
   const size_t elem = precision == PGXC_PREC_FP64 ? 8 : 4;
-  if ((rc = ensure_work(c, elem, s))) return rc;
+  if ((rc = ensure_work(c, elem, precision, s))) return rc;
   std::vector<pgxc_ctx *> mols{c};
   std::vector<int> incrv{incr};
   if ((rc = get_plan(mols, functional, precision, incrv, {}, -1, c->sched, c->plan, c))) return rc;
@@ -1133,7 +1162,7 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
   c->timing[4] = t;
   HIPCHK(c, hipEventElapsedTime(&t, c->ev[kEvStart], c->ev[kEvEnd]));
   c->timing[5] = t;
-  mark_basis_cached(c, elem);
+  mark_basis_cached(c, elem, precision);
   if ((rc = collect(c, s, functional))) return rc;
   return pgxc_get_result(c, ham, exc, charge);
 }
@@ -1279,13 +1308,14 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
   for (int m = 0; m < nm; ++m) {
     pgxc_ctx *c = b->mols[m];
     const void *phi0 = c->phi_base, *w0 = c->W;
-    const int skip0 = c->skip_basis;
-    int rc = ensure_work(c, elem, s);
+    const int skip0 = c->skip_basis, split0 = c->run_split;
+    int rc = ensure_work(c, elem, precision, s);
     if (rc) {
       b->err = "molecule " + std::to_string(m) + ": " + c->err;
       return rc;
     }
-    moved = moved || phi0 != c->phi_base || w0 != c->W || skip0 != c->skip_basis;
+    moved = moved || phi0 != c->phi_base || w0 != c->W || skip0 != c->skip_basis ||
+            split0 != c->run_split;
   }
   if (moved) free_plan(b->plan);
   int rc = get_plan(b->mols, functional, precision, incr, b->prio, b->order_version, b->sched,
@@ -1339,7 +1369,7 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
       return rc;
     }
     c->timing[5] = t;
-    mark_basis_cached(c, elem);
+    mark_basis_cached(c, elem, precision);
   }
   return PGXC_OK;
 }

@@ -38,8 +38,10 @@ struct UnitD {
   const ShellD *shells;
   const double *exps, *norms, *contr;
   void *phi;        // [4][NP][Mc] values, d/dx, d/dy, d/dz (T)
-  void *W;          // [NP][Mc] (T)
-  const void *D;    // [NP][NP] density operand (T)
+  void *phis;       // split-BF16 builds: values as bf16 hi / lo planes [2][NP][Mc]
+  void *phit;       //   and transposed, [2][Mc][NP]
+  void *W;          // [NP][Mc] (T); split-BF16 builds: bf16 hi / lo planes [2][NP][Mc]
+  const void *D;    // [NP][NP] density operand (T); split-BF16: transposed hi / lo planes
   double *part;     // [NP/tf][4][Mc] density partials
   double *coef;     // [4][Mc] potential coefficients a, b
   double *state;    // [S_NSLOT][npad] molecule state, offset to the unit's first point
@@ -102,7 +104,9 @@ template <typename T>
 void launch_density(hipStream_t s, bool grad, int tf, int var, const StageList &L);
 void launch_functional(hipStream_t s, bool gga, const StageList &L);
 template <typename T>
-void launch_wmat(hipStream_t s, bool grad, const StageList &L);
+void launch_wmat(hipStream_t s, bool grad, bool split, const StageList &L);
+// split-BF16 operand planes of the basis values (item = 32 functions x 64 points)
+void launch_split(hipStream_t s, const StageList &L);
 template <typename T>
 void launch_potential(hipStream_t s, int tf, int var, const StageList &L);
 void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L);
@@ -114,8 +118,9 @@ unsigned contraction_grid(const StageList &L, int per_cu);
 // cost model internals used by the planner (cost_model.hip)
 long costmodel_version(const pgxc_costmodel *m);
 int costmodel_features(const pgxc_costmodel *m);
+// mode of the density operand: 0 FP64, 1 FP32, 2 split-BF16 planes of D^T
 void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, long total,
-                 bool to_float, bool finish);
+                 int mode, bool finish);
 
 // item counts per unit, shared by the planner and the kernels' decoding
 inline long items_basis(long Mc, int n_atoms) { return (Mc / kThreads) * n_atoms; }
@@ -123,6 +128,7 @@ inline long items_density(long Mc, int NP, int tf) { return (Mc / 128) * (NP / t
 inline long items_functional(long Mc) { return Mc / kThreads; }
 inline long items_wmat(long Mc, int NP) { return (Mc / kThreads) * (NP / 16); }
 inline long items_potential(int NP, int tf, int nsplit) { return (long)(NP / tf) * (NP / tf) * nsplit; }
+inline long items_split(long Mc, int NP) { return (Mc / 64) * (NP / 32); }
 // points per K step of the potential contraction across all variants
 constexpr int kSplitStep = 32;
 

@@ -1,29 +1,26 @@
 // Split-BF16 contractions ("bf16x3") for gfx950.
 //
-// Every FP32 operand x is split while it is staged into LDS as
+// Every FP32 operand x is split as
 //   hi = bf16(x), lo = bf16(x - hi)
 // and a product is formed from three v_mfma_f32_16x16x32_bf16 passes,
 //   a*b ~ a_lo*b_hi + a_hi*b_lo + a_hi*b_hi,   FP32 accumulation,
-// dropping a_lo*b_lo (relative 2^-16).  Same tiles and epilogues as the
-// FP32/FP64 contractions (xc_kernels.hip); K advances 32 per MFMA.
+// dropping a_lo*b_lo (relative 2^-16).  The build stores its operands already
+// split, k-contiguous (basis values once per geometry by k_split, W by
+// k_wmat, D^T by k_prep), so staging is a copy of 16-byte vectors into LDS.
+// Same tiles and epilogues as the FP32/FP64 contractions (xc_kernels.hip);
+// K advances 32 per MFMA.
 #include "pgxc_internal.h"
 #include "xc_device.h"
 
 namespace pgxc {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float float4_ __attribute__((ext_vector_type(4)));
 
 // 16x16x32 bf16 operand placement (wave64): A(i,k) in lane i + 16 (k / 8),
 // element k % 8; B(k,j) in lane j + 16 (k / 8), element k % 8.  Result as the
 // FP32 16x16x4 MFMA.
 __device__ __forceinline__ f4 mfma_bf16(bf16x8 a, bf16x8 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-  hi = (__bf16)x;
-  lo = (__bf16)(x - (float)hi);
 }
 
 constexpr int kXBK = 32;       // K per stage
@@ -34,97 +31,73 @@ struct XStage {
   static constexpr int n = 2 * BM * kXBKP + 2 * BN * kXBKP;
 };
 
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-// Staging of one operand tile of R rows x 32 K into the split LDS planes
-// [row][k] (k contiguous, rows padded to kXBKP).  Work unit: a 4 x 4 block
-// (4 rows x 4 k) per thread, read as four 16-byte loads along the contiguous
-// dimension (KC: k contiguous, else rows contiguous, then transposed in
-// registers) and written as 8-byte hi / lo vectors.
-template <int R, bool KC>
-struct XOperand {
-  static constexpr int NB = R * kXBK / 16;              // 4 x 4 blocks
-  static constexpr int PER = (NB + kThreads - 1) / kThreads;
-  float4_ v[PER][4];
+// Staging of an operand stored split and k-contiguous: X(r,k) = hi / lo
+// [r*s + k]; per stage R rows x 32 k x 2 planes, moved as 16-byte vectors.
+template <int R>
+struct PSOperand {
+  struct Src {
+    const __bf16 *hi, *lo;
+    long s;
+  };
+  static constexpr int NV = R * (kXBK / 8) * 2;  // 16-byte vectors per stage
+  static constexpr int PER = NV / kThreads;
+  static_assert(PER * kThreads == NV, "whole vectors per thread");
+  u32x4 v[PER];
 
-  // Consecutive threads take consecutive k blocks of one row block for both
-  // layouts: their 8-byte LDS writes land on distinct banks (rows are 20
-  // dwords apart), and each group of 8 lanes still reads whole 128-byte lines
-  // (a row run when k is contiguous, a column run of 4 k columns otherwise).
-  __device__ __forceinline__ void block_rc(int blk, int &r4, int &k4) const {
-    k4 = blk % (kXBK / 4);
-    r4 = blk / (kXBK / 4);
+  __device__ __forceinline__ static void place(int idx, int &plane, int &row, int &c) {
+    plane = idx / (R * 4);
+    const int rem = idx % (R * 4);
+    row = rem >> 2;
+    c = rem & 3;
   }
-  // X(r,k) = X[r*s + k] if KC else X[r + k*s]; kb = first k of the slab
-  __device__ __forceinline__ void load(const float *__restrict__ X, long s, long kb) {
+  __device__ __forceinline__ void load(const Src &S, long kb) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const int blk = threadIdx.x + kThreads * e;
-      if (NB % kThreads != 0 && blk >= NB) break;
-      int r4, k4;
-      block_rc(blk, r4, k4);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (KC)
-          v[e][i] = *(const float4_ *)(X + (long)(4 * r4 + i) * s + kb + 4 * k4);
-        else
-          v[e][i] = *(const float4_ *)(X + 4 * r4 + (kb + 4 * k4 + i) * s);
-      }
+      int plane, row, c;
+      place(threadIdx.x + kThreads * e, plane, row, c);
+      const __bf16 *P = plane ? S.lo : S.hi;
+      v[e] = *(const u32x4 *)(P + (long)row * S.s + kb + 8 * c);
     }
   }
   __device__ __forceinline__ void store(__bf16 *hi, __bf16 *lo) const {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const int blk = threadIdx.x + kThreads * e;
-      if (NB % kThreads != 0 && blk >= NB) break;
-      int r4, k4;
-      block_rc(blk, r4, k4);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        bf16x4 h, l;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const float x = KC ? v[e][i][t] : v[e][t][i];
-          __bf16 hh, ll;
-          split_bf16(x, hh, ll);
-          h[t] = hh;
-          l[t] = ll;
-        }
-        const int off = (4 * r4 + i) * kXBKP + 4 * k4;
-        *(bf16x4 *)(hi + off) = h;
-        *(bf16x4 *)(lo + off) = l;
-      }
+      int plane, row, c;
+      place(threadIdx.x + kThreads * e, plane, row, c);
+      *(u32x4 *)((plane ? lo : hi) + row * kXBKP + 8 * c) = v[e];
     }
   }
 };
 
-// AKC: A(m,k) = A[m*sA + k] (k contiguous) else A[m + k*sA]; same for B(k,n).
+// OA / OB: operand staging policies (PSOperand<R>).
 // K step kt starts at k = kmap(kt).  Pipeline: two LDS stages and two
 // register slots; the global loads of step kt + 2 are issued before the MFMAs
 // of step kt, so each load has two steps of MFMA work to arrive in.
-template <int BM, int BN, bool AKC, bool BKC, class KMap>
-__device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, long sA,
-                                                 const float *__restrict__ B, long sB,
-                                                 int ksteps, KMap kmap, __bf16 *lds,
+template <int BM, int BN, class OA, class OB, class KMap>
+__device__ __forceinline__ void gemm_tile_bf16x3(const typename OA::Src &A,
+                                                 const typename OB::Src &B, int ksteps,
+                                                 KMap kmap, __bf16 *lds,
                                                  f4 (&acc)[BM / 32][BN / 32]) {
   constexpr int BKP = kXBKP, WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int SA = BM * BKP, SB = BN * BKP, STAGE = 2 * SA + 2 * SB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  XOperand<BM, AKC> ra0, ra1;
-  XOperand<BN, BKC> rb0, rb1;
+  OA ra0, ra1;
+  OB rb0, rb1;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
   if (ksteps <= 0) return;
-  auto load = [&](int kt, XOperand<BM, AKC> &ra, XOperand<BN, BKC> &rb) {
+  auto load = [&](int kt, OA &ra, OB &rb) {
     const long kb = kmap(kt);
-    ra.load(A, sA, kb);
-    rb.load(B, sB, kb);
+    ra.load(A, kb);
+    rb.load(B, kb);
   };
-  auto store = [&](int buf, const XOperand<BM, AKC> &ra, const XOperand<BN, BKC> &rb) {
+  auto store = [&](int buf, const OA &ra, const OB &rb) {
     __bf16 *ah = lds + buf * STAGE, *al = ah + SA, *bh = al + SA, *bl = bh + SB;
     ra.store(ah, al);
     rb.store(bh, bl);
@@ -155,8 +128,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float *__restrict__ A, lo
       }
   };
   // step kt lives in register slot kt & 1 and LDS stage kt & 1
-  auto step = [&](int kt, XOperand<BM, AKC> &ra_cur, XOperand<BN, BKC> &rb_cur,
-                  XOperand<BM, AKC> &ra_nxt, XOperand<BN, BKC> &rb_nxt) {
+  auto step = [&](int kt, OA &ra_cur, OB &rb_cur, OA &ra_nxt, OB &rb_nxt) {
     if (kt + 2 < ksteps) load(kt + 2, ra_cur, rb_cur);  // slot of kt is free: kt is in LDS
     compute(kt & 1);
     if (kt + 1 < ksteps) store((kt + 1) & 1, ra_nxt, rb_nxt);
@@ -201,7 +173,7 @@ __global__ __launch_bounds__(kThreads) void k_density_bf16x3(const StageList S) 
     const int ntn = U.NP / BN;
     const long m0 = (local / ntn) * BM;
     const int tn = (int)(local % ntn), n0 = tn * BN;
-    const float *phi = (const float *)U.phi, *D = (const float *)U.D;
+    const float *phi = (const float *)U.phi;
     bool scr = U.screen != 0;
     int ksteps = U.NP / kXBK;
     if (scr) {
@@ -215,7 +187,12 @@ __global__ __launch_bounds__(kThreads) void k_density_bf16x3(const StageList S) 
     }
     auto kmap = [&](int kt) -> long { return scr ? (long)kl[kt] * kGroup : (long)kt * kXBK; };
     f4 acc[BM / 32][BN / 32];
-    gemm_tile_bf16x3<BM, BN, false, false>(phi + m0, U.Mc, D + n0, U.NP, ksteps, kmap, lds, acc);
+    // A(p, k) = Phi^T rows of the point tile; B(k, n) = D^T rows of the function tile
+    const __bf16 *th = (const __bf16 *)U.phit, *dh = (const __bf16 *)U.D;
+    const long np2 = (long)U.NP * U.NP, pn = (long)U.Mc * U.NP;
+    const typename PSOperand<BM>::Src sa{th + m0 * U.NP, th + pn + m0 * U.NP, U.NP};
+    const typename PSOperand<BN>::Src sb{dh + (long)n0 * U.NP, dh + np2 + (long)n0 * U.NP, U.NP};
+    gemm_tile_bf16x3<BM, BN, PSOperand<BM>, PSOperand<BN>>(sa, sb, ksteps, kmap, lds, acc);
     density_epilogue<float, f4, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn, U.part, red,
                                               scr ? U.gbits : nullptr, U.nfw);
     item_log(S, it, t0);
@@ -263,14 +240,65 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
         U.screen ? potential_screen(U, kb, kdense * kXBK, i0 / BT, j0 / BT, gl, wtot) : kdense;
     const bool scr = ksteps < kdense;  // a complete list is the dense walk
     auto kmap = [&](int kt) -> long { return scr ? (long)gl[kt] * kGroup : (long)kt * kXBK; };
-    const float *W = (const float *)U.W, *phi = (const float *)U.phi;
+    const __bf16 *wh = (const __bf16 *)U.W, *ph = (const __bf16 *)U.phis;
+    const long pn = (long)U.NP * Mc;
+    const typename PSOperand<BT>::Src sa{wh + (long)i0 * Mc + kb, wh + pn + (long)i0 * Mc + kb, Mc};
+    const typename PSOperand<BT>::Src sb{ph + (long)j0 * Mc + kb, ph + pn + (long)j0 * Mc + kb, Mc};
     f4 acc[BT / 32][BT / 32];
-    gemm_tile_bf16x3<BT, BT, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb, Mc,
-                                         ksteps, kmap, lds, acc);
+    gemm_tile_bf16x3<BT, BT, PSOperand<BT>, PSOperand<BT>>(sa, sb, ksteps, kmap, lds, acc);
     potential_epilogue<float, f4, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP, U.NP, i0, j0,
                                           U.first);
     item_log(S, it, t0);
   }
+}
+
+// ------------------------------------------------------------------ split
+// item = (32 functions, 64 points) of a unit: the FP32 basis values become
+// bf16 hi / lo planes in the value layout [n][p] (B operand of the potential
+// contraction) and transposed [p][n] (A operand of the density contraction),
+// through an LDS tile so both writes run along their contiguous dimension.
+__global__ __launch_bounds__(kThreads) void k_split(const StageList S) {
+  __shared__ float t[32][65];
+  const int tid = threadIdx.x;
+  for (long L = blockIdx.x; L < S.total; L += gridDim.x) {
+    const int u = find_unit_x3(S.base, S.nu, L);
+    const UnitD &U = S.units[u];
+    const long local = L - S.base[u];
+    const int nfbk = U.NP / 32;
+    const long n0 = (local % nfbk) * 32, p0 = (local / nfbk) * 64;
+    const long Mc = U.Mc, NP = U.NP, pn = NP * Mc;
+    const float *phi = (const float *)U.phi;
+    __bf16 *sh = (__bf16 *)U.phis, *th = (__bf16 *)U.phit;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = tid + kThreads * e, r = idx >> 6, c = idx & 63;
+      const long q = (n0 + r) * Mc + p0 + c;
+      const float v = phi[q];
+      t[r][c] = v;
+      __bf16 hi, lo;
+      split_bf16(v, hi, lo);
+      sh[q] = hi;
+      sh[pn + q] = lo;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = tid + kThreads * e, r = idx >> 5, c = idx & 31;
+      const long q = (p0 + r) * NP + n0 + c;
+      __bf16 hi, lo;
+      split_bf16(t[c][r], hi, lo);
+      th[q] = hi;
+      th[pn + q] = lo;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_split(hipStream_t s, const StageList &L) {
+  if (!L.total) return;
+  const long cap = 256L * 8;
+  const unsigned g = (unsigned)(L.total < cap ? L.total : cap);
+  hipLaunchKernelGGL(k_split, dim3(g), dim3(kThreads), 0, s, L);
 }
 
 void launch_potential_bf16x3(hipStream_t s, int tf, const StageList &L) {

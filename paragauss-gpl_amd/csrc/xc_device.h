@@ -46,6 +46,12 @@ __device__ __forceinline__ void item_log(const StageList &S, long it, uint64_t t
   if (threadIdx.x == 0) S.tlog[it] = (uint32_t)(wall_clock64() - t0);
 }
 
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi)
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
 template <typename T>
 struct MF;
 // 16x16x4 MFMA operand/result placement (wave64), both types:

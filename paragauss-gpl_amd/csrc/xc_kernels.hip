@@ -11,6 +11,8 @@
 //   k_wmat        W = a Phi + b . grad Phi  (modules/xc_hamiltonian.f90:1429-1483)
 //   k_potential   H = W^T Phi on MFMA, split over points  (:1528-1531)
 //   k_pack        V = H + H^T into the packed lower triangle (:1561-1567)
+#include <type_traits>
+
 #include "pgxc_internal.h"
 #include "xc_device.h"
 
@@ -665,8 +667,9 @@ __global__ __launch_bounds__(kThreads) void k_functional(const UnitD *__restrict
 }
 
 // -------------------------------------------------------------------- W
-// item = (256-point block, 16-function group): W = a Phi + b . grad Phi
-template <typename T, bool GRAD>
+// item = (256-point block, 16-function group): W = a Phi + b . grad Phi;
+// SPLIT (split-BF16 builds, T = float): W stored as bf16 hi / lo planes
+template <typename T, bool GRAD, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ units, int nu,
                                                    const long *__restrict__ base,
                                                    const int *__restrict__ order, long total) {
@@ -681,11 +684,22 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
     const long Mc = U.Mc, cs = (long)U.NP * Mc;
     const T *phi = (const T *)U.phi;
     T *W = (T *)U.W;
+    __bf16 *Wh = (__bf16 *)U.W, *Wl = Wh + cs;
+    auto put = [&](long q, double v) {
+      if constexpr (SPLIT) {
+        __bf16 hi, lo;
+        split_bf16((float)v, hi, lo);
+        Wh[q] = hi;
+        Wl[q] = lo;
+      } else {
+        W[q] = (T)v;
+      }
+    };
     if (U.screen) {  // a block cut off on the point's group: W = 0 without reading Phi
       const long b = f0 / kGroup;
       if (!((U.gbits[(p / kGroup) * U.nfw + (b >> 5)] >> (b & 31)) & 1u)) {
 #pragma unroll 4
-        for (int k = 0; k < 16; ++k) W[(f0 + k) * Mc + p] = T(0);
+        for (int k = 0; k < 16; ++k) put((f0 + k) * Mc + p, 0.0);
         continue;
       }
     }
@@ -702,7 +716,7 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
       double v = a * (double)phi[q];
       if (GRAD)
         v = v + b0 * (double)phi[cs + q] + b1 * (double)phi[2 * cs + q] + b2 * (double)phi[3 * cs + q];
-      W[q] = (T)v;
+      put(q, v);
     }
   }
 }
@@ -778,7 +792,7 @@ __global__ __launch_bounds__(kThreads) void k_pack(const PackD *__restrict__ mol
 // contraction precision; afterwards P_ref = P.  item = 256 elements.
 __global__ __launch_bounds__(kThreads) void k_prep(const PackD *__restrict__ mols, int nm,
                                                    const long *__restrict__ base, long total,
-                                                   int to_float, int finish) {
+                                                   int mode, int finish) {
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const int m = find_unit(base, nm, L);
     const PackD &M = mols[m];
@@ -788,10 +802,17 @@ __global__ __launch_bounds__(kThreads) void k_prep(const PackD *__restrict__ mol
       M.Pref[i] = M.P[i];
     } else {
       const double d = M.incremental ? M.P[i] - M.Pref[i] : M.P[i];
-      if (to_float)
+      if (mode == 2) {  // split planes of D^T: element (r, c) goes to (c, r)
+        const long np2 = M.NP * M.NP, r = i / M.NP, c = i % M.NP;
+        __bf16 hi, lo;
+        split_bf16((float)d, hi, lo);
+        ((__bf16 *)M.D)[c * M.NP + r] = hi;
+        ((__bf16 *)M.D)[np2 + c * M.NP + r] = lo;
+      } else if (mode == 1) {
         ((float *)M.D)[i] = (float)d;
-      else
+      } else {
         ((double *)M.D)[i] = d;
+      }
     }
   }
 }
@@ -857,13 +878,22 @@ void launch_functional(hipStream_t s, bool gga, const StageList &L) {
 }
 
 template <typename T>
-void launch_wmat(hipStream_t s, bool grad, const StageList &L) {
+void launch_wmat(hipStream_t s, bool grad, bool split, const StageList &L) {
   if (!L.total) return;
   dim3 g(persistent_grid(L.total, 8));
+  if constexpr (std::is_same<T, float>::value) {
+    if (split) {
+      if (grad)
+        hipLaunchKernelGGL((k_wmat<T, true, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+      else
+        hipLaunchKernelGGL((k_wmat<T, false, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+      return;
+    }
+  }
   if (grad)
-    hipLaunchKernelGGL((k_wmat<T, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+    hipLaunchKernelGGL((k_wmat<T, true, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
   else
-    hipLaunchKernelGGL((k_wmat<T, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+    hipLaunchKernelGGL((k_wmat<T, false, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
 }
 
 template <typename T, int VAR>
@@ -903,16 +933,16 @@ void launch_pack(hipStream_t s, const PackD *mols, int nm, const long *base, lon
 }
 
 void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, long total,
-                 bool to_float, bool finish) {
+                 int mode, bool finish) {
   if (!total) return;
   hipLaunchKernelGGL(k_prep, dim3(persistent_grid(total, 8)), dim3(kThreads), 0, s, mols, nm, base,
-                     total, (int)to_float, (int)finish);
+                     total, mode, (int)finish);
 }
 
 #define PGXC_INST(T)                                                                  \
   template void launch_basis<T>(hipStream_t, bool, const StageList &);               \
   template void launch_density<T>(hipStream_t, bool, int, int, const StageList &);   \
-  template void launch_wmat<T>(hipStream_t, bool, const StageList &);                \
+  template void launch_wmat<T>(hipStream_t, bool, bool, const StageList &);          \
   template void launch_potential<T>(hipStream_t, int, int, const StageList &);
 PGXC_INST(double)
 PGXC_INST(float)
