@@ -79,7 +79,7 @@ typedef struct pgxc_costmodel pgxc_costmodel;
 /* Layout options of a context (pgxc_create_opt).  tile: function tile of the
  * MFMA contractions, 64, 96 or 128 (0: the one padding nbas least).
  * chunk_points: largest number of grid points held in one work chunk, a
- * multiple of 256 (0: as many as keep the FP64 basis values under 2 GiB).
+ * multiple of 2048 (0: at most 256 Ki points and 8 GiB of FP64 basis values).
  * variant_fp64 / variant_fp32: contraction kernel variant for FP64 and FP32
  * builds, 0 (K step 16, two LDS stages), 1 (K step 32, one LDS stage),
  * 2 (K step 32, two LDS stages); -1 keeps the default.  pgxc/autotune.py picks

@@ -87,20 +87,26 @@ struct pgxc_ctx {
   std::vector<hipEvent_t> ev;
   Plan *plan = nullptr;
   double exps_per_point = 0;
-  // screening tables over the padded grid (see launch_screen)
-  int screen = 0, nfb = 0;
-  long gstride = 0;
-  uint16_t *klist = nullptr;
-  int *kcount = nullptr, *nbt = nullptr;
-  uint8_t *gsig = nullptr;
-  uint32_t *gbits = nullptr;
-  int nfw = 0;
+  // screening and work tables over the padded grid (setup_screen)
+  int screen = 0, n8 = 0, nfw = 0, CH = 0;
+  int *nbt = nullptr;
+  uint32_t *sig8 = nullptr, *gbits = nullptr;
   ScreenShell *sshells = nullptr;
   double flops_screened = 0, flops_dense = 0;  // algorithmic work of one build
+  // host and device copies: per tile run lists, per split run unions and
+  // [group][chunk] flags, per chunk density and potential items
+  std::vector<int> h_tl_off, h_su_off;
+  std::vector<uint16_t> h_tl, h_su;
+  std::vector<long> h_gcf_off, h_di_off, h_pi_off;
+  std::vector<uint8_t> h_gcf;
+  std::vector<uint32_t> h_di, h_pi;
+  int *tl_off = nullptr, *su_off = nullptr;
+  uint16_t *tl = nullptr, *su = nullptr;
+  long *gcf_off = nullptr;
+  uint8_t *gcf = nullptr;
+  uint32_t *di = nullptr, *pi = nullptr;
   std::vector<int> fpos;          // internal position of each caller function
   int *d_fpos = nullptr;
-  std::vector<int> h_kcount;     // host copies of the screening tables
-  std::vector<uint8_t> h_gsig;
   Sched sched;
 };
 
@@ -153,6 +159,7 @@ struct Plan {
   long model_version = -1;
   int *d_ctr = nullptr;  // queue counters of every contraction list
   long nctr = 0;
+  std::vector<std::pair<int *, long>> hzero;  // H partials, cleared at the start
   // item log: features of every contraction item (row-major), and per list
   // the device log and the list's first feature row
   std::vector<double> feat;
@@ -212,22 +219,22 @@ long unit_items(const UnitD &U, ItemKind k, int tf) {
     case K_BASIS: return items_basis(U.Mc, U.n_atoms);
     case K_FUNC: return items_functional(U.Mc);
     case K_WMAT: return items_wmat(U.Mc, U.NP);
-    case K_DENS: return items_density(U.Mc, U.NP, tf);
+    case K_DENS: return U.ndi;
     case K_SPLIT: return items_split(U.Mc, U.NP);
-    default: return items_potential(U.NP, tf, U.nsplit);
+    default: return U.npi;
   }
 }
 
 }  // namespace
 
 // Features (PGXC_NFEATURES) and counted flops of the contraction items of
-// one unit, in the items' natural order.  kact: K length left after
-// screening; kdense: the dense K length.
+// one unit, in the items' order.  kact: K length left after screening
+// (gathered functions of a density item, points of a potential item);
+// kdense: the dense K length.
 static void unit_item_features(const pgxc_ctx *c, int step, ItemKind kind, int precision,
                                std::vector<double> &feat, std::vector<double> &flops) {
-  const int tf = c->tf, ntn = c->NP / tf;
+  const int tf = c->tf, CH = c->CH;
   const int var = precision == PGXC_PREC_FP64 ? c->var[0] : precision == PGXC_PREC_FP32 ? c->var[1] : 0;
-  const long p0 = (long)step * c->Mc;
   auto push = [&](int kd, double kact, double kdense, bool active, double fl) {
     const double f[PGXC_NFEATURES] = {(double)kd,
                                       precision == PGXC_PREC_FP64 ? 1.0 : 0.0,
@@ -241,38 +248,28 @@ static void unit_item_features(const pgxc_ctx *c, int step, ItemKind kind, int p
     feat.insert(feat.end(), f, f + PGXC_NFEATURES);
     flops.push_back(fl);
   };
-  const bool scr = c->screen != 0;
   if (kind == K_DENS) {
-    const long ntile = c->Mc / kScreenTile;
-    for (long t = 0; t < ntile; ++t) {
-      const long T = p0 / kScreenTile + t;
-      for (int tn = 0; tn < ntn; ++tn) {
-        bool active = true;
-        double kact = c->NP;
-        if (scr) {
-          const uint8_t *g = c->h_gsig.data() + (size_t)tn * c->gstride + T * (kScreenTile / kGroup);
-          active = (g[0] | g[1] | g[2] | g[3]) != 0;
-          kact = active ? (double)c->h_kcount[(size_t)T] * kGroup : 0.0;
-        }
-        push(0, kact, c->NP, active, 2.0 * kScreenTile * tf * kact);
-      }
+    const long T0 = (long)step * c->Mc / kScreenTile;
+    for (long i = c->h_di_off[step]; i < c->h_di_off[step + 1]; ++i) {
+      const uint32_t e = c->h_di[(size_t)i];
+      const long T = T0 + (e >> 8);
+      const int ch = (int)(e & 255), c8 = c->h_tl_off[(size_t)T + 1] - c->h_tl_off[(size_t)T];
+      const double cols = (double)kRun * std::min(CH, c8 - ch * CH);
+      push(0, (double)c8 * kRun, c->NP, true, 2.0 * kScreenTile * cols * c8 * kRun);
     }
   } else {
     const long spts = c->Mc / c->nsplit, G = spts / kGroup;
-    for (int sl = 0; sl < c->nsplit; ++sl) {
-      const long g0 = p0 / kGroup + sl * G;
-      for (int ti = 0; ti < ntn; ++ti)
-        for (int tj = 0; tj < ntn; ++tj) {
-          double kact = (double)spts;
-          if (scr) {
-            const uint8_t *a = c->h_gsig.data() + (size_t)ti * c->gstride + g0;
-            const uint8_t *b = c->h_gsig.data() + (size_t)tj * c->gstride + g0;
-            long cnt = 0;
-            for (long g = 0; g < G; ++g) cnt += (a[g] & b[g]) != 0;
-            kact = (double)cnt * kGroup;
-          }
-          push(1, kact, (double)spts, kact > 0, 2.0 * tf * tf * kact);
-        }
+    for (long i = c->h_pi_off[step]; i < c->h_pi_off[step + 1]; ++i) {
+      const uint32_t e = c->h_pi[(size_t)i];
+      const long S = (long)step * c->nsplit + (e >> 20);
+      const int ci = (int)((e >> 10) & 1023), cj = (int)(e & 1023);
+      const int nu = c->h_su_off[(size_t)S + 1] - c->h_su_off[(size_t)S], nch = (nu + CH - 1) / CH;
+      const uint8_t *fl = c->h_gcf.data() + c->h_gcf_off[(size_t)S];
+      long cnt = 0;
+      for (long g = 0; g < G; ++g) cnt += (fl[g * nch + ci] & fl[g * nch + cj]) != 0;
+      const double rows = (double)kRun * std::min(CH, nu - ci * CH);
+      const double cols = (double)kRun * std::min(CH, nu - cj * CH);
+      push(1, (double)cnt * kGroup, (double)spts, cnt > 0, 2.0 * rows * cols * cnt * kGroup);
     }
   }
 }
@@ -395,19 +392,21 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.NP = c->NP;
       U.ntiles = c->ntiles;
       U.nsplit = c->nsplit;
-      U.first = (k == 0);
       U.incremental = incr[m];
       U.mask = functional;
-      U.screen = c->screen;
-      U.nfb = c->nfb;
+      U.CH = c->CH;
       U.nfw = c->nfw;
-      U.gstride = c->gstride;
-      if (c->screen) {
-        U.klist = c->klist + (p0 / kScreenTile) * c->nfb;
-        U.kcount = c->kcount + p0 / kScreenTile;
-        U.gsig = c->gsig + p0 / kGroup;
-        U.gbits = c->gbits + (p0 / kGroup) * c->nfw;
-      }
+      U.ditems = c->di + c->h_di_off[k];
+      U.pitems = c->pi + c->h_pi_off[k];
+      U.ndi = c->h_di_off[k + 1] - c->h_di_off[k];
+      U.npi = c->h_pi_off[k + 1] - c->h_pi_off[k];
+      U.tl_off = c->tl_off + p0 / kScreenTile;
+      U.tl = c->tl;
+      U.su_off = c->su_off + (long)k * c->nsplit;
+      U.su = c->su;
+      U.gcf_off = c->gcf_off + (long)k * c->nsplit;
+      U.gcf = c->gcf;
+      U.gbits = c->gbits + (p0 / kGroup) * c->nfw;
       units.push_back(U);
       umol.push_back(m);
     }
@@ -512,6 +511,8 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     pbase[m + 1] = pbase[m] + (npk + kThreads - 1) / kThreads;
     qbase[m + 1] = qbase[m] + ((long)c->NP * c->NP + kThreads - 1) / kThreads;
   }
+  for (int m = 0; m < nm; ++m)
+    plan->hzero.push_back({(int *)mols[m]->Hpart, 2L * mols[m]->nsplit * mols[m]->NP * mols[m]->NP});
   const size_t pack_off = put(packs.data(), sizeof(PackD) * nm);
   const size_t pb_off = put(pbase.data(), sizeof(long) * pbase.size());
   const size_t qb_off = put(qbase.data(), sizeof(long) * qbase.size());
@@ -589,6 +590,7 @@ template <typename T>
 static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hipEvent_t> *ev) {
   const bool x3 = pl.precision == PGXC_PREC_BF16X3;
   if (pl.nctr) launch_zero_ints(s, pl.d_ctr, pl.nctr);
+  for (auto &z : pl.hzero) launch_zero_ints(s, z.first, z.second);  // potential items add into H
   launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total,
               x3 ? 2 : std::is_same<T, double>::value ? 0 : 1, false);
   for (size_t k = 0; k < pl.steps.size(); ++k) {
@@ -821,17 +823,31 @@ static void function_order(std::vector<ShellD> &sh, const double *exps, const do
   }
 }
 
-// Screening tables of the padded grid (kernel k_screen) and the algorithmic
-// work of one build with and without them.
+// Screening and work tables of the padded grid and the algorithmic work of one
+// build.  k_screen marks, per 32-point group, the runs of 8 functions that can
+// be nonzero there (all runs when screening is off); the host turns the marks
+// into run lists per 128-point tile and per point split, [group][chunk]
+// flags per split, and the work items of every chunk.
+template <typename V>
+static int upload(pgxc_ctx *c, V **dst, const std::vector<V> &src) {
+  int rc = dalloc(c, dst, src.size());
+  if (rc) return rc;
+  if (!src.empty()) HIPCHK(c, hipMemcpy(*dst, src.data(), sizeof(V) * src.size(), hipMemcpyHostToDevice));
+  return PGXC_OK;
+}
+
 static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double *exps, bool on) {
-  const long ntp = c->npad / kScreenTile;
+  const long ntp = c->npad / kScreenTile, ngr = c->npad / kGroup;
   const double n = (double)c->nbas;
   c->flops_dense = 4.0 * c->npts * n * n + 16.0 * c->npts * n;
-  c->flops_screened = c->flops_dense;
-  c->nfb = c->NP / kGroup;
-  c->gstride = c->npad / kGroup;
-  c->screen = on && c->nfb <= kMaxBlocks && c->Mc / c->nsplit / kGroup <= kMaxSplitGroups;
-  if (!c->screen) return PGXC_OK;
+  c->n8 = c->NP / kRun;
+  c->CH = c->tf / kRun;
+  c->nfw = (c->NP / 32 + 31) / 32;
+  c->screen = on;
+  const int n8 = c->n8, nw8 = (n8 + 31) / 32, CH = c->CH;
+  const long G = c->Mc / c->nsplit / kGroup;
+  if (n8 > kMaxRuns) return fail(c, PGXC_ERR_UNSUPPORTED, "more than 8192 basis functions (padded)");
+  if (G > kMaxSplitGroups) return fail(c, PGXC_ERR_UNSUPPORTED, "point split above 65536 points");
   std::vector<ScreenShell> ss(sh.size());
   for (size_t i = 0; i < sh.size(); ++i) {
     double amin = exps[sh[i].exp_off];
@@ -843,33 +859,99 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
     ss[i].pad = 0;
   }
   int rc;
-  if ((rc = dalloc(c, &c->sshells, ss.size()))) return rc;
-  if ((rc = dalloc(c, &c->klist, (size_t)ntp * c->nfb))) return rc;
-  if ((rc = dalloc(c, &c->kcount, ntp))) return rc;
+  if ((rc = upload(c, &c->sshells, ss))) return rc;
   if ((rc = dalloc(c, &c->nbt, ntp))) return rc;
-  if ((rc = dalloc(c, &c->gsig, (size_t)c->ntiles * c->gstride))) return rc;
-  c->nfw = (c->nfb + 31) / 32;
-  if ((rc = dalloc(c, &c->gbits, (size_t)c->gstride * c->nfw))) return rc;
-  HIPCHK(c, hipMemcpy(c->sshells, ss.data(), sizeof(ScreenShell) * ss.size(), hipMemcpyHostToDevice));
+  if ((rc = dalloc(c, &c->sig8, (size_t)ngr * nw8))) return rc;
+  if ((rc = dalloc(c, &c->gbits, (size_t)ngr * c->nfw))) return rc;
   launch_screen(c->stream, ntp, c->grid, c->grid + c->npad, c->grid + 2 * c->npad, c->n_atoms,
-                c->atoms, c->n_shells, c->sshells, c->nfb, c->tf, c->ntiles, c->gstride, c->klist,
-                c->kcount, c->nbt, c->gsig, c->gbits);
+                c->atoms, c->n_shells, c->sshells, n8, c->nbt, c->sig8, c->gbits);
   HIPCHK(c, hipGetLastError());
   std::vector<int> nb((size_t)ntp);
+  std::vector<uint32_t> sig((size_t)ngr * nw8);
   HIPCHK(c, hipMemcpyAsync(nb.data(), c->nbt, sizeof(int) * ntp, hipMemcpyDeviceToHost, c->stream));
-  c->h_kcount.resize((size_t)ntp);
-  c->h_gsig.resize((size_t)c->ntiles * c->gstride);
-  HIPCHK(c, hipMemcpyAsync(c->h_kcount.data(), c->kcount, sizeof(int) * ntp, hipMemcpyDeviceToHost,
-                           c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->h_gsig.data(), c->gsig, c->h_gsig.size(), hipMemcpyDeviceToHost,
+  HIPCHK(c, hipMemcpyAsync(sig.data(), c->sig8, sizeof(uint32_t) * sig.size(), hipMemcpyDeviceToHost,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  double f = 0.0;
-  for (long t = 0; t < ntp; ++t) {
-    const double B = (double)std::max<long>(0, std::min<long>(kScreenTile, c->npts - t * kScreenTile));
-    f += 4.0 * B * nb[(size_t)t] * nb[(size_t)t] + 16.0 * B * nb[(size_t)t];
+  if (on) {
+    double f = 0.0;
+    for (long t = 0; t < ntp; ++t) {
+      const double B = (double)std::max<long>(0, std::min<long>(kScreenTile, c->npts - t * kScreenTile));
+      f += 4.0 * B * nb[(size_t)t] * nb[(size_t)t] + 16.0 * B * nb[(size_t)t];
+    }
+    c->flops_screened = f;
+  } else {  // every run counts on every group
+    c->flops_screened = c->flops_dense;
+    for (long g = 0; g < ngr; ++g)
+      for (int r = 0; r < n8; ++r) sig[(size_t)g * nw8 + r / 32] |= 1u << (r % 32);
+    std::vector<uint32_t> all((size_t)ngr * c->nfw, 0u);
+    for (long g = 0; g < ngr; ++g)
+      for (int b = 0; b < c->NP / 32; ++b) all[(size_t)g * c->nfw + b / 32] |= 1u << (b % 32);
+    HIPCHK(c, hipMemcpy(c->gbits, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice));
   }
-  c->flops_screened = f;
+  auto bit = [&](long g, int r) { return (sig[(size_t)g * nw8 + r / 32] >> (r % 32)) & 1u; };
+  // per 128-point tile: its runs (union over its four groups)
+  c->h_tl_off.assign(1, 0);
+  c->h_tl.clear();
+  for (long t = 0; t < ntp; ++t) {
+    for (int r = 0; r < n8; ++r)
+      if (bit(4 * t, r) | bit(4 * t + 1, r) | bit(4 * t + 2, r) | bit(4 * t + 3, r))
+        c->h_tl.push_back((uint16_t)r);
+    c->h_tl_off.push_back((int)c->h_tl.size());
+  }
+  // per split: union of its groups' runs, [group][chunk] flags
+  const long NS = (long)c->nchunks * c->nsplit;
+  c->h_su_off.assign(1, 0);
+  c->h_su.clear();
+  c->h_gcf_off.clear();
+  c->h_gcf.clear();
+  for (long S = 0; S < NS; ++S) {
+    const long g0 = S * G;
+    const size_t u0 = c->h_su.size();
+    for (int r = 0; r < n8; ++r) {
+      uint32_t any = 0;
+      for (long g = g0; g < g0 + G && !any; ++g) any = bit(g, r);
+      if (any) c->h_su.push_back((uint16_t)r);
+    }
+    c->h_su_off.push_back((int)c->h_su.size());
+    const int nu = (int)(c->h_su.size() - u0), nch = (nu + CH - 1) / CH;
+    c->h_gcf_off.push_back((long)c->h_gcf.size());
+    for (long g = g0; g < g0 + G; ++g)
+      for (int ch = 0; ch < nch; ++ch) {
+        uint8_t f = 0;
+        for (int e = ch * CH; e < std::min(nu, (ch + 1) * CH) && !f; ++e) f = (uint8_t)bit(g, c->h_su[u0 + e]);
+        c->h_gcf.push_back(f);
+      }
+  }
+  // work items per chunk
+  const long tpc = c->Mc / kScreenTile;
+  c->h_di_off.assign(1, 0);
+  c->h_pi_off.assign(1, 0);
+  c->h_di.clear();
+  c->h_pi.clear();
+  for (int k = 0; k < c->nchunks; ++k) {
+    for (long t = 0; t < tpc; ++t) {
+      const long T = k * tpc + t;
+      const int c8 = c->h_tl_off[(size_t)T + 1] - c->h_tl_off[(size_t)T];
+      for (int ch = 0; ch < (c8 + CH - 1) / CH; ++ch) c->h_di.push_back((uint32_t)(t << 8 | ch));
+    }
+    c->h_di_off.push_back((long)c->h_di.size());
+    for (int sl = 0; sl < c->nsplit; ++sl) {
+      const long S = (long)k * c->nsplit + sl;
+      const int nu = c->h_su_off[(size_t)S + 1] - c->h_su_off[(size_t)S], nch = (nu + CH - 1) / CH;
+      for (int ci = 0; ci < nch; ++ci)
+        for (int cj = 0; cj < nch; ++cj)
+          c->h_pi.push_back((uint32_t)sl << 20 | (uint32_t)ci << 10 | (uint32_t)cj);
+    }
+    c->h_pi_off.push_back((long)c->h_pi.size());
+  }
+  if ((rc = upload(c, &c->tl_off, c->h_tl_off))) return rc;
+  if ((rc = upload(c, &c->tl, c->h_tl))) return rc;
+  if ((rc = upload(c, &c->su_off, c->h_su_off))) return rc;
+  if ((rc = upload(c, &c->su, c->h_su))) return rc;
+  if ((rc = upload(c, &c->gcf_off, c->h_gcf_off))) return rc;
+  if ((rc = upload(c, &c->gcf, c->h_gcf))) return rc;
+  if ((rc = upload(c, &c->di, c->h_di))) return rc;
+  if ((rc = upload(c, &c->pi, c->h_pi))) return rc;
   return PGXC_OK;
 }
 
@@ -892,8 +974,8 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
     return fail(c, PGXC_ERR_ARG, "pgxc_create: null pointer or non-positive size");
   if (opt && opt->tile != 0 && opt->tile != 64 && opt->tile != 96 && opt->tile != 128)
     return fail(c, PGXC_ERR_ARG, "pgxc_create: tile must be 0, 64, 96 or 128");
-  if (opt && (opt->chunk_points < 0 || opt->chunk_points % 256 != 0))
-    return fail(c, PGXC_ERR_ARG, "pgxc_create: chunk_points must be a non-negative multiple of 256");
+  if (opt && (opt->chunk_points < 0 || opt->chunk_points % kSplitPts != 0))
+    return fail(c, PGXC_ERR_ARG, "pgxc_create: chunk_points must be a non-negative multiple of 2048");
   if (opt && (opt->variant_fp64 < -1 || opt->variant_fp64 >= kNumVariants || opt->variant_fp32 < -1 ||
               opt->variant_fp32 >= kNumVariants))
     return fail(c, PGXC_ERR_ARG, "pgxc_create: variant must be -1 (default) or 0.." +
@@ -980,20 +1062,21 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   c->ntiles = c->NP / c->tf;
 
   // ---- chunking of the grid ----------------------------------------------
-  // phi chunk = 4 x NP x Mc doubles, kept under ~2 GiB unless requested
-  long mc_max = std::max<long>(256, ((1L << 31) / (4L * c->NP * 8)) / 256 * 256);
-  mc_max = std::min<long>(mc_max, 65536);
+  // a chunk holds at most 256 Ki points and 8 GiB of FP64 basis values; a
+  // point split of the potential contraction is a whole number of 2048-point
+  // blocks, the fewest that keep the splits of a chunk at 64 or less
+  long mc_max = std::max<long>(kSplitPts, ((8L << 30) / (32L * c->NP)) / kSplitPts * kSplitPts);
+  mc_max = std::min<long>(mc_max, 1L << 18);
   if (opt && opt->chunk_points) mc_max = opt->chunk_points;
   c->nchunks = (int)((npts + mc_max - 1) / mc_max);
-  c->Mc = round_up((npts + c->nchunks - 1) / c->nchunks, 256);
+  c->Mc = round_up((npts + c->nchunks - 1) / c->nchunks, kSplitPts);
   c->npad = c->Mc * c->nchunks;
-  // split of the point range in the potential contraction: enough items
-  // for 256 CUs, each split a whole number of 32-point steps
-  const int ntile_pot = (c->NP / c->tf) * (c->NP / c->tf);
-  c->nsplit = 1;
-  while (c->nsplit < 16 && ntile_pot * c->nsplit < 512 &&
-         (c->Mc / kSplitStep) % (c->nsplit * 2) == 0)
-    c->nsplit *= 2;
+  {
+    const long q = c->Mc / kSplitPts;
+    long m = 1;
+    while (q % m != 0 || q / m > 64) ++m;
+    c->nsplit = (int)(q / m);
+  }
   c->nblk_total = c->npad / kThreads;
 
   HIPCHK(c, hipGetDevice(&c->device));
@@ -1219,8 +1302,9 @@ int pgxc_destroy(pgxc_ctx *c) {
   free_plan(c->plan);
   void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
-                  c->Vstate, c->Vout, c->epart, c->klist, c->kcount, c->nbt, c->gsig,
-                  c->sshells, c->d_fpos, c->phic[0], c->phic[1], c->gbits};
+                  c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
+                  c->phic[0], c->phic[1], c->gbits, c->tl_off, c->tl, c->su_off, c->su,
+                  c->gcf_off, c->gcf, c->di, c->pi};
   for (void *b : bufs) dfree(b);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);

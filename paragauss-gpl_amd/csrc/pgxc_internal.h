@@ -47,21 +47,27 @@ struct UnitD {
   double *state;    // [S_NSLOT][npad] molecule state, offset to the unit's first point
   double *epart;    // [Mc/256][2] block sums of w f and w rho (offset to the unit)
   double *Hpart;    // [nsplit][NP][NP] potential partials of the molecule
-  // screening (screen != 0): for each 128-point tile of the unit, the count
-  // and ascending list of 32-function blocks that can be nonzero on it
-  // (klist row stride nfb); for each function tile, a flag per 32-point group
-  // of the molecule's grid telling whether the tile can be nonzero there
-  // (gsig row stride gstride, offset to the unit's first group)
-  // gbits: per 32-point group of the unit, nfw words of flags, one per
-  // 32-function block that can be nonzero on the group
-  const uint16_t *klist;
-  const int *kcount;
-  const uint8_t *gsig;
+  // Work of the contractions, over gathered runs of 8 functions (all runs
+  // when screening is off).  ditems / pitems: the unit's density items
+  // (tile << 8 | column chunk) and potential items (split << 20 | row chunk
+  // << 10 | column chunk).  tl_off / tl: per 128-point tile of the unit, its
+  // ascending run list (tl_off offset to the unit's first tile, values index
+  // tl).  su_off / su: per point split of the unit, the union of its groups'
+  // runs; a chunk is CH = tile / 8 consecutive entries of a list.  gcf_off /
+  // gcf: per split, [group][chunk] flags, whether the chunk can be nonzero on
+  // the group.  gbits: per 32-point group, nfw words of 32-function block
+  // flags (W kernel).
+  const uint32_t *ditems, *pitems;
+  const int *tl_off, *su_off;
+  const uint16_t *tl, *su;
+  const long *gcf_off;
+  const uint8_t *gcf;
   const uint32_t *gbits;
-  long Mc, npad, gstride;
-  int n_atoms, NP, ntiles, nsplit, first, incremental;
+  long Mc, npad;
+  int n_atoms, NP, ntiles, nsplit, incremental;
   uint32_t mask;
-  int screen, nfb, nfw;
+  int CH, nfw;
+  long ndi, npi;  // density / potential items of the unit
 };
 
 // Per-molecule descriptors of the operand preparation and the final packing.
@@ -124,37 +130,33 @@ void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, lon
 
 // item counts per unit, shared by the planner and the kernels' decoding
 inline long items_basis(long Mc, int n_atoms) { return (Mc / kThreads) * n_atoms; }
-inline long items_density(long Mc, int NP, int tf) { return (Mc / 128) * (NP / tf); }
 inline long items_functional(long Mc) { return Mc / kThreads; }
 inline long items_wmat(long Mc, int NP) { return (Mc / kThreads) * (NP / 16); }
-inline long items_potential(int NP, int tf, int nsplit) { return (long)(NP / tf) * (NP / tf) * nsplit; }
 inline long items_split(long Mc, int NP) { return (Mc / 64) * (NP / 32); }
-// points per K step of the potential contraction across all variants
-constexpr int kSplitStep = 32;
+// point splits of the potential contraction are whole multiples of this
+constexpr int kSplitPts = 2048;
 
 // ---- screening ------------------------------------------------------------
 // A radial factor is exactly zero where alpha r^2 >= 50 (the cut-off of
 // ParaGauss's radial, modules/orbital_module.f90:1839-1855), so a shell is
 // zero on a set of points when alpha_min r^2 >= 50 at every one of them.
-// Granularity: 32-point groups x 32-function blocks; the contractions skip
-// K steps whose operand block is zero on the whole step.  Skipped terms are
-// exact zeros, so screened and dense builds give the same bits.
+// Granularity: 32-point groups x runs of 8 functions.  The contractions work
+// on gathered runs: a 128-point tile contracts over the runs marked on any of
+// its groups, a point split over the union of its groups' runs; the terms
+// left out are exact zeros.
 constexpr int kGroup = 32;
+constexpr int kRun = 8;
 constexpr int kScreenTile = 128;       // point tile of the density contraction
-constexpr int kMaxBlocks = 256;        // function blocks per molecule (NP <= 8192)
+constexpr int kMaxRuns = 1024;         // runs of 8 functions per molecule (NP <= 8192)
 constexpr int kMaxSplitGroups = 2048;  // point groups per potential split
 constexpr double kCutoff = 50.0 * (1.0 + 1.0e-6);  // margin: a superset in every precision
 struct ScreenShell {
   double amin;                 // smallest exponent of the shell
   int atom, func_off, nfunc, pad;
 };
-// per 128-point tile: kcount, klist[nfb], and the number of basis functions
-// of shells that can be nonzero on it (n_b); per function tile of width tf and
-// 32-point group: gsig
 void launch_screen(hipStream_t s, long ntiles_pts, const double *gx, const double *gy,
                    const double *gz, int n_atoms, const AtomD *atoms, int n_shells,
-                   const ScreenShell *shells, int nfb, int tf, int ntile_f, long gstride,
-                   uint16_t *klist, int *kcount, int *nb, uint8_t *gsig, uint32_t *gbits);
+                   const ScreenShell *shells, int n8, int *nb, uint32_t *sig8, uint32_t *gbits);
 
 // P = C diag(occ) C^T into an NP x NP padded array
 void launch_form_density(hipStream_t s, const double *C, const double *occ, int n,

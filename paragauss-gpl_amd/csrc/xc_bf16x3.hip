@@ -33,13 +33,14 @@ struct XStage {
 
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-// Staging of an operand stored split and k-contiguous: X(r,k) = hi / lo
-// [r*s + k]; per stage R rows x 32 k x 2 planes, moved as 16-byte vectors.
+// Staging of an operand stored split and k-contiguous (bf16 hi and lo planes
+// at the same offsets); per stage R rows x 32 k x 2 planes, moved as 16-byte
+// vectors (runs of 8).  addr(r, kt, kk) gives the element offset of row r's
+// run starting at local k index kk of step kt, or -1 for zeros.
 template <int R>
 struct PSOperand {
   struct Src {
     const __bf16 *hi, *lo;
-    long s;
   };
   static constexpr int NV = R * (kXBK / 8) * 2;  // 16-byte vectors per stage
   static constexpr int PER = NV / kThreads;
@@ -52,13 +53,15 @@ struct PSOperand {
     row = rem >> 2;
     c = rem & 3;
   }
-  __device__ __forceinline__ void load(const Src &S, long kb) {
+  template <class Addr>
+  __device__ __forceinline__ void load(const Src &S, const Addr &addr, int kt) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       int plane, row, c;
       place(threadIdx.x + kThreads * e, plane, row, c);
+      const long o = addr(row, kt, 8 * c);
       const __bf16 *P = plane ? S.lo : S.hi;
-      v[e] = *(const u32x4 *)(P + (long)row * S.s + kb + 8 * c);
+      v[e] = o >= 0 ? *(const u32x4 *)(P + o) : u32x4{0u, 0u, 0u, 0u};
     }
   }
   __device__ __forceinline__ void store(__bf16 *hi, __bf16 *lo) const {
@@ -75,10 +78,10 @@ struct PSOperand {
 // K step kt starts at k = kmap(kt).  Pipeline: two LDS stages and two
 // register slots; the global loads of step kt + 2 are issued before the MFMAs
 // of step kt, so each load has two steps of MFMA work to arrive in.
-template <int BM, int BN, class OA, class OB, class KMap>
+template <int BM, int BN, class OA, class OB, class AAddr, class BAddr>
 __device__ __forceinline__ void gemm_tile_bf16x3(const typename OA::Src &A,
                                                  const typename OB::Src &B, int ksteps,
-                                                 KMap kmap, __bf16 *lds,
+                                                 AAddr aaddr, BAddr baddr, __bf16 *lds,
                                                  f4 (&acc)[BM / 32][BN / 32]) {
   constexpr int BKP = kXBKP, WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int SA = BM * BKP, SB = BN * BKP, STAGE = 2 * SA + 2 * SB;
@@ -93,9 +96,8 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const typename OA::Src &A,
 
   if (ksteps <= 0) return;
   auto load = [&](int kt, OA &ra, OB &rb) {
-    const long kb = kmap(kt);
-    ra.load(A, kb);
-    rb.load(B, kb);
+    ra.load(A, aaddr, kt);
+    rb.load(B, baddr, kt);
   };
   auto store = [&](int buf, const OA &ra, const OB &rb) {
     __bf16 *ah = lds + buf * STAGE, *al = ah + SA, *bh = al + SA, *bl = bh + SB;
@@ -161,40 +163,40 @@ __global__ __launch_bounds__(kThreads) void k_density_bf16x3(const StageList S) 
   constexpr int BM = kXBM;
   __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BM, BN>::n];
   __shared__ double red[2][4][BM];
-  __shared__ uint16_t kl[kMaxBlocks];
+  __shared__ uint16_t kl[kMaxRuns];
   __shared__ long slot;
-  static_assert(kXBK == kGroup, "one K step per 32-function block");
+  static_assert(kXBK == 4 * kRun, "four runs per K step");
   for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
     const uint64_t t0 = S.tlog ? wall_clock64() : 0;
     const long it = walk_item(S, L);
     const int u = find_unit_x3(S.base, S.nu, it);
     const UnitD &U = S.units[u];
-    const long local = it - S.base[u];
-    const int ntn = U.NP / BN;
-    const long m0 = (local / ntn) * BM;
-    const int tn = (int)(local % ntn), n0 = tn * BN;
-    const float *phi = (const float *)U.phi;
-    bool scr = U.screen != 0;
-    int ksteps = U.NP / kXBK;
-    if (scr) {
-      ksteps = density_screen(U, m0, tn, kl);
-      if (ksteps < 0) {
-        density_zero<BM, GRAD>(U.part, U.Mc, m0, tn);
-        item_log(S, it, t0);
-        continue;
-      }
-      scr = ksteps < U.nfb;  // a complete list is the dense walk
-    }
-    auto kmap = [&](int kt) -> long { return scr ? (long)kl[kt] * kGroup : (long)kt * kXBK; };
-    f4 acc[BM / 32][BN / 32];
-    // A(p, k) = Phi^T rows of the point tile; B(k, n) = D^T rows of the function tile
+    const uint32_t e = U.ditems[it - S.base[u]];
+    const int tile = (int)(e >> 8), ch = (int)(e & 255);
+    const long m0 = (long)tile * BM, NP = U.NP;
+    const int off = U.tl_off[tile], c8 = U.tl_off[tile + 1] - off;
+    __syncthreads();  // the previous item's readers of kl are done
+    lds_list(U.tl + off, c8, kl);
+    const uint16_t *crun = kl + ch * U.CH;
+    const int ncr = min(U.CH, c8 - ch * U.CH);
+    // A(p, k) = Phi^T[m0 + p][f(k)], B(k, n) = D^T[col(n)][f(k)]
+    auto aaddr = [&](int m, int kt, int kk) -> long {
+      const long f = run_fn(kl, c8, kt * kXBK + kk);
+      return f < 0 ? -1 : (m0 + m) * NP + f;
+    };
+    auto baddr = [&](int n, int kt, int kk) -> long {
+      const long f = run_fn(kl, c8, kt * kXBK + kk), c = run_fn(crun, ncr, n);
+      return (f < 0 || c < 0) ? -1 : c * NP + f;
+    };
     const __bf16 *th = (const __bf16 *)U.phit, *dh = (const __bf16 *)U.D;
-    const long np2 = (long)U.NP * U.NP, pn = (long)U.Mc * U.NP;
-    const typename PSOperand<BM>::Src sa{th + m0 * U.NP, th + pn + m0 * U.NP, U.NP};
-    const typename PSOperand<BN>::Src sb{dh + (long)n0 * U.NP, dh + np2 + (long)n0 * U.NP, U.NP};
-    gemm_tile_bf16x3<BM, BN, PSOperand<BM>, PSOperand<BN>>(sa, sb, ksteps, kmap, lds, acc);
-    density_epilogue<float, f4, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn, U.part, red,
-                                              scr ? U.gbits : nullptr, U.nfw);
+    const long np2 = NP * NP, pn = U.Mc * NP;
+    const typename PSOperand<BM>::Src sa{th, th + pn};
+    const typename PSOperand<BN>::Src sb{dh, dh + np2};
+    f4 acc[BM / 32][BN / 32];
+    gemm_tile_bf16x3<BM, BN, PSOperand<BM>, PSOperand<BN>>(sa, sb, (c8 + 3) / 4, aaddr, baddr,
+                                                           lds, acc);
+    density_epilogue<float, f4, BM, BN, GRAD>(acc, (const float *)U.phi, U.Mc, U.NP, m0, crun, ncr,
+                                              ch, U.part, red);
     item_log(S, it, t0);
   }
 }
@@ -222,32 +224,43 @@ void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L)
 template <int BT>
 __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S) {
   __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BT, BT>::n];
+  __shared__ uint16_t ul[kMaxRuns];
   __shared__ uint16_t gl[kMaxSplitGroups];
   __shared__ int wtot[4];
   __shared__ long slot;
+  static_assert(kXBK == kGroup, "one K step per 32-point group");
   for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
     const uint64_t t0 = S.tlog ? wall_clock64() : 0;
     const long it = walk_item(S, L);
     const int u = find_unit_x3(S.base, S.nu, it);
     const UnitD &U = S.units[u];
-    const long local = it - S.base[u];
-    const int nt = U.NP / BT, nt2 = nt * nt;
-    const int sl = (int)(local / nt2), tile = (int)(local % nt2);
-    const int i0 = (tile / nt) * BT, j0 = (tile % nt) * BT;
-    const int kdense = (int)(U.Mc / kXBK) / U.nsplit;
-    const long kb = (long)sl * kdense * kXBK, Mc = U.Mc;
-    const int ksteps =
-        U.screen ? potential_screen(U, kb, kdense * kXBK, i0 / BT, j0 / BT, gl, wtot) : kdense;
-    const bool scr = ksteps < kdense;  // a complete list is the dense walk
-    auto kmap = [&](int kt) -> long { return scr ? (long)gl[kt] * kGroup : (long)kt * kXBK; };
+    const uint32_t e = U.pitems[it - S.base[u]];
+    const int sl = (int)(e >> 20), ci = (int)((e >> 10) & 1023), cj = (int)(e & 1023);
+    const long Mc = U.Mc;
+    const int so = U.su_off[sl], nu = U.su_off[sl + 1] - so, CH = U.CH;
+    const int nch = (nu + CH - 1) / CH, G = (int)(Mc / U.nsplit) / kGroup;
+    const long kb = (long)sl * G * kGroup;
+    __syncthreads();  // the previous item's readers of ul / gl are done
+    lds_list(U.su + so, nu, ul);
+    const int ng = potential_groups(U.gcf + U.gcf_off[sl], G, nch, ci, cj, gl, wtot);
+    const uint16_t *ri = ul + ci * CH, *rj = ul + cj * CH;
+    const int nri = min(CH, nu - ci * CH), nrj = min(CH, nu - cj * CH);
+    auto aaddr = [&](int m, int kt, int kk) -> long {
+      const long f = run_fn(ri, nri, m);
+      return f < 0 ? -1 : f * Mc + kb + (long)gl[kt] * kGroup + kk;
+    };
+    auto baddr = [&](int n, int kt, int kk) -> long {
+      const long f = run_fn(rj, nrj, n);
+      return f < 0 ? -1 : f * Mc + kb + (long)gl[kt] * kGroup + kk;
+    };
     const __bf16 *wh = (const __bf16 *)U.W, *ph = (const __bf16 *)U.phis;
     const long pn = (long)U.NP * Mc;
-    const typename PSOperand<BT>::Src sa{wh + (long)i0 * Mc + kb, wh + pn + (long)i0 * Mc + kb, Mc};
-    const typename PSOperand<BT>::Src sb{ph + (long)j0 * Mc + kb, ph + pn + (long)j0 * Mc + kb, Mc};
+    const typename PSOperand<BT>::Src sa{wh, wh + pn};
+    const typename PSOperand<BT>::Src sb{ph, ph + pn};
     f4 acc[BT / 32][BT / 32];
-    gemm_tile_bf16x3<BT, BT, PSOperand<BT>, PSOperand<BT>>(sa, sb, ksteps, kmap, lds, acc);
-    potential_epilogue<float, f4, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP, U.NP, i0, j0,
-                                          U.first);
+    gemm_tile_bf16x3<BT, BT, PSOperand<BT>, PSOperand<BT>>(sa, sb, ng, aaddr, baddr, lds, acc);
+    potential_epilogue<float, f4, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP, U.NP, ri, nri, rj,
+                                          nrj);
     item_log(S, it, t0);
   }
 }

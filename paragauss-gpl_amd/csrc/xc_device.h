@@ -99,44 +99,26 @@ __device__ __forceinline__ int block_compact(bool act, int value, uint16_t *list
   return total;
 }
 
-// Density item (128-point tile at m0, function tile tn): -1 when the function
-// tile is zero on the whole point tile; otherwise the number of 32-function
-// blocks that can be nonzero on the point tile, whose ascending list is
-// copied to kl.
-__device__ __forceinline__ int density_screen(const UnitD &U, long m0, int tn, uint16_t *kl) {
-  const long tile = m0 / kScreenTile;
-  const uint8_t *gs = U.gsig + (long)tn * U.gstride + tile * (kScreenTile / kGroup);
-  const bool active = (gs[0] | gs[1] | gs[2] | gs[3]) != 0;
-  const int cnt = U.kcount[tile];
-  if (active)
-    for (int i = threadIdx.x; i < cnt; i += kThreads) kl[i] = U.klist[tile * U.nfb + i];
+// Copy n list entries into LDS (every thread calls it; ends with a barrier).
+__device__ __forceinline__ void lds_list(const uint16_t *__restrict__ src, int n, uint16_t *dst) {
+  for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = src[i];
   __syncthreads();
-  return active ? cnt : -1;
 }
 
-// Density contributions of an item whose function tile is zero on its points.
-template <int BM, bool GRAD>
-__device__ __forceinline__ void density_zero(double *__restrict__ part, long Mc, long m0, int tn) {
-  constexpr int NC = GRAD ? 4 : 1;
-  for (int idx = threadIdx.x; idx < NC * BM; idx += kThreads) {
-    const int c = idx / BM, row = idx % BM;
-    part[((long)tn * 4 + c) * Mc + m0 + row] = 0.0;
-  }
+// Function (row of Phi / W / D) of entry e of a gathered run list, or -1
+// past the list's n runs.
+__device__ __forceinline__ long run_fn(const uint16_t *runs, int n, int e) {
+  return (e >> 3) < n ? (long)runs[e >> 3] * kRun + (e & 7) : -1;
 }
 
-// Potential item (split starting at point kb of the unit, function tiles ti,
-// tj): the groups of the split on which both tiles can be nonzero, relative to
-// the split start, into gl.  Returns the count.
-__device__ __forceinline__ int potential_screen(const UnitD &U, long kb, int npts_split, int ti,
-                                                int tj, uint16_t *gl, int *wtot) {
-  const int G = npts_split / kGroup;
-  const long g0 = kb / kGroup;
-  const uint8_t *si = U.gsig + (long)ti * U.gstride + g0;
-  const uint8_t *sj = U.gsig + (long)tj * U.gstride + g0;
+// Potential item: the groups of its split on which both of its chunks can be
+// nonzero (flags [G][nch]), ascending, into gl.  Returns the count.
+__device__ __forceinline__ int potential_groups(const uint8_t *__restrict__ flags, int G, int nch,
+                                                int ci, int cj, uint16_t *gl, int *wtot) {
   int cnt = 0;
   for (int b = 0; b < G; b += kThreads) {
     const int g = b + (int)threadIdx.x;
-    const bool act = g < G && si[g] != 0 && sj[g] != 0;
+    const bool act = g < G && flags[(long)g * nch + ci] != 0 && flags[(long)g * nch + cj] != 0;
     cnt = block_compact(act, g, gl, cnt, wtot);
   }
   return cnt;
@@ -177,21 +159,20 @@ __device__ __forceinline__ double group_reduce(double (&v)[M], int lane, int &in
   return v[0];
 }
 
-// Epilogue of X = Phi D for one point tile x function tile: contributions
-// rho += sum_n Phi(p,n) X(p,n), grad rho += 2 sum_n dPhi(p,n) X(p,n), summed
-// over the tile's columns in a fixed order (j, then the 16-lane group
-// reduction) and written to part[tn][c][p].  With the FP32 result placement a lane holds
-// four consecutive points of one column, read as one 16-byte load per
-// component; the FP64 placement reads point by point.  Both orders of the
-// sums are the same.  With gbits (screening), 16-column blocks whose basis
-// values are exactly zero on the 16 points are skipped: they add zeros.
+// Epilogue of X = Phi D for one point tile x gathered column chunk:
+// contributions rho += sum_n Phi(p,n) X(p,n), grad rho += 2 sum_n dPhi(p,n)
+// X(p,n), summed over the chunk's columns in a fixed order (j, then the
+// 16-lane group reduction) and written to part[ch][c][p].  Column n of the
+// chunk is function run_fn(crun, ncr, n) (absent columns add nothing).  With
+// the FP32 result placement a lane holds four consecutive points of one
+// column, read as one 16-byte load per component; the FP64 placement reads
+// point by point.
 template <typename T, typename ACC, int BM, int BN, bool GRAD>
 __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 32],
                                                  const T *__restrict__ phi, long Mc, int NP,
-                                                 long m0, int n0, int tn,
+                                                 long m0, const uint16_t *crun, int ncr, int ch,
                                                  double *__restrict__ part,
-                                                 double (&red)[2][4][BM],
-                                                 const uint32_t *__restrict__ gbits, int nfw) {
+                                                 double (&red)[2][4][BM]) {
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int NC = GRAD ? 4 : 1;
   constexpr bool QUAD = MF<T>::kQuadRows;
@@ -207,12 +188,8 @@ __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 
       for (int r = 0; r < 4; ++r) s[c][r] = 0.0;
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      if (gbits) {  // 16 columns cut off on these 16 points: exact zeros, skip
-        const long g = (m0 + wm * WM + i * 16) / kGroup;
-        const int b = (n0 + wn * WN + j * 16) / kGroup;
-        if (!((gbits[g * nfw + (b >> 5)] >> (b & 31)) & 1u)) continue;
-      }
-      const long n = n0 + wn * WN + j * 16 + mf_col(lane);
+      const long n = run_fn(crun, ncr, wn * WN + j * 16 + mf_col(lane));
+      if (n < 0) continue;
       T v[NC][4];
       if constexpr (QUAD) {
         const long q = n * Mc + m0 + wm * WM + i * 16 + MF<T>::row(lane, 0);
@@ -252,30 +229,34 @@ __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 
   __syncthreads();
   for (int idx = tid; idx < NC * BM; idx += kThreads) {
     const int c = idx / BM, row = idx % BM;
-    part[((long)tn * 4 + c) * Mc + m0 + row] = red[0][c][row] + red[1][c][row];
+    part[((long)ch * 4 + c) * Mc + m0 + row] = red[0][c][row] + red[1][c][row];
   }
 }
 
-// Epilogue of H = W^T Phi: the tile is added into the split's FP64 slot
-// (stored on the first chunk).
+// Epilogue of H = W^T Phi for one gathered row chunk x column chunk: the
+// tile is added into the split's FP64 slot at the functions of its rows and
+// columns (slots are zeroed at the start of a build).
 template <typename T, typename ACC, int BM, int BN>
 __device__ __forceinline__ void potential_epilogue(const ACC (&acc)[BM / 32][BN / 32], double *H,
-                                                   int NP, int i0, int j0, int first) {
+                                                   int NP, const uint16_t *rrun, int nrr,
+                                                   const uint16_t *crun, int ncr) {
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int j = 0; j < TN; ++j) {
+    const long col = run_fn(crun, ncr, wn * WN + j * 16 + mf_col(lane));
+    if (col < 0) continue;
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long row = i0 + wm * WM + i * 16 + MF<T>::row(lane, r);
-        const long col = j0 + wn * WN + j * 16 + mf_col(lane);
-        const double v = (double)acc[i][j][r];
+        const long row = run_fn(rrun, nrr, wm * WM + i * 16 + MF<T>::row(lane, r));
+        if (row < 0) continue;
         const long q = row * NP + col;
-        H[q] = first ? v : H[q] + v;
+        H[q] += (double)acc[i][j][r];
       }
+  }
 }
 
 }  // namespace pgxc

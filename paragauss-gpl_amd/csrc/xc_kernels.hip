@@ -59,11 +59,14 @@ constexpr int kc_row() { return BK + kc_elems<T>(); }
 template <typename T, int R, int BK, int ST, bool KC>
 constexpr int lds_elems() { return ST * R * (KC ? kc_row<T, BK>() : BK); }
 
-// K step kt starts at k = kmap(kt) (a dense walk or a screened list).
-template <typename T, int BM, int BN, int BK, int ST, bool AKC, bool BKC, class KMap>
-__device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
-                                          const T *__restrict__ B, long sB, int ksteps,
-                                          KMap kmap, T *As, T *Bs,
+// Operand addressing (gathered rows and K runs): aaddr(i, kt, kk) is the
+// element offset in A of the vector that starts at tile row / column i and
+// local k index kk of K step kt (vectors run along the contiguous dimension),
+// or -1 for a vector of zeros; baddr likewise for B.
+template <typename T, int BM, int BN, int BK, int ST, bool AKC, bool BKC, class AAddr,
+          class BAddr>
+__device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__restrict__ B,
+                                          int ksteps, AAddr aaddr, BAddr baddr, T *As, T *Bs,
                                           typename MF<T>::acc (&acc)[BM / 32][BN / 32]) {
   static_assert(AKC == BKC, "operands share one LDS layout");
   constexpr bool KC = AKC;
@@ -88,28 +91,31 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, long sA,
 
   if (ksteps <= 0) return;
   auto load = [&](int kt) {
-    const long kb = kmap(kt);
 #pragma unroll
     for (int e = 0; e < LA; ++e) {
       const int idx = tid + kThreads * e;
+      long o;
       if (AKC) {
         const int mm = idx / (BK / VA), kv = idx % (BK / VA);
-        ra[e] = *(const typename VA_t::type *)(A + (long)mm * sA + kb + kv * VA);
+        o = aaddr(mm, kt, kv * VA);
       } else {
         const int mv = idx % (BM / VA), kk = idx / (BM / VA);
-        ra[e] = *(const typename VA_t::type *)(A + mv * VA + (kb + kk) * sA);
+        o = aaddr(mv * VA, kt, kk);
       }
+      ra[e] = o >= 0 ? *(const typename VA_t::type *)(A + o) : typename VA_t::type{};
     }
 #pragma unroll
     for (int e = 0; e < LB; ++e) {
       const int idx = tid + kThreads * e;
+      long o;
       if (BKC) {
         const int nn = idx / (BK / VB), kv = idx % (BK / VB);
-        rb[e] = *(const typename VB_t::type *)(B + (long)nn * sB + kb + kv * VB);
+        o = baddr(nn, kt, kv * VB);
       } else {
         const int nv = idx % (BN / VB), kk = idx / (BN / VB);
-        rb[e] = *(const typename VB_t::type *)(B + (kb + kk) * sB + nv * VB);
+        o = baddr(nv * VB, kt, kk);
       }
+      rb[e] = o >= 0 ? *(const typename VB_t::type *)(B + o) : typename VB_t::type{};
     }
   };
   auto store = [&](int buf) {
@@ -535,43 +541,40 @@ template <> struct Var<2> { static constexpr int BK = 32, ST = 2; };
 template <typename T, int BN, int VAR, bool GRAD>
 __global__ __launch_bounds__(kThreads) void k_density(const StageList S) {
   constexpr int BM = kDBM, BK = Var<VAR>::BK, ST = Var<VAR>::ST, TM = BM / 32, TN = BN / 32;
-  constexpr int R = kGroup / BK;  // K steps per 32-function block
   __shared__ T As[ST * BK * BM];
   __shared__ T Bs[ST * BK * BN];
   __shared__ double red[2][4][BM];
-  __shared__ uint16_t kl[kMaxBlocks];
+  __shared__ uint16_t kl[kMaxRuns];
   __shared__ long slot;
   for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
     const uint64_t t0 = S.tlog ? wall_clock64() : 0;
     const long it = walk_item(S, L);
     const int u = find_unit(S.base, S.nu, it);
     const UnitD &U = S.units[u];
-    const long local = it - S.base[u];
-    const int ntn = U.NP / BN;
-    const long m0 = (local / ntn) * BM;
-    const int tn = (int)(local % ntn), n0 = tn * BN;
-    const T *phi = (const T *)U.phi, *D = (const T *)U.D;
-    bool scr = U.screen != 0;
-    int ksteps = U.NP / BK;
-    if (scr) {
-      const int cnt = density_screen(U, m0, tn, kl);
-      if (cnt < 0) {
-        density_zero<BM, GRAD>(U.part, U.Mc, m0, tn);
-        item_log(S, it, t0);
-        continue;
-      }
-      scr = cnt < U.nfb;  // a complete list is the dense walk
-      ksteps = cnt * R;
-    }
-    auto kmap = [&](int kt) -> long {
-      return scr ? (long)kl[kt / R] * kGroup + (kt % R) * BK : (long)kt * BK;
+    const uint32_t e = U.ditems[it - S.base[u]];
+    const int tile = (int)(e >> 8), ch = (int)(e & 255);
+    const long m0 = (long)tile * BM, Mc = U.Mc, NP = U.NP;
+    const int off = U.tl_off[tile], c8 = U.tl_off[tile + 1] - off;
+    __syncthreads();  // the previous item's readers of kl are done
+    lds_list(U.tl + off, c8, kl);
+    const uint16_t *crun = kl + ch * U.CH;
+    const int ncr = min(U.CH, c8 - ch * U.CH);
+    const int ksteps = (c8 * kRun + BK - 1) / BK;
+    // A(m, k) = Phi[f(k)][m0 + m], B(k, n) = D[f(k)][col(n)]: K and the
+    // columns run over the tile's gathered runs
+    auto aaddr = [&](int m, int kt, int kk) -> long {
+      const long f = run_fn(kl, c8, kt * BK + kk);
+      return f < 0 ? -1 : f * Mc + m;
     };
+    auto baddr = [&](int n, int kt, int kk) -> long {
+      const long f = run_fn(kl, c8, kt * BK + kk), c = run_fn(crun, ncr, n);
+      return (f < 0 || c < 0) ? -1 : f * NP + c;
+    };
+    const T *phi = (const T *)U.phi, *D = (const T *)U.D;
     typename MF<T>::acc acc[TM][TN];
-    gemm_tile<T, BM, BN, BK, ST, false, false>(phi + m0, U.Mc, D + n0, U.NP, ksteps, kmap, As, Bs,
-                                               acc);
-    density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, U.Mc, U.NP, m0, n0, tn,
-                                                          U.part, red, scr ? U.gbits : nullptr,
-                                                          U.nfw);
+    gemm_tile<T, BM, BN, BK, ST, false, false>(phi + m0, D, ksteps, aaddr, baddr, As, Bs, acc);
+    density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, Mc, U.NP, m0, crun, ncr, ch,
+                                                          U.part, red);
     item_log(S, it, t0);
   }
 }
@@ -593,8 +596,10 @@ __global__ __launch_bounds__(kThreads) void k_functional(const UnitD *__restrict
     const double *part = U.part;
     double *state = U.state, *coef = U.coef;  // state is offset to the unit's first point
     double rho = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    const int ntiles = U.ntiles;
-    for (int t = 0; t < ntiles; ++t) {
+    // partial sums of the density items of this point's tile, one per chunk
+    const int tile = (int)(p / kScreenTile);
+    const int nch = (U.tl_off[tile + 1] - U.tl_off[tile] + U.CH - 1) / U.CH;
+    for (int t = 0; t < nch; ++t) {
       rho += part[((long)t * 4 + 0) * Mc + p];
       if (GGA) {
         g0 += part[((long)t * 4 + 1) * Mc + p];
@@ -695,7 +700,7 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
         W[q] = (T)v;
       }
     };
-    if (U.screen) {  // a block cut off on the point's group: W = 0 without reading Phi
+    {  // a block cut off on the point's group: W = 0 without reading Phi
       const long b = f0 / kGroup;
       if (!((U.gbits[(p / kGroup) * U.nfw + (b >> 5)] >> (b & 31)) & 1u)) {
 #pragma unroll 4
@@ -729,6 +734,7 @@ __global__ __launch_bounds__(kThreads) void k_potential(const StageList S) {
   constexpr int R = kGroup / BK;  // K steps per 32-point group
   __shared__ __attribute__((aligned(16))) T As[lds_elems<T, BT, BK, ST, true>()];
   __shared__ __attribute__((aligned(16))) T Bs[lds_elems<T, BT, BK, ST, true>()];
+  __shared__ uint16_t ul[kMaxRuns];
   __shared__ uint16_t gl[kMaxSplitGroups];
   __shared__ int wtot[4];
   __shared__ long slot;
@@ -737,24 +743,33 @@ __global__ __launch_bounds__(kThreads) void k_potential(const StageList S) {
     const long it = walk_item(S, L);
     const int u = find_unit(S.base, S.nu, it);
     const UnitD &U = S.units[u];
-    const long local = it - S.base[u];
-    const int nt = U.NP / BT, nt2 = nt * nt;
-    const int sl = (int)(local / nt2), tile = (int)(local % nt2);
-    const int i0 = (tile / nt) * BT, j0 = (tile % nt) * BT;
-    const int kdense = (int)(U.Mc / BK) / U.nsplit;
-    const long kb = (long)sl * kdense * BK, Mc = U.Mc;
-    const int ksteps = U.screen ? potential_screen(U, kb, kdense * BK, i0 / BT, j0 / BT, gl, wtot) * R
-                                : kdense;
-    const bool scr = ksteps < kdense;  // a complete list is the dense walk
-    auto kmap = [&](int kt) -> long {
-      return scr ? (long)gl[kt / R] * kGroup + (kt % R) * BK : (long)kt * BK;
+    const uint32_t e = U.pitems[it - S.base[u]];
+    const int sl = (int)(e >> 20), ci = (int)((e >> 10) & 1023), cj = (int)(e & 1023);
+    const long Mc = U.Mc;
+    const int so = U.su_off[sl], nu = U.su_off[sl + 1] - so, CH = U.CH;
+    const int nch = (nu + CH - 1) / CH, G = (int)(Mc / U.nsplit) / kGroup;
+    const long kb = (long)sl * G * kGroup;  // first point of the split
+    __syncthreads();  // the previous item's readers of ul / gl are done
+    lds_list(U.su + so, nu, ul);
+    const int ng = potential_groups(U.gcf + U.gcf_off[sl], G, nch, ci, cj, gl, wtot);
+    const uint16_t *ri = ul + ci * CH, *rj = ul + cj * CH;
+    const int nri = min(CH, nu - ci * CH), nrj = min(CH, nu - cj * CH);
+    // A(m, k) = W[f_i(m)][kb + p(k)], B(k, n) = Phi[f_j(n)][kb + p(k)]: rows from
+    // the split's runs, K over the groups where both chunks can be nonzero
+    auto kpt = [&](int kt, int kk) -> long { return (long)gl[kt / R] * kGroup + (kt % R) * BK + kk; };
+    auto aaddr = [&](int m, int kt, int kk) -> long {
+      const long f = run_fn(ri, nri, m);
+      return f < 0 ? -1 : f * Mc + kb + kpt(kt, kk);
+    };
+    auto baddr = [&](int n, int kt, int kk) -> long {
+      const long f = run_fn(rj, nrj, n);
+      return f < 0 ? -1 : f * Mc + kb + kpt(kt, kk);
     };
     const T *W = (const T *)U.W, *phi = (const T *)U.phi;
     typename MF<T>::acc acc[TT][TT];
-    gemm_tile<T, BT, BT, BK, ST, true, true>(W + (long)i0 * Mc + kb, Mc, phi + (long)j0 * Mc + kb,
-                                             Mc, ksteps, kmap, As, Bs, acc);
+    gemm_tile<T, BT, BT, BK, ST, true, true>(W, phi, ng * R, aaddr, baddr, As, Bs, acc);
     potential_epilogue<T, typename MF<T>::acc, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP,
-                                                       U.NP, i0, j0, U.first);
+                                                       U.NP, ri, nri, rj, nrj);
     item_log(S, it, t0);
   }
 }
@@ -951,25 +966,26 @@ PGXC_INST(float)
 // ---------------------------------------------------------------- screening
 // item = one 128-point tile (four 32-point groups).  The smallest r^2 of each
 // group to each atom (FP64, reduced over the group's 32 lanes), then per
-// shell and group alpha_min r^2_min < cut-off marks the shell's function
-// blocks.  Outputs: the tile's ascending block list and count, n_b (functions
-// of the shells marked on the tile), per group its block flags (gbits) and,
-// per function tile, a flag per group.
+// shell and group alpha_min r^2_min < cut-off marks the shell's runs of 8
+// functions.  Outputs per group: the run flags (nw8 words, sig8) and the
+// 32-function block flags (nfw words, gbits); per tile: n_b, the functions
+// of the shells marked on any of its groups.
 __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *__restrict__ gx,
                                                      const double *__restrict__ gy,
                                                      const double *__restrict__ gz, int n_atoms,
                                                      const AtomD *__restrict__ atoms, int n_shells,
                                                      const ScreenShell *__restrict__ shells,
-                                                     int nfb, int tf, int ntile_f, long gstride,
-                                                     uint16_t *klist, int *kcount, int *nb,
-                                                     uint8_t *gsig, uint32_t *gbits) {
+                                                     int n8, int *nb, uint32_t *sig8,
+                                                     uint32_t *gbits) {
   constexpr int AC = 512;  // atoms per pass
+  constexpr int W8 = kMaxRuns / 32;
   __shared__ double r2g[4][AC];
-  __shared__ unsigned bits[4][kMaxBlocks / 32];
+  __shared__ unsigned bits[4][W8];
   __shared__ int nbs;
   const int tid = threadIdx.x;
+  const int nw8 = (n8 + 31) / 32, nfb = n8 / 4, nfw = (nfb + 31) / 32;
   for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    for (int i = tid; i < 4 * (kMaxBlocks / 32); i += kThreads) (&bits[0][0])[i] = 0u;
+    for (int i = tid; i < 4 * W8; i += kThreads) (&bits[0][0])[i] = 0u;
     if (tid == 0) nbs = 0;
     const long p = t * kScreenTile + (tid & 127);
     const double x = gx[p], y = gy[p], z = gz[p];
@@ -990,7 +1006,7 @@ __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *
         const ScreenShell S = shells[si];
         if (S.atom < a0 || S.atom >= a0 + na) continue;
         bool any = false;
-        const int b0 = S.func_off / kGroup, b1 = (S.func_off + S.nfunc - 1) / kGroup;
+        const int b0 = S.func_off / kRun, b1 = (S.func_off + S.nfunc - 1) / kRun;
         for (int q = 0; q < 4; ++q) {
           if (S.amin * r2g[q][S.atom - a0] < kCutoff) {
             any = true;
@@ -1001,29 +1017,19 @@ __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *
       }
     }
     __syncthreads();
-    if (tid == 0) {
-      int cnt = 0;
-      for (int w = 0; w < (nfb + 31) / 32; ++w) {
-        unsigned m = bits[0][w] | bits[1][w] | bits[2][w] | bits[3][w];
-        while (m) {
-          klist[t * nfb + cnt++] = (uint16_t)(w * 32 + __builtin_ctz(m));
-          m &= m - 1;
-        }
-      }
-      kcount[t] = cnt;
-      nb[t] = nbs;
+    if (tid == 0) nb[t] = nbs;
+    for (int i = tid; i < 4 * nw8; i += kThreads) {
+      const int q = i / nw8, w = i % nw8;
+      sig8[(t * 4 + q) * nw8 + w] = bits[q][w];
     }
-    const int nfw = (nfb + 31) / 32;
     for (int i = tid; i < 4 * nfw; i += kThreads) {
       const int q = i / nfw, w = i % nfw;
-      gbits[(t * 4 + q) * nfw + w] = bits[q][w];
-    }
-    const int bpt = tf / kGroup;
-    for (int i = tid; i < 4 * ntile_f; i += kThreads) {
-      const int q = i & 3, ft = i >> 2;
       unsigned f = 0;
-      for (int b = ft * bpt; b < (ft + 1) * bpt; ++b) f |= (bits[q][b >> 5] >> (b & 31)) & 1u;
-      gsig[(long)ft * gstride + t * 4 + q] = (uint8_t)f;
+      for (int k = 0; k < 32; ++k) {
+        const int b = w * 32 + k;  // 32-function block = runs 4b .. 4b+3
+        if (b < nfb && ((bits[q][b >> 3] >> ((4 * b) & 31)) & 0xFu)) f |= 1u << k;
+      }
+      gbits[(t * 4 + q) * nfw + w] = f;
     }
     __syncthreads();
   }
@@ -1031,12 +1037,10 @@ __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *
 
 void launch_screen(hipStream_t s, long ntiles_pts, const double *gx, const double *gy,
                    const double *gz, int n_atoms, const AtomD *atoms, int n_shells,
-                   const ScreenShell *shells, int nfb, int tf, int ntile_f, long gstride,
-                   uint16_t *klist, int *kcount, int *nb, uint8_t *gsig, uint32_t *gbits) {
+                   const ScreenShell *shells, int n8, int *nb, uint32_t *sig8, uint32_t *gbits) {
   if (ntiles_pts <= 0) return;
   hipLaunchKernelGGL(k_screen, dim3(persistent_grid(ntiles_pts, 8)), dim3(kThreads), 0, s,
-                     ntiles_pts, gx, gy, gz, n_atoms, atoms, n_shells, shells, nfb, tf, ntile_f,
-                     gstride, klist, kcount, nb, gsig, gbits);
+                     ntiles_pts, gx, gy, gz, n_atoms, atoms, n_shells, shells, n8, nb, sig8, gbits);
 }
 
 // ------------------------------------------------------------ small kernels

@@ -35,7 +35,7 @@ def _ptr(a):
 
 class XCBuilder:
     """tile: function tile of the contractions (64/96/128, 0 = least padding);
-    chunk_points: points per work chunk (multiple of 256, 0 = default);
+    chunk_points: points per work chunk (multiple of 2048, 0 = default);
     variants: (FP64, FP32) contraction kernel variants (-1 = default);
     screen: skip the blocks the radial cut-off makes exactly zero;
     basis_cache: keep every chunk's basis values on the device across builds."""

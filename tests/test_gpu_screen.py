@@ -1,10 +1,12 @@
-"""Screening: the contractions skip 32-function x 32-point blocks on which
-the radial cut-off (alpha r^2 >= 50, ParaGauss modules/orbital_module.f90:
-1839-1855) makes every basis value exactly zero.  The skipped terms are exact
-zeros, so screened builds must reproduce dense builds bit for bit in every
-precision, and the FP64 oracle (which applies the same cut-off per point)
-to the FP64 bar.  The algorithmic work reported by pgxc_last_stats is checked
-against n_b counted here from the same cut-off rule."""
+"""Screening: the contractions work on gathered runs of 8 basis functions,
+leaving out the runs on which the radial cut-off (alpha r^2 >= 50, ParaGauss
+modules/orbital_module.f90:1839-1855) makes every basis value exactly zero.
+The left-out terms are exact zeros, so screened builds reproduce dense builds
+up to the rounding of the regrouped sums (FP64 to 1e-12 relative; the mixed
+forms to their own rounding level), and the FP64 oracle (which applies the
+same cut-off per point) to the FP64 bar.  The algorithmic work reported by
+pgxc_last_stats is checked against n_b counted here from the same cut-off
+rule."""
 import os
 
 import numpy as np
@@ -72,8 +74,13 @@ def screened_flops(b, g):
     return f
 
 
+# relative agreement of screened and dense builds: FP64 rounding of regrouped
+# sums; FP32 / split-BF16 accumulate each MFMA K step in FP32
+REL = {FP64: 1e-12, FP32: 2e-6, BF16X3: 2e-6}
+
+
 @pytest.mark.parametrize("prec", [FP64, FP32, BF16X3])
-def test_screened_equals_dense_bitwise(dimer, prec):
+def test_screened_equals_dense(dimer, prec):
     _, _, b, g, C, occ = dimer
     out = []
     for screen in (True, False):
@@ -81,8 +88,10 @@ def test_screened_equals_dense_bitwise(dimer, prec):
             xc.set_orbitals(C, occ)
             out.append(xc.build(PBE, prec))
     (h1, e1, q1), (h2, e2, q2) = out
-    assert np.array_equal(h1, h2)
-    assert e1 == e2 and q1 == q2
+    print("dE=%.3e dV=%.3e" % (e1 - e2, np.max(np.abs(h1 - h2))), flush=True)
+    assert abs(e1 - e2) <= REL[prec] * abs(e2)
+    assert np.max(np.abs(h1 - h2)) <= REL[prec] * np.max(np.abs(h2))
+    assert abs(q1 - q2) <= REL[prec] * abs(q2)
 
 
 def test_screened_incremental_equals_dense(dimer):
@@ -95,7 +104,9 @@ def test_screened_incremental_equals_dense(dimer):
             xc.build(PBE, FP64, fetch=False)
             xc.set_orbitals(C2, occ, incremental=True)
             out.append(xc.build(PBE, BF16X3))
-    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+    # the incremental difference is ~1e-3 of the result: its rounding is far smaller
+    assert abs(out[0][1] - out[1][1]) <= 1e-10 * abs(out[1][1])
+    assert np.max(np.abs(out[0][0] - out[1][0])) <= 1e-10 * np.max(np.abs(out[1][0]))
 
 
 def test_screened_fp64_matches_oracle(dimer):
