@@ -159,10 +159,13 @@ __device__ __forceinline__ int find_unit_x3(const long *__restrict__ base, int n
 
 // item = (128-point tile, function tile), as k_density
 template <int BN, bool GRAD>
-__global__ __launch_bounds__(kThreads) void k_density_bf16x3(const StageList S) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_density_bf16x3(const StageList S) {
   constexpr int BM = kXBM;
   __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BM, BN>::n];
-  __shared__ double red[2][4][BM];
+  // the epilogue's reduction buffer reuses the staging LDS (free after the
+  // contraction), so two workgroups fit a CU's 160 KB
+  static_assert(sizeof(double) * 2 * 4 * BM <= sizeof(__bf16) * 2 * XStage<BM, BN>::n, "alias");
+  double(&red)[2][4][BM] = *reinterpret_cast<double(*)[2][4][BM]>(lds);
   __shared__ uint16_t kl[kMaxRuns];
   __shared__ long slot;
   static_assert(kXBK == 4 * kRun, "four runs per K step");
