@@ -4,7 +4,8 @@
 // molecule taking part, and per stage the list of work items over those units.
 // Stage kernels walk their lists persistently; one molecule is a batch of one.
 // Per super-step:  basis -> density contraction -> functional -> W ->
-// potential contraction; then one packing into the lower triangle per molecule.
+// potential contraction; then per molecule a fold of the split partials (H + H^T)
+// and one packing into the lower triangle.
 // Chunks are whole numbers of reference batches, so the sums are those of
 // calc_xcks_nl (ParaGauss modules/xc_hamiltonian.f90:1348-1575), regrouped.
 #include <hip/hip_runtime.h>
@@ -90,7 +91,7 @@ struct pgxc_ctx {
   // screening and work tables over the padded grid (setup_screen)
   int screen = 0, n8 = 0, nfw = 0, CH = 0;
   int *nbt = nullptr;
-  uint32_t *sig8 = nullptr, *gbits = nullptr;
+  uint32_t *sig8 = nullptr, *gbits = nullptr, *wbits = nullptr;
   ScreenShell *sshells = nullptr;
   double flops_screened = 0, flops_dense = 0;  // algorithmic work of one build
   // host and device copies: per tile run lists, per split run unions and
@@ -167,8 +168,8 @@ struct Plan {
   double tick_us = 0.0;
   std::vector<StepPlan> steps;
   PackD *d_pack = nullptr;
-  long *d_pack_base = nullptr, *d_prep_base = nullptr;
-  long pack_total = 0, prep_total = 0;
+  long *d_pack_base = nullptr, *d_prep_base = nullptr, *d_fold_base = nullptr;
+  long pack_total = 0, prep_total = 0, fold_total = 0;
   int nmol = 0;
   char *blob = nullptr;  // device memory holding every list
   hipGraphExec_t graph = nullptr;
@@ -407,6 +408,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.gcf_off = c->gcf_off + (long)k * c->nsplit;
       U.gcf = c->gcf;
       U.gbits = c->gbits + (p0 / kGroup) * c->nfw;
+      U.wbits = c->wbits + (p0 / kGroup) * c->nfw;
       units.push_back(U);
       umol.push_back(m);
     }
@@ -492,7 +494,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   }
   // per-molecule operand preparation and packing
   std::vector<PackD> packs(nm);
-  std::vector<long> pbase(nm + 1, 0), qbase(nm + 1, 0);
+  std::vector<long> pbase(nm + 1, 0), qbase(nm + 1, 0), fbase(nm + 1, 0);
   for (int m = 0; m < nm; ++m) {
     pgxc_ctx *c = mols[m];
     PackD &Q = packs[m];
@@ -510,12 +512,15 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     const long npk = (long)c->nbas * (c->nbas + 1) / 2;
     pbase[m + 1] = pbase[m] + (npk + kThreads - 1) / kThreads;
     qbase[m + 1] = qbase[m] + ((long)c->NP * c->NP + kThreads - 1) / kThreads;
+    const long nb = c->NP / 32;
+    fbase[m + 1] = fbase[m] + nb * (nb + 1) / 2;
   }
   for (int m = 0; m < nm; ++m)
     plan->hzero.push_back({(int *)mols[m]->Hpart, 2L * mols[m]->nsplit * mols[m]->NP * mols[m]->NP});
   const size_t pack_off = put(packs.data(), sizeof(PackD) * nm);
   const size_t pb_off = put(pbase.data(), sizeof(long) * pbase.size());
   const size_t qb_off = put(qbase.data(), sizeof(long) * qbase.size());
+  const size_t fb_off = put(fbase.data(), sizeof(long) * fbase.size());
   HIPCHK(errc, hipMalloc((void **)&plan->blob, std::max<size_t>(img.size(), 256)));
   HIPCHK(errc, hipMemcpy(plan->blob, img.data(), img.size(), hipMemcpyHostToDevice));
   char *B = plan->blob;
@@ -547,6 +552,8 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   plan->d_pack = (PackD *)(B + pack_off);
   plan->d_pack_base = (long *)(B + pb_off);
   plan->d_prep_base = (long *)(B + qb_off);
+  plan->d_fold_base = (long *)(B + fb_off);
+  plan->fold_total = fbase[nm];
   plan->pack_total = pbase[nm];
   plan->prep_total = qbase[nm];
   return PGXC_OK;
@@ -627,6 +634,7 @@ static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hip
     if (E) (void)hipEventRecord(E[4], s);
   }
   if (ev) (void)hipEventRecord((*ev)[kEvPack], s);
+  launch_fold(s, pl.d_pack, pl.nmol, pl.d_fold_base, pl.fold_total);
   launch_pack(s, pl.d_pack, pl.nmol, pl.d_pack_base, pl.pack_total);
   launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total, 0, true);
   if (ev) (void)hipEventRecord((*ev)[kEvEnd], s);
@@ -900,6 +908,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   }
   // per split: union of its groups' runs, [group][chunk] flags
   const long NS = (long)c->nchunks * c->nsplit;
+  std::vector<uint32_t> wneed((size_t)ngr * c->nfw, 0u);
   c->h_su_off.assign(1, 0);
   c->h_su.clear();
   c->h_gcf_off.clear();
@@ -920,8 +929,15 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
         uint8_t f = 0;
         for (int e = ch * CH; e < std::min(nu, (ch + 1) * CH) && !f; ++e) f = (uint8_t)bit(g, c->h_su[u0 + e]);
         c->h_gcf.push_back(f);
+        // the potential contraction reads W on this group for every run of a flagged chunk
+        if (f)
+          for (int e = ch * CH; e < std::min(nu, (ch + 1) * CH); ++e) {
+            const int b = c->h_su[u0 + e] * kRun / 32;
+            wneed[(size_t)g * c->nfw + b / 32] |= 1u << (b % 32);
+          }
       }
   }
+  if ((rc = upload(c, &c->wbits, wneed))) return rc;
   // work items per chunk
   const long tpc = c->Mc / kScreenTile;
   c->h_di_off.assign(1, 0);
@@ -1303,7 +1319,7 @@ int pgxc_destroy(pgxc_ctx *c) {
   void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
-                  c->phic[0], c->phic[1], c->gbits, c->tl_off, c->tl, c->su_off, c->su,
+                  c->phic[0], c->phic[1], c->gbits, c->wbits, c->tl_off, c->tl, c->su_off, c->su,
                   c->gcf_off, c->gcf, c->di, c->pi};
   for (void *b : bufs) dfree(b);
   for (auto &e : c->ev)

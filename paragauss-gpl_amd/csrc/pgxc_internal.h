@@ -56,13 +56,14 @@ struct UnitD {
   // runs; a chunk is CH = tile / 8 consecutive entries of a list.  gcf_off /
   // gcf: per split, [group][chunk] flags, whether the chunk can be nonzero on
   // the group.  gbits: per 32-point group, nfw words of 32-function block
-  // flags (W kernel).
+  // flags (W kernel).  wbits: the same shape, blocks the potential
+  // contraction reads on the group (W is written there, 0 where cut off).
   const uint32_t *ditems, *pitems;
   const int *tl_off, *su_off;
   const uint16_t *tl, *su;
   const long *gcf_off;
   const uint8_t *gcf;
-  const uint32_t *gbits;
+  const uint32_t *gbits, *wbits;
   long Mc, npad;
   int n_atoms, NP, ntiles, nsplit, incremental;
   uint32_t mask;
@@ -72,7 +73,7 @@ struct UnitD {
 
 // Per-molecule descriptors of the operand preparation and the final packing.
 struct PackD {
-  const double *Hpart;
+  double *Hpart;  // split partials; the fold leaves the symmetric sum in split 0
   double *Vout, *Vstate;
   const double *P;
   double *Pref;
@@ -117,6 +118,7 @@ template <typename T>
 void launch_potential(hipStream_t s, int tf, int var, const StageList &L);
 void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L);
 void launch_potential_bf16x3(hipStream_t s, int tf, const StageList &L);
+void launch_fold(hipStream_t s, const PackD *mols, int nm, const long *base, long total);
 void launch_pack(hipStream_t s, const PackD *mols, int nm, const long *base, long total);
 void launch_zero_ints(hipStream_t s, int *p, long n);
 unsigned contraction_grid(const StageList &L, int per_cu);
@@ -131,7 +133,7 @@ void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, lon
 // item counts per unit, shared by the planner and the kernels' decoding
 inline long items_basis(long Mc, int n_atoms) { return (Mc / kThreads) * n_atoms; }
 inline long items_functional(long Mc) { return Mc / kThreads; }
-inline long items_wmat(long Mc, int NP) { return (Mc / kThreads) * (NP / 16); }
+inline long items_wmat(long Mc, int NP) { return (Mc / (4 * kThreads)) * (NP / 16); }
 inline long items_split(long Mc, int NP) { return (Mc / 64) * (NP / 32); }
 // point splits of the potential contraction are whole multiples of this
 constexpr int kSplitPts = 2048;

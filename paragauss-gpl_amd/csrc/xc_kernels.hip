@@ -672,55 +672,84 @@ __global__ __launch_bounds__(kThreads) void k_functional(const UnitD *__restrict
 }
 
 // -------------------------------------------------------------------- W
-// item = (256-point block, 16-function group): W = a Phi + b . grad Phi;
-// SPLIT (split-BF16 builds, T = float): W stored as bf16 hi / lo planes
+// item = (1024-point block, 16-function group), 4 consecutive points of one
+// 32-point group per thread: W = a Phi + b . grad Phi.  SPLIT (split-BF16
+// builds, T = float): W stored as bf16 hi / lo planes.  A block cut off on the
+// group is 0 without reading Phi, and is not written at all where the
+// potential contraction never reads it.
 template <typename T, bool GRAD, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ units, int nu,
                                                    const long *__restrict__ base,
                                                    const int *__restrict__ order, long total) {
+  typedef T t4 __attribute__((ext_vector_type(4)));
+  typedef uint16_t h4 __attribute__((ext_vector_type(4)));
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const long it = item_at(order, L, total);
     const int u = find_unit(base, nu, it);
     const UnitD &U = units[u];
     const long local = it - base[u];
     const int ng = U.NP / 16;
-    const long p = (local / ng) * kThreads + threadIdx.x;
+    const long p = (local / ng) * (4 * kThreads) + 4 * threadIdx.x;
     const long f0 = (local % ng) * 16;
     const long Mc = U.Mc, cs = (long)U.NP * Mc;
     const T *phi = (const T *)U.phi;
     T *W = (T *)U.W;
-    __bf16 *Wh = (__bf16 *)U.W, *Wl = Wh + cs;
-    auto put = [&](long q, double v) {
+    uint16_t *Wh = (uint16_t *)U.W, *Wl = Wh + cs;
+    auto put = [&](long q, const double (&v)[4]) {
       if constexpr (SPLIT) {
-        __bf16 hi, lo;
-        split_bf16((float)v, hi, lo);
-        Wh[q] = hi;
-        Wl[q] = lo;
+        h4 hi, lo;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          __bf16 h, l;
+          split_bf16((float)v[j], h, l);
+          hi[j] = __builtin_bit_cast(uint16_t, h);
+          lo[j] = __builtin_bit_cast(uint16_t, l);
+        }
+        *(h4 *)(Wh + q) = hi;
+        *(h4 *)(Wl + q) = lo;
       } else {
-        W[q] = (T)v;
+        t4 w;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w[j] = (T)v[j];
+        *(t4 *)(W + q) = w;
       }
     };
-    {  // a block cut off on the point's group: W = 0 without reading Phi
-      const long b = f0 / kGroup;
-      if (!((U.gbits[(p / kGroup) * U.nfw + (b >> 5)] >> (b & 31)) & 1u)) {
+    const long wd = (p / kGroup) * U.nfw + (f0 >> 10);
+    const uint32_t bit = 1u << ((f0 >> 5) & 31);
+    if (!(U.gbits[wd] & bit)) {
+      if (U.wbits[wd] & bit) {
+        const double z[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
-        for (int k = 0; k < 16; ++k) put((f0 + k) * Mc + p, 0.0);
-        continue;
+        for (int k = 0; k < 16; ++k) put((f0 + k) * Mc + p, z);
       }
+      continue;
     }
-    const double a = U.coef[p];
-    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
-    if (GRAD) {
-      b0 = U.coef[Mc + p];
-      b1 = U.coef[2 * Mc + p];
-      b2 = U.coef[3 * Mc + p];
+    double a[4], b0[4] = {}, b1[4] = {}, b2[4] = {};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a[j] = U.coef[p + j];
+      if (GRAD) {
+        b0[j] = U.coef[Mc + p + j];
+        b1[j] = U.coef[2 * Mc + p + j];
+        b2[j] = U.coef[3 * Mc + p + j];
+      }
     }
 #pragma unroll 4
     for (int k = 0; k < 16; ++k) {
       const long q = (f0 + k) * Mc + p;
-      double v = a * (double)phi[q];
-      if (GRAD)
-        v = v + b0 * (double)phi[cs + q] + b1 * (double)phi[2 * cs + q] + b2 * (double)phi[3 * cs + q];
+      const t4 v0 = *(const t4 *)(phi + q);
+      double v[4];
+      if (GRAD) {
+        const t4 v1 = *(const t4 *)(phi + cs + q), v2 = *(const t4 *)(phi + 2 * cs + q),
+                 v3 = *(const t4 *)(phi + 3 * cs + q);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          v[j] = a[j] * (double)v0[j] + b0[j] * (double)v1[j] + b1[j] * (double)v2[j] +
+                 b2[j] * (double)v3[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = a[j] * (double)v0[j];
+      }
       put(q, v);
     }
   }
@@ -775,9 +804,51 @@ __global__ __launch_bounds__(kThreads) void k_potential(const StageList S) {
 }
 
 // -------------------------------------------------------------------- pack
+// Fold, item = 32 x 32 tile (rb >= cb) of one molecule's internal matrix:
+// S(r, c) = sum_split H(r, c) + sum_split H(c, r) for both the tile and its
+// mirror, read along rows; the mirror's sums change places through LDS.  S
+// replaces split 0's partial on the tile (only this item reads that tile and
+// its mirror, and it writes after every read).
+__global__ __launch_bounds__(kThreads) void k_fold(const PackD *__restrict__ mols, int nm,
+                                                   const long *__restrict__ base, long total) {
+  __shared__ double t[32][33];
+  const int tid = threadIdx.x;
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const int m = find_unit(base, nm, L);
+    const PackD &M = mols[m];
+    long e = L - base[m], rb = (long)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+    while (rb * (rb + 1) / 2 > e) --rb;
+    while ((rb + 1) * (rb + 2) / 2 <= e) ++rb;
+    const long cb = e - rb * (rb + 1) / 2, NP = M.NP, np2 = NP * NP;
+    double sa[4] = {}, sb[4] = {};
+    for (int s = 0; s < M.nsplit; ++s) {
+      const double *H = M.Hpart + (long)s * np2;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = tid + kThreads * i, r = x >> 5, c = x & 31;
+        sa[i] += H[(rb * 32 + r) * NP + cb * 32 + c];
+        sb[i] += H[(cb * 32 + r) * NP + rb * 32 + c];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int x = tid + kThreads * i;
+      t[x >> 5][x & 31] = sb[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int x = tid + kThreads * i, r = x >> 5, c = x & 31;
+      M.Hpart[(rb * 32 + r) * NP + cb * 32 + c] = sa[i] + t[c][r];
+    }
+    __syncthreads();
+  }
+}
+
 // item = 256 packed elements of one molecule (caller's function order):
-// V = base + sum_split (H + H^T), H read at the functions' internal rows;
-// then the molecule's state for the next incremental build: V_state = V.
+// V = base + S at the functions' internal rows (S folded above, symmetric,
+// read from its lower tiles); then the molecule's state for the next
+// incremental build: V_state = V.
 __global__ __launch_bounds__(kThreads) void k_pack(const PackD *__restrict__ mols, int nm,
                                                    const long *__restrict__ base, long total) {
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
@@ -791,12 +862,8 @@ __global__ __launch_bounds__(kThreads) void k_pack(const PackD *__restrict__ mol
     while ((j + 1) * (j + 2) / 2 <= idx) ++j;
     const long k = idx - j * (j + 1) / 2;
     const long pj = M.fpos[j], pk = M.fpos[k];  // internal rows of caller functions j, k
-    double hjk = 0.0, hkj = 0.0;
-    for (int s = 0; s < M.nsplit; ++s) {
-      hjk += M.Hpart[(long)s * NP * NP + pj * NP + pk];
-      hkj += M.Hpart[(long)s * NP * NP + pk * NP + pj];
-    }
-    const double v = hjk + hkj;
+    const long r = pj > pk ? pj : pk, c = pj > pk ? pk : pj;
+    const double v = M.Hpart[r * NP + c];
     const double out = M.incremental ? M.Vstate[idx] + v : v;
     M.Vout[idx] = out;
     M.Vstate[idx] = out;
@@ -940,6 +1007,11 @@ void launch_zero_ints(hipStream_t s, int *p, long n) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_zero_ints, dim3(persistent_grid((n + kThreads - 1) / kThreads, 1)),
                      dim3(kThreads), 0, s, p, n);
+}
+
+void launch_fold(hipStream_t s, const PackD *mols, int nm, const long *base, long total) {
+  if (!total) return;
+  hipLaunchKernelGGL(k_fold, dim3(persistent_grid(total, 8)), dim3(kThreads), 0, s, mols, nm, base, total);
 }
 
 void launch_pack(hipStream_t s, const PackD *mols, int nm, const long *base, long total) {
