@@ -89,9 +89,9 @@ struct pgxc_ctx {
   Plan *plan = nullptr;
   double exps_per_point = 0;
   // screening and work tables over the padded grid (setup_screen)
-  int screen = 0, n8 = 0, nfw = 0, CH = 0;
+  int screen = 0, n8 = 0, nw8 = 0, CH = 0;
   int *nbt = nullptr;
-  uint32_t *sig8 = nullptr, *gbits = nullptr, *wbits = nullptr;
+  uint32_t *sig8 = nullptr, *wbits = nullptr;
   ScreenShell *sshells = nullptr;
   double flops_screened = 0, flops_dense = 0;  // algorithmic work of one build
   // host and device copies: per tile run lists, per split run unions and
@@ -396,7 +396,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.incremental = incr[m];
       U.mask = functional;
       U.CH = c->CH;
-      U.nfw = c->nfw;
+      U.nw8 = c->nw8;
       U.ditems = c->di + c->h_di_off[k];
       U.pitems = c->pi + c->h_pi_off[k];
       U.ndi = c->h_di_off[k + 1] - c->h_di_off[k];
@@ -407,8 +407,8 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.su = c->su;
       U.gcf_off = c->gcf_off + (long)k * c->nsplit;
       U.gcf = c->gcf;
-      U.gbits = c->gbits + (p0 / kGroup) * c->nfw;
-      U.wbits = c->wbits + (p0 / kGroup) * c->nfw;
+      U.sig8 = c->sig8 + (p0 / kGroup) * c->nw8;
+      U.wbits = c->wbits + (p0 / kGroup) * c->nw8;
       units.push_back(U);
       umol.push_back(m);
     }
@@ -850,9 +850,9 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   c->flops_dense = 4.0 * c->npts * n * n + 16.0 * c->npts * n;
   c->n8 = c->NP / kRun;
   c->CH = c->tf / kRun;
-  c->nfw = (c->NP / 32 + 31) / 32;
   c->screen = on;
-  const int n8 = c->n8, nw8 = (n8 + 31) / 32, CH = c->CH;
+  c->nw8 = (c->n8 + 31) / 32;
+  const int n8 = c->n8, nw8 = c->nw8, CH = c->CH;
   const long G = c->Mc / c->nsplit / kGroup;
   if (n8 > kMaxRuns) return fail(c, PGXC_ERR_UNSUPPORTED, "more than 8192 basis functions (padded)");
   if (G > kMaxSplitGroups) return fail(c, PGXC_ERR_UNSUPPORTED, "point split above 65536 points");
@@ -870,9 +870,8 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   if ((rc = upload(c, &c->sshells, ss))) return rc;
   if ((rc = dalloc(c, &c->nbt, ntp))) return rc;
   if ((rc = dalloc(c, &c->sig8, (size_t)ngr * nw8))) return rc;
-  if ((rc = dalloc(c, &c->gbits, (size_t)ngr * c->nfw))) return rc;
   launch_screen(c->stream, ntp, c->grid, c->grid + c->npad, c->grid + 2 * c->npad, c->n_atoms,
-                c->atoms, c->n_shells, c->sshells, n8, c->nbt, c->sig8, c->gbits);
+                c->atoms, c->n_shells, c->sshells, n8, c->nbt, c->sig8);
   HIPCHK(c, hipGetLastError());
   std::vector<int> nb((size_t)ntp);
   std::vector<uint32_t> sig((size_t)ngr * nw8);
@@ -891,10 +890,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
     c->flops_screened = c->flops_dense;
     for (long g = 0; g < ngr; ++g)
       for (int r = 0; r < n8; ++r) sig[(size_t)g * nw8 + r / 32] |= 1u << (r % 32);
-    std::vector<uint32_t> all((size_t)ngr * c->nfw, 0u);
-    for (long g = 0; g < ngr; ++g)
-      for (int b = 0; b < c->NP / 32; ++b) all[(size_t)g * c->nfw + b / 32] |= 1u << (b % 32);
-    HIPCHK(c, hipMemcpy(c->gbits, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->sig8, sig.data(), sizeof(uint32_t) * sig.size(), hipMemcpyHostToDevice));
   }
   auto bit = [&](long g, int r) { return (sig[(size_t)g * nw8 + r / 32] >> (r % 32)) & 1u; };
   // per 128-point tile: its runs (union over its four groups)
@@ -908,7 +904,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   }
   // per split: union of its groups' runs, [group][chunk] flags
   const long NS = (long)c->nchunks * c->nsplit;
-  std::vector<uint32_t> wneed((size_t)ngr * c->nfw, 0u);
+  std::vector<uint32_t> wneed((size_t)ngr * nw8, 0u);
   c->h_su_off.assign(1, 0);
   c->h_su.clear();
   c->h_gcf_off.clear();
@@ -932,8 +928,8 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
         // the potential contraction reads W on this group for every run of a flagged chunk
         if (f)
           for (int e = ch * CH; e < std::min(nu, (ch + 1) * CH); ++e) {
-            const int b = c->h_su[u0 + e] * kRun / 32;
-            wneed[(size_t)g * c->nfw + b / 32] |= 1u << (b % 32);
+            const int r = c->h_su[u0 + e];
+            wneed[(size_t)g * nw8 + r / 32] |= 1u << (r % 32);
           }
       }
   }
@@ -1319,7 +1315,7 @@ int pgxc_destroy(pgxc_ctx *c) {
   void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
-                  c->phic[0], c->phic[1], c->gbits, c->wbits, c->tl_off, c->tl, c->su_off, c->su,
+                  c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->su_off, c->su,
                   c->gcf_off, c->gcf, c->di, c->pi};
   for (void *b : bufs) dfree(b);
   for (auto &e : c->ev)

@@ -55,19 +55,20 @@ struct UnitD {
   // tl).  su_off / su: per point split of the unit, the union of its groups'
   // runs; a chunk is CH = tile / 8 consecutive entries of a list.  gcf_off /
   // gcf: per split, [group][chunk] flags, whether the chunk can be nonzero on
-  // the group.  gbits: per 32-point group, nfw words of 32-function block
-  // flags (W kernel).  wbits: the same shape, blocks the potential
-  // contraction reads on the group (W is written there, 0 where cut off).
+  // the group.  sig8: per 32-point group (from the unit's first), nw8 words
+  // of run flags, the runs not cut off on the group.  wbits: the same shape,
+  // the runs the potential contraction reads on the group (W is written
+  // there, 0 where cut off).
   const uint32_t *ditems, *pitems;
   const int *tl_off, *su_off;
   const uint16_t *tl, *su;
   const long *gcf_off;
   const uint8_t *gcf;
-  const uint32_t *gbits, *wbits;
+  const uint32_t *sig8, *wbits;
   long Mc, npad;
   int n_atoms, NP, ntiles, nsplit, incremental;
   uint32_t mask;
-  int CH, nfw;
+  int CH, nw8;
   long ndi, npi;  // density / potential items of the unit
 };
 
@@ -158,7 +159,7 @@ struct ScreenShell {
 };
 void launch_screen(hipStream_t s, long ntiles_pts, const double *gx, const double *gy,
                    const double *gz, int n_atoms, const AtomD *atoms, int n_shells,
-                   const ScreenShell *shells, int n8, int *nb, uint32_t *sig8, uint32_t *gbits);
+                   const ScreenShell *shells, int n8, int *nb, uint32_t *sig8);
 
 // P = C diag(occ) C^T into an NP x NP padded array
 void launch_form_density(hipStream_t s, const double *C, const double *occ, int n,

@@ -674,9 +674,9 @@ __global__ __launch_bounds__(kThreads) void k_functional(const UnitD *__restrict
 // -------------------------------------------------------------------- W
 // item = (1024-point block, 16-function group), 4 consecutive points of one
 // 32-point group per thread: W = a Phi + b . grad Phi.  SPLIT (split-BF16
-// builds, T = float): W stored as bf16 hi / lo planes.  A block cut off on the
-// group is 0 without reading Phi, and is not written at all where the
-// potential contraction never reads it.
+// builds, T = float): W stored as bf16 hi / lo planes.  A run of 8 functions
+// cut off on the group is 0 without reading Phi, and is not written at all
+// where the potential contraction never reads it.
 template <typename T, bool GRAD, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ units, int nu,
                                                    const long *__restrict__ base,
@@ -714,43 +714,47 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
         *(t4 *)(W + q) = w;
       }
     };
-    const long wd = (p / kGroup) * U.nfw + (f0 >> 10);
-    const uint32_t bit = 1u << ((f0 >> 5) & 31);
-    if (!(U.gbits[wd] & bit)) {
-      if (U.wbits[wd] & bit) {
-        const double z[4] = {0.0, 0.0, 0.0, 0.0};
+    const uint32_t *sg = U.sig8 + (p / kGroup) * U.nw8, *wn = U.wbits + (p / kGroup) * U.nw8;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {  // the item's two runs of 8 functions
+      const long r = f0 / kRun + h, fr = r * kRun;
+      const uint32_t bit = 1u << (r & 31);
+      if (!(sg[r >> 5] & bit)) {
+        if (wn[r >> 5] & bit) {
+          const double z[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
-        for (int k = 0; k < 16; ++k) put((f0 + k) * Mc + p, z);
+          for (int k = 0; k < kRun; ++k) put((fr + k) * Mc + p, z);
+        }
+        continue;
       }
-      continue;
-    }
-    double a[4], b0[4] = {}, b1[4] = {}, b2[4] = {};
+      double a[4], b0[4] = {}, b1[4] = {}, b2[4] = {};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      a[j] = U.coef[p + j];
-      if (GRAD) {
-        b0[j] = U.coef[Mc + p + j];
-        b1[j] = U.coef[2 * Mc + p + j];
-        b2[j] = U.coef[3 * Mc + p + j];
+      for (int j = 0; j < 4; ++j) {
+        a[j] = U.coef[p + j];
+        if (GRAD) {
+          b0[j] = U.coef[Mc + p + j];
+          b1[j] = U.coef[2 * Mc + p + j];
+          b2[j] = U.coef[3 * Mc + p + j];
+        }
       }
-    }
 #pragma unroll 4
-    for (int k = 0; k < 16; ++k) {
-      const long q = (f0 + k) * Mc + p;
-      const t4 v0 = *(const t4 *)(phi + q);
-      double v[4];
-      if (GRAD) {
-        const t4 v1 = *(const t4 *)(phi + cs + q), v2 = *(const t4 *)(phi + 2 * cs + q),
-                 v3 = *(const t4 *)(phi + 3 * cs + q);
+      for (int k = 0; k < kRun; ++k) {
+        const long q = (fr + k) * Mc + p;
+        const t4 v0 = *(const t4 *)(phi + q);
+        double v[4];
+        if (GRAD) {
+          const t4 v1 = *(const t4 *)(phi + cs + q), v2 = *(const t4 *)(phi + 2 * cs + q),
+                   v3 = *(const t4 *)(phi + 3 * cs + q);
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          v[j] = a[j] * (double)v0[j] + b0[j] * (double)v1[j] + b1[j] * (double)v2[j] +
-                 b2[j] * (double)v3[j];
-      } else {
+          for (int j = 0; j < 4; ++j)
+            v[j] = a[j] * (double)v0[j] + b0[j] * (double)v1[j] + b1[j] * (double)v2[j] +
+                   b2[j] * (double)v3[j];
+        } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = a[j] * (double)v0[j];
+          for (int j = 0; j < 4; ++j) v[j] = a[j] * (double)v0[j];
+        }
+        put(q, v);
       }
-      put(q, v);
     }
   }
 }
@@ -1039,23 +1043,21 @@ PGXC_INST(float)
 // item = one 128-point tile (four 32-point groups).  The smallest r^2 of each
 // group to each atom (FP64, reduced over the group's 32 lanes), then per
 // shell and group alpha_min r^2_min < cut-off marks the shell's runs of 8
-// functions.  Outputs per group: the run flags (nw8 words, sig8) and the
-// 32-function block flags (nfw words, gbits); per tile: n_b, the functions
-// of the shells marked on any of its groups.
+// functions.  Outputs per group: the run flags (nw8 words, sig8); per tile:
+// n_b, the functions of the shells marked on any of its groups.
 __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *__restrict__ gx,
                                                      const double *__restrict__ gy,
                                                      const double *__restrict__ gz, int n_atoms,
                                                      const AtomD *__restrict__ atoms, int n_shells,
                                                      const ScreenShell *__restrict__ shells,
-                                                     int n8, int *nb, uint32_t *sig8,
-                                                     uint32_t *gbits) {
+                                                     int n8, int *nb, uint32_t *sig8) {
   constexpr int AC = 512;  // atoms per pass
   constexpr int W8 = kMaxRuns / 32;
   __shared__ double r2g[4][AC];
   __shared__ unsigned bits[4][W8];
   __shared__ int nbs;
   const int tid = threadIdx.x;
-  const int nw8 = (n8 + 31) / 32, nfb = n8 / 4, nfw = (nfb + 31) / 32;
+  const int nw8 = (n8 + 31) / 32;
   for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
     for (int i = tid; i < 4 * W8; i += kThreads) (&bits[0][0])[i] = 0u;
     if (tid == 0) nbs = 0;
@@ -1094,25 +1096,16 @@ __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *
       const int q = i / nw8, w = i % nw8;
       sig8[(t * 4 + q) * nw8 + w] = bits[q][w];
     }
-    for (int i = tid; i < 4 * nfw; i += kThreads) {
-      const int q = i / nfw, w = i % nfw;
-      unsigned f = 0;
-      for (int k = 0; k < 32; ++k) {
-        const int b = w * 32 + k;  // 32-function block = runs 4b .. 4b+3
-        if (b < nfb && ((bits[q][b >> 3] >> ((4 * b) & 31)) & 0xFu)) f |= 1u << k;
-      }
-      gbits[(t * 4 + q) * nfw + w] = f;
-    }
     __syncthreads();
   }
 }
 
 void launch_screen(hipStream_t s, long ntiles_pts, const double *gx, const double *gy,
                    const double *gz, int n_atoms, const AtomD *atoms, int n_shells,
-                   const ScreenShell *shells, int n8, int *nb, uint32_t *sig8, uint32_t *gbits) {
+                   const ScreenShell *shells, int n8, int *nb, uint32_t *sig8) {
   if (ntiles_pts <= 0) return;
   hipLaunchKernelGGL(k_screen, dim3(persistent_grid(ntiles_pts, 8)), dim3(kThreads), 0, s,
-                     ntiles_pts, gx, gy, gz, n_atoms, atoms, n_shells, shells, n8, nb, sig8, gbits);
+                     ntiles_pts, gx, gy, gz, n_atoms, atoms, n_shells, shells, n8, nb, sig8);
 }
 
 // ------------------------------------------------------------ small kernels
