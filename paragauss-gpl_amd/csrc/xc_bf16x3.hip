@@ -9,6 +9,8 @@
 // k_wmat, D^T by k_prep), so staging is a copy of 16-byte vectors into LDS.
 // Same tiles and epilogues as the FP32/FP64 contractions (xc_kernels.hip);
 // K advances 32 per MFMA.
+#include <type_traits>
+
 #include "pgxc_internal.h"
 #include "xc_device.h"
 
@@ -32,11 +34,17 @@ struct XStage {
 };
 
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef const u32x4 __attribute__((address_space(1))) *gvec;
 
 // Staging of an operand stored split and k-contiguous (bf16 hi and lo planes
 // at the same offsets); per stage R rows x 32 k x 2 planes, moved as 16-byte
-// vectors (runs of 8).  addr(r, kt, kk) gives the element offset of row r's
-// run starting at local k index kk of step kt, or -1 for zeros.
+// vectors (runs of 8).  A thread's vectors keep their rows for the whole
+// item and all take k-run c = threadIdx.x & 3 of a step, so an element offset
+// is row part + k part: the row pointers (plane included) are set once per
+// item, and a step supplies one k offset.  Loads are unconditional global
+// loads (a missing row or k run reads the plane's first vector instead) and
+// the zeros are applied when the vector is written to LDS, so no load sits in
+// a branch and the loads of later steps stay in flight.
 template <int R>
 struct PSOperand {
   struct Src {
@@ -45,65 +53,72 @@ struct PSOperand {
   static constexpr int NV = R * (kXBK / 8) * 2;  // 16-byte vectors per stage
   static constexpr int PER = NV / kThreads;
   static_assert(PER * kThreads == NV, "whole vectors per thread");
-  u32x4 v[PER];
+  static_assert((R * 4) % 4 == 0 && kThreads % 4 == 0, "k run = threadIdx.x & 3");
+  const __bf16 *row[PER];  // plane + row offset of vector e, nullptr: row absent
+  const __bf16 *dummy;
+  u32x4 v[2][PER];
+  unsigned ok[2];
 
-  __device__ __forceinline__ static void place(int idx, int &plane, int &row, int &c) {
+  __device__ __forceinline__ static void place(int idx, int &plane, int &r, int &c) {
     plane = idx / (R * 4);
     const int rem = idx % (R * 4);
-    row = rem >> 2;
+    r = rem >> 2;
     c = rem & 3;
   }
-  template <class Addr>
-  __device__ __forceinline__ void load(const Src &S, const Addr &addr, int kt) {
+  // rowf(r): element offset of row r (k = 0), or -1
+  template <class RowF>
+  __device__ __forceinline__ void init(const Src &S, const RowF &rowf) {
+    dummy = S.hi;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      int plane, row, c;
-      place(threadIdx.x + kThreads * e, plane, row, c);
-      const long o = addr(row, kt, 8 * c);
-      const __bf16 *P = plane ? S.lo : S.hi;
-      v[e] = o >= 0 ? *(const u32x4 *)(P + o) : u32x4{0u, 0u, 0u, 0u};
+      int plane, r, c;
+      place(threadIdx.x + kThreads * e, plane, r, c);
+      const long o = rowf(r);
+      row[e] = o >= 0 ? (plane ? S.lo : S.hi) + o : nullptr;
     }
   }
+  // ko: element offset of this thread's k run in the step, or -1
+  template <int SL>
+  __device__ __forceinline__ void load(long ko) {
+    unsigned m = 0;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const bool g = row[e] != nullptr && ko >= 0;
+      v[SL][e] = *(gvec)(g ? row[e] + ko : dummy);
+      m |= (unsigned)g << e;
+    }
+    ok[SL] = m;
+  }
+  template <int SL>
   __device__ __forceinline__ void store(__bf16 *hi, __bf16 *lo) const {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      int plane, row, c;
-      place(threadIdx.x + kThreads * e, plane, row, c);
-      *(u32x4 *)((plane ? lo : hi) + row * kXBKP + 8 * c) = v[e];
+      int plane, r, c;
+      place(threadIdx.x + kThreads * e, plane, r, c);
+      const u32x4 x = ((ok[SL] >> e) & 1u) ? v[SL][e] : u32x4{0u, 0u, 0u, 0u};
+      *(u32x4 *)((plane ? lo : hi) + r * kXBKP + 8 * c) = x;
     }
   }
 };
 
-// OA / OB: operand staging policies (PSOperand<R>).
-// K step kt starts at k = kmap(kt).  Pipeline: two LDS stages and two
-// register slots; the global loads of step kt + 2 are issued before the MFMAs
-// of step kt, so each load has two steps of MFMA work to arrive in.
-template <int BM, int BN, class OA, class OB, class AAddr, class BAddr>
-__device__ __forceinline__ void gemm_tile_bf16x3(const typename OA::Src &A,
-                                                 const typename OB::Src &B, int ksteps,
-                                                 AAddr aaddr, BAddr baddr, __bf16 *lds,
-                                                 f4 (&acc)[BM / 32][BN / 32]) {
+// OA / OB: operand staging (PSOperand<R>, rows set by init).  kofs(kt, c):
+// element offset of k run c (8 k) of step kt, or -1 for zeros.  Pipeline: two
+// LDS stages and two register slots; the global loads of step kt + 2 are
+// issued before the MFMAs of step kt, so each load has two steps of MFMA work
+// to arrive in.
+template <int BM, int BN, class OA, class OB, class KOfs>
+__device__ __forceinline__ void gemm_tile_bf16x3(OA &A, OB &B, int ksteps, const KOfs &kofs,
+                                                 __bf16 *lds, f4 (&acc)[BM / 32][BN / 32]) {
   constexpr int BKP = kXBKP, WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int SA = BM * BKP, SB = BN * BKP, STAGE = 2 * SA + 2 * SB;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, kc = threadIdx.x & 3;
   const int wm = wave >> 1, wn = wave & 1;
-  OA ra0, ra1;
-  OB rb0, rb1;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
   if (ksteps <= 0) return;
-  auto load = [&](int kt, OA &ra, OB &rb) {
-    ra.load(A, aaddr, kt);
-    rb.load(B, baddr, kt);
-  };
-  auto store = [&](int buf, const OA &ra, const OB &rb) {
-    __bf16 *ah = lds + buf * STAGE, *al = ah + SA, *bh = al + SA, *bl = bh + SB;
-    ra.store(ah, al);
-    rb.store(bh, bl);
-  };
   auto compute = [&](int cur) {
     const __bf16 *ah = lds + cur * STAGE, *al = ah + SA, *bh = al + SA, *bl = bh + SB;
     const int ko = 8 * (lane >> 4);
@@ -129,22 +144,41 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const typename OA::Src &A,
         acc[i][j] = mfma_bf16(ahi[i], bhi[j], acc[i][j]);
       }
   };
-  // step kt lives in register slot kt & 1 and LDS stage kt & 1
-  auto step = [&](int kt, OA &ra_cur, OB &rb_cur, OA &ra_nxt, OB &rb_nxt) {
-    if (kt + 2 < ksteps) load(kt + 2, ra_cur, rb_cur);  // slot of kt is free: kt is in LDS
-    compute(kt & 1);
-    if (kt + 1 < ksteps) store((kt + 1) & 1, ra_nxt, rb_nxt);
+  auto stage = [&](int buf) { return lds + buf * STAGE; };
+  auto kofs_at = [&](int kt) -> long { return kt < ksteps ? kofs(kt, kc) : -1; };
+  // step kt lives in register slot kt & 1 and LDS stage kt & 1.  Loads and
+  // stores run unconditionally (past the last step they move zeros), so the
+  // loop body is straight-line code and the compiler's wait counts keep the
+  // loads of step kt + 2 in flight while step kt + 1 is written to LDS.
+  auto step = [&](auto slot, int kt) {
+    constexpr int C = decltype(slot)::value, N = 1 - C;
+    const long ko = kofs_at(kt + 2);  // slot C is free: step kt is in LDS
+    A.template load<C>(ko);
+    B.template load<C>(ko);
+    compute(C);
+    __bf16 *ah = stage(N), *al = ah + SA, *bh = al + SA, *bl = bh + SB;
+    A.template store<N>(ah, al);
+    B.template store<N>(bh, bl);
     __syncthreads();
   };
 
-  load(0, ra0, rb0);
-  if (ksteps > 1) load(1, ra1, rb1);
-  store(0, ra0, rb0);
-  __syncthreads();
-  for (int kt = 0; kt < ksteps; kt += 2) {
-    step(kt, ra0, rb0, ra1, rb1);
-    if (kt + 1 < ksteps) step(kt + 1, ra1, rb1, ra0, rb0);
+  {
+    const long k0 = kofs_at(0), k1 = kofs_at(1);
+    A.template load<0>(k0);
+    B.template load<0>(k0);
+    A.template load<1>(k1);
+    B.template load<1>(k1);
+    __bf16 *ah = stage(0), *al = ah + SA, *bh = al + SA, *bl = bh + SB;
+    A.template store<0>(ah, al);
+    B.template store<0>(bh, bl);
   }
+  __syncthreads();
+  int kt = 0;
+  for (; kt + 1 < ksteps; kt += 2) {
+    step(std::integral_constant<int, 0>{}, kt);
+    step(std::integral_constant<int, 1>{}, kt + 1);
+  }
+  if (kt < ksteps) step(std::integral_constant<int, 0>{}, kt);
 }
 
 __device__ __forceinline__ int find_unit_x3(const long *__restrict__ base, int nu, long it) {
@@ -183,21 +217,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const uint16_t *crun = kl + ch * U.CH;
     const int ncr = min(U.CH, c8 - ch * U.CH);
     // A(p, k) = Phi^T[m0 + p][f(k)], B(k, n) = D^T[col(n)][f(k)]
-    auto aaddr = [&](int m, int kt, int kk) -> long {
-      const long f = run_fn(kl, c8, kt * kXBK + kk);
-      return f < 0 ? -1 : (m0 + m) * NP + f;
-    };
-    auto baddr = [&](int n, int kt, int kk) -> long {
-      const long f = run_fn(kl, c8, kt * kXBK + kk), c = run_fn(crun, ncr, n);
-      return (f < 0 || c < 0) ? -1 : c * NP + f;
-    };
     const __bf16 *th = (const __bf16 *)U.phit, *dh = (const __bf16 *)U.D;
     const long np2 = NP * NP, pn = U.Mc * NP;
-    const typename PSOperand<BM>::Src sa{th, th + pn};
-    const typename PSOperand<BN>::Src sb{dh, dh + np2};
+    PSOperand<BM> opa;
+    PSOperand<BN> opb;
+    opa.init({th, th + pn}, [&](int m) -> long { return (m0 + m) * NP; });
+    opb.init({dh, dh + np2}, [&](int n) -> long {
+      const long c = run_fn(crun, ncr, n);
+      return c < 0 ? -1 : c * NP;
+    });
+    auto kofs = [&](int kt, int c) -> long { return run_fn(kl, c8, kt * kXBK + 8 * c); };
     f4 acc[BM / 32][BN / 32];
-    gemm_tile_bf16x3<BM, BN, PSOperand<BM>, PSOperand<BN>>(sa, sb, (c8 + 3) / 4, aaddr, baddr,
-                                                           lds, acc);
+    gemm_tile_bf16x3<BM, BN>(opa, opb, (c8 + 3) / 4, kofs, lds, acc);
     density_epilogue<float, f4, BM, BN, GRAD>(acc, (const float *)U.phi, U.Mc, U.NP, m0, crun, ncr,
                                               ch, U.part, red);
     item_log(S, it, t0);
@@ -248,20 +279,21 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
     const int ng = potential_groups(U.gcf + U.gcf_off[sl], G, nch, ci, cj, gl, wtot);
     const uint16_t *ri = ul + ci * CH, *rj = ul + cj * CH;
     const int nri = min(CH, nu - ci * CH), nrj = min(CH, nu - cj * CH);
-    auto aaddr = [&](int m, int kt, int kk) -> long {
-      const long f = run_fn(ri, nri, m);
-      return f < 0 ? -1 : f * Mc + kb + (long)gl[kt] * kGroup + kk;
-    };
-    auto baddr = [&](int n, int kt, int kk) -> long {
-      const long f = run_fn(rj, nrj, n);
-      return f < 0 ? -1 : f * Mc + kb + (long)gl[kt] * kGroup + kk;
-    };
     const __bf16 *wh = (const __bf16 *)U.W, *ph = (const __bf16 *)U.phis;
     const long pn = (long)U.NP * Mc;
-    const typename PSOperand<BT>::Src sa{wh, wh + pn};
-    const typename PSOperand<BT>::Src sb{ph, ph + pn};
+    // A(m, k) = W[f_i(m)][kb + p(k)], B(k, n) = Phi[f_j(n)][kb + p(k)]
+    PSOperand<BT> opa, opb;
+    opa.init({wh, wh + pn}, [&](int m) -> long {
+      const long f = run_fn(ri, nri, m);
+      return f < 0 ? -1 : f * Mc;
+    });
+    opb.init({ph, ph + pn}, [&](int n) -> long {
+      const long f = run_fn(rj, nrj, n);
+      return f < 0 ? -1 : f * Mc;
+    });
+    auto kofs = [&](int kt, int c) -> long { return kb + (long)gl[kt] * kGroup + 8 * c; };
     f4 acc[BT / 32][BT / 32];
-    gemm_tile_bf16x3<BT, BT, PSOperand<BT>, PSOperand<BT>>(sa, sb, ng, aaddr, baddr, lds, acc);
+    gemm_tile_bf16x3<BT, BT>(opa, opb, ng, kofs, lds, acc);
     potential_epilogue<float, f4, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP, U.NP, ri, nri, rj,
                                           nrj);
     item_log(S, it, t0);

@@ -7,6 +7,18 @@
 namespace pgxc {
 
 // ------------------------------------------------------------------ helpers
+// Global-memory access through pointers the compiler cannot place (they are
+// read from descriptors): global_load / global_store instead of flat ones,
+// which would also count against the LDS wait counter.
+template <typename T>
+__device__ __forceinline__ T gld(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T *p, T v) {
+  *(__attribute__((address_space(1))) T *)p = v;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; give every XCD a
 // contiguous range of tile ids so that tiles sharing an operand panel share
 // that XCD's L2.
@@ -101,7 +113,7 @@ __device__ __forceinline__ int block_compact(bool act, int value, uint16_t *list
 
 // Copy n list entries into LDS (every thread calls it; ends with a barrier).
 __device__ __forceinline__ void lds_list(const uint16_t *__restrict__ src, int n, uint16_t *dst) {
-  for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = src[i];
+  for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = gld(src + i);
   __syncthreads();
 }
 
@@ -118,7 +130,7 @@ __device__ __forceinline__ int potential_groups(const uint8_t *__restrict__ flag
   int cnt = 0;
   for (int b = 0; b < G; b += kThreads) {
     const int g = b + (int)threadIdx.x;
-    const bool act = g < G && flags[(long)g * nch + ci] != 0 && flags[(long)g * nch + cj] != 0;
+    const bool act = g < G && gld(flags + (long)g * nch + ci) != 0 && gld(flags + (long)g * nch + cj) != 0;
     cnt = block_compact(act, g, gl, cnt, wtot);
   }
   return cnt;
@@ -195,7 +207,7 @@ __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 
         const long q = n * Mc + m0 + wm * WM + i * 16 + MF<T>::row(lane, 0);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          const f4 t = *(const f4 *)(phi + c * cs + q);
+          const f4 t = gld((const f4 *)(phi + c * cs + q));
           v[c][0] = t.x; v[c][1] = t.y; v[c][2] = t.z; v[c][3] = t.w;
         }
       } else {
@@ -203,7 +215,7 @@ __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 
         for (int r = 0; r < 4; ++r) {
           const long q = n * Mc + m0 + wm * WM + i * 16 + MF<T>::row(lane, r);
 #pragma unroll
-          for (int c = 0; c < NC; ++c) v[c][r] = phi[c * cs + q];
+          for (int c = 0; c < NC; ++c) v[c][r] = gld(phi + c * cs + q);
         }
       }
 #pragma unroll
@@ -229,7 +241,7 @@ __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 
   __syncthreads();
   for (int idx = tid; idx < NC * BM; idx += kThreads) {
     const int c = idx / BM, row = idx % BM;
-    part[((long)ch * 4 + c) * Mc + m0 + row] = red[0][c][row] + red[1][c][row];
+    gst(part + ((long)ch * 4 + c) * Mc + m0 + row, red[0][c][row] + red[1][c][row]);
   }
 }
 
@@ -254,7 +266,7 @@ __device__ __forceinline__ void potential_epilogue(const ACC (&acc)[BM / 32][BN 
         const long row = run_fn(rrun, nrr, wm * WM + i * 16 + MF<T>::row(lane, r));
         if (row < 0) continue;
         const long q = row * NP + col;
-        H[q] += (double)acc[i][j][r];
+        gst(H + q, gld(H + q) + (double)acc[i][j][r]);
       }
   }
 }
