@@ -119,8 +119,12 @@ __device__ __forceinline__ void lds_list(const uint16_t *__restrict__ src, int n
 
 // Function (row of Phi / W / D) of entry e of a gathered run list, or -1
 // past the list's n runs.
+// The list read is unconditional (clamped into the list), so the lookup
+// compiles to a select rather than a branch.
 __device__ __forceinline__ long run_fn(const uint16_t *runs, int n, int e) {
-  return (e >> 3) < n ? (long)runs[e >> 3] * kRun + (e & 7) : -1;
+  const int r = e >> 3;
+  const long v = (long)runs[min(r, max(n - 1, 0))] * kRun + (e & 7);
+  return r < n ? v : -1;
 }
 
 // Potential item: the groups of its split on which both of its chunks can be

@@ -84,13 +84,21 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
   const int wm = wave >> 1, wn = wave & 1;
   typename VA_t::type ra[LA];
   typename VB_t::type rb[LB];
+  unsigned oka = 0, okb = 0;  // bit e: vector e is real (else zeros)
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = typename MF<T>::acc{0, 0, 0, 0};
 
   if (ksteps <= 0) return;
+  // Loads are unconditional global loads (a vector of zeros reads the
+  // operand's first vector instead, and the zeros are applied when it is
+  // written to LDS), and a step past the last one loads zeros: the pipeline
+  // is straight-line code, so the compiler's wait counts keep the next
+  // step's loads in flight during the MFMAs.
   auto load = [&](int kt) {
+    const bool live = kt < ksteps;
+    unsigned m = 0;
 #pragma unroll
     for (int e = 0; e < LA; ++e) {
       const int idx = tid + kThreads * e;
@@ -102,8 +110,12 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
         const int mv = idx % (BM / VA), kk = idx / (BM / VA);
         o = aaddr(mv * VA, kt, kk);
       }
-      ra[e] = o >= 0 ? *(const typename VA_t::type *)(A + o) : typename VA_t::type{};
+      const bool g = live && o >= 0;
+      ra[e] = gld((const typename VA_t::type *)(A + (g ? o : 0)));
+      m |= (unsigned)g << e;
     }
+    oka = m;
+    m = 0;
 #pragma unroll
     for (int e = 0; e < LB; ++e) {
       const int idx = tid + kThreads * e;
@@ -115,10 +127,19 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
         const int nv = idx % (BN / VB), kk = idx / (BN / VB);
         o = baddr(nv * VB, kt, kk);
       }
-      rb[e] = o >= 0 ? *(const typename VB_t::type *)(B + o) : typename VB_t::type{};
+      const bool g = live && o >= 0;
+      rb[e] = gld((const typename VB_t::type *)(B + (g ? o : 0)));
+      m |= (unsigned)g << e;
     }
+    okb = m;
   };
   auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < LA; ++e)
+      if (!((oka >> e) & 1u)) ra[e] = typename VA_t::type{};
+#pragma unroll
+    for (int e = 0; e < LB; ++e)
+      if (!((okb >> e) & 1u)) rb[e] = typename VB_t::type{};
     if constexpr (KC) {
       T *as = As + buf * BM * BKP, *bs = Bs + buf * BN * BKP;
 #pragma unroll
@@ -204,9 +225,9 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     __syncthreads();
     for (int kt = 0; kt < ksteps; ++kt) {
       const int cur = kt & 1;
-      if (kt + 1 < ksteps) load(kt + 1);
+      load(kt + 1);
       compute(cur);
-      if (kt + 1 < ksteps) store(cur ^ 1);
+      store(cur ^ 1);
       __syncthreads();
     }
   } else {
@@ -214,7 +235,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     for (int kt = 0; kt < ksteps; ++kt) {
       store(0);
       __syncthreads();
-      if (kt + 1 < ksteps) load(kt + 1);
+      load(kt + 1);
       compute(0);
       __syncthreads();
     }
