@@ -269,7 +269,10 @@ def bench_batch64(args):
     xcs = []
     for b, g, C, occ in mols:
         tu = autotune.tune(b, g, C, occ, sample_points=8192, reps=1, mixed=BF16X3)
-        xc = XCBuilder(b, g, tile=tu["tile"], variants=(tu["variant_fp64"], tu["variant_fp32"]))
+        # every context of a batch holds its own work chunk while the batch
+        # runs, so chunks of 64 Ki points keep 64 of them within HBM
+        xc = XCBuilder(b, g, tile=tu["tile"], variants=(tu["variant_fp64"], tu["variant_fp32"]),
+                       chunk_points=65536)
         xc.set_orbitals(C, occ)
         xcs.append(xc)
     log("autotuned and created 64 contexts in %.1f s" % (time.time() - t))

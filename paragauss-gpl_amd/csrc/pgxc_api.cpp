@@ -14,6 +14,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <type_traits>
@@ -69,6 +71,7 @@ struct pgxc_ctx {
   int cache_off = 0;
   void *phic[2] = {nullptr, nullptr};
   int phic_valid[2] = {0, 0}, phic_failed[2] = {0, 0};
+  size_t phic_bytes[2] = {0, 0};
   void *phi_base = nullptr;  // Phi of the current build: chunk k at phi_base + k * phi_stride
   size_t phi_stride = 0;
   int skip_basis = 0;        // the current build reads cached basis values
@@ -640,6 +643,10 @@ static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hip
   if (ev) (void)hipEventRecord((*ev)[kEvEnd], s);
 }
 
+// Bytes of basis caches held per device by all contexts of the process.
+static std::mutex g_cache_mu;
+static std::map<int, size_t> g_cache_bytes;
+
 // bytes of basis values per (function, point): 4 planes, plus for FP32 the
 // split-BF16 planes in both layouts (2 x 2 x 2 bytes)
 static size_t phi_elem_bytes(size_t elem) { return elem == 8 ? 32 : 24; }
@@ -657,15 +664,21 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
     moved = true;
   }
   // basis cache: every chunk's values, when they take at most a quarter of
-  // the device and leave 32 GiB free
+  // the device, leave 32 GiB free, and the caches of all contexts on the
+  // device stay within half of it (the members of a batch keep room for their
+  // own work buffers)
   const int e = elem == 8 ? 0 : 1;
   if (!c->cache_off && !c->phic[e] && !c->phic_failed[e]) {
     const size_t need = need_phi * (size_t)c->nchunks;
     size_t fr = 0, tot = 0;
     c->phic_failed[e] = 1;
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    size_t &held = g_cache_bytes[c->device];
     if (hipMemGetInfo(&fr, &tot) == hipSuccess && need <= tot / 4 && need + (32ull << 30) <= fr &&
-        hipMalloc(&c->phic[e], need) == hipSuccess) {
+        held + need <= tot / 2 && hipMalloc(&c->phic[e], need) == hipSuccess) {
       c->phic_failed[e] = 0;
+      c->phic_bytes[e] = need;
+      held += need;
       HIPCHK(c, hipMemsetAsync(c->phic[e], 0, need, s));  // padding rows read as zero
       c->phic_valid[e] = 0;
     } else {
@@ -1075,20 +1088,21 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
 
   // ---- chunking of the grid ----------------------------------------------
   // a chunk holds at most 256 Ki points and 8 GiB of FP64 basis values; a
-  // point split of the potential contraction is a whole number of 2048-point
-  // blocks, the fewest that keep the splits of a chunk at 64 or less
+  // point split of the potential contraction is a whole number m of
+  // 2048-point blocks, the fewest that keep the splits of a chunk at 64 or
+  // less, and the chunk is rounded up to whole splits (one more chunk when
+  // that would pass the limit)
   long mc_max = std::max<long>(kSplitPts, ((8L << 30) / (32L * c->NP)) / kSplitPts * kSplitPts);
   mc_max = std::min<long>(mc_max, 1L << 18);
   if (opt && opt->chunk_points) mc_max = opt->chunk_points;
-  c->nchunks = (int)((npts + mc_max - 1) / mc_max);
-  c->Mc = round_up((npts + c->nchunks - 1) / c->nchunks, kSplitPts);
-  c->npad = c->Mc * c->nchunks;
-  {
-    const long q = c->Mc / kSplitPts;
-    long m = 1;
-    while (q % m != 0 || q / m > 64) ++m;
-    c->nsplit = (int)(q / m);
+  for (c->nchunks = (int)((npts + mc_max - 1) / mc_max);; ++c->nchunks) {
+    const long q = round_up((npts + c->nchunks - 1) / c->nchunks, kSplitPts) / kSplitPts;
+    const long m = (q + 63) / 64;
+    c->nsplit = (int)((q + m - 1) / m);
+    c->Mc = (long)c->nsplit * m * kSplitPts;
+    if (c->Mc <= mc_max) break;
   }
+  c->npad = c->Mc * c->nchunks;
   c->nblk_total = c->npad / kThreads;
 
   HIPCHK(c, hipGetDevice(&c->device));
@@ -1312,6 +1326,10 @@ int pgxc_destroy(pgxc_ctx *c) {
   if (!c) return PGXC_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_plan(c->plan);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache_bytes[c->device] -= c->phic_bytes[0] + c->phic_bytes[1];
+  }
   void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,

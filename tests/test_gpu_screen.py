@@ -157,3 +157,26 @@ def test_basis_cache_reuse_is_bitwise(dimer, prec):
                 assert max(tb[2:]) < 0.5 * tb[0]
     for (h1, e1, q1), (h2, e2, q2) in zip(out[True], out[False]):
         assert np.array_equal(h1, h2) and e1 == e2 and q1 == q2
+
+
+def test_prime_block_count_layout(dimer):
+    """A grid of 67 2048-point blocks (a prime above 64) fits one chunk of 34
+    point splits of 4096 points (one block of padding) and builds what a
+    layout of several chunks builds, to rounding."""
+    els, xyz, b, _, C, occ = dimer
+    g = grid.molecular_grid(els, xyz, nrad=90, nang=302)[:136000]  # a part of a grid is a grid
+    assert g.shape[0] == 136000 and -(-g.shape[0] // 2048) == 67
+    out = []
+    for chunk in (0, 32768):
+        with XCBuilder(b, g, chunk_points=chunk) as xc:
+            lay = xc.layout()
+            print("chunk_points=%d layout %s" % (chunk, lay), flush=True)
+            if chunk == 0:
+                assert (lay["chunks"], lay["nsplit"], lay["chunk_points"]) == (1, 34, 34 * 4096)
+            else:
+                assert lay["chunk_points"] <= chunk
+            xc.set_orbitals(C, occ)
+            out.append(xc.build(PBE, FP64))
+    (h1, e1, q1), (h2, e2, q2) = out
+    assert abs(e1 - e2) <= 1e-10 * abs(e1) and abs(q1 - q2) <= 1e-10 * abs(q1)
+    assert np.max(np.abs(h1 - h2)) <= 1e-11
