@@ -103,10 +103,12 @@ def cpu_baseline(b, g, C, occ, budget_s):
     nt = min(16, os.cpu_count() or 1)
     veclen = 128
     nb = (g.shape[0] + veclen - 1) // veclen
-    # probe with one batch per thread, then size the sample to the budget
+    # probe with one batch per thread from the middle of the grid (where the
+    # sample is taken), then size the sample to the budget
+    mid = max(0, nb // 2 - nt // 2)
     t = time.time()
-    oracle.xc_build(b, g, oracle.MASK_PBE, C=C, occ=occ, veclen=veclen, batch_range=(0, nt),
-                    nthreads=nt)
+    oracle.xc_build(b, g, oracle.MASK_PBE, C=C, occ=occ, veclen=veclen,
+                    batch_range=(mid, min(nb, mid + nt)), nthreads=nt)
     per = time.time() - t  # wall time of one batch per thread
     nsamp = int(min(nb, max(nt, budget_s / max(per, 1e-6) * nt)))
     lo = (nb - nsamp) // 2  # middle of the grid: a mix of inner and outer shells
