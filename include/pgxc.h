@@ -240,11 +240,14 @@ int pgxc_batch_destroy(pgxc_batch *batch);
  * dlb_give_more, dlb/dlb.f90:203-224,290-311).  Here the work items of a
  * contraction launch (point tile x function tile, or point split x function
  * tile pair) are either walked round-robin (PGXC_SCHED_STATIC), or cut into
- * one queue per XCD of about equal predicted cost and drained dynamically
- * (PGXC_SCHED_ANALYTIC: counted flops left after screening;
+ * one queue per XCD of about equal predicted cost, each ordered costliest
+ * first, and drained dynamically (a workgroup whose queue is empty takes from
+ * the next) (PGXC_SCHED_ANALYTIC: counted flops left after screening;
  * PGXC_SCHED_MODEL: the cost model).  The model is a 3-layer MLP
  * (PGXC_NFEATURES -> 64 -> 64 -> 1, tanh) predicting log(item time / us),
- * trained on the GPU from the per-item timings this library logs.
+ * trained on the GPU from the per-item timings this library logs.  The
+ * context (batch) keeps the model pointer: the model must outlive it, or the
+ * schedule must be changed before the model is destroyed.
  */
 int pgxc_set_schedule(pgxc_ctx *ctx, int32_t mode, pgxc_costmodel *model);
 int pgxc_batch_set_schedule(pgxc_batch *batch, int32_t mode, pgxc_costmodel *model);

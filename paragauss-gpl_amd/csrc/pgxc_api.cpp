@@ -48,6 +48,11 @@ void dfree(void *p) {
 struct pgxc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // chunks alternate between two lanes (this stream and stream2, each with its
+  // own work buffers) so one chunk's stages overlap the next chunk's
+  hipStream_t stream2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  int nlanes = 1;
   std::string err;
   // sizes
   int n_atoms = 0, n_shells = 0, nbas = 0, NP = 0, veclen = 128, tf = 128;
@@ -66,6 +71,7 @@ struct pgxc_ctx {
   // device: work buffers of one chunk
   void *phi = nullptr, *W = nullptr;
   size_t phi_bytes = 0, w_bytes = 0, phi_elem = 0;
+  size_t w_lane = 0, phi_lane = 0;  // bytes of one lane's W / basis chunk buffer
   // basis values of every chunk kept across builds, per element type
   // ([0] FP64, [1] FP32): built once, reused while the geometry lives
   int cache_off = 0;
@@ -170,6 +176,7 @@ struct Plan {
   std::vector<std::pair<uint32_t *, long>> logs;  // (device ticks, items)
   double tick_us = 0.0;
   std::vector<StepPlan> steps;
+  int nlanes = 1;  // 2: chunk k runs on lane k % 2 (two streams)
   PackD *d_pack = nullptr;
   long *d_pack_base = nullptr, *d_prep_base = nullptr, *d_fold_base = nullptr;
   long pack_total = 0, prep_total = 0, fold_total = 0;
@@ -378,18 +385,19 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.exps = c->exps;
       U.norms = c->norms;
       U.contr = c->contr;
-      U.phi = (char *)c->phi_base + (size_t)k * c->phi_stride;
+      const int lane = k % c->nlanes;  // work buffers of chunk k
+      U.phi = (char *)c->phi_base + (c->phi_stride ? (size_t)k * c->phi_stride : (size_t)lane * c->phi_lane);
       if (precision != PGXC_PREC_FP64) {  // split planes after the four FP32 planes
         U.phis = (char *)U.phi + 16 * (size_t)c->NP * c->Mc;
         U.phit = (char *)U.phis + 4 * (size_t)c->NP * c->Mc;
       }
-      U.W = c->W;
+      U.W = (char *)c->W + (size_t)lane * c->w_lane;
       U.D = c->D;
-      U.part = c->part;
-      U.coef = c->coef;
+      U.part = c->part + (size_t)lane * c->ntiles * 4 * c->Mc;
+      U.coef = c->coef + (size_t)lane * 4 * c->Mc;
       U.state = c->state + p0;
       U.epart = c->epart + 2 * (p0 / kThreads);
-      U.Hpart = c->Hpart;
+      U.Hpart = c->Hpart + (size_t)lane * c->nsplit * c->NP * c->NP;
       U.Mc = c->Mc;
       U.npad = c->npad;
       U.n_atoms = c->n_atoms;
@@ -491,6 +499,17 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
         }
         while (q <= kQueues) qb[q++] = P.total;
         P.qbeg_off = put(qb, sizeof(qb));
+        // within a queue the costliest items go first, so its last items,
+        // which set when the stage ends, are the cheapest; equal costs keep
+        // their natural order (the items of one density tile stay together)
+        std::vector<int> order((size_t)P.total);
+        std::iota(order.begin(), order.end(), 0);
+        for (int k = 0; k < kQueues; ++k)
+          std::stable_sort(order.begin() + qb[k], order.begin() + qb[k + 1], [&](int a, int b) {
+            return cost[P.frow + a] > cost[P.frow + b];
+          });
+        P.has_order = true;
+        P.order_off = put(order.data(), sizeof(int) * order.size());
         P.ctr_off = ctr0 + sizeof(int) * kQueues * qi++;
       }
     plan_ctr_off_tmp = ctr0;
@@ -510,7 +529,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     Q.fpos = c->d_fpos;
     Q.n = c->nbas;
     Q.NP = c->NP;
-    Q.nsplit = c->nsplit;
+    Q.nsplit = c->nsplit * c->nlanes;  // the fold sums the partials of every lane
     Q.incremental = incr[m];
     const long npk = (long)c->nbas * (c->nbas + 1) / 2;
     pbase[m + 1] = pbase[m] + (npk + kThreads - 1) / kThreads;
@@ -519,7 +538,8 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     fbase[m + 1] = fbase[m] + nb * (nb + 1) / 2;
   }
   for (int m = 0; m < nm; ++m)
-    plan->hzero.push_back({(int *)mols[m]->Hpart, 2L * mols[m]->nsplit * mols[m]->NP * mols[m]->NP});
+    plan->hzero.push_back({(int *)mols[m]->Hpart,
+                           2L * mols[m]->nlanes * mols[m]->nsplit * mols[m]->NP * mols[m]->NP});
   const size_t pack_off = put(packs.data(), sizeof(PackD) * nm);
   const size_t pb_off = put(pbase.data(), sizeof(long) * pbase.size());
   const size_t qb_off = put(qbase.data(), sizeof(long) * qbase.size());
@@ -552,6 +572,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     S.split = mk(pend[k][3 + 2 * kLists]);
     S.nunits = pend[k][0].nu;
   }
+  for (int m = 0; m < nm; ++m) plan->nlanes = std::max(plan->nlanes, mols[m]->nlanes);
   plan->d_pack = (PackD *)(B + pack_off);
   plan->d_pack_base = (long *)(B + pb_off);
   plan->d_prep_base = (long *)(B + qb_off);
@@ -595,46 +616,61 @@ static int check_sched(int32_t mode, const pgxc_costmodel *model) {
 // until the build has ended, so timing adds no host synchronisation.
 constexpr int kEvStart = 0, kEvPack = 1, kEvEnd = 2, kEvStep = 3;
 
-// Launch a plan on stream s, recording the events above when ev is given.
+// Launch a plan, recording the events above when ev is given.  Chunk k runs
+// on lane k % 2 of a two-lane plan: stream s or s2, each lane with its own
+// work buffers, so one chunk's stages (and the ends of its launches) overlap
+// the next chunk's; s2 starts after the operand preparation (fork) and the
+// fold waits for its last chunk (join).
 template <typename T>
-static void launch_plan(hipStream_t s, const Plan &pl, bool gga, std::vector<hipEvent_t> *ev) {
+static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join,
+                        const Plan &pl, bool gga, std::vector<hipEvent_t> *ev) {
   const bool x3 = pl.precision == PGXC_PREC_BF16X3;
+  const bool two = pl.nlanes > 1 && pl.steps.size() > 1;
   if (pl.nctr) launch_zero_ints(s, pl.d_ctr, pl.nctr);
   for (auto &z : pl.hzero) launch_zero_ints(s, z.first, z.second);  // potential items add into H
   launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total,
               x3 ? 2 : std::is_same<T, double>::value ? 0 : 1, false);
+  if (two) {
+    (void)hipEventRecord(fork, s);
+    (void)hipStreamWaitEvent(s2, fork, 0);
+  }
   for (size_t k = 0; k < pl.steps.size(); ++k) {
     const StepPlan &S = pl.steps[k];
+    const hipStream_t t = two && (k & 1) ? s2 : s;
     hipEvent_t *E = ev ? ev->data() + kEvStep + 5 * k : nullptr;
-    if (E) (void)hipEventRecord(E[0], s);
-    launch_basis<T>(s, gga, S.basis);
-    if (x3) launch_split(s, S.split);
-    if (E) (void)hipEventRecord(E[1], s);
+    if (E) (void)hipEventRecord(E[0], t);
+    launch_basis<T>(t, gga, S.basis);
+    if (x3) launch_split(t, S.split);
+    if (E) (void)hipEventRecord(E[1], t);
     for (int li = 0; li < kLists; ++li) {
       const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;
       if constexpr (std::is_same<T, float>::value) {
         if (x3) {
-          launch_density_bf16x3(s, gga, tf, S.dens[li]);
+          launch_density_bf16x3(t, gga, tf, S.dens[li]);
           continue;
         }
       }
-      launch_density<T>(s, gga, tf, v, S.dens[li]);
+      launch_density<T>(t, gga, tf, v, S.dens[li]);
     }
-    if (E) (void)hipEventRecord(E[2], s);
-    launch_functional(s, gga, S.func);
-    launch_wmat<T>(s, gga, x3, S.wmat);
-    if (E) (void)hipEventRecord(E[3], s);
+    if (E) (void)hipEventRecord(E[2], t);
+    launch_functional(t, gga, S.func);
+    launch_wmat<T>(t, gga, x3, S.wmat);
+    if (E) (void)hipEventRecord(E[3], t);
     for (int li = 0; li < kLists; ++li) {
       const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;
       if constexpr (std::is_same<T, float>::value) {
         if (x3) {
-          launch_potential_bf16x3(s, tf, S.pot[li]);
+          launch_potential_bf16x3(t, tf, S.pot[li]);
           continue;
         }
       }
-      launch_potential<T>(s, tf, v, S.pot[li]);
+      launch_potential<T>(t, tf, v, S.pot[li]);
     }
-    if (E) (void)hipEventRecord(E[4], s);
+    if (E) (void)hipEventRecord(E[4], t);
+  }
+  if (two) {
+    (void)hipEventRecord(join, s2);
+    (void)hipStreamWaitEvent(s, join, 0);
   }
   if (ev) (void)hipEventRecord((*ev)[kEvPack], s);
   launch_fold(s, pl.d_pack, pl.nmol, pl.d_fold_base, pl.fold_total);
@@ -653,7 +689,8 @@ static size_t phi_elem_bytes(size_t elem) { return elem == 8 ? 32 : 24; }
 
 static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
   const size_t need_phi = phi_elem_bytes(elem) * (size_t)c->NP * c->Mc;
-  const size_t need_w = (size_t)c->NP * c->Mc * elem;
+  const size_t need_w = (size_t)c->NP * c->Mc * elem * c->nlanes;
+  c->w_lane = (size_t)c->NP * c->Mc * elem;
   bool moved = false;
   if (c->w_bytes < need_w) {
     dfree(c->W);
@@ -686,13 +723,14 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
       (void)hipGetLastError();
     }
   }
-  if (!c->phic[e]) {  // one chunk buffer, rewritten by every build
-    if (c->phi_bytes < need_phi) {
+  c->phi_lane = need_phi;
+  if (!c->phic[e]) {  // one chunk buffer per lane, rewritten by every build
+    if (c->phi_bytes < need_phi * c->nlanes) {
       dfree(c->phi);
       c->phi = nullptr;
       c->phi_bytes = 0;
-      HIPCHK(c, hipMalloc(&c->phi, need_phi));
-      c->phi_bytes = need_phi;
+      HIPCHK(c, hipMalloc(&c->phi, need_phi * c->nlanes));
+      c->phi_bytes = need_phi * c->nlanes;
       c->phi_elem = 0;
     }
     // function rows nbas..NP-1 are never written and must read as zero in the
@@ -1107,6 +1145,10 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
 
   HIPCHK(c, hipGetDevice(&c->device));
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->nlanes = c->nchunks > 1 ? 2 : 1;
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  HIPCHK(c, hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+  HIPCHK(c, hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
   int rc;
   if ((rc = dalloc(c, &c->grid, 4 * (size_t)c->npad))) return rc;
   if ((rc = dalloc(c, &c->atoms, n_atoms))) return rc;
@@ -1143,11 +1185,12 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   if ((rc = dalloc(c, &c->Vstate, npk))) return rc;
   if ((rc = dalloc(c, &c->Vout, npk))) return rc;
   if ((rc = dalloc(c, &c->epart, 2 * (size_t)c->nblk_total))) return rc;
-  if ((rc = dalloc(c, &c->part, (size_t)c->ntiles * 4 * c->Mc))) return rc;
-  if ((rc = dalloc(c, &c->coef, 4 * (size_t)c->Mc))) return rc;
-  if ((rc = dalloc(c, &c->Hpart, (size_t)c->nsplit * np2))) return rc;
-  HIPCHK(c, hipMemset(c->part, 0, (size_t)c->ntiles * 4 * c->Mc * 8));
-  HIPCHK(c, hipMemset(c->coef, 0, 4 * (size_t)c->Mc * 8));
+  const size_t L = (size_t)c->nlanes;
+  if ((rc = dalloc(c, &c->part, L * c->ntiles * 4 * c->Mc))) return rc;
+  if ((rc = dalloc(c, &c->coef, L * 4 * c->Mc))) return rc;
+  if ((rc = dalloc(c, &c->Hpart, L * c->nsplit * np2))) return rc;
+  HIPCHK(c, hipMemset(c->part, 0, L * c->ntiles * 4 * c->Mc * 8));
+  HIPCHK(c, hipMemset(c->coef, 0, L * 4 * c->Mc * 8));
   return PGXC_OK;
 }
 
@@ -1253,9 +1296,9 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
   }
   HIPCHK(c, hipEventRecord(c->ev[kEvStart], s));
   if (elem == 8)
-    launch_plan<double>(s, *c->plan, gga, &c->ev);
+    launch_plan<double>(s, c->stream2, c->fork, c->join, *c->plan, gga, &c->ev);
   else
-    launch_plan<float>(s, *c->plan, gga, &c->ev);
+    launch_plan<float>(s, c->stream2, c->fork, c->join, *c->plan, gga, &c->ev);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(s));
   for (int k = 0; k < 4; ++k) c->timing[k] = 0.0;
@@ -1325,6 +1368,7 @@ int64_t pgxc_item_log(pgxc_ctx *c, double *features, double *us, int64_t cap) {
 int pgxc_destroy(pgxc_ctx *c) {
   if (!c) return PGXC_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   free_plan(c->plan);
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
@@ -1339,6 +1383,9 @@ int pgxc_destroy(pgxc_ctx *c) {
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
   delete c;
   return PGXC_OK;
 }
@@ -1348,7 +1395,8 @@ int pgxc_destroy(pgxc_ctx *c) {
 // ------------------------------------------------------------------ batches
 struct pgxc_batch {
   std::vector<pgxc_ctx *> mols;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr, stream2 = nullptr;  // the two lanes of its plans
+  hipEvent_t fork = nullptr, join = nullptr;
   std::string err;
   Plan *plan = nullptr;
   std::vector<double> prio;
@@ -1387,6 +1435,9 @@ int pgxc_batch_create(pgxc_batch **out, int32_t n, pgxc_ctx *const *ctxs) {
     b->mols.push_back(ctxs[i]);
   }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
     b->err = "pgxc_batch_create: stream/event creation failed";
     return PGXC_ERR_HIP;
@@ -1443,9 +1494,9 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       if (elem == 8)
-        launch_plan<double>(s, pl, gga, nullptr);
+        launch_plan<double>(s, b->stream2, b->fork, b->join, pl, gga, nullptr);
       else
-        launch_plan<float>(s, pl, gga, nullptr);
+        launch_plan<float>(s, b->stream2, b->fork, b->join, pl, gga, nullptr);
       e = hipStreamEndCapture(s, &pl.graph_src);
       if (e == hipSuccess) e = hipGraphInstantiate(&pl.graph, pl.graph_src, nullptr, nullptr, 0);
     }
@@ -1461,9 +1512,9 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
       return PGXC_ERR_HIP;
     }
   } else if (elem == 8) {
-    launch_plan<double>(s, pl, gga, nullptr);
+    launch_plan<double>(s, b->stream2, b->fork, b->join, pl, gga, nullptr);
   } else {
-    launch_plan<float>(s, pl, gga, nullptr);
+    launch_plan<float>(s, b->stream2, b->fork, b->join, pl, gga, nullptr);
   }
   (void)hipEventRecord(b->e1, s);
   hipError_t e = hipGetLastError();
@@ -1519,10 +1570,14 @@ int pgxc_batch_last_ms(const pgxc_batch *b, double *ms) {
 int pgxc_batch_destroy(pgxc_batch *b) {
   if (!b) return PGXC_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
   free_plan(b->plan);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->join) (void)hipEventDestroy(b->join);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->stream2) (void)hipStreamDestroy(b->stream2);
   delete b;
   return PGXC_OK;
 }

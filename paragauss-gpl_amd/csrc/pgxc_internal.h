@@ -88,7 +88,8 @@ struct PackD {
 // Queues (ctr != nullptr, contraction stages only): positions are split into
 // kQueues contiguous ranges qbeg[q] .. qbeg[q+1]-1 of about equal predicted
 // cost; workgroup b drains queue b % kQueues (the XCD it runs on), taking the
-// next position from the counter ctr[q].  tlog (optional): per item, the
+// next position from the counter ctr[q], then helps with the queues after
+// it.  tlog (optional): per item, the
 // workgroup's wall-clock ticks spent on it.
 constexpr int kQueues = 8;
 struct StageList {

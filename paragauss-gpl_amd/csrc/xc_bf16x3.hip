@@ -203,7 +203,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
   __shared__ uint16_t kl[kMaxRuns];
   __shared__ long slot;
   static_assert(kXBK == 4 * kRun, "four runs per K step");
-  for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
+  for (long L = walk_next(S, -1, &slot); L < S.total;) {
     const uint64_t t0 = S.tlog ? wall_clock64() : 0;
     const long it = walk_item(S, L);
     const int u = find_unit_x3(S.base, S.nu, it);
@@ -229,9 +229,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     auto kofs = [&](int kt, int c) -> long { return run_fn(kl, c8, kt * kXBK + 8 * c); };
     f4 acc[BM / 32][BN / 32];
     gemm_tile_bf16x3<BM, BN>(opa, opb, (c8 + 3) / 4, kofs, lds, acc);
-    density_epilogue<float, f4, BM, BN, GRAD>(acc, (const float *)U.phi, U.Mc, U.NP, m0, crun, ncr,
+    const int ahead = walk_ahead(S);
+    density_epilogue<float, f4, BM, BN, GRAD, 2>(acc, (const float *)U.phi, U.Mc, U.NP, m0, crun, ncr,
                                               ch, U.part, red);
     item_log(S, it, t0);
+    L = walk_take(S, L, ahead, &slot);
   }
 }
 
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
   __shared__ int wtot[4];
   __shared__ long slot;
   static_assert(kXBK == kGroup, "one K step per 32-point group");
-  for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
+  for (long L = walk_next(S, -1, &slot); L < S.total;) {
     const uint64_t t0 = S.tlog ? wall_clock64() : 0;
     const long it = walk_item(S, L);
     const int u = find_unit_x3(S.base, S.nu, it);
@@ -294,9 +296,11 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
     auto kofs = [&](int kt, int c) -> long { return kb + (long)gl[kt] * kGroup + 8 * c; };
     f4 acc[BT / 32][BT / 32];
     gemm_tile_bf16x3<BT, BT>(opa, opb, ng, kofs, lds, acc);
+    const int ahead = walk_ahead(S);
     potential_epilogue<float, f4, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP, U.NP, ri, nri, rj,
                                           nrj);
     item_log(S, it, t0);
+    L = walk_take(S, L, ahead, &slot);
   }
 }
 

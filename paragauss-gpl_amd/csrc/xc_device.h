@@ -28,20 +28,55 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return (bid & 7) * per + (bid >> 3);
 }
 
+// A position of a queued list (thread 0 only) from the queues after the
+// workgroup's own (blockIdx.x % kQueues), from `first` on; total when all are
+// drained.
+__device__ __forceinline__ long walk_claim_queue(const StageList &S, int first) {
+  const int q = (int)(blockIdx.x % kQueues);
+  for (int t = first; t < kQueues; ++t) {
+    const int qq = (q + t) % kQueues;
+    const long c = S.qbeg[qq] + (long)atomicAdd(S.ctr + qq, 1);
+    if (c < S.qbeg[qq + 1]) return c;
+  }
+  return S.total;
+}
+
 // Walk of a stage list's positions.  Static: blockIdx.x, + gridDim.x, ...
-// With queues: the next position of queue blockIdx.x % kQueues, or total
-// when that queue is drained.  Every thread of the workgroup calls it.
+// With queues: the next position of the workgroup's own queue; once that is
+// drained, of the next queue that is not; total when all are.  Every thread
+// of the workgroup calls it.
 __device__ __forceinline__ long walk_next(const StageList &S, long prev, long *slot) {
   if (!S.ctr) return prev < 0 ? (long)blockIdx.x : prev + (long)gridDim.x;
-  const int q = (int)(blockIdx.x % kQueues);
-  if (threadIdx.x == 0) {
-    const long p = S.qbeg[q] + (long)atomicAdd(S.ctr + q, 1);
-    *slot = p < S.qbeg[q + 1] ? p : S.total;
-  }
+  if (threadIdx.x == 0) *slot = walk_claim_queue(S, 0);
   __syncthreads();
   const long p = *slot;
   __syncthreads();
   return p;
+}
+
+// The same walk with the next position claimed during the current item:
+// walk_ahead (every thread, before the item's last phase) issues thread 0's
+// claim on its own queue without waiting for it, and walk_take (every thread,
+// after the item) checks it (falling back to the other queues) and hands it
+// to the workgroup, so the claim's round trip overlaps the last phase.  The
+// slot is written again only after the next item's barriers.
+__device__ __forceinline__ int walk_ahead(const StageList &S) {
+  int raw = 0;  // thread 0: the claim's counter value, still in flight here
+  if (S.ctr && threadIdx.x == 0) raw = atomicAdd(S.ctr + blockIdx.x % kQueues, 1);
+  return raw;
+}
+__device__ __forceinline__ long walk_take(const StageList &S, long cur, int raw, long *slot) {
+  if (!S.ctr) return cur + (long)gridDim.x;
+  // the claim's value is first read here: an empty asm that takes it keeps
+  // the compiler from converting it (and so waiting for it) where it arrives
+  asm volatile("" : "+v"(raw));
+  if (threadIdx.x == 0) {
+    const int q = (int)(blockIdx.x % kQueues);
+    const long p = S.qbeg[q] + (long)raw;
+    *slot = p < S.qbeg[q + 1] ? p : walk_claim_queue(S, 1);
+  }
+  __syncthreads();
+  return *slot;
 }
 
 // item at a position: the schedule's order, else (static walk) the
@@ -182,8 +217,11 @@ __device__ __forceinline__ double group_reduce(double (&v)[M], int lane, int &in
 // chunk is function run_fn(crun, ncr, n) (absent columns add nothing).  With
 // the FP32 result placement a lane holds four consecutive points of one
 // column, read as one 16-byte load per component; the FP64 placement reads
-// point by point.
-template <typename T, typename ACC, int BM, int BN, bool GRAD>
+// point by point.  A lane's columns are looked up once; the loads of IB row
+// blocks (IB divides BM / 32) are issued together and unconditionally (an
+// absent column reads function 0 and adds nothing), so their latencies
+// overlap instead of following one another.
+template <typename T, typename ACC, int BM, int BN, bool GRAD, int IB = 1>
 __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 32],
                                                  const T *__restrict__ phi, long Mc, int NP,
                                                  long m0, const uint16_t *crun, int ncr, int ch,
@@ -192,54 +230,70 @@ __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int NC = GRAD ? 4 : 1;
   constexpr bool QUAD = MF<T>::kQuadRows;
+  static_assert(TM % IB == 0, "whole rounds of row blocks");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const long cs = (long)NP * Mc;
+  long col[TN];
+  bool live[TN];
 #pragma unroll
-  for (int i = 0; i < TM; ++i) {
-    double s[NC][4];
+  for (int j = 0; j < TN; ++j) {
+    const long n = run_fn(crun, ncr, wn * WN + j * 16 + mf_col(lane));
+    live[j] = n >= 0;
+    col[j] = live[j] ? n * Mc : 0;
+  }
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
+  for (int i0 = 0; i0 < TM; i0 += IB) {
+    T v[IB][TN][NC][4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) s[c][r] = 0.0;
+    for (int ii = 0; ii < IB; ++ii)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const long n = run_fn(crun, ncr, wn * WN + j * 16 + mf_col(lane));
-      if (n < 0) continue;
-      T v[NC][4];
-      if constexpr (QUAD) {
-        const long q = n * Mc + m0 + wm * WM + i * 16 + MF<T>::row(lane, 0);
+      for (int j = 0; j < TN; ++j) {
+        const long q0 = col[j] + m0 + wm * WM + (i0 + ii) * 16;
+        if constexpr (QUAD) {
+          const long q = q0 + MF<T>::row(lane, 0);
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          const f4 t = gld((const f4 *)(phi + c * cs + q));
-          v[c][0] = t.x; v[c][1] = t.y; v[c][2] = t.z; v[c][3] = t.w;
+          for (int c = 0; c < NC; ++c) {
+            const f4 t = gld((const f4 *)(phi + c * cs + q));
+            v[ii][j][c][0] = t.x; v[ii][j][c][1] = t.y; v[ii][j][c][2] = t.z; v[ii][j][c][3] = t.w;
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[ii][j][c][r] = gld(phi + c * cs + q0 + MF<T>::row(lane, r));
         }
-      } else {
+      }
+#pragma unroll
+    for (int ii = 0; ii < IB; ++ii) {
+      const int i = i0 + ii;
+      double s[NC][4];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s[c][r] = 0.0;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        if (!live[j]) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const long q = n * Mc + m0 + wm * WM + i * 16 + MF<T>::row(lane, r);
+          const double xv = (double)acc[i][j][r];
+          s[0][r] += (double)v[ii][j][0][r] * xv;
 #pragma unroll
-          for (int c = 0; c < NC; ++c) v[c][r] = gld(phi + c * cs + q);
+          for (int c = 1; c < NC; ++c) s[c][r] += 2.0 * (double)v[ii][j][c][r] * xv;
         }
       }
+      double w[NC * 4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const double xv = (double)acc[i][j][r];
-        s[0][r] += (double)v[0][r] * xv;
+      for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int c = 1; c < NC; ++c) s[c][r] += 2.0 * (double)v[c][r] * xv;
+        for (int r = 0; r < 4; ++r) w[c * 4 + r] = s[c][r];
+      int idx;
+      const double tot = group_reduce<NC * 4>(w, lane, idx);
+      if (NC == 4 || (lane & 3) == 0) {
+        const int c = idx / 4, r = idx % 4;
+        red[wn][c][wm * WM + i * 16 + MF<T>::row(lane, r)] = tot;
       }
-    }
-    double v[NC * 4];
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) v[c * 4 + r] = s[c][r];
-    int idx;
-    const double tot = group_reduce<NC * 4>(v, lane, idx);
-    if (NC == 4 || (lane & 3) == 0) {
-      const int c = idx / 4, r = idx % 4;
-      red[wn][c][wm * WM + i * 16 + MF<T>::row(lane, r)] = tot;
     }
   }
   __syncthreads();

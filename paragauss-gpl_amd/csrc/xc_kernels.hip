@@ -560,14 +560,15 @@ template <> struct Var<2> { static constexpr int BK = 32, ST = 2; };
 // item = (128-point tile, function tile) of X = Phi D, then the tile's
 // partial contractions per point
 template <typename T, int BN, int VAR, bool GRAD>
-__global__ __launch_bounds__(kThreads) void k_density(const StageList S) {
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void k_density(const StageList S) {
   constexpr int BM = kDBM, BK = Var<VAR>::BK, ST = Var<VAR>::ST, TM = BM / 32, TN = BN / 32;
   __shared__ T As[ST * BK * BM];
   __shared__ T Bs[ST * BK * BN];
   __shared__ double red[2][4][BM];
   __shared__ uint16_t kl[kMaxRuns];
   __shared__ long slot;
-  for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
+  for (long L = walk_next(S, -1, &slot); L < S.total;) {
     const uint64_t t0 = S.tlog ? wall_clock64() : 0;
     const long it = walk_item(S, L);
     const int u = find_unit(S.base, S.nu, it);
@@ -594,9 +595,11 @@ __global__ __launch_bounds__(kThreads) void k_density(const StageList S) {
     const T *phi = (const T *)U.phi, *D = (const T *)U.D;
     typename MF<T>::acc acc[TM][TN];
     gemm_tile<T, BM, BN, BK, ST, false, false>(phi + m0, D, ksteps, aaddr, baddr, As, Bs, acc);
+    const int ahead = walk_ahead(S);
     density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, Mc, U.NP, m0, crun, ncr, ch,
                                                           U.part, red);
     item_log(S, it, t0);
+    L = walk_take(S, L, ahead, &slot);
   }
 }
 
@@ -792,7 +795,7 @@ __global__ __launch_bounds__(kThreads) void k_potential(const StageList S) {
   __shared__ uint16_t gl[kMaxSplitGroups];
   __shared__ int wtot[4];
   __shared__ long slot;
-  for (long L = walk_next(S, -1, &slot); L < S.total; L = walk_next(S, L, &slot)) {
+  for (long L = walk_next(S, -1, &slot); L < S.total;) {
     const uint64_t t0 = S.tlog ? wall_clock64() : 0;
     const long it = walk_item(S, L);
     const int u = find_unit(S.base, S.nu, it);
@@ -822,9 +825,11 @@ __global__ __launch_bounds__(kThreads) void k_potential(const StageList S) {
     const T *W = (const T *)U.W, *phi = (const T *)U.phi;
     typename MF<T>::acc acc[TT][TT];
     gemm_tile<T, BT, BT, BK, ST, true, true>(W, phi, ng * R, aaddr, baddr, As, Bs, acc);
+    const int ahead = walk_ahead(S);
     potential_epilogue<T, typename MF<T>::acc, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP,
                                                        U.NP, ri, nri, rj, nrj);
     item_log(S, it, t0);
+    L = walk_take(S, L, ahead, &slot);
   }
 }
 

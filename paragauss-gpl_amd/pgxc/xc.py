@@ -119,8 +119,10 @@ class XCBuilder:
 
 
     def set_schedule(self, mode, model=None):
-        """mode: _lib.SCHED_STATIC / SCHED_ANALYTIC / SCHED_MODEL (with a CostModel)."""
+        """mode: _lib.SCHED_STATIC / SCHED_ANALYTIC / SCHED_MODEL (with a CostModel).
+        The builder keeps a reference to the model: the library holds its pointer."""
         self._check(self._L.pgxc_set_schedule(self._h, mode, None if model is None else model.handle))
+        self._model = model
 
     def set_item_log(self, on=True):
         self._check(self._L.pgxc_set_item_log(self._h, 1 if on else 0))
@@ -198,8 +200,10 @@ class Batch:
 
 
     def set_schedule(self, mode, model=None):
-        """mode: _lib.SCHED_STATIC / SCHED_ANALYTIC / SCHED_MODEL (with a CostModel)."""
+        """mode: _lib.SCHED_STATIC / SCHED_ANALYTIC / SCHED_MODEL (with a CostModel).
+        The batch keeps a reference to the model: the library holds its pointer."""
         self._check(self._L.pgxc_batch_set_schedule(self._h, mode, None if model is None else model.handle))
+        self._model = model
 
     def set_item_log(self, on=True):
         self._check(self._L.pgxc_batch_set_item_log(self._h, 1 if on else 0))
