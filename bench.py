@@ -444,12 +444,24 @@ def main():
         executed[prec] = {"density": float(fl[F[:, 0] == 0].sum()),
                           "potential": float(fl[F[:, 0] == 1].sum())}
     xc.set_item_log(False)
-    perr = precision_errors(xc, C, occ, precs)
-    log("precision errors per ||dP||_F:", perr)
-    xc.close()
     ok = [p for p in precs if abs(runs[p]["dE"]) <= 1e-6] or precs
     best = min(ok, key=lambda p: runs[p]["ms"])
     R = runs[best]
+    # per-kernel times for the roofline: builds of the chosen form on one lane,
+    # so that each launch has the device to itself (the timed builds above
+    # overlap the launches of two chunks)
+    xc.set_lanes(1)
+    one = []
+    for _ in range(max(1, args.steps)):
+        xc.set_orbitals(chain[-1], occ, incremental=True)
+        xc.build(PBE, PRECISIONS[best], fetch=False)
+        one.append(xc.timing())
+    xc.set_lanes(2)
+    st1 = {k: float(np.mean([t[k] for t in one])) for k in one[0]}
+    log("%s on one lane: %.2f ms/build stages %s" % (best, st1["total"], st1))
+    perr = precision_errors(xc, C, occ, precs)
+    log("precision errors per ||dP||_F:", perr)
+    xc.close()
 
     ms64 = sum(t64) / len(t64)
     msmx = R["ms"]
@@ -457,7 +469,7 @@ def main():
     # one contraction: half of pgxc_last_stats' sum over 128-point tiles of
     # 4 B n_b^2 + 16 B n_b (n_b: functions not cut off on the tile; the 16 B n_b
     # elementwise share is ~1 %); the dense equivalent 2 npts n^2 beside it.
-    st = R["st"]
+    st = st1
     dom = "density" if st["density"] >= st["potential"] else "potential"
     flops = stats["flops"] / 2.0
     achieved = flops / (st[dom] * 1e-3) / 1e12
@@ -500,9 +512,11 @@ def main():
         "schedule": sched,
         "precision_errors": perr,
         "layout": {k: int(v) for k, v in layout.items()},
-        "stages_ms": {k: round(float(v), 3) for k, v in st.items()},
+        "stages_ms": {k: round(float(v), 3) for k, v in R["st"].items()},
+        "stages_ms_one_lane": {k: round(float(v), 3) for k, v in st1.items()},
         "fp64_stages_ms": {k: round(float(v), 3) for k, v in st64[-1].items()},
-        "roofline": {"bound": "mfma", "kernel": kname,
+        "roofline": {"bound": "mfma", "kernel": kname, "timed": "one-lane builds (%d launches)"
+                     % (len(one) * layout["chunks"]),
                      "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s" + (" (FP32-equivalent: BF16 peak / 3 passes)" if best == "bf16x3" else ""),
                      "frac": achieved / peak, "traffic": traffic,

@@ -252,6 +252,15 @@ int pgxc_batch_destroy(pgxc_batch *batch);
 int pgxc_set_schedule(pgxc_ctx *ctx, int32_t mode, pgxc_costmodel *model);
 int pgxc_batch_set_schedule(pgxc_batch *batch, int32_t mode, pgxc_costmodel *model);
 
+/* Lanes of later builds (no counterpart in the reference: the reference's
+ * batches run one after another on one CPU core).  A grid of more than one
+ * chunk is built on two lanes by default: chunk k runs on stream k % 2 with
+ * its own work buffers, so one chunk's stages overlap the next chunk's.
+ * lanes = 1 runs every chunk on one stream (each launch then has the device
+ * to itself, which is how per-kernel timings are measured); 2 restores the
+ * default.  Results agree to rounding (the H partials are summed per lane). */
+int pgxc_set_lanes(pgxc_ctx *ctx, int32_t lanes);
+
 /* on = 1: later builds record, per contraction work item, its features and
  * the device time its workgroup spent on it. */
 int pgxc_set_item_log(pgxc_ctx *ctx, int32_t on);

@@ -52,7 +52,7 @@ struct pgxc_ctx {
   // own work buffers) so one chunk's stages overlap the next chunk's
   hipStream_t stream2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
-  int nlanes = 1;
+  int nlanes = 1, nlanes_cap = 1;  // lanes in use / with buffers
   std::string err;
   // sizes
   int n_atoms = 0, n_shells = 0, nbas = 0, NP = 0, veclen = 128, tf = 128;
@@ -177,6 +177,7 @@ struct Plan {
   double tick_us = 0.0;
   std::vector<StepPlan> steps;
   int nlanes = 1;  // 2: chunk k runs on lane k % 2 (two streams)
+  std::vector<int> lanes;  // lanes in use per molecule when the plan was made
   PackD *d_pack = nullptr;
   long *d_pack_base = nullptr, *d_prep_base = nullptr, *d_fold_base = nullptr;
   long pack_total = 0, prep_total = 0, fold_total = 0;
@@ -290,8 +291,11 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
                     const std::vector<int> &incr, const std::vector<double> &unit_prio,
                     long order_version, const Sched &sc, Plan *&plan, pgxc_ctx *errc) {
   const int smode = sc.model ? sc.mode : std::min(sc.mode, (int)PGXC_SCHED_ANALYTIC);
+  std::vector<int> lanes;
+  for (auto *c : mols) lanes.push_back(c->nlanes);
   if (plan && plan->functional == functional && plan->precision == precision &&
-      plan->incr == incr && plan->order_version == order_version && plan->sched_mode == smode &&
+      plan->incr == incr && plan->lanes == lanes && plan->order_version == order_version &&
+      plan->sched_mode == smode &&
       plan->log == sc.log && plan->model == sc.model &&
       plan->model_version == costmodel_version(sc.model))
     return PGXC_OK;
@@ -300,6 +304,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   plan->functional = functional;
   plan->precision = precision;
   plan->incr = incr;
+  plan->lanes = lanes;
   plan->order_version = order_version;
   plan->sched_mode = smode;
   plan->log = sc.log;
@@ -689,7 +694,7 @@ static size_t phi_elem_bytes(size_t elem) { return elem == 8 ? 32 : 24; }
 
 static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
   const size_t need_phi = phi_elem_bytes(elem) * (size_t)c->NP * c->Mc;
-  const size_t need_w = (size_t)c->NP * c->Mc * elem * c->nlanes;
+  const size_t need_w = (size_t)c->NP * c->Mc * elem * c->nlanes_cap;
   c->w_lane = (size_t)c->NP * c->Mc * elem;
   bool moved = false;
   if (c->w_bytes < need_w) {
@@ -725,12 +730,12 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
   }
   c->phi_lane = need_phi;
   if (!c->phic[e]) {  // one chunk buffer per lane, rewritten by every build
-    if (c->phi_bytes < need_phi * c->nlanes) {
+    if (c->phi_bytes < need_phi * c->nlanes_cap) {
       dfree(c->phi);
       c->phi = nullptr;
       c->phi_bytes = 0;
-      HIPCHK(c, hipMalloc(&c->phi, need_phi * c->nlanes));
-      c->phi_bytes = need_phi * c->nlanes;
+      HIPCHK(c, hipMalloc(&c->phi, need_phi * c->nlanes_cap));
+      c->phi_bytes = need_phi * c->nlanes_cap;
       c->phi_elem = 0;
     }
     // function rows nbas..NP-1 are never written and must read as zero in the
@@ -1145,7 +1150,7 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
 
   HIPCHK(c, hipGetDevice(&c->device));
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  c->nlanes = c->nchunks > 1 ? 2 : 1;
+  c->nlanes = c->nlanes_cap = c->nchunks > 1 ? 2 : 1;
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
   HIPCHK(c, hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   HIPCHK(c, hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
@@ -1185,7 +1190,7 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   if ((rc = dalloc(c, &c->Vstate, npk))) return rc;
   if ((rc = dalloc(c, &c->Vout, npk))) return rc;
   if ((rc = dalloc(c, &c->epart, 2 * (size_t)c->nblk_total))) return rc;
-  const size_t L = (size_t)c->nlanes;
+  const size_t L = (size_t)c->nlanes_cap;
   if ((rc = dalloc(c, &c->part, L * c->ntiles * 4 * c->Mc))) return rc;
   if ((rc = dalloc(c, &c->coef, L * 4 * c->Mc))) return rc;
   if ((rc = dalloc(c, &c->Hpart, L * c->nsplit * np2))) return rc;
@@ -1351,6 +1356,13 @@ int pgxc_set_schedule(pgxc_ctx *c, int32_t mode, pgxc_costmodel *model) {
     return fail(c, PGXC_ERR_ARG, "pgxc_set_schedule: mode 0..2, and a model for PGXC_SCHED_MODEL");
   c->sched.mode = mode;
   c->sched.model = mode == PGXC_SCHED_MODEL ? model : nullptr;
+  return PGXC_OK;
+}
+
+int pgxc_set_lanes(pgxc_ctx *c, int32_t lanes) {
+  if (!c) return PGXC_ERR_ARG;
+  if (lanes < 1 || lanes > 2) return fail(c, PGXC_ERR_ARG, "pgxc_set_lanes: lanes must be 1 or 2");
+  c->nlanes = std::min<int>(lanes, c->nlanes_cap);
   return PGXC_OK;
 }
 

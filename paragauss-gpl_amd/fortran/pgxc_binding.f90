@@ -27,7 +27,7 @@ module pgxc_binding
   public :: pgxc_becke_weights, pgxc_create_opt, pgxc_layout, pgxc_options
   public :: pgxc_batch_create, pgxc_batch_set_priority, pgxc_batch_build, pgxc_batch_last_ms
   public :: pgxc_batch_last_error, pgxc_batch_destroy
-  public :: pgxc_set_schedule, pgxc_batch_set_schedule, pgxc_set_item_log
+  public :: pgxc_set_schedule, pgxc_batch_set_schedule, pgxc_set_lanes, pgxc_set_item_log
   public :: pgxc_batch_set_item_log, pgxc_item_log, pgxc_batch_item_log
   public :: pgxc_costmodel_create, pgxc_costmodel_nparams, pgxc_costmodel_params
   public :: pgxc_costmodel_set_params, pgxc_costmodel_train, pgxc_costmodel_gradient
@@ -189,6 +189,12 @@ module pgxc_binding
        type(c_ptr), value :: batch, model
        integer(c_int32_t), value :: mode
      end function pgxc_batch_set_schedule
+
+     integer(c_int) function pgxc_set_lanes(ctx, lanes) bind(C, name="pgxc_set_lanes")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: lanes
+     end function pgxc_set_lanes
 
      integer(c_int) function pgxc_set_item_log(ctx, on) bind(C, name="pgxc_set_item_log")
        import :: c_int, c_int32_t, c_ptr

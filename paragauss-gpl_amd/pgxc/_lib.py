@@ -19,7 +19,7 @@ EXPORTS = ("pgxc_create", "pgxc_create_opt", "pgxc_layout", "pgxc_set_density",
            "pgxc_last_timing", "pgxc_last_error", "pgxc_destroy", "pgxc_becke_weights",
            "pgxc_batch_create", "pgxc_batch_set_priority", "pgxc_batch_build",
            "pgxc_batch_last_ms", "pgxc_batch_last_error", "pgxc_batch_destroy",
-           "pgxc_set_schedule", "pgxc_batch_set_schedule", "pgxc_set_item_log",
+           "pgxc_set_schedule", "pgxc_batch_set_schedule", "pgxc_set_lanes", "pgxc_set_item_log",
            "pgxc_batch_set_item_log", "pgxc_item_log", "pgxc_batch_item_log",
            "pgxc_costmodel_create", "pgxc_costmodel_nparams", "pgxc_costmodel_params",
            "pgxc_costmodel_set_params", "pgxc_costmodel_train", "pgxc_costmodel_gradient",
@@ -82,6 +82,7 @@ def load():
     L.pgxc_becke_weights.restype = ctypes.c_int
     L.pgxc_set_schedule.argtypes = [_vp, ctypes.c_int32, _vp]
     L.pgxc_batch_set_schedule.argtypes = [_vp, ctypes.c_int32, _vp]
+    L.pgxc_set_lanes.argtypes = [_vp, ctypes.c_int32]
     L.pgxc_set_item_log.argtypes = [_vp, ctypes.c_int32]
     L.pgxc_batch_set_item_log.argtypes = [_vp, ctypes.c_int32]
     L.pgxc_item_log.argtypes = [_vp, _dp, _dp, ctypes.c_int64]

@@ -124,6 +124,11 @@ class XCBuilder:
         self._check(self._L.pgxc_set_schedule(self._h, mode, None if model is None else model.handle))
         self._model = model
 
+    def set_lanes(self, lanes):
+        """1: every chunk on one stream; 2 (default for grids of several chunks):
+        chunks alternate between two streams (pgxc_set_lanes)."""
+        self._check(self._L.pgxc_set_lanes(self._h, int(lanes)))
+
     def set_item_log(self, on=True):
         self._check(self._L.pgxc_set_item_log(self._h, 1 if on else 0))
 

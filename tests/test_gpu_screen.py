@@ -180,3 +180,22 @@ def test_prime_block_count_layout(dimer):
     (h1, e1, q1), (h2, e2, q2) = out
     assert abs(e1 - e2) <= 1e-10 * abs(e1) and abs(q1 - q2) <= 1e-10 * abs(q1)
     assert np.max(np.abs(h1 - h2)) <= 1e-11
+
+
+@pytest.mark.parametrize("prec", [FP64, BF16X3])
+def test_one_lane_and_two_lanes_agree(dimer, prec):
+    """Chunks on one stream or alternating between two (pgxc_set_lanes) give
+    the same build to the rounding of the per-lane H sums; switching back and
+    forth keeps working."""
+    _, _, b, g, C, occ = dimer
+    with XCBuilder(b, g, chunk_points=8192) as xc:
+        assert xc.layout()["chunks"] > 2
+        out = []
+        for lanes in (2, 1, 2, 1):
+            xc.set_lanes(lanes)
+            xc.set_orbitals(C, occ)
+            out.append(xc.build(PBE, prec))
+    (h2, e2, q2), (h1, e1, q1), (h2b, e2b, _), (h1b, e1b, _) = out
+    assert np.array_equal(h2, h2b) and e2 == e2b and np.array_equal(h1, h1b) and e1 == e1b
+    assert abs(e1 - e2) <= 1e-12 * abs(e1) and abs(q1 - q2) <= 1e-12 * abs(q1)
+    assert np.max(np.abs(h1 - h2)) <= 1e-12 * max(1.0, np.max(np.abs(h1)))
