@@ -72,6 +72,19 @@ def test_h2o_lda_fp64(h2o):
     check(res, oracle_result("h2o_lda", b, g, oracle.MASK_LDA, C=C, occ=occ))
 
 
+@pytest.mark.parametrize("functional,mask,key", [(PBE, oracle.MASK_PBE, "h2o_pbe"),
+                                                  (LDA, oracle.MASK_LDA, "h2o_lda")])
+def test_h2o_fp64_chunked_two_lanes(h2o, functional, mask, key):
+    """Chunks of 16 Ki points alternating between two streams, each with
+    its own work buffers: the same FP64 bar against the oracle."""
+    b, g, C, occ = h2o
+    with XCBuilder(b, g, chunk_points=16384) as xc:
+        assert xc.layout()["chunks"] >= 3
+        xc.set_orbitals(C, occ)
+        res = xc.build(functional, FP64)
+    check(res, oracle_result(key, b, g, mask, C=C, occ=occ))
+
+
 def test_density_matrix_input_and_determinism(h2o):
     b, g, C, occ = h2o
     P = (C * occ) @ C.T
