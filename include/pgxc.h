@@ -252,6 +252,24 @@ int pgxc_batch_destroy(pgxc_batch *batch);
 int pgxc_set_schedule(pgxc_ctx *ctx, int32_t mode, pgxc_costmodel *model);
 int pgxc_batch_set_schedule(pgxc_batch *batch, int32_t mode, pgxc_costmodel *model);
 
+/*
+ * State of incremental builds, for checkpoint / restart.  The reference
+ * saves ham_xc_arr and exc_int to its SCF state file
+ * (xc_hamiltonian_store / xc_hamiltonian_recover,
+ * modules/xc_hamiltonian.f90:2407-2633); those records are unchanged here.
+ * The library's own state between builds (the previous density matrix, the
+ * packed V of the last build and the FP64 per-point density and potential
+ * coefficients) is what an incremental build continues from: saving it
+ * beside those records lets a restarted SCF continue with incremental builds
+ * that give the same bits as without the restart.  pgxc_state_bytes: size of
+ * the record.  pgxc_save_state: needs a build on the context.
+ * pgxc_load_state: only into a context of the same atoms, basis, grid and
+ * layout (checked by a hash recorded in the state); PGXC_ERR_ARG otherwise.
+ */
+int64_t pgxc_state_bytes(const pgxc_ctx *ctx);
+int pgxc_save_state(pgxc_ctx *ctx, void *buf, int64_t bytes);
+int pgxc_load_state(pgxc_ctx *ctx, const void *buf, int64_t bytes);
+
 /* Lanes of later builds (no counterpart in the reference: the reference's
  * batches run one after another on one CPU core).  A grid of more than one
  * chunk is built on two lanes by default: chunk k runs on stream k % 2 with

@@ -91,6 +91,7 @@ struct pgxc_ctx {
   // host
   int have_density = 0, density_mode = 0, have_state = 0;
   uint32_t state_functional = 0;
+  uint64_t geom = 0;  // hash of the inputs that fix the layout (state files)
   double last_exc = 0, last_charge = 0;
   double timing[6] = {0, 0, 0, 0, 0, 0};
   double stats[5] = {0, 0, 0, 0, 0};
@@ -816,6 +817,19 @@ static int collect(pgxc_ctx *c, hipStream_t s, uint32_t functional) {
   return PGXC_OK;
 }
 
+// FNV-1a style hash over 8-byte words (a shorter tail byte by byte).
+static uint64_t hash_words(uint64_t h, const void *p, size_t bytes) {
+  const unsigned char *b = (const unsigned char *)p;
+  size_t i = 0;
+  for (; i + 8 <= bytes; i += 8) {
+    uint64_t w;
+    memcpy(&w, b + i, 8);
+    h = (h ^ w) * 1099511628211ull;
+  }
+  for (; i < bytes; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  return h;
+}
+
 // Order of n points (coordinate k of point p at xyz[k * stride + p * step])
 // along a Morton curve over their bounding box (10 bits per axis), ties in
 // input order.
@@ -1170,6 +1184,17 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
       g[(size_t)k * c->npad + p] =
           p < npts ? grid[(size_t)k * npts + perm[p]] : (k < 3 ? kFar : 0.0);
   HIPCHK(c, hipMemcpy(c->grid, g.data(), g.size() * 8, hipMemcpyHostToDevice));
+  {
+    uint64_t h = 1469598103934665603ull;
+    h = hash_words(h, atom_xyz, 3 * (size_t)n_atoms * 8);
+    h = hash_words(h, shell_info, 5 * (size_t)n_shells * 4);
+    h = hash_words(h, exps, (size_t)eoff * 8);
+    h = hash_words(h, norms, (size_t)eoff * 8);
+    h = hash_words(h, contr, (size_t)coff * 8);
+    h = hash_words(h, grid, 4 * (size_t)npts * 8);
+    const int64_t lay[2] = {c->NP, c->npad};
+    c->geom = hash_words(h, lay, sizeof(lay));
+  }
   HIPCHK(c, hipMemcpy(c->atoms, at.data(), sizeof(AtomD) * n_atoms, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->shells, sh.data(), sizeof(ShellD) * n_shells, hipMemcpyHostToDevice));
   if ((rc = dalloc(c, &c->d_fpos, c->nbas))) return rc;
@@ -1356,6 +1381,76 @@ int pgxc_set_schedule(pgxc_ctx *c, int32_t mode, pgxc_costmodel *model) {
     return fail(c, PGXC_ERR_ARG, "pgxc_set_schedule: mode 0..2, and a model for PGXC_SCHED_MODEL");
   c->sched.mode = mode;
   c->sched.model = mode == PGXC_SCHED_MODEL ? model : nullptr;
+  return PGXC_OK;
+}
+
+// ---- state of incremental builds, for checkpoint / restart ----------------
+struct StateHeader {
+  char magic[8];
+  uint64_t geom;
+  int64_t nbas, NP, npad, npk, nslot;
+  uint32_t functional;
+  int32_t reserved;
+  double exc, charge;
+};
+static const char kStateMagic[8] = {'P', 'G', 'X', 'C', 'S', 'T', '0', '1'};
+
+static int64_t state_bytes(const pgxc_ctx *c) {
+  const int64_t npk = (int64_t)c->nbas * (c->nbas + 1) / 2;
+  return (int64_t)sizeof(StateHeader) +
+         8 * ((int64_t)c->NP * c->NP + npk + (int64_t)S_NSLOT * c->npad);
+}
+
+int64_t pgxc_state_bytes(const pgxc_ctx *c) { return c ? state_bytes(c) : -PGXC_ERR_ARG; }
+
+int pgxc_save_state(pgxc_ctx *c, void *buf, int64_t bytes) {
+  if (!c || !buf) return PGXC_ERR_ARG;
+  if (!c->have_state) return fail(c, PGXC_ERR_STATE, "pgxc_save_state: no build has run on this context");
+  if (bytes < state_bytes(c)) return fail(c, PGXC_ERR_ARG, "pgxc_save_state: buffer smaller than pgxc_state_bytes");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  StateHeader h{};
+  memcpy(h.magic, kStateMagic, 8);
+  h.geom = c->geom;
+  h.nbas = c->nbas;
+  h.NP = c->NP;
+  h.npad = c->npad;
+  h.npk = (int64_t)c->nbas * (c->nbas + 1) / 2;
+  h.nslot = S_NSLOT;
+  h.functional = c->state_functional;
+  h.exc = c->last_exc;
+  h.charge = c->last_charge;
+  char *o = (char *)buf;
+  memcpy(o, &h, sizeof(h));
+  o += sizeof(h);
+  HIPCHK(c, hipMemcpy(o, c->Pref, 8 * (size_t)c->NP * c->NP, hipMemcpyDeviceToHost));
+  o += 8 * (size_t)c->NP * c->NP;
+  HIPCHK(c, hipMemcpy(o, c->Vstate, 8 * (size_t)h.npk, hipMemcpyDeviceToHost));
+  o += 8 * (size_t)h.npk;
+  HIPCHK(c, hipMemcpy(o, c->state, 8 * (size_t)S_NSLOT * c->npad, hipMemcpyDeviceToHost));
+  return PGXC_OK;
+}
+
+int pgxc_load_state(pgxc_ctx *c, const void *buf, int64_t bytes) {
+  if (!c || !buf) return PGXC_ERR_ARG;
+  if (bytes < (int64_t)sizeof(StateHeader)) return fail(c, PGXC_ERR_ARG, "pgxc_load_state: buffer too short");
+  StateHeader h;
+  memcpy(&h, buf, sizeof(h));
+  if (memcmp(h.magic, kStateMagic, 8) != 0) return fail(c, PGXC_ERR_ARG, "pgxc_load_state: not a state record");
+  if (h.geom != c->geom || h.nbas != c->nbas || h.NP != c->NP || h.npad != c->npad ||
+      h.nslot != S_NSLOT || h.npk != (int64_t)c->nbas * (c->nbas + 1) / 2)
+    return fail(c, PGXC_ERR_ARG, "pgxc_load_state: the state belongs to another geometry or layout");
+  if (bytes < state_bytes(c)) return fail(c, PGXC_ERR_ARG, "pgxc_load_state: buffer too short");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const char *o = (const char *)buf + sizeof(h);
+  HIPCHK(c, hipMemcpy(c->Pref, o, 8 * (size_t)c->NP * c->NP, hipMemcpyHostToDevice));
+  o += 8 * (size_t)c->NP * c->NP;
+  HIPCHK(c, hipMemcpy(c->Vstate, o, 8 * (size_t)h.npk, hipMemcpyHostToDevice));
+  o += 8 * (size_t)h.npk;
+  HIPCHK(c, hipMemcpy(c->state, o, 8 * (size_t)S_NSLOT * c->npad, hipMemcpyHostToDevice));
+  c->state_functional = h.functional;
+  c->last_exc = h.exc;
+  c->last_charge = h.charge;
+  c->have_state = 1;
   return PGXC_OK;
 }
 

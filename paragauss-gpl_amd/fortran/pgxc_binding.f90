@@ -28,6 +28,7 @@ module pgxc_binding
   public :: pgxc_batch_create, pgxc_batch_set_priority, pgxc_batch_build, pgxc_batch_last_ms
   public :: pgxc_batch_last_error, pgxc_batch_destroy
   public :: pgxc_set_schedule, pgxc_batch_set_schedule, pgxc_set_lanes, pgxc_set_item_log
+  public :: pgxc_state_bytes, pgxc_save_state, pgxc_load_state
   public :: pgxc_batch_set_item_log, pgxc_item_log, pgxc_batch_item_log
   public :: pgxc_costmodel_create, pgxc_costmodel_nparams, pgxc_costmodel_params
   public :: pgxc_costmodel_set_params, pgxc_costmodel_train, pgxc_costmodel_gradient
@@ -189,6 +190,23 @@ module pgxc_binding
        type(c_ptr), value :: batch, model
        integer(c_int32_t), value :: mode
      end function pgxc_batch_set_schedule
+
+     integer(c_int64_t) function pgxc_state_bytes(ctx) bind(C, name="pgxc_state_bytes")
+       import :: c_int64_t, c_ptr
+       type(c_ptr), value :: ctx
+     end function pgxc_state_bytes
+
+     integer(c_int) function pgxc_save_state(ctx, buf, bytes) bind(C, name="pgxc_save_state")
+       import :: c_int, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx, buf
+       integer(c_int64_t), value :: bytes
+     end function pgxc_save_state
+
+     integer(c_int) function pgxc_load_state(ctx, buf, bytes) bind(C, name="pgxc_load_state")
+       import :: c_int, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx, buf
+       integer(c_int64_t), value :: bytes
+     end function pgxc_load_state
 
      integer(c_int) function pgxc_set_lanes(ctx, lanes) bind(C, name="pgxc_set_lanes")
        import :: c_int, c_int32_t, c_ptr

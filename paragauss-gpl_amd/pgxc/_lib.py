@@ -20,6 +20,7 @@ EXPORTS = ("pgxc_create", "pgxc_create_opt", "pgxc_layout", "pgxc_set_density",
            "pgxc_batch_create", "pgxc_batch_set_priority", "pgxc_batch_build",
            "pgxc_batch_last_ms", "pgxc_batch_last_error", "pgxc_batch_destroy",
            "pgxc_set_schedule", "pgxc_batch_set_schedule", "pgxc_set_lanes", "pgxc_set_item_log",
+           "pgxc_state_bytes", "pgxc_save_state", "pgxc_load_state",
            "pgxc_batch_set_item_log", "pgxc_item_log", "pgxc_batch_item_log",
            "pgxc_costmodel_create", "pgxc_costmodel_nparams", "pgxc_costmodel_params",
            "pgxc_costmodel_set_params", "pgxc_costmodel_train", "pgxc_costmodel_gradient",
@@ -83,6 +84,9 @@ def load():
     L.pgxc_set_schedule.argtypes = [_vp, ctypes.c_int32, _vp]
     L.pgxc_batch_set_schedule.argtypes = [_vp, ctypes.c_int32, _vp]
     L.pgxc_set_lanes.argtypes = [_vp, ctypes.c_int32]
+    L.pgxc_state_bytes.argtypes = [_vp]
+    L.pgxc_save_state.argtypes = [_vp, _vp, ctypes.c_int64]
+    L.pgxc_load_state.argtypes = [_vp, _vp, ctypes.c_int64]
     L.pgxc_set_item_log.argtypes = [_vp, ctypes.c_int32]
     L.pgxc_batch_set_item_log.argtypes = [_vp, ctypes.c_int32]
     L.pgxc_item_log.argtypes = [_vp, _dp, _dp, ctypes.c_int64]
@@ -102,7 +106,7 @@ def load():
                "pgxc_batch_last_error": ctypes.c_char_p,
                "pgxc_costmodel_last_error": ctypes.c_char_p,
                "pgxc_costmodel_nparams": ctypes.c_int32, "pgxc_item_log": ctypes.c_int64,
-               "pgxc_batch_item_log": ctypes.c_int64}
+               "pgxc_batch_item_log": ctypes.c_int64, "pgxc_state_bytes": ctypes.c_int64}
     for name in EXPORTS:
         getattr(L, name).restype = special.get(name, ctypes.c_int)
     _lib = L

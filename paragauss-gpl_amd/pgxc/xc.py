@@ -124,6 +124,21 @@ class XCBuilder:
         self._check(self._L.pgxc_set_schedule(self._h, mode, None if model is None else model.handle))
         self._model = model
 
+    def save_state(self):
+        """The state incremental builds continue from (pgxc_save_state), as bytes."""
+        n = self._L.pgxc_state_bytes(self._h)
+        if n < 0:
+            self._check(int(-n))
+        buf = np.zeros(n, dtype=np.uint8)
+        self._check(self._L.pgxc_save_state(self._h, buf.ctypes.data_as(ctypes.c_void_p), n))
+        return buf.tobytes()
+
+    def load_state(self, data):
+        """Continue from a state saved by save_state on a builder of the same
+        geometry (pgxc_load_state)."""
+        buf = np.frombuffer(data, dtype=np.uint8)
+        self._check(self._L.pgxc_load_state(self._h, buf.ctypes.data_as(ctypes.c_void_p), buf.size))
+
     def set_lanes(self, lanes):
         """1: every chunk on one stream; 2 (default for grids of several chunks):
         chunks alternate between two streams (pgxc_set_lanes)."""

@@ -199,3 +199,29 @@ def test_one_lane_and_two_lanes_agree(dimer, prec):
     assert np.array_equal(h2, h2b) and e2 == e2b and np.array_equal(h1, h1b) and e1 == e1b
     assert abs(e1 - e2) <= 1e-12 * abs(e1) and abs(q1 - q2) <= 1e-12 * abs(q1)
     assert np.max(np.abs(h1 - h2)) <= 1e-12 * max(1.0, np.max(np.abs(h1)))
+
+
+def test_state_save_and_load_continue_bitwise(dimer):
+    """An incremental build continued from a saved state in a new builder of
+    the same geometry (a restarted SCF) gives the bits of the uninterrupted
+    one; a builder of another grid refuses the state."""
+    from pgxc._lib import PgxcError
+    _, _, b, g, C, occ = dimer
+    C1 = systems.perturbed_orbitals(C, 1e-3, seed=31)
+    C2 = systems.perturbed_orbitals(C1, 1e-3, seed=32)
+    with XCBuilder(b, g, chunk_points=8192) as xa:
+        xa.set_orbitals(C, occ)
+        xa.build(PBE, FP64)
+        xa.set_orbitals(C1, occ, incremental=True)
+        xa.build(PBE, BF16X3)
+        blob = xa.save_state()
+        xa.set_orbitals(C2, occ, incremental=True)
+        ref = xa.build(PBE, BF16X3)
+    with XCBuilder(b, g, chunk_points=8192) as xb:
+        xb.load_state(blob)
+        xb.set_orbitals(C2, occ, incremental=True)
+        out = xb.build(PBE, BF16X3)
+    assert np.array_equal(ref[0], out[0]) and ref[1] == out[1] and ref[2] == out[2]
+    with XCBuilder(b, g[:-4096], chunk_points=8192) as xc:
+        with pytest.raises(PgxcError):
+            xc.load_state(blob)
