@@ -729,13 +729,13 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
           hi[j] = __builtin_bit_cast(uint16_t, h);
           lo[j] = __builtin_bit_cast(uint16_t, l);
         }
-        *(h4 *)(Wh + q) = hi;
-        *(h4 *)(Wl + q) = lo;
+        gst((h4 *)(Wh + q), hi);
+        gst((h4 *)(Wl + q), lo);
       } else {
         t4 w;
 #pragma unroll
         for (int j = 0; j < 4; ++j) w[j] = (T)v[j];
-        *(t4 *)(W + q) = w;
+        gst((t4 *)(W + q), w);
       }
     };
     const uint32_t *sg = U.sig8 + (p / kGroup) * U.nw8, *wn = U.wbits + (p / kGroup) * U.nw8;
@@ -743,8 +743,8 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
     for (int h = 0; h < 2; ++h) {  // the item's two runs of 8 functions
       const long r = f0 / kRun + h, fr = r * kRun;
       const uint32_t bit = 1u << (r & 31);
-      if (!(sg[r >> 5] & bit)) {
-        if (wn[r >> 5] & bit) {
+      if (!(gld(sg + (r >> 5)) & bit)) {
+        if (gld(wn + (r >> 5)) & bit) {
           const double z[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
           for (int k = 0; k < kRun; ++k) put((fr + k) * Mc + p, z);
@@ -754,21 +754,21 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
       double a[4], b0[4] = {}, b1[4] = {}, b2[4] = {};
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        a[j] = U.coef[p + j];
+        a[j] = gld(U.coef + p + j);
         if (GRAD) {
-          b0[j] = U.coef[Mc + p + j];
-          b1[j] = U.coef[2 * Mc + p + j];
-          b2[j] = U.coef[3 * Mc + p + j];
+          b0[j] = gld(U.coef + Mc + p + j);
+          b1[j] = gld(U.coef + 2 * Mc + p + j);
+          b2[j] = gld(U.coef + 3 * Mc + p + j);
         }
       }
 #pragma unroll 4
       for (int k = 0; k < kRun; ++k) {
         const long q = (fr + k) * Mc + p;
-        const t4 v0 = *(const t4 *)(phi + q);
+        const t4 v0 = gld((const t4 *)(phi + q));
         double v[4];
         if (GRAD) {
-          const t4 v1 = *(const t4 *)(phi + cs + q), v2 = *(const t4 *)(phi + 2 * cs + q),
-                   v3 = *(const t4 *)(phi + 3 * cs + q);
+          const t4 v1 = gld((const t4 *)(phi + cs + q)), v2 = gld((const t4 *)(phi + 2 * cs + q)),
+                   v3 = gld((const t4 *)(phi + 3 * cs + q));
 #pragma unroll
           for (int j = 0; j < 4; ++j)
             v[j] = a[j] * (double)v0[j] + b0[j] * (double)v1[j] + b1[j] * (double)v2[j] +
