@@ -266,8 +266,8 @@ static void unit_item_features(const pgxc_ctx *c, int step, ItemKind kind, int p
     for (long i = c->h_di_off[step]; i < c->h_di_off[step + 1]; ++i) {
       const uint32_t e = c->h_di[(size_t)i];
       const long T = T0 + (e >> 8);
-      const int ch = (int)(e & 255), c8 = c->h_tl_off[(size_t)T + 1] - c->h_tl_off[(size_t)T];
-      const double cols = (double)kRun * std::min(CH, c8 - ch * CH);
+      const int pr = (int)(e & 255), c8 = c->h_tl_off[(size_t)T + 1] - c->h_tl_off[(size_t)T];
+      const double cols = (double)kRun * std::min(2 * CH, c8 - 2 * pr * CH);
       push(0, (double)c8 * kRun, c->NP, true, 2.0 * kScreenTile * cols * c8 * kRun);
     }
   } else {
@@ -1014,7 +1014,8 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
     for (long t = 0; t < tpc; ++t) {
       const long T = k * tpc + t;
       const int c8 = c->h_tl_off[(size_t)T + 1] - c->h_tl_off[(size_t)T];
-      for (int ch = 0; ch < (c8 + CH - 1) / CH; ++ch) c->h_di.push_back((uint32_t)(t << 8 | ch));
+      // density items: pairs of column chunks of the tile's runs
+      for (int pr = 0; pr < (c8 + 2 * CH - 1) / (2 * CH); ++pr) c->h_di.push_back((uint32_t)(t << 8 | pr));
     }
     c->h_di_off.push_back((long)c->h_di.size());
     for (int sl = 0; sl < c->nsplit; ++sl) {

@@ -49,8 +49,8 @@ struct UnitD {
   double *Hpart;    // [nsplit][NP][NP] potential partials of the molecule
   // Work of the contractions, over gathered runs of 8 functions (all runs
   // when screening is off).  ditems / pitems: the unit's density items
-  // (tile << 8 | column chunk) and potential items (split << 20 | row chunk
-  // << 10 | column chunk).  tl_off / tl: per 128-point tile of the unit, its
+  // (tile << 8 | pair of column chunks) and potential items (split << 20 |
+  // row chunk << 10 | column chunk).  tl_off / tl: per 128-point tile of the unit, its
   // ascending run list (tl_off offset to the unit's first tile, values index
   // tl).  su_off / su: per point split of the unit, the union of its groups'
   // runs; a chunk is CH = tile / 8 consecutive entries of a list.  gcf_off /

@@ -45,15 +45,15 @@ typedef const u32x4 __attribute__((address_space(1))) *gvec;
 // loads (a missing row or k run reads the plane's first vector instead) and
 // the zeros are applied when the vector is written to LDS, so no load sits in
 // a branch and the loads of later steps stay in flight.
-template <int R>
+template <int R, int NT = kThreads>
 struct PSOperand {
   struct Src {
     const __bf16 *hi, *lo;
   };
   static constexpr int NV = R * (kXBK / 8) * 2;  // 16-byte vectors per stage
-  static constexpr int PER = NV / kThreads;
-  static_assert(PER * kThreads == NV, "whole vectors per thread");
-  static_assert((R * 4) % 4 == 0 && kThreads % 4 == 0, "k run = threadIdx.x & 3");
+  static constexpr int PER = NV / NT;
+  static_assert(PER * NT == NV, "whole vectors per thread");
+  static_assert(NT % 4 == 0, "k run = threadIdx.x & 3");
   const __bf16 *row[PER];  // plane + row offset of vector e, nullptr: row absent
   const __bf16 *dummy;
   u32x4 v[2][PER];
@@ -72,7 +72,7 @@ struct PSOperand {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       int plane, r, c;
-      place(threadIdx.x + kThreads * e, plane, r, c);
+      place(threadIdx.x + NT * e, plane, r, c);
       const long o = rowf(r);
       row[e] = o >= 0 ? (plane ? S.lo : S.hi) + o : nullptr;
     }
@@ -94,25 +94,29 @@ struct PSOperand {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       int plane, r, c;
-      place(threadIdx.x + kThreads * e, plane, r, c);
+      place(threadIdx.x + NT * e, plane, r, c);
       const u32x4 x = ((ok[SL] >> e) & 1u) ? v[SL][e] : u32x4{0u, 0u, 0u, 0u};
       *(u32x4 *)((plane ? lo : hi) + r * kXBKP + 8 * c) = x;
     }
   }
 };
 
-// OA / OB: operand staging (PSOperand<R>, rows set by init).  kofs(kt, c):
-// element offset of k run c (8 k) of step kt, or -1 for zeros.  Pipeline: two
-// LDS stages and two register slots; the global loads of step kt + 2 are
-// issued before the MFMAs of step kt, so each load has two steps of MFMA work
-// to arrive in.
-template <int BM, int BN, class OA, class OB, class KOfs>
-__device__ __forceinline__ void gemm_tile_bf16x3(OA &A, OB &B, int ksteps, const KOfs &kofs,
-                                                 __bf16 *lds, f4 (&acc)[BM / 32][BN / 32]) {
-  constexpr int BKP = kXBKP, WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+// OA / OB: operand staging (PSOperand<R, NT>, rows set by init).  kofs(kt,
+// c): element offset of k run c (8 k) of step kt, or -1 for zeros.  The NT
+// threads form NT / 64 waves in a (NT / 64 / WGN) x WGN grid, each with a
+// (BM / rows) x (BN / WGN) tile of 16x16 blocks; acc holds one wave's blocks.
+// Pipeline: two LDS stages and two register slots; the global loads of step
+// kt + 2 are issued before the MFMAs of step kt, so each load has two steps
+// of MFMA work to arrive in.
+template <int BM, int BN, int NT, int WGN, class OA, class OB, class KOfs>
+__device__ __forceinline__ void gemm_tile_bf16x3(
+    OA &A, OB &B, int ksteps, const KOfs &kofs, __bf16 *lds,
+    f4 (&acc)[BM / (NT / 64 / WGN) / 16][BN / WGN / 16]) {
+  constexpr int WGM = NT / 64 / WGN;
+  constexpr int BKP = kXBKP, WM = BM / WGM, WN = BN / WGN, TM = WM / 16, TN = WN / 16;
   constexpr int SA = BM * BKP, SB = BN * BKP, STAGE = 2 * SA + 2 * SB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, kc = threadIdx.x & 3;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WGN, wn = wave % WGN;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -191,15 +195,19 @@ __device__ __forceinline__ int find_unit_x3(const long *__restrict__ base, int n
 }
 
 
-// item = (128-point tile, function tile), as k_density
+// item = (128-point tile, pair of BN-column chunks of its gathered runs), as
+// k_density: eight waves in 2 x 4, each with the 64 x 48 tile of the
+// four-wave kernels, so the point panel A is staged once for 2 BN columns;
+// wave columns 0-1 give the first chunk's partials, 2-3 the second's.
+constexpr int kDensThreads = 512;
 template <int BN, bool GRAD>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_density_bf16x3(const StageList S) {
-  constexpr int BM = kXBM;
-  __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BM, BN>::n];
+__global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_density_bf16x3(const StageList S) {
+  constexpr int BM = kXBM, NT = kDensThreads, WGN = 4, BN2 = 2 * BN;
+  __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BM, BN2>::n];
   // the epilogue's reduction buffer reuses the staging LDS (free after the
-  // contraction), so two workgroups fit a CU's 160 KB
-  static_assert(sizeof(double) * 2 * 4 * BM <= sizeof(__bf16) * 2 * XStage<BM, BN>::n, "alias");
-  double(&red)[2][4][BM] = *reinterpret_cast<double(*)[2][4][BM]>(lds);
+  // contraction)
+  static_assert(sizeof(double) * WGN * 4 * BM <= sizeof(__bf16) * 2 * XStage<BM, BN2>::n, "alias");
+  double(&red)[WGN][4][BM] = *reinterpret_cast<double(*)[WGN][4][BM]>(lds);
   __shared__ uint16_t kl[kMaxRuns];
   __shared__ long slot;
   static_assert(kXBK == 4 * kRun, "four runs per K step");
@@ -209,29 +217,31 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const int u = find_unit_x3(S.base, S.nu, it);
     const UnitD &U = S.units[u];
     const uint32_t e = U.ditems[it - S.base[u]];
-    const int tile = (int)(e >> 8), ch = (int)(e & 255);
+    const int tile = (int)(e >> 8), ch = 2 * (int)(e & 255);
     const long m0 = (long)tile * BM, NP = U.NP;
     const int off = U.tl_off[tile], c8 = U.tl_off[tile + 1] - off;
     __syncthreads();  // the previous item's readers of kl are done
     lds_list(U.tl + off, c8, kl);
     const uint16_t *crun = kl + ch * U.CH;
-    const int ncr = min(U.CH, c8 - ch * U.CH);
+    const int ncr = min(2 * U.CH, c8 - ch * U.CH), nslots = ncr > U.CH ? 2 : 1;
     // A(p, k) = Phi^T[m0 + p][f(k)], B(k, n) = D^T[col(n)][f(k)]
     const __bf16 *th = (const __bf16 *)U.phit, *dh = (const __bf16 *)U.D;
     const long np2 = NP * NP, pn = U.Mc * NP;
-    PSOperand<BM> opa;
-    PSOperand<BN> opb;
+    PSOperand<BM, NT> opa;
+    PSOperand<BN2, NT> opb;
     opa.init({th, th + pn}, [&](int m) -> long { return (m0 + m) * NP; });
     opb.init({dh, dh + np2}, [&](int n) -> long {
       const long c = run_fn(crun, ncr, n);
       return c < 0 ? -1 : c * NP;
     });
     auto kofs = [&](int kt, int c) -> long { return run_fn(kl, c8, kt * kXBK + 8 * c); };
-    f4 acc[BM / 32][BN / 32];
-    gemm_tile_bf16x3<BM, BN>(opa, opb, (c8 + 3) / 4, kofs, lds, acc);
+    f4 acc[BM / 2 / 16][BN2 / WGN / 16];
+    gemm_tile_bf16x3<BM, BN2, NT, WGN>(opa, opb, (c8 + 3) / 4, kofs, lds, acc);
     const int ahead = walk_ahead(S);
-    density_epilogue<float, f4, BM, BN, GRAD, 2>(acc, (const float *)U.phi, U.Mc, U.NP, m0, crun, ncr,
-                                              ch, U.part, red);
+    constexpr int IB = BN2 / WGN / 16 > 3 ? 1 : 2;  // row blocks per round: within 256 registers
+    density_epilogue<float, f4, BM, BN2, GRAD, IB, NT, WGN>(acc, (const float *)U.phi, U.Mc,
+                                                           U.NP, m0, crun, ncr, ch, U.part, red,
+                                                           nslots);
     item_log(S, it, t0);
     L = walk_take(S, L, ahead, &slot);
   }
@@ -240,11 +250,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 
 template <int BN>
 static void density_x3(hipStream_t s, bool grad, const StageList &L) {
-  dim3 g(contraction_grid(L, 2));
+  dim3 g(contraction_grid(L, 1));
   if (grad)
-    hipLaunchKernelGGL((k_density_bf16x3<BN, true>), g, dim3(kThreads), 0, s, L);
+    hipLaunchKernelGGL((k_density_bf16x3<BN, true>), g, dim3(kDensThreads), 0, s, L);
   else
-    hipLaunchKernelGGL((k_density_bf16x3<BN, false>), g, dim3(kThreads), 0, s, L);
+    hipLaunchKernelGGL((k_density_bf16x3<BN, false>), g, dim3(kDensThreads), 0, s, L);
 }
 
 void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L) {
@@ -295,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
     });
     auto kofs = [&](int kt, int c) -> long { return kb + (long)gl[kt] * kGroup + 8 * c; };
     f4 acc[BT / 32][BT / 32];
-    gemm_tile_bf16x3<BT, BT>(opa, opb, ng, kofs, lds, acc);
+    gemm_tile_bf16x3<BT, BT, kThreads, 2>(opa, opb, ng, kofs, lds, acc);
     const int ahead = walk_ahead(S);
     potential_epilogue<float, f4, BT, BT>(acc, U.Hpart + (long)sl * U.NP * U.NP, U.NP, ri, nri, rj,
                                           nrj);

@@ -148,7 +148,7 @@ __device__ __forceinline__ int block_compact(bool act, int value, uint16_t *list
 
 // Copy n list entries into LDS (every thread calls it; ends with a barrier).
 __device__ __forceinline__ void lds_list(const uint16_t *__restrict__ src, int n, uint16_t *dst) {
-  for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = gld(src + i);
+  for (int i = threadIdx.x; i < n; i += (int)blockDim.x) dst[i] = gld(src + i);
   __syncthreads();
 }
 
@@ -218,21 +218,24 @@ __device__ __forceinline__ double group_reduce(double (&v)[M], int lane, int &in
 // the FP32 result placement a lane holds four consecutive points of one
 // column, read as one 16-byte load per component; the FP64 placement reads
 // point by point.  A lane's columns are looked up once; the loads of IB row
-// blocks (IB divides BM / 32) are issued together and unconditionally (an
-// absent column reads function 0 and adds nothing), so their latencies
-// overlap instead of following one another.
-template <typename T, typename ACC, int BM, int BN, bool GRAD, int IB = 1>
-__device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 32],
-                                                 const T *__restrict__ phi, long Mc, int NP,
-                                                 long m0, const uint16_t *crun, int ncr, int ch,
-                                                 double *__restrict__ part,
-                                                 double (&red)[2][4][BM]) {
-  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
-  constexpr int NC = GRAD ? 4 : 1;
+// blocks (IB divides the wave's row blocks) are issued together and
+// unconditionally (an absent column reads function 0 and adds nothing), so
+// their latencies overlap instead of following one another.  Waves as in
+// gemm_tile_bf16x3 (NT threads, WGN wave columns); every two wave columns
+// make one partial slot: slot ch + s sums wave columns 2s and 2s + 1, and
+// the first nslots slots are written.
+template <typename T, typename ACC, int BM, int BN, bool GRAD, int IB = 1, int NT = kThreads,
+          int WGN = 2>
+__device__ __forceinline__ void density_epilogue(
+    const ACC (&acc)[BM / (NT / 64 / WGN) / 16][BN / WGN / 16], const T *__restrict__ phi,
+    long Mc, int NP, long m0, const uint16_t *crun, int ncr, int ch, double *__restrict__ part,
+    double (&red)[WGN][4][BM], int nslots = 1) {
+  constexpr int WGM = NT / 64 / WGN, WM = BM / WGM, WN = BN / WGN, TM = WM / 16, TN = WN / 16;
+  constexpr int NC = GRAD ? 4 : 1, SLOTS = WGN / 2;
   constexpr bool QUAD = MF<T>::kQuadRows;
-  static_assert(TM % IB == 0, "whole rounds of row blocks");
+  static_assert(TM % IB == 0 && WGN % 2 == 0, "whole rounds of row blocks, wave column pairs");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WGN, wn = wave % WGN;
   const long cs = (long)NP * Mc;
   long col[TN];
   bool live[TN];
@@ -297,9 +300,11 @@ __device__ __forceinline__ void density_epilogue(const ACC (&acc)[BM / 32][BN / 
     }
   }
   __syncthreads();
-  for (int idx = tid; idx < NC * BM; idx += kThreads) {
-    const int c = idx / BM, row = idx % BM;
-    gst(part + ((long)ch * 4 + c) * Mc + m0 + row, red[0][c][row] + red[1][c][row]);
+  for (int idx = tid; idx < SLOTS * NC * BM; idx += NT) {
+    const int sl = idx / (NC * BM), c = (idx / BM) % NC, row = idx % BM;
+    if (sl < nslots)
+      gst(part + ((long)(ch + sl) * 4 + c) * Mc + m0 + row,
+          red[2 * sl][c][row] + red[2 * sl + 1][c][row]);
   }
 }
 

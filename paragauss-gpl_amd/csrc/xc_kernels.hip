@@ -557,8 +557,9 @@ template <> struct Var<0> { static constexpr int BK = 16, ST = 2; };
 template <> struct Var<1> { static constexpr int BK = 32, ST = 1; };
 template <> struct Var<2> { static constexpr int BK = 32, ST = 2; };
 
-// item = (128-point tile, function tile) of X = Phi D, then the tile's
-// partial contractions per point
+// item = (128-point tile, pair of function tiles of its gathered runs) of
+// X = Phi D, then the tile's partial contractions per point, one slot per
+// function tile
 template <typename T, int BN, int VAR, bool GRAD>
 __global__ __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void k_density(const StageList S) {
@@ -574,30 +575,34 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void k_density(cons
     const int u = find_unit(S.base, S.nu, it);
     const UnitD &U = S.units[u];
     const uint32_t e = U.ditems[it - S.base[u]];
-    const int tile = (int)(e >> 8), ch = (int)(e & 255);
+    const int tile = (int)(e >> 8), pair = (int)(e & 255);
     const long m0 = (long)tile * BM, Mc = U.Mc, NP = U.NP;
     const int off = U.tl_off[tile], c8 = U.tl_off[tile + 1] - off;
     __syncthreads();  // the previous item's readers of kl are done
     lds_list(U.tl + off, c8, kl);
-    const uint16_t *crun = kl + ch * U.CH;
-    const int ncr = min(U.CH, c8 - ch * U.CH);
     const int ksteps = (c8 * kRun + BK - 1) / BK;
-    // A(m, k) = Phi[f(k)][m0 + m], B(k, n) = D[f(k)][col(n)]: K and the
-    // columns run over the tile's gathered runs
-    auto aaddr = [&](int m, int kt, int kk) -> long {
-      const long f = run_fn(kl, c8, kt * BK + kk);
-      return f < 0 ? -1 : f * Mc + m;
-    };
-    auto baddr = [&](int n, int kt, int kk) -> long {
-      const long f = run_fn(kl, c8, kt * BK + kk), c = run_fn(crun, ncr, n);
-      return (f < 0 || c < 0) ? -1 : f * NP + c;
-    };
     const T *phi = (const T *)U.phi, *D = (const T *)U.D;
-    typename MF<T>::acc acc[TM][TN];
-    gemm_tile<T, BM, BN, BK, ST, false, false>(phi + m0, D, ksteps, aaddr, baddr, As, Bs, acc);
-    const int ahead = walk_ahead(S);
-    density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, Mc, U.NP, m0, crun, ncr, ch,
-                                                          U.part, red);
+    int ahead = 0;
+    // the item's two column chunks in turn (the second may be absent)
+    for (int ch = 2 * pair; ch < 2 * pair + 2 && ch * U.CH < c8; ++ch) {
+      const uint16_t *crun = kl + ch * U.CH;
+      const int ncr = min(U.CH, c8 - ch * U.CH);
+      // A(m, k) = Phi[f(k)][m0 + m], B(k, n) = D[f(k)][col(n)]: K and the
+      // columns run over the tile's gathered runs
+      auto aaddr = [&](int m, int kt, int kk) -> long {
+        const long f = run_fn(kl, c8, kt * BK + kk);
+        return f < 0 ? -1 : f * Mc + m;
+      };
+      auto baddr = [&](int n, int kt, int kk) -> long {
+        const long f = run_fn(kl, c8, kt * BK + kk), c = run_fn(crun, ncr, n);
+        return (f < 0 || c < 0) ? -1 : f * NP + c;
+      };
+      typename MF<T>::acc acc[TM][TN];
+      gemm_tile<T, BM, BN, BK, ST, false, false>(phi + m0, D, ksteps, aaddr, baddr, As, Bs, acc);
+      if (ch == 2 * pair + 1 || (ch + 1) * U.CH >= c8) ahead = walk_ahead(S);  // the last chunk
+      density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, Mc, U.NP, m0, crun, ncr,
+                                                            ch, U.part, red);
+    }
     item_log(S, it, t0);
     L = walk_take(S, L, ahead, &slot);
   }
