@@ -52,6 +52,10 @@ def lib():
             getattr(L, name).argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.or_functional.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.or_functional.restype = ctypes.c_int
+        L.or_functional_spin.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 8
+        L.or_functional_spin.restype = ctypes.c_int
+        L.or_pw92_spin.argtypes = [ctypes.c_int] + [_dp] * 5
+        L.or_pbec_spin.argtypes = [ctypes.c_int] + [_dp] * 9
         L.or_xc_build.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _dp,
                                   _dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp,
@@ -71,6 +75,8 @@ def ref_lib():
         L = ctypes.CDLL(path, mode=os.RTLD_LAZY)
         L.pgref_pw92.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         L.pgref_pbec.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.pgref_pw92_spin.argtypes = [ctypes.c_int] + [_dp] * 5
+        L.pgref_pbec_spin.argtypes = [ctypes.c_int] + [_dp] * 11
         L.pgref_solid_harmonics.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
         _REF = L
     return _REF
@@ -114,6 +120,49 @@ def functional(mask, rho, gam):
     if rc != 0:
         raise ValueError("or_functional: unsupported functional mask %#x (%d)" % (mask, rc))
     return f, fr, fg
+
+
+def functional_spin(mask, na, nb, gaa, gbb, gab):
+    """ispin = 2: f, fr (n, 2) = dF/d(na, nb), fg (n, 3) = dF/d(gaa, gbb, gab)."""
+    na, nb, gaa, gbb, gab = (_c(a) for a in (na, nb, gaa, gbb, gab))
+    n = na.size
+    f, fr, fg = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3))
+    rc = lib().or_functional_spin(mask, n, _d(na), _d(nb), _d(gaa), _d(gbb), _d(gab), _d(f),
+                                  _d(fr), _d(fg))
+    if rc != 0:
+        raise ValueError("or_functional_spin: unsupported functional mask %#x (%d)" % (mask, rc))
+    return f, fr, fg
+
+
+def pw92_spin(na, nb):
+    na, nb = _c(na), _c(nb)
+    f, fa, fb = np.zeros_like(na), np.zeros_like(na), np.zeros_like(na)
+    lib().or_pw92_spin(na.size, _d(na), _d(nb), _d(f), _d(fa), _d(fb))
+    return f, fa, fb
+
+
+def pbec_spin(na, nb, gaa, gbb, gab):
+    """f, dF/dna, dF/dnb, dF/dg2 (g2 = gaa + gbb + 2 gab)."""
+    na, nb, gaa, gbb, gab = (_c(a) for a in (na, nb, gaa, gbb, gab))
+    out = [np.zeros_like(na) for _ in range(4)]
+    lib().or_pbec_spin(na.size, _d(na), _d(nb), _d(gaa), _d(gbb), _d(gab), *(_d(o) for o in out))
+    return tuple(out)
+
+
+def ref_pw92_spin(na, nb):
+    na, nb = _c(na), _c(nb)
+    f, fa, fb = np.zeros_like(na), np.zeros_like(na), np.zeros_like(na)
+    ref_lib().pgref_pw92_spin(na.size, _d(na), _d(nb), _d(f), _d(fa), _d(fb))
+    return f, fa, fb
+
+
+def ref_pbec_spin(na, nb, gaa, gbb, gab):
+    """f, dF/dna, dF/dnb, dF/dgaa, dF/dgbb, dF/dgab from the reference module."""
+    na, nb, gaa, gbb, gab = (_c(a) for a in (na, nb, gaa, gbb, gab))
+    out = [np.zeros_like(na) for _ in range(6)]
+    ref_lib().pgref_pbec_spin(na.size, _d(na), _d(nb), _d(gaa), _d(gbb), _d(gab),
+                              *(_d(o) for o in out))
+    return tuple(out)
 
 
 def ref_pw92(rho):

@@ -450,6 +450,210 @@ int or_functional(int mask, int n, const double *rho, const double *gam,
 }
 
 /* ------------------------------------------------------------------------ */
+/* functionals, spin polarised (ispin = 2)                                   */
+/* ------------------------------------------------------------------------ */
+/* Forward-mode values with first derivatives in three variables, as the
+ * reference's ad2x3 objects (autodiff/autodiff2.f90) carry them; the
+ * expressions below follow the reference's unrestricted branches line by
+ * line. */
+typedef struct { double v, d[3]; } dual;
+
+static dual dv(double v, int k)
+{
+  dual r = {v, {0, 0, 0}};
+  if (k >= 0) r.d[k] = 1.0;
+  return r;
+}
+static dual dc(double v) { return dv(v, -1); }
+static dual dadd(dual a, dual b)
+{
+  dual r = {a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]}};
+  return r;
+}
+static dual dsub(dual a, dual b)
+{
+  dual r = {a.v - b.v, {a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2]}};
+  return r;
+}
+static dual dmul(dual a, dual b)
+{
+  dual r;
+  r.v = a.v * b.v;
+  for (int k = 0; k < 3; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+static dual ddiv(dual a, dual b)
+{
+  dual r;
+  r.v = a.v / b.v;
+  for (int k = 0; k < 3; ++k) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v);
+  return r;
+}
+static dual dscale(dual a, double c)
+{
+  dual r = {a.v * c, {a.d[0] * c, a.d[1] * c, a.d[2] * c}};
+  return r;
+}
+static dual dpowc(dual a, double c) /* a**c, a > 0 */
+{
+  dual r;
+  r.v = pow(a.v, c);
+  const double g = c * pow(a.v, c - 1.0);
+  for (int k = 0; k < 3; ++k) r.d[k] = g * a.d[k];
+  return r;
+}
+static dual dlog(dual a)
+{
+  dual r;
+  r.v = log(a.v);
+  for (int k = 0; k < 3; ++k) r.d[k] = a.d[k] / a.v;
+  return r;
+}
+static dual dexp(dual a)
+{
+  dual r;
+  r.v = exp(a.v);
+  for (int k = 0; k < 3; ++k) r.d[k] = r.v * a.d[k];
+  return r;
+}
+
+/* PW92 parameter sets (do_ECU, do_ECP, do_mALF; RPA = false), p = 1 */
+static const double PW92_SET[3][6] = {
+    {0.031091, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294},
+    {0.015545, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517},
+    {0.016887, 0.11125, 10.357, 3.6231, 0.88026, 0.49671}};
+
+/* g(rs): pw_ldac_module.f90 c_lda's contained g */
+static dual pw92_gd(dual rs, int set)
+{
+  const double *q = PW92_SET[set];
+  const double a = q[0], a1 = q[1], b1 = q[2], b2 = q[3], b3 = q[4], b4 = q[5];
+  const dual rs1d2 = dpowc(rs, 0.5), rs3d2 = dmul(rs, rs1d2), rsp = rs; /* p = 1 */
+  const dual Q0 = dscale(dadd(dc(1.0), dscale(rs, a1)), -2.0 * a);
+  const dual Q1 = dscale(dadd(dadd(dadd(dscale(rs1d2, b1), dscale(rs, b2)), dscale(rs3d2, b3)),
+                              dscale(dmul(rs, rsp), b4)), 2.0 * a);
+  const dual Q2 = dlog(dadd(dc(1.0), ddiv(dc(1.0), Q1)));
+  return dmul(Q0, Q2);
+}
+
+/* c_lda(na, nb) of pw_ldac_module.f90:127-195 (same in pbe_ggcxc_module) */
+static dual pw92_clda(dual na, dual nb)
+{
+  const double c1d3 = 1.0 / 3.0, c4d3 = 4.0 / 3.0;
+  const double crs = pow(3.0 / (4.0 * PI), c1d3);
+  const double cf = 1.0 / (pow(2.0, c4d3) - 2.0);
+  const double cfzz = 2.0 * c4d3 * c1d3 * cf;
+  dual nn = dadd(na, nb);
+  if (nn.v < DEPS) nn = dc(DEPS);
+  const dual rs = ddiv(dc(crs), dpowc(nn, c1d3));
+  const dual ECU = pw92_gd(rs, 0), ECP = pw92_gd(rs, 1), mALF = pw92_gd(rs, 2);
+  const dual z = ddiv(dsub(na, nb), nn), z2 = dmul(z, z), z4 = dmul(z2, z2);
+  const dual f = dscale(dsub(dadd(dpowc(dadd(dadd(dc(1.0), z), dc(DEPS)), c4d3),
+                                  dpowc(dadd(dsub(dc(1.0), z), dc(DEPS)), c4d3)), dc(2.0)), cf);
+  const dual EC = dsub(dadd(ECU, dmul(dmul(dsub(ECP, ECU), f), z4)),
+                       dscale(dmul(dmul(mALF, f), dsub(dc(1.0), z4)), 1.0 / cfzz));
+  return dmul(nn, EC);
+}
+
+/* c_gga(na, nb, g2, beta) of pbe_ggcxc_module.f90:96-140 */
+static dual pbe_cgga(dual na, dual nb, dual g2)
+{
+  const double beta = 0.066725, gamma = 0.03109069086965489503494086371;
+  const dual n = dadd(na, nb);
+  if (n.v <= 0.0) return dc(0.0);
+  const dual ELDAC = ddiv(pw92_clda(na, nb), n);
+  dual za = ddiv(na, n), zb = ddiv(nb, n);
+  if (za.v < DEPS) za = dc(DEPS);
+  if (zb.v < DEPS) zb = dc(DEPS);
+  const dual fi = dscale(dadd(dpowc(za, 2.0 / 3.0), dpowc(zb, 2.0 / 3.0)), 1.0 / pow(2.0, 1.0 / 3.0));
+  const dual fi2 = dmul(fi, fi), fi3 = dmul(fi2, fi);
+  const dual A = ddiv(dc(beta / gamma),
+                      dadd(dsub(dexp(ddiv(dscale(ELDAC, -1.0), dscale(fi3, gamma))), dc(1.0)), dc(DEPS)));
+  const dual t2 = dscale(ddiv(g2, dadd(dmul(fi2, dpowc(n, 7.0 / 3.0)), dc(DEPS))),
+                         PI / (4.0 * 4.0 * pow(3.0 * PI * PI, 1.0 / 3.0)));
+  const dual x = dmul(A, t2);
+  const dual rat = ddiv(dadd(dc(1.0), x), dadd(dadd(dc(1.0), x), dmul(x, x)));
+  const dual H = dmul(dscale(fi3, gamma), dlog(dadd(dc(1.0), dmul(dscale(t2, beta / gamma), rat))));
+  return dmul(n, H);
+}
+
+/* PW92 correlation, unrestricted (pw_ldac iop = PWLDAC_UNRESTR,
+ * :222-245): F, dF/dna, dF/dnb */
+void or_pw92_spin(int n, const double *na, const double *nb, double *f, double *fa, double *fb)
+{
+  for (int i = 0; i < n; ++i) {
+    const dual F = pw92_clda(dv(na[i], 0), dv(nb[i], 1));
+    f[i] = F.v;
+    fa[i] = F.d[0];
+    fb[i] = F.d[1];
+  }
+}
+
+/* PBE correlation, ispin = 2 (pbe_ggcc, :301-316): F, dF/dna, dF/dnb and
+ * dF/dg2 = dF/dgaa = dF/dgbb = dF/dgab / 2 (g2 = gaa + gbb + 2 gab) */
+void or_pbec_spin(int n, const double *na, const double *nb, const double *gaa,
+                  const double *gbb, const double *gab, double *f, double *fa, double *fb,
+                  double *fg2)
+{
+  for (int i = 0; i < n; ++i) {
+    const dual F = pbe_cgga(dv(na[i], 0), dv(nb[i], 1), dv(gaa[i] + gbb[i] + 2.0 * gab[i], 2));
+    f[i] = F.v;
+    fa[i] = F.d[0];
+    fb[i] = F.d[1];
+    fg2[i] = F.d[2];
+  }
+}
+
+/* The functional for ispin = 2 (xc_functionals, xc_func.f90:187-305): per
+ * point f, fr[2] = (dF/dna, dF/dnb), fg[3] = (dF/dgaa, dF/dgbb, dF/dgab).
+ * Exchange by spin scaling (exchange.f90:253-288, :511-601):
+ * F = 1/2 Fx(2 n_s, 4 g_ss) summed over s, dF/dn_s = Fx'(2 n_s),
+ * dF/dg_ss = 2 dFx/dg(2 n_s, 4 g_ss). */
+int or_functional_spin(int mask, int n, const double *na, const double *nb, const double *gaa,
+                       const double *gbb, const double *gab, double *f, double *fr, double *fg)
+{
+  if (mask & ~((1 << XC_XALPHA) | (1 << XC_PWLDAC) | (1 << XC_PBEX) | (1 << XC_PBEC)))
+    return -1;
+  if ((mask & (1 << XC_PBEC)) && !(mask & (1 << XC_PWLDAC))) return -2;
+  for (int i = 0; i < n; ++i) {
+    f[i] = 0;
+    fr[2 * i] = fr[2 * i + 1] = 0;
+    fg[3 * i] = fg[3 * i + 1] = fg[3 * i + 2] = 0;
+  }
+  for (int term = 1; term <= XC_PBEC; ++term) {
+    if (!(mask & (1 << term))) continue;
+    for (int i = 0; i < n; ++i) {
+      double x = 0, xa = 0, xb = 0, ga = 0, gb = 0, gab_ = 0;
+      if (term == XC_XALPHA || term == XC_PBEX) {
+        for (int s = 0; s < 2; ++s) {
+          const double r2 = 2.0 * (s ? nb[i] : na[i]), g4 = 4.0 * (s ? gbb[i] : gaa[i]);
+          double e = 0, er = 0, eg = 0;
+          if (term == XC_XALPHA) or_xalpha(1, &r2, &e, &er);
+          else or_pbex(1, &r2, &g4, &e, &er, &eg);
+          x += 0.5 * e;
+          if (s) { xb = er; gb = 2.0 * eg; } else { xa = er; ga = 2.0 * eg; }
+        }
+      } else if (term == XC_PWLDAC) {
+        or_pw92_spin(1, na + i, nb + i, &x, &xa, &xb);
+      } else {
+        double g2;
+        or_pbec_spin(1, na + i, nb + i, gaa + i, gbb + i, gab + i, &x, &xa, &xb, &g2);
+        ga = g2;
+        gb = g2;
+        gab_ = 2.0 * g2;
+      }
+      f[i] = f[i] + x * 1.0;
+      fr[2 * i] = fr[2 * i] + xa * 1.0;
+      fr[2 * i + 1] = fr[2 * i + 1] + xb * 1.0;
+      fg[3 * i] = fg[3 * i] + ga * 1.0;
+      fg[3 * i + 1] = fg[3 * i + 1] + gb * 1.0;
+      fg[3 * i + 2] = fg[3 * i + 2] + gab_ * 1.0;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* the XC build                                                              */
 /* ------------------------------------------------------------------------ */
 

@@ -4,8 +4,8 @@
 ! library pins the oracle; nothing in the product links it.
 !
 ! Wrapped reference routines:
-!   pw_ldac(nv, iop=1, ...)            modules/pw_ldac_module.f90:222-270
-!   pbe_ggcc("pbe", vla, ispin=1, ...) modules/pbe_ggcxc_module.f90:214-301
+!   pw_ldac(nv, iop=1 or 2, ...)       modules/pw_ldac_module.f90:198-270
+!   pbe_ggcc("pbe", vla, ispin=1 or 2) modules/pbe_ggcxc_module.f90:214-349
 !   solid_harmonics_calculate(_grads)  modules/solid_harmonics_module.f90:369-568
 module pgref_shim
   use iso_c_binding, only: c_int, c_double
@@ -40,6 +40,44 @@ contains
     fr = dr(:, 1)
     fg = dg(:, 1)
   end subroutine pgref_pbec
+
+  subroutine pgref_pw92_spin(n, na, nb, f, fa, fb) bind(C, name="pgref_pw92_spin")
+    use pw_ldac_module, only: pw_ldac
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: na(n), nb(n)
+    real(c_double), intent(out) :: f(n), fa(n), fb(n)
+    real(c_double) :: rr(n, 2), dr(n, 2)
+    rr(:, 1) = na
+    rr(:, 2) = nb
+    f = 0
+    dr = 0
+    call pw_ldac(n, 2, rr, f, dr)
+    fa = dr(:, 1)
+    fb = dr(:, 2)
+  end subroutine pgref_pw92_spin
+
+  subroutine pgref_pbec_spin(n, na, nb, gaa, gbb, gab, f, fa, fb, fgaa, fgbb, fgab) &
+       bind(C, name="pgref_pbec_spin")
+    use pbe_ggcxc_module, only: pbe_ggcc
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: na(n), nb(n), gaa(n), gbb(n), gab(n)
+    real(c_double), intent(out) :: f(n), fa(n), fb(n), fgaa(n), fgbb(n), fgab(n)
+    real(c_double) :: rr(n, 2), gg(n, 3), dr(n, 2), dg(n, 3)
+    rr(:, 1) = na
+    rr(:, 2) = nb
+    gg(:, 1) = gaa
+    gg(:, 2) = gbb
+    gg(:, 3) = gab
+    f = 0
+    dr = 0
+    dg = 0
+    call pbe_ggcc("pbe", n, 2, rr, gg, f, dr, dg)
+    fa = dr(:, 1)
+    fb = dr(:, 2)
+    fgaa = dg(:, 1)
+    fgbb = dg(:, 2)
+    fgab = dg(:, 3)
+  end subroutine pgref_pbec_spin
 
   ! c(npts, (lmax+1)**2), g(npts, 3, (lmax+1)**2); pts(npts, 3) absolute
   subroutine pgref_solid_harmonics(npts, lmax, pts, center, c, g) &
