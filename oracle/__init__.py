@@ -56,11 +56,13 @@ def lib():
         L.or_functional_spin.restype = ctypes.c_int
         L.or_pw92_spin.argtypes = [ctypes.c_int] + [_dp] * 5
         L.or_pbec_spin.argtypes = [ctypes.c_int] + [_dp] * 9
-        L.or_xc_build.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _dp,
-                                  _dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
-                                  ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp,
-                                  ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
-        L.or_xc_build.restype = ctypes.c_int
+        for name in ("or_xc_build", "or_xc_build_spin"):
+            fn = getattr(L, name)
+            fn.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _dp,
+                           _dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp,
+                           ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
+            fn.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -227,25 +229,37 @@ def basis_eval(basis, pts):
 
 
 def xc_build(basis, grid, mask, P=None, C=None, occ=None, veclen=128,
-             batch_range=(0, -1), nthreads=1):
+             batch_range=(0, -1), nthreads=1, ispin=1):
     """Reference-order FP64 XC build.  grid: (npts,4) x,y,z,w.
-    Returns (ham_packed, exc, charge)."""
+    ispin = 2: P is (n, n, 2) or C (n, nocc, 2) with occ (nocc, 2), alpha
+    then beta, and ham holds the two spins interleaved (alpha at even, beta
+    at odd positions).  Returns (ham_packed, exc, charge)."""
     na, axyz, ns, info, ex, nm, ct = _basis_args(basis)
     G = np.asfortranarray(grid, dtype=np.float64)
     npts = G.shape[0]
     n = basis.nbas
+    if ispin not in (1, 2):
+        raise ValueError("ispin must be 1 or 2")
     if P is not None:
-        dform, dens, nocc, oc = 0, np.asfortranarray(P, dtype=np.float64), 0, np.zeros(1)
+        P = np.asarray(P, dtype=np.float64)
+        if P.shape != ((n, n) if ispin == 1 else (n, n, 2)):
+            raise ValueError("P has shape %s" % (P.shape,))
+        dform, dens, nocc, oc = 0, np.asfortranarray(P), 0, np.zeros(2)
     else:
-        dform, dens = 1, np.asfortranarray(C, dtype=np.float64)
-        nocc, oc = dens.shape[1], _c(occ)
-    ham = np.zeros(n * (n + 1) // 2)
+        C = np.asarray(C, dtype=np.float64)
+        if C.ndim != (2 if ispin == 1 else 3) or C.shape[0] != n:
+            raise ValueError("C has shape %s" % (C.shape,))
+        dform, dens = 1, np.asfortranarray(C)
+        nocc = C.shape[1]
+        oc = np.asfortranarray(np.asarray(occ, dtype=np.float64).reshape(C.shape[1:]))
+    ham = np.zeros(ispin * n * (n + 1) // 2)
     e = np.zeros(1)
     q = np.zeros(1)
-    rc = lib().or_xc_build(na, _d(axyz), ns, info.ctypes.data_as(_ip), _d(ex), _d(nm), _d(ct),
-                           G.ctypes.data_as(_dp), npts, veclen, mask, dform,
-                           dens.ctypes.data_as(_dp), nocc, _d(oc), _d(ham), _d(e), _d(q),
-                           batch_range[0], batch_range[1], nthreads)
+    fn = lib().or_xc_build if ispin == 1 else lib().or_xc_build_spin
+    rc = fn(na, _d(axyz), ns, info.ctypes.data_as(_ip), _d(ex), _d(nm), _d(ct),
+            G.ctypes.data_as(_dp), npts, veclen, mask, dform,
+            dens.ctypes.data_as(_dp), nocc, oc.ctypes.data_as(_dp), _d(ham), _d(e), _d(q),
+            batch_range[0], batch_range[1], nthreads)
     if rc != 0:
         raise ValueError("or_xc_build failed: %d" % rc)
     return ham, float(e[0]), float(q[0])

@@ -710,19 +710,22 @@ static void gemm_tn(int vl, int n, const double *W, const double *F, double *H)
 }
 
 /*
- * Full XC build over contiguous batches of <= veclen points.
+ * Full XC build over contiguous batches of <= veclen points, ispin = 1 or 2.
  *   grid     (npts,4) column-major: x, y, z, w   (grdpts layout)
- *   dform    0: dens = P (n x n, column-major);  1: dens = C_occ (n x nocc), occ
- *   ham      packed lower triangle, n(n+1)/2, (j,k<=j) row order; ADDED INTO
+ *   dform    0: dens = P (n x n x ispin, column-major);
+ *            1: dens = C_occ (n x nocc x ispin), occ (nocc x ispin)
+ *   ham      packed lower triangle, n(n+1)/2, (j,k<=j) row order, spins
+ *            interleaved (alpha / beta at even / odd positions,
+ *            modules/xc_hamiltonian.f90:1455-1460); ADDED INTO
  *   batch_lo/hi   process only batches [lo, hi) (for bounded CPU samples)
  *   nthreads  contiguous static split of the batch range over threads
  */
-int or_xc_build(int n_atoms, const double *atom_xyz, int n_shells,
-                const int *shell_info, const double *exps, const double *norms,
-                const double *contr, const double *grid, int64_t npts, int veclen,
-                int mask, int dform, const double *dens, int nocc,
-                const double *occ, double *ham, double *exc, double *charge,
-                int64_t batch_lo, int64_t batch_hi, int nthreads)
+static int xc_build_impl(int ispin, int n_atoms, const double *atom_xyz, int n_shells,
+                         const int *shell_info, const double *exps, const double *norms,
+                         const double *contr, const double *grid, int64_t npts, int veclen,
+                         int mask, int dform, const double *dens, int nocc,
+                         const double *occ, double *ham, double *exc, double *charge,
+                         int64_t batch_lo, int64_t batch_hi, int nthreads)
 {
   or_basis b = {n_atoms, atom_xyz, n_shells, shell_info, exps, norms, contr};
   const int n = or_nbas(n_shells, shell_info);
@@ -734,9 +737,10 @@ int or_xc_build(int n_atoms, const double *atom_xyz, int n_shells,
   if (batch_hi < 0 || batch_hi > nbatch) batch_hi = nbatch;
   if (batch_lo < 0) batch_lo = 0;
   const int ncol = dform == 0 ? n : nocc;
-  const size_t npk = (size_t)n * (n + 1) / 2;
+  const size_t npk = (size_t)n * (n + 1) / 2, nh = (size_t)ispin * npk;
+  if (ispin != 1 && ispin != 2) return -3;
   if (nthreads < 1) nthreads = 1;
-  double *hp = calloc((size_t)nthreads * npk, sizeof(double));
+  double *hp = calloc((size_t)nthreads * nh, sizeof(double));
   double *ep = calloc((size_t)nthreads * 2, sizeof(double));
 #pragma omp parallel num_threads(nthreads)
   {
@@ -756,11 +760,13 @@ int or_xc_build(int n_atoms, const double *atom_xyz, int n_shells,
     double *XG = malloc(sizeof(double) * (size_t)ncol * veclen * 3);
     double *help = malloc(sizeof(double) * (size_t)n * veclen);
     double *H = malloc(sizeof(double) * (size_t)n * n);
-    double *rho = malloc(sizeof(double) * veclen * 12);
-    double *gam = rho + veclen, *gr = rho + 2 * veclen, *f = rho + 5 * veclen;
-    double *fr = rho + 6 * veclen, *fg = rho + 7 * veclen, *A = rho + 8 * veclen;
-    double *B = rho + 9 * veclen;
-    double *hv = hp + (size_t)tid * npk;
+    /* per spin s: rho[s], gr[s][3]; gam[aa, bb, ab]; f; fr[2]; fg[3]; A; B[3];
+       the spin functional's interleaved fr (2 rows) and fg (3 rows) */
+    double *rho = malloc(sizeof(double) * veclen * 26);
+    double *gr = rho + 2 * veclen, *gam = rho + 8 * veclen, *f = rho + 11 * veclen;
+    double *fr = rho + 12 * veclen, *fg = rho + 14 * veclen, *A = rho + 17 * veclen;
+    double *B = rho + 18 * veclen, *fri = rho + 21 * veclen, *fgi = rho + 23 * veclen;
+    double *hv = hp + (size_t)tid * nh;
     double e_loc = 0.0, q_loc = 0.0;
     for (int64_t bt = b0; bt < b1; ++bt) {
       const int64_t p0 = bt * veclen;
@@ -773,80 +779,116 @@ int or_xc_build(int n_atoms, const double *atom_xyz, int n_shells,
           gy[(size_t)fi * vl + p] = grd[((size_t)fi * 3 + 1) * vl + p];
           gz[(size_t)fi * vl + p] = grd[((size_t)fi * 3 + 2) * vl + p];
         }
-      /* density (modules/density_calc_module.f90:1494-1577) */
-      for (int p = 0; p < vl; ++p) { rho[p] = 0; gr[p] = gr[veclen + p] = gr[2 * veclen + p] = 0; }
-      gemm_nn(vl, n, ncol, phi, dens, X);
-      if (dform == 1) {
-        if (gga) {
-          gemm_nn(vl, n, ncol, gx, dens, XG);
-          gemm_nn(vl, n, ncol, gy, dens, XG + (size_t)ncol * vl);
-          gemm_nn(vl, n, ncol, gz, dens, XG + (size_t)2 * ncol * vl);
-        }
-        for (int j = 0; j < nocc; ++j) {
-          const double oc = occ[j];
-          const double *ph = X + (size_t)j * vl;
-          for (int p = 0; p < vl; ++p) rho[p] = rho[p] + ph[p] * ph[p] * oc;
-          if (gga)
-            for (int c = 0; c < 3; ++c) {
-              const double *gp = XG + ((size_t)c * ncol + j) * vl;
-              for (int p = 0; p < vl; ++p)
-                gr[c * veclen + p] = gr[c * veclen + p] + 2 * ph[p] * gp[p] * oc;
-            }
-        }
-      } else {
-        for (int k = 0; k < n; ++k) {
-          const double *xk = X + (size_t)k * vl, *pk = phi + (size_t)k * vl;
-          for (int p = 0; p < vl; ++p) rho[p] += pk[p] * xk[p];
+      /* density per spin (modules/density_calc_module.f90:1494-1577) */
+      for (int sp = 0; sp < ispin; ++sp) {
+        const double *dm = dens + (size_t)sp * n * ncol;
+        const double *oc_s = occ + (size_t)sp * nocc;
+        double *rs = rho + (size_t)sp * veclen, *gs = gr + (size_t)sp * 3 * veclen;
+        for (int p = 0; p < vl; ++p) { rs[p] = 0; gs[p] = gs[veclen + p] = gs[2 * veclen + p] = 0; }
+        gemm_nn(vl, n, ncol, phi, dm, X);
+        if (dform == 1) {
           if (gga) {
-            const double *g0 = gx + (size_t)k * vl, *g1 = gy + (size_t)k * vl,
-                         *g2 = gz + (size_t)k * vl;
-            for (int p = 0; p < vl; ++p) {
-              gr[p] += 2 * g0[p] * xk[p];
-              gr[veclen + p] += 2 * g1[p] * xk[p];
-              gr[2 * veclen + p] += 2 * g2[p] * xk[p];
+            gemm_nn(vl, n, ncol, gx, dm, XG);
+            gemm_nn(vl, n, ncol, gy, dm, XG + (size_t)ncol * vl);
+            gemm_nn(vl, n, ncol, gz, dm, XG + (size_t)2 * ncol * vl);
+          }
+          for (int j = 0; j < nocc; ++j) {
+            const double oc = oc_s[j];
+            const double *ph = X + (size_t)j * vl;
+            for (int p = 0; p < vl; ++p) rs[p] = rs[p] + ph[p] * ph[p] * oc;
+            if (gga)
+              for (int c = 0; c < 3; ++c) {
+                const double *gp = XG + ((size_t)c * ncol + j) * vl;
+                for (int p = 0; p < vl; ++p)
+                  gs[c * veclen + p] = gs[c * veclen + p] + 2 * ph[p] * gp[p] * oc;
+              }
+          }
+        } else {
+          for (int k = 0; k < n; ++k) {
+            const double *xk = X + (size_t)k * vl, *pk = phi + (size_t)k * vl;
+            for (int p = 0; p < vl; ++p) rs[p] += pk[p] * xk[p];
+            if (gga) {
+              const double *g0 = gx + (size_t)k * vl, *g1 = gy + (size_t)k * vl,
+                           *g2 = gz + (size_t)k * vl;
+              for (int p = 0; p < vl; ++p) {
+                gs[p] += 2 * g0[p] * xk[p];
+                gs[veclen + p] += 2 * g1[p] * xk[p];
+                gs[2 * veclen + p] += 2 * g2[p] * xk[p];
+              }
             }
           }
         }
       }
-      for (int p = 0; p < vl; ++p)
-        gam[p] = gr[p] * gr[p] + gr[veclen + p] * gr[veclen + p] +
-                 gr[2 * veclen + p] * gr[2 * veclen + p];
+      /* gamma: |grad rho|^2, or (aa, bb, ab) */
+      for (int g = 0; g < 2 * ispin - 1; ++g) {
+        const double *u = gr + (size_t)(g == 1 ? 3 : 0) * veclen;
+        const double *v = gr + (size_t)(g == 0 ? 0 : 3) * veclen;
+        for (int p = 0; p < vl; ++p)
+          gam[g * veclen + p] = u[p] * v[p] + u[veclen + p] * v[veclen + p] +
+                                u[2 * veclen + p] * v[2 * veclen + p];
+      }
       double qs = 0.0;
-      for (int p = 0; p < vl; ++p) qs += rho[p] * w[p];
+      if (ispin == 1)
+        for (int p = 0; p < vl; ++p) qs += rho[p] * w[p];
+      else
+        for (int p = 0; p < vl; ++p) qs += (rho[p] + rho[veclen + p]) * w[p];
       q_loc = q_loc + qs;
-      or_functional(mask, vl, rho, gam, f, fr, fg);
+      if (ispin == 1) {
+        or_functional(mask, vl, rho, gam, f, fr, fg);
+      } else {
+        or_functional_spin(mask, vl, rho, rho + veclen, gam, gam + veclen, gam + 2 * veclen,
+                           f, fri, fgi);
+        for (int p = 0; p < vl; ++p) {
+          fr[p] = fri[2 * p];
+          fr[veclen + p] = fri[2 * p + 1];
+          for (int g = 0; g < 3; ++g) fg[g * veclen + p] = fgi[3 * p + g];
+        }
+      }
       double es = 0.0;
       for (int p = 0; p < vl; ++p) es += f[p] * w[p];
       e_loc = e_loc + es;
-      if (gga) {
-        /* A_SCAL, B_VECT, orbs_help (modules/xc_hamiltonian.f90:1429-1483) */
-        for (int p = 0; p < vl; ++p) {
-          A[p] = 0.5 * fr[p] * w[p];
-          for (int c = 0; c < 3; ++c) B[c * veclen + p] = w[p] * 2.0 * fg[p] * gr[c * veclen + p];
+      for (int sp = 0; sp < ispin; ++sp) {
+        const double *frs = fr + (size_t)sp * veclen;
+        if (gga) {
+          /* A_SCAL, B_VECT, orbs_help (modules/xc_hamiltonian.f90:1429-1483) */
+          const double *gs = gr + (size_t)sp * 3 * veclen, *go = gr + (size_t)(1 - sp) * 3 * veclen;
+          const double *fss = fg + (size_t)sp * veclen, *fab = fg + 2 * (size_t)veclen;
+          for (int p = 0; p < vl; ++p) {
+            A[p] = 0.5 * frs[p] * w[p];
+            for (int c = 0; c < 3; ++c)
+              B[c * veclen + p] = ispin == 1
+                  ? w[p] * 2.0 * fss[p] * gs[c * veclen + p]
+                  : w[p] * (2.0 * fss[p] * gs[c * veclen + p] + fab[p] * go[c * veclen + p]);
+          }
+          for (int j = 0; j < n; ++j)
+            for (int p = 0; p < vl; ++p) {
+              const size_t q = (size_t)j * vl + p;
+              help[q] = A[p] * phi[q] + B[p] * gx[q] + B[veclen + p] * gy[q] +
+                        B[2 * veclen + p] * gz[q];
+            }
+          gemm_tn(vl, n, help, phi, H);
+          /* remap: ham_xc_arr(counter) += ham(j,k) + ham(k,j), k <= j */
+          size_t cnt = 0;
+          for (int j = 0; j < n; ++j)
+            for (int k = 0; k <= j; ++k, ++cnt) {
+              const size_t o = cnt * ispin + sp;
+              hv[o] = hv[o] + (H[(size_t)j + (size_t)k * n] + H[(size_t)k + (size_t)j * n]);
+            }
+        } else {
+          /* integrate + tristore (modules/xc_hamiltonian.f90:987-1034) */
+          for (int j = 0; j < n; ++j)
+            for (int p = 0; p < vl; ++p) {
+              const size_t q = (size_t)j * vl + p;
+              help[q] = phi[q] * (frs[p] * w[p]);
+            }
+          gemm_tn(vl, n, help, phi, H);
+          size_t cnt = 0;
+          for (int j = 0; j < n; ++j)
+            for (int k = 0; k <= j; ++k, ++cnt) {
+              const size_t o = cnt * ispin + sp;
+              hv[o] = hv[o] + H[(size_t)k + (size_t)j * n];
+            }
         }
-        for (int j = 0; j < n; ++j)
-          for (int p = 0; p < vl; ++p) {
-            const size_t q = (size_t)j * vl + p;
-            help[q] = A[p] * phi[q] + B[p] * gx[q] + B[veclen + p] * gy[q] +
-                      B[2 * veclen + p] * gz[q];
-          }
-        gemm_tn(vl, n, help, phi, H);
-        /* remap: ham_xc_arr(counter) += ham(j,k) + ham(k,j), k <= j */
-        size_t cnt = 0;
-        for (int j = 0; j < n; ++j)
-          for (int k = 0; k <= j; ++k, ++cnt)
-            hv[cnt] = hv[cnt] + (H[(size_t)j + (size_t)k * n] + H[(size_t)k + (size_t)j * n]);
-      } else {
-        /* integrate + tristore (modules/xc_hamiltonian.f90:987-1034) */
-        for (int j = 0; j < n; ++j)
-          for (int p = 0; p < vl; ++p) {
-            const size_t q = (size_t)j * vl + p;
-            help[q] = phi[q] * (fr[p] * w[p]);
-          }
-        gemm_tn(vl, n, help, phi, H);
-        size_t cnt = 0;
-        for (int j = 0; j < n; ++j)
-          for (int k = 0; k <= j; ++k, ++cnt) hv[cnt] = hv[cnt] + H[(size_t)k + (size_t)j * n];
       }
     }
     ep[2 * tid] = e_loc;
@@ -856,12 +898,38 @@ int or_xc_build(int n_atoms, const double *atom_xyz, int n_shells,
   }
   /* the reduction xc_sndrcv would do, in rank order */
   for (int t = 0; t < nthreads; ++t) {
-    const double *hv = hp + (size_t)t * npk;
-    for (size_t q = 0; q < npk; ++q) ham[q] += hv[q];
+    const double *hv = hp + (size_t)t * nh;
+    for (size_t q = 0; q < nh; ++q) ham[q] += hv[q];
     *exc += ep[2 * t];
     *charge += ep[2 * t + 1];
   }
   free(hp);
   free(ep);
   return 0;
+}
+
+int or_xc_build(int n_atoms, const double *atom_xyz, int n_shells,
+                const int *shell_info, const double *exps, const double *norms,
+                const double *contr, const double *grid, int64_t npts, int veclen,
+                int mask, int dform, const double *dens, int nocc,
+                const double *occ, double *ham, double *exc, double *charge,
+                int64_t batch_lo, int64_t batch_hi, int nthreads)
+{
+  return xc_build_impl(1, n_atoms, atom_xyz, n_shells, shell_info, exps, norms, contr, grid,
+                       npts, veclen, mask, dform, dens, nocc, occ, ham, exc, charge, batch_lo,
+                       batch_hi, nthreads);
+}
+
+/* ispin = 2: dens and occ hold the alpha then the beta spin, ham 2 n(n+1)/2
+ * interleaved; exc and charge over both spins */
+int or_xc_build_spin(int n_atoms, const double *atom_xyz, int n_shells,
+                     const int *shell_info, const double *exps, const double *norms,
+                     const double *contr, const double *grid, int64_t npts, int veclen,
+                     int mask, int dform, const double *dens, int nocc,
+                     const double *occ, double *ham, double *exc, double *charge,
+                     int64_t batch_lo, int64_t batch_hi, int nthreads)
+{
+  return xc_build_impl(2, n_atoms, atom_xyz, n_shells, shell_info, exps, norms, contr, grid,
+                       npts, veclen, mask, dform, dens, nocc, occ, ham, exc, charge, batch_lo,
+                       batch_hi, nthreads);
 }
