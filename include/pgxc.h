@@ -143,26 +143,32 @@ int pgxc_create_opt(pgxc_ctx **ctx, int32_t n_atoms, const double *atom_xyz,
 int pgxc_layout(const pgxc_ctx *ctx, int64_t *out8);
 
 /*
- * Density matrix P (nbas x nbas, column-major, symmetric), total density for
- * ispin = 1.  Equivalent to eigvec_occ / occ_num_occ
- * (modules/occupied_levels_module.f90:112-117) through P = C diag(occ) C^T.
- * mode = PGXC_DENSITY_INCREMENTAL makes the next build integrate dP against
- * the state left by the previous build on this context.
+ * Density matrix P (nbas x nbas x ispin, column-major, symmetric): the total
+ * density for ispin = 1; the alpha then the beta spin's for ispin = 2
+ * (rho(vl, ispin) of modules/density_calc_module.f90:1478-1577).  Equivalent
+ * to eigvec_occ / occ_num_occ (modules/occupied_levels_module.f90:112-117)
+ * through P_s = C_s diag(occ_s) C_s^T.  mode = PGXC_DENSITY_INCREMENTAL makes
+ * the next build integrate dP against the state left by the previous build
+ * on this context (same ispin, else PGXC_ERR_STATE); a full density with
+ * another ispin than the last one drops that state.
  */
 int pgxc_set_density(pgxc_ctx *ctx, int32_t ispin, int32_t nbas, const double *P,
                      int32_t mode);
 
-/* Occupied orbitals C (nbas x nocc, column-major) and occupations occ(nocc),
- * exactly the arguments density_calc_nl reads
- * (modules/density_calc_module.f90:1490-1540); P is formed on the device. */
+/* Occupied orbitals C (nbas x nocc x ispin, column-major) and occupations
+ * occ (nocc x ispin), exactly the arguments density_calc_nl reads
+ * (eigvec_occ(irr)%m(N, n_occ, ispin), occ_num_occ(irr)%m(n_occ, ispin),
+ * modules/density_calc_module.f90:1490-1540); P is formed on the device. */
 int pgxc_set_orbitals(pgxc_ctx *ctx, int32_t ispin, int32_t nbas, int32_t nocc,
                       const double *C, const double *occ, int32_t mode);
 
 /*
  * One XC build over all batches.  Replaces the body of calc_xcks_nl (GGA) /
- * calc_xcks (LDA).  ham_packed has nbas*(nbas+1)/2 doubles and receives
+ * calc_xcks (LDA).  ham_packed has ispin*nbas*(nbas+1)/2 doubles and receives
  * ham_xc_arr ADDED INTO (the reference accumulates into the zeroed array,
- * modules/xc_hamiltonian.f90:1561-1567); packing (j = 1..n, k = 1..j).
+ * modules/xc_hamiltonian.f90:1561-1567); packing (j = 1..n, k = 1..j), for
+ * ispin = 2 the alpha and beta spins interleaved at even / odd positions
+ * (counter = s, step ispin, :1455-1460).
  * exc and charge are also added into (exc_int, charge_int, :1408,1423).
  * Any of ham_packed / exc / charge may be NULL to leave the result on the
  * device (pgxc_get_result fetches it).
@@ -265,6 +271,9 @@ int pgxc_batch_set_schedule(pgxc_batch *batch, int32_t mode, pgxc_costmodel *mod
  * the record.  pgxc_save_state: needs a build on the context.
  * pgxc_load_state: only into a context of the same atoms, basis, grid and
  * layout (checked by a hash recorded in the state); PGXC_ERR_ARG otherwise.
+ * The record holds every spin of the build (ispin = 2: twice the size) and
+ * loading it sets the context's ispin, which the next (incremental) density
+ * must then use.
  */
 int64_t pgxc_state_bytes(const pgxc_ctx *ctx);
 int pgxc_save_state(pgxc_ctx *ctx, void *buf, int64_t bytes);

@@ -84,6 +84,16 @@ struct pgxc_ctx {
   int split_valid = 0;       // the FP32 cache holds its split-BF16 planes too
   int run_split = 0;         // the current build writes the split planes
   double *part = nullptr, *coef = nullptr, *Hpart = nullptr;
+  // open shell (ispin = 2): the fields above and below serve the alpha spin,
+  // these the beta spin (allocated with the first open-shell density; the
+  // basis values and the work tables are shared)
+  int ispin = 1;
+  struct Beta {
+    double *P = nullptr, *Pref = nullptr, *part = nullptr, *coef = nullptr, *Hpart = nullptr;
+    double *state = nullptr, *Vstate = nullptr, *Vout = nullptr;
+    void *D = nullptr, *W = nullptr;
+    size_t w_bytes = 0;
+  } beta;
   // device: state and results
   double *state = nullptr;                    // [S_NSLOT][npad]
   double *Vstate = nullptr, *Vout = nullptr;  // packed
@@ -156,7 +166,7 @@ constexpr int kLists = 3 * kNumVariants;
 
 struct StepPlan {
   int nunits = 0;
-  StageList basis, func, wmat, split;
+  StageList basis, func, func2, wmat, split;  // func2: open-shell units
   StageList dens[kLists], pot[kLists];
 };
 
@@ -179,6 +189,7 @@ struct Plan {
   std::vector<StepPlan> steps;
   int nlanes = 1;  // 2: chunk k runs on lane k % 2 (two streams)
   std::vector<int> lanes;  // lanes in use per molecule when the plan was made
+  std::vector<int> spins;  // ispin per molecule when the plan was made
   PackD *d_pack = nullptr;
   long *d_pack_base = nullptr, *d_prep_base = nullptr, *d_fold_base = nullptr;
   long pack_total = 0, prep_total = 0, fold_total = 0;
@@ -292,10 +303,14 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
                     const std::vector<int> &incr, const std::vector<double> &unit_prio,
                     long order_version, const Sched &sc, Plan *&plan, pgxc_ctx *errc) {
   const int smode = sc.model ? sc.mode : std::min(sc.mode, (int)PGXC_SCHED_ANALYTIC);
-  std::vector<int> lanes;
-  for (auto *c : mols) lanes.push_back(c->nlanes);
+  std::vector<int> lanes, spins;
+  for (auto *c : mols) {
+    lanes.push_back(c->nlanes);
+    spins.push_back(c->ispin);
+  }
   if (plan && plan->functional == functional && plan->precision == precision &&
-      plan->incr == incr && plan->lanes == lanes && plan->order_version == order_version &&
+      plan->incr == incr && plan->lanes == lanes && plan->spins == spins &&
+      plan->order_version == order_version &&
       plan->sched_mode == smode &&
       plan->log == sc.log && plan->model == sc.model &&
       plan->model_version == costmodel_version(sc.model))
@@ -306,6 +321,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   plan->precision = precision;
   plan->incr = incr;
   plan->lanes = lanes;
+  plan->spins = spins;
   plan->order_version = order_version;
   plan->sched_mode = smode;
   plan->log = sc.log;
@@ -320,7 +336,6 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   }
   const bool queues = smode != PGXC_SCHED_STATIC;
   const int nm = (int)mols.size();
-  plan->nmol = nm;
   int nsteps = 0;
   for (auto *c : mols) nsteps = std::max(nsteps, c->nchunks);
 
@@ -372,10 +387,11 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     }
     return P;
   };
-  std::vector<std::vector<Pending>> pend(nsteps);  // basis, func, wmat, dens, pot
+  // per step: basis, func, wmat, dens lists, pot lists, split, func2
+  std::vector<std::vector<Pending>> pend(nsteps);
   for (int k = 0; k < nsteps; ++k) {
     std::vector<UnitD> units;
-    std::vector<int> umol;
+    std::vector<int> umol, uspin;
     for (int m = 0; m < nm; ++m) {
       pgxc_ctx *c = mols[m];
       if (k >= c->nchunks) continue;
@@ -426,20 +442,54 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.gcf = c->gcf;
       U.sig8 = c->sig8 + (p0 / kGroup) * c->nw8;
       U.wbits = c->wbits + (p0 / kGroup) * c->nw8;
-      units.push_back(U);
-      umol.push_back(m);
+      if (c->ispin == 2) {  // the beta spin's unit: same points and basis values
+        UnitD B = U;
+        B.W = (char *)c->beta.W + (size_t)lane * c->w_lane;
+        B.D = c->beta.D;
+        B.part = c->beta.part + (size_t)lane * c->ntiles * 4 * c->Mc;
+        B.coef = c->beta.coef + (size_t)lane * 4 * c->Mc;
+        B.state = c->beta.state + p0;
+        B.Hpart = c->beta.Hpart + (size_t)lane * c->nsplit * c->NP * c->NP;
+        U.part2 = B.part;
+        U.coef2 = B.coef;
+        U.state2 = B.state;
+        units.push_back(U);
+        umol.push_back(m);
+        uspin.push_back(0);
+        units.push_back(B);
+        umol.push_back(m);
+        uspin.push_back(1);
+      } else {
+        units.push_back(U);
+        umol.push_back(m);
+        uspin.push_back(0);
+      }
     }
+    // the units of one spin (0: closed-shell and alpha units, 1: beta)
+    // meeting a condition on the molecule
+    auto pick = [&](int spin, auto cond, std::vector<UnitD> &out, std::vector<int> &om) {
+      for (size_t i = 0; i < units.size(); ++i)
+        if (uspin[i] == spin && cond(mols[umol[i]])) {
+          out.push_back(units[i]);
+          om.push_back(umol[i]);
+        }
+    };
     {  // molecules whose basis values are cached skip the basis stage
       std::vector<UnitD> bu;
       std::vector<int> bm;
       for (size_t i = 0; i < units.size(); ++i)
-        if (!mols[umol[i]]->skip_basis) {
+        if (uspin[i] == 0 && !mols[umol[i]]->skip_basis) {
           bu.push_back(units[i]);
           bm.push_back(umol[i]);
         }
       pend[k].push_back(stage(bu, bm, K_BASIS, 0, k));
     }
-    pend[k].push_back(stage(units, umol, K_FUNC, 0, k));
+    {
+      std::vector<UnitD> fu;
+      std::vector<int> fm;
+      pick(0, [](const pgxc_ctx *c) { return c->ispin == 1; }, fu, fm);
+      pend[k].push_back(stage(fu, fm, K_FUNC, 0, k));
+    }
     pend[k].push_back(stage(units, umol, K_WMAT, 0, k));
     const int cls = precision == PGXC_PREC_FP64 ? 0 : 1;
     for (int kind = 0; kind < 2; ++kind)
@@ -459,13 +509,14 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     {  // split-BF16 operand planes of the basis values, unless already cached
       std::vector<UnitD> su;
       std::vector<int> sm;
-      if (precision == PGXC_PREC_BF16X3)
-        for (size_t i = 0; i < units.size(); ++i)
-          if (mols[umol[i]]->run_split) {
-            su.push_back(units[i]);
-            sm.push_back(umol[i]);
-          }
+      if (precision == PGXC_PREC_BF16X3) pick(0, [](const pgxc_ctx *c) { return c->run_split != 0; }, su, sm);
       pend[k].push_back(stage(su, sm, K_SPLIT, 0, k));
+    }
+    {  // spin functional over the alpha units of open-shell molecules
+      std::vector<UnitD> fu;
+      std::vector<int> fm;
+      pick(0, [](const pgxc_ctx *c) { return c->ispin == 2; }, fu, fm);
+      pend[k].push_back(stage(fu, fm, K_FUNC, 0, k));
     }
   }
   // queue bounds: each list's positions (natural order, so neighbouring items
@@ -520,33 +571,40 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       }
     plan_ctr_off_tmp = ctr0;
   }
-  // per-molecule operand preparation and packing
-  std::vector<PackD> packs(nm);
-  std::vector<long> pbase(nm + 1, 0), qbase(nm + 1, 0), fbase(nm + 1, 0);
+  // operand preparation and packing per molecule and spin
+  std::vector<PackD> packs;
   for (int m = 0; m < nm; ++m) {
     pgxc_ctx *c = mols[m];
-    PackD &Q = packs[m];
-    Q.Hpart = c->Hpart;
-    Q.Vout = c->Vout;
-    Q.Vstate = c->Vstate;
-    Q.P = c->P;
-    Q.Pref = c->Pref;
-    Q.D = c->D;
-    Q.fpos = c->d_fpos;
-    Q.n = c->nbas;
-    Q.NP = c->NP;
-    Q.nsplit = c->nsplit * c->nlanes;  // the fold sums the partials of every lane
-    Q.incremental = incr[m];
-    const long npk = (long)c->nbas * (c->nbas + 1) / 2;
-    pbase[m + 1] = pbase[m] + (npk + kThreads - 1) / kThreads;
-    qbase[m + 1] = qbase[m] + ((long)c->NP * c->NP + kThreads - 1) / kThreads;
-    const long nb = c->NP / 32;
-    fbase[m + 1] = fbase[m] + nb * (nb + 1) / 2;
+    for (int sp = 0; sp < c->ispin; ++sp) {
+      PackD Q;
+      memset(&Q, 0, sizeof(Q));
+      Q.Hpart = sp ? c->beta.Hpart : c->Hpart;
+      Q.Vout = sp ? c->beta.Vout : c->Vout;
+      Q.Vstate = sp ? c->beta.Vstate : c->Vstate;
+      Q.P = sp ? c->beta.P : c->P;
+      Q.Pref = sp ? c->beta.Pref : c->Pref;
+      Q.D = sp ? c->beta.D : c->D;
+      Q.fpos = c->d_fpos;
+      Q.n = c->nbas;
+      Q.NP = c->NP;
+      Q.nsplit = c->nsplit * c->nlanes;  // the fold sums the partials of every lane
+      Q.incremental = incr[m];
+      packs.push_back(Q);
+      plan->hzero.push_back({(int *)Q.Hpart, 2L * c->nlanes * c->nsplit * c->NP * c->NP});
+    }
   }
-  for (int m = 0; m < nm; ++m)
-    plan->hzero.push_back({(int *)mols[m]->Hpart,
-                           2L * mols[m]->nlanes * mols[m]->nsplit * mols[m]->NP * mols[m]->NP});
-  const size_t pack_off = put(packs.data(), sizeof(PackD) * nm);
+  const int npack = (int)packs.size();
+  std::vector<long> pbase(npack + 1, 0), qbase(npack + 1, 0), fbase(npack + 1, 0);
+  for (int q = 0; q < npack; ++q) {
+    const PackD &Q = packs[q];
+    const long npk = Q.n * (Q.n + 1) / 2;
+    pbase[q + 1] = pbase[q] + (npk + kThreads - 1) / kThreads;
+    qbase[q + 1] = qbase[q] + (Q.NP * Q.NP + kThreads - 1) / kThreads;
+    const long nb = Q.NP / 32;
+    fbase[q + 1] = fbase[q] + nb * (nb + 1) / 2;
+  }
+  plan->nmol = npack;
+  const size_t pack_off = put(packs.data(), sizeof(PackD) * npack);
   const size_t pb_off = put(pbase.data(), sizeof(long) * pbase.size());
   const size_t qb_off = put(qbase.data(), sizeof(long) * qbase.size());
   const size_t fb_off = put(fbase.data(), sizeof(long) * fbase.size());
@@ -576,6 +634,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     for (int li = 0; li < kLists; ++li) S.dens[li] = mk(pend[k][3 + li]);
     for (int li = 0; li < kLists; ++li) S.pot[li] = mk(pend[k][3 + kLists + li]);
     S.split = mk(pend[k][3 + 2 * kLists]);
+    S.func2 = mk(pend[k][4 + 2 * kLists]);
     S.nunits = pend[k][0].nu;
   }
   for (int m = 0; m < nm; ++m) plan->nlanes = std::max(plan->nlanes, mols[m]->nlanes);
@@ -583,9 +642,9 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   plan->d_pack_base = (long *)(B + pb_off);
   plan->d_prep_base = (long *)(B + qb_off);
   plan->d_fold_base = (long *)(B + fb_off);
-  plan->fold_total = fbase[nm];
-  plan->pack_total = pbase[nm];
-  plan->prep_total = qbase[nm];
+  plan->fold_total = fbase[npack];
+  plan->pack_total = pbase[npack];
+  plan->prep_total = qbase[npack];
   return PGXC_OK;
 }
 
@@ -660,6 +719,7 @@ static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent
     }
     if (E) (void)hipEventRecord(E[2], t);
     launch_functional(t, gga, S.func);
+    launch_functional_spin(t, gga, S.func2);
     launch_wmat<T>(t, gga, x3, S.wmat);
     if (E) (void)hipEventRecord(E[3], t);
     for (int li = 0; li < kLists; ++li) {
@@ -704,6 +764,14 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
     c->w_bytes = 0;
     HIPCHK(c, hipMalloc(&c->W, need_w));
     c->w_bytes = need_w;
+    moved = true;
+  }
+  if (c->ispin == 2 && c->beta.w_bytes < need_w) {
+    dfree(c->beta.W);
+    c->beta.W = nullptr;
+    c->beta.w_bytes = 0;
+    HIPCHK(c, hipMalloc(&c->beta.W, need_w));
+    c->beta.w_bytes = need_w;
     moved = true;
   }
   // basis cache: every chunk's values, when they take at most a quarter of
@@ -795,7 +863,7 @@ static void set_stats(pgxc_ctx *c) {
   c->stats[1] = std::ceil(B / c->veclen);
   // sum over 128-point tiles of 4 B n_b^2 + 16 B n_b, n_b = functions of the
   // shells not cut off on the tile (dense: n_b = nbas)
-  c->stats[2] = c->screen ? c->flops_screened : c->flops_dense;
+  c->stats[2] = (c->screen ? c->flops_screened : c->flops_dense) * c->ispin;
   c->stats[3] = B * c->exps_per_point;
   c->stats[4] = 32.0 * B + 16.0 * n * n;
 }
@@ -1247,17 +1315,55 @@ int pgxc_layout(const pgxc_ctx *c, int64_t *out8) {
 
 }  // extern "C"
 
+// Buffers of the beta spin, zeroed like the alpha spin's at creation.
+static int alloc_beta(pgxc_ctx *c) {
+  auto &B = c->beta;
+  if (B.P) return PGXC_OK;
+  const size_t np2 = (size_t)c->NP * c->NP, npk = (size_t)c->nbas * (c->nbas + 1) / 2;
+  const size_t L = (size_t)c->nlanes_cap;
+  int rc;
+  if ((rc = dalloc(c, &B.P, np2)) || (rc = dalloc(c, &B.Pref, np2))) return rc;
+  HIPCHK(c, hipMalloc(&B.D, np2 * 8));
+  if ((rc = dalloc(c, &B.state, (size_t)S_NSLOT * c->npad)) || (rc = dalloc(c, &B.Vstate, npk)) ||
+      (rc = dalloc(c, &B.Vout, npk)) || (rc = dalloc(c, &B.part, L * c->ntiles * 4 * c->Mc)) ||
+      (rc = dalloc(c, &B.coef, L * 4 * c->Mc)) || (rc = dalloc(c, &B.Hpart, L * c->nsplit * np2)))
+    return rc;
+  HIPCHK(c, hipMemset(B.P, 0, np2 * 8));
+  HIPCHK(c, hipMemset(B.Pref, 0, np2 * 8));
+  HIPCHK(c, hipMemset(B.state, 0, (size_t)S_NSLOT * c->npad * 8));
+  HIPCHK(c, hipMemset(B.part, 0, L * c->ntiles * 4 * c->Mc * 8));
+  HIPCHK(c, hipMemset(B.coef, 0, L * 4 * c->Mc * 8));
+  return PGXC_OK;
+}
+
+// Switch the context to ispin spins: a build state of the other spin count
+// is dropped, and the plan (whose units depend on it) rebuilt.
+static int set_spins(pgxc_ctx *c, int ispin) {
+  if (ispin == c->ispin) return PGXC_OK;
+  if (ispin == 2) {
+    int rc = alloc_beta(c);
+    if (rc) return rc;
+  }
+  c->ispin = ispin;
+  c->have_state = 0;
+  c->have_density = 0;
+  free_plan(c->plan);
+  return PGXC_OK;
+}
+
 static int check_density_args(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32_t mode) {
   if (!c) return PGXC_ERR_ARG;
-  if (ispin != 1)
-    return fail(c, PGXC_ERR_UNSUPPORTED, "ispin = " + std::to_string(ispin) +
-                                             ": only the closed-shell build (ispin = 1) is implemented");
+  if (ispin != 1 && ispin != 2)
+    return fail(c, PGXC_ERR_ARG, "ispin = " + std::to_string(ispin) + ": must be 1 or 2");
   if (nbas != c->nbas)
     return fail(c, PGXC_ERR_ARG, "nbas = " + std::to_string(nbas) + " but the basis has " +
                                      std::to_string(c->nbas) + " functions");
   if (mode != PGXC_DENSITY_FULL && mode != PGXC_DENSITY_INCREMENTAL)
     return fail(c, PGXC_ERR_ARG, "unknown density mode " + std::to_string(mode));
-  return PGXC_OK;
+  if (mode == PGXC_DENSITY_INCREMENTAL && c->have_state && ispin != c->ispin)
+    return fail(c, PGXC_ERR_STATE, "incremental density with ispin = " + std::to_string(ispin) +
+                                       " after a build with ispin = " + std::to_string(c->ispin));
+  return set_spins(c, ispin);
 }
 
 extern "C" int pgxc_set_density(pgxc_ctx *c, int32_t ispin, int32_t nbas, const double *P,
@@ -1268,10 +1374,12 @@ extern "C" int pgxc_set_density(pgxc_ctx *c, int32_t ispin, int32_t nbas, const 
   const int n = c->nbas, NP = c->NP;
   std::vector<double> h((size_t)NP * NP, 0.0);
   const int *pos = c->fpos.data();
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) h[(size_t)pos[i] * NP + pos[j]] = P[(size_t)i + (size_t)j * n];
-  HIPCHK(c, hipMemcpyAsync(c->P, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int sp = 0; sp < ispin; ++sp) {
+    const double *Ps = P + (size_t)sp * n * n;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) h[(size_t)pos[i] * NP + pos[j]] = Ps[(size_t)i + (size_t)j * n];
+    HIPCHK(c, hipMemcpy(sp ? c->beta.P : c->P, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  }
   c->have_density = 1;
   c->density_mode = mode;
   return PGXC_OK;
@@ -1285,16 +1393,21 @@ extern "C" int pgxc_set_orbitals(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32
     return fail(c, PGXC_ERR_ARG, "pgxc_set_orbitals: bad nocc or null C/occ");
   const int n = c->nbas;
   double *dC = nullptr, *docc = nullptr;
-  HIPCHK(c, hipMemsetAsync(c->P, 0, (size_t)c->NP * c->NP * 8, c->stream));
+  for (int sp = 0; sp < ispin; ++sp)
+    HIPCHK(c, hipMemsetAsync(sp ? c->beta.P : c->P, 0, (size_t)c->NP * c->NP * 8, c->stream));
   if (nocc > 0) {
-    HIPCHK(c, hipMalloc((void **)&dC, (size_t)n * nocc * 8));
-    HIPCHK(c, hipMalloc((void **)&docc, (size_t)nocc * 8));
-    std::vector<double> Cp((size_t)n * nocc);  // rows in the internal function order
-    for (int k = 0; k < nocc; ++k)
-      for (int i = 0; i < n; ++i) Cp[(size_t)c->fpos[i] + (size_t)k * n] = C[(size_t)i + (size_t)k * n];
-    HIPCHK(c, hipMemcpy(dC, Cp.data(), (size_t)n * nocc * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpyAsync(docc, occ, (size_t)nocc * 8, hipMemcpyHostToDevice, c->stream));
-    launch_form_density(c->stream, dC, docc, n, nocc, c->NP, c->P);
+    HIPCHK(c, hipMalloc((void **)&dC, (size_t)ispin * n * nocc * 8));
+    HIPCHK(c, hipMalloc((void **)&docc, (size_t)ispin * nocc * 8));
+    std::vector<double> Cp((size_t)ispin * n * nocc);  // rows in the internal function order
+    for (int sp = 0; sp < ispin; ++sp)
+      for (int k = 0; k < nocc; ++k)
+        for (int i = 0; i < n; ++i)
+          Cp[(size_t)c->fpos[i] + ((size_t)sp * nocc + k) * n] = C[(size_t)i + ((size_t)sp * nocc + k) * n];
+    HIPCHK(c, hipMemcpy(dC, Cp.data(), Cp.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(docc, occ, (size_t)ispin * nocc * 8, hipMemcpyHostToDevice));
+    for (int sp = 0; sp < ispin; ++sp)
+      launch_form_density(c->stream, dC + (size_t)sp * n * nocc, docc + (size_t)sp * nocc, n, nocc,
+                          c->NP, sp ? c->beta.P : c->P);
     HIPCHK(c, hipGetLastError());
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1353,11 +1466,13 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
 int pgxc_get_result(pgxc_ctx *c, double *ham, double *exc, double *charge) {
   if (!c) return PGXC_ERR_ARG;
   if (!c->have_state) return fail(c, PGXC_ERR_STATE, "no build has run on this context");
-  if (ham) {
-    const size_t npk = (size_t)c->nbas * (c->nbas + 1) / 2;
+  if (ham) {  // ispin = 2: alpha / beta at even / odd positions
+    const size_t npk = (size_t)c->nbas * (c->nbas + 1) / 2, is = (size_t)c->ispin;
     std::vector<double> v(npk);
-    HIPCHK(c, hipMemcpy(v.data(), c->Vout, npk * 8, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < npk; ++i) ham[i] += v[i];
+    for (size_t sp = 0; sp < is; ++sp) {
+      HIPCHK(c, hipMemcpy(v.data(), sp ? c->beta.Vout : c->Vout, npk * 8, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < npk; ++i) ham[i * is + sp] += v[i];
+    }
   }
   if (exc) *exc += c->last_exc;
   if (charge) *charge += c->last_charge;
@@ -1391,15 +1506,19 @@ struct StateHeader {
   uint64_t geom;
   int64_t nbas, NP, npad, npk, nslot;
   uint32_t functional;
-  int32_t reserved;
+  int32_t ispin;
   double exc, charge;
 };
 static const char kStateMagic[8] = {'P', 'G', 'X', 'C', 'S', 'T', '0', '1'};
 
-static int64_t state_bytes(const pgxc_ctx *c) {
+// one block per spin: Pref, Vstate, state
+static int64_t state_block(const pgxc_ctx *c) {
   const int64_t npk = (int64_t)c->nbas * (c->nbas + 1) / 2;
-  return (int64_t)sizeof(StateHeader) +
-         8 * ((int64_t)c->NP * c->NP + npk + (int64_t)S_NSLOT * c->npad);
+  return 8 * ((int64_t)c->NP * c->NP + npk + (int64_t)S_NSLOT * c->npad);
+}
+
+static int64_t state_bytes(const pgxc_ctx *c) {
+  return (int64_t)sizeof(StateHeader) + c->ispin * state_block(c);
 }
 
 int64_t pgxc_state_bytes(const pgxc_ctx *c) { return c ? state_bytes(c) : -PGXC_ERR_ARG; }
@@ -1418,16 +1537,21 @@ int pgxc_save_state(pgxc_ctx *c, void *buf, int64_t bytes) {
   h.npk = (int64_t)c->nbas * (c->nbas + 1) / 2;
   h.nslot = S_NSLOT;
   h.functional = c->state_functional;
+  h.ispin = c->ispin;
   h.exc = c->last_exc;
   h.charge = c->last_charge;
   char *o = (char *)buf;
   memcpy(o, &h, sizeof(h));
   o += sizeof(h);
-  HIPCHK(c, hipMemcpy(o, c->Pref, 8 * (size_t)c->NP * c->NP, hipMemcpyDeviceToHost));
-  o += 8 * (size_t)c->NP * c->NP;
-  HIPCHK(c, hipMemcpy(o, c->Vstate, 8 * (size_t)h.npk, hipMemcpyDeviceToHost));
-  o += 8 * (size_t)h.npk;
-  HIPCHK(c, hipMemcpy(o, c->state, 8 * (size_t)S_NSLOT * c->npad, hipMemcpyDeviceToHost));
+  for (int sp = 0; sp < c->ispin; ++sp) {
+    HIPCHK(c, hipMemcpy(o, sp ? c->beta.Pref : c->Pref, 8 * (size_t)c->NP * c->NP, hipMemcpyDeviceToHost));
+    o += 8 * (size_t)c->NP * c->NP;
+    HIPCHK(c, hipMemcpy(o, sp ? c->beta.Vstate : c->Vstate, 8 * (size_t)h.npk, hipMemcpyDeviceToHost));
+    o += 8 * (size_t)h.npk;
+    HIPCHK(c, hipMemcpy(o, sp ? c->beta.state : c->state, 8 * (size_t)S_NSLOT * c->npad,
+                        hipMemcpyDeviceToHost));
+    o += 8 * (size_t)S_NSLOT * c->npad;
+  }
   return PGXC_OK;
 }
 
@@ -1438,16 +1562,26 @@ int pgxc_load_state(pgxc_ctx *c, const void *buf, int64_t bytes) {
   memcpy(&h, buf, sizeof(h));
   if (memcmp(h.magic, kStateMagic, 8) != 0) return fail(c, PGXC_ERR_ARG, "pgxc_load_state: not a state record");
   if (h.geom != c->geom || h.nbas != c->nbas || h.NP != c->NP || h.npad != c->npad ||
-      h.nslot != S_NSLOT || h.npk != (int64_t)c->nbas * (c->nbas + 1) / 2)
+      h.nslot != S_NSLOT || h.npk != (int64_t)c->nbas * (c->nbas + 1) / 2 ||
+      (h.ispin != 1 && h.ispin != 2))
     return fail(c, PGXC_ERR_ARG, "pgxc_load_state: the state belongs to another geometry or layout");
-  if (bytes < state_bytes(c)) return fail(c, PGXC_ERR_ARG, "pgxc_load_state: buffer too short");
+  if (bytes < (int64_t)sizeof(StateHeader) + h.ispin * state_block(c))
+    return fail(c, PGXC_ERR_ARG, "pgxc_load_state: buffer too short");
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  {  // the next density must be set with the state's spin count
+    int rc = set_spins(c, h.ispin);
+    if (rc) return rc;
+  }
   const char *o = (const char *)buf + sizeof(h);
-  HIPCHK(c, hipMemcpy(c->Pref, o, 8 * (size_t)c->NP * c->NP, hipMemcpyHostToDevice));
-  o += 8 * (size_t)c->NP * c->NP;
-  HIPCHK(c, hipMemcpy(c->Vstate, o, 8 * (size_t)h.npk, hipMemcpyHostToDevice));
-  o += 8 * (size_t)h.npk;
-  HIPCHK(c, hipMemcpy(c->state, o, 8 * (size_t)S_NSLOT * c->npad, hipMemcpyHostToDevice));
+  for (int sp = 0; sp < c->ispin; ++sp) {
+    HIPCHK(c, hipMemcpy(sp ? c->beta.Pref : c->Pref, o, 8 * (size_t)c->NP * c->NP, hipMemcpyHostToDevice));
+    o += 8 * (size_t)c->NP * c->NP;
+    HIPCHK(c, hipMemcpy(sp ? c->beta.Vstate : c->Vstate, o, 8 * (size_t)h.npk, hipMemcpyHostToDevice));
+    o += 8 * (size_t)h.npk;
+    HIPCHK(c, hipMemcpy(sp ? c->beta.state : c->state, o, 8 * (size_t)S_NSLOT * c->npad,
+                        hipMemcpyHostToDevice));
+    o += 8 * (size_t)S_NSLOT * c->npad;
+  }
   c->state_functional = h.functional;
   c->last_exc = h.exc;
   c->last_charge = h.charge;
@@ -1486,7 +1620,9 @@ int pgxc_destroy(pgxc_ctx *c) {
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
                   c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->su_off, c->su,
-                  c->gcf_off, c->gcf, c->di, c->pi};
+                  c->gcf_off, c->gcf, c->di, c->pi, c->beta.P, c->beta.Pref, c->beta.D,
+                  c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
+                  c->beta.Vstate, c->beta.Vout};
   for (void *b : bufs) dfree(b);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1580,14 +1716,14 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
   bool moved = false;
   for (int m = 0; m < nm; ++m) {
     pgxc_ctx *c = b->mols[m];
-    const void *phi0 = c->phi_base, *w0 = c->W;
+    const void *phi0 = c->phi_base, *w0 = c->W, *wb0 = c->beta.W;
     const int skip0 = c->skip_basis, split0 = c->run_split;
     int rc = ensure_work(c, elem, precision, s);
     if (rc) {
       b->err = "molecule " + std::to_string(m) + ": " + c->err;
       return rc;
     }
-    moved = moved || phi0 != c->phi_base || w0 != c->W || skip0 != c->skip_basis ||
+    moved = moved || phi0 != c->phi_base || w0 != c->W || wb0 != c->beta.W || skip0 != c->skip_basis ||
             split0 != c->run_split;
   }
   if (moved) free_plan(b->plan);

@@ -46,6 +46,11 @@ struct UnitD {
   double *coef;     // [4][Mc] potential coefficients a, b
   double *state;    // [S_NSLOT][npad] molecule state, offset to the unit's first point
   double *epart;    // [Mc/256][2] block sums of w f and w rho (offset to the unit)
+  // open-shell (ispin = 2) molecules: the alpha spin's unit also points at
+  // the beta spin's partials, coefficients and state (the spin functional
+  // reads both); nullptr otherwise
+  const double *part2;
+  double *coef2, *state2;
   double *Hpart;    // [nsplit][NP][NP] potential partials of the molecule
   // Work of the contractions, over gathered runs of 8 functions (all runs
   // when screening is off).  ditems / pitems: the unit's density items
@@ -112,6 +117,7 @@ constexpr int kNumVariants = 3;
 template <typename T>
 void launch_density(hipStream_t s, bool grad, int tf, int var, const StageList &L);
 void launch_functional(hipStream_t s, bool gga, const StageList &L);
+void launch_functional_spin(hipStream_t s, bool gga, const StageList &L);
 template <typename T>
 void launch_wmat(hipStream_t s, bool grad, bool split, const StageList &L);
 // split-BF16 operand planes of the basis values (item = 32 functions x 64 points)

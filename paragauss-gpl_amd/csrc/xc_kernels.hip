@@ -700,6 +700,226 @@ __global__ __launch_bounds__(kThreads) void k_functional(const UnitD *__restrict
   }
 }
 
+// ------------------------------------------------------- functional, ispin = 2
+// Spin-polarised correlation on values with first derivatives in
+// (n_a, n_b, g2): PW92 c_lda (modules/pw_ldac_module.f90:127-195) and PBE
+// c_gga (modules/pbe_ggcxc_module.f90:96-140); exchange by spin scaling
+// (modules/exchange.f90:253-288, :511-601) through the closed-shell terms.
+namespace xcs {
+using xcf::PI;
+using xcf::DEPS;
+struct Dv {
+  double v, d0, d1, d2;
+};
+__device__ __forceinline__ Dv cst(double v) { return {v, 0.0, 0.0, 0.0}; }
+__device__ __forceinline__ Dv operator+(Dv a, Dv b) { return {a.v + b.v, a.d0 + b.d0, a.d1 + b.d1, a.d2 + b.d2}; }
+__device__ __forceinline__ Dv operator-(Dv a, Dv b) { return {a.v - b.v, a.d0 - b.d0, a.d1 - b.d1, a.d2 - b.d2}; }
+__device__ __forceinline__ Dv operator*(Dv a, Dv b) {
+  return {a.v * b.v, a.d0 * b.v + a.v * b.d0, a.d1 * b.v + a.v * b.d1, a.d2 * b.v + a.v * b.d2};
+}
+__device__ __forceinline__ Dv operator*(Dv a, double c) { return {a.v * c, a.d0 * c, a.d1 * c, a.d2 * c}; }
+__device__ __forceinline__ Dv operator/(Dv a, Dv b) {
+  const double q = a.v / b.v, r = 1.0 / b.v;
+  return {q, (a.d0 - q * b.d0) * r, (a.d1 - q * b.d1) * r, (a.d2 - q * b.d2) * r};
+}
+__device__ __forceinline__ Dv chain(Dv a, double v, double g) { return {v, g * a.d0, g * a.d1, g * a.d2}; }
+__device__ __forceinline__ Dv powc(Dv a, double c) { return chain(a, pow(a.v, c), c * pow(a.v, c - 1.0)); }
+__device__ __forceinline__ Dv logd(Dv a) { return chain(a, log(a.v), 1.0 / a.v); }
+__device__ __forceinline__ Dv expd(Dv a) {
+  const double e = exp(a.v);
+  return chain(a, e, e);
+}
+
+// PW92 G(rs) for the parameter sets of eps_c(zeta = 0), eps_c(zeta = 1) and
+// -alpha_c (p = 1)
+__device__ __forceinline__ Dv pw92_gs(Dv rs, int set) {
+  const double A[3] = {0.031091, 0.015545, 0.016887};
+  const double A1[3] = {0.21370, 0.20548, 0.11125};
+  const double B1[3] = {7.5957, 14.1189, 10.357};
+  const double B2[3] = {3.5876, 6.1977, 3.6231};
+  const double B3[3] = {1.6382, 3.3662, 0.88026};
+  const double B4[3] = {0.49294, 0.62517, 0.49671};
+  const double a = A[set];
+  const Dv s = powc(rs, 0.5);
+  const Dv Q0 = (cst(1.0) + rs * A1[set]) * (-2.0 * a);
+  const Dv Q1 = (s * B1[set] + rs * B2[set] + rs * s * B3[set] + rs * rs * B4[set]) * (2.0 * a);
+  const Dv Q2 = logd(cst(1.0) + cst(1.0) / Q1);
+  return Q0 * Q2;
+}
+
+// n eps_c^PW92(n_a, n_b)
+__device__ __forceinline__ Dv pw92_spin(Dv na, Dv nb) {
+  const double c13 = 1.0 / 3.0, c43 = 4.0 / 3.0;
+  const double crs = pow(3.0 / (4.0 * PI), c13);
+  const double cf = 1.0 / (pow(2.0, c43) - 2.0);
+  const double cfzz = 2.0 * c43 * c13 * cf;
+  Dv n = na + nb;
+  if (n.v < DEPS) n = cst(DEPS);
+  const Dv rs = cst(crs) / powc(n, c13);
+  const Dv eu = pw92_gs(rs, 0), ep = pw92_gs(rs, 1), ma = pw92_gs(rs, 2);
+  const Dv z = (na - nb) / n, z2 = z * z, z4 = z2 * z2;
+  const Dv fz = (powc(cst(1.0) + z + cst(DEPS), c43) + powc(cst(1.0) - z + cst(DEPS), c43) - cst(2.0)) * cf;
+  const Dv ec = eu + (ep - eu) * fz * z4 - ma * fz * (cst(1.0) - z4) * (1.0 / cfzz);
+  return n * ec;
+}
+
+// n H(n_a, n_b, g2), g2 = |grad n|^2
+__device__ __forceinline__ Dv pbec_spin(Dv na, Dv nb, Dv g2) {
+  const double beta = 0.066725, gc = 0.03109069086965489503494086371;
+  const Dv n = na + nb;
+  if (n.v <= 0.0) return cst(0.0);
+  const Dv el = pw92_spin(na, nb) / n;
+  Dv za = na / n, zb = nb / n;
+  if (za.v < DEPS) za = cst(DEPS);
+  if (zb.v < DEPS) zb = cst(DEPS);
+  const Dv fi = (powc(za, 2.0 / 3.0) + powc(zb, 2.0 / 3.0)) * (1.0 / pow(2.0, 1.0 / 3.0));
+  const Dv fi2 = fi * fi, fi3 = fi2 * fi;
+  const Dv A = cst(beta / gc) / (expd(el * (-1.0) / (fi3 * gc)) - cst(1.0) + cst(DEPS));
+  const Dv t2 = g2 / (fi2 * powc(n, 7.0 / 3.0) + cst(DEPS)) * (PI / (16.0 * pow(3.0 * PI * PI, 1.0 / 3.0)));
+  const Dv x = A * t2;
+  const Dv rat = (cst(1.0) + x) / (cst(1.0) + x + x * x);
+  const Dv H = fi3 * gc * logd(cst(1.0) + t2 * (beta / gc) * rat);
+  return n * H;
+}
+}  // namespace xcs
+
+// item = 256-point block of an open-shell unit (the alpha spin's unit, which
+// carries the beta spin's partials, coefficients and state in part2, coef2,
+// state2).  Per point: rho_s, grad rho_s from the partials, f, dF/drho_s,
+// dF/dg_aa, dF/dg_bb, dF/dg_ab, then a_s = w dF/drho_s / 2 and
+// b_s = w (2 dF/dg_ss grad rho_s + dF/dg_ab grad rho_s')
+// (modules/xc_hamiltonian.f90:1428-1445).
+template <bool GGA>
+__global__ __launch_bounds__(kThreads) void k_functional_spin(const UnitD *__restrict__ units, int nu,
+                                                              const long *__restrict__ base,
+                                                              const int *__restrict__ order,
+                                                              long total) {
+  __shared__ double se[kThreads], sq[kThreads];
+  const int tid = threadIdx.x;
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const long it = item_at(order, L, total);
+    const int u = find_unit(base, nu, it);
+    const UnitD &U = units[u];
+    const long blk = it - base[u];
+    const long p = blk * kThreads + tid, Mc = U.Mc, npad = U.npad;
+    const double *part[2] = {U.part, U.part2};
+    double *state[2] = {U.state, U.state2};
+    double *coef[2] = {U.coef, U.coef2};
+    double rho[2], g[2][3];
+    const int tile = (int)(p / kScreenTile);
+    const int nch = (U.tl_off[tile + 1] - U.tl_off[tile] + U.CH - 1) / U.CH;
+    for (int sp = 0; sp < 2; ++sp) {
+      double r = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+      for (int t = 0; t < nch; ++t) {
+        r += part[sp][((long)t * 4 + 0) * Mc + p];
+        if (GGA) {
+          g0 += part[sp][((long)t * 4 + 1) * Mc + p];
+          g1 += part[sp][((long)t * 4 + 2) * Mc + p];
+          g2 += part[sp][((long)t * 4 + 3) * Mc + p];
+        }
+      }
+      if (U.incremental) {
+        r = state[sp][S_RHO * npad + p] + r;
+        if (GGA) {
+          g0 = state[sp][S_GX * npad + p] + g0;
+          g1 = state[sp][S_GY * npad + p] + g1;
+          g2 = state[sp][S_GZ * npad + p] + g2;
+        }
+      }
+      rho[sp] = r;
+      g[sp][0] = g0;
+      g[sp][1] = g1;
+      g[sp][2] = g2;
+    }
+    const double gaa = g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2];
+    const double gbb = g[1][0] * g[1][0] + g[1][1] * g[1][1] + g[1][2] * g[1][2];
+    const double gab = g[0][0] * g[1][0] + g[0][1] * g[1][1] + g[0][2] * g[1][2];
+    double f = 0.0, fr[2] = {0.0, 0.0}, fgs[2] = {0.0, 0.0}, fab = 0.0;
+    const uint32_t mask = U.mask;
+    const double gss[2] = {gaa, gbb};
+    if (mask & (1u << 1))
+      for (int sp = 0; sp < 2; ++sp) {
+        double e = 0.0, er = 0.0;
+        xcf::xalpha(2.0 * rho[sp], e, er);
+        f += 0.5 * e;
+        fr[sp] += er;
+      }
+    if (mask & (1u << 3)) {
+      const xcs::Dv F = xcs::pw92_spin({rho[0], 1.0, 0.0, 0.0}, {rho[1], 0.0, 1.0, 0.0});
+      f += F.v;
+      fr[0] += F.d0;
+      fr[1] += F.d1;
+    }
+    if (GGA) {
+      if (mask & (1u << 10))
+        for (int sp = 0; sp < 2; ++sp) {
+          double e = 0.0, er = 0.0, eg = 0.0;
+          xcf::pbex(2.0 * rho[sp], 4.0 * gss[sp], e, er, eg);
+          f += 0.5 * e;
+          fr[sp] += er;
+          fgs[sp] += 2.0 * eg;
+        }
+      if (mask & (1u << 18)) {
+        const xcs::Dv F = xcs::pbec_spin({rho[0], 1.0, 0.0, 0.0}, {rho[1], 0.0, 1.0, 0.0},
+                                         {gaa + gbb + 2.0 * gab, 0.0, 0.0, 1.0});
+        f += F.v;
+        fr[0] += F.d0;
+        fr[1] += F.d1;
+        fgs[0] += F.d2;
+        fgs[1] += F.d2;
+        fab += 2.0 * F.d2;
+      }
+    }
+    const double ww = U.gw[p];
+    for (int sp = 0; sp < 2; ++sp) {
+      const double a = 0.5 * fr[sp] * ww;
+      double b[3];
+      for (int c = 0; c < 3; ++c) b[c] = ww * (2.0 * fgs[sp] * g[sp][c] + fab * g[1 - sp][c]);
+      double *cf = coef[sp], *st = state[sp];
+      if (U.incremental) {
+        cf[0 * Mc + p] = a - st[S_A * npad + p];
+        if (GGA) {
+          cf[1 * Mc + p] = b[0] - st[S_BX * npad + p];
+          cf[2 * Mc + p] = b[1] - st[S_BY * npad + p];
+          cf[3 * Mc + p] = b[2] - st[S_BZ * npad + p];
+        }
+      } else {
+        cf[0 * Mc + p] = a;
+        if (GGA) {
+          cf[1 * Mc + p] = b[0];
+          cf[2 * Mc + p] = b[1];
+          cf[3 * Mc + p] = b[2];
+        }
+      }
+      st[S_RHO * npad + p] = rho[sp];
+      st[S_A * npad + p] = a;
+      if (GGA) {
+        st[S_GX * npad + p] = g[sp][0];
+        st[S_GY * npad + p] = g[sp][1];
+        st[S_GZ * npad + p] = g[sp][2];
+        st[S_BX * npad + p] = b[0];
+        st[S_BY * npad + p] = b[1];
+        st[S_BZ * npad + p] = b[2];
+      }
+    }
+    se[tid] = f * ww;
+    sq[tid] = (rho[0] + rho[1]) * ww;
+    __syncthreads();
+    for (int st = kThreads / 2; st > 0; st >>= 1) {
+      if (tid < st) {
+        se[tid] += se[tid + st];
+        sq[tid] += sq[tid + st];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      U.epart[2 * blk + 0] = se[0];
+      U.epart[2 * blk + 1] = sq[0];
+    }
+    __syncthreads();
+  }
+}
+
 // -------------------------------------------------------------------- W
 // item = (1024-point block, 16-function group), 4 consecutive points of one
 // 32-point group per thread: W = a Phi + b . grad Phi.  SPLIT (split-BF16
@@ -992,6 +1212,15 @@ void launch_functional(hipStream_t s, bool gga, const StageList &L) {
     hipLaunchKernelGGL((k_functional<true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
   else
     hipLaunchKernelGGL((k_functional<false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
+void launch_functional_spin(hipStream_t s, bool gga, const StageList &L) {
+  if (!L.total) return;
+  dim3 g(persistent_grid(L.total, 8));
+  if (gga)
+    hipLaunchKernelGGL((k_functional_spin<true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  else
+    hipLaunchKernelGGL((k_functional_spin<false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
 }
 
 template <typename T>

@@ -61,6 +61,7 @@ class XCBuilder:
                                      info.ctypes.data_as(_ip), _ptr(ex), _ptr(nm), _ptr(ct),
                                      _ptr(g), self.npts, veclen, ctypes.byref(opt))
         self._h = h
+        self.ispin = 1
         self._check(rc)
 
     def _check(self, rc):
@@ -69,15 +70,27 @@ class XCBuilder:
             raise _lib.PgxcError(rc, msg)
 
     def set_density(self, P, incremental=False):
+        """P: (n, n), or (n, n, 2) for an open shell (alpha, beta)."""
         P = np.asfortranarray(P, dtype=np.float64)
-        self._check(self._L.pgxc_set_density(self._h, 1, self.nbas, _ptr(P),
+        if P.shape[:2] != (self.nbas, self.nbas) or P.ndim not in (2, 3) or \
+                (P.ndim == 3 and P.shape[2] != 2):
+            raise ValueError("P must be (nbas, nbas) or (nbas, nbas, 2)")
+        ispin = 1 if P.ndim == 2 else 2
+        self._check(self._L.pgxc_set_density(self._h, ispin, self.nbas, _ptr(P),
                                              INCREMENTAL if incremental else FULL))
+        self.ispin = ispin
 
     def set_orbitals(self, C, occ, incremental=False):
+        """C: (n, nocc) and occ (nocc,), or (n, nocc, 2) and (nocc, 2) for an
+        open shell."""
         C = np.asfortranarray(C, dtype=np.float64)
-        occ = np.ascontiguousarray(occ, dtype=np.float64)
-        self._check(self._L.pgxc_set_orbitals(self._h, 1, self.nbas, C.shape[1], _ptr(C),
+        ispin = 1 if C.ndim == 2 else 2
+        occ = np.asfortranarray(occ, dtype=np.float64)
+        if C.ndim not in (2, 3) or C.shape[0] != self.nbas or occ.shape != C.shape[1:]:
+            raise ValueError("C must be (nbas, nocc[, 2]) with occ (nocc[, 2])")
+        self._check(self._L.pgxc_set_orbitals(self._h, ispin, self.nbas, C.shape[1], _ptr(C),
                                               _ptr(occ), INCREMENTAL if incremental else FULL))
+        self.ispin = ispin
 
     def build(self, functional=PBE, precision=FP64, fetch=True):
         if isinstance(functional, str):
@@ -87,14 +100,15 @@ class XCBuilder:
         if not fetch:
             self._check(self._L.pgxc_build(self._h, functional, precision, None, None, None))
             return None
-        ham = np.zeros(self.nbas * (self.nbas + 1) // 2)
+        # ispin = 2: alpha / beta interleaved (ham[0::2], ham[1::2])
+        ham = np.zeros(self.ispin * self.nbas * (self.nbas + 1) // 2)
         e = np.zeros(1)
         q = np.zeros(1)
         self._check(self._L.pgxc_build(self._h, functional, precision, _ptr(ham), _ptr(e), _ptr(q)))
         return ham, float(e[0]), float(q[0])
 
     def result(self):
-        ham = np.zeros(self.nbas * (self.nbas + 1) // 2)
+        ham = np.zeros(self.ispin * self.nbas * (self.nbas + 1) // 2)
         e = np.zeros(1)
         q = np.zeros(1)
         self._check(self._L.pgxc_get_result(self._h, _ptr(ham), _ptr(e), _ptr(q)))
@@ -138,6 +152,9 @@ class XCBuilder:
         geometry (pgxc_load_state)."""
         buf = np.frombuffer(data, dtype=np.uint8)
         self._check(self._L.pgxc_load_state(self._h, buf.ctypes.data_as(ctypes.c_void_p), buf.size))
+        # the record's spin count: int32 after the 8-byte magic, the 8-byte
+        # geometry hash, five int64 sizes and the uint32 functional mask
+        self.ispin = int(np.frombuffer(data, dtype=np.int32, count=1, offset=60)[0])
 
     def set_lanes(self, lanes):
         """1: every chunk on one stream; 2 (default for grids of several chunks):

@@ -1,0 +1,169 @@
+"""Open-shell builds (ispin = 2) on the GPU against the FP64 oracle's
+open-shell build on identical inputs (modules/xc_hamiltonian.f90:1428-1460,
+:1555-1567): the same bars as the closed-shell parity tests,
+|dE_xc| <= 1e-9 Eh and max |dV_xc| <= 1e-10 for FP64, over both spins of
+the interleaved output."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from pgxc import XCBuilder, basis, grid, systems
+from pgxc.xc import BF16X3, FP64, LDA, PBE
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+
+
+def spin_orbitals(nbas, seed):
+    """Alpha: 5 orbitals, beta: a rotated set with fractional occupations, so
+    the spin densities differ at every point."""
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.normal(size=(nbas, 6)))
+    C = np.stack([Q[:, :5], Q[:, 1:6]], axis=2)
+    occ = np.stack([np.ones(5), np.array([1.0, 1.0, 0.8, 0.5, 0.0])], axis=1)
+    return C, occ
+
+
+@pytest.fixture(scope="module")
+def case():
+    els, xyz = systems.h2o()
+    b = basis.seeded_basis(els, xyz, seed=1, shell_set="small")
+    g = grid.molecular_grid(els, xyz, nrad=75, nang=302)
+    C, occ = spin_orbitals(b.nbas, 4)
+    return b, g, C, occ
+
+
+_REF = {}
+
+
+def ref(case, mask):
+    b, g, C, occ = case
+    if mask not in _REF:
+        _REF[mask] = oracle.xc_build(b, g, mask, C=C, occ=occ, ispin=2, nthreads=NT)
+    return _REF[mask]
+
+
+def check(res, r, e_tol=1e-9, v_tol=1e-10, q_tol=1e-9):
+    ham, e, q = res
+    hr, er, qr = r
+    assert ham.shape == hr.shape
+    dva = np.max(np.abs(ham[0::2] - hr[0::2]))
+    dvb = np.max(np.abs(ham[1::2] - hr[1::2]))
+    print("dE=%.3e dV_alpha=%.3e dV_beta=%.3e dQ=%.3e" % (e - er, dva, dvb, q - qr), flush=True)
+    assert abs(e - er) <= e_tol
+    assert dva <= v_tol and dvb <= v_tol
+    assert abs(q - qr) <= q_tol
+
+
+@pytest.mark.parametrize("functional,mask", [(PBE, oracle.MASK_PBE), (LDA, oracle.MASK_LDA)],
+                         ids=["PBE", "LDA"])
+def test_open_shell_fp64_matches_oracle(case, functional, mask):
+    b, g, C, occ = case
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        res = xc.build(functional, FP64)
+    check(res, ref(case, mask))
+
+
+def test_open_shell_chunked_two_lanes(case):
+    b, g, C, occ = case
+    with XCBuilder(b, g, chunk_points=16384) as xc:
+        assert xc.layout()["chunks"] >= 3
+        xc.set_orbitals(C, occ)
+        res = xc.build(PBE, FP64)
+    check(res, ref(case, oracle.MASK_PBE))
+
+
+def test_equal_spins_give_the_closed_shell_build(case):
+    b, g, C, occ = case
+    P = (C[:, :, 0] * occ[:, 0]) @ C[:, :, 0].T
+    with XCBuilder(b, g) as xc:
+        xc.set_density(P)
+        h1, e1, q1 = xc.build(PBE, FP64)
+        xc.set_density(np.stack([P / 2, P / 2], axis=2))
+        h2, e2, q2 = xc.build(PBE, FP64)
+    scale = np.max(np.abs(h1))
+    assert abs(e2 - e1) <= 1e-11 * abs(e1)
+    assert abs(q2 - q1) <= 1e-12 * abs(q1)
+    assert np.max(np.abs(h2[0::2] - h1)) <= 1e-11 * scale
+    assert np.max(np.abs(h2[1::2] - h1)) <= 1e-11 * scale
+
+
+def test_open_shell_split_bf16(case):
+    # the closed-shell mixed-precision bar: relative 5e-5 of |E_xc| and the
+    # charge, 10x that of max |V| (this basis reaches exponents of 5000)
+    b, g, C, occ = case
+    r = ref(case, oracle.MASK_PBE)
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        res = xc.build(PBE, BF16X3)
+    rel = 5e-5
+    check(res, r, e_tol=rel * abs(r[1]), v_tol=10 * rel * np.max(np.abs(r[0])),
+          q_tol=rel * abs(r[2]))
+
+
+def test_open_shell_incremental_and_restart(case):
+    """A full build, then an incremental one with changed spin densities:
+    the FP64 bar against the oracle's full build of the new densities; a
+    state saved after the first build and loaded into a fresh builder
+    continues bit for bit."""
+    # perturbed orbitals keep both spin densities non-negative (a negative
+    # spin density has no real spin-polarised correlation energy)
+    b, g, C, occ = case
+    C1 = np.stack([systems.perturbed_orbitals(C[:, :, s], 1e-2, seed=9 + s) for s in range(2)],
+                  axis=2)
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        xc.build(PBE, FP64)
+        state = xc.save_state()
+        xc.set_orbitals(C1, occ, incremental=True)
+        inc = xc.build(PBE, FP64)
+    with XCBuilder(b, g) as xc2:
+        xc2.load_state(state)
+        assert xc2.ispin == 2
+        xc2.set_orbitals(C1, occ, incremental=True)
+        inc2 = xc2.build(PBE, FP64)
+    check(inc, oracle.xc_build(b, g, oracle.MASK_PBE, C=C1, occ=occ, ispin=2, nthreads=NT))
+    assert inc2[1] == inc[1] and np.array_equal(inc2[0], inc[0])
+
+
+def test_incremental_with_other_spin_count_is_refused(case):
+    b, g, C, occ = case
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        xc.build(PBE, FP64)
+        P = (C[:, :, 0] * occ[:, 0]) @ C[:, :, 0].T
+        with pytest.raises(Exception):
+            xc.set_density(P, incremental=True)
+
+
+def test_batch_of_open_and_closed_shells_is_bitwise(case):
+    """One batch holding an open-shell and a closed-shell molecule gives each
+    the bits of its own single build."""
+    from pgxc import Batch
+    b, g, C, occ = case
+    els, xyz = systems.random_molecule(4, seed=103)
+    b2 = basis.seeded_basis(els, xyz, seed=203, shell_set="small")
+    g2 = grid.molecular_grid(els, xyz, nrad=24, nang=302)
+    C2, occ2 = systems.random_orbitals(b2.nbas, 4, seed=303)
+    jobs = [(b, g, (C, occ)), (b2, g2, (C2, occ2))]
+    singles = []
+    for (bb, gg, (cc, oo)) in jobs:
+        with XCBuilder(bb, gg, tile=64, chunk_points=8192) as xb:
+            xb.set_orbitals(cc, oo)
+            singles.append(xb.build(PBE, BF16X3))
+    bs = []
+    for (bb, gg, (cc, oo)) in jobs:
+        xb = XCBuilder(bb, gg, tile=64, chunk_points=8192)
+        xb.set_orbitals(cc, oo)
+        bs.append(xb)
+    with Batch(bs) as batch:
+        batch.build(PBE, BF16X3)
+        res = batch.results()
+    for r, s in zip(res, singles):
+        assert np.array_equal(r[0], s[0]) and r[1] == s[1] and r[2] == s[2]
+    for xb in bs:
+        xb.close()
