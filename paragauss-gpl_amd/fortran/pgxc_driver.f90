@@ -4,22 +4,22 @@
 !
 ! usage: pgxc_driver <input.bin> <output.bin>
 ! input  (stream, native endianness): int32 n_atoms, n_shells, nbas, nocc,
-!        veclen, precision, mask; int64 npts; real64 atom_xyz(3,n_atoms);
+!        veclen, precision, mask, ispin; int64 npts; real64 atom_xyz(3,n_atoms);
 !        int32 shell_info(5,n_shells); int32 n_exp_total, n_contr_total;
 !        real64 exps(n_exp_total), norms(n_exp_total), contr(n_contr_total),
-!        grdpts(npts,4), eigvec_occ(nbas,nocc), occ_num_occ(nocc)
-! output: real64 exc_int, charge_int, ham_xc_arr(nbas*(nbas+1)/2)
+!        grdpts(npts,4), eigvec_occ(nbas,nocc,ispin), occ_num_occ(nocc,ispin)
+! output: real64 exc_int, charge_int, ham_xc_arr(ispin*nbas*(nbas+1)/2)
 !
 program pgxc_driver
   use iso_c_binding
   use xc_gpu
   implicit none
   character(len=1024) :: fin, fout
-  integer(c_int32_t) :: n_atoms, n_shells, nbas, nocc, veclen, precision, ne, nc
+  integer(c_int32_t) :: n_atoms, n_shells, nbas, nocc, veclen, precision, ispin, ne, nc
   integer(c_uint32_t) :: mask
   integer(c_int64_t) :: npts
   real(c_double), allocatable :: atom_xyz(:, :), exps(:), norms(:), contr(:), grdpts(:, :)
-  real(c_double), allocatable :: eigvec_occ(:, :), occ(:), ham(:)
+  real(c_double), allocatable :: eigvec_occ(:, :, :), occ(:, :), ham(:)
   integer(c_int32_t), allocatable :: shell_info(:, :)
   real(c_double) :: exc_int, charge_int
   integer :: u
@@ -27,13 +27,14 @@ program pgxc_driver
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
-  read(u) n_atoms, n_shells, nbas, nocc, veclen, precision, mask
+  read(u) n_atoms, n_shells, nbas, nocc, veclen, precision, mask, ispin
   read(u) npts
   allocate(atom_xyz(3, n_atoms), shell_info(5, n_shells))
   read(u) atom_xyz
   read(u) shell_info
   read(u) ne, nc
-  allocate(exps(ne), norms(ne), contr(max(nc, 1)), grdpts(npts, 4), eigvec_occ(nbas, nocc), occ(nocc))
+  allocate(exps(ne), norms(ne), contr(max(nc, 1)), grdpts(npts, 4), &
+       eigvec_occ(nbas, nocc, ispin), occ(nocc, ispin))
   read(u) exps, norms
   if (nc > 0) read(u) contr(1:nc)
   read(u) grdpts, eigvec_occ, occ
@@ -43,7 +44,7 @@ program pgxc_driver
   call xc_gpu_setup(atom_xyz, shell_info, exps, norms, contr, grdpts, int(veclen))
   ! xc_build: zero the accumulators (modules/xc_hamiltonian.f90:1101-1116),
   ! then the calc_xcks_nl body
-  allocate(ham(nbas * (nbas + 1) / 2))
+  allocate(ham(ispin * nbas * (nbas + 1) / 2))
   ham = 0.0_c_double
   exc_int = 0.0_c_double
   charge_int = 0.0_c_double

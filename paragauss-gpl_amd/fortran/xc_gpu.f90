@@ -5,7 +5,8 @@
 !
 ! Life cycle, mirroring the xc_hamiltonian module:
 !   xc_setup    -> xc_gpu_setup(basis arrays, grdpts, veclen)      (once per geometry)
-!   per SCF     -> xc_gpu_set_orbitals(eigvec_occ, occ_num_occ)    (density_calc input)
+!   per SCF     -> xc_gpu_set_orbitals(eigvec_occ, occ_num_occ)    (density_calc input;
+!                  rank 2 / 1 arrays: ispin = 1, rank 3 / 2: (.., ispin))
 !   calc_xcks_nl-> xc_gpu_build(mask, precision, ham_xc_arr, exc_int, charge_int)
 !   xc_close    -> xc_gpu_close()
 ! Non-zero library status is passed to xc_gpu_fail, which in ParaGauss is
@@ -20,8 +21,18 @@ module xc_gpu
   public :: xc_gpu_setup, xc_gpu_set_orbitals, xc_gpu_set_density, xc_gpu_build, &
        xc_gpu_close, xc_gpu_mask, xc_gpu_status
 
+  ! closed shell: eigvec_occ(nbas, nocc), occ_num_occ(nocc); open or closed
+  ! shell as ParaGauss holds them: eigvec_occ(nbas, nocc, ispin),
+  ! occ_num_occ(nocc, ispin)
+  interface xc_gpu_set_orbitals
+     module procedure set_orbitals_1, set_orbitals_s
+  end interface xc_gpu_set_orbitals
+  interface xc_gpu_set_density
+     module procedure set_density_1, set_density_s
+  end interface xc_gpu_set_density
+
   type(c_ptr), save :: ctx = c_null_ptr
-  integer, save :: nbas_ctx = 0
+  integer, save :: nbas_ctx = 0, ispin_ctx = 1
   integer(c_int), save :: last_status = PGXC_OK
 
 contains
@@ -74,42 +85,67 @@ contains
     nbas_ctx = int(pgxc_nbas(ctx))
   end subroutine xc_gpu_setup
 
+  integer(c_int32_t) function density_mode(incremental)
+    logical, intent(in), optional :: incremental
+    density_mode = PGXC_DENSITY_FULL
+    if (present(incremental)) then
+       if (incremental) density_mode = PGXC_DENSITY_INCREMENTAL
+    endif
+  end function density_mode
+
   ! eigvec_occ(nbas, nocc), occ_num_occ(nocc)  (modules/occupied_levels_module.f90:112-117)
-  subroutine xc_gpu_set_orbitals(eigvec_occ, occ_num_occ, incremental)
+  subroutine set_orbitals_1(eigvec_occ, occ_num_occ, incremental)
     real(c_double), intent(in) :: eigvec_occ(:, :), occ_num_occ(:)
     logical, intent(in), optional :: incremental
-    integer(c_int32_t) :: mode
     integer(c_int) :: rc
-    mode = PGXC_DENSITY_FULL
-    if (present(incremental)) then
-       if (incremental) mode = PGXC_DENSITY_INCREMENTAL
-    endif
     rc = pgxc_set_orbitals(ctx, 1_c_int32_t, int(size(eigvec_occ, 1), c_int32_t), &
-         int(size(eigvec_occ, 2), c_int32_t), eigvec_occ, occ_num_occ, mode)
+         int(size(eigvec_occ, 2), c_int32_t), eigvec_occ, occ_num_occ, density_mode(incremental))
     if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_set_orbitals', rc)
-  end subroutine xc_gpu_set_orbitals
+    ispin_ctx = 1
+  end subroutine set_orbitals_1
 
-  subroutine xc_gpu_set_density(P, incremental)
+  ! eigvec_occ(nbas, nocc, ispin), occ_num_occ(nocc, ispin)
+  subroutine set_orbitals_s(eigvec_occ, occ_num_occ, incremental)
+    real(c_double), intent(in) :: eigvec_occ(:, :, :), occ_num_occ(:, :)
+    logical, intent(in), optional :: incremental
+    integer(c_int) :: rc
+    rc = pgxc_set_orbitals(ctx, int(size(eigvec_occ, 3), c_int32_t), &
+         int(size(eigvec_occ, 1), c_int32_t), int(size(eigvec_occ, 2), c_int32_t), &
+         eigvec_occ, occ_num_occ, density_mode(incremental))
+    if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_set_orbitals', rc)
+    ispin_ctx = size(eigvec_occ, 3)
+  end subroutine set_orbitals_s
+
+  subroutine set_density_1(P, incremental)
     real(c_double), intent(in) :: P(:, :)
     logical, intent(in), optional :: incremental
-    integer(c_int32_t) :: mode
     integer(c_int) :: rc
-    mode = PGXC_DENSITY_FULL
-    if (present(incremental)) then
-       if (incremental) mode = PGXC_DENSITY_INCREMENTAL
-    endif
-    rc = pgxc_set_density(ctx, 1_c_int32_t, int(size(P, 1), c_int32_t), P, mode)
+    rc = pgxc_set_density(ctx, 1_c_int32_t, int(size(P, 1), c_int32_t), P, density_mode(incremental))
     if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_set_density', rc)
-  end subroutine xc_gpu_set_density
+    ispin_ctx = 1
+  end subroutine set_density_1
 
-  ! The body of calc_xcks_nl: accumulates into ham_xc_arr, exc_int, charge_int
+  ! P(nbas, nbas, ispin)
+  subroutine set_density_s(P, incremental)
+    real(c_double), intent(in) :: P(:, :, :)
+    logical, intent(in), optional :: incremental
+    integer(c_int) :: rc
+    rc = pgxc_set_density(ctx, int(size(P, 3), c_int32_t), int(size(P, 1), c_int32_t), P, &
+         density_mode(incremental))
+    if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_set_density', rc)
+    ispin_ctx = size(P, 3)
+  end subroutine set_density_s
+
+  ! The body of calc_xcks_nl: accumulates into ham_xc_arr, exc_int,
+  ! charge_int; ham_xc_arr has ispin * nbas (nbas + 1) / 2 entries, the spins
+  ! interleaved as the reference stores them
   subroutine xc_gpu_build(mask, precision, ham_xc_arr, exc_int, charge_int)
     integer(c_uint32_t), intent(in) :: mask
     integer(c_int32_t), intent(in) :: precision
     real(c_double), intent(inout) :: ham_xc_arr(:)
     real(c_double), intent(inout) :: exc_int, charge_int
     integer(c_int) :: rc
-    if (size(ham_xc_arr) /= nbas_ctx * (nbas_ctx + 1) / 2) &
+    if (size(ham_xc_arr) /= ispin_ctx * nbas_ctx * (nbas_ctx + 1) / 2) &
          call xc_gpu_fail('xc_gpu_build: ham_xc_arr size', PGXC_ERR_ARG)
     rc = pgxc_build(ctx, mask, precision, ham_xc_arr, exc_int, charge_int)
     if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_build', rc)

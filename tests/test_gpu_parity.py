@@ -174,7 +174,8 @@ def test_fortran_driver_matches(h2o, tmp_path):
     b, g, C, occ = h2o
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as f:
-        np.array([b.n_atoms, b.n_shells, b.nbas, C.shape[1], 128, FP64, PBE], dtype=np.int32).tofile(f)
+        np.array([b.n_atoms, b.n_shells, b.nbas, C.shape[1], 128, FP64, PBE, 1],
+                 dtype=np.int32).tofile(f)
         np.array([g.shape[0]], dtype=np.int64).tofile(f)
         np.ascontiguousarray(b.atom_xyz).tofile(f)
         np.ascontiguousarray(b.shell_info, dtype=np.int32).tofile(f)

@@ -167,3 +167,31 @@ def test_batch_of_open_and_closed_shells_is_bitwise(case):
         assert np.array_equal(r[0], s[0]) and r[1] == s[1] and r[2] == s[2]
     for xb in bs:
         xb.close()
+
+
+def test_fortran_driver_open_shell(case, tmp_path):
+    """The Fortran caller with eigvec_occ(nbas, nocc, 2), occ_num_occ(nocc, 2)
+    and an interleaved ham_xc_arr of 2 nbas (nbas + 1) / 2 entries."""
+    import subprocess
+    b, g, C, occ = case
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([b.n_atoms, b.n_shells, b.nbas, C.shape[1], 128, FP64, PBE, 2],
+                 dtype=np.int32).tofile(f)
+        np.array([g.shape[0]], dtype=np.int64).tofile(f)
+        np.ascontiguousarray(b.atom_xyz).tofile(f)
+        np.ascontiguousarray(b.shell_info, dtype=np.int32).tofile(f)
+        np.array([len(b.exps), len(b.contr)], dtype=np.int32).tofile(f)
+        b.exps.tofile(f)
+        b.norms.tofile(f)
+        np.ascontiguousarray(b.contr).tofile(f)
+        np.asarray(g, dtype=np.float64).ravel(order="F").tofile(f)
+        np.asarray(C, dtype=np.float64).ravel(order="F").tofile(f)
+        np.asarray(occ, dtype=np.float64).ravel(order="F").tofile(f)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "paragauss-gpl_amd", "fortran", "pgxc_driver")
+    out = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=300)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    raw = np.fromfile(fout, dtype=np.float64)
+    check((raw[2:], raw[0], raw[1]), ref(case, oracle.MASK_PBE))
