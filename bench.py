@@ -331,6 +331,64 @@ def bench_batch64(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_open_shell(args):
+    """(H2O)32 with an open shell (ispin = 2): alpha 160, beta 159 occupied
+    orbitals.  FP64 full builds and split-BF16 incremental builds along a
+    chain of densities, against the closed-shell build of the same grid."""
+    import numpy as np
+    from pgxc import XCBuilder, autotune, systems
+    from pgxc.xc import BF16X3, FP64, PBE
+    els, b, g, C, occ = workload("h2o32")
+    npts = g.shape[0]
+    Cb = systems.perturbed_orbitals(C, 1e-2, seed=21)
+    Cs = np.stack([C, Cb], axis=2)
+    occs = np.stack([occ, np.concatenate([occ[:-1], [0.0]])], axis=1)
+    tu = None if args.no_tune else autotune.tune(b, g, C, occ, mixed=BF16X3, log=log)
+    xc = XCBuilder(b, g, tile=tu["tile"] if tu else 0,
+                   variants=(tu["variant_fp64"], tu["variant_fp32"]) if tu else (-1, -1))
+
+    def timed(prec, incremental_chain=None):
+        out = []
+        for i in range(args.warmup + args.steps):
+            if incremental_chain is not None:
+                xc.set_orbitals(incremental_chain[i], occs, incremental=True)
+            xc.build(PBE, prec, fetch=False)
+            if i >= args.warmup:
+                out.append(xc.timing()["total"])
+        return sum(out) / len(out)
+
+    xc.set_orbitals(C, occ)
+    t_closed = timed(FP64)
+    xc.set_orbitals(Cs, occs)
+    t64 = timed(FP64)
+    chain, cur = [], Cs
+    for k in range(args.warmup + args.steps):
+        cur = np.stack([systems.perturbed_orbitals(cur[:, :, s], 1e-4, seed=40 + 2 * k + s)
+                        for s in range(2)], axis=2)
+        chain.append(cur)
+    xc.set_orbitals(Cs, occs)
+    xc.build(PBE, FP64, fetch=False)
+    tx3 = timed(BF16X3, chain)
+    v, e, _ = xc.result()
+    xc.set_orbitals(chain[-1], occs)
+    v64, e64, _ = xc.build(PBE, FP64)
+    xc.close()
+    log("open shell: closed-shell fp64 %.2f ms, open-shell fp64 %.2f ms, bf16x3 %.2f ms"
+        % (t_closed, t64, tx3))
+    out = {
+        "metric": "XC-build grid pts/s, open shell (ispin = 2), mixed vs FP64 GPU",
+        "value": npts / (tx3 * 1e-3), "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": tx3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+bf16x3", "data": "synthetic",
+        "config": {"workload": "(H2O)32 open shell, alpha 160 / beta 159 occupied, %d grid points, "
+                               "N = %d, PBE" % (npts, b.nbas)},
+        "closed_shell_fp64_ms": t_closed, "open_shell_fp64_ms": t64, "open_shell_bf16x3_ms": tx3,
+        "speedup_vs_fp64": t64 / tx3, "abs_dE_bf16x3_vs_fp64": abs(e - e64),
+        "max_abs_dV_bf16x3_vs_fp64": float(np.max(np.abs(v - v64))),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -338,7 +396,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="h2o32",
                     help="h2o32 (headline), h2o, batch64 (64-molecule batch), "
-                         "learned ((H2O)128 + solute, cost-model schedules)")
+                         "learned ((H2O)128 + solute, cost-model schedules), "
+                         "h2o32-open ((H2O)32 open shell)")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16x3"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -359,6 +418,8 @@ def main():
         return bench_learned(args)
     if args.workload == "batch64":
         return bench_batch64(args)
+    if args.workload == "h2o32-open":
+        return bench_open_shell(args)
 
     import numpy as np
     from pgxc import XCBuilder
