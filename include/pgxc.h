@@ -179,6 +179,21 @@ int pgxc_build(pgxc_ctx *ctx, uint32_t functional, int32_t precision,
 /* Copy the last build's results (same semantics as pgxc_build's outputs). */
 int pgxc_get_result(pgxc_ctx *ctx, double *ham_packed, double *exc, double *charge);
 
+/*
+ * XC energy of each functional term on the current density, without the
+ * potential matrix: the post-SCF energy evaluation of post_scf_calc_xc_energy
+ * (modules/post_scf_module.f90:2204-2420), which integrates every enabled
+ * functional's energy on the converged density.  terms4 receives (written,
+ * not added) the energies of XAlpha, PWLDAc, PBEx and PBEc (0 for a term not
+ * in the mask; PBEx and PBEc are the gradient corrections, so the PBE energy
+ * is their sum with the first two), charge the integrated density.  Uses the
+ * full density of the last pgxc_set_density / pgxc_set_orbitals (either
+ * ispin) and runs only the basis, density and functional stages; the state of
+ * incremental builds and the last build's results are left as they are.
+ */
+int pgxc_energy(pgxc_ctx *ctx, uint32_t functional, int32_t precision, double *terms4,
+                double *charge);
+
 /* Number of basis functions of the context (dims(1) in C1). */
 int64_t pgxc_nbas(const pgxc_ctx *ctx);
 

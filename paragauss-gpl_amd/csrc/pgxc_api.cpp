@@ -107,6 +107,9 @@ struct pgxc_ctx {
   double stats[5] = {0, 0, 0, 0, 0};
   std::vector<hipEvent_t> ev;
   Plan *plan = nullptr;
+  Plan *plan_e = nullptr;  // energy-only builds (pgxc_energy)
+  double *eterm = nullptr;  // [nblk_total][5] energy-term block sums
+  double last_terms[5] = {0, 0, 0, 0, 0};
   double exps_per_point = 0;
   // screening and work tables over the padded grid (setup_screen)
   int screen = 0, n8 = 0, nw8 = 0, CH = 0;
@@ -176,6 +179,7 @@ struct Plan {
   std::vector<int> incr;  // per molecule
   long order_version = -1;
   int sched_mode = 0, log = 0;
+  int energy = 0;  // energy terms only: no W, potential, fold or packing
   const pgxc_costmodel *model = nullptr;
   long model_version = -1;
   int *d_ctr = nullptr;  // queue counters of every contraction list
@@ -301,7 +305,8 @@ static void unit_item_features(const pgxc_ctx *c, int step, ItemKind kind, int p
 // Build (or reuse) the plan of a set of molecules for one configuration.
 static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int precision,
                     const std::vector<int> &incr, const std::vector<double> &unit_prio,
-                    long order_version, const Sched &sc, Plan *&plan, pgxc_ctx *errc) {
+                    long order_version, const Sched &sc, Plan *&plan, pgxc_ctx *errc,
+                    int energy = 0) {
   const int smode = sc.model ? sc.mode : std::min(sc.mode, (int)PGXC_SCHED_ANALYTIC);
   std::vector<int> lanes, spins;
   for (auto *c : mols) {
@@ -310,6 +315,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   }
   if (plan && plan->functional == functional && plan->precision == precision &&
       plan->incr == incr && plan->lanes == lanes && plan->spins == spins &&
+      plan->energy == energy &&
       plan->order_version == order_version &&
       plan->sched_mode == smode &&
       plan->log == sc.log && plan->model == sc.model &&
@@ -322,6 +328,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   plan->incr = incr;
   plan->lanes = lanes;
   plan->spins = spins;
+  plan->energy = energy;
   plan->order_version = order_version;
   plan->sched_mode = smode;
   plan->log = sc.log;
@@ -419,6 +426,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.coef = c->coef + (size_t)lane * 4 * c->Mc;
       U.state = c->state + p0;
       U.epart = c->epart + 2 * (p0 / kThreads);
+      U.eterm = c->eterm + 5 * (p0 / kThreads);
       U.Hpart = c->Hpart + (size_t)lane * c->nsplit * c->NP * c->NP;
       U.Mc = c->Mc;
       U.npad = c->npad;
@@ -692,7 +700,8 @@ static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent
   const bool x3 = pl.precision == PGXC_PREC_BF16X3;
   const bool two = pl.nlanes > 1 && pl.steps.size() > 1;
   if (pl.nctr) launch_zero_ints(s, pl.d_ctr, pl.nctr);
-  for (auto &z : pl.hzero) launch_zero_ints(s, z.first, z.second);  // potential items add into H
+  if (!pl.energy)
+    for (auto &z : pl.hzero) launch_zero_ints(s, z.first, z.second);  // potential items add into H
   launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total,
               x3 ? 2 : std::is_same<T, double>::value ? 0 : 1, false);
   if (two) {
@@ -718,6 +727,11 @@ static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent
       launch_density<T>(t, gga, tf, v, S.dens[li]);
     }
     if (E) (void)hipEventRecord(E[2], t);
+    if (pl.energy) {
+      launch_energy(t, gga, false, S.func);
+      launch_energy(t, gga, true, S.func2);
+      continue;
+    }
     launch_functional(t, gga, S.func);
     launch_functional_spin(t, gga, S.func2);
     launch_wmat<T>(t, gga, x3, S.wmat);
@@ -738,6 +752,7 @@ static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent
     (void)hipEventRecord(join, s2);
     (void)hipStreamWaitEvent(s, join, 0);
   }
+  if (pl.energy) return;
   if (ev) (void)hipEventRecord((*ev)[kEvPack], s);
   launch_fold(s, pl.d_pack, pl.nmol, pl.d_fold_base, pl.fold_total);
   launch_pack(s, pl.d_pack, pl.nmol, pl.d_pack_base, pl.pack_total);
@@ -826,7 +841,10 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
   c->phi_stride = stride;
   c->skip_basis = skip;
   c->run_split = rs;
-  if (moved) free_plan(c->plan);  // its unit descriptors hold the old pointers
+  if (moved) {  // their unit descriptors hold the old pointers
+    free_plan(c->plan);
+    free_plan(c->plan_e);
+  }
   return PGXC_OK;
 }
 
@@ -1284,6 +1302,7 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   if ((rc = dalloc(c, &c->Vstate, npk))) return rc;
   if ((rc = dalloc(c, &c->Vout, npk))) return rc;
   if ((rc = dalloc(c, &c->epart, 2 * (size_t)c->nblk_total))) return rc;
+  if ((rc = dalloc(c, &c->eterm, 5 * (size_t)c->nblk_total))) return rc;
   const size_t L = (size_t)c->nlanes_cap;
   if ((rc = dalloc(c, &c->part, L * c->ntiles * 4 * c->Mc))) return rc;
   if ((rc = dalloc(c, &c->coef, L * 4 * c->Mc))) return rc;
@@ -1348,6 +1367,7 @@ static int set_spins(pgxc_ctx *c, int ispin) {
   c->have_state = 0;
   c->have_density = 0;
   free_plan(c->plan);
+  free_plan(c->plan_e);
   return PGXC_OK;
 }
 
@@ -1476,6 +1496,37 @@ int pgxc_get_result(pgxc_ctx *c, double *ham, double *exc, double *charge) {
   }
   if (exc) *exc += c->last_exc;
   if (charge) *charge += c->last_charge;
+  return PGXC_OK;
+}
+
+int pgxc_energy(pgxc_ctx *c, uint32_t functional, int32_t precision, double *terms4,
+                double *charge) {
+  int incr = 0;
+  int rc = validate_build(c, functional, precision, incr);
+  if (rc == PGXC_ERR_STATE && c->have_density) rc = PGXC_OK;  // the full density is always there
+  if (rc) return rc;
+  const bool gga = (functional & (PGXC_XC_PBEX | PGXC_XC_PBEC)) != 0;
+  hipStream_t s = c->stream;
+  const size_t elem = precision == PGXC_PREC_FP64 ? 8 : 4;
+  if ((rc = ensure_work(c, elem, precision, s))) return rc;
+  std::vector<pgxc_ctx *> mols{c};
+  std::vector<int> full{0};
+  if ((rc = get_plan(mols, functional, precision, full, {}, -1, c->sched, c->plan_e, c, 1))) return rc;
+  if (elem == 8)
+    launch_plan<double>(s, c->stream2, c->fork, c->join, *c->plan_e, gga, nullptr);
+  else
+    launch_plan<float>(s, c->stream2, c->fork, c->join, *c->plan_e, gga, nullptr);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> et(5 * (size_t)c->nblk_total);
+  HIPCHK(c, hipMemcpyAsync(et.data(), c->eterm, et.size() * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  double t[5] = {0, 0, 0, 0, 0};
+  for (long b = 0; b < c->nblk_total; ++b)
+    for (int k = 0; k < 5; ++k) t[k] += et[5 * b + k];
+  memcpy(c->last_terms, t, sizeof(t));
+  mark_basis_cached(c, elem, precision);
+  if (terms4) memcpy(terms4, t, 4 * sizeof(double));
+  if (charge) *charge = t[4];
   return PGXC_OK;
 }
 
@@ -1612,6 +1663,7 @@ int pgxc_destroy(pgxc_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   free_plan(c->plan);
+  free_plan(c->plan_e);
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
     g_cache_bytes[c->device] -= c->phic_bytes[0] + c->phic_bytes[1];
@@ -1622,7 +1674,7 @@ int pgxc_destroy(pgxc_ctx *c) {
                   c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->su_off, c->su,
                   c->gcf_off, c->gcf, c->di, c->pi, c->beta.P, c->beta.Pref, c->beta.D,
                   c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
-                  c->beta.Vstate, c->beta.Vout};
+                  c->beta.Vstate, c->beta.Vout, c->eterm};
   for (void *b : bufs) dfree(b);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);

@@ -51,6 +51,7 @@ struct UnitD {
   // reads both); nullptr otherwise
   const double *part2;
   double *coef2, *state2;
+  double *eterm;    // [Mc/256][5] block sums of the energy terms and the charge
   double *Hpart;    // [nsplit][NP][NP] potential partials of the molecule
   // Work of the contractions, over gathered runs of 8 functions (all runs
   // when screening is off).  ditems / pitems: the unit's density items
@@ -118,6 +119,8 @@ template <typename T>
 void launch_density(hipStream_t s, bool grad, int tf, int var, const StageList &L);
 void launch_functional(hipStream_t s, bool gga, const StageList &L);
 void launch_functional_spin(hipStream_t s, bool gga, const StageList &L);
+// energy terms only (no coefficients, state untouched); spin: open-shell units
+void launch_energy(hipStream_t s, bool gga, bool spin, const StageList &L);
 template <typename T>
 void launch_wmat(hipStream_t s, bool grad, bool split, const StageList &L);
 // split-BF16 operand planes of the basis values (item = 32 functions x 64 points)

@@ -920,6 +920,81 @@ __global__ __launch_bounds__(kThreads) void k_functional_spin(const UnitD *__res
   }
 }
 
+// ------------------------------------------------------- energy terms
+// item = 256-point block of a unit (SPIN: an open-shell molecule's alpha unit):
+// the XC energy of each functional term and the charge on the block, without
+// potential coefficients and without touching the per-point state
+// (post_scf_calc_xc_energy, modules/post_scf_module.f90:2204-2420).  Block
+// sums go to eterm[blk][5]: XAlpha, PWLDAc, PBEx, PBEc, charge.
+template <bool GGA, bool SPIN>
+__global__ __launch_bounds__(kThreads) void k_energy(const UnitD *__restrict__ units, int nu,
+                                                     const long *__restrict__ base,
+                                                     const int *__restrict__ order, long total) {
+  __shared__ double sh[5][kThreads];
+  const int tid = threadIdx.x;
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const long it = item_at(order, L, total);
+    const int u = find_unit(base, nu, it);
+    const UnitD &U = units[u];
+    const long blk = it - base[u];
+    const long p = blk * kThreads + tid, Mc = U.Mc;
+    const int tile = (int)(p / kScreenTile);
+    const int nch = (U.tl_off[tile + 1] - U.tl_off[tile] + U.CH - 1) / U.CH;
+    double rho[2] = {0.0, 0.0}, g[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    for (int sp = 0; sp < (SPIN ? 2 : 1); ++sp) {
+      const double *part = sp ? U.part2 : U.part;
+      for (int t = 0; t < nch; ++t) {
+        rho[sp] += part[((long)t * 4 + 0) * Mc + p];
+        if (GGA)
+          for (int c = 0; c < 3; ++c) g[sp][c] += part[((long)t * 4 + 1 + c) * Mc + p];
+      }
+    }
+    const double ww = U.gw[p];
+    const uint32_t mask = U.mask;
+    double e[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!SPIN) {
+      const double gam = g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2];
+      double fr = 0.0, fg = 0.0;
+      if (mask & (1u << 1)) xcf::xalpha(rho[0], e[0], fr);
+      if (mask & (1u << 3)) xcf::pwldac(rho[0], e[1], fr);
+      if (GGA) {
+        if (mask & (1u << 10)) xcf::pbex(rho[0], gam, e[2], fr, fg);
+        if (mask & (1u << 18)) xcf::pbec(rho[0], gam, e[3], fr, fg);
+      }
+    } else {
+      const double gss[2] = {g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2],
+                             g[1][0] * g[1][0] + g[1][1] * g[1][1] + g[1][2] * g[1][2]};
+      const double gab = g[0][0] * g[1][0] + g[0][1] * g[1][1] + g[0][2] * g[1][2];
+      for (int sp = 0; sp < 2; ++sp) {
+        double x = 0.0, xr = 0.0, xg = 0.0;
+        if (mask & (1u << 1)) {
+          xcf::xalpha(2.0 * rho[sp], x, xr);
+          e[0] += 0.5 * x;
+        }
+        if (GGA && (mask & (1u << 10))) {
+          x = 0.0;
+          xcf::pbex(2.0 * rho[sp], 4.0 * gss[sp], x, xr, xg);
+          e[2] += 0.5 * x;
+        }
+      }
+      if (mask & (1u << 3)) e[1] = xcs::pw92_spin(xcs::cst(rho[0]), xcs::cst(rho[1])).v;
+      if (GGA && (mask & (1u << 18)))
+        e[3] = xcs::pbec_spin(xcs::cst(rho[0]), xcs::cst(rho[1]),
+                              xcs::cst(gss[0] + gss[1] + 2.0 * gab)).v;
+    }
+    for (int k = 0; k < 4; ++k) sh[k][tid] = e[k] * ww;
+    sh[4][tid] = (rho[0] + rho[1]) * ww;
+    __syncthreads();
+    for (int st = kThreads / 2; st > 0; st >>= 1) {
+      if (tid < st)
+        for (int k = 0; k < 5; ++k) sh[k][tid] += sh[k][tid + st];
+      __syncthreads();
+    }
+    if (tid < 5) U.eterm[5 * blk + tid] = sh[tid][0];
+    __syncthreads();
+  }
+}
+
 // -------------------------------------------------------------------- W
 // item = (1024-point block, 16-function group), 4 consecutive points of one
 // 32-point group per thread: W = a Phi + b . grad Phi.  SPLIT (split-BF16
@@ -1212,6 +1287,14 @@ void launch_functional(hipStream_t s, bool gga, const StageList &L) {
     hipLaunchKernelGGL((k_functional<true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
   else
     hipLaunchKernelGGL((k_functional<false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
+void launch_energy(hipStream_t s, bool gga, bool spin, const StageList &L) {
+  if (!L.total) return;
+  dim3 g(persistent_grid(L.total, 8));
+  auto k = spin ? (gga ? k_energy<true, true> : k_energy<false, true>)
+                : (gga ? k_energy<true, false> : k_energy<false, false>);
+  hipLaunchKernelGGL(k, g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
 }
 
 void launch_functional_spin(hipStream_t s, bool gga, const StageList &L) {

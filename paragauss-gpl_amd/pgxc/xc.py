@@ -107,6 +107,18 @@ class XCBuilder:
         self._check(self._L.pgxc_build(self._h, functional, precision, _ptr(ham), _ptr(e), _ptr(q)))
         return ham, float(e[0]), float(q[0])
 
+    def energy(self, functional=PBE, precision=FP64):
+        """Energies of the functional's terms on the current density, without
+        the potential (pgxc_energy): ({"xalpha", "pwldac", "pbex", "pbec"}, charge)."""
+        if isinstance(functional, str):
+            functional = FUNCTIONALS[functional]
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        t = np.zeros(4)
+        q = np.zeros(1)
+        self._check(self._L.pgxc_energy(self._h, functional, precision, _ptr(t), _ptr(q)))
+        return dict(zip(("xalpha", "pwldac", "pbex", "pbec"), t.tolist())), float(q[0])
+
     def result(self):
         ham = np.zeros(self.ispin * self.nbas * (self.nbas + 1) // 2)
         e = np.zeros(1)
