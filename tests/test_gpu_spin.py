@@ -195,3 +195,22 @@ def test_fortran_driver_open_shell(case, tmp_path):
     assert out.returncode == 0
     raw = np.fromfile(fout, dtype=np.float64)
     check((raw[2:], raw[0], raw[1]), ref(case, oracle.MASK_PBE))
+
+
+def test_fully_polarised_and_empty_spin(case):
+    """Beta spin empty (a fully polarised density, as for a doublet radical
+    with every electron in alpha): zeta = 1 everywhere, the DEPS-regularised
+    branches of PW92 and PBE-c; and a build with no electrons at all."""
+    b, g, C, occ = case
+    occ2 = occ.copy()
+    occ2[:, 1] = 0.0
+    r = oracle.xc_build(b, g, oracle.MASK_PBE, C=C, occ=occ2, ispin=2, nthreads=NT)
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ2)
+        res = xc.build(PBE, FP64)
+        assert np.all(np.isfinite(res[0]))
+        check(res, r)
+        xc.set_orbitals(C[:, :0, :], occ[:0, :])
+        ham, e, q = xc.build(PBE, FP64)
+    # PW92 clamps n below DEPS = 1e-30 to DEPS: a vanishing, not zero, energy
+    assert abs(e) <= 1e-20 and q == 0.0 and not np.any(ham)
