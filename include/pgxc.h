@@ -194,6 +194,19 @@ int pgxc_get_result(pgxc_ctx *ctx, double *ham_packed, double *exc, double *char
 int pgxc_energy(pgxc_ctx *ctx, uint32_t functional, int32_t precision, double *terms4,
                 double *charge);
 
+/*
+ * Nuclear gradient of E_xc, the basis-centre term F1 of post_scf_module
+ * (modules/post_scf_module.f90:150-185: -2 sum P_ij <d/dR xi_i | V_xc | xi_j>,
+ * summed over spins), which is grad_xc when weight_grads is off; the grid
+ * and its weights are held fixed (the weight-derivative term F2 is not
+ * computed).  grad (n_atoms x 3, row-major by atom, caller atom order)
+ * receives dE_xc/dR (written, not added; the reference stores the negative).
+ * Evaluated at the density matrix and potential coefficients of the last
+ * build on the context, in FP64 whatever that build's precision: basis
+ * values and Hessians per chunk, X = Phi P and (d/dc Phi) P as DGEMMs.
+ */
+int pgxc_gradient(pgxc_ctx *ctx, double *grad);
+
 /* Number of basis functions of the context (dims(1) in C1). */
 int64_t pgxc_nbas(const pgxc_ctx *ctx);
 

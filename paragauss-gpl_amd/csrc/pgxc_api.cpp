@@ -22,6 +22,8 @@
 #include <vector>
 
 #include "../../include/pgxc.h"
+#include <rocblas/rocblas.h>
+
 #include "pgxc_internal.h"
 
 using namespace pgxc;
@@ -77,6 +79,7 @@ struct pgxc_ctx {
   int cache_off = 0;
   void *phic[2] = {nullptr, nullptr};
   int phic_valid[2] = {0, 0}, phic_failed[2] = {0, 0};
+  int phic_grad[2] = {0, 0};  // the cache also holds the gradient planes (a GGA basis pass ran)
   size_t phic_bytes[2] = {0, 0};
   void *phi_base = nullptr;  // Phi of the current build: chunk k at phi_base + k * phi_stride
   size_t phi_stride = 0;
@@ -130,6 +133,7 @@ struct pgxc_ctx {
   uint8_t *gcf = nullptr;
   uint32_t *di = nullptr, *pi = nullptr;
   std::vector<int> fpos;          // internal position of each caller function
+  std::vector<int> fatom;         // atom of each internal function position
   int *d_fpos = nullptr;
   Sched sched;
 };
@@ -768,7 +772,8 @@ static std::map<int, size_t> g_cache_bytes;
 // split-BF16 planes in both layouts (2 x 2 x 2 bytes)
 static size_t phi_elem_bytes(size_t elem) { return elem == 8 ? 32 : 24; }
 
-static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
+// grad: the build reads the gradient planes of the basis values (GGA)
+static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s, bool grad) {
   const size_t need_phi = phi_elem_bytes(elem) * (size_t)c->NP * c->Mc;
   const size_t need_w = (size_t)c->NP * c->Mc * elem * c->nlanes_cap;
   c->w_lane = (size_t)c->NP * c->Mc * elem;
@@ -807,6 +812,7 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
       held += need;
       HIPCHK(c, hipMemsetAsync(c->phic[e], 0, need, s));  // padding rows read as zero
       c->phic_valid[e] = 0;
+      c->phic_grad[e] = 0;
     } else {
       c->phic[e] = nullptr;
       (void)hipGetLastError();
@@ -832,7 +838,8 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
   }
   void *base = c->phic[e] ? c->phic[e] : c->phi;
   const size_t stride = c->phic[e] ? need_phi : 0;
-  const int skip = c->phic[e] && c->phic_valid[e];
+  // an LDA pass writes only the value planes: a GGA build after it evaluates again
+  const int skip = c->phic[e] && c->phic_valid[e] && (c->phic_grad[e] || !grad);
   const int rs = precision == PGXC_PREC_BF16X3 && !(c->phic[1] && c->split_valid);
   if (base != c->phi_base || stride != c->phi_stride || skip != c->skip_basis ||
       rs != c->run_split)
@@ -849,9 +856,12 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s) {
 }
 
 // after a build: the cache of this element type now holds valid values
-static void mark_basis_cached(pgxc_ctx *c, size_t elem, int precision) {
+static void mark_basis_cached(pgxc_ctx *c, size_t elem, int precision, bool grad) {
   const int e = elem == 8 ? 0 : 1;
-  if (c->phic[e]) c->phic_valid[e] = 1;
+  if (c->phic[e]) {
+    c->phic_valid[e] = 1;
+    if (grad) c->phic_grad[e] = 1;
+  }
   if (c->phic[1] && precision == PGXC_PREC_BF16X3) c->split_valid = 1;
 }
 
@@ -1215,6 +1225,9 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   c->nbas = (int)foff;
   c->exps_per_point = (double)eoff;
   function_order(sh, exps, atom_xyz, n_atoms, c->fpos);
+  c->fatom.assign(c->nbas, 0);
+  for (const ShellD &d : sh)
+    for (int j = 0; j < (2 * d.l + 1) * (d.n_unc + d.n_contr); ++j) c->fatom[d.func_off + j] = d.atom;
   // function tile: requested, or the one of 128 / 96 / 64 that pads nbas least
   if (opt && opt->tile) {
     c->tf = opt->tile;
@@ -1448,7 +1461,7 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
   const bool gga = (functional & (PGXC_XC_PBEX | PGXC_XC_PBEC)) != 0;
   hipStream_t s = c->stream;
   const size_t elem = precision == PGXC_PREC_FP64 ? 8 : 4;
-  if ((rc = ensure_work(c, elem, precision, s))) return rc;
+  if ((rc = ensure_work(c, elem, precision, s, gga))) return rc;
   std::vector<pgxc_ctx *> mols{c};
   std::vector<int> incrv{incr};
   if ((rc = get_plan(mols, functional, precision, incrv, {}, -1, c->sched, c->plan, c))) return rc;
@@ -1478,7 +1491,7 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
   c->timing[4] = t;
   HIPCHK(c, hipEventElapsedTime(&t, c->ev[kEvStart], c->ev[kEvEnd]));
   c->timing[5] = t;
-  mark_basis_cached(c, elem, precision);
+  mark_basis_cached(c, elem, precision, gga);
   if ((rc = collect(c, s, functional))) return rc;
   return pgxc_get_result(c, ham, exc, charge);
 }
@@ -1508,7 +1521,7 @@ int pgxc_energy(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ter
   const bool gga = (functional & (PGXC_XC_PBEX | PGXC_XC_PBEC)) != 0;
   hipStream_t s = c->stream;
   const size_t elem = precision == PGXC_PREC_FP64 ? 8 : 4;
-  if ((rc = ensure_work(c, elem, precision, s))) return rc;
+  if ((rc = ensure_work(c, elem, precision, s, gga))) return rc;
   std::vector<pgxc_ctx *> mols{c};
   std::vector<int> full{0};
   if ((rc = get_plan(mols, functional, precision, full, {}, -1, c->sched, c->plan_e, c, 1))) return rc;
@@ -1524,10 +1537,83 @@ int pgxc_energy(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ter
   for (long b = 0; b < c->nblk_total; ++b)
     for (int k = 0; k < 5; ++k) t[k] += et[5 * b + k];
   memcpy(c->last_terms, t, sizeof(t));
-  mark_basis_cached(c, elem, precision);
+  mark_basis_cached(c, elem, precision, gga);
   if (terms4) memcpy(terms4, t, 4 * sizeof(double));
   if (charge) *charge = t[4];
   return PGXC_OK;
+}
+
+int pgxc_gradient(pgxc_ctx *c, double *grad) {
+  if (!c || !grad) return PGXC_ERR_ARG;
+  if (!c->have_state) return fail(c, PGXC_ERR_STATE, "pgxc_gradient before a build on this context");
+  const uint32_t functional = c->state_functional;
+  const bool gga = (functional & (PGXC_XC_PBEX | PGXC_XC_PBEC)) != 0;
+  hipStream_t s = c->stream;
+  int rc;
+  if ((rc = ensure_work(c, 8, PGXC_PREC_FP64, s, true))) return rc;
+  // an FP64 plan for its basis stage: chunk k's values land where its unit
+  // points (the FP64 cache, or the lane's work buffer when there is none)
+  Plan *gp = nullptr;
+  std::vector<pgxc_ctx *> mols{c};
+  std::vector<int> full{0};
+  if ((rc = get_plan(mols, functional, PGXC_PREC_FP64, full, {}, -1, c->sched, gp, c))) return rc;
+  const long Mc = c->Mc, NP = c->NP, cs = NP * Mc;
+  const int nseg = (int)(Mc / kGradSeg);
+  double *hess = nullptr, *XY = nullptr, *part = nullptr;
+  rocblas_handle hb = nullptr;
+  std::vector<double> gfn(3 * (size_t)NP, 0.0), hp(3 * (size_t)NP * nseg);
+  auto done = [&](int code) {
+    dfree(hess);
+    dfree(XY);
+    dfree(part);
+    if (hb) rocblas_destroy_handle(hb);
+    free_plan(gp);
+    return code;
+  };
+  if (hipMalloc((void **)&hess, 6 * (size_t)cs * 8) != hipSuccess ||
+      hipMalloc((void **)&XY, 4 * (size_t)cs * 8) != hipSuccess ||
+      hipMalloc((void **)&part, hp.size() * 8) != hipSuccess)
+    return done(fail(c, PGXC_ERR_NOMEM, "pgxc_gradient: no device memory for the chunk's Hessian and products"));
+  if (hipMemsetAsync(hess, 0, 6 * (size_t)cs * 8, s) != hipSuccess)  // padding functions stay zero
+    return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: hipMemsetAsync"));
+  if (rocblas_create_handle(&hb) != rocblas_status_success || rocblas_set_stream(hb, s) != rocblas_status_success)
+    return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: rocBLAS handle"));
+  const double one = 1.0, zero = 0.0;
+  for (int k = 0; k < c->nchunks; ++k) {
+    const long p0 = (long)k * Mc;
+    const int lane = k % c->nlanes;
+    const double *phi = (const double *)((const char *)c->phi_base +
+                                         (c->phi_stride ? (size_t)k * c->phi_stride : (size_t)lane * c->phi_lane));
+    launch_basis<double>(s, true, gp->steps[k].basis);
+    launch_basis_hess(s, c->grid + p0, c->grid + c->npad + p0, c->grid + 2 * c->npad + p0, c->atoms,
+                      c->n_atoms, c->shells, c->exps, c->norms, c->contr, (int)NP, Mc, hess);
+    for (int sp = 0; sp < c->ispin; ++sp) {
+      // X = Phi P, Y_c = (d/dc Phi) P: Phi planes as Mc x NP column-major
+      // matrices; P of the last build (P_ref), symmetric
+      const double *P = sp ? c->beta.Pref : c->Pref;
+      for (int pl = 0; pl < (gga ? 4 : 1); ++pl)
+        if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (rocblas_int)Mc,
+                          (rocblas_int)NP, (rocblas_int)NP, &one, phi + pl * cs, (rocblas_int)Mc, P,
+                          (rocblas_int)NP, &zero, XY + pl * cs, (rocblas_int)Mc) != rocblas_status_success)
+          return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: rocblas_dgemm"));
+      const double *st = (sp ? c->beta.state : c->state) + p0;
+      const long np = c->npad;
+      launch_grad_fn(s, gga, (int)NP, Mc, XY, phi, hess, st + S_A * np, st + S_BX * np,
+                     st + S_BY * np, st + S_BZ * np, part);
+      if (hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: kernel or copy failed"));
+      for (long i = 0; i < NP; ++i)
+        for (int sg = 0; sg < nseg; ++sg)
+          for (int d = 0; d < 3; ++d) gfn[3 * i + d] += hp[(i * nseg + sg) * 3 + d];
+    }
+  }
+  if (hipGetLastError() != hipSuccess) return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: launch failed"));
+  mark_basis_cached(c, 8, PGXC_PREC_FP64, true);
+  for (int a = 0; a < 3 * c->n_atoms; ++a) grad[a] = 0.0;
+  for (int i = 0; i < c->nbas; ++i)
+    for (int d = 0; d < 3; ++d) grad[3 * c->fatom[i] + d] += -2.0 * gfn[3 * (size_t)i + d];
+  return done(PGXC_OK);
 }
 
 int pgxc_last_stats(const pgxc_ctx *c, double *s5) {
@@ -1770,7 +1856,7 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
     pgxc_ctx *c = b->mols[m];
     const void *phi0 = c->phi_base, *w0 = c->W, *wb0 = c->beta.W;
     const int skip0 = c->skip_basis, split0 = c->run_split;
-    int rc = ensure_work(c, elem, precision, s);
+    int rc = ensure_work(c, elem, precision, s, gga);
     if (rc) {
       b->err = "molecule " + std::to_string(m) + ": " + c->err;
       return rc;
@@ -1830,7 +1916,7 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
       return rc;
     }
     c->timing[5] = t;
-    mark_basis_cached(c, elem, precision);
+    mark_basis_cached(c, elem, precision, gga);
   }
   return PGXC_OK;
 }

@@ -176,6 +176,18 @@ void launch_form_density(hipStream_t s, const double *C, const double *occ, int 
                          int nocc, int NP, double *P);
 
 
+// Nuclear gradient, basis-centre term: the basis Hessian planes [6][NP][Mc]
+// (xx, xy, xz, yy, yz, zz) of one chunk, and per function i and segment of
+// kGradSeg points the sums of grad phi_i (2 a X_i + b . Y_i) + X_i (b . grad)
+// grad phi_i into part[i][seg][3] (XY = [4][NP][Mc]: X = Phi P, Y_c)
+constexpr int kGradSeg = 2048;
+void launch_basis_hess(hipStream_t s, const double *gx, const double *gy, const double *gz,
+                       const AtomD *atoms, int n_atoms, const ShellD *shells, const double *exps,
+                       const double *norms, const double *contr, int NP, long Mc, double *hess);
+void launch_grad_fn(hipStream_t s, bool gga, int NP, long Mc, const double *XY,
+                    const double *phi, const double *hess, const double *a, const double *bx,
+                    const double *by, const double *bz, double *part);
+
 // Becke partition weights of points owned by atoms (grid generation)
 void launch_becke(hipStream_t s, int na, const double *axyz, const double *inv_dist, long npts,
                   const double *pts, const int *owner, const double *wat, double *wout);

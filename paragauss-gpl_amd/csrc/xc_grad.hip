@@ -1,0 +1,269 @@
+// Nuclear gradient of E_xc, the basis-centre term: ParaGauss's F1
+// (modules/post_scf_module.f90:150-185, stored in grad_xc when weight_grads
+// is off), on a fixed grid:
+//
+//   dE/dR_A = -2 sum_p sum_{i on A} [ grad_d phi_i (2 a X_i + b . Y_i)
+//                                     + X_i (b . grad) grad_d phi_i ]
+//
+// per spin, with X = Phi P, Y_c = (d/dc Phi) P (plain DGEMMs), a = w dF/drho / 2
+// and b = w (2 dF/dg_ss grad rho_s + dF/dg_ab grad rho_s') the potential
+// coefficients of the build (the second term only for GGA).  This file holds
+// the basis Hessian and the per-function reduction over points.
+#include "pgxc_internal.h"
+#include "xc_device.h"
+
+namespace pgxc {
+namespace {
+
+// value, gradient and Hessian (xx, xy, xz, yy, yz, zz) of a function of r
+struct J2 {
+  double v, g[3], h[6];
+};
+__device__ constexpr int kHc[6] = {0, 0, 0, 1, 1, 2}, kHd[6] = {0, 1, 2, 1, 2, 2};
+// position of the symmetric pair (c, d) in that order
+__device__ __forceinline__ int hidx(int c, int d) {
+  const int lo = c < d ? c : d, hi = c < d ? d : c;
+  return lo == 0 ? hi : lo == 1 ? 2 + hi : 5;
+}
+
+__device__ __forceinline__ J2 jconst(double v) {
+  J2 r;
+  r.v = v;
+  for (int c = 0; c < 3; ++c) r.g[c] = 0.0;
+  for (int k = 0; k < 6; ++k) r.h[k] = 0.0;
+  return r;
+}
+__device__ __forceinline__ J2 jcoord(double x, int c) {  // the coordinate x_c
+  J2 r = jconst(x);
+  r.g[c] = 1.0;
+  return r;
+}
+__device__ __forceinline__ J2 jmul(const J2 &a, const J2 &b) {
+  J2 r;
+  r.v = a.v * b.v;
+  for (int c = 0; c < 3; ++c) r.g[c] = a.g[c] * b.v + a.v * b.g[c];
+  for (int k = 0; k < 6; ++k) {
+    const int c = kHc[k], d = kHd[k];
+    r.h[k] = a.h[k] * b.v + a.g[c] * b.g[d] + a.g[d] * b.g[c] + a.v * b.h[k];
+  }
+  return r;
+}
+__device__ __forceinline__ J2 jaxpby(double s, const J2 &a, double t, const J2 &b) {
+  J2 r;
+  r.v = s * a.v + t * b.v;
+  for (int c = 0; c < 3; ++c) r.g[c] = s * a.g[c] + t * b.g[c];
+  for (int k = 0; k < 6; ++k) r.h[k] = s * a.h[k] + t * b.h[k];
+  return r;
+}
+
+// Solid harmonics up to L as jets: the recursion of the value evaluation
+// (m = 1 z-like, (2m, 2m+1) pairs, top pair by the (x, y) rotation) carried
+// out on value + first + second derivatives.
+template <int L>
+__device__ __forceinline__ void sh_jets(double x, double y, double z, J2 *C) {
+  const J2 X = jcoord(x, 0), Y = jcoord(y, 1), Z = jcoord(z, 2);
+  J2 R2 = jconst(x * x + y * y + z * z);
+  R2.g[0] = 2.0 * x;
+  R2.g[1] = 2.0 * y;
+  R2.g[2] = 2.0 * z;
+  R2.h[0] = R2.h[3] = R2.h[5] = 2.0;
+  C[0] = jconst(1.0);
+  if constexpr (L >= 1) {
+    C[1] = Z;
+    C[2] = X;
+    C[3] = Y;
+  }
+#pragma unroll
+  for (int l = 2; l <= L; ++l) {
+    const int o = l * l, o1 = (l - 1) * (l - 1), o2 = (l - 2) * (l - 2);
+    const double lr = l, l1r = l - 1, ll1r = 2 * l - 1;
+    C[o] = jaxpby(ll1r / lr, jmul(Z, C[o1]), -l1r / lr, jmul(R2, C[o2]));
+#pragma unroll
+    for (int im = 1; im <= l - 2; ++im) {
+      const double sim = (double)(im * im);
+      const double rlm = 1.0 / sqrt(lr * lr - sim), rl1m = sqrt(l1r * l1r - sim);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int m = 2 * im + k - 1;
+        C[o + m] = jaxpby(ll1r * rlm, jmul(Z, C[o1 + m]), -rl1m * rlm, jmul(R2, C[o2 + m]));
+      }
+    }
+    {
+      const double rlm = 1.0 / sqrt(lr * lr - (double)((l - 1) * (l - 1)));
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int m = 2 * l - 3 + k;
+        C[o + m] = jaxpby(ll1r * rlm, jmul(Z, C[o1 + m]), 0.0, C[o1 + m]);
+      }
+    }
+    {
+      const double rll = sqrt(ll1r / (2.0 * lr));
+      const int i2 = o1 + 2 * l - 3, i1 = o1 + 2 * l - 2;
+      C[o + 2 * l - 1] = jaxpby(rll, jmul(X, C[i2]), -rll, jmul(Y, C[i1]));
+      C[o + 2 * l] = jaxpby(rll, jmul(Y, C[i2]), rll, jmul(X, C[i1]));
+    }
+  }
+}
+
+// Hessian planes of the functions of one shell at one point: phi = C_lm R(r)
+// with R a contraction of norm exp(-alpha r^2) (zero where alpha r^2 >= 50,
+// as the value evaluation)
+template <int L>
+__device__ __forceinline__ void shell_hess(const ShellD &sh, const double *__restrict__ exps,
+                                           const double *__restrict__ norms,
+                                           const double *__restrict__ contr, double x, double y,
+                                           double z, const J2 *C, double *hess, long Mc, long cs,
+                                           long p) {
+  const int ne = sh.n_exp, nu = sh.n_unc, nc = sh.n_contr, nf = nu + nc;
+  const double r2 = x * x + y * y + z * z, xs[3] = {x, y, z};
+  // radial jet of R = sum_e c_e r0_e: grad = R1 x, Hessian = R2 x x^T + R1 I,
+  // R1 = sum c (-2 alpha) r0, R2 = sum c (4 alpha^2) r0
+  auto emit = [&](int f, double R0, double R1, double R2) {
+    J2 R = jconst(R0);
+    for (int c = 0; c < 3; ++c) R.g[c] = R1 * xs[c];
+    for (int k = 0; k < 6; ++k)
+      R.h[k] = R2 * xs[kHc[k]] * xs[kHd[k]] + (kHc[k] == kHd[k] ? R1 : 0.0);
+#pragma unroll
+    for (int m = 0; m < 2 * L + 1; ++m) {
+      const J2 F = jmul(C[L * L + m], R);
+      const long q = (long)(f + m * nf) * Mc + p;
+      for (int k = 0; k < 6; ++k) hess[k * cs + q] = F.h[k];
+    }
+  };
+  double s0[kMaxContr], s1[kMaxContr], s2[kMaxContr];
+  for (int k = 0; k < kMaxContr; ++k) s0[k] = s1[k] = s2[k] = 0.0;
+  for (int e = 0; e < ne; ++e) {
+    const double alpha = exps[sh.exp_off + e], ar2 = alpha * r2;
+    double r0 = 0.0;
+    if (ar2 < 50.0) r0 = norms[sh.exp_off + e] * exp(-ar2);
+    const double r1 = -2.0 * alpha * r0, rr = 4.0 * alpha * alpha * r0;
+    if (e < nu) emit(sh.func_off + e, r0, r1, rr);
+    for (int k = 0; k < nc; ++k) {
+      const double cf = contr[sh.contr_off + k * ne + e];
+      s0[k] += r0 * cf;
+      s1[k] += r1 * cf;
+      s2[k] += rr * cf;
+    }
+  }
+  for (int k = 0; k < nc; ++k) emit(sh.func_off + nu + k, s0[k], s1[k], s2[k]);
+}
+
+template <int LA>
+__device__ __forceinline__ void atom_hess(const AtomD &at, const ShellD *__restrict__ shells,
+                                          const double *exps, const double *norms,
+                                          const double *contr, double x, double y, double z,
+                                          double *hess, long Mc, long cs, long p) {
+  J2 C[(LA + 1) * (LA + 1)];
+  sh_jets<LA>(x, y, z, C);
+  for (int s = at.shell_begin; s < at.shell_end; ++s) {
+    const ShellD sh = shells[s];
+    switch (sh.l) {
+      case 0: shell_hess<0>(sh, exps, norms, contr, x, y, z, C, hess, Mc, cs, p); break;
+      case 1: if constexpr (LA >= 1) shell_hess<1>(sh, exps, norms, contr, x, y, z, C, hess, Mc, cs, p); break;
+      case 2: if constexpr (LA >= 2) shell_hess<2>(sh, exps, norms, contr, x, y, z, C, hess, Mc, cs, p); break;
+      case 3: if constexpr (LA >= 3) shell_hess<3>(sh, exps, norms, contr, x, y, z, C, hess, Mc, cs, p); break;
+      case 4: if constexpr (LA >= 4) shell_hess<4>(sh, exps, norms, contr, x, y, z, C, hess, Mc, cs, p); break;
+    }
+  }
+}
+
+// item = (256-point block, atom); hess = [6][NP][Mc] of one chunk
+__global__ __launch_bounds__(kThreads) void k_basis_hess(
+    const double *__restrict__ gx, const double *__restrict__ gy, const double *__restrict__ gz,
+    const AtomD *__restrict__ atoms, int n_atoms, const ShellD *__restrict__ shells,
+    const double *__restrict__ exps, const double *__restrict__ norms,
+    const double *__restrict__ contr, int NP, long Mc, double *__restrict__ hess) {
+  const long total = (Mc / kThreads) * n_atoms, cs = (long)NP * Mc;
+  for (long it = blockIdx.x; it < total; it += gridDim.x) {
+    const int atom = (int)(it % n_atoms);
+    const long p = (it / n_atoms) * kThreads + threadIdx.x;
+    const AtomD at = atoms[atom];
+    const double x = gx[p] - at.x, y = gy[p] - at.y, z = gz[p] - at.z;
+    switch (at.lmax) {
+      case 0: atom_hess<0>(at, shells, exps, norms, contr, x, y, z, hess, Mc, cs, p); break;
+      case 1: atom_hess<1>(at, shells, exps, norms, contr, x, y, z, hess, Mc, cs, p); break;
+      case 2: atom_hess<2>(at, shells, exps, norms, contr, x, y, z, hess, Mc, cs, p); break;
+      case 3: atom_hess<3>(at, shells, exps, norms, contr, x, y, z, hess, Mc, cs, p); break;
+      default: atom_hess<4>(at, shells, exps, norms, contr, x, y, z, hess, Mc, cs, p); break;
+    }
+  }
+}
+
+// item = (function i, segment of kGradSeg points): the segment's sum of
+//   t_d = grad_d phi_i (2 a X_i + b . Y_i) + X_i sum_c b_c H_dc phi_i
+// into part[i][seg][3].  XY = [4][NP][Mc]: X, then Y_x, Y_y, Y_z; phi the
+// chunk's [4][NP][Mc] basis values; a, b the build's per-point coefficients.
+template <bool GGA>
+__global__ __launch_bounds__(kThreads) void k_grad_fn(int NP, long Mc, int nseg,
+                                                      const double *__restrict__ XY,
+                                                      const double *__restrict__ phi,
+                                                      const double *__restrict__ hess,
+                                                      const double *__restrict__ a,
+                                                      const double *__restrict__ bx,
+                                                      const double *__restrict__ by,
+                                                      const double *__restrict__ bz,
+                                                      double *__restrict__ part) {
+  __shared__ double sh[3][kThreads];
+  const long cs = (long)NP * Mc, total = (long)NP * nseg;
+  const int tid = threadIdx.x;
+  for (long it = blockIdx.x; it < total; it += gridDim.x) {
+    const long i = it / nseg, seg = it % nseg;
+    double t[3] = {0.0, 0.0, 0.0};
+    for (long p = seg * kGradSeg + tid; p < (seg + 1) * kGradSeg; p += kThreads) {
+      const long q = i * Mc + p;
+      const double X = XY[q], a2 = 2.0 * a[p];
+      double s = a2 * X, b[3] = {0.0, 0.0, 0.0};
+      if (GGA) {
+        b[0] = bx[p];
+        b[1] = by[p];
+        b[2] = bz[p];
+        s += b[0] * XY[cs + q] + b[1] * XY[2 * cs + q] + b[2] * XY[3 * cs + q];
+      }
+      for (int d = 0; d < 3; ++d) {
+        double u = phi[(1 + d) * cs + q] * s;
+        if (GGA) {
+          double hb = 0.0;
+          for (int c = 0; c < 3; ++c) hb += b[c] * hess[hidx(c, d) * cs + q];
+          u += X * hb;
+        }
+        t[d] += u;
+      }
+    }
+    for (int d = 0; d < 3; ++d) sh[d][tid] = t[d];
+    __syncthreads();
+    for (int st = kThreads / 2; st > 0; st >>= 1) {
+      if (tid < st)
+        for (int d = 0; d < 3; ++d) sh[d][tid] += sh[d][tid + st];
+      __syncthreads();
+    }
+    if (tid < 3) part[(i * nseg + seg) * 3 + tid] = sh[tid][0];
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+void launch_basis_hess(hipStream_t s, const double *gx, const double *gy, const double *gz,
+                       const AtomD *atoms, int n_atoms, const ShellD *shells, const double *exps,
+                       const double *norms, const double *contr, int NP, long Mc, double *hess) {
+  const long total = (Mc / kThreads) * n_atoms;
+  const unsigned g = (unsigned)std::min<long>(total, 256L * 8);
+  hipLaunchKernelGGL(k_basis_hess, dim3(g), dim3(kThreads), 0, s, gx, gy, gz, atoms, n_atoms,
+                     shells, exps, norms, contr, NP, Mc, hess);
+}
+
+void launch_grad_fn(hipStream_t s, bool gga, int NP, long Mc, const double *XY,
+                    const double *phi, const double *hess, const double *a, const double *bx,
+                    const double *by, const double *bz, double *part) {
+  const int nseg = (int)(Mc / kGradSeg);
+  const long total = (long)NP * nseg;
+  const unsigned g = (unsigned)std::min<long>(total, 256L * 8);
+  if (gga)
+    hipLaunchKernelGGL(k_grad_fn<true>, dim3(g), dim3(kThreads), 0, s, NP, Mc, nseg, XY, phi,
+                       hess, a, bx, by, bz, part);
+  else
+    hipLaunchKernelGGL(k_grad_fn<false>, dim3(g), dim3(kThreads), 0, s, NP, Mc, nseg, XY, phi,
+                       hess, a, bx, by, bz, part);
+}
+
+}  // namespace pgxc

@@ -119,6 +119,13 @@ class XCBuilder:
         self._check(self._L.pgxc_energy(self._h, functional, precision, _ptr(t), _ptr(q)))
         return dict(zip(("xalpha", "pwldac", "pbex", "pbec"), t.tolist())), float(q[0])
 
+    def gradient(self):
+        """dE_xc/dR (n_atoms, 3), basis-centre term on the fixed grid, at the
+        density and potential of the last build (pgxc_gradient)."""
+        g = np.zeros((self.basis.n_atoms, 3))
+        self._check(self._L.pgxc_gradient(self._h, _ptr(g)))
+        return g
+
     def result(self):
         ham = np.zeros(self.ispin * self.nbas * (self.nbas + 1) // 2)
         e = np.zeros(1)

@@ -14,7 +14,7 @@ import pytest
 
 import oracle
 from pgxc import XCBuilder, basis, grid, systems
-from pgxc.xc import BF16X3, FP32, FP64, PBE
+from pgxc.xc import LDA, BF16X3, FP32, FP64, PBE
 
 pytestmark = pytest.mark.gpu
 
@@ -157,6 +157,24 @@ def test_basis_cache_reuse_is_bitwise(dimer, prec):
                 assert max(tb[2:]) < 0.5 * tb[0]
     for (h1, e1, q1), (h2, e2, q2) in zip(out[True], out[False]):
         assert np.array_equal(h1, h2) and e1 == e2 and q1 == q2
+
+
+@pytest.mark.parametrize("prec", [FP64, BF16X3])
+def test_lda_then_gga_on_one_context(dimer, prec):
+    """An LDA build caches only the basis values: a GGA build after it on the
+    same context evaluates the gradients and equals a GGA build on a fresh
+    context bit for bit (and a further GGA build reuses the cache)."""
+    _, _, b, g, C, occ = dimer
+    with XCBuilder(b, g, chunk_points=8192) as xc:
+        xc.set_orbitals(C, occ)
+        xc.build(LDA, prec)
+        r1 = xc.build(PBE, prec)
+        r2 = xc.build(PBE, prec)
+    with XCBuilder(b, g, chunk_points=8192) as xc:
+        xc.set_orbitals(C, occ)
+        ref = xc.build(PBE, prec)
+    for r in (r1, r2):
+        assert np.array_equal(r[0], ref[0]) and r[1] == ref[1] and r[2] == ref[2]
 
 
 def test_prime_block_count_layout(dimer):
