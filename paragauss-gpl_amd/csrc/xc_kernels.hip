@@ -723,7 +723,27 @@ __device__ __forceinline__ Dv operator/(Dv a, Dv b) {
   return {q, (a.d0 - q * b.d0) * r, (a.d1 - q * b.d1) * r, (a.d2 - q * b.d2) * r};
 }
 __device__ __forceinline__ Dv chain(Dv a, double v, double g) { return {v, g * a.d0, g * a.d1, g * a.d2}; }
-__device__ __forceinline__ Dv powc(Dv a, double c) { return chain(a, pow(a.v, c), c * pow(a.v, c - 1.0)); }
+// the powers the formulas take, through sqrt / cbrt (arguments > 0 here)
+__device__ __forceinline__ Dv sqrtd(Dv a) {
+  const double t = sqrt(a.v);
+  return chain(a, t, 0.5 / t);
+}
+__device__ __forceinline__ Dv cbrtd(Dv a) {  // a^(1/3)
+  const double t = cbrt(a.v);
+  return chain(a, t, t / (3.0 * a.v));
+}
+__device__ __forceinline__ Dv p23(Dv a) {  // a^(2/3)
+  const double t = cbrt(a.v);
+  return chain(a, t * t, (2.0 / 3.0) / t);
+}
+__device__ __forceinline__ Dv p43(Dv a) {  // a^(4/3)
+  const double t = cbrt(a.v);
+  return chain(a, a.v * t, (4.0 / 3.0) * t);
+}
+__device__ __forceinline__ Dv p73(Dv a) {  // a^(7/3)
+  const double t = cbrt(a.v);
+  return chain(a, a.v * a.v * t, (7.0 / 3.0) * a.v * t);
+}
 __device__ __forceinline__ Dv logd(Dv a) { return chain(a, log(a.v), 1.0 / a.v); }
 __device__ __forceinline__ Dv expd(Dv a) {
   const double e = exp(a.v);
@@ -740,7 +760,7 @@ __device__ __forceinline__ Dv pw92_gs(Dv rs, int set) {
   const double B3[3] = {1.6382, 3.3662, 0.88026};
   const double B4[3] = {0.49294, 0.62517, 0.49671};
   const double a = A[set];
-  const Dv s = powc(rs, 0.5);
+  const Dv s = sqrtd(rs);
   const Dv Q0 = (cst(1.0) + rs * A1[set]) * (-2.0 * a);
   const Dv Q1 = (s * B1[set] + rs * B2[set] + rs * s * B3[set] + rs * rs * B4[set]) * (2.0 * a);
   const Dv Q2 = logd(cst(1.0) + cst(1.0) / Q1);
@@ -750,32 +770,33 @@ __device__ __forceinline__ Dv pw92_gs(Dv rs, int set) {
 // n eps_c^PW92(n_a, n_b)
 __device__ __forceinline__ Dv pw92_spin(Dv na, Dv nb) {
   const double c13 = 1.0 / 3.0, c43 = 4.0 / 3.0;
-  const double crs = pow(3.0 / (4.0 * PI), c13);
-  const double cf = 1.0 / (pow(2.0, c43) - 2.0);
+  const double crs = cbrt(3.0 / (4.0 * PI));
+  const double cf = 1.0 / (2.0 * cbrt(2.0) - 2.0);
   const double cfzz = 2.0 * c43 * c13 * cf;
   Dv n = na + nb;
   if (n.v < DEPS) n = cst(DEPS);
-  const Dv rs = cst(crs) / powc(n, c13);
+  const Dv rs = cst(crs) / cbrtd(n);
   const Dv eu = pw92_gs(rs, 0), ep = pw92_gs(rs, 1), ma = pw92_gs(rs, 2);
   const Dv z = (na - nb) / n, z2 = z * z, z4 = z2 * z2;
-  const Dv fz = (powc(cst(1.0) + z + cst(DEPS), c43) + powc(cst(1.0) - z + cst(DEPS), c43) - cst(2.0)) * cf;
+  const Dv fz = (p43(cst(1.0) + z + cst(DEPS)) + p43(cst(1.0) - z + cst(DEPS)) - cst(2.0)) * cf;
   const Dv ec = eu + (ep - eu) * fz * z4 - ma * fz * (cst(1.0) - z4) * (1.0 / cfzz);
   return n * ec;
 }
 
-// n H(n_a, n_b, g2), g2 = |grad n|^2
-__device__ __forceinline__ Dv pbec_spin(Dv na, Dv nb, Dv g2) {
+// n H(n_a, n_b, g2), g2 = |grad n|^2, on lda = pw92_spin(n_a, n_b) (the
+// PW92 term PBE-c is built on, computed once for both)
+__device__ __forceinline__ Dv pbec_spin(Dv na, Dv nb, Dv g2, Dv lda) {
   const double beta = 0.066725, gc = 0.03109069086965489503494086371;
   const Dv n = na + nb;
   if (n.v <= 0.0) return cst(0.0);
-  const Dv el = pw92_spin(na, nb) / n;
+  const Dv el = lda / n;
   Dv za = na / n, zb = nb / n;
   if (za.v < DEPS) za = cst(DEPS);
   if (zb.v < DEPS) zb = cst(DEPS);
-  const Dv fi = (powc(za, 2.0 / 3.0) + powc(zb, 2.0 / 3.0)) * (1.0 / pow(2.0, 1.0 / 3.0));
+  const Dv fi = (p23(za) + p23(zb)) * (1.0 / cbrt(2.0));
   const Dv fi2 = fi * fi, fi3 = fi2 * fi;
   const Dv A = cst(beta / gc) / (expd(el * (-1.0) / (fi3 * gc)) - cst(1.0) + cst(DEPS));
-  const Dv t2 = g2 / (fi2 * powc(n, 7.0 / 3.0) + cst(DEPS)) * (PI / (16.0 * pow(3.0 * PI * PI, 1.0 / 3.0)));
+  const Dv t2 = g2 / (fi2 * p73(n) + cst(DEPS)) * (PI / (16.0 * cbrt(3.0 * PI * PI)));
   const Dv x = A * t2;
   const Dv rat = (cst(1.0) + x) / (cst(1.0) + x + x * x);
   const Dv H = fi3 * gc * logd(cst(1.0) + t2 * (beta / gc) * rat);
@@ -844,11 +865,13 @@ __global__ __launch_bounds__(kThreads) void k_functional_spin(const UnitD *__res
         f += 0.5 * e;
         fr[sp] += er;
       }
+    // PW92 once: its own term, and the base of PBE-c (which needs PWLDAc)
+    xcs::Dv lda = xcs::cst(0.0);
     if (mask & (1u << 3)) {
-      const xcs::Dv F = xcs::pw92_spin({rho[0], 1.0, 0.0, 0.0}, {rho[1], 0.0, 1.0, 0.0});
-      f += F.v;
-      fr[0] += F.d0;
-      fr[1] += F.d1;
+      lda = xcs::pw92_spin({rho[0], 1.0, 0.0, 0.0}, {rho[1], 0.0, 1.0, 0.0});
+      f += lda.v;
+      fr[0] += lda.d0;
+      fr[1] += lda.d1;
     }
     if (GGA) {
       if (mask & (1u << 10))
@@ -861,7 +884,7 @@ __global__ __launch_bounds__(kThreads) void k_functional_spin(const UnitD *__res
         }
       if (mask & (1u << 18)) {
         const xcs::Dv F = xcs::pbec_spin({rho[0], 1.0, 0.0, 0.0}, {rho[1], 0.0, 1.0, 0.0},
-                                         {gaa + gbb + 2.0 * gab, 0.0, 0.0, 1.0});
+                                         {gaa + gbb + 2.0 * gab, 0.0, 0.0, 1.0}, lda);
         f += F.v;
         fr[0] += F.d0;
         fr[1] += F.d1;
@@ -977,10 +1000,11 @@ __global__ __launch_bounds__(kThreads) void k_energy(const UnitD *__restrict__ u
           e[2] += 0.5 * x;
         }
       }
-      if (mask & (1u << 3)) e[1] = xcs::pw92_spin(xcs::cst(rho[0]), xcs::cst(rho[1])).v;
+      const xcs::Dv lda = xcs::pw92_spin(xcs::cst(rho[0]), xcs::cst(rho[1]));
+      if (mask & (1u << 3)) e[1] = lda.v;
       if (GGA && (mask & (1u << 18)))
         e[3] = xcs::pbec_spin(xcs::cst(rho[0]), xcs::cst(rho[1]),
-                              xcs::cst(gss[0] + gss[1] + 2.0 * gab)).v;
+                              xcs::cst(gss[0] + gss[1] + 2.0 * gab), lda).v;
     }
     for (int k = 0; k < 4; ++k) sh[k][tid] = e[k] * ww;
     sh[4][tid] = (rho[0] + rho[1]) * ww;
