@@ -99,3 +99,18 @@ def test_gradient_f_and_g_shells():
     fd = fd_gradient(b, g, oracle.MASK_PBE, C=C, occ=occ)
     check(gr, fd)
     assert np.max(np.abs(gm - gr)) <= 5e-4 * np.max(np.abs(gr))
+
+
+def test_gradient_and_energy_need_a_build(water):
+    """The gradient reads the last build's coefficients and the energy the
+    current density: before them both fail with the library's state error."""
+    from pgxc import _lib
+    b, g, C, occ = water
+    with XCBuilder(b, g) as xc:
+        with pytest.raises(_lib.PgxcError):
+            xc.energy(PBE, FP64)
+        xc.set_orbitals(C, occ)
+        with pytest.raises(_lib.PgxcError):
+            xc.gradient()
+        terms, q = xc.energy(PBE, FP64)  # the density suffices for energies
+        assert q > 0
