@@ -207,6 +207,19 @@ int pgxc_energy(pgxc_ctx *ctx, uint32_t functional, int32_t precision, double *t
  */
 int pgxc_gradient(pgxc_ctx *ctx, double *grad);
 
+/*
+ * The full nuclear gradient of E_xc with the grid moving with the atoms
+ * (weight_grads on, the reference's default, modules/grid_module.f90:240-245):
+ * F1 + F2 of modules/post_scf_module.f90:150-172, F2 = sum_a dw_a/dR e_xc(r_a)
+ * + sum_a w_a dr_a/dR . grad e_xc(r_a) for Becke-partitioned weights (no
+ * atomic-size adjustment) of points that move rigidly with their owner atom.
+ * owner[npts] (caller point order): the atom whose atomic grid holds the
+ * point; the grid's weights must be the Becke partition of those atomic
+ * grids.  grad (n_atoms x 3) receives dE_xc/dR, at the last build's density
+ * and coefficients, FP64.
+ */
+int pgxc_gradient_grid(pgxc_ctx *ctx, const int32_t *owner, double *grad);
+
 /* Number of basis functions of the context (dims(1) in C1). */
 int64_t pgxc_nbas(const pgxc_ctx *ctx);
 

@@ -134,6 +134,8 @@ struct pgxc_ctx {
   uint32_t *di = nullptr, *pi = nullptr;
   std::vector<int> fpos;          // internal position of each caller function
   std::vector<int> fatom;         // atom of each internal function position
+  std::vector<long> perm;         // caller index of each internal point
+  std::vector<double> axyz;       // atom positions (bohr), caller order
   int *d_fpos = nullptr;
   Sched sched;
 };
@@ -1278,6 +1280,8 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   // grid in space-filling-curve order, padded with far points of zero weight
   std::vector<long> perm;
   morton_order(grid, npts, npts, 1, perm);
+  c->perm = perm;
+  c->axyz.assign(atom_xyz, atom_xyz + 3 * (size_t)n_atoms);
   std::vector<double> g(4 * (size_t)c->npad);
   for (int k = 0; k < 4; ++k)
     for (long p = 0; p < c->npad; ++p)
@@ -1543,7 +1547,9 @@ int pgxc_energy(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ter
   return PGXC_OK;
 }
 
-int pgxc_gradient(pgxc_ctx *c, double *grad) {
+// The basis-centre term into grad; with pm (device [3][npad], zeroed) also
+// w grad f at every point, summed over spins.
+static int gradient_impl(pgxc_ctx *c, double *grad, double *pm) {
   if (!c || !grad) return PGXC_ERR_ARG;
   if (!c->have_state) return fail(c, PGXC_ERR_STATE, "pgxc_gradient before a build on this context");
   const uint32_t functional = c->state_functional;
@@ -1600,6 +1606,9 @@ int pgxc_gradient(pgxc_ctx *c, double *grad) {
       const long np = c->npad;
       launch_grad_fn(s, gga, (int)NP, Mc, XY, phi, hess, st + S_A * np, st + S_BX * np,
                      st + S_BY * np, st + S_BZ * np, part);
+      if (pm)
+        launch_grad_pt(s, gga, (int)NP, Mc, XY, phi, hess, st + S_A * np, st + S_BX * np,
+                       st + S_BY * np, st + S_BZ * np, pm + p0, np);
       if (hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
           hipStreamSynchronize(s) != hipSuccess)
         return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: kernel or copy failed"));
@@ -1613,6 +1622,83 @@ int pgxc_gradient(pgxc_ctx *c, double *grad) {
   for (int a = 0; a < 3 * c->n_atoms; ++a) grad[a] = 0.0;
   for (int i = 0; i < c->nbas; ++i)
     for (int d = 0; d < 3; ++d) grad[3 * c->fatom[i] + d] += -2.0 * gfn[3 * (size_t)i + d];
+  return done(PGXC_OK);
+}
+
+int pgxc_gradient(pgxc_ctx *c, double *grad) { return gradient_impl(c, grad, nullptr); }
+
+int pgxc_gradient_grid(pgxc_ctx *c, const int32_t *owner, double *grad) {
+  if (!c || !owner || !grad) return PGXC_ERR_ARG;
+  const long npts = (long)c->perm.size(), np = c->npad, Mc = c->Mc;
+  const int na = c->n_atoms;
+  std::vector<int> own((size_t)np, -1);
+  for (long p = 0; p < npts; ++p) {
+    const int a = owner[c->perm[p]];
+    if (a < 0 || a >= na) return fail(c, PGXC_ERR_ARG, "pgxc_gradient_grid: owner out of range");
+    own[p] = a;
+  }
+  std::vector<double> inv((size_t)na * na, 0.0);
+  for (int i = 0; i < na; ++i)
+    for (int k = 0; k < na; ++k)
+      if (i != k) {
+        const double dx = c->axyz[3 * i] - c->axyz[3 * k], dy = c->axyz[3 * i + 1] - c->axyz[3 * k + 1],
+                     dz = c->axyz[3 * i + 2] - c->axyz[3 * k + 2];
+        const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (!(d > 0.0)) return fail(c, PGXC_ERR_ARG, "pgxc_gradient_grid: two atoms at one position");
+        inv[(size_t)i * na + k] = 1.0 / d;
+      }
+  double *pm = nullptr, *d_inv = nullptr, *fp = nullptr, *q = nullptr, *rm = nullptr, *part = nullptr;
+  int *d_own = nullptr;
+  const long nblk = Mc / kThreads;
+  auto done = [&](int code) {
+    dfree(pm);
+    dfree(d_inv);
+    dfree(fp);
+    dfree(q);
+    dfree(rm);
+    dfree(part);
+    dfree(d_own);
+    return code;
+  };
+  if (hipMalloc((void **)&pm, 3 * (size_t)np * 8) != hipSuccess ||
+      hipMalloc((void **)&d_inv, inv.size() * 8) != hipSuccess ||
+      hipMalloc((void **)&fp, (size_t)Mc * 8) != hipSuccess ||
+      hipMalloc((void **)&q, (size_t)na * Mc * 8) != hipSuccess ||
+      hipMalloc((void **)&rm, 3 * (size_t)Mc * 8) != hipSuccess ||
+      hipMalloc((void **)&part, (size_t)nblk * na * 3 * 8) != hipSuccess ||
+      hipMalloc((void **)&d_own, (size_t)np * 4) != hipSuccess)
+    return done(fail(c, PGXC_ERR_NOMEM, "pgxc_gradient_grid: no device memory"));
+  hipStream_t s = c->stream;
+  if (hipMemsetAsync(pm, 0, 3 * (size_t)np * 8, s) != hipSuccess ||
+      hipMemcpy(d_inv, inv.data(), inv.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_own, own.data(), own.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient_grid: upload"));
+  // basis-centre term, and w grad f per point (the points move with their atoms)
+  int rc = gradient_impl(c, grad, pm);
+  if (rc) return done(rc);
+  std::vector<double> hpm(3 * (size_t)np);
+  if (hipMemcpy(hpm.data(), pm, hpm.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient_grid: copy"));
+  for (long p = 0; p < np; ++p)
+    if (own[p] >= 0)
+      for (int d = 0; d < 3; ++d) grad[3 * own[p] + d] += hpm[(size_t)d * np + p];
+  // Becke weight derivatives times f
+  const uint32_t functional = c->state_functional;
+  const bool gga = (functional & (PGXC_XC_PBEX | PGXC_XC_PBEC)) != 0;
+  std::vector<double> hp((size_t)nblk * na * 3);
+  for (int k = 0; k < c->nchunks; ++k) {
+    const long p0 = (long)k * Mc;
+    const double *st = c->state + p0, *st2 = c->ispin == 2 ? c->beta.state + p0 : st;
+    launch_point_f(s, gga, c->ispin == 2, functional, Mc, np, st, st2, fp);
+    launch_becke_grad(s, na, c->atoms, d_inv, Mc, c->grid + p0, c->grid + np + p0,
+                      c->grid + 2 * np + p0, c->grid + 3 * np + p0, fp, d_own + p0, q, rm, part);
+    if (hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient_grid: weight derivatives failed"));
+    for (long b = 0; b < nblk; ++b)
+      for (int a = 0; a < na; ++a)
+        for (int d = 0; d < 3; ++d) grad[3 * a + d] += hp[((size_t)b * na + a) * 3 + d];
+  }
   return done(PGXC_OK);
 }
 

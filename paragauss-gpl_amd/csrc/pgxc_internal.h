@@ -187,6 +187,19 @@ void launch_basis_hess(hipStream_t s, const double *gx, const double *gy, const 
 void launch_grad_fn(hipStream_t s, bool gga, int NP, long Mc, const double *XY,
                     const double *phi, const double *hess, const double *a, const double *bx,
                     const double *by, const double *bz, double *part);
+// Weight-derivative and point-motion terms (the grid moves with the atoms):
+// pm[d * pstride + p] += 2 sum_i t_d(i, p) (w grad f at each point); f per
+// point from the state (st2: the beta spin's, spin builds); the Becke weight
+// derivatives times w f into part[blk][atom][3] (q: [na][Mc], rm: [3][Mc]
+// scratch; owner -1 for padding points)
+void launch_grad_pt(hipStream_t s, bool gga, int NP, long Mc, const double *XY, const double *phi,
+                    const double *hess, const double *a, const double *bx, const double *by,
+                    const double *bz, double *pm, long pstride);
+void launch_point_f(hipStream_t s, bool gga, bool spin, uint32_t mask, long Mc, long npad,
+                    const double *st, const double *st2, double *fp);
+void launch_becke_grad(hipStream_t s, int na, const AtomD *atoms, const double *inv_dist, long Mc,
+                       const double *gx, const double *gy, const double *gz, const double *gw,
+                       const double *fp, const int *owner, double *q, double *rm, double *part);
 
 // Becke partition weights of points owned by atoms (grid generation)
 void launch_becke(hipStream_t s, int na, const double *axyz, const double *inv_dist, long npts,

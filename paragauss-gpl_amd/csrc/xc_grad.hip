@@ -241,6 +241,195 @@ __global__ __launch_bounds__(kThreads) void k_grad_fn(int NP, long Mc, int nseg,
   }
 }
 
+// item = a point of the chunk: pm[d][p] += 2 sum_i t_d(i, p), the same terms
+// as k_grad_fn summed over functions instead of points: w grad f at the point
+// (the motion of the point with its atom)
+template <bool GGA>
+__global__ __launch_bounds__(kThreads) void k_grad_pt(int NP, long Mc, const double *__restrict__ XY,
+                                                      const double *__restrict__ phi,
+                                                      const double *__restrict__ hess,
+                                                      const double *__restrict__ a,
+                                                      const double *__restrict__ bx,
+                                                      const double *__restrict__ by,
+                                                      const double *__restrict__ bz,
+                                                      double *__restrict__ pm, long pstride) {
+  const long cs = (long)NP * Mc;
+  for (long p = (long)blockIdx.x * kThreads + threadIdx.x; p < Mc; p += (long)gridDim.x * kThreads) {
+    const double a2 = 2.0 * a[p];
+    double b[3] = {0.0, 0.0, 0.0};
+    if (GGA) {
+      b[0] = bx[p];
+      b[1] = by[p];
+      b[2] = bz[p];
+    }
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < NP; ++i) {
+      const long q = (long)i * Mc + p;
+      const double X = XY[q];
+      double s = a2 * X;
+      if (GGA) s += b[0] * XY[cs + q] + b[1] * XY[2 * cs + q] + b[2] * XY[3 * cs + q];
+      for (int d = 0; d < 3; ++d) {
+        double u = phi[(1 + d) * cs + q] * s;
+        if (GGA) {
+          double hb = 0.0;
+          for (int c = 0; c < 3; ++c) hb += b[c] * hess[hidx(c, d) * cs + q];
+          u += X * hb;
+        }
+        t[d] += u;
+      }
+    }
+    for (int d = 0; d < 3; ++d) pm[d * pstride + p] += 2.0 * t[d];
+  }
+}
+
+// ------------------------------------------------- Becke weight derivatives
+// w_p = w_atomic P_A(r_p) / Z, P_C = prod_{D != C} s(mu_CD), Z = sum_C P_C,
+// mu_CD = (|r - R_C| - |r - R_D|) / |R_C - R_D|, s = (1 - f3(mu)) / 2 with f3
+// the three-fold iterated polynomial x (3 - x^2) / 2 (no atomic-size
+// adjustment); the point moves with its owner A.  With sigma = s'/s:
+//   d log P_C / dR_B = sum_{D != C} sigma(mu_CD) d mu_CD / dR_B
+//   dw / dR_B = w sum_C (delta_CA - P_C / Z) d log P_C / dR_B.
+// s = 0 only at mu = +-1, where s' = 0 too: sigma is taken as 0 there.
+__device__ __forceinline__ void becke_s(double mu, double &s, double &sig) {
+  const double f1 = mu * (3.0 - mu * mu) / 2.0, f2 = f1 * (3.0 - f1 * f1) / 2.0,
+               f3 = f2 * (3.0 - f2 * f2) / 2.0;
+  const double d = 1.5 * (1.0 - f2 * f2) * 1.5 * (1.0 - f1 * f1) * 1.5 * (1.0 - mu * mu);
+  s = 0.5 * (1.0 - f3);
+  sig = s > 0.0 ? -0.5 * d / s : 0.0;
+}
+
+struct Pt {
+  double x, y, z;
+};
+__device__ __forceinline__ double dist(const AtomD &A, Pt r, double (&e)[3]) {
+  e[0] = r.x - A.x;
+  e[1] = r.y - A.y;
+  e[2] = r.z - A.z;
+  const double d = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  const double id = d > 0.0 ? 1.0 / d : 0.0;
+  e[0] *= id;
+  e[1] *= id;
+  e[2] *= id;
+  return d;
+}
+
+// item = a point: q[C][p] = P_C / Z, and rm[p] = sum_C (delta_CA - q_C)
+// d log P_C / dr (the owner's share through the point's motion)
+__global__ __launch_bounds__(kThreads) void k_becke_q(int na, const AtomD *__restrict__ atoms,
+                                                      const double *__restrict__ inv_dist, long Mc,
+                                                      const double *__restrict__ gx,
+                                                      const double *__restrict__ gy,
+                                                      const double *__restrict__ gz,
+                                                      const int *__restrict__ owner,
+                                                      double *__restrict__ q, double *__restrict__ rm) {
+  for (long p = (long)blockIdx.x * kThreads + threadIdx.x; p < Mc; p += (long)gridDim.x * kThreads) {
+    const int A = owner[p];
+    if (A < 0) {
+      for (int C = 0; C < na; ++C) q[C * Mc + p] = 0.0;
+      for (int d = 0; d < 3; ++d) rm[d * Mc + p] = 0.0;
+      continue;
+    }
+    const Pt r = {gx[p], gy[p], gz[p]};
+    double Z = 0.0;
+    for (int C = 0; C < na; ++C) {
+      double eC[3];
+      const double rC = dist(atoms[C], r, eC);
+      double P = 1.0;
+      for (int D = 0; D < na && P != 0.0; ++D) {
+        if (D == C) continue;
+        double eD[3], s, sig;
+        const double rD = dist(atoms[D], r, eD);
+        becke_s((rC - rD) * inv_dist[(long)C * na + D], s, sig);
+        P *= s;
+      }
+      q[C * Mc + p] = P;
+      Z += P;
+    }
+    const double iz = 1.0 / Z;
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int C = 0; C < na; ++C) {
+      const double qc = q[C * Mc + p] * iz;
+      q[C * Mc + p] = qc;
+      const double k = (C == A ? 1.0 : 0.0) - qc;
+      if (k == 0.0) continue;
+      double eC[3];
+      const double rC = dist(atoms[C], r, eC);
+      for (int D = 0; D < na; ++D) {
+        if (D == C) continue;
+        double eD[3], s, sig;
+        const double rD = dist(atoms[D], r, eD);
+        const double iR = inv_dist[(long)C * na + D];
+        becke_s((rC - rD) * iR, s, sig);
+        for (int d = 0; d < 3; ++d) g[d] += k * sig * (eC[d] - eD[d]) * iR;
+      }
+    }
+    for (int d = 0; d < 3; ++d) rm[d * Mc + p] = g[d];
+  }
+}
+
+// item = (256-point block, atom B): the block's sum over its points of
+//   w f [ sum_{C != B} (delta_CA - q_C) sigma(mu_CB) d mu_CB / dR_B
+//       + (delta_BA - q_B) sum_{D != B} sigma(mu_BD) d mu_BD / dR_B
+//       + [B == A] rm ]
+// with d mu_CB / dR_B = e_B / R + mu (R_C - R_B) / R^2 and
+// d mu_BD / dR_B = -e_B / R - mu (R_B - R_D) / R^2 (R = |R_C - R_B|, ...),
+// into part[blk][B][3]
+__global__ __launch_bounds__(kThreads) void k_becke_atom(
+    int na, const AtomD *__restrict__ atoms, const double *__restrict__ inv_dist, long Mc,
+    const double *__restrict__ gx, const double *__restrict__ gy, const double *__restrict__ gz,
+    const double *__restrict__ gw, const double *__restrict__ fp, const int *__restrict__ owner,
+    const double *__restrict__ q, const double *__restrict__ rm, double *__restrict__ part) {
+  __shared__ double sh[3][kThreads];
+  const int tid = threadIdx.x;
+  const long total = (Mc / kThreads) * na;
+  for (long it = blockIdx.x; it < total; it += gridDim.x) {
+    const int B = (int)(it % na);
+    const long blk = it / na, p = blk * kThreads + tid;
+    const int A = owner[p];
+    double g[3] = {0.0, 0.0, 0.0};
+    const double wf = gw[p] * fp[p];
+    if (A >= 0 && wf != 0.0) {
+      const Pt r = {gx[p], gy[p], gz[p]};
+      const AtomD &AB = atoms[B];
+      double eB[3];
+      const double rB = dist(AB, r, eB);
+      const double kB = (B == A ? 1.0 : 0.0) - q[B * Mc + p];
+      for (int C = 0; C < na; ++C) {
+        if (C == B) continue;
+        const AtomD &AC = atoms[C];
+        double eC[3], s, sig;
+        const double rC = dist(AC, r, eC);
+        const double iR = inv_dist[(long)C * na + B];
+        const double RCB[3] = {AC.x - AB.x, AC.y - AB.y, AC.z - AB.z};
+        // pair (C, B): B is the second atom
+        const double kC = (C == A ? 1.0 : 0.0) - q[C * Mc + p];
+        const double muCB = (rC - rB) * iR;
+        if (kC != 0.0) {
+          becke_s(muCB, s, sig);
+          for (int d = 0; d < 3; ++d) g[d] += kC * sig * (eB[d] * iR + muCB * RCB[d] * iR * iR);
+        }
+        // pair (B, C): B is the first atom, mu_BC = -mu_CB, R_B - R_C = -RCB
+        if (kB != 0.0) {
+          becke_s(-muCB, s, sig);
+          for (int d = 0; d < 3; ++d) g[d] += kB * sig * (-eB[d] * iR - muCB * RCB[d] * iR * iR);
+        }
+      }
+      if (B == A)
+        for (int d = 0; d < 3; ++d) g[d] += rm[d * Mc + p];
+      for (int d = 0; d < 3; ++d) g[d] *= wf;
+    }
+    for (int d = 0; d < 3; ++d) sh[d][tid] = g[d];
+    __syncthreads();
+    for (int st = kThreads / 2; st > 0; st >>= 1) {
+      if (tid < st)
+        for (int d = 0; d < 3; ++d) sh[d][tid] += sh[d][tid + st];
+      __syncthreads();
+    }
+    if (tid < 3) part[(blk * na + B) * 3 + tid] = sh[tid][0];
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 void launch_basis_hess(hipStream_t s, const double *gx, const double *gy, const double *gz,
@@ -266,4 +455,30 @@ void launch_grad_fn(hipStream_t s, bool gga, int NP, long Mc, const double *XY,
                        hess, a, bx, by, bz, part);
 }
 
+}  // namespace pgxc
+
+namespace pgxc {
+void launch_grad_pt(hipStream_t s, bool gga, int NP, long Mc, const double *XY, const double *phi,
+                    const double *hess, const double *a, const double *bx, const double *by,
+                    const double *bz, double *pm, long pstride) {
+  const unsigned g = (unsigned)std::min<long>((Mc + kThreads - 1) / kThreads, 256L * 8);
+  if (gga)
+    hipLaunchKernelGGL(k_grad_pt<true>, dim3(g), dim3(kThreads), 0, s, NP, Mc, XY, phi, hess, a,
+                       bx, by, bz, pm, pstride);
+  else
+    hipLaunchKernelGGL(k_grad_pt<false>, dim3(g), dim3(kThreads), 0, s, NP, Mc, XY, phi, hess, a,
+                       bx, by, bz, pm, pstride);
+}
+
+void launch_becke_grad(hipStream_t s, int na, const AtomD *atoms, const double *inv_dist, long Mc,
+                       const double *gx, const double *gy, const double *gz, const double *gw,
+                       const double *fp, const int *owner, double *q, double *rm, double *part) {
+  const unsigned g1 = (unsigned)std::min<long>((Mc + kThreads - 1) / kThreads, 256L * 8);
+  hipLaunchKernelGGL(k_becke_q, dim3(g1), dim3(kThreads), 0, s, na, atoms, inv_dist, Mc, gx, gy,
+                     gz, owner, q, rm);
+  const long total = (Mc / kThreads) * na;
+  const unsigned g2 = (unsigned)std::min<long>(total, 256L * 8);
+  hipLaunchKernelGGL(k_becke_atom, dim3(g2), dim3(kThreads), 0, s, na, atoms, inv_dist, Mc, gx, gy,
+                     gz, gw, fp, owner, q, rm, part);
+}
 }  // namespace pgxc

@@ -1019,6 +1019,52 @@ __global__ __launch_bounds__(kThreads) void k_energy(const UnitD *__restrict__ u
   }
 }
 
+// ------------------------------------------------ energy density per point
+// f at each point of a chunk from the per-point state of the last build
+// (rho, grad rho per spin): the weight-derivative term of the nuclear
+// gradient multiplies it by dw/dR.
+template <bool GGA, bool SPIN>
+__global__ __launch_bounds__(kThreads) void k_point_f(long Mc, long npad, uint32_t mask,
+                                                      const double *__restrict__ st,
+                                                      const double *__restrict__ st2,
+                                                      double *__restrict__ fp) {
+  for (long p = (long)blockIdx.x * kThreads + threadIdx.x; p < Mc; p += (long)gridDim.x * kThreads) {
+    double rho[2], g[2][3];
+    for (int sp = 0; sp < (SPIN ? 2 : 1); ++sp) {
+      const double *S = sp ? st2 : st;
+      rho[sp] = S[S_RHO * npad + p];
+      g[sp][0] = GGA ? S[S_GX * npad + p] : 0.0;
+      g[sp][1] = GGA ? S[S_GY * npad + p] : 0.0;
+      g[sp][2] = GGA ? S[S_GZ * npad + p] : 0.0;
+    }
+    double f = 0.0, xr = 0.0, xg = 0.0;
+    if (!SPIN) {
+      const double gam = g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2];
+      if (mask & (1u << 1)) xcf::xalpha(rho[0], f, xr);
+      if (mask & (1u << 3)) xcf::pwldac(rho[0], f, xr);
+      if (GGA && (mask & (1u << 10))) xcf::pbex(rho[0], gam, f, xr, xg);
+      if (GGA && (mask & (1u << 18))) xcf::pbec(rho[0], gam, f, xr, xg);
+    } else {
+      double gss[2];
+      for (int sp = 0; sp < 2; ++sp)
+        gss[sp] = g[sp][0] * g[sp][0] + g[sp][1] * g[sp][1] + g[sp][2] * g[sp][2];
+      const double gab = g[0][0] * g[1][0] + g[0][1] * g[1][1] + g[0][2] * g[1][2];
+      for (int sp = 0; sp < 2; ++sp) {
+        double x = 0.0;
+        if (mask & (1u << 1)) xcf::xalpha(2.0 * rho[sp], x, xr);
+        if (GGA && (mask & (1u << 10))) xcf::pbex(2.0 * rho[sp], 4.0 * gss[sp], x, xr, xg);
+        f += 0.5 * x;
+      }
+      const xcs::Dv lda = xcs::pw92_spin(xcs::cst(rho[0]), xcs::cst(rho[1]));
+      if (mask & (1u << 3)) f += lda.v;
+      if (GGA && (mask & (1u << 18)))
+        f += xcs::pbec_spin(xcs::cst(rho[0]), xcs::cst(rho[1]),
+                            xcs::cst(gss[0] + gss[1] + 2.0 * gab), lda).v;
+    }
+    fp[p] = f;
+  }
+}
+
 // -------------------------------------------------------------------- W
 // item = (1024-point block, 16-function group), 4 consecutive points of one
 // 32-point group per thread: W = a Phi + b . grad Phi.  SPLIT (split-BF16
@@ -1311,6 +1357,14 @@ void launch_functional(hipStream_t s, bool gga, const StageList &L) {
     hipLaunchKernelGGL((k_functional<true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
   else
     hipLaunchKernelGGL((k_functional<false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
+void launch_point_f(hipStream_t s, bool gga, bool spin, uint32_t mask, long Mc, long npad,
+                    const double *st, const double *st2, double *fp) {
+  const unsigned g = (unsigned)std::min<long>((Mc + kThreads - 1) / kThreads, 256L * 8);
+  auto k = spin ? (gga ? k_point_f<true, true> : k_point_f<false, true>)
+                : (gga ? k_point_f<true, false> : k_point_f<false, false>);
+  hipLaunchKernelGGL(k, dim3(g), dim3(kThreads), 0, s, Mc, npad, mask, st, st2, fp);
 }
 
 void launch_energy(hipStream_t s, bool gga, bool spin, const StageList &L) {

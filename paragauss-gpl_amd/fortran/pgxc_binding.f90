@@ -23,7 +23,7 @@ module pgxc_binding
   integer(c_int32_t), parameter, public :: PGXC_DENSITY_FULL = 0, PGXC_DENSITY_INCREMENTAL = 1
 
   public :: pgxc_create, pgxc_set_density, pgxc_set_orbitals, pgxc_build, pgxc_get_result
-  public :: pgxc_energy, pgxc_gradient
+  public :: pgxc_energy, pgxc_gradient, pgxc_gradient_grid
   public :: pgxc_nbas, pgxc_last_error, pgxc_destroy, pgxc_last_timing, pgxc_last_stats
   public :: pgxc_becke_weights, pgxc_create_opt, pgxc_layout, pgxc_options
   public :: pgxc_batch_create, pgxc_batch_set_priority, pgxc_batch_build, pgxc_batch_last_ms
@@ -103,6 +103,14 @@ module pgxc_binding
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: grad(*)
      end function pgxc_gradient
+
+     ! dE_xc/dR with the grid moving with the atoms (weight_grads on)
+     integer(c_int) function pgxc_gradient_grid(ctx, owner, grad) bind(C, name="pgxc_gradient_grid")
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: owner(*)
+       real(c_double), intent(out) :: grad(*)
+     end function pgxc_gradient_grid
 
      integer(c_int64_t) function pgxc_nbas(ctx) bind(C, name="pgxc_nbas")
        import :: c_int64_t, c_ptr

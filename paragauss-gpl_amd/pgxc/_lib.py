@@ -15,7 +15,7 @@ ERRORS = {1: "PGXC_ERR_ARG", 2: "PGXC_ERR_UNSUPPORTED", 3: "PGXC_ERR_HIP",
 
 # header symbols, checked by the tests against include/pgxc.h
 EXPORTS = ("pgxc_create", "pgxc_create_opt", "pgxc_layout", "pgxc_set_density",
-           "pgxc_set_orbitals", "pgxc_build", "pgxc_get_result", "pgxc_energy", "pgxc_gradient",
+           "pgxc_set_orbitals", "pgxc_build", "pgxc_get_result", "pgxc_energy", "pgxc_gradient", "pgxc_gradient_grid",
            "pgxc_nbas",
            "pgxc_last_stats",
            "pgxc_last_timing", "pgxc_last_error", "pgxc_destroy", "pgxc_becke_weights",
@@ -76,6 +76,7 @@ def load():
     L.pgxc_get_result.argtypes = [_vp, _dp, _dp, _dp]
     L.pgxc_energy.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int32, _dp, _dp]
     L.pgxc_gradient.argtypes = [_vp, _dp]
+    L.pgxc_gradient_grid.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), _dp]
     L.pgxc_nbas.argtypes = [_vp]
     L.pgxc_nbas.restype = ctypes.c_int64
     L.pgxc_last_stats.argtypes = [_vp, _dp]

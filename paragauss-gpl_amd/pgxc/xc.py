@@ -126,6 +126,17 @@ class XCBuilder:
         self._check(self._L.pgxc_gradient(self._h, _ptr(g)))
         return g
 
+    def gradient_grid(self, owner):
+        """dE_xc/dR (n_atoms, 3) with the grid moving with the atoms: owner[p]
+        is the atom whose atomic grid holds point p (pgxc_gradient_grid)."""
+        own = np.ascontiguousarray(owner, dtype=np.int32)
+        if own.shape != (self.npts,):
+            raise ValueError("owner must have one entry per grid point")
+        g = np.zeros((self.basis.n_atoms, 3))
+        self._check(self._L.pgxc_gradient_grid(
+            self._h, own.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _ptr(g)))
+        return g
+
     def result(self):
         ham = np.zeros(self.ispin * self.nbas * (self.nbas + 1) // 2)
         e = np.zeros(1)

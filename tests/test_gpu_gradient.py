@@ -114,3 +114,53 @@ def test_gradient_and_energy_need_a_build(water):
             xc.gradient()
         terms, q = xc.energy(PBE, FP64)  # the density suffices for energies
         assert q > 0
+
+
+def water_grid(xyz, nrad=30, nang=302):
+    els, _ = systems.h2o()
+    return grid.molecular_grid(els, xyz, nrad=nrad, nang=nang, partition="numpy")
+
+
+@pytest.mark.parametrize("functional,mask,ispin", [(PBE, oracle.MASK_PBE, 1),
+                                                   (LDA, oracle.MASK_LDA, 1),
+                                                   (PBE, oracle.MASK_PBE, 2)],
+                         ids=["PBE", "LDA", "PBE-open-shell"])
+def test_gradient_with_moving_grid(functional, mask, ispin):
+    """weight_grads on: every atom's grid moves with it and the Becke weights
+    follow.  Against Richardson-extrapolated central differences of the
+    oracle's E_xc on grids regenerated at the displaced geometry; and the sum
+    over atoms vanishes (a rigid translation leaves E_xc unchanged)."""
+    els, xyz = systems.h2o()
+    b = basis.seeded_basis(els, xyz, seed=1, shell_set="small")
+    g = water_grid(xyz)
+    # atoms' grids in order, each truncated where the radial grid ends
+    sizes = [len(grid.atomic_grid(c, 30, 302, grid.RADIUS[e])[0]) for e, c in zip(els, xyz)]
+    owner = np.repeat(np.arange(len(els)), sizes)
+    assert owner.shape[0] == g.shape[0]
+    C, occ = systems.random_orbitals(b.nbas, 5, seed=2)
+    if ispin == 2:
+        rng = np.random.default_rng(4)
+        Q, _ = np.linalg.qr(rng.normal(size=(b.nbas, 6)))
+        C = np.stack([Q[:, :5], Q[:, 1:6]], axis=2)
+        occ = np.stack([np.ones(5), np.array([1.0, 1.0, 0.8, 0.5, 0.0])], axis=1)
+    dens = dict(C=C, occ=occ) if ispin == 1 else dict(C=C, occ=occ, ispin=2)
+    with XCBuilder(b, g, chunk_points=8192) as xc:
+        xc.set_orbitals(C, occ)
+        xc.build(functional, FP64)
+        gr = xc.gradient_grid(owner)
+
+    def central(A, d, h):
+        e = []
+        for s in (1, -1):
+            x2 = np.array(xyz, dtype=np.float64, copy=True)
+            x2[A, d] += s * h
+            e.append(oracle.xc_build(dataclasses.replace(b, atom_xyz=x2), water_grid(x2), mask,
+                                     nthreads=NT, **dens)[1])
+        return (e[0] - e[1]) / (2 * h)
+    fd = np.zeros_like(gr)
+    for A in range(len(els)):
+        for d in range(3):
+            fd[A, d] = (4 * central(A, d, H / 2) - central(A, d, H)) / 3
+    print("sum over atoms", gr.sum(0), flush=True)
+    check(gr, fd)
+    assert np.max(np.abs(gr.sum(0))) <= 1e-7 * np.max(np.abs(gr)) + 1e-9
