@@ -1,7 +1,7 @@
 """GPU parity: the HIP build (through the C ABI) against the FP64 oracle on
 identical inputs.
 
-Tolerances (north-star bar for the plain FP64 path):
+Tolerances (the bar for the plain FP64 path):
   |dE_xc| <= 1e-9 Eh, max |dV_xc| <= 1e-10, |dcharge| <= 1e-9.
 Mixed precision: a contraction in FP32 carries a relative error ~1e-6 after
 accumulation, split-BF16 ~2^-16 per product (a_lo b_lo dropped, lo rounded).
