@@ -389,6 +389,45 @@ def bench_open_shell(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_post_scf(args):
+    """(H2O)32 after an FP64 build: post-SCF energies per term (FP64 and
+    split-BF16), the basis-centre XC gradient and the full gradient with the
+    grid moving with the atoms."""
+    import numpy as np
+    from pgxc import XCBuilder, grid
+    from pgxc.xc import BF16X3, FP64, PBE
+    els, b, g, C, occ = workload("h2o32")
+    sizes = [len(grid.atomic_grid(c, 75, 302, grid.RADIUS[e])[0]) for e, c in zip(els, b.atom_xyz)]
+    owner = np.repeat(np.arange(len(els)), sizes)
+    xc = XCBuilder(b, g, tile=96)
+    xc.set_orbitals(C, occ)
+    xc.build(PBE, FP64, fetch=False)
+    out = {}
+    for prec, name in ((FP64, "fp64"), (BF16X3, "bf16x3")):
+        t = []
+        for _ in range(args.warmup + args.steps):
+            t0 = time.time()
+            xc.energy(PBE, prec)
+            t.append(time.time() - t0)
+        out["energy_%s_ms" % name] = 1e3 * min(t[args.warmup:])
+    xc.build(PBE, FP64, fetch=False)
+    t0 = time.time()
+    g1 = xc.gradient()
+    out["gradient_s"] = time.time() - t0
+    t0 = time.time()
+    g2 = xc.gradient_grid(owner)
+    out["gradient_grid_s"] = time.time() - t0
+    xc.close()
+    out["gradient_grid_translation_residual"] = float(np.abs(g2.sum(0)).max())
+    out["gradient_max"] = float(np.abs(g2).max())
+    log("post-SCF:", out)
+    print(json.dumps({"metric": "post-SCF XC energies and gradients, (H2O)32",
+                      "value": out["gradient_grid_s"], "unit": "s per full gradient",
+                      "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": "(H2O)32, N = %d, %d grid points, PBE" % (b.nbas, g.shape[0])},
+                      **out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -397,7 +436,8 @@ def main():
     ap.add_argument("--workload", default="h2o32",
                     help="h2o32 (headline), h2o, batch64 (64-molecule batch), "
                          "learned ((H2O)128 + solute, cost-model schedules), "
-                         "h2o32-open ((H2O)32 open shell)")
+                         "h2o32-open ((H2O)32 open shell), "
+                         "post-scf ((H2O)32 energies per term and XC gradients)")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16x3"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -420,6 +460,8 @@ def main():
         return bench_batch64(args)
     if args.workload == "h2o32-open":
         return bench_open_shell(args)
+    if args.workload == "post-scf":
+        return bench_post_scf(args)
 
     import numpy as np
     from pgxc import XCBuilder
