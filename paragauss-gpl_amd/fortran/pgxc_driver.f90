@@ -8,7 +8,9 @@
 !        int32 shell_info(5,n_shells); int32 n_exp_total, n_contr_total;
 !        real64 exps(n_exp_total), norms(n_exp_total), contr(n_contr_total),
 !        grdpts(npts,4), eigvec_occ(nbas,nocc,ispin), occ_num_occ(nocc,ispin)
-! output: real64 exc_int, charge_int, ham_xc_arr(ispin*nbas*(nbas+1)/2)
+! output: real64 exc_int, charge_int, ham_xc_arr(ispin*nbas*(nbas+1)/2), then
+!         the post-SCF energy terms(4) and charge of the same density and the
+!         basis-centre gradient dE_xc/dR(3, n_atoms)
 !
 program pgxc_driver
   use iso_c_binding
@@ -21,7 +23,8 @@ program pgxc_driver
   real(c_double), allocatable :: atom_xyz(:, :), exps(:), norms(:), contr(:), grdpts(:, :)
   real(c_double), allocatable :: eigvec_occ(:, :, :), occ(:, :), ham(:)
   integer(c_int32_t), allocatable :: shell_info(:, :)
-  real(c_double) :: exc_int, charge_int
+  real(c_double) :: exc_int, charge_int, terms(4), charge_e
+  real(c_double), allocatable :: grad(:, :)
   integer :: u
 
   call get_command_argument(1, fin)
@@ -50,10 +53,14 @@ program pgxc_driver
   charge_int = 0.0_c_double
   call xc_gpu_set_orbitals(eigvec_occ, occ)
   call xc_gpu_build(mask, precision, ham, exc_int, charge_int)
+  ! post-SCF: energies per term and the gradient at this density
+  allocate(grad(3, n_atoms))
+  call xc_gpu_energy(mask, precision, terms, charge_e)
+  call xc_gpu_gradient(grad)
   call xc_gpu_close()
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write(u) exc_int, charge_int, ham
+  write(u) exc_int, charge_int, ham, terms, charge_e, grad
   close(u)
   write(*, '(a, es24.16, a, es24.16)') 'exc_int = ', exc_int, '  charge_int = ', charge_int
 end program pgxc_driver

@@ -19,7 +19,7 @@ module xc_gpu
   private
 
   public :: xc_gpu_setup, xc_gpu_set_orbitals, xc_gpu_set_density, xc_gpu_build, &
-       xc_gpu_close, xc_gpu_mask, xc_gpu_status
+       xc_gpu_close, xc_gpu_mask, xc_gpu_status, xc_gpu_energy, xc_gpu_gradient
 
   ! closed shell: eigvec_occ(nbas, nocc), occ_num_occ(nocc); open or closed
   ! shell as ParaGauss holds them: eigvec_occ(nbas, nocc, ispin),
@@ -150,6 +150,33 @@ contains
     rc = pgxc_build(ctx, mask, precision, ham_xc_arr, exc_int, charge_int)
     if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_build', rc)
   end subroutine xc_gpu_build
+
+  ! post_scf_calc_xc_energy: energy of each term (XAlpha, PWLDAc, PBEx, PBEc)
+  ! on the current density, without the potential
+  subroutine xc_gpu_energy(mask, precision, terms, charge)
+    integer(c_uint32_t), intent(in) :: mask
+    integer(c_int32_t), intent(in) :: precision
+    real(c_double), intent(out) :: terms(4), charge
+    integer(c_int) :: rc
+    rc = pgxc_energy(ctx, int(mask, c_int32_t), precision, terms, charge)
+    if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_energy', rc)
+  end subroutine xc_gpu_energy
+
+  ! dE_xc/dR (3, n_atoms) at the last build: with owner (the atom of each
+  ! grid point's atomic grid, 0-based) the grid moves with the atoms
+  ! (weight_grads on), without it the grid is held (grad_xc = -grad)
+  subroutine xc_gpu_gradient(grad, owner)
+    real(c_double), intent(out) :: grad(:, :)
+    integer(c_int32_t), intent(in), optional :: owner(:)
+    integer(c_int) :: rc
+    if (present(owner)) then
+       rc = pgxc_gradient_grid(ctx, owner, grad)
+       if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_gradient_grid', rc)
+    else
+       rc = pgxc_gradient(ctx, grad)
+       if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_gradient', rc)
+    endif
+  end subroutine xc_gpu_gradient
 
   subroutine xc_gpu_close()
     integer(c_int) :: rc

@@ -191,5 +191,14 @@ def test_fortran_driver_matches(h2o, tmp_path):
     print(out.stdout, out.stderr, flush=True)
     assert out.returncode == 0
     raw = np.fromfile(fout, dtype=np.float64)
-    e, q, ham = raw[0], raw[1], raw[2:]
+    npk = b.nbas * (b.nbas + 1) // 2
+    e, q, ham = raw[0], raw[1], raw[2:2 + npk]
+    terms, qe, grad = raw[2 + npk:6 + npk], raw[6 + npk], raw[7 + npk:].reshape(b.n_atoms, 3)
     check((ham, e, q), oracle_result("h2o_pbe", b, g, oracle.MASK_PBE, C=C, occ=occ))
+    # the Fortran post-SCF calls: energy terms of the same density, and the
+    # gradient the Python binding gives for the same build
+    assert abs(terms.sum() - e) <= 1e-12 * abs(e) and abs(qe - q) <= 1e-12 * abs(q)
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        xc.build(PBE, FP64)
+        assert np.array_equal(xc.gradient(), grad)

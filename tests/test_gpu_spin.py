@@ -194,7 +194,9 @@ def test_fortran_driver_open_shell(case, tmp_path):
     print(out.stdout, out.stderr, flush=True)
     assert out.returncode == 0
     raw = np.fromfile(fout, dtype=np.float64)
-    check((raw[2:], raw[0], raw[1]), ref(case, oracle.MASK_PBE))
+    nh = b.nbas * (b.nbas + 1)  # two spins
+    check((raw[2:2 + nh], raw[0], raw[1]), ref(case, oracle.MASK_PBE))
+    assert abs(raw[2 + nh:6 + nh].sum() - raw[0]) <= 1e-12 * abs(raw[0])
 
 
 def test_fully_polarised_and_empty_spin(case):
