@@ -637,6 +637,16 @@ def main():
                      "flops_per_build_algorithmic": stats["flops"],
                      "flops_per_build_dense": 4.0 * npts * n * n + 16.0 * npts * n},
     }
+    if traffic:
+        # the same kernel against the HBM roof: profiled bytes per dispatch
+        # over the live duration of one dispatch (the flop rate above is far
+        # below the MFMA ridge, so this is the bound that applies)
+        per_ms = st[dom] / layout["chunks"]
+        hbm = traffic / (per_ms * 1e-3) / 1e12
+        out["roofline"]["hbm"] = {"achieved": hbm, "peak": 8.0, "unit": "TB/s", "frac": hbm / 8.0,
+                                  "dispatch_ms": per_ms,
+                                  "executed_flop_per_byte": executed[best][dom] / layout["chunks"] / traffic,
+                                  "ridge_flop_per_byte": peak / 8.0}
     if args.workload == "h2o":
         # a build this small is latency bound, so only its HBM fraction is
         # reported: algorithmic bytes of the whole build (grid, P sub-blocks
