@@ -498,6 +498,21 @@ __device__ __forceinline__ void pbec(double rho, double g2, double &f, double &f
   fr += H + nv * pre * (dt2n * rat + t2 * drat * dxn);
   fg += nv * pre * (dt2g * rat + t2 * drat * dxg);
 }
+
+// The terms of mask in enum order (XAlpha, PWLDAc, PBEx, PBEc): each term's
+// F into e[0..3] (zeroed by the caller), dF/drho and dF/dgamma summed over
+// them; the functional's F is ((e0 + e1) + e2) + e3.
+template <bool GGA>
+__device__ __forceinline__ void terms(uint32_t mask, double rho, double gam, double (&e)[4],
+                                      double &fr, double &fg) {
+  if (mask & (1u << 1)) xalpha(rho, e[0], fr);
+  if (mask & (1u << 3)) pwldac(rho, e[1], fr);
+  if (GGA) {
+    if (mask & (1u << 10)) pbex(rho, gam, e[2], fr, fg);
+    if (mask & (1u << 18)) pbec(rho, gam, e[3], fr, fg);
+  }
+}
+__device__ __forceinline__ double sum4(const double (&e)[4]) { return ((e[0] + e[1]) + e[2]) + e[3]; }
 }  // namespace xcf
 
 // ============================================================ work lists
@@ -645,14 +660,9 @@ __global__ __launch_bounds__(kThreads) void k_functional(const UnitD *__restrict
       }
     }
     const double gam = g0 * g0 + g1 * g1 + g2 * g2;
-    double f = 0.0, fr = 0.0, fg = 0.0;
-    const uint32_t mask = U.mask;
-    if (mask & (1u << 1)) xcf::xalpha(rho, f, fr);
-    if (mask & (1u << 3)) xcf::pwldac(rho, f, fr);
-    if (GGA) {
-      if (mask & (1u << 10)) xcf::pbex(rho, gam, f, fr, fg);
-      if (mask & (1u << 18)) xcf::pbec(rho, gam, f, fr, fg);
-    }
+    double e[4] = {0.0, 0.0, 0.0, 0.0}, fr = 0.0, fg = 0.0;
+    xcf::terms<GGA>(U.mask, rho, gam, e, fr, fg);
+    const double f = xcf::sum4(e);
     const double ww = U.gw[p];
     const double a = 0.5 * fr * ww;
     const double bq = ww * 2.0 * fg;
@@ -802,6 +812,50 @@ __device__ __forceinline__ Dv pbec_spin(Dv na, Dv nb, Dv g2, Dv lda) {
   const Dv H = fi3 * gc * logd(cst(1.0) + t2 * (beta / gc) * rat);
   return n * H;
 }
+
+// The spin-polarised terms of mask in enum order: exchange by spin scaling,
+// PW92 once (its own term and the base of PBE-c, which needs it); each
+// term's F into e[0..3], dF/drho_s into fr[s], dF/dg_ss into fgs[s] and
+// dF/dg_ab into fab (all zeroed by the caller).
+template <bool GGA>
+__device__ __forceinline__ void terms(uint32_t mask, const double (&rho)[2], const double (&gss)[2],
+                                      double gab, double (&e)[4], double (&fr)[2],
+                                      double (&fgs)[2], double &fab) {
+  if (mask & (1u << 1))
+    for (int sp = 0; sp < 2; ++sp) {
+      double x = 0.0, xr = 0.0;
+      xcf::xalpha(2.0 * rho[sp], x, xr);
+      e[0] += 0.5 * x;
+      fr[sp] += xr;
+    }
+  Dv lda = cst(0.0);
+  if (mask & (1u << 3)) {
+    lda = pw92_spin({rho[0], 1.0, 0.0, 0.0}, {rho[1], 0.0, 1.0, 0.0});
+    e[1] += lda.v;
+    fr[0] += lda.d0;
+    fr[1] += lda.d1;
+  }
+  if (GGA) {
+    if (mask & (1u << 10))
+      for (int sp = 0; sp < 2; ++sp) {
+        double x = 0.0, xr = 0.0, xg = 0.0;
+        xcf::pbex(2.0 * rho[sp], 4.0 * gss[sp], x, xr, xg);
+        e[2] += 0.5 * x;
+        fr[sp] += xr;
+        fgs[sp] += 2.0 * xg;
+      }
+    if (mask & (1u << 18)) {
+      const Dv F = pbec_spin({rho[0], 1.0, 0.0, 0.0}, {rho[1], 0.0, 1.0, 0.0},
+                             {gss[0] + gss[1] + 2.0 * gab, 0.0, 0.0, 1.0}, lda);
+      e[3] += F.v;
+      fr[0] += F.d0;
+      fr[1] += F.d1;
+      fgs[0] += F.d2;
+      fgs[1] += F.d2;
+      fab += 2.0 * F.d2;
+    }
+  }
+}
 }  // namespace xcs
 
 // item = 256-point block of an open-shell unit (the alpha spin's unit, which
@@ -855,44 +909,10 @@ __global__ __launch_bounds__(kThreads) void k_functional_spin(const UnitD *__res
     const double gaa = g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2];
     const double gbb = g[1][0] * g[1][0] + g[1][1] * g[1][1] + g[1][2] * g[1][2];
     const double gab = g[0][0] * g[1][0] + g[0][1] * g[1][1] + g[0][2] * g[1][2];
-    double f = 0.0, fr[2] = {0.0, 0.0}, fgs[2] = {0.0, 0.0}, fab = 0.0;
-    const uint32_t mask = U.mask;
     const double gss[2] = {gaa, gbb};
-    if (mask & (1u << 1))
-      for (int sp = 0; sp < 2; ++sp) {
-        double e = 0.0, er = 0.0;
-        xcf::xalpha(2.0 * rho[sp], e, er);
-        f += 0.5 * e;
-        fr[sp] += er;
-      }
-    // PW92 once: its own term, and the base of PBE-c (which needs PWLDAc)
-    xcs::Dv lda = xcs::cst(0.0);
-    if (mask & (1u << 3)) {
-      lda = xcs::pw92_spin({rho[0], 1.0, 0.0, 0.0}, {rho[1], 0.0, 1.0, 0.0});
-      f += lda.v;
-      fr[0] += lda.d0;
-      fr[1] += lda.d1;
-    }
-    if (GGA) {
-      if (mask & (1u << 10))
-        for (int sp = 0; sp < 2; ++sp) {
-          double e = 0.0, er = 0.0, eg = 0.0;
-          xcf::pbex(2.0 * rho[sp], 4.0 * gss[sp], e, er, eg);
-          f += 0.5 * e;
-          fr[sp] += er;
-          fgs[sp] += 2.0 * eg;
-        }
-      if (mask & (1u << 18)) {
-        const xcs::Dv F = xcs::pbec_spin({rho[0], 1.0, 0.0, 0.0}, {rho[1], 0.0, 1.0, 0.0},
-                                         {gaa + gbb + 2.0 * gab, 0.0, 0.0, 1.0}, lda);
-        f += F.v;
-        fr[0] += F.d0;
-        fr[1] += F.d1;
-        fgs[0] += F.d2;
-        fgs[1] += F.d2;
-        fab += 2.0 * F.d2;
-      }
-    }
+    double e[4] = {0.0, 0.0, 0.0, 0.0}, fr[2] = {0.0, 0.0}, fgs[2] = {0.0, 0.0}, fab = 0.0;
+    xcs::terms<GGA>(U.mask, rho, gss, gab, e, fr, fgs, fab);
+    const double f = xcf::sum4(e);
     const double ww = U.gw[p];
     for (int sp = 0; sp < 2; ++sp) {
       const double a = 0.5 * fr[sp] * ww;
@@ -943,6 +963,23 @@ __global__ __launch_bounds__(kThreads) void k_functional_spin(const UnitD *__res
   }
 }
 
+// Each term's F at a point from rho and grad rho (per spin when SPIN)
+template <bool GGA, bool SPIN>
+__device__ __forceinline__ void point_terms(uint32_t mask, const double (&rho)[2],
+                                            const double (&g)[2][3], double (&e)[4]) {
+  if (!SPIN) {
+    const double gam = g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2];
+    double fr = 0.0, fg = 0.0;
+    xcf::terms<GGA>(mask, rho[0], gam, e, fr, fg);
+  } else {
+    const double gss[2] = {g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2],
+                           g[1][0] * g[1][0] + g[1][1] * g[1][1] + g[1][2] * g[1][2]};
+    const double gab = g[0][0] * g[1][0] + g[0][1] * g[1][1] + g[0][2] * g[1][2];
+    double fr[2] = {0.0, 0.0}, fgs[2] = {0.0, 0.0}, fab = 0.0;
+    xcs::terms<GGA>(mask, rho, gss, gab, e, fr, fgs, fab);
+  }
+}
+
 // ------------------------------------------------------- energy terms
 // item = 256-point block of a unit (SPIN: an open-shell molecule's alpha unit):
 // the XC energy of each functional term and the charge on the block, without
@@ -973,39 +1010,8 @@ __global__ __launch_bounds__(kThreads) void k_energy(const UnitD *__restrict__ u
       }
     }
     const double ww = U.gw[p];
-    const uint32_t mask = U.mask;
     double e[4] = {0.0, 0.0, 0.0, 0.0};
-    if (!SPIN) {
-      const double gam = g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2];
-      double fr = 0.0, fg = 0.0;
-      if (mask & (1u << 1)) xcf::xalpha(rho[0], e[0], fr);
-      if (mask & (1u << 3)) xcf::pwldac(rho[0], e[1], fr);
-      if (GGA) {
-        if (mask & (1u << 10)) xcf::pbex(rho[0], gam, e[2], fr, fg);
-        if (mask & (1u << 18)) xcf::pbec(rho[0], gam, e[3], fr, fg);
-      }
-    } else {
-      const double gss[2] = {g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2],
-                             g[1][0] * g[1][0] + g[1][1] * g[1][1] + g[1][2] * g[1][2]};
-      const double gab = g[0][0] * g[1][0] + g[0][1] * g[1][1] + g[0][2] * g[1][2];
-      for (int sp = 0; sp < 2; ++sp) {
-        double x = 0.0, xr = 0.0, xg = 0.0;
-        if (mask & (1u << 1)) {
-          xcf::xalpha(2.0 * rho[sp], x, xr);
-          e[0] += 0.5 * x;
-        }
-        if (GGA && (mask & (1u << 10))) {
-          x = 0.0;
-          xcf::pbex(2.0 * rho[sp], 4.0 * gss[sp], x, xr, xg);
-          e[2] += 0.5 * x;
-        }
-      }
-      const xcs::Dv lda = xcs::pw92_spin(xcs::cst(rho[0]), xcs::cst(rho[1]));
-      if (mask & (1u << 3)) e[1] = lda.v;
-      if (GGA && (mask & (1u << 18)))
-        e[3] = xcs::pbec_spin(xcs::cst(rho[0]), xcs::cst(rho[1]),
-                              xcs::cst(gss[0] + gss[1] + 2.0 * gab), lda).v;
-    }
+    point_terms<GGA, SPIN>(U.mask, rho, g, e);
     for (int k = 0; k < 4; ++k) sh[k][tid] = e[k] * ww;
     sh[4][tid] = (rho[0] + rho[1]) * ww;
     __syncthreads();
@@ -1029,38 +1035,16 @@ __global__ __launch_bounds__(kThreads) void k_point_f(long Mc, long npad, uint32
                                                       const double *__restrict__ st2,
                                                       double *__restrict__ fp) {
   for (long p = (long)blockIdx.x * kThreads + threadIdx.x; p < Mc; p += (long)gridDim.x * kThreads) {
-    double rho[2], g[2][3];
+    double rho[2] = {0.0, 0.0}, g[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     for (int sp = 0; sp < (SPIN ? 2 : 1); ++sp) {
       const double *S = sp ? st2 : st;
       rho[sp] = S[S_RHO * npad + p];
-      g[sp][0] = GGA ? S[S_GX * npad + p] : 0.0;
-      g[sp][1] = GGA ? S[S_GY * npad + p] : 0.0;
-      g[sp][2] = GGA ? S[S_GZ * npad + p] : 0.0;
+      if (GGA)
+        for (int c = 0; c < 3; ++c) g[sp][c] = S[(S_GX + c) * npad + p];
     }
-    double f = 0.0, xr = 0.0, xg = 0.0;
-    if (!SPIN) {
-      const double gam = g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2];
-      if (mask & (1u << 1)) xcf::xalpha(rho[0], f, xr);
-      if (mask & (1u << 3)) xcf::pwldac(rho[0], f, xr);
-      if (GGA && (mask & (1u << 10))) xcf::pbex(rho[0], gam, f, xr, xg);
-      if (GGA && (mask & (1u << 18))) xcf::pbec(rho[0], gam, f, xr, xg);
-    } else {
-      double gss[2];
-      for (int sp = 0; sp < 2; ++sp)
-        gss[sp] = g[sp][0] * g[sp][0] + g[sp][1] * g[sp][1] + g[sp][2] * g[sp][2];
-      const double gab = g[0][0] * g[1][0] + g[0][1] * g[1][1] + g[0][2] * g[1][2];
-      for (int sp = 0; sp < 2; ++sp) {
-        double x = 0.0;
-        if (mask & (1u << 1)) xcf::xalpha(2.0 * rho[sp], x, xr);
-        if (GGA && (mask & (1u << 10))) xcf::pbex(2.0 * rho[sp], 4.0 * gss[sp], x, xr, xg);
-        f += 0.5 * x;
-      }
-      const xcs::Dv lda = xcs::pw92_spin(xcs::cst(rho[0]), xcs::cst(rho[1]));
-      if (mask & (1u << 3)) f += lda.v;
-      if (GGA && (mask & (1u << 18)))
-        f += xcs::pbec_spin(xcs::cst(rho[0]), xcs::cst(rho[1]),
-                            xcs::cst(gss[0] + gss[1] + 2.0 * gab), lda).v;
-    }
+    double e[4] = {0.0, 0.0, 0.0, 0.0};
+    point_terms<GGA, SPIN>(mask, rho, g, e);
+    const double f = xcf::sum4(e);
     fp[p] = f;
   }
 }
