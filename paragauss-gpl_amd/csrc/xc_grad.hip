@@ -189,6 +189,27 @@ __global__ __launch_bounds__(kThreads) void k_basis_hess(
   }
 }
 
+// The gradient bracket of function i at point p (q = i Mc + p) added into t:
+//   t_d += grad_d phi_i (a2 X_i + b . Y_i) + X_i sum_c b_c H_dc phi_i
+template <bool GGA>
+__device__ __forceinline__ void grad_bracket(const double *__restrict__ XY,
+                                             const double *__restrict__ phi,
+                                             const double *__restrict__ hess, long cs, long q,
+                                             double a2, const double (&b)[3], double (&t)[3]) {
+  const double X = XY[q];
+  double s = a2 * X;
+  if (GGA) s += b[0] * XY[cs + q] + b[1] * XY[2 * cs + q] + b[2] * XY[3 * cs + q];
+  for (int d = 0; d < 3; ++d) {
+    double u = phi[(1 + d) * cs + q] * s;
+    if (GGA) {
+      double hb = 0.0;
+      for (int c = 0; c < 3; ++c) hb += b[c] * hess[hidx(c, d) * cs + q];
+      u += X * hb;
+    }
+    t[d] += u;
+  }
+}
+
 // item = (function i, segment of kGradSeg points): the segment's sum of
 //   t_d = grad_d phi_i (2 a X_i + b . Y_i) + X_i sum_c b_c H_dc phi_i
 // into part[i][seg][3].  XY = [4][NP][Mc]: X, then Y_x, Y_y, Y_z; phi the
@@ -210,24 +231,8 @@ __global__ __launch_bounds__(kThreads) void k_grad_fn(int NP, long Mc, int nseg,
     const long i = it / nseg, seg = it % nseg;
     double t[3] = {0.0, 0.0, 0.0};
     for (long p = seg * kGradSeg + tid; p < (seg + 1) * kGradSeg; p += kThreads) {
-      const long q = i * Mc + p;
-      const double X = XY[q], a2 = 2.0 * a[p];
-      double s = a2 * X, b[3] = {0.0, 0.0, 0.0};
-      if (GGA) {
-        b[0] = bx[p];
-        b[1] = by[p];
-        b[2] = bz[p];
-        s += b[0] * XY[cs + q] + b[1] * XY[2 * cs + q] + b[2] * XY[3 * cs + q];
-      }
-      for (int d = 0; d < 3; ++d) {
-        double u = phi[(1 + d) * cs + q] * s;
-        if (GGA) {
-          double hb = 0.0;
-          for (int c = 0; c < 3; ++c) hb += b[c] * hess[hidx(c, d) * cs + q];
-          u += X * hb;
-        }
-        t[d] += u;
-      }
+      const double b[3] = {GGA ? bx[p] : 0.0, GGA ? by[p] : 0.0, GGA ? bz[p] : 0.0};
+      grad_bracket<GGA>(XY, phi, hess, cs, i * Mc + p, 2.0 * a[p], b, t);
     }
     for (int d = 0; d < 3; ++d) sh[d][tid] = t[d];
     __syncthreads();
@@ -263,21 +268,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_pt(int NP, long Mc, const dou
       b[2] = bz[p];
     }
     double t[3] = {0.0, 0.0, 0.0};
-    for (int i = 0; i < NP; ++i) {
-      const long q = (long)i * Mc + p;
-      const double X = XY[q];
-      double s = a2 * X;
-      if (GGA) s += b[0] * XY[cs + q] + b[1] * XY[2 * cs + q] + b[2] * XY[3 * cs + q];
-      for (int d = 0; d < 3; ++d) {
-        double u = phi[(1 + d) * cs + q] * s;
-        if (GGA) {
-          double hb = 0.0;
-          for (int c = 0; c < 3; ++c) hb += b[c] * hess[hidx(c, d) * cs + q];
-          u += X * hb;
-        }
-        t[d] += u;
-      }
-    }
+    for (int i = 0; i < NP; ++i) grad_bracket<GGA>(XY, phi, hess, cs, (long)i * Mc + p, a2, b, t);
     for (int d = 0; d < 3; ++d) pm[d * pstride + p] += 2.0 * t[d];
   }
 }
