@@ -114,6 +114,13 @@ def test_gradient_and_energy_need_a_build(water):
             xc.gradient()
         terms, q = xc.energy(PBE, FP64)  # the density suffices for energies
         assert q > 0
+        xc.build(PBE, FP64)
+        bad = np.zeros(g.shape[0], dtype=np.int32)
+        bad[7] = b.n_atoms  # an atom index out of range
+        with pytest.raises(_lib.PgxcError):
+            xc.gradient_grid(bad)
+        with pytest.raises(ValueError):
+            xc.gradient_grid(bad[:-1])
 
 
 def water_grid(xyz, nrad=30, nang=302):
