@@ -175,7 +175,7 @@ constexpr int kLists = 3 * kNumVariants;
 
 struct StepPlan {
   int nunits = 0;
-  StageList basis, func, func2, wmat, split;  // func2: open-shell units
+  StageList basis, func, func2, wmat, wmat2, split;  // func2, wmat2: open-shell units
   StageList dens[kLists], pot[kLists];
 };
 
@@ -400,7 +400,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     }
     return P;
   };
-  // per step: basis, func, wmat, dens lists, pot lists, split, func2
+  // per step: basis, func, wmat, dens lists, pot lists, split, func2, wmat2
   std::vector<std::vector<Pending>> pend(nsteps);
   for (int k = 0; k < nsteps; ++k) {
     std::vector<UnitD> units;
@@ -467,6 +467,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
         U.part2 = B.part;
         U.coef2 = B.coef;
         U.state2 = B.state;
+        U.W2 = B.W;
         units.push_back(U);
         umol.push_back(m);
         uspin.push_back(0);
@@ -504,7 +505,12 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       pick(0, [](const pgxc_ctx *c) { return c->ispin == 1; }, fu, fm);
       pend[k].push_back(stage(fu, fm, K_FUNC, 0, k));
     }
-    pend[k].push_back(stage(units, umol, K_WMAT, 0, k));
+    {  // closed-shell units; the open-shell ones in wmat2 below
+      std::vector<UnitD> wu;
+      std::vector<int> wm;
+      pick(0, [](const pgxc_ctx *c) { return c->ispin == 1; }, wu, wm);
+      pend[k].push_back(stage(wu, wm, K_WMAT, 0, k));
+    }
     const int cls = precision == PGXC_PREC_FP64 ? 0 : 1;
     for (int kind = 0; kind < 2; ++kind)
       for (int li = 0; li < kLists; ++li) {
@@ -531,6 +537,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       std::vector<int> fm;
       pick(0, [](const pgxc_ctx *c) { return c->ispin == 2; }, fu, fm);
       pend[k].push_back(stage(fu, fm, K_FUNC, 0, k));
+      pend[k].push_back(stage(fu, fm, K_WMAT, 0, k));  // both spins' W per alpha unit
     }
   }
   // queue bounds: each list's positions (natural order, so neighbouring items
@@ -649,6 +656,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     for (int li = 0; li < kLists; ++li) S.pot[li] = mk(pend[k][3 + kLists + li]);
     S.split = mk(pend[k][3 + 2 * kLists]);
     S.func2 = mk(pend[k][4 + 2 * kLists]);
+    S.wmat2 = mk(pend[k][5 + 2 * kLists]);
     S.nunits = pend[k][0].nu;
   }
   for (int m = 0; m < nm; ++m) plan->nlanes = std::max(plan->nlanes, mols[m]->nlanes);
@@ -740,7 +748,8 @@ static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent
     }
     launch_functional(t, gga, S.func);
     launch_functional_spin(t, gga, S.func2);
-    launch_wmat<T>(t, gga, x3, S.wmat);
+    launch_wmat<T>(t, gga, x3, false, S.wmat);
+    launch_wmat<T>(t, gga, x3, true, S.wmat2);
     if (E) (void)hipEventRecord(E[3], t);
     for (int li = 0; li < kLists; ++li) {
       const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;

@@ -51,6 +51,7 @@ struct UnitD {
   // reads both); nullptr otherwise
   const double *part2;
   double *coef2, *state2;
+  void *W2;         // and the beta spin's W, written by the same W pass
   double *eterm;    // [Mc/256][5] block sums of the energy terms and the charge
   double *Hpart;    // [nsplit][NP][NP] potential partials of the molecule
   // Work of the contractions, over gathered runs of 8 functions (all runs
@@ -121,8 +122,10 @@ void launch_functional(hipStream_t s, bool gga, const StageList &L);
 void launch_functional_spin(hipStream_t s, bool gga, const StageList &L);
 // energy terms only (no coefficients, state untouched); spin: open-shell units
 void launch_energy(hipStream_t s, bool gga, bool spin, const StageList &L);
+// two: open-shell alpha units, whose W pass writes both spins' W (W, W2 from
+// coef, coef2) from one read of the basis values
 template <typename T>
-void launch_wmat(hipStream_t s, bool grad, bool split, const StageList &L);
+void launch_wmat(hipStream_t s, bool grad, bool split, bool two, const StageList &L);
 // split-BF16 operand planes of the basis values (item = 32 functions x 64 points)
 void launch_split(hipStream_t s, const StageList &L);
 template <typename T>

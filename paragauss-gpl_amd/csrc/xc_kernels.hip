@@ -1055,12 +1055,13 @@ __global__ __launch_bounds__(kThreads) void k_point_f(long Mc, long npad, uint32
 // builds, T = float): W stored as bf16 hi / lo planes.  A run of 8 functions
 // cut off on the group is 0 without reading Phi, and is not written at all
 // where the potential contraction never reads it.
-template <typename T, bool GRAD, bool SPLIT>
+template <typename T, bool GRAD, bool SPLIT, bool TWO>
 __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ units, int nu,
                                                    const long *__restrict__ base,
                                                    const int *__restrict__ order, long total) {
   typedef T t4 __attribute__((ext_vector_type(4)));
   typedef uint16_t h4 __attribute__((ext_vector_type(4)));
+  constexpr int NS = TWO ? 2 : 1;  // spins written from one read of the basis values
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const long it = item_at(order, L, total);
     const int u = find_unit(base, nu, it);
@@ -1071,10 +1072,11 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
     const long f0 = (local % ng) * 16;
     const long Mc = U.Mc, cs = (long)U.NP * Mc;
     const T *phi = (const T *)U.phi;
-    T *W = (T *)U.W;
-    uint16_t *Wh = (uint16_t *)U.W, *Wl = Wh + cs;
-    auto put = [&](long q, const double (&v)[4]) {
+    void *Wout[2] = {U.W, TWO ? U.W2 : U.W};
+    const double *cf[2] = {U.coef, TWO ? U.coef2 : U.coef};
+    auto put = [&](int sp, long q, const double (&v)[4]) {
       if constexpr (SPLIT) {
+        uint16_t *Wh = (uint16_t *)Wout[sp], *Wl = Wh + cs;
         h4 hi, lo;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -1089,7 +1091,7 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
         t4 w;
 #pragma unroll
         for (int j = 0; j < 4; ++j) w[j] = (T)v[j];
-        gst((t4 *)(W + q), w);
+        gst((t4 *)((T *)Wout[sp] + q), w);
       }
     };
     const uint32_t *sg = U.sig8 + (p / kGroup) * U.nw8, *wn = U.wbits + (p / kGroup) * U.nw8;
@@ -1101,37 +1103,44 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
         if (gld(wn + (r >> 5)) & bit) {
           const double z[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
-          for (int k = 0; k < kRun; ++k) put((fr + k) * Mc + p, z);
+          for (int k = 0; k < kRun; ++k)
+#pragma unroll
+            for (int sp = 0; sp < NS; ++sp) put(sp, (fr + k) * Mc + p, z);
         }
         continue;
       }
-      double a[4], b0[4] = {}, b1[4] = {}, b2[4] = {};
+      double a[NS][4], b0[NS][4] = {}, b1[NS][4] = {}, b2[NS][4] = {};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        a[j] = gld(U.coef + p + j);
-        if (GRAD) {
-          b0[j] = gld(U.coef + Mc + p + j);
-          b1[j] = gld(U.coef + 2 * Mc + p + j);
-          b2[j] = gld(U.coef + 3 * Mc + p + j);
+      for (int sp = 0; sp < NS; ++sp)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          a[sp][j] = gld(cf[sp] + p + j);
+          if (GRAD) {
+            b0[sp][j] = gld(cf[sp] + Mc + p + j);
+            b1[sp][j] = gld(cf[sp] + 2 * Mc + p + j);
+            b2[sp][j] = gld(cf[sp] + 3 * Mc + p + j);
+          }
         }
-      }
 #pragma unroll 4
       for (int k = 0; k < kRun; ++k) {
         const long q = (fr + k) * Mc + p;
         const t4 v0 = gld((const t4 *)(phi + q));
-        double v[4];
+        t4 v1 = {}, v2 = {}, v3 = {};
         if (GRAD) {
-          const t4 v1 = gld((const t4 *)(phi + cs + q)), v2 = gld((const t4 *)(phi + 2 * cs + q)),
-                   v3 = gld((const t4 *)(phi + 3 * cs + q));
+          v1 = gld((const t4 *)(phi + cs + q));
+          v2 = gld((const t4 *)(phi + 2 * cs + q));
+          v3 = gld((const t4 *)(phi + 3 * cs + q));
+        }
+#pragma unroll
+        for (int sp = 0; sp < NS; ++sp) {
+          double v[4];
 #pragma unroll
           for (int j = 0; j < 4; ++j)
-            v[j] = a[j] * (double)v0[j] + b0[j] * (double)v1[j] + b1[j] * (double)v2[j] +
-                   b2[j] * (double)v3[j];
-        } else {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) v[j] = a[j] * (double)v0[j];
+            v[j] = GRAD ? a[sp][j] * (double)v0[j] + b0[sp][j] * (double)v1[j] +
+                              b1[sp][j] * (double)v2[j] + b2[sp][j] * (double)v3[j]
+                        : a[sp][j] * (double)v0[j];
+          put(sp, q, v);
         }
-        put(q, v);
       }
     }
   }
@@ -1369,22 +1378,34 @@ void launch_functional_spin(hipStream_t s, bool gga, const StageList &L) {
 }
 
 template <typename T>
-void launch_wmat(hipStream_t s, bool grad, bool split, const StageList &L) {
+void launch_wmat(hipStream_t s, bool grad, bool split, bool two, const StageList &L) {
   if (!L.total) return;
   dim3 g(persistent_grid(L.total, 8));
+#define PGXC_WMAT(SPLIT)                                                                           \
+  {                                                                                                \
+    if (two) {                                                                                     \
+      if (grad)                                                                                    \
+        hipLaunchKernelGGL((k_wmat<T, true, SPLIT, true>), g, dim3(kThreads), 0, s, L.units, L.nu, \
+                           L.base, L.order, L.total);                                              \
+      else                                                                                         \
+        hipLaunchKernelGGL((k_wmat<T, false, SPLIT, true>), g, dim3(kThreads), 0, s, L.units,     \
+                           L.nu, L.base, L.order, L.total);                                        \
+    } else if (grad) {                                                                             \
+      hipLaunchKernelGGL((k_wmat<T, true, SPLIT, false>), g, dim3(kThreads), 0, s, L.units, L.nu,  \
+                         L.base, L.order, L.total);                                                \
+    } else {                                                                                       \
+      hipLaunchKernelGGL((k_wmat<T, false, SPLIT, false>), g, dim3(kThreads), 0, s, L.units, L.nu, \
+                         L.base, L.order, L.total);                                                \
+    }                                                                                              \
+  }
   if constexpr (std::is_same<T, float>::value) {
     if (split) {
-      if (grad)
-        hipLaunchKernelGGL((k_wmat<T, true, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
-      else
-        hipLaunchKernelGGL((k_wmat<T, false, true>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+      PGXC_WMAT(true)
       return;
     }
   }
-  if (grad)
-    hipLaunchKernelGGL((k_wmat<T, true, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
-  else
-    hipLaunchKernelGGL((k_wmat<T, false, false>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  PGXC_WMAT(false)
+#undef PGXC_WMAT
 }
 
 template <typename T, int VAR>
@@ -1438,7 +1459,7 @@ void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, lon
 #define PGXC_INST(T)                                                                  \
   template void launch_basis<T>(hipStream_t, bool, const StageList &);               \
   template void launch_density<T>(hipStream_t, bool, int, int, const StageList &);   \
-  template void launch_wmat<T>(hipStream_t, bool, bool, const StageList &);          \
+  template void launch_wmat<T>(hipStream_t, bool, bool, bool, const StageList &);    \
   template void launch_potential<T>(hipStream_t, int, int, const StageList &);
 PGXC_INST(double)
 PGXC_INST(float)
