@@ -216,3 +216,21 @@ def test_fully_polarised_and_empty_spin(case):
         ham, e, q = xc.build(PBE, FP64)
     # PW92 clamps n below DEPS = 1e-30 to DEPS: a vanishing, not zero, energy
     assert abs(e) <= 1e-20 and q == 0.0 and not np.any(ham)
+
+
+def test_open_shell_water_cluster_medium_basis():
+    """(H2O)4 in the 33-function water basis, 40 x 302 grid in chunks of 16 Ki
+    points on two lanes: screening, several chunks and both spins' units in
+    every stage; FP64 against the oracle's open-shell build."""
+    els, xyz = systems.water_cluster(4, seed=3)
+    b = basis.seeded_basis(els, xyz, seed=4, shell_set="medium")
+    g = grid.molecular_grid(els, xyz, nrad=40, nang=302, partition="gpu")
+    rng = np.random.default_rng(6)
+    Q, _ = np.linalg.qr(rng.normal(size=(b.nbas, 21)))
+    C = np.stack([Q[:, :20], Q[:, 1:21]], axis=2)
+    occ = np.stack([np.ones(20), np.r_[np.ones(19), 0.0]], axis=1)
+    with XCBuilder(b, g, chunk_points=16384) as xc:
+        assert xc.layout()["chunks"] >= 3
+        xc.set_orbitals(C, occ)
+        res = xc.build(PBE, FP64)
+    check(res, oracle.xc_build(b, g, oracle.MASK_PBE, C=C, occ=occ, ispin=2, nthreads=NT))
