@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 from pgxc import XCBuilder, basis, grid, systems
-from pgxc.xc import BF16X3, FP64, LDA, PBE
+from pgxc.xc import BF16X3, FP32, FP64, LDA, PBE
 
 pytestmark = pytest.mark.gpu
 
@@ -234,3 +234,21 @@ def test_open_shell_water_cluster_medium_basis():
         xc.set_orbitals(C, occ)
         res = xc.build(PBE, FP64)
     check(res, oracle.xc_build(b, g, oracle.MASK_PBE, C=C, occ=occ, ispin=2, nthreads=NT))
+
+
+@pytest.mark.parametrize("prec", [FP32, BF16X3], ids=["FP32", "BF16X3"])
+@pytest.mark.parametrize("norm", [1e-2, 1e-4])
+def test_open_shell_incremental_mixed_error_tracks_dP(case, norm, prec):
+    """Incremental mixed-precision open-shell builds from an FP64 state: the
+    closed-shell bar, the FP64 floor plus a multiple of ||dP||."""
+    b, g, C, occ = case
+    C1 = np.stack([systems.perturbed_orbitals(C[:, :, s], norm, seed=30 + s) for s in range(2)],
+                  axis=2)
+    with XCBuilder(b, g) as xc:
+        xc.set_orbitals(C, occ)
+        xc.build(PBE, FP64)
+        xc.set_orbitals(C1, occ, incremental=True)
+        inc = xc.build(PBE, prec)
+    r = oracle.xc_build(b, g, oracle.MASK_PBE, C=C1, occ=occ, ispin=2, nthreads=NT)
+    k = 1e-3 if prec == FP32 else 5e-3
+    check(inc, r, e_tol=1e-9 + k * norm, v_tol=1e-10 + k * norm, q_tol=1e-9 + k * norm)
