@@ -346,6 +346,13 @@ def bench_open_shell(args):
     tu = None if args.no_tune else autotune.tune(b, g, C, occ, mixed=BF16X3, log=log)
     xc = XCBuilder(b, g, tile=tu["tile"] if tu else 0,
                    variants=(tu["variant_fp64"], tu["variant_fp32"]) if tu else (-1, -1))
+    from pgxc import _lib
+    from pgxc.costmodel import default_model
+    model = default_model()  # the headline's schedule: the shipped cost model's queues
+    if model is not None:
+        xc.set_schedule(_lib.SCHED_MODEL, model)
+    else:
+        xc.set_schedule(_lib.SCHED_ANALYTIC, None)
 
     def timed(prec, incremental_chain=None):
         out = []
