@@ -224,9 +224,17 @@ int pgxc_gradient_grid(pgxc_ctx *ctx, const int32_t *owner, double *grad);
 int64_t pgxc_nbas(const pgxc_ctx *ctx);
 
 /* Work counters of the last build: [0] points, [1] batches, [2] algorithmic
- * flops (4 B n^2 + 16 B n summed over batches, n = screened functions),
- * [3] exp evaluations, [4] HBM bytes of the algorithmic model. */
+ * flops (4 B n^2 + 16 B n summed over 128-point batches, n = screened
+ * functions, times ispin), [3] exp evaluations, [4] HBM bytes of the
+ * algorithmic model: the same sum of 32 B (x, y, z, w) + 16 n^2 ispin (P
+ * sub-block read, V partial written) + 4 s B n (basis values and gradients
+ * at s = 8 bytes in FP64 builds, 4 in FP32 and split-BF16 builds). */
 int pgxc_last_stats(const pgxc_ctx *ctx, double *stats5);
+
+/* The screened sizes behind those counters (no counterpart in the
+ * reference): [0] sum over 128-point batches of B n (points times screened
+ * functions), [1] sum of n^2, [2] number of batches. */
+int pgxc_last_work(const pgxc_ctx *ctx, double *work3);
 
 /* Device timing of the last build's kernels in milliseconds, by stage:
  * [0] basis, [1] density contraction, [2] functional, [3] potential
@@ -326,7 +334,10 @@ int pgxc_load_state(pgxc_ctx *ctx, const void *buf, int64_t bytes);
  * its own work buffers, so one chunk's stages overlap the next chunk's.
  * lanes = 1 runs every chunk on one stream (each launch then has the device
  * to itself, which is how per-kernel timings are measured); 2 restores the
- * default.  Results agree to rounding (the H partials are summed per lane). */
+ * default.  Results agree to rounding (the H partials are summed per lane).
+ * A batch runs two lanes only when every member of more than one chunk has
+ * two; otherwise all of its members run on one lane, and each member's
+ * results are the bits of its own build at that lane count. */
 int pgxc_set_lanes(pgxc_ctx *ctx, int32_t lanes);
 
 /* on = 1: later builds record, per contraction work item, its features and

@@ -63,6 +63,17 @@ def lib():
                            ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp,
                            ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
             fn.restype = ctypes.c_int
+        L.or_slater_radius.argtypes = [ctypes.c_int]
+        L.or_slater_radius.restype = ctypes.c_double
+        L.or_radial_grid.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.or_radial_grid.restype = ctypes.c_int
+        L.or_lebedev.argtypes = [ctypes.c_int, _dp]
+        L.or_lebedev.restype = ctypes.c_int
+        L.or_atomic_grid.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp,
+                                     ctypes.c_long, _dp]
+        L.or_atomic_grid.restype = ctypes.c_long
+        L.or_becke0.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_long, ctypes.c_long,
+                                _dp, _dp, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -263,3 +274,61 @@ def xc_build(basis, grid, mask, P=None, C=None, occ=None, veclen=128,
     if rc != 0:
         raise ValueError("or_xc_build failed: %d" % rc)
     return ham, float(e[0]), float(q[0])
+
+
+# ----------------------------------------------------------------------- grid
+ZNUM = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Ne": 10}
+ICGRD = {194: 231, 302: 291}  # NANG of the reference's Lebedev rules
+
+
+def slater_radius(el):
+    return lib().or_slater_radius(ZNUM[el])
+
+
+def radial_grid(nradpts, R):
+    r, w = np.zeros(nradpts), np.zeros(nradpts)
+    n = lib().or_radial_grid(nradpts, R, _d(r), _d(w))
+    return r[:n], w[:n]
+
+
+def lebedev(nang):
+    """(points (n,3), weights (n,)) in INIGRD's order."""
+    p = np.zeros(4 * nang)
+    n = lib().or_lebedev(ICGRD[nang], _d(p))
+    if n != nang:
+        raise ValueError("or_lebedev: %d points" % n)
+    p = p.reshape(4, nang).T
+    return np.ascontiguousarray(p[:, :3]), np.ascontiguousarray(p[:, 3])
+
+
+def atomic_grid(nradpts, R, nang, centre):
+    cap = nradpts * nang
+    out = np.zeros(4 * cap)
+    n = lib().or_atomic_grid(nradpts, R, ICGRD[nang], _d(_c(centre)), cap, _d(out))
+    if n < 0:
+        raise ValueError("or_atomic_grid failed")
+    g = out.reshape(4, cap).T[:n]
+    return np.ascontiguousarray(g[:, :3]), np.ascontiguousarray(g[:, 3])
+
+
+def becke0(atom_xyz, host, pts, nthreads=1):
+    """Partition weight of atom host at the points (npts, 3)."""
+    A = _c(atom_xyz)
+    X = np.asfortranarray(pts, dtype=np.float64)
+    w = np.zeros(X.shape[0])
+    lib().or_becke0(A.shape[0], _d(A), host, X.shape[0], X.shape[0], X.ctypes.data_as(_dp),
+                    _d(w), nthreads)
+    return w
+
+
+def molecular_grid(elements, atom_xyz, nrad, nang, nthreads=1):
+    """The reference's linear grid grdpts(npts, 4) (mkgrid,
+    modules/grid_module.f90:3045-3164): per atom in order, Slater radius R,
+    int(nrad R) Pople shells times the Lebedev rule, weights times becke0."""
+    parts = []
+    for a, (el, c) in enumerate(zip(elements, np.asarray(atom_xyz))):
+        R = slater_radius(el)
+        x, w = atomic_grid(int(nrad * R), R, nang, c)
+        w = w * becke0(atom_xyz, a, x, nthreads) * 1  # times n_equal_atoms = 1 (C1)
+        parts.append(np.column_stack([x, w]))
+    return np.asfortranarray(np.vstack(parts))

@@ -29,6 +29,12 @@
  *   or_pbec            modules/pbe_ggcxc_module.f90:96-212 (c_gga), :273-301
  *   or_functional      modules/xc_func.f90:187-305 (term loop, frac = 1),
  *                      term order by modules/xc_cntrl.f90:61-104 enum
+ *   or_slater_radius   modules/grid_module.f90:1029-1079 (rslater_f, Z <= 10)
+ *   or_radial_grid     modules/grid_module.f90:1246-1301 (RADIALGRID)
+ *   or_lebedev         modules/grid_module.f90:1359-2100 (INIGRD, 194 and 302)
+ *   or_atomic_grid     modules/grid_module.f90:407-433 (product grid)
+ *   or_becke0          modules/grid_module.f90:2442-2497 (becke0, art = 0) and
+ *                      autodiff/becke_step_func.f90:84-124 (mu0)
  *   or_xc_build        modules/xc_hamiltonian.f90:1280-1580 (GGA),
  *                      :812-1036 (LDA: integrate, tristore),
  *                      modules/density_calc_module.f90:1478-1577 (density),
@@ -48,6 +54,13 @@
  *                 (exchange.f90 cannot be compiled standalone: it uses vwnc,
  *                 which pulls in the SCF state modules.)
  *   or_basis_eval brute-force Python evaluation from the pinned pieces.
+ *   or_radial_grid  w = r^2 dr/di of the continuous map r(i) (central
+ *                 differences); Gaussian radial moments in closed form.
+ *   or_lebedev    unit points, octahedral orbits, exactness on monomials to
+ *                 the rule's degree, agreement with the exact rules solved
+ *                 independently (pgxc/lebedev.py solve()) to the table's digits.
+ *   or_becke0     partition of unity; weight 1 at the host nucleus; a
+ *                 brute-force product over all ordered pairs.
  *   or_xc_build   brute-force numpy einsum of the same sums on tiny inputs;
  *                 P-form == occupied-orbital form; DGEMM arithmetic is the
  *                 textbook triple loop ("parity unpinned" against the
@@ -932,4 +945,235 @@ int or_xc_build_spin(int n_atoms, const double *atom_xyz, int n_shells,
   return xc_build_impl(2, n_atoms, atom_xyz, n_shells, shell_info, exps, norms, contr, grid,
                        npts, veclen, mask, dform, dens, nocc, occ, ham, exc, charge, batch_lo,
                        batch_hi, nthreads);
+}
+
+/* ------------------------------------------------------------------------ */
+/* the integration grid (modules/grid_module.f90)                            */
+/* ------------------------------------------------------------------------ */
+
+/* Slater radius in bohr, Z = 1..10 (rslater_f :1029-1079: angstrom values of
+ * :1051-1054 times angstrom = 1/bohr, modules/constants.f90:105-108) */
+double or_slater_radius(int z)
+{
+  static const double ra[11] = {0.0, 0.53, 0.25, 1.45, 1.05, 0.85, 0.70, 0.65, 0.60, 0.50, 0.50};
+  const double angstrom = 1.0 / 0.52917706;
+  if (z < 1 || z > 10) return -1.0;
+  return ra[z] * angstrom;
+}
+
+/* RADIALGRID (:1246-1301), Pople grid: the largest i <= nradpts with
+ * R (i/(n+1-i))^2 < 40 bounds the shells; r, w hold nradpts entries;
+ * returns the number of shells */
+int or_radial_grid(int nradpts, double rcur, double *r, double *w)
+{
+  const double rad_limit = 40.0, rhn = nradpts + 1;
+  int i;
+  double rr = 0.0;
+  for (i = nradpts; i >= 1; --i) {
+    const double rhni = rhn - i, x = i / rhni;
+    rr = rcur * (x * x);
+    if (rr < rad_limit) break;
+  }
+  const int n = i;
+  for (int k = n; k >= 1; --k) {
+    const double rhni = rhn - k, x = k / rhni, x2 = x * x;
+    r[k - 1] = rcur * x2;
+    w[k - 1] = 2 * (rcur * rcur * rcur) * rhn * (x2 * x2 * x) / (rhni * rhni);
+  }
+  return n;
+}
+
+/* INIGRD orbit generators (:1826-2098); each appends its points with
+ * weight wt to p (column-major n x 4: x, y, z, w) at row *np */
+static void lb_add(double *p, int ntot, int *np, double x, double y, double z, double wt)
+{
+  p[*np] = x;
+  p[ntot + *np] = y;
+  p[2 * ntot + *np] = z;
+  p[3 * ntot + *np] = wt;
+  ++*np;
+}
+
+static void lb_a1(double *p, int nt, int *np, double wt)
+{
+  lb_add(p, nt, np, 0, 0, 1, wt); lb_add(p, nt, np, 0, 0, -1, wt);
+  lb_add(p, nt, np, 0, 1, 0, wt); lb_add(p, nt, np, 0, -1, 0, wt);
+  lb_add(p, nt, np, 1, 0, 0, wt); lb_add(p, nt, np, -1, 0, 0, wt);
+}
+
+static void lb_a2(double *p, int nt, int *np, double wt)
+{
+  const double s = 1.0 / sqrt(2.0);
+  const double sg[4][2] = {{s, s}, {s, -s}, {-s, s}, {-s, -s}};
+  for (int k = 0; k < 4; ++k) lb_add(p, nt, np, sg[k][0], sg[k][1], 0, wt);
+  for (int k = 0; k < 4; ++k) lb_add(p, nt, np, sg[k][0], 0, sg[k][1], wt);
+  for (int k = 0; k < 4; ++k) lb_add(p, nt, np, 0, sg[k][0], sg[k][1], wt);
+}
+
+static void lb_a3(double *p, int nt, int *np, double wt)
+{
+  const double t = 1.0 / sqrt(3.0);
+  for (int i = -1; i <= 1; i += 2)
+    for (int j = -1; j <= 1; j += 2)
+      for (int k = -1; k <= 1; k += 2) lb_add(p, nt, np, i * t, j * t, k * t, wt);
+}
+
+static void lb_b(double *p, int nt, int *np, double m, double wt)
+{
+  const double l = sqrt((1.0 - m * m) / 2.0);
+  for (int v = 0; v < 3; ++v)  /* N1, N2, N3 blocks of 8 */
+    for (int i = -1; i <= 1; i += 2)
+      for (int j = -1; j <= 1; j += 2)
+        for (int k = -1; k <= 1; k += 2) {
+          const double a = v == 2 ? m : l, b = v == 1 ? m : l, c = v == 0 ? m : l;
+          lb_add(p, nt, np, i * a, j * b, k * c, wt);
+        }
+}
+
+static void lb_c(double *p, int nt, int *np, double vv, double wt)
+{
+  const double d = sqrt(1.0 - 4.0 * vv);
+  const double p1 = sqrt((1.0 + d) / 2.0), q1 = sqrt((1.0 - d) / 2.0);
+  const double pr[2][2] = {{p1, q1}, {q1, p1}};
+  const double sg[4][2] = {{1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+  for (int h = 0; h < 2; ++h) {
+    const double a = pr[h][0], b = pr[h][1];
+    for (int k = 0; k < 4; ++k) lb_add(p, nt, np, sg[k][0] * a, sg[k][1] * b, 0, wt);
+    for (int k = 0; k < 4; ++k) lb_add(p, nt, np, sg[k][0] * a, 0, sg[k][1] * b, wt);
+    for (int k = 0; k < 4; ++k) lb_add(p, nt, np, 0, sg[k][0] * a, sg[k][1] * b, wt);
+  }
+}
+
+static void lb_d(double *p, int nt, int *np, double u, double w, double wt)
+{
+  const double r = sqrt(1.0 - u * u - w * w);
+  const int first = *np;
+  for (int v = 0; v < 3; ++v)
+    for (int i = -1; i <= 1; i += 2)
+      for (int j = -1; j <= 1; j += 2)
+        for (int k = -1; k <= 1; k += 2) {
+          const double a = v == 0 ? u : v == 1 ? w : r;
+          const double b = v == 0 ? w : v == 1 ? r : u;
+          const double c = v == 0 ? r : v == 1 ? u : w;
+          lb_add(p, nt, np, i * a, j * b, k * c, wt);
+        }
+  for (int q = 0; q < 24; ++q)  /* DK(25:48) = DK(1:24) with y and z swapped */
+    lb_add(p, nt, np, p[first + q], p[2 * nt + first + q], p[nt + first + q], wt);
+}
+
+/* INIGRD (:1359-2100) for ICGRD = 231 (L = 23, 194 points, :1698-1751) and
+ * 291 (L = 29, 302 points, :1753-1814); returns the number of points, or
+ * -1 for another ICGRD.  p: n x 4 column-major (x, y, z, weight). */
+int or_lebedev(int icgrd, double *p)
+{
+  int np = 0;
+  if (icgrd == 231) {
+    const int nt = 194;
+    lb_a1(p, nt, &np, 1.78234044724E-03);
+    lb_a2(p, nt, &np, 5.71690594998E-03);
+    lb_a3(p, nt, &np, 5.57338317884E-03);
+    lb_b(p, nt, &np, 0.777493219315, 5.51877146727E-03);
+    lb_b(p, nt, &np, 0.912509096867, 5.15823771181E-03);
+    lb_b(p, nt, &np, 0.314196994183, 5.60870408259E-03);
+    lb_b(p, nt, &np, 0.982972302707, 4.10677702817E-03);
+    lb_c(p, nt, &np, 2.00 / 19.0, 5.05184606462E-03);
+    lb_d(p, nt, &np, 0.159041710538, 0.525118572443, 5.53024891623E-03);
+    return np;
+  }
+  if (icgrd == 291) {
+    const int nt = 302;
+    lb_a1(p, nt, &np, 8.54591172878E-04);
+    lb_a3(p, nt, &np, 3.59911928502E-03);
+    lb_b(p, nt, &np, 0.129238672710, 3.65004580768E-03);
+    lb_b(p, nt, &np, 0.371034178385, 3.60482260142E-03);
+    lb_b(p, nt, &np, 0.743452042987, 3.57672966173E-03);
+    lb_b(p, nt, &np, 0.867643624544, 3.44978842429E-03);
+    lb_b(p, nt, &np, 0.949454317226, 3.10895312238E-03);
+    lb_b(p, nt, &np, 0.990705621379, 2.35210141366E-03);
+    lb_c(p, nt, &np, 0.220093335298, 3.60082093222E-03);
+    lb_c(p, nt, &np, 0.650272754659E-01, 2.98234496317E-03);
+    lb_d(p, nt, &np, 0.800072749407, 0.544867737258, 3.57154055427E-03);
+    lb_d(p, nt, &np, 0.902442529533, 0.412772408317, 3.39231220501E-03);
+    return np;
+  }
+  return -1;
+}
+
+/* One atomic grid (grid_main_calc :407-433, no pruning): shells of
+ * RADIALGRID times INIGRD's points, positions ANG * r + centre, weights
+ * w_ang * w_rad * 16 atan(1).  xyzw: (cap x 4) column-major; returns the
+ * number of points, or -1 if cap is too small or icgrd unknown. */
+long or_atomic_grid(int nradpts, double rcur, int icgrd, const double *centre, long cap,
+                    double *xyzw)
+{
+  double ang[302 * 4];
+  const int na = or_lebedev(icgrd, ang);
+  if (na < 0) return -1;
+  double *r = malloc(sizeof(double) * (size_t)(nradpts + 1));
+  double *w = malloc(sizeof(double) * (size_t)(nradpts + 1));
+  const int nr = or_radial_grid(nradpts, rcur, r, w);
+  const long n = (long)nr * na;
+  if (n > cap) { free(r); free(w); return -1; }
+  const double four_pi = 16.0 * atan(1.0);
+  for (int i = 0; i < nr; ++i)
+    for (int a = 0; a < na; ++a) {
+      const long q = (long)i * na + a;
+      for (int c = 0; c < 3; ++c) xyzw[c * cap + q] = ang[c * na + a] * r[i] + centre[c];
+      xyzw[3 * cap + q] = ang[3 * na + a] * w[i] * four_pi;
+    }
+  free(r);
+  free(w);
+  return n;
+}
+
+/* becke0 (:2442-2497) with art = 0 (weight_grads = T), mu0 of
+ * autodiff/becke_step_func.f90:84-124 at a = 0: the partition weight of
+ * host atom a (0-based) at each of npts points x (npts x 3 column-major,
+ * leading dimension ld); atm (3 x natm).  Pair distances as atomicweight2
+ * precomputes them (:2326-2333). */
+void or_becke0(int natm, const double *atm, int a, long npts, long ld, const double *x,
+               double *wout, int nthreads)
+{
+  double *dst = malloc(sizeof(double) * (size_t)natm * natm);
+  for (int j = 0; j < natm; ++j)
+    for (int i = 0; i <= j; ++i) {
+      double s = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        const double d = atm[3 * i + c] - atm[3 * j + c];
+        s += d * d;
+      }
+      dst[(size_t)i * natm + j] = dst[(size_t)j * natm + i] = sqrt(s);
+    }
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+  {
+    double *r = malloc(sizeof(double) * (size_t)natm);
+    double *p = malloc(sizeof(double) * (size_t)natm);
+#pragma omp for schedule(static)
+    for (long q = 0; q < npts; ++q) {
+      for (int i = 0; i < natm; ++i) {
+        double s = 0.0;
+        for (int c = 0; c < 3; ++c) {
+          const double v = atm[3 * i + c] - x[c * ld + q];
+          s += v * v;
+        }
+        r[i] = sqrt(s);
+        p[i] = 1.0;
+      }
+      for (int i = 0; i < natm; ++i)
+        for (int k = 0; k < i; ++k) {
+          double mu = (r[i] - r[k]) / dst[(size_t)k * natm + i];
+          mu = (mu - 0.0) / (1.0 - 0.0 * mu);
+          for (int it = 0; it < 3; ++it) mu = mu * (3.0 - mu * mu) / 2.0;
+          p[i] = p[i] * ((1 - mu) / 2);
+          p[k] = p[k] * ((1 + mu) / 2);
+        }
+      double sum = 0.0;
+      for (int i = 0; i < natm; ++i) sum += p[i];
+      wout[q] = p[a] / sum;
+    }
+    free(r);
+    free(p);
+  }
+  free(dst);
 }

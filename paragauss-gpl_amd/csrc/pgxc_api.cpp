@@ -120,6 +120,7 @@ struct pgxc_ctx {
   uint32_t *sig8 = nullptr, *wbits = nullptr;
   ScreenShell *sshells = nullptr;
   double flops_screened = 0, flops_dense = 0;  // algorithmic work of one build
+  double sum_bn = 0, sum_n2 = 0;  // sums over 128-point tiles of B n_b and n_b^2 (screened)
   // host and device copies: per tile run lists, per split run unions and
   // [group][chunk] flags, per chunk density and potential items
   std::vector<int> h_tl_off, h_su_off;
@@ -351,6 +352,15 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
   const int nm = (int)mols.size();
   int nsteps = 0;
   for (auto *c : mols) nsteps = std::max(nsteps, c->nchunks);
+  // Two lanes (step k on stream k & 1) only when every member of more than
+  // one chunk asked for two.  A member's chunk k then takes the work buffers
+  // of the lane whose stream runs it, so two chunks that share buffers never
+  // run at the same time; a member of one chunk only has lane 0.
+  int plan_lanes = 0;
+  for (auto *c : mols)
+    if (c->nchunks > 1) plan_lanes = plan_lanes ? std::min(plan_lanes, c->nlanes) : c->nlanes;
+  if (plan_lanes < 1 || nsteps < 2) plan_lanes = 1;
+  auto lanes_of = [plan_lanes](const pgxc_ctx *c) { return c->nchunks > 1 ? plan_lanes : 1; };
 
   // host image of everything that goes to the device
   std::vector<char> img;
@@ -420,7 +430,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.exps = c->exps;
       U.norms = c->norms;
       U.contr = c->contr;
-      const int lane = k % c->nlanes;  // work buffers of chunk k
+      const int lane = k % lanes_of(c);  // work buffers of chunk k: its stream's lane
       U.phi = (char *)c->phi_base + (c->phi_stride ? (size_t)k * c->phi_stride : (size_t)lane * c->phi_lane);
       if (precision != PGXC_PREC_FP64) {  // split planes after the four FP32 planes
         U.phis = (char *)U.phi + 16 * (size_t)c->NP * c->Mc;
@@ -608,10 +618,10 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       Q.fpos = c->d_fpos;
       Q.n = c->nbas;
       Q.NP = c->NP;
-      Q.nsplit = c->nsplit * c->nlanes;  // the fold sums the partials of every lane
+      Q.nsplit = c->nsplit * lanes_of(c);  // the fold sums the partials of every lane
       Q.incremental = incr[m];
       packs.push_back(Q);
-      plan->hzero.push_back({(int *)Q.Hpart, 2L * c->nlanes * c->nsplit * c->NP * c->NP});
+      plan->hzero.push_back({(int *)Q.Hpart, 2L * lanes_of(c) * c->nsplit * c->NP * c->NP});
     }
   }
   const int npack = (int)packs.size();
@@ -659,7 +669,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     S.wmat2 = mk(pend[k][5 + 2 * kLists]);
     S.nunits = pend[k][0].nu;
   }
-  for (int m = 0; m < nm; ++m) plan->nlanes = std::max(plan->nlanes, mols[m]->nlanes);
+  plan->nlanes = plan_lanes;
   plan->d_pack = (PackD *)(B + pack_off);
   plan->d_pack_base = (long *)(B + pb_off);
   plan->d_prep_base = (long *)(B + qb_off);
@@ -896,18 +906,22 @@ static int validate_build(pgxc_ctx *c, uint32_t functional, int32_t precision, i
   return PGXC_OK;
 }
 
-static void set_stats(pgxc_ctx *c) {
-  const double B = (double)c->npts, n = (double)c->nbas;
+static void set_stats(pgxc_ctx *c, int precision) {
+  const double B = (double)c->npts;
   c->stats[0] = B;
   c->stats[1] = std::ceil(B / c->veclen);
   // sum over 128-point tiles of 4 B n_b^2 + 16 B n_b, n_b = functions of the
   // shells not cut off on the tile (dense: n_b = nbas)
   c->stats[2] = (c->screen ? c->flops_screened : c->flops_dense) * c->ispin;
   c->stats[3] = B * c->exps_per_point;
-  c->stats[4] = 32.0 * B + 16.0 * n * n;
+  // sum over the same tiles of 32 B (x, y, z, w) + 16 n_b^2 per spin (P
+  // sub-block read, V partial written) + 4 s B n_b (basis values and
+  // gradients materialised at s bytes per element)
+  const double sb = precision == PGXC_PREC_FP64 ? 8.0 : 4.0;
+  c->stats[4] = 32.0 * B + 16.0 * c->sum_n2 * c->ispin + 4.0 * sb * c->sum_bn;
 }
 
-static int collect(pgxc_ctx *c, hipStream_t s, uint32_t functional) {
+static int collect(pgxc_ctx *c, hipStream_t s, uint32_t functional, int precision) {
   std::vector<double> ep(2 * (size_t)c->nblk_total);
   HIPCHK(c, hipMemcpyAsync(ep.data(), c->epart, ep.size() * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
@@ -920,7 +934,7 @@ static int collect(pgxc_ctx *c, hipStream_t s, uint32_t functional) {
   c->last_charge = q;
   c->have_state = 1;
   c->state_functional = functional;
-  set_stats(c);
+  set_stats(c, precision);
   return PGXC_OK;
 }
 
@@ -1061,10 +1075,16 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
     for (long t = 0; t < ntp; ++t) {
       const double B = (double)std::max<long>(0, std::min<long>(kScreenTile, c->npts - t * kScreenTile));
       f += 4.0 * B * nb[(size_t)t] * nb[(size_t)t] + 16.0 * B * nb[(size_t)t];
+      if (B > 0) {
+        c->sum_bn += B * nb[(size_t)t];
+        c->sum_n2 += (double)nb[(size_t)t] * nb[(size_t)t];
+      }
     }
     c->flops_screened = f;
   } else {  // every run counts on every group
     c->flops_screened = c->flops_dense;
+    c->sum_bn = (double)c->npts * n;
+    c->sum_n2 = std::ceil((double)c->npts / kScreenTile) * n * n;
     for (long g = 0; g < ngr; ++g)
       for (int r = 0; r < n8; ++r) sig[(size_t)g * nw8 + r / 32] |= 1u << (r % 32);
     HIPCHK(c, hipMemcpy(c->sig8, sig.data(), sizeof(uint32_t) * sig.size(), hipMemcpyHostToDevice));
@@ -1505,7 +1525,7 @@ int pgxc_build(pgxc_ctx *c, uint32_t functional, int32_t precision, double *ham,
   HIPCHK(c, hipEventElapsedTime(&t, c->ev[kEvStart], c->ev[kEvEnd]));
   c->timing[5] = t;
   mark_basis_cached(c, elem, precision, gga);
-  if ((rc = collect(c, s, functional))) return rc;
+  if ((rc = collect(c, s, functional, precision))) return rc;
   return pgxc_get_result(c, ham, exc, charge);
 }
 
@@ -1714,6 +1734,14 @@ int pgxc_gradient_grid(pgxc_ctx *c, const int32_t *owner, double *grad) {
 int pgxc_last_stats(const pgxc_ctx *c, double *s5) {
   if (!c || !s5) return PGXC_ERR_ARG;
   for (int i = 0; i < 5; ++i) s5[i] = c->stats[i];
+  return PGXC_OK;
+}
+
+int pgxc_last_work(const pgxc_ctx *c, double *w3) {
+  if (!c || !w3) return PGXC_ERR_ARG;
+  w3[0] = c->sum_bn;
+  w3[1] = c->sum_n2;
+  w3[2] = std::ceil((double)c->npts / kScreenTile);
   return PGXC_OK;
 }
 
@@ -2005,7 +2033,7 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
   b->last_ms = t;
   for (int m = 0; m < nm; ++m) {
     pgxc_ctx *c = b->mols[m];
-    rc = collect(c, s, functional);
+    rc = collect(c, s, functional, precision);
     if (rc) {
       b->err = c->err;
       return rc;

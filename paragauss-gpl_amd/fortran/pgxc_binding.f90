@@ -25,6 +25,7 @@ module pgxc_binding
   public :: pgxc_create, pgxc_set_density, pgxc_set_orbitals, pgxc_build, pgxc_get_result
   public :: pgxc_energy, pgxc_gradient, pgxc_gradient_grid
   public :: pgxc_nbas, pgxc_last_error, pgxc_destroy, pgxc_last_timing, pgxc_last_stats
+  public :: pgxc_last_work
   public :: pgxc_becke_weights, pgxc_create_opt, pgxc_layout, pgxc_options
   public :: pgxc_batch_create, pgxc_batch_set_priority, pgxc_batch_build, pgxc_batch_last_ms
   public :: pgxc_batch_last_error, pgxc_batch_destroy
@@ -128,6 +129,12 @@ module pgxc_binding
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: s5(5)
      end function pgxc_last_stats
+
+     integer(c_int) function pgxc_last_work(ctx, w3) bind(C, name="pgxc_last_work")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: w3(3)
+     end function pgxc_last_work
 
      type(c_ptr) function pgxc_last_error(ctx) bind(C, name="pgxc_last_error")
        import :: c_ptr

@@ -17,7 +17,7 @@ ERRORS = {1: "PGXC_ERR_ARG", 2: "PGXC_ERR_UNSUPPORTED", 3: "PGXC_ERR_HIP",
 EXPORTS = ("pgxc_create", "pgxc_create_opt", "pgxc_layout", "pgxc_set_density",
            "pgxc_set_orbitals", "pgxc_build", "pgxc_get_result", "pgxc_energy", "pgxc_gradient", "pgxc_gradient_grid",
            "pgxc_nbas",
-           "pgxc_last_stats",
+           "pgxc_last_stats", "pgxc_last_work",
            "pgxc_last_timing", "pgxc_last_error", "pgxc_destroy", "pgxc_becke_weights",
            "pgxc_batch_create", "pgxc_batch_set_priority", "pgxc_batch_build",
            "pgxc_batch_last_ms", "pgxc_batch_last_error", "pgxc_batch_destroy",
@@ -80,6 +80,7 @@ def load():
     L.pgxc_nbas.argtypes = [_vp]
     L.pgxc_nbas.restype = ctypes.c_int64
     L.pgxc_last_stats.argtypes = [_vp, _dp]
+    L.pgxc_last_work.argtypes = [_vp, _dp]
     L.pgxc_last_timing.argtypes = [_vp, _dp]
     L.pgxc_last_error.argtypes = [_vp]
     L.pgxc_last_error.restype = ctypes.c_char_p

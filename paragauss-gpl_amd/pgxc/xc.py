@@ -153,7 +153,9 @@ class XCBuilder:
     def stats(self):
         s = np.zeros(5)
         self._check(self._L.pgxc_last_stats(self._h, _ptr(s)))
-        return dict(points=s[0], batches=s[1], flops=s[2], exps=s[3], bytes=s[4])
+        w = np.zeros(3)
+        self._check(self._L.pgxc_last_work(self._h, _ptr(w)))
+        return dict(points=s[0], batches=s[1], flops=s[2], exps=s[3], bytes=s[4], bn=w[0], n2=w[1])
 
     def timing(self):
         t = np.zeros(6)

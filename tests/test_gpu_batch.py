@@ -86,3 +86,26 @@ def test_graph_replay_and_schedule_do_not_change_bits(mols, prec):
             bs[0].build(PBE, prec)  # eager build on one member between replays
     for xb in bs:
         xb.close()
+
+
+def test_batch_with_mixed_lane_counts(mols):
+    """One multi-chunk member set to one lane, another left at two: the batch
+    then runs one lane for all, each member's chunks on the buffers of the
+    stream that runs them, and every member gets the bits of its own build
+    at one lane (no two chunks of a member share buffers concurrently)."""
+    singles = []
+    for (b, g, C, occ) in mols:
+        with XCBuilder(b, g, tile=64, chunk_points=2048) as xb:
+            assert xb.layout()["chunks"] > 1
+            xb.set_lanes(1)
+            xb.set_orbitals(C, occ)
+            singles.append(xb.build(PBE, FP64))
+    bs = make_builders(mols)
+    bs[0].set_lanes(1)  # the others keep two lanes
+    with Batch(bs) as batch:
+        for graph in (False, True, True):
+            batch.build(PBE, FP64, graph=graph)
+            for r, s in zip(batch.results(), singles):
+                assert np.array_equal(r[0], s[0]) and r[1] == s[1] and r[2] == s[2]
+    for xb in bs:
+        xb.close()
