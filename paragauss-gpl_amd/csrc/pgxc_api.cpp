@@ -133,6 +133,12 @@ struct pgxc_ctx {
   long *gcf_off = nullptr;
   uint8_t *gcf = nullptr;
   uint32_t *di = nullptr, *pi = nullptr;
+  // blocks of the basis values some stage reads (see setup_screen): per
+  // group the value and gradient run words, per chunk the basis items
+  // (256-point block << 16 | atom) and split items (64-point block << 16 |
+  // 32-function block) that touch them
+  uint32_t *bneed = nullptr, *bi = nullptr, *si = nullptr;
+  std::vector<long> h_bi_off, h_si_off;
   std::vector<int> fpos;          // internal position of each caller function
   std::vector<int> fatom;         // atom of each internal function position
   std::vector<long> perm;         // caller index of each internal point
@@ -251,11 +257,11 @@ enum ItemKind { K_BASIS, K_FUNC, K_WMAT, K_DENS, K_POT, K_SPLIT };
 
 long unit_items(const UnitD &U, ItemKind k, int tf) {
   switch (k) {
-    case K_BASIS: return items_basis(U.Mc, U.n_atoms);
+    case K_BASIS: return U.bscreen ? U.nbi : items_basis(U.Mc, U.n_atoms);
     case K_FUNC: return items_functional(U.Mc);
     case K_WMAT: return items_wmat(U.Mc, U.NP);
     case K_DENS: return U.ndi;
-    case K_SPLIT: return items_split(U.Mc, U.NP);
+    case K_SPLIT: return U.bscreen ? U.nsi : items_split(U.Mc, U.NP);
     default: return U.npi;
   }
 }
@@ -313,7 +319,7 @@ static void unit_item_features(const pgxc_ctx *c, int step, ItemKind kind, int p
 static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int precision,
                     const std::vector<int> &incr, const std::vector<double> &unit_prio,
                     long order_version, const Sched &sc, Plan *&plan, pgxc_ctx *errc,
-                    int energy = 0) {
+                    int energy = 0, int dense_basis = 0) {
   const int smode = sc.model ? sc.mode : std::min(sc.mode, (int)PGXC_SCHED_ANALYTIC);
   std::vector<int> lanes, spins;
   for (auto *c : mols) {
@@ -466,6 +472,12 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.gcf = c->gcf;
       U.sig8 = c->sig8 + (p0 / kGroup) * c->nw8;
       U.wbits = c->wbits + (p0 / kGroup) * c->nw8;
+      U.bscreen = dense_basis ? 0 : 1;  // the basis stage writes the blocks read later
+      U.bneed = c->bneed + (p0 / kGroup) * 2 * c->nw8;
+      U.bitems = c->bi + c->h_bi_off[k];
+      U.sitems = c->si + c->h_si_off[k];
+      U.nbi = c->h_bi_off[k + 1] - c->h_bi_off[k];
+      U.nsi = c->h_si_off[k + 1] - c->h_si_off[k];
       if (c->ispin == 2) {  // the beta spin's unit: same points and basis values
         UnitD B = U;
         B.W = (char *)c->beta.W + (size_t)lane * c->w_lane;
@@ -503,7 +515,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       std::vector<UnitD> bu;
       std::vector<int> bm;
       for (size_t i = 0; i < units.size(); ++i)
-        if (uspin[i] == 0 && !mols[umol[i]]->skip_basis) {
+        if (uspin[i] == 0 && (dense_basis || !mols[umol[i]]->skip_basis)) {
           bu.push_back(units[i]);
           bm.push_back(umol[i]);
         }
@@ -793,6 +805,31 @@ static std::map<int, size_t> g_cache_bytes;
 // split-BF16 planes in both layouts (2 x 2 x 2 bytes)
 static size_t phi_elem_bytes(size_t elem) { return elem == 8 ? 32 : 24; }
 
+// Test switch PGXC_POISON_BASIS=1: basis buffers, when they get their first
+// contents (allocation, or the zeroing of the lane buffers at a change of
+// precision), are filled with NaN bytes wherever a build writes basis values
+// (rows of the real functions; the transposed split planes whole), so a stage
+// reading a block the basis stage left unwritten turns the results into NaN.
+static hipError_t poison_basis(const pgxc_ctx *c, void *base, size_t elem, size_t per_buf,
+                               size_t nbuf, hipStream_t s) {
+  const char *pz = getenv("PGXC_POISON_BASIS");
+  if (!pz || pz[0] != '1') return hipSuccess;
+  const size_t rows = (size_t)c->nbas * c->Mc, plane = (size_t)c->NP * c->Mc;
+  hipError_t e = hipSuccess;
+  for (size_t k = 0; k < nbuf && e == hipSuccess; ++k) {
+    char *b = (char *)base + k * per_buf;
+    for (int pl = 0; pl < 4 && e == hipSuccess; ++pl)
+      e = hipMemsetAsync(b + pl * plane * elem, 0xff, rows * elem, s);
+    if (elem == 4 && e == hipSuccess) {
+      char *hs = b + 16 * plane, *ht = hs + 4 * plane;
+      for (int pl = 0; pl < 2 && e == hipSuccess; ++pl)
+        e = hipMemsetAsync(hs + pl * plane * 2, 0xff, rows * 2, s);
+      if (e == hipSuccess) e = hipMemsetAsync(ht, 0xff, 4 * plane, s);
+    }
+  }
+  return e;
+}
+
 // grad: the build reads the gradient planes of the basis values (GGA)
 static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s, bool grad) {
   const size_t need_phi = phi_elem_bytes(elem) * (size_t)c->NP * c->Mc;
@@ -832,6 +869,7 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s, b
       c->phic_bytes[e] = need;
       held += need;
       HIPCHK(c, hipMemsetAsync(c->phic[e], 0, need, s));  // padding rows read as zero
+      HIPCHK(c, poison_basis(c, c->phic[e], elem, need_phi, c->nchunks, s));
       c->phic_valid[e] = 0;
       c->phic_grad[e] = 0;
     } else {
@@ -855,6 +893,7 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s, b
     if (c->phi_elem != elem) {
       HIPCHK(c, hipMemsetAsync(c->phi, 0, c->phi_bytes, s));
       c->phi_elem = elem;
+      HIPCHK(c, poison_basis(c, c->phi, elem, need_phi, c->nlanes_cap, s));
     }
   }
   void *base = c->phic[e] ? c->phic[e] : c->phi;
@@ -1131,6 +1170,65 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
       }
   }
   if ((rc = upload(c, &c->wbits, wneed))) return rc;
+  // Blocks of the basis values some stage reads, per 32-point group: values
+  // on the runs of its tile's list (density operand and epilogue) and on the
+  // runs of the chunks the potential contraction reads there (wneed);
+  // gradients on the tile's runs (density epilogue; the W pass reads the runs
+  // not cut off on the group, a subset).  The basis stage writes exactly
+  // these blocks and the split those of the values; no stage reads any other
+  // block of a basis-value buffer.
+  {
+    std::vector<uint32_t> need((size_t)ngr * 2 * nw8, 0u), tw((size_t)nw8);
+    for (long t = 0; t < ntp; ++t) {
+      std::fill(tw.begin(), tw.end(), 0u);
+      for (int i = c->h_tl_off[(size_t)t]; i < c->h_tl_off[(size_t)t + 1]; ++i)
+        tw[c->h_tl[(size_t)i] >> 5] |= 1u << (c->h_tl[(size_t)i] & 31);
+      for (long g = 4 * t; g < 4 * t + 4; ++g)
+        for (int w = 0; w < nw8; ++w) {
+          need[(size_t)g * 2 * nw8 + w] = tw[(size_t)w] | wneed[(size_t)g * nw8 + w];
+          need[(size_t)g * 2 * nw8 + nw8 + w] = tw[(size_t)w];
+        }
+    }
+    // runs holding each atom's functions
+    std::vector<uint32_t> amask((size_t)c->n_atoms * nw8, 0u);
+    for (const ShellD &d : sh) {
+      const int nf = (2 * d.l + 1) * (d.n_unc + d.n_contr);
+      for (int r = d.func_off / kRun; r <= (d.func_off + nf - 1) / kRun; ++r)
+        amask[(size_t)d.atom * nw8 + r / 32] |= 1u << (r % 32);
+    }
+    std::vector<uint32_t> bi, si, bm((size_t)nw8);
+    c->h_bi_off.assign(1, 0);
+    c->h_si_off.assign(1, 0);
+    const long gpb = kThreads / kGroup;  // groups per basis block
+    for (int k = 0; k < c->nchunks; ++k) {
+      const long g0 = (long)k * c->Mc / kGroup;
+      for (long b = 0; b < c->Mc / kThreads; ++b) {
+        std::fill(bm.begin(), bm.end(), 0u);
+        for (long g = g0 + b * gpb; g < g0 + (b + 1) * gpb; ++g)
+          for (int w = 0; w < nw8; ++w) bm[(size_t)w] |= need[(size_t)g * 2 * nw8 + w];
+        for (int a = 0; a < c->n_atoms; ++a) {
+          uint32_t any = 0;
+          for (int w = 0; w < nw8 && !any; ++w) any = bm[(size_t)w] & amask[(size_t)a * nw8 + w];
+          if (any) bi.push_back((uint32_t)(b << 16 | a));
+        }
+      }
+      c->h_bi_off.push_back((long)bi.size());
+      for (long pb = 0; pb < c->Mc / 64; ++pb) {
+        const long g = g0 + 2 * pb;
+        for (int fb = 0; fb < c->NP / 32; ++fb) {
+          // the four runs 4 fb .. 4 fb + 3 lie in one word
+          const uint32_t m = 0xfu << ((4 * fb) % 32);
+          const int w = 4 * fb / 32;
+          if ((need[(size_t)g * 2 * nw8 + w] | need[(size_t)(g + 1) * 2 * nw8 + w]) & m)
+            si.push_back((uint32_t)(pb << 16 | fb));
+        }
+      }
+      c->h_si_off.push_back((long)si.size());
+    }
+    if ((rc = upload(c, &c->bneed, need))) return rc;
+    if ((rc = upload(c, &c->bi, bi))) return rc;
+    if ((rc = upload(c, &c->si, si))) return rc;
+  }
   // work items per chunk
   const long tpc = c->Mc / kScreenTile;
   c->h_di_off.assign(1, 0);
@@ -1591,7 +1689,9 @@ static int gradient_impl(pgxc_ctx *c, double *grad, double *pm) {
   Plan *gp = nullptr;
   std::vector<pgxc_ctx *> mols{c};
   std::vector<int> full{0};
-  if ((rc = get_plan(mols, functional, PGXC_PREC_FP64, full, {}, -1, c->sched, gp, c))) return rc;
+  // every block of the values: the products below read all functions on all points
+  if ((rc = get_plan(mols, functional, PGXC_PREC_FP64, full, {}, -1, c->sched, gp, c, 0, 1)))
+    return rc;
   const long Mc = c->Mc, NP = c->NP, cs = NP * Mc;
   const int nseg = (int)(Mc / kGradSeg);
   double *hess = nullptr, *XY = nullptr, *part = nullptr;
@@ -1881,7 +1981,7 @@ int pgxc_destroy(pgxc_ctx *c) {
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
                   c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->su_off, c->su,
-                  c->gcf_off, c->gcf, c->di, c->pi, c->beta.P, c->beta.Pref, c->beta.D,
+                  c->gcf_off, c->gcf, c->di, c->pi, c->bneed, c->bi, c->si, c->beta.P, c->beta.Pref, c->beta.D,
                   c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
                   c->beta.Vstate, c->beta.Vout, c->eterm};
   for (void *b : bufs) dfree(b);
@@ -2095,17 +2195,14 @@ int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64_t npts, co
   if (npts == 0) return PGXC_OK;
   for (int64_t p = 0; p < npts; ++p)
     if (owner[p] < 0 || owner[p] >= n_atoms) return PGXC_ERR_ARG;
-  std::vector<double> inv((size_t)n_atoms * n_atoms, 0.0);
   for (int i = 0; i < n_atoms; ++i)
-    for (int k = 0; k < n_atoms; ++k) {
-      if (i == k) continue;
+    for (int k = 0; k < i; ++k) {
       const double dx = atom_xyz[3 * i] - atom_xyz[3 * k];
       const double dy = atom_xyz[3 * i + 1] - atom_xyz[3 * k + 1];
       const double dz = atom_xyz[3 * i + 2] - atom_xyz[3 * k + 2];
-      const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
-      if (!(d > 0.0)) return PGXC_ERR_ARG;  // coincident centres
-      inv[(size_t)i * n_atoms + k] = 1.0 / d;
+      if (!(dx * dx + dy * dy + dz * dz > 0.0)) return PGXC_ERR_ARG;  // coincident centres
     }
+  const size_t ninv = (size_t)n_atoms * n_atoms;
   double *d_a = nullptr, *d_inv = nullptr, *d_p = nullptr, *d_w = nullptr, *d_o = nullptr;
   int *d_own = nullptr;
   hipStream_t s = nullptr;
@@ -2116,19 +2213,19 @@ int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64_t npts, co
   };
   chk(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   chk(hipMalloc((void **)&d_a, 24 * (size_t)n_atoms));
-  chk(hipMalloc((void **)&d_inv, inv.size() * 8));
+  chk(hipMalloc((void **)&d_inv, ninv * 8));
   chk(hipMalloc((void **)&d_p, 24 * (size_t)npts));
   chk(hipMalloc((void **)&d_w, 8 * (size_t)npts));
   chk(hipMalloc((void **)&d_o, 8 * (size_t)npts));
   chk(hipMalloc((void **)&d_own, 4 * (size_t)npts));
   if (rc == PGXC_OK) {
     chk(hipMemcpyAsync(d_a, atom_xyz, 24 * (size_t)n_atoms, hipMemcpyHostToDevice, s));
-    chk(hipMemcpyAsync(d_inv, inv.data(), inv.size() * 8, hipMemcpyHostToDevice, s));
     chk(hipMemcpyAsync(d_p, pts, 24 * (size_t)npts, hipMemcpyHostToDevice, s));
     chk(hipMemcpyAsync(d_w, w_atomic, 8 * (size_t)npts, hipMemcpyHostToDevice, s));
     chk(hipMemcpyAsync(d_own, owner, 4 * (size_t)npts, hipMemcpyHostToDevice, s));
   }
   if (rc == PGXC_OK) {
+    launch_inv_dist(s, n_atoms, d_a, d_inv);
     launch_becke(s, n_atoms, d_a, d_inv, (long)npts, d_p, d_own, d_w, d_o);
     chk(hipGetLastError());
     chk(hipMemcpyAsync(w_out, d_o, 8 * (size_t)npts, hipMemcpyDeviceToHost, s));
@@ -2143,5 +2240,6 @@ int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64_t npts, co
   if (s) (void)hipStreamDestroy(s);
   return rc;
 }
+
 
 }  // extern "C"

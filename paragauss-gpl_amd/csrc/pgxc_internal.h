@@ -72,6 +72,14 @@ struct UnitD {
   const long *gcf_off;
   const uint8_t *gcf;
   const uint32_t *sig8, *wbits;
+  // Basis stage over the blocks the other stages read (bscreen = 1; 0: every
+  // block).  bneed: per 32-point group (from the unit's first), 2 nw8 words:
+  // the runs whose values, then the runs whose gradients a stage reads there.
+  // bitems: basis items (256-point block << 16 | atom) touching them;
+  // sitems: split items (64-point block << 16 | 32-function block).
+  const uint32_t *bneed, *bitems, *sitems;
+  long nbi, nsi;
+  int bscreen, pad_b;
   long Mc, npad;
   int n_atoms, NP, ntiles, nsplit, incremental;
   uint32_t mask;
@@ -205,6 +213,7 @@ void launch_becke_grad(hipStream_t s, int na, const AtomD *atoms, const double *
                        const double *fp, const int *owner, double *q, double *rm, double *part);
 
 // Becke partition weights of points owned by atoms (grid generation)
+void launch_inv_dist(hipStream_t s, int na, const double *axyz, double *inv);
 void launch_becke(hipStream_t s, int na, const double *axyz, const double *inv_dist, long npts,
                   const double *pts, const int *owner, const double *wat, double *wout);
 

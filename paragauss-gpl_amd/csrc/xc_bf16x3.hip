@@ -319,6 +319,8 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
 // bf16 hi / lo planes in the value layout [n][p] (B operand of the potential
 // contraction) and transposed [p][n] (A operand of the density contraction),
 // through an LDS tile so both writes run along their contiguous dimension.
+// Screened units walk the items touching blocks whose values a contraction
+// reads, and split only those (run of 8 functions x 32-point group) blocks.
 __global__ __launch_bounds__(kThreads) void k_split(const StageList S) {
   __shared__ float t[32][65];
   const int tid = threadIdx.x;
@@ -327,13 +329,28 @@ __global__ __launch_bounds__(kThreads) void k_split(const StageList S) {
     const UnitD &U = S.units[u];
     const long local = L - S.base[u];
     const int nfbk = U.NP / 32;
-    const long n0 = (local % nfbk) * 32, p0 = (local / nfbk) * 64;
+    long n0, p0;
+    if (U.bscreen) {
+      const uint32_t e = U.sitems[local];
+      n0 = (long)(e & 0xffffu) * 32;
+      p0 = (long)(e >> 16) * 64;
+    } else {
+      n0 = (local % nfbk) * 32;
+      p0 = (local / nfbk) * 64;
+    }
     const long Mc = U.Mc, NP = U.NP, pn = NP * Mc;
     const float *phi = (const float *)U.phi;
     __bf16 *sh = (__bf16 *)U.phis, *th = (__bf16 *)U.phit;
+    // whether the block of function n0 + r and point p0 + c is read
+    auto live = [&](int r, int c) -> bool {
+      if (!U.bscreen) return true;
+      const long run = (n0 + r) / kRun;
+      return (U.bneed[((p0 + c) / kGroup) * 2 * U.nw8 + (run >> 5)] >> (run & 31)) & 1u;
+    };
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int idx = tid + kThreads * e, r = idx >> 6, c = idx & 63;
+      if (!live(r, c)) continue;
       const long q = (n0 + r) * Mc + p0 + c;
       const float v = phi[q];
       t[r][c] = v;
@@ -346,6 +363,7 @@ __global__ __launch_bounds__(kThreads) void k_split(const StageList S) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int idx = tid + kThreads * e, r = idx >> 5, c = idx & 31;
+      if (!live(c, r)) continue;
       const long q = (p0 + r) * NP + n0 + c;
       __bf16 hi, lo;
       split_bf16(t[c][r], hi, lo);

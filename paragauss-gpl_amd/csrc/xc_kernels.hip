@@ -316,14 +316,14 @@ __device__ __forceinline__ void sh_eval(R x, R y, R z, R *C, R *Cx, R *Cy, R *Cz
 template <int L, bool GRAD, typename T>
 __device__ __forceinline__ void put_fn(long f0, int nf, T r0, T r1, T x, T y, T z, const T *C,
                                        const T *Cx, const T *Cy, const T *Cz, T *phi, long Mc,
-                                       long cs, int p) {
+                                       long cs, int p, bool wg) {
   constexpr int o = L * L;
 #pragma unroll
   for (int m = 0; m < 2 * L + 1; ++m) {
     const long q = (f0 + (long)m * nf) * Mc + p;
     const T sv = C[o + m];
     phi[q] = sv * r0;
-    if (GRAD) {
+    if (GRAD && wg) {
       const T hv = sv * r1;
       phi[cs + q] = x * hv + Cx[o + m] * r0;
       phi[2 * cs + q] = y * hv + Cy[o + m] * r0;
@@ -337,7 +337,7 @@ __device__ __forceinline__ void shell_emit(const ShellD &sh, const double *__res
                                            const double *__restrict__ norms,
                                            const double *__restrict__ contr, T x, T y, T z, T r2,
                                            const T *C, const T *Cx, const T *Cy, const T *Cz,
-                                           T *phi, long Mc, long cs, int p) {
+                                           T *phi, long Mc, long cs, int p, bool wg) {
   const int ne = sh.n_exp, nu = sh.n_unc, nc = sh.n_contr, nf = nu + nc;
   T rc0[kMaxContr], rc1[kMaxContr];
 #pragma unroll
@@ -350,7 +350,7 @@ __device__ __forceinline__ void shell_emit(const ShellD &sh, const double *__res
       r0 = (T)norms[sh.exp_off + e] * exp(-ar2);
       r1 = (T(-2) * alpha) * r0;
     }
-    if (e < nu) put_fn<L, GRAD>(sh.func_off + e, nf, r0, r1, x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p);
+    if (e < nu) put_fn<L, GRAD>(sh.func_off + e, nf, r0, r1, x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p, wg);
 #pragma unroll
     for (int k = 0; k < kMaxContr; ++k)
       if (k < nc) {
@@ -362,26 +362,42 @@ __device__ __forceinline__ void shell_emit(const ShellD &sh, const double *__res
 #pragma unroll
   for (int k = 0; k < kMaxContr; ++k)
     if (k < nc)
-      put_fn<L, GRAD>(sh.func_off + nu + k, nf, rc0[k], rc1[k], x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p);
+      put_fn<L, GRAD>(sh.func_off + nu + k, nf, rc0[k], rc1[k], x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p, wg);
 }
 
 template <int LA, bool GRAD, typename T>
 __device__ __forceinline__ void basis_atom(const AtomD &at, const ShellD *__restrict__ shells,
                                            const double *exps, const double *norms,
                                            const double *contr, T x, T y, T z, T *phi, long Mc,
-                                           long cs, int p) {
+                                           long cs, int p, const uint32_t *__restrict__ need,
+                                           int nw8) {
   constexpr int NLM = (LA + 1) * (LA + 1);
   T C[NLM], Cx[NLM], Cy[NLM], Cz[NLM];
   sh_eval<LA, T>(x, y, z, C, Cx, Cy, Cz);
   const T r2 = x * x + y * y + z * z;
   for (int s = at.shell_begin; s < at.shell_end; ++s) {
     const ShellD sh = shells[s];
+    // need (screened basis stage): write the shell's values if a stage reads
+    // the values of one of its runs on this point's group, its gradients if
+    // one reads their gradients; everything else stays unwritten
+    bool wg = GRAD;
+    if (need) {
+      const int r0 = sh.func_off / kRun;
+      const int r1 = (sh.func_off + (2 * sh.l + 1) * (sh.n_unc + sh.n_contr) - 1) / kRun;
+      uint32_t v = 0, g = 0;
+      for (int r = r0; r <= r1; ++r) {
+        v |= need[r >> 5] >> (r & 31);
+        g |= need[nw8 + (r >> 5)] >> (r & 31);
+      }
+      if (!(v & 1u)) continue;
+      wg = GRAD && (g & 1u);
+    }
     switch (sh.l) {
-      case 0: shell_emit<0, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p); break;
-      case 1: if constexpr (LA >= 1) shell_emit<1, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p); break;
-      case 2: if constexpr (LA >= 2) shell_emit<2, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p); break;
-      case 3: if constexpr (LA >= 3) shell_emit<3, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p); break;
-      case 4: if constexpr (LA >= 4) shell_emit<4, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p); break;
+      case 0: shell_emit<0, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
+      case 1: if constexpr (LA >= 1) shell_emit<1, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
+      case 2: if constexpr (LA >= 2) shell_emit<2, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
+      case 3: if constexpr (LA >= 3) shell_emit<3, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
+      case 4: if constexpr (LA >= 4) shell_emit<4, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
     }
   }
 }
@@ -536,7 +552,10 @@ __device__ __forceinline__ long item_at(const int *__restrict__ order, long L, l
 }
 
 // ------------------------------------------------------------------- basis
-// item = (256-point block, atom) of a unit; one thread per (point, atom)
+// item = (256-point block, atom) of a unit; one thread per (point, atom).
+// Screened units (bscreen) walk their list of the (block, atom) pairs whose
+// values a later stage reads, and write only the shells read on each
+// point's group.
 template <bool GRAD, typename T>
 __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ units, int nu,
                                                     const long *__restrict__ base,
@@ -546,19 +565,29 @@ __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ un
     const int u = find_unit(base, nu, it);
     const UnitD &U = units[u];
     const long local = it - base[u];
-    const int atom = (int)(local % U.n_atoms);
-    const long p = (local / U.n_atoms) * kThreads + threadIdx.x;
+    long blk;
+    int atom;
+    if (U.bscreen) {
+      const uint32_t e = U.bitems[local];
+      blk = e >> 16;
+      atom = (int)(e & 0xffffu);
+    } else {
+      atom = (int)(local % U.n_atoms);
+      blk = local / U.n_atoms;
+    }
+    const long p = blk * kThreads + threadIdx.x;
+    const uint32_t *need = U.bscreen ? U.bneed + (p / kGroup) * 2 * U.nw8 : nullptr;
     const AtomD at = U.atoms[atom];
     // centre offsets in FP64, then the evaluation precision
     const T x = (T)(U.gx[p] - at.x), y = (T)(U.gy[p] - at.y), z = (T)(U.gz[p] - at.z);
     T *phi = (T *)U.phi;
     const long cs = (long)U.NP * U.Mc;
     switch (at.lmax) {
-      case 0: basis_atom<0, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p); break;
-      case 1: basis_atom<1, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p); break;
-      case 2: basis_atom<2, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p); break;
-      case 3: basis_atom<3, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p); break;
-      default: basis_atom<4, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p); break;
+      case 0: basis_atom<0, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p, need, U.nw8); break;
+      case 1: basis_atom<1, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p, need, U.nw8); break;
+      case 2: basis_atom<2, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p, need, U.nw8); break;
+      case 3: basis_atom<3, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p, need, U.nw8); break;
+      default: basis_atom<4, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p, need, U.nw8); break;
     }
   }
 }
@@ -1593,6 +1622,28 @@ __global__ __launch_bounds__(kThreads) void k_becke(int na, const double *__rest
     if (i == own) pown = pi;
   }
   wout[p] = wat[p] * (pown / sum);
+}
+
+// Inverse distances of every pair of centres, inv[i][k] = 1/|R_i - R_k|
+// (0 on the diagonal), one thread per pair; the table k_becke reads.
+__global__ __launch_bounds__(kThreads) void k_inv_dist(int na, const double *__restrict__ axyz,
+                                                       double *__restrict__ inv) {
+  const long q = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= (long)na * na) return;
+  const int i = (int)(q / na), k = (int)(q % na);
+  double v = 0.0;
+  if (i != k) {
+    const double dx = axyz[3 * i] - axyz[3 * k], dy = axyz[3 * i + 1] - axyz[3 * k + 1],
+                 dz = axyz[3 * i + 2] - axyz[3 * k + 2];
+    v = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  inv[q] = v;
+}
+
+void launch_inv_dist(hipStream_t s, int na, const double *axyz, double *inv) {
+  const long n = (long)na * na;
+  hipLaunchKernelGGL(k_inv_dist, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     s, na, axyz, inv);
 }
 
 void launch_becke(hipStream_t s, int na, const double *axyz, const double *inv_dist, long npts,

@@ -243,3 +243,49 @@ def test_state_save_and_load_continue_bitwise(dimer):
     with XCBuilder(b, g[:-4096], chunk_points=8192) as xc:
         with pytest.raises(PgxcError):
             xc.load_state(blob)
+
+
+def odd_molecule():
+    """Six atoms, 75 functions: the last run of 8 is partly padding."""
+    els, xyz = systems.random_molecule(6, seed=1)
+    b = basis.seeded_basis(els, xyz, seed=1, shell_set="small")
+    g = grid.molecular_grid(els, xyz, nrad=40, nang=194)
+    C, occ = systems.random_orbitals(b.nbas, 12, seed=2)
+    return b, g, C, occ
+
+
+@pytest.mark.parametrize("system", ["dimer", "odd"])
+@pytest.mark.parametrize("cache", [False, True])
+def test_unwritten_basis_blocks_are_never_read(dimer, system, cache, monkeypatch):
+    """The basis stage writes only the (32-point group x run of 8) blocks a
+    later stage reads, the split only those of the values.  With every basis
+    buffer filled with NaN bytes first (PGXC_POISON_BASIS), LDA and GGA builds
+    in every precision, on several chunks, still match the oracle, and the
+    nuclear gradient (whose products read every block, so it evaluates the
+    basis densely) stays finite: no stage reads a block left unwritten."""
+    monkeypatch.setenv("PGXC_POISON_BASIS", "1")
+    if system == "dimer":
+        _, _, b, g, C, occ = dimer
+    else:
+        b, g, C, occ = odd_molecule()
+    refs = {fn: oracle.xc_build(b, g, mask, C=C, occ=occ, nthreads=NT)
+            for fn, mask in ((LDA, oracle.MASK_LDA), (PBE, oracle.MASK_PBE))}
+    with XCBuilder(b, g, basis_cache=cache, chunk_points=8192) as xc:
+        assert xc.layout()["chunks"] > 1
+        xc.set_orbitals(C, occ)
+        for fn in (LDA, PBE):
+            hr, er, qr = refs[fn]
+            for prec, rel in ((FP64, 0.0), (FP32, 2e-6), (BF16X3, 5e-5)):
+                ham, e, q = xc.build(fn, prec)
+                dv = np.max(np.abs(ham - hr))
+                print("%s nbas=%d cache=%s fn=%#x prec=%d dE=%.2e dV=%.2e"
+                      % (system, b.nbas, cache, fn, prec, e - er, dv), flush=True)
+                assert np.all(np.isfinite(ham)) and np.isfinite(e) and np.isfinite(q)
+                assert abs(e - er) <= 1e-9 + rel * abs(er)
+                assert dv <= 1e-10 + 10 * rel * np.max(np.abs(hr))
+                assert abs(q - qr) <= 1e-9 + rel * abs(qr)
+        xc.build(PBE, FP64)
+        gr = xc.gradient()
+        assert np.all(np.isfinite(gr))
+        again = xc.build(PBE, FP64)  # after the dense pass, the same bits
+        assert np.array_equal(again[0], xc.build(PBE, FP64)[0])
