@@ -241,6 +241,8 @@ int pgxc_last_work(const pgxc_ctx *ctx, double *work3);
  * contraction, [4] reductions/packing, [5] whole build. */
 int pgxc_last_timing(const pgxc_ctx *ctx, double *ms6);
 
+/* Message of the context's last failure; with ctx = NULL, that of the last
+ * failed context-free call (pgxc_becke_weights_noalloc). */
 const char *pgxc_last_error(const pgxc_ctx *ctx);
 
 /*
@@ -256,6 +258,27 @@ const char *pgxc_last_error(const pgxc_ctx *ctx);
 int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64_t npts,
                        const double *pts, const int32_t *owner, const double *w_atomic,
                        double *w_out);
+
+/*
+ * pgxc_becke_weights without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Every
+ * array is DEVICE memory the caller owns, in pgxc_becke_weights' layouts:
+ * atom_xyz (3 per atom), pts (npts x 3 column-major), owner, w_atomic and
+ * w_out (npts each).  The work is queued on stream (a hipStream_t of the
+ * current device; NULL: the default stream) and the call returns without
+ * waiting; w_out then holds the bits pgxc_becke_weights gives.  The table of
+ * inverse pair distances lives in a file-scope device array of 1024 x 1024
+ * doubles (8 MiB on each device that runs the call, held until the process
+ * ends): n_atoms above 1024 is refused with PGXC_ERR_ARG, and
+ * pgxc_last_error(NULL) says why.  Calls that overlap on one device share
+ * that array and race: order them on one stream.  An owner out of range
+ * gives weight 0 and coincident centres give NaN (pgxc_becke_weights reports
+ * both as PGXC_ERR_ARG).
+ */
+int pgxc_becke_weights_noalloc(int32_t n_atoms, const double *atom_xyz, int64_t npts,
+                               const double *pts, const int32_t *owner, const double *w_atomic,
+                               double *w_out, void *stream);
 
 int pgxc_destroy(pgxc_ctx *ctx);
 

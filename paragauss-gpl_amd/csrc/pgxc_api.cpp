@@ -1265,7 +1265,12 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
 
 extern "C" {
 
-const char *pgxc_last_error(const pgxc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+static char g_free_err[256];  // message of the last failed context-free call
+
+const char *pgxc_last_error(const pgxc_ctx *ctx) {
+  if (ctx) return ctx->err.c_str();
+  return g_free_err[0] ? g_free_err : "null context";
+}
 
 int64_t pgxc_nbas(const pgxc_ctx *ctx) { return ctx ? ctx->nbas : -1; }
 
@@ -2241,5 +2246,51 @@ int pgxc_becke_weights(int32_t n_atoms, const double *atom_xyz, int64_t npts, co
   return rc;
 }
 
+// ---- Becke weights without allocation: file-scope scratch -----------------
+// The table of inverse pair distances lives in `scratch`, one copy per
+// device, held until the process ends.  Host code never passes the array's
+// name to a kernel (it is a host-side stand-in there): each device's address
+// is taken once with hipGetSymbolAddress and kept in g_scratch_ptr.
+constexpr int kNoAllocMaxAtoms = 1024;  // 8 MiB of scratch per device
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) double scratch[(size_t)kNoAllocMaxAtoms * kNoAllocMaxAtoms];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kNoAllocMaxDevices];
+static int free_fail(int code, const char *msg) {
+  snprintf(g_free_err, sizeof(g_free_err), "%s", msg);
+  return code;
+}
+
+int pgxc_becke_weights_noalloc(int32_t n_atoms, const double *atom_xyz, int64_t npts,
+                               const double *pts, const int32_t *owner, const double *w_atomic,
+                               double *w_out, void *stream) {
+  if (n_atoms <= 0 || npts < 0 || !atom_xyz ||
+      (npts > 0 && (!pts || !owner || !w_atomic || !w_out)))
+    return free_fail(PGXC_ERR_ARG, "pgxc_becke_weights_noalloc: null pointer or negative size");
+  if (n_atoms > kNoAllocMaxAtoms)
+    return free_fail(PGXC_ERR_ARG, "pgxc_becke_weights_noalloc: more than 1024 atoms (the scratch "
+                                   "table holds 1024 x 1024 inverse distances)");
+  if (npts == 0) return PGXC_OK;
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices)
+    return free_fail(PGXC_ERR_HIP, "pgxc_becke_weights_noalloc: no usable current device");
+  double *ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void *a = nullptr;
+      if (hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)) != hipSuccess || !a)
+        return free_fail(PGXC_ERR_HIP, "pgxc_becke_weights_noalloc: hipGetSymbolAddress(scratch)");
+      g_scratch_ptr[dev] = (double *)a;
+    }
+    ptr = g_scratch_ptr[dev];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  launch_inv_dist(s, n_atoms, atom_xyz, ptr);  // every entry rewritten: nothing to clear
+  launch_becke(s, n_atoms, atom_xyz, ptr, (long)npts, pts, owner, w_atomic, w_out);
+  if (hipGetLastError() != hipSuccess)
+    return free_fail(PGXC_ERR_HIP, "pgxc_becke_weights_noalloc: kernel launch failed");
+  return PGXC_OK;
+}
 
 }  // extern "C"

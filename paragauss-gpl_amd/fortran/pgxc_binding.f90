@@ -27,6 +27,7 @@ module pgxc_binding
   public :: pgxc_nbas, pgxc_last_error, pgxc_destroy, pgxc_last_timing, pgxc_last_stats
   public :: pgxc_last_work
   public :: pgxc_becke_weights, pgxc_create_opt, pgxc_layout, pgxc_options
+  public :: pgxc_becke_weights_noalloc
   public :: pgxc_batch_create, pgxc_batch_set_priority, pgxc_batch_build, pgxc_batch_last_ms
   public :: pgxc_batch_last_error, pgxc_batch_destroy
   public :: pgxc_set_schedule, pgxc_batch_set_schedule, pgxc_set_lanes, pgxc_set_item_log
@@ -155,6 +156,16 @@ module pgxc_binding
        integer(c_int32_t), intent(in) :: owner(*)
        real(c_double), intent(out) :: w_out(*)
      end function pgxc_becke_weights
+
+     ! the same weights from device arrays, with no device allocation
+     ! (file-scope scratch table, at most 1024 atoms); queued on stream
+     integer(c_int) function pgxc_becke_weights_noalloc(n_atoms, atom_xyz, npts, pts, owner, &
+          w_atomic, w_out, stream) bind(C, name="pgxc_becke_weights_noalloc")
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       integer(c_int32_t), value :: n_atoms
+       integer(c_int64_t), value :: npts
+       type(c_ptr), value :: atom_xyz, pts, owner, w_atomic, w_out, stream
+     end function pgxc_becke_weights_noalloc
 
      integer(c_int) function pgxc_create_opt(ctx, n_atoms, atom_xyz, n_shells, shell_info, &
           exps, norms, contr, grid, npts, veclen, opt) bind(C, name="pgxc_create_opt")

@@ -19,6 +19,7 @@ EXPORTS = ("pgxc_create", "pgxc_create_opt", "pgxc_layout", "pgxc_set_density",
            "pgxc_nbas",
            "pgxc_last_stats", "pgxc_last_work",
            "pgxc_last_timing", "pgxc_last_error", "pgxc_destroy", "pgxc_becke_weights",
+           "pgxc_becke_weights_noalloc",
            "pgxc_batch_create", "pgxc_batch_set_priority", "pgxc_batch_build",
            "pgxc_batch_last_ms", "pgxc_batch_last_error", "pgxc_batch_destroy",
            "pgxc_set_schedule", "pgxc_batch_set_schedule", "pgxc_set_lanes", "pgxc_set_item_log",
@@ -87,6 +88,9 @@ def load():
     L.pgxc_destroy.argtypes = [_vp]
     L.pgxc_becke_weights.argtypes = [ctypes.c_int32, _dp, ctypes.c_int64, _dp, _ip, _dp, _dp]
     L.pgxc_becke_weights.restype = ctypes.c_int
+    L.pgxc_becke_weights_noalloc.argtypes = [ctypes.c_int32, _vp, ctypes.c_int64, _vp, _vp, _vp,
+                                             _vp, _vp]
+    L.pgxc_becke_weights_noalloc.restype = ctypes.c_int
     L.pgxc_set_schedule.argtypes = [_vp, ctypes.c_int32, _vp]
     L.pgxc_batch_set_schedule.argtypes = [_vp, ctypes.c_int32, _vp]
     L.pgxc_set_lanes.argtypes = [_vp, ctypes.c_int32]
