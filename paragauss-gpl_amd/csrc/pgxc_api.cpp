@@ -136,8 +136,9 @@ struct pgxc_ctx {
   // blocks of the basis values some stage reads (see setup_screen): per
   // group the value and gradient run words, per chunk the basis items
   // (256-point block << 16 | atom) and split items (64-point block << 16 |
-  // 32-function block) that touch them
-  uint32_t *bneed = nullptr, *bi = nullptr, *si = nullptr;
+  // 64-function block) that touch them
+  uint32_t *bneed = nullptr, *bi = nullptr, *bfl = nullptr;
+  uint64_t *si = nullptr;
   std::vector<long> h_bi_off, h_si_off;
   std::vector<int> fpos;          // internal position of each caller function
   std::vector<int> fatom;         // atom of each internal function position
@@ -475,6 +476,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.bscreen = dense_basis ? 0 : 1;  // the basis stage writes the blocks read later
       U.bneed = c->bneed + (p0 / kGroup) * 2 * c->nw8;
       U.bitems = c->bi + c->h_bi_off[k];
+      U.bflags = c->bfl + 8 * c->h_bi_off[k];
       U.sitems = c->si + c->h_si_off[k];
       U.nbi = c->h_bi_off[k + 1] - c->h_bi_off[k];
       U.nsi = c->h_si_off[k + 1] - c->h_si_off[k];
@@ -1189,6 +1191,12 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
           need[(size_t)g * 2 * nw8 + nw8 + w] = tw[(size_t)w];
         }
     }
+    // each atom's contiguous shell range (first, count)
+    std::vector<std::pair<int, int>> ssh((size_t)c->n_atoms, {0, 0});
+    for (int i = (int)sh.size() - 1; i >= 0; --i) {
+      ssh[(size_t)sh[(size_t)i].atom].first = i;
+      ++ssh[(size_t)sh[(size_t)i].atom].second;
+    }
     // runs holding each atom's functions
     std::vector<uint32_t> amask((size_t)c->n_atoms * nw8, 0u);
     for (const ShellD &d : sh) {
@@ -1196,7 +1204,8 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
       for (int r = d.func_off / kRun; r <= (d.func_off + nf - 1) / kRun; ++r)
         amask[(size_t)d.atom * nw8 + r / 32] |= 1u << (r % 32);
     }
-    std::vector<uint32_t> bi, si, bm((size_t)nw8);
+    std::vector<uint32_t> bi, bfl, bm((size_t)nw8);
+    std::vector<uint64_t> si;
     c->h_bi_off.assign(1, 0);
     c->h_si_off.assign(1, 0);
     const long gpb = kThreads / kGroup;  // groups per basis block
@@ -1209,25 +1218,60 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
         for (int a = 0; a < c->n_atoms; ++a) {
           uint32_t any = 0;
           for (int w = 0; w < nw8 && !any; ++w) any = bm[(size_t)w] & amask[(size_t)a * nw8 + w];
-          if (any) bi.push_back((uint32_t)(b << 16 | a));
+          if (!any) continue;
+          bi.push_back((uint32_t)(b << 16 | a));
+          // per (shell j of the atom, group g of the block): values, gradients read
+          uint32_t fw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          for (int j = 0; j < ssh[(size_t)a].second; ++j) {
+            const ShellD &d = sh[(size_t)(ssh[(size_t)a].first + j)];
+            const int nf = (2 * d.l + 1) * (d.n_unc + d.n_contr);
+            for (long gg = 0; gg < gpb; ++gg) {
+              const uint32_t *nd = need.data() + (size_t)(g0 + b * gpb + gg) * 2 * nw8;
+              uint32_t v = 0, gr = 0;
+              for (int r = d.func_off / kRun; r <= (d.func_off + nf - 1) / kRun; ++r) {
+                v |= nd[r >> 5] >> (r & 31);
+                gr |= nd[nw8 + (r >> 5)] >> (r & 31);
+              }
+              const int bit = 8 * j + (int)gg;
+              fw[bit >> 5] |= (v & 1u) << (bit & 31);
+              fw[4 + (bit >> 5)] |= (gr & 1u) << (bit & 31);
+            }
+          }
+          bfl.insert(bfl.end(), fw, fw + 8);
         }
       }
       c->h_bi_off.push_back((long)bi.size());
       for (long pb = 0; pb < c->Mc / 64; ++pb) {
         const long g = g0 + 2 * pb;
-        for (int fb = 0; fb < c->NP / 32; ++fb) {
-          // the four runs 4 fb .. 4 fb + 3 lie in one word
-          const uint32_t m = 0xfu << ((4 * fb) % 32);
-          const int w = 4 * fb / 32;
-          if ((need[(size_t)g * 2 * nw8 + w] | need[(size_t)(g + 1) * 2 * nw8 + w]) & m)
-            si.push_back((uint32_t)(pb << 16 | fb));
+        for (int fb = 0; fb < (c->NP + 63) / 64; ++fb) {
+          // the eight runs 8 fb .. 8 fb + 7 lie in one word (runs past the
+          // last function have no bits)
+          const int w = 8 * fb / 32, sh = (8 * fb) % 32;
+          const uint64_t m0 = (need[(size_t)g * 2 * nw8 + w] >> sh) & 0xffu;
+          const uint64_t m1 = (need[(size_t)(g + 1) * 2 * nw8 + w] >> sh) & 0xffu;
+          // the item's runs read on each of its two groups ride along
+          if (m0 | m1) si.push_back((m0 | m1 << 8) << 32 | (uint64_t)(pb << 16 | fb));
         }
       }
       c->h_si_off.push_back((long)si.size());
     }
     if ((rc = upload(c, &c->bneed, need))) return rc;
     if ((rc = upload(c, &c->bi, bi))) return rc;
+    if ((rc = upload(c, &c->bfl, bfl))) return rc;
     if ((rc = upload(c, &c->si, si))) return rc;
+    if (getenv("PGXC_DEBUG")) {  // sizes of the screened basis work
+      long live = 0, nv = 0, ng = 0;
+      for (long g = 0; g < ngr; ++g)
+        for (int w = 0; w < nw8; ++w) {
+          live += __builtin_popcount(sig[(size_t)g * nw8 + w]);
+          nv += __builtin_popcount(need[(size_t)g * 2 * nw8 + w]);
+          ng += __builtin_popcount(need[(size_t)g * 2 * nw8 + nw8 + w]);
+        }
+      fprintf(stderr, "pgxc: %ld groups x %d runs; blocks live %ld, values read %ld, gradients read %ld; "
+                      "basis items %zu (of %ld), split items %zu (of %ld)\n",
+              ngr, n8, live, nv, ng, bi.size(), (long)c->nchunks * (c->Mc / kThreads) * c->n_atoms,
+              si.size(), (long)c->nchunks * (c->Mc / 64) * ((c->NP + 63) / 64));
+    }
   }
   // work items per chunk
   const long tpc = c->Mc / kScreenTile;
@@ -1314,6 +1358,7 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
     at[a].shell_begin = at[a].shell_end = 0;
     at[a].lmax = 0;
     at[a].pad = 0;
+    at[a].exp_begin = at[a].n_exp = at[a].contr_begin = at[a].n_contr = 0;
   }
   long eoff = 0, coff = 0, foff = 0;
   int prev_atom = -1;
@@ -1352,6 +1397,25 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
     eoff += d.n_exp;
     coff += (long)d.n_exp * d.n_contr;
     foff += (long)(2 * d.l + 1) * (d.n_unc + d.n_contr);
+  }
+  for (int a = 0; a < n_atoms; ++a) {
+    long np = 0, nc = 0;
+    for (int s = at[a].shell_begin; s < at[a].shell_end; ++s) {
+      np += sh[s].n_exp;
+      nc += (long)sh[s].n_exp * sh[s].n_contr;
+    }
+    if (at[a].shell_end > at[a].shell_begin) {
+      const ShellD &f = sh[at[a].shell_begin];
+      at[a].exp_begin = f.exp_off;
+      at[a].contr_begin = f.contr_off;
+    }
+    at[a].n_exp = (int)np;
+    at[a].n_contr = (int)nc;
+    if (at[a].shell_end - at[a].shell_begin > kAtomShells || np > kAtomPrims || nc > kAtomContr)
+      return fail(c, PGXC_ERR_UNSUPPORTED,
+                  "pgxc_create: atom " + std::to_string(a) + " has more than " +
+                      std::to_string(kAtomShells) + " shells, " + std::to_string(kAtomPrims) +
+                      " primitives or " + std::to_string(kAtomContr) + " contraction coefficients");
   }
   for (long e = 0; e < eoff; ++e)
     if (!(exps[e] > 0.0) || !std::isfinite(norms[e]))
@@ -1986,7 +2050,7 @@ int pgxc_destroy(pgxc_ctx *c) {
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
                   c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->su_off, c->su,
-                  c->gcf_off, c->gcf, c->di, c->pi, c->bneed, c->bi, c->si, c->beta.P, c->beta.Pref, c->beta.D,
+                  c->gcf_off, c->gcf, c->di, c->pi, c->bneed, c->bi, c->bfl, c->si, c->beta.P, c->beta.Pref, c->beta.D,
                   c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
                   c->beta.Vstate, c->beta.Vout, c->eterm};
   for (void *b : bufs) dfree(b);

@@ -12,6 +12,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kLMax = 4;      // highest angular momentum of a shell
 constexpr int kMaxContr = 8;  // contracted functions per shell
+// per atom, what the basis kernel stages in LDS: shells, primitives and
+// contraction coefficients
+constexpr int kAtomShells = 16, kAtomPrims = 256, kAtomContr = 1024;
 constexpr int kThreads = 256; // 4 wave64 per workgroup everywhere
 constexpr double kFar = 1.0e6;  // coordinate of padding points (all exps cut off)
 
@@ -20,10 +23,13 @@ struct ShellD {
   int atom, l, n_exp, n_unc, n_contr, exp_off, contr_off, func_off;
 };
 
-// One centre with the contiguous shell range it carries.
+// One centre with the contiguous shell range it carries, and the contiguous
+// ranges of its shells' primitives (exp_begin, n_exp) and contraction
+// coefficients (contr_begin, n_contr).
 struct AtomD {
   double x, y, z;
   int shell_begin, shell_end, lmax, pad;
+  int exp_begin, n_exp, contr_begin, n_contr;
 };
 
 // Per-point state kept across builds (incremental mode), SoA of length npad.
@@ -75,9 +81,14 @@ struct UnitD {
   // Basis stage over the blocks the other stages read (bscreen = 1; 0: every
   // block).  bneed: per 32-point group (from the unit's first), 2 nw8 words:
   // the runs whose values, then the runs whose gradients a stage reads there.
-  // bitems: basis items (256-point block << 16 | atom) touching them;
-  // sitems: split items (64-point block << 16 | 32-function block).
-  const uint32_t *bneed, *bitems, *sitems;
+  // bitems: basis items (256-point block << 16 | atom) touching them, with
+  // bflags: 8 words per item, bit 8 j + g of words 0-3 (4-7): the values
+  // (gradients) of the atom's shell j are read on group g of the block;
+  // sitems: split items (64-point block << 16 | 64-function block), the
+  // item's eight runs read on its first group in bits 32-39, on its second
+  // in bits 40-47.
+  const uint32_t *bneed, *bitems, *bflags;
+  const uint64_t *sitems;
   long nbi, nsi;
   int bscreen, pad_b;
   long Mc, npad;
@@ -156,7 +167,7 @@ void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, lon
 inline long items_basis(long Mc, int n_atoms) { return (Mc / kThreads) * n_atoms; }
 inline long items_functional(long Mc) { return Mc / kThreads; }
 inline long items_wmat(long Mc, int NP) { return (Mc / (4 * kThreads)) * (NP / 16); }
-inline long items_split(long Mc, int NP) { return (Mc / 64) * (NP / 32); }
+inline long items_split(long Mc, int NP) { return (Mc / 64) * ((NP + 63) / 64); }
 // point splits of the potential contraction are whole multiples of this
 constexpr int kSplitPts = 2048;
 

@@ -315,60 +315,82 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
 }
 
 // ------------------------------------------------------------------ split
-// item = (32 functions, 64 points) of a unit: the FP32 basis values become
+// item = (64 functions, 64 points) of a unit: the FP32 basis values become
 // bf16 hi / lo planes in the value layout [n][p] (B operand of the potential
 // contraction) and transposed [p][n] (A operand of the density contraction),
-// through an LDS tile so both writes run along their contiguous dimension.
+// through an LDS tile so both writes run along their contiguous dimension,
+// 16 bytes per store (a transposed row segment of 64 functions is 128 bytes).
 // Screened units walk the items touching blocks whose values a contraction
 // reads, and split only those (run of 8 functions x 32-point group) blocks.
 __global__ __launch_bounds__(kThreads) void k_split(const StageList S) {
-  __shared__ float t[32][65];
+  typedef uint16_t u8v __attribute__((ext_vector_type(8)));
+  __shared__ float t[64][65];
   const int tid = threadIdx.x;
   for (long L = blockIdx.x; L < S.total; L += gridDim.x) {
     const int u = find_unit_x3(S.base, S.nu, L);
     const UnitD &U = S.units[u];
     const long local = L - S.base[u];
-    const int nfbk = U.NP / 32;
+    const int nfbk = (U.NP + 63) / 64;
     long n0, p0;
+    uint32_t lm = 0xffffu;  // bit 8 g + run: the run's values are read on group g
     if (U.bscreen) {
-      const uint32_t e = U.sitems[local];
-      n0 = (long)(e & 0xffffu) * 32;
-      p0 = (long)(e >> 16) * 64;
+      const uint64_t e = U.sitems[local];
+      n0 = (long)(e & 0xffffu) * 64;
+      p0 = (long)((e >> 16) & 0xffffu) * 64;
+      lm = (uint32_t)(e >> 32);
     } else {
-      n0 = (local % nfbk) * 32;
+      n0 = (local % nfbk) * 64;
       p0 = (local / nfbk) * 64;
     }
     const long Mc = U.Mc, NP = U.NP, pn = NP * Mc;
+    const int nfr = (int)(NP - n0 < 64 ? NP - n0 : 64);  // NP is a multiple of 32
     const float *phi = (const float *)U.phi;
-    __bf16 *sh = (__bf16 *)U.phis, *th = (__bf16 *)U.phit;
+    uint16_t *sh = (uint16_t *)U.phis, *th = (uint16_t *)U.phit;
     // whether the block of function n0 + r and point p0 + c is read
-    auto live = [&](int r, int c) -> bool {
-      if (!U.bscreen) return true;
-      const long run = (n0 + r) / kRun;
-      return (U.bneed[((p0 + c) / kGroup) * 2 * U.nw8 + (run >> 5)] >> (run & 31)) & 1u;
-    };
+    auto live = [&](int r, int c) -> bool { return (lm >> (8 * (c / kGroup) + r / kRun)) & 1u; };
+    {  // function row r, points c0 .. c0 + 15 (one group)
+      const int r = tid >> 2, c0 = (tid & 3) * 16;
+      if (r < nfr && live(r, c0)) {
+        const long q = (n0 + r) * Mc + p0 + c0;
+        u8v hi[2], lo[2];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int idx = tid + kThreads * e, r = idx >> 6, c = idx & 63;
-      if (!live(r, c)) continue;
-      const long q = (n0 + r) * Mc + p0 + c;
-      const float v = phi[q];
-      t[r][c] = v;
-      __bf16 hi, lo;
-      split_bf16(v, hi, lo);
-      sh[q] = hi;
-      sh[pn + q] = lo;
+        for (int j = 0; j < 4; ++j) {
+          const f4 v = gld((const f4 *)(phi + q + 4 * j));
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            t[r][c0 + 4 * j + e] = v[e];
+            __bf16 h, l;
+            split_bf16(v[e], h, l);
+            hi[j >> 1][4 * (j & 1) + e] = __builtin_bit_cast(uint16_t, h);
+            lo[j >> 1][4 * (j & 1) + e] = __builtin_bit_cast(uint16_t, l);
+          }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          gst((u8v *)(sh + q + 8 * h), hi[h]);
+          gst((u8v *)(sh + pn + q + 8 * h), lo[h]);
+        }
+      }
     }
     __syncthreads();
+    {  // point row pr, functions f0 .. f0 + 15 as two runs of 8
+      const int pr = tid >> 2, f0 = (tid & 3) * 16;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int idx = tid + kThreads * e, r = idx >> 5, c = idx & 31;
-      if (!live(c, r)) continue;
-      const long q = (p0 + r) * NP + n0 + c;
-      __bf16 hi, lo;
-      split_bf16(t[c][r], hi, lo);
-      th[q] = hi;
-      th[pn + q] = lo;
+      for (int h = 0; h < 2; ++h) {
+        const int fa = f0 + 8 * h;
+        if (fa >= nfr || !live(fa, pr)) continue;
+        u8v hi, lo;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          __bf16 hh, ll;
+          split_bf16(t[fa + e][pr], hh, ll);
+          hi[e] = __builtin_bit_cast(uint16_t, hh);
+          lo[e] = __builtin_bit_cast(uint16_t, ll);
+        }
+        const long q = (p0 + pr) * NP + n0 + fa;
+        gst((u8v *)(th + q), hi);
+        gst((u8v *)(th + pn + q), lo);
+      }
     }
     __syncthreads();
   }

@@ -332,29 +332,40 @@ __device__ __forceinline__ void put_fn(long f0, int nf, T r0, T r1, T x, T y, T 
   }
 }
 
+// One atom's basis data staged in LDS for a basis item: its shells, their
+// exponents, norms and contraction coefficients in the evaluation
+// precision, and per (shell, 32-point group of the item) what to write:
+// bit 0 the values, bit 1 the gradients.
+template <typename T>
+struct AtomLds {
+  ShellD sh[kAtomShells];
+  T ex[kAtomPrims], nm[kAtomPrims], ct[kAtomContr];
+  uint8_t fl[kAtomShells][kThreads / kGroup];
+};
+
+// ex, nm: the shell's primitives; ct: its n_exp x n_contr coefficients
 template <int L, bool GRAD, typename T>
-__device__ __forceinline__ void shell_emit(const ShellD &sh, const double *__restrict__ exps,
-                                           const double *__restrict__ norms,
-                                           const double *__restrict__ contr, T x, T y, T z, T r2,
-                                           const T *C, const T *Cx, const T *Cy, const T *Cz,
-                                           T *phi, long Mc, long cs, int p, bool wg) {
+__device__ __forceinline__ void shell_emit(const ShellD &sh, const T *ex, const T *nm, const T *ct,
+                                           T x, T y, T z, T r2, const T *C, const T *Cx,
+                                           const T *Cy, const T *Cz, T *phi, long Mc, long cs,
+                                           int p, bool wg) {
   const int ne = sh.n_exp, nu = sh.n_unc, nc = sh.n_contr, nf = nu + nc;
   T rc0[kMaxContr], rc1[kMaxContr];
 #pragma unroll
   for (int k = 0; k < kMaxContr; ++k) rc0[k] = rc1[k] = T(0);
   for (int e = 0; e < ne; ++e) {
-    const T alpha = (T)exps[sh.exp_off + e];
+    const T alpha = ex[e];
     const T ar2 = alpha * r2;
     T r0 = T(0), r1 = T(0);
     if (ar2 < T(50)) {  // primitive cut-off of radial1()
-      r0 = (T)norms[sh.exp_off + e] * exp(-ar2);
+      r0 = nm[e] * exp(-ar2);
       r1 = (T(-2) * alpha) * r0;
     }
     if (e < nu) put_fn<L, GRAD>(sh.func_off + e, nf, r0, r1, x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p, wg);
 #pragma unroll
     for (int k = 0; k < kMaxContr; ++k)
       if (k < nc) {
-        const T c = (T)contr[sh.contr_off + k * ne + e];
+        const T c = ct[k * ne + e];
         rc0[k] += r0 * c;
         rc1[k] += r1 * c;
       }
@@ -365,39 +376,29 @@ __device__ __forceinline__ void shell_emit(const ShellD &sh, const double *__res
       put_fn<L, GRAD>(sh.func_off + nu + k, nf, rc0[k], rc1[k], x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p, wg);
 }
 
+// The atom's shells at one point, from the LDS image A (nsh shells, first
+// primitive e0 and coefficient c0 of the atom); grp: the point's group in
+// the item.
 template <int LA, bool GRAD, typename T>
-__device__ __forceinline__ void basis_atom(const AtomD &at, const ShellD *__restrict__ shells,
-                                           const double *exps, const double *norms,
-                                           const double *contr, T x, T y, T z, T *phi, long Mc,
-                                           long cs, int p, const uint32_t *__restrict__ need,
-                                           int nw8) {
+__device__ __forceinline__ void basis_atom(const AtomLds<T> &A, int nsh, int e0, int c0, T x, T y,
+                                           T z, T *phi, long Mc, long cs, int p, int grp) {
   constexpr int NLM = (LA + 1) * (LA + 1);
   T C[NLM], Cx[NLM], Cy[NLM], Cz[NLM];
   sh_eval<LA, T>(x, y, z, C, Cx, Cy, Cz);
   const T r2 = x * x + y * y + z * z;
-  for (int s = at.shell_begin; s < at.shell_end; ++s) {
-    const ShellD sh = shells[s];
-    // need (screened basis stage): write the shell's values if a stage reads
-    // the values of one of its runs on this point's group, its gradients if
-    // one reads their gradients; everything else stays unwritten
-    bool wg = GRAD;
-    if (need) {
-      const int r0 = sh.func_off / kRun;
-      const int r1 = (sh.func_off + (2 * sh.l + 1) * (sh.n_unc + sh.n_contr) - 1) / kRun;
-      uint32_t v = 0, g = 0;
-      for (int r = r0; r <= r1; ++r) {
-        v |= need[r >> 5] >> (r & 31);
-        g |= need[nw8 + (r >> 5)] >> (r & 31);
-      }
-      if (!(v & 1u)) continue;
-      wg = GRAD && (g & 1u);
-    }
+  for (int s = 0; s < nsh; ++s) {
+    const int f = A.fl[s][grp];
+    if (!(f & 1)) continue;  // no stage reads the shell's values on this group
+    const bool wg = GRAD && (f & 2);
+    const ShellD sh = A.sh[s];
+    const T *ex = A.ex + (sh.exp_off - e0), *nm = A.nm + (sh.exp_off - e0);
+    const T *ct = A.ct + (sh.contr_off - c0);
     switch (sh.l) {
-      case 0: shell_emit<0, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
-      case 1: if constexpr (LA >= 1) shell_emit<1, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
-      case 2: if constexpr (LA >= 2) shell_emit<2, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
-      case 3: if constexpr (LA >= 3) shell_emit<3, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
-      case 4: if constexpr (LA >= 4) shell_emit<4, GRAD>(sh, exps, norms, contr, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
+      case 0: shell_emit<0, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
+      case 1: if constexpr (LA >= 1) shell_emit<1, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
+      case 2: if constexpr (LA >= 2) shell_emit<2, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
+      case 3: if constexpr (LA >= 3) shell_emit<3, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
+      case 4: if constexpr (LA >= 4) shell_emit<4, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
     }
   }
 }
@@ -553,13 +554,17 @@ __device__ __forceinline__ long item_at(const int *__restrict__ order, long L, l
 
 // ------------------------------------------------------------------- basis
 // item = (256-point block, atom) of a unit; one thread per (point, atom).
-// Screened units (bscreen) walk their list of the (block, atom) pairs whose
-// values a later stage reads, and write only the shells read on each
-// point's group.
+// The workgroup first stages the atom's shells, primitives and coefficients
+// and the per-(shell, group) write flags in LDS, so the evaluation itself
+// waits on no global load.  Screened units (bscreen) walk their list of the
+// (block, atom) pairs whose values a later stage reads, and write only the
+// shells read on each point's group; dense units write every shell.
 template <bool GRAD, typename T>
 __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ units, int nu,
                                                     const long *__restrict__ base,
                                                     const int *__restrict__ order, long total) {
+  __shared__ AtomLds<T> A;
+  const int tid = threadIdx.x;
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const long it = item_at(order, L, total);
     const int u = find_unit(base, nu, it);
@@ -575,19 +580,40 @@ __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ un
       atom = (int)(local % U.n_atoms);
       blk = local / U.n_atoms;
     }
-    const long p = blk * kThreads + threadIdx.x;
-    const uint32_t *need = U.bscreen ? U.bneed + (p / kGroup) * 2 * U.nw8 : nullptr;
     const AtomD at = U.atoms[atom];
-    // centre offsets in FP64, then the evaluation precision
-    const T x = (T)(U.gx[p] - at.x), y = (T)(U.gy[p] - at.y), z = (T)(U.gz[p] - at.z);
+    const int s0 = at.shell_begin, nsh = at.shell_end - s0;
+    if (nsh <= 0) continue;  // an atom without shells (uniform over the workgroup)
+    const int e0 = at.exp_begin, c0 = at.contr_begin;
+    const long p = blk * kThreads + tid;
+    // the point's centre offsets in FP64, loads in flight across the staging
+    const double gx = U.gx[p], gy = U.gy[p], gz = U.gz[p];
+    __syncthreads();  // the previous item's reads of A are done
+    for (int i = tid; i < nsh; i += kThreads) A.sh[i] = U.shells[s0 + i];
+    for (int i = tid; i < at.n_exp; i += kThreads) {
+      A.ex[i] = (T)U.exps[e0 + i];
+      A.nm[i] = (T)U.norms[e0 + i];
+    }
+    for (int i = tid; i < at.n_contr; i += kThreads) A.ct[i] = (T)U.contr[c0 + i];
+    if (tid < nsh * (kThreads / kGroup)) {  // flag of (shell tid / 8, group tid % 8)
+      uint32_t f = GRAD ? 3u : 1u;
+      if (U.bscreen) {
+        const uint32_t *fw = U.bflags + 8 * local;
+        f = ((fw[tid >> 5] >> (tid & 31)) & 1u) |
+            (GRAD ? ((fw[4 + (tid >> 5)] >> (tid & 31)) & 1u) << 1 : 0u);
+      }
+      A.fl[tid / (kThreads / kGroup)][tid % (kThreads / kGroup)] = (uint8_t)f;
+    }
+    __syncthreads();
+    const T x = (T)(gx - at.x), y = (T)(gy - at.y), z = (T)(gz - at.z);
     T *phi = (T *)U.phi;
     const long cs = (long)U.NP * U.Mc;
+    const int grp = tid / kGroup;
     switch (at.lmax) {
-      case 0: basis_atom<0, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p, need, U.nw8); break;
-      case 1: basis_atom<1, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p, need, U.nw8); break;
-      case 2: basis_atom<2, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p, need, U.nw8); break;
-      case 3: basis_atom<3, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p, need, U.nw8); break;
-      default: basis_atom<4, GRAD>(at, U.shells, U.exps, U.norms, U.contr, x, y, z, phi, U.Mc, cs, p, need, U.nw8); break;
+      case 0: basis_atom<0, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+      case 1: basis_atom<1, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+      case 2: basis_atom<2, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+      case 3: basis_atom<3, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+      default: basis_atom<4, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
     }
   }
 }
