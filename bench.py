@@ -75,6 +75,29 @@ def precision_errors(xc, C, occ, precs, norms=(1e-2, 1e-4, 1e-6)):
     return out
 
 
+def schedule_ab(xc, chain, occ, prec, model, steps):
+    """The mixed build under the static split, counted-flop queues and
+    cost-model queues, interleaved build by build on one device; mean device
+    ms per build and the model's gains."""
+    import numpy as np
+    from pgxc import _lib
+    from pgxc.xc import PBE
+    modes = [("static", _lib.SCHED_STATIC, None), ("analytic", _lib.SCHED_ANALYTIC, None),
+             ("model", _lib.SCHED_MODEL, model)]
+    ms = {n: [] for n, _, _ in modes}
+    for r in range(max(2, steps)):
+        for name, mode, m in modes:
+            xc.set_schedule(mode, m)
+            for k in range(2):  # the first build after a change of schedule plans
+                xc.set_orbitals(chain[(2 * r + k) % len(chain)], occ, incremental=True)
+                xc.build(PBE, prec, fetch=False)
+            ms[name].append(xc.timing()["total"])
+    out = {n: float(np.mean(v)) for n, v in ms.items()}
+    out["model_gain_vs_analytic"] = out["analytic"] / out["model"]
+    out["model_gain_vs_static"] = out["static"] / out["model"]
+    return out
+
+
 def copy_calibration(mib):
     """A plain streaming kernel of known size (y = -x over mib MiB of FP32) for
     scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -334,12 +357,7 @@ def bench_open_shell(args):
     xc = XCBuilder(b, g, tile=tu["tile"] if tu else 0,
                    variants=(tu["variant_fp64"], tu["variant_fp32"]) if tu else (-1, -1))
     from pgxc import _lib
-    from pgxc.costmodel import default_model
-    model = default_model()  # the headline's schedule: the shipped cost model's queues
-    if model is not None:
-        xc.set_schedule(_lib.SCHED_MODEL, model)
-    else:
-        xc.set_schedule(_lib.SCHED_ANALYTIC, None)
+    xc.set_schedule(_lib.SCHED_ANALYTIC, None)  # the headline's schedule
 
     def timed(prec, incremental_chain=None):
         out = []
@@ -440,8 +458,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=2000, help="cost-model training steps")
     ap.add_argument("--save-model", default="", help="learned: write the trained model here (.npz)")
     ap.add_argument("--schedule", default="auto", choices=["auto", "static", "analytic", "model"],
-                    help="contraction work schedule (auto: the shipped cost model if present, "
-                         "else analytic)")
+                    help="contraction work schedule: static split, queues balanced by counted "
+                         "flops, or by the shipped cost model's predictions (auto: the model "
+                         "when shipped, else analytic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -476,10 +495,11 @@ def main():
         kw = dict(tile=tuned["tile"], variants=(tuned["variant_fp64"], tuned["variant_fp32"]))
     from pgxc import _lib
     from pgxc.costmodel import default_model
-    sched, model = args.schedule, None
-    if sched in ("auto", "model"):
-        model = default_model()
+    sched, model = args.schedule, default_model()
+    if sched == "auto":
         sched = "model" if model is not None else "analytic"
+    if sched == "model" and model is None:
+        raise SystemExit("bench.py: no shipped cost model (pgxc/cost_model_mi355x.npz)")
     sched_mode = {"static": _lib.SCHED_STATIC, "analytic": _lib.SCHED_ANALYTIC,
                   "model": _lib.SCHED_MODEL}[sched]
     log("schedule:", sched)
@@ -548,6 +568,10 @@ def main():
     st1, nlaunch = one_lane(xc, best)
     log("%s on one lane (basis evaluated): %.2f ms/build stages %s" % (best, st1["total"], st1))
     stats = xc.stats()
+    sched_ab = schedule_ab(xc, chain, occ, PRECISIONS[best], model, args.steps) if model else None
+    if sched_ab:
+        log("schedules (interleaved, ms per build):", sched_ab)
+    xc.set_schedule(sched_mode, model)
     xc.close()
 
     # ---- SCF-cycle builds with the basis values kept in HBM across builds
@@ -669,6 +693,7 @@ def main():
                        "stages_ms_one_lane": {k: round(float(v), 3) for k, v in st1w.items()},
                        "dominant_contraction": roof_warm},
         "schedule": sched,
+        "schedules_ab": sched_ab,
         "precision_errors": perr,
         "layout": {k: int(v) for k, v in layout.items()},
         "stages_ms": {k: round(float(v), 3) for k, v in R["st"].items()},

@@ -19,7 +19,8 @@ module xc_gpu
   private
 
   public :: xc_gpu_setup, xc_gpu_set_orbitals, xc_gpu_set_density, xc_gpu_build, &
-       xc_gpu_close, xc_gpu_mask, xc_gpu_status, xc_gpu_energy, xc_gpu_gradient
+       xc_gpu_close, xc_gpu_mask, xc_gpu_status, xc_gpu_energy, xc_gpu_gradient, &
+       xc_gpu_state_words, xc_gpu_load_state_words
 
   ! closed shell: eigvec_occ(nbas, nocc), occ_num_occ(nocc); open or closed
   ! shell as ParaGauss holds them: eigvec_occ(nbas, nocc, ispin),
@@ -177,6 +178,36 @@ contains
        if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_gradient', rc)
     endif
   end subroutine xc_gpu_gradient
+
+  ! The library's incremental state as real(8) words for the SCF-state tape:
+  ! words(1) = its size in bytes, then the bytes.  main_scf writes them with
+  ! readwriteblocked_write right after xc_hamiltonian_store's records
+  ! (modules/xc_hamiltonian.f90:2407-2486) and, on a restart, reads them back
+  ! after xc_hamiltonian_recover (:2488-2634) and after xc_gpu_setup of the
+  ! same geometry, so the next build can be incremental.
+  subroutine xc_gpu_state_words(words)
+    real(c_double), allocatable, target, intent(out) :: words(:)
+    integer(c_int64_t) :: nb
+    integer(c_int) :: rc
+    nb = pgxc_state_bytes(ctx)
+    if (nb < 0) call xc_gpu_fail('pgxc_state_bytes', int(-nb, c_int))
+    allocate(words(1 + (nb + 7) / 8))
+    words = 0.0_c_double
+    words(1) = real(nb, c_double)
+    rc = pgxc_save_state(ctx, c_loc(words(2)), nb)
+    if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_save_state', rc)
+  end subroutine xc_gpu_state_words
+
+  subroutine xc_gpu_load_state_words(words)
+    real(c_double), target, intent(in) :: words(:)
+    integer(c_int64_t) :: nb
+    integer(c_int) :: rc
+    nb = int(words(1), c_int64_t)
+    if (size(words) < 1 + (nb + 7) / 8) call xc_gpu_fail('xc_gpu_load_state_words: short record', &
+         PGXC_ERR_ARG)
+    rc = pgxc_load_state(ctx, c_loc(words(2)), nb)
+    if (rc /= PGXC_OK) call xc_gpu_fail('pgxc_load_state', rc)
+  end subroutine xc_gpu_load_state_words
 
   subroutine xc_gpu_close()
     integer(c_int) :: rc

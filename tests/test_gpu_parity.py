@@ -193,7 +193,13 @@ def test_fortran_driver_matches(h2o, tmp_path):
     raw = np.fromfile(fout, dtype=np.float64)
     npk = b.nbas * (b.nbas + 1) // 2
     e, q, ham = raw[0], raw[1], raw[2:2 + npk]
-    terms, qe, grad = raw[2 + npk:6 + npk], raw[6 + npk], raw[7 + npk:].reshape(b.n_atoms, 3)
+    terms, qe = raw[2 + npk:6 + npk], raw[6 + npk]
+    grad = raw[7 + npk:7 + npk + 3 * b.n_atoms].reshape(b.n_atoms, 3)
+    # after the restart (state words, new setup, incremental build with
+    # dP = 0): the same bits as the build before it
+    tail = raw[7 + npk + 3 * b.n_atoms:]
+    assert tail.size == 2 + npk
+    assert tail[0] == e and tail[1] == q and np.array_equal(tail[2:], ham)
     check((ham, e, q), oracle_result("h2o_pbe", b, g, oracle.MASK_PBE, C=C, occ=occ))
     # the Fortran post-SCF calls: energy terms of the same density, and the
     # gradient the Python binding gives for the same build

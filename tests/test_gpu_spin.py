@@ -197,6 +197,11 @@ def test_fortran_driver_open_shell(case, tmp_path):
     nh = b.nbas * (b.nbas + 1)  # two spins
     check((raw[2:2 + nh], raw[0], raw[1]), ref(case, oracle.MASK_PBE))
     assert abs(raw[2 + nh:6 + nh].sum() - raw[0]) <= 1e-12 * abs(raw[0])
+    # after the restart (state words, new setup, incremental build with
+    # dP = 0): the same bits as the build before it
+    tail = raw[7 + nh + 3 * b.n_atoms:]
+    assert tail.size == 2 + nh
+    assert tail[0] == raw[0] and tail[1] == raw[1] and np.array_equal(tail[2:], raw[2:2 + nh])
 
 
 def test_fully_polarised_and_empty_spin(case):

@@ -162,15 +162,25 @@ def test_water_solute_schedules_and_oracle_slices():
     with XCBuilder(b, g) as xc:
         xc.set_orbitals(C, occ)
         for prec in (FP64, BF16X3):
-            ref = None
+            ref, ms = None, {}
             for name, mode, m in modes:
-                say("  %s schedule, precision %d" % (name, prec))
                 xc.set_schedule(mode, m)
                 out = xc.build(PBE, prec)
+                ts = []
+                for _ in range(2):  # timed builds after the one that plans
+                    xc.build(PBE, prec, fetch=False)
+                    ts.append(xc.timing()["total"])
+                ms[name] = min(ts)
+                say("  %s schedule, precision %d: %.1f ms" % (name, prec, ms[name]))
                 if ref is None:
                     ref = out
                 else:
                     assert np.array_equal(out[0], ref[0]) and out[1] == ref[1] and out[2] == ref[2]
+            # what the trained network gains over counted flops and over the
+            # static split (the reference's method); no schedule falls far behind
+            say("  model gain: %.3fx over analytic, %.3fx over static"
+                % (ms["analytic"] / ms["model"], ms["static"] / ms["model"]))
+            assert max(ms.values()) <= 1.25 * min(ms.values())
     model.close()
     npts = g.shape[0]
     for lo in (npts // 7, npts // 2, (5 * npts) // 6):
