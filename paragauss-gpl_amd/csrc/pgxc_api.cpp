@@ -440,8 +440,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       const int lane = k % lanes_of(c);  // work buffers of chunk k: its stream's lane
       U.phi = (char *)c->phi_base + (c->phi_stride ? (size_t)k * c->phi_stride : (size_t)lane * c->phi_lane);
       if (precision != PGXC_PREC_FP64) {  // split planes after the four FP32 planes
-        U.phis = (char *)U.phi + 16 * (size_t)c->NP * c->Mc;
-        U.phit = (char *)U.phis + 4 * (size_t)c->NP * c->Mc;
+        U.phit = (char *)U.phi + 16 * (size_t)c->NP * c->Mc;
       }
       U.W = (char *)c->W + (size_t)lane * c->w_lane;
       U.D = c->D;
@@ -804,8 +803,8 @@ static std::mutex g_cache_mu;
 static std::map<int, size_t> g_cache_bytes;
 
 // bytes of basis values per (function, point): 4 planes, plus for FP32 the
-// split-BF16 planes in both layouts (2 x 2 x 2 bytes)
-static size_t phi_elem_bytes(size_t elem) { return elem == 8 ? 32 : 24; }
+// transposed split-BF16 planes of the values (2 x 2 bytes)
+static size_t phi_elem_bytes(size_t elem) { return elem == 8 ? 32 : 20; }
 
 // Test switch PGXC_POISON_BASIS=1: basis buffers, when they get their first
 // contents (allocation, or the zeroing of the lane buffers at a change of
@@ -822,12 +821,7 @@ static hipError_t poison_basis(const pgxc_ctx *c, void *base, size_t elem, size_
     char *b = (char *)base + k * per_buf;
     for (int pl = 0; pl < 4 && e == hipSuccess; ++pl)
       e = hipMemsetAsync(b + pl * plane * elem, 0xff, rows * elem, s);
-    if (elem == 4 && e == hipSuccess) {
-      char *hs = b + 16 * plane, *ht = hs + 4 * plane;
-      for (int pl = 0; pl < 2 && e == hipSuccess; ++pl)
-        e = hipMemsetAsync(hs + pl * plane * 2, 0xff, rows * 2, s);
-      if (e == hipSuccess) e = hipMemsetAsync(ht, 0xff, 4 * plane, s);
-    }
+    if (elem == 4 && e == hipSuccess) e = hipMemsetAsync(b + 16 * plane, 0xff, 4 * plane, s);
   }
   return e;
 }

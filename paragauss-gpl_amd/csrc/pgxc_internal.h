@@ -44,8 +44,7 @@ struct UnitD {
   const ShellD *shells;
   const double *exps, *norms, *contr;
   void *phi;        // [4][NP][Mc] values, d/dx, d/dy, d/dz (T)
-  void *phis;       // split-BF16 builds: values as bf16 hi / lo planes [2][NP][Mc]
-  void *phit;       //   and transposed, [2][Mc][NP]
+  void *phit;       // split-BF16 builds: the values as bf16 hi / lo planes, transposed [2][Mc][NP]
   void *W;          // [NP][Mc] (T); split-BF16 builds: bf16 hi / lo planes [2][NP][Mc]
   const void *D;    // [NP][NP] density operand (T); split-BF16: transposed hi / lo planes
   double *part;     // [NP/tf][4][Mc] density partials

@@ -5,8 +5,10 @@
 // and a product is formed from three v_mfma_f32_16x16x32_bf16 passes,
 //   a*b ~ a_lo*b_hi + a_hi*b_lo + a_hi*b_hi,   FP32 accumulation,
 // dropping a_lo*b_lo (relative 2^-16).  The build stores its operands already
-// split, k-contiguous (basis values once per geometry by k_split, W by
-// k_wmat, D^T by k_prep), so staging is a copy of 16-byte vectors into LDS.
+// split, k-contiguous (the transposed basis values once per geometry by
+// k_split, W by k_wmat, D^T by k_prep), so staging is a copy of 16-byte
+// vectors into LDS; the potential's basis-value operand is read in FP32 and
+// split while it is staged.
 // Same tiles and epilogues as the FP32/FP64 contractions (xc_kernels.hip);
 // K advances 32 per MFMA.
 #include <type_traits>
@@ -77,9 +79,11 @@ struct PSOperand {
       row[e] = o >= 0 ? (plane ? S.lo : S.hi) + o : nullptr;
     }
   }
-  // ko: element offset of this thread's k run in the step, or -1
-  template <int SL>
-  __device__ __forceinline__ void load(long ko) {
+  // kofs(kt, c): element offset of k run c of step kt, or -1; steps past
+  // ksteps load zeros
+  template <int SL, class KOfs>
+  __device__ __forceinline__ void load(const KOfs &kofs, int kt, int ksteps) {
+    const long ko = kt < ksteps ? kofs(kt, (int)(threadIdx.x & 3)) : -1;
     unsigned m = 0;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
@@ -101,6 +105,68 @@ struct PSOperand {
   }
 };
 
+// Staging of an FP32 operand stored k-contiguous, split into bf16 hi and lo
+// planes on its way to LDS (split_bf16, the split every stored split operand
+// gets, so the MFMA inputs are the same bits).  Per stage R rows x 32 k as
+// k-quads: one 16-byte load of 4 values gives 8 bytes of hi and 8 of lo.  A
+// thread's quads all take quad q = threadIdx.x & 7 of a step (k run q / 2,
+// half q % 2); rows, zeros and the pipeline as PSOperand.
+template <int R, int NT = kThreads>
+struct PFOperand {
+  static constexpr int NQ = R * (kXBK / 4);  // k-quads per stage
+  static constexpr int PER = NQ / NT;
+  static_assert(PER * NT == NQ, "whole quads per thread");
+  static_assert(NT % 8 == 0, "k quad = threadIdx.x & 7");
+  const float *row[PER];  // plane + row offset of quad e, nullptr: row absent
+  const float *dummy;
+  f4 v[2][PER];
+  unsigned ok[2];
+
+  template <class RowF>
+  __device__ __forceinline__ void init(const float *plane, const RowF &rowf) {
+    dummy = plane;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const long o = rowf((int)((threadIdx.x + NT * e) >> 3));
+      row[e] = o >= 0 ? plane + o : nullptr;
+    }
+  }
+  template <int SL, class KOfs>
+  __device__ __forceinline__ void load(const KOfs &kofs, int kt, int ksteps) {
+    const int q = (int)(threadIdx.x & 7);
+    const long kr = kt < ksteps ? kofs(kt, q >> 1) : -1;
+    const long ko = kr >= 0 ? kr + 4 * (q & 1) : -1;
+    unsigned m = 0;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const bool g = row[e] != nullptr && ko >= 0;
+      v[SL][e] = gld((const f4 *)(g ? row[e] + ko : dummy));
+      m |= (unsigned)g << e;
+    }
+    ok[SL] = m;
+  }
+  template <int SL>
+  __device__ __forceinline__ void store(__bf16 *hi, __bf16 *lo) const {
+    typedef uint16_t u4v __attribute__((ext_vector_type(4)));
+    const int q = (int)(threadIdx.x & 7);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int r = (int)((threadIdx.x + NT * e) >> 3);
+      const bool g = (ok[SL] >> e) & 1u;
+      u4v h, l;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        __bf16 hh, ll;
+        split_bf16(g ? v[SL][e][j] : 0.0f, hh, ll);
+        h[j] = __builtin_bit_cast(uint16_t, hh);
+        l[j] = __builtin_bit_cast(uint16_t, ll);
+      }
+      *(u4v *)(hi + r * kXBKP + 4 * q) = h;
+      *(u4v *)(lo + r * kXBKP + 4 * q) = l;
+    }
+  }
+};
+
 // OA / OB: operand staging (PSOperand<R, NT>, rows set by init).  kofs(kt,
 // c): element offset of k run c (8 k) of step kt, or -1 for zeros.  The NT
 // threads form NT / 64 waves in a (NT / 64 / WGN) x WGN grid, each with a
@@ -115,7 +181,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(
   constexpr int WGM = NT / 64 / WGN;
   constexpr int BKP = kXBKP, WM = BM / WGM, WN = BN / WGN, TM = WM / 16, TN = WN / 16;
   constexpr int SA = BM * BKP, SB = BN * BKP, STAGE = 2 * SA + 2 * SB;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, kc = threadIdx.x & 3;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave / WGN, wn = wave % WGN;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -149,16 +215,14 @@ __device__ __forceinline__ void gemm_tile_bf16x3(
       }
   };
   auto stage = [&](int buf) { return lds + buf * STAGE; };
-  auto kofs_at = [&](int kt) -> long { return kt < ksteps ? kofs(kt, kc) : -1; };
   // step kt lives in register slot kt & 1 and LDS stage kt & 1.  Loads and
   // stores run unconditionally (past the last step they move zeros), so the
   // loop body is straight-line code and the compiler's wait counts keep the
   // loads of step kt + 2 in flight while step kt + 1 is written to LDS.
   auto step = [&](auto slot, int kt) {
     constexpr int C = decltype(slot)::value, N = 1 - C;
-    const long ko = kofs_at(kt + 2);  // slot C is free: step kt is in LDS
-    A.template load<C>(ko);
-    B.template load<C>(ko);
+    A.template load<C>(kofs, kt + 2, ksteps);  // slot C is free: step kt is in LDS
+    B.template load<C>(kofs, kt + 2, ksteps);
     compute(C);
     __bf16 *ah = stage(N), *al = ah + SA, *bh = al + SA, *bl = bh + SB;
     A.template store<N>(ah, al);
@@ -167,11 +231,10 @@ __device__ __forceinline__ void gemm_tile_bf16x3(
   };
 
   {
-    const long k0 = kofs_at(0), k1 = kofs_at(1);
-    A.template load<0>(k0);
-    B.template load<0>(k0);
-    A.template load<1>(k1);
-    B.template load<1>(k1);
+    A.template load<0>(kofs, 0, ksteps);
+    B.template load<0>(kofs, 0, ksteps);
+    A.template load<1>(kofs, 1, ksteps);
+    B.template load<1>(kofs, 1, ksteps);
     __bf16 *ah = stage(0), *al = ah + SA, *bh = al + SA, *bl = bh + SB;
     A.template store<0>(ah, al);
     B.template store<0>(bh, bl);
@@ -291,15 +354,17 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
     const int ng = potential_groups(U.gcf + U.gcf_off[sl], G, nch, ci, cj, gl, wtot);
     const uint16_t *ri = ul + ci * CH, *rj = ul + cj * CH;
     const int nri = min(CH, nu - ci * CH), nrj = min(CH, nu - cj * CH);
-    const __bf16 *wh = (const __bf16 *)U.W, *ph = (const __bf16 *)U.phis;
+    const __bf16 *wh = (const __bf16 *)U.W;
     const long pn = (long)U.NP * Mc;
-    // A(m, k) = W[f_i(m)][kb + p(k)], B(k, n) = Phi[f_j(n)][kb + p(k)]
-    PSOperand<BT> opa, opb;
+    // A(m, k) = W[f_i(m)][kb + p(k)] (stored split), B(k, n) =
+    // Phi[f_j(n)][kb + p(k)] (FP32 values, split while staged)
+    PSOperand<BT> opa;
+    PFOperand<BT> opb;
     opa.init({wh, wh + pn}, [&](int m) -> long {
       const long f = run_fn(ri, nri, m);
       return f < 0 ? -1 : f * Mc;
     });
-    opb.init({ph, ph + pn}, [&](int n) -> long {
+    opb.init((const float *)U.phi, [&](int n) -> long {
       const long f = run_fn(rj, nrj, n);
       return f < 0 ? -1 : f * Mc;
     });
@@ -316,10 +381,10 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
 
 // ------------------------------------------------------------------ split
 // item = (64 functions, 64 points) of a unit: the FP32 basis values become
-// bf16 hi / lo planes in the value layout [n][p] (B operand of the potential
-// contraction) and transposed [p][n] (A operand of the density contraction),
-// through an LDS tile so both writes run along their contiguous dimension,
-// 16 bytes per store (a transposed row segment of 64 functions is 128 bytes).
+// bf16 hi / lo planes transposed, [p][n] (A operand of the density
+// contraction; the potential contraction splits the values as it stages
+// them), through an LDS tile so reads and writes run along their contiguous
+// dimension, 16 bytes per store (a row segment of 64 functions is 128 bytes).
 // Screened units walk the items touching blocks whose values a contraction
 // reads, and split only those (run of 8 functions x 32-point group) blocks.
 __global__ __launch_bounds__(kThreads) void k_split(const StageList S) {
@@ -345,30 +410,18 @@ __global__ __launch_bounds__(kThreads) void k_split(const StageList S) {
     const long Mc = U.Mc, NP = U.NP, pn = NP * Mc;
     const int nfr = (int)(NP - n0 < 64 ? NP - n0 : 64);  // NP is a multiple of 32
     const float *phi = (const float *)U.phi;
-    uint16_t *sh = (uint16_t *)U.phis, *th = (uint16_t *)U.phit;
+    uint16_t *th = (uint16_t *)U.phit;
     // whether the block of function n0 + r and point p0 + c is read
     auto live = [&](int r, int c) -> bool { return (lm >> (8 * (c / kGroup) + r / kRun)) & 1u; };
     {  // function row r, points c0 .. c0 + 15 (one group)
       const int r = tid >> 2, c0 = (tid & 3) * 16;
       if (r < nfr && live(r, c0)) {
         const long q = (n0 + r) * Mc + p0 + c0;
-        u8v hi[2], lo[2];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const f4 v = gld((const f4 *)(phi + q + 4 * j));
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            t[r][c0 + 4 * j + e] = v[e];
-            __bf16 h, l;
-            split_bf16(v[e], h, l);
-            hi[j >> 1][4 * (j & 1) + e] = __builtin_bit_cast(uint16_t, h);
-            lo[j >> 1][4 * (j & 1) + e] = __builtin_bit_cast(uint16_t, l);
-          }
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          gst((u8v *)(sh + q + 8 * h), hi[h]);
-          gst((u8v *)(sh + pn + q + 8 * h), lo[h]);
+          for (int e = 0; e < 4; ++e) t[r][c0 + 4 * j + e] = v[e];
         }
       }
     }
