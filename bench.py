@@ -307,18 +307,20 @@ def bench_batch64(args):
             b64.append(time.time() - t0)
         same = all(np.array_equal(x.result()[0], r[0]) for x, r in zip(xcs, ref))
         # mixed precision: incremental split-BF16 builds from the FP64 state
-        chains = [systems.perturbed_orbitals(m[2], 1e-4, seed=4000 + i) for i, m in enumerate(mols)]
-        for xc, Ck, m in zip(xcs, chains, mols):
-            xc.set_orbitals(Ck, m[3], incremental=True)
-        bt.build(PBE, BF16X3, graph=True)
-        mixed = [x.result() for x in xcs]
+        # along a chain of densities, ||dP||_F = 1e-4 per step and molecule
+        cur = [m[2] for m in mols]
         bx3 = []
-        for _ in range(args.steps):  # same density again: dP = 0 after the first
-            for xc, Ck, m in zip(xcs, chains, mols):
+        for step in range(args.steps + 1):  # the first (untimed) captures the graph
+            cur = [systems.perturbed_orbitals(Ck, 1e-4, seed=4000 + 97 * step + i)
+                   for i, Ck in enumerate(cur)]
+            for xc, Ck, m in zip(xcs, cur, mols):
                 xc.set_orbitals(Ck, m[3], incremental=True)
             t0 = time.time()
             bt.build(PBE, BF16X3, graph=True)
-            bx3.append(time.time() - t0)
+            if step:
+                bx3.append(time.time() - t0)
+        mixed = [x.result() for x in xcs]
+        chains = cur
     dE = []
     for xc, Ck, m, mx in zip(xcs, chains, mols, mixed):
         xc.set_orbitals(Ck, m[3])
@@ -440,6 +442,30 @@ def bench_post_scf(args):
                       **out}), flush=True)
 
 
+def bench_profile(args):
+    """The kernels the headline's roofline is taken from, alone, for rocprofv3
+    (kernel trace and traffic counters): a copy kernel of known size (counter
+    calibration), then (H2O)32 builds on one lane, with the basis evaluated and
+    with it cached, FP64 full builds and split-BF16 incremental builds."""
+    from pgxc import XCBuilder, systems
+    from pgxc.xc import BF16X3, FP64, PBE
+    copy_calibration(1024)
+    els, b, g, C, occ = workload("h2o32")
+    for cache in (False, True):
+        xc = XCBuilder(b, g, tile=96, variants=(2, 1), basis_cache=cache)
+        xc.set_lanes(1)
+        xc.set_orbitals(C, occ)
+        for prec in (FP64, BF16X3):
+            for i in range(args.steps):
+                xc.set_orbitals(systems.perturbed_orbitals(C, 1e-4, seed=50 + i), occ,
+                                incremental=prec != FP64)
+                xc.build(PBE, prec, fetch=False)
+                log("basis %s, %s, build %d:" % ("cached" if cache else "evaluated",
+                                                  "fp64" if prec == FP64 else "bf16x3", i),
+                    {k: round(v, 3) for k, v in xc.timing().items()})
+        xc.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -449,7 +475,8 @@ def main():
                     help="h2o32 (headline), h2o, batch64 (64-molecule batch), "
                          "learned ((H2O)128 + solute, cost-model schedules), "
                          "h2o32-open ((H2O)32 open shell), "
-                         "post-scf ((H2O)32 energies per term and XC gradients)")
+                         "post-scf ((H2O)32 energies per term and XC gradients), "
+                         "h2o32-profile (one-lane builds for rocprofv3)")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16x3"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
@@ -474,6 +501,8 @@ def main():
         return bench_open_shell(args)
     if args.workload == "post-scf":
         return bench_post_scf(args)
+    if args.workload == "h2o32-profile":
+        return bench_profile(args)
 
     import numpy as np
     from pgxc import XCBuilder
@@ -608,36 +637,37 @@ def main():
     # of one contraction: half of pgxc_last_stats' sum over 128-point tiles of
     # 4 B n_b^2 + 16 B n_b (n_b: functions not cut off on the tile; the 16 B n_b
     # elementwise share is ~1 %); the dense equivalent 2 npts n^2 beside it.
-    # Basis stage: algorithmic bytes = 32 B (x, y, z, w) + 4 s B n_b (the
-    # FP32 values and gradients of the live functions, written once).
+    # The basis stage times two kernels (k_basis, then k_split in split-BF16
+    # builds), each shorter per chunk than the density contraction (kernel
+    # trace of --workload h2o32-profile), so the dominant kernel is the larger
+    # contraction; the basis stage gets its own HBM roofline below.
     st = st1
-    dom = max(("basis", "density", "potential"), key=lambda k: st[k])
+    dom = max(("density", "potential"), key=lambda k: st[k])
     tf = layout["tile"]
     peak = PEAK["fp32"] if best == "fp32" else PEAK["bf16"] / 3.0
-    if dom == "basis":
-        # FP32 values and gradients of the live functions, written once
-        bytes_alg = 32.0 * npts + 16.0 * stats["bn"]
-        kname = "k_basis<true,float>" + (" + k_split" if best == "bf16x3" else "")
-        ach = bytes_alg / (st[dom] * 1e-3) / 1e12
-        roof = {"bound": "hbm", "kernel": kname, "achieved": ach, "peak": 8.0, "unit": "TB/s",
-                "frac": ach / 8.0, "traffic": profiled_traffic(kname.split(" ")[0]),
-                "bytes_per_build_algorithmic": bytes_alg}
+    flops = stats["flops"] / 2.0
+    achieved = flops / (st[dom] * 1e-3) / 1e12
+    if best == "bf16x3":
+        kname = "k_%s_bf16x3<%d%s>" % (dom, tf, ",true" if dom == "density" else "")
     else:
-        flops = stats["flops"] / 2.0
-        achieved = flops / (st[dom] * 1e-3) / 1e12
-        if best == "bf16x3":
-            kname = "k_%s_bf16x3<%d%s>" % (dom, tf, ",true" if dom == "density" else "")
-        else:
-            v = layout["variant_fp32"]
-            kname = "k_%s<float,%d,%d%s>" % (dom, tf, v, ",true" if dom == "density" else "")
-        roof = {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s" + (" (FP32-equivalent: BF16 peak / 3 passes)" if best == "bf16x3" else ""),
-                "frac": achieved / peak, "traffic": profiled_traffic(kname),
-                "achieved_dense_equivalent": 2.0 * npts * n * n / (st[dom] * 1e-3) / 1e12,
-                "achieved_executed": executed[best][dom] / (st[dom] * 1e-3) / 1e12,
-                "flops_per_build_executed": executed[best],
-                "flops_per_build_algorithmic": stats["flops"],
-                "flops_per_build_dense": 4.0 * npts * n * n + 16.0 * npts * n}
+        v = layout["variant_fp32"]
+        kname = "k_%s<float,%d,%d%s>" % (dom, tf, v, ",true" if dom == "density" else "")
+    roof = {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": peak,
+            "unit": "TFLOP/s" + (" (FP32-equivalent: BF16 peak / 3 passes)" if best == "bf16x3" else ""),
+            "frac": achieved / peak, "traffic": profiled_traffic(kname),
+            "achieved_dense_equivalent": 2.0 * npts * n * n / (st[dom] * 1e-3) / 1e12,
+            "achieved_executed": executed[best][dom] / (st[dom] * 1e-3) / 1e12,
+            "flops_per_build_executed": executed[best],
+            "flops_per_build_algorithmic": stats["flops"],
+            "flops_per_build_dense": 4.0 * npts * n * n + 16.0 * npts * n}
+    # basis stage (k_basis + k_split) against the HBM roof: FP32 values and
+    # gradients of the live functions written once, and the grid read
+    bytes_basis = 32.0 * npts + 16.0 * stats["bn"]
+    ach_b = bytes_basis / (st["basis"] * 1e-3) / 1e12
+    roof_basis = {"bound": "hbm", "kernels": "k_basis<true,float>" + (" + k_split" if best == "bf16x3" else ""),
+                  "stage_ms_one_lane": st["basis"], "bytes_per_build_algorithmic": bytes_basis,
+                  "achieved": ach_b, "peak": 8.0, "unit": "TB/s", "frac": ach_b / 8.0,
+                  "traffic": profiled_traffic("k_basis<true,float>")}
     roof["timed"] = "one-lane builds with the basis evaluated (%d launches)" % (nlaunch * layout["chunks"])
     roof["traffic_unit"] = ("bytes per dispatch (one grid chunk), rocprofv3 FETCH_SIZE + WRITE_SIZE "
                             "scaled by a copy kernel")
@@ -702,6 +732,7 @@ def main():
         "algorithmic_per_build": {"flops": stats["flops"], "bytes": stats["bytes"],
                                   "exps": stats["exps"]},
         "roofline": roof,
+        "roofline_basis_stage": roof_basis,
     }
     if args.workload == "h2o":
         # a build this small is latency bound, so only its HBM fraction is

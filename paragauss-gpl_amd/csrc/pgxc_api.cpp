@@ -1612,6 +1612,19 @@ extern "C" int pgxc_set_density(pgxc_ctx *c, int32_t ispin, int32_t nbas, const 
   return PGXC_OK;
 }
 
+// Temporary device buffers of a call, freed when the call returns on any path.
+struct ScopedDev {
+  std::vector<void *> bufs;
+  ~ScopedDev() {
+    for (void *b : bufs) dfree(b);
+  }
+  template <typename T>
+  T *keep(T *p) {
+    bufs.push_back((void *)p);
+    return p;
+  }
+};
+
 extern "C" int pgxc_set_orbitals(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32_t nocc,
                                  const double *C, const double *occ, int32_t mode) {
   int rc = check_density_args(c, ispin, nbas, mode);
@@ -1620,11 +1633,14 @@ extern "C" int pgxc_set_orbitals(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32
     return fail(c, PGXC_ERR_ARG, "pgxc_set_orbitals: bad nocc or null C/occ");
   const int n = c->nbas;
   double *dC = nullptr, *docc = nullptr;
+  ScopedDev tmp;  // dC and docc, freed on every return below
   for (int sp = 0; sp < ispin; ++sp)
     HIPCHK(c, hipMemsetAsync(sp ? c->beta.P : c->P, 0, (size_t)c->NP * c->NP * 8, c->stream));
   if (nocc > 0) {
     HIPCHK(c, hipMalloc((void **)&dC, (size_t)ispin * n * nocc * 8));
+    tmp.keep(dC);
     HIPCHK(c, hipMalloc((void **)&docc, (size_t)ispin * nocc * 8));
+    tmp.keep(docc);
     std::vector<double> Cp((size_t)ispin * n * nocc);  // rows in the internal function order
     for (int sp = 0; sp < ispin; ++sp)
       for (int k = 0; k < nocc; ++k)
@@ -1638,8 +1654,6 @@ extern "C" int pgxc_set_orbitals(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32
     HIPCHK(c, hipGetLastError());
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  dfree(dC);
-  dfree(docc);
   c->have_density = 1;
   c->density_mode = mode;
   return PGXC_OK;

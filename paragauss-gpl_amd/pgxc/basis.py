@@ -112,6 +112,11 @@ SHELL_SETS = {
         "C": [(0, 6, 1, 2), (1, 4, 1, 1), (2, 1, 1, 0)],
         "N": [(0, 6, 1, 2), (1, 4, 1, 1), (2, 1, 1, 0)],
     },
+    # angular momenta up to g: O 2s2p1d1f1g (43 functions), H 2s1p1d (10)
+    "spdfg": {
+        "O": [(0, 4, 1, 1), (1, 3, 1, 1), (2, 1, 1, 0), (3, 1, 1, 0), (4, 1, 1, 0)],
+        "H": [(0, 3, 1, 1), (1, 1, 1, 0), (2, 1, 1, 0)],
+    },
     # 33 functions per water: O 4s3p2d, H 2s1p
     "medium": {
         "O": [(0, 8, 1, 3), (1, 5, 1, 2), (2, 2, 2, 0)],

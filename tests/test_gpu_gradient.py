@@ -80,10 +80,6 @@ def test_gradient_open_shell(water):
 
 def test_gradient_f_and_g_shells():
     els, xyz = systems.h2o()
-    basis.SHELL_SETS.setdefault("spdfg", {
-        "O": [(0, 4, 1, 1), (1, 3, 1, 1), (2, 1, 1, 0), (3, 1, 1, 0), (4, 1, 1, 0)],
-        "H": [(0, 3, 1, 1), (1, 1, 1, 0), (2, 1, 1, 0)],
-    })
     b = basis.seeded_basis(els, xyz, seed=7, shell_set="spdfg", exp_range=(0.1, 500.0))
     g = grid.molecular_grid(els, xyz, nrad=24, nang=302)
     C, occ = systems.random_orbitals(b.nbas, 5, seed=8)

@@ -157,6 +157,32 @@ def test_water_cluster_medium_basis_fp64():
     check(res, ref)
 
 
+@pytest.mark.parametrize("functional,mask", [(PBE, oracle.MASK_PBE), (LDA, oracle.MASK_LDA)])
+def test_spdfg_basis_matches_oracle(functional, mask):
+    """Shells up to l = 4 (O 2s2p1d1f1g, H 2s1p1d) on a water dimer: the
+    f and g solid harmonics, their gradients and the cut-off screening across
+    atoms, against the oracle at the FP64 bar; the mixed full builds within
+    their bars; an incremental split-BF16 build at ||dP||_F = 1e-4."""
+    els, xyz = systems.water_cluster(2, seed=5)
+    b = basis.seeded_basis(els, xyz, seed=7, shell_set="spdfg", exp_range=(0.1, 500.0))
+    g = grid.molecular_grid(els, xyz, nrad=40, nang=302)
+    C, occ = systems.random_orbitals(b.nbas, 10, seed=8)
+    C1 = systems.perturbed_orbitals(C, 1e-4, seed=9)
+    ref = oracle.xc_build(b, g, mask, C=C, occ=occ, nthreads=NT)
+    ref1 = oracle.xc_build(b, g, mask, C=C1, occ=occ, nthreads=NT)
+    vmax = np.max(np.abs(ref[0]))
+    with XCBuilder(b, g, chunk_points=16384) as xc:
+        xc.set_orbitals(C, occ)
+        check(xc.build(functional, FP64), ref)
+        for prec, rel in ((FP32, 2e-6), (BF16X3, 5e-5)):
+            check(xc.build(functional, prec), ref, e_tol=rel * abs(ref[1]), v_tol=10 * rel * vmax,
+                  q_tol=rel * abs(ref[2]))
+        xc.build(functional, FP64)
+        xc.set_orbitals(C1, occ, incremental=True)
+        check(xc.build(functional, BF16X3), ref1, e_tol=1e-9 + 5e-7, v_tol=1e-10 + 5e-7,
+              q_tol=1e-9 + 5e-7)
+
+
 def test_becke_gpu_matches_numpy():
     els, xyz = systems.water_cluster(2, seed=1)
     g_np = grid.molecular_grid(els, xyz, nrad=30, nang=302, partition="numpy")
