@@ -637,10 +637,10 @@ def main():
     # of one contraction: half of pgxc_last_stats' sum over 128-point tiles of
     # 4 B n_b^2 + 16 B n_b (n_b: functions not cut off on the tile; the 16 B n_b
     # elementwise share is ~1 %); the dense equivalent 2 npts n^2 beside it.
-    # The basis stage times two kernels (k_basis, then k_split in split-BF16
-    # builds), each shorter per chunk than the density contraction (kernel
-    # trace of --workload h2o32-profile), so the dominant kernel is the larger
-    # contraction; the basis stage gets its own HBM roofline below.
+    # The basis stage (k_basis) is shorter per chunk than the density
+    # contraction (kernel trace of --workload h2o32-profile), so the dominant
+    # kernel is the larger contraction; the basis stage gets its own HBM
+    # roofline below.
     st = st1
     dom = max(("density", "potential"), key=lambda k: st[k])
     tf = layout["tile"]
@@ -660,11 +660,11 @@ def main():
             "flops_per_build_executed": executed[best],
             "flops_per_build_algorithmic": stats["flops"],
             "flops_per_build_dense": 4.0 * npts * n * n + 16.0 * npts * n}
-    # basis stage (k_basis + k_split) against the HBM roof: FP32 values and
+    # basis stage (k_basis) against the HBM roof: FP32 values and
     # gradients of the live functions written once, and the grid read
     bytes_basis = 32.0 * npts + 16.0 * stats["bn"]
     ach_b = bytes_basis / (st["basis"] * 1e-3) / 1e12
-    roof_basis = {"bound": "hbm", "kernels": "k_basis<true,float>" + (" + k_split" if best == "bf16x3" else ""),
+    roof_basis = {"bound": "hbm", "kernels": "k_basis<true,float>",
                   "stage_ms_one_lane": st["basis"], "bytes_per_build_algorithmic": bytes_basis,
                   "achieved": ach_b, "peak": 8.0, "unit": "TB/s", "frac": ach_b / 8.0,
                   "traffic": profiled_traffic("k_basis<true,float>")}

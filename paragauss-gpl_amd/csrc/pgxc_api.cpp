@@ -84,8 +84,6 @@ struct pgxc_ctx {
   void *phi_base = nullptr;  // Phi of the current build: chunk k at phi_base + k * phi_stride
   size_t phi_stride = 0;
   int skip_basis = 0;        // the current build reads cached basis values
-  int split_valid = 0;       // the FP32 cache holds its split-BF16 planes too
-  int run_split = 0;         // the current build writes the split planes
   double *part = nullptr, *coef = nullptr, *Hpart = nullptr;
   // open shell (ispin = 2): the fields above and below serve the alpha spin,
   // these the beta spin (allocated with the first open-shell density; the
@@ -135,11 +133,9 @@ struct pgxc_ctx {
   uint32_t *di = nullptr, *pi = nullptr;
   // blocks of the basis values some stage reads (see setup_screen): per
   // group the value and gradient run words, per chunk the basis items
-  // (256-point block << 16 | atom) and split items (64-point block << 16 |
-  // 64-function block) that touch them
+  // (256-point block << 16 | atom) that touch them and their write flags
   uint32_t *bneed = nullptr, *bi = nullptr, *bfl = nullptr;
-  uint64_t *si = nullptr;
-  std::vector<long> h_bi_off, h_si_off;
+  std::vector<long> h_bi_off;
   std::vector<int> fpos;          // internal position of each caller function
   std::vector<int> fatom;         // atom of each internal function position
   std::vector<long> perm;         // caller index of each internal point
@@ -183,7 +179,7 @@ constexpr int kLists = 3 * kNumVariants;
 
 struct StepPlan {
   int nunits = 0;
-  StageList basis, func, func2, wmat, wmat2, split;  // func2, wmat2: open-shell units
+  StageList basis, func, func2, wmat, wmat2;  // func2, wmat2: open-shell units
   StageList dens[kLists], pot[kLists];
 };
 
@@ -254,7 +250,7 @@ void make_order(const std::vector<long> &base, const std::vector<double> &prio,
   }
 }
 
-enum ItemKind { K_BASIS, K_FUNC, K_WMAT, K_DENS, K_POT, K_SPLIT };
+enum ItemKind { K_BASIS, K_FUNC, K_WMAT, K_DENS, K_POT };
 
 long unit_items(const UnitD &U, ItemKind k, int tf) {
   switch (k) {
@@ -262,7 +258,6 @@ long unit_items(const UnitD &U, ItemKind k, int tf) {
     case K_FUNC: return items_functional(U.Mc);
     case K_WMAT: return items_wmat(U.Mc, U.NP);
     case K_DENS: return U.ndi;
-    case K_SPLIT: return U.bscreen ? U.nsi : items_split(U.Mc, U.NP);
     default: return U.npi;
   }
 }
@@ -417,7 +412,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     }
     return P;
   };
-  // per step: basis, func, wmat, dens lists, pot lists, split, func2, wmat2
+  // per step: basis, func, wmat, dens lists, pot lists, func2, wmat2
   std::vector<std::vector<Pending>> pend(nsteps);
   for (int k = 0; k < nsteps; ++k) {
     std::vector<UnitD> units;
@@ -439,9 +434,6 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.contr = c->contr;
       const int lane = k % lanes_of(c);  // work buffers of chunk k: its stream's lane
       U.phi = (char *)c->phi_base + (c->phi_stride ? (size_t)k * c->phi_stride : (size_t)lane * c->phi_lane);
-      if (precision != PGXC_PREC_FP64) {  // split planes after the four FP32 planes
-        U.phit = (char *)U.phi + 16 * (size_t)c->NP * c->Mc;
-      }
       U.W = (char *)c->W + (size_t)lane * c->w_lane;
       U.D = c->D;
       U.part = c->part + (size_t)lane * c->ntiles * 4 * c->Mc;
@@ -476,9 +468,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.bneed = c->bneed + (p0 / kGroup) * 2 * c->nw8;
       U.bitems = c->bi + c->h_bi_off[k];
       U.bflags = c->bfl + 8 * c->h_bi_off[k];
-      U.sitems = c->si + c->h_si_off[k];
       U.nbi = c->h_bi_off[k + 1] - c->h_bi_off[k];
-      U.nsi = c->h_si_off[k + 1] - c->h_si_off[k];
       if (c->ispin == 2) {  // the beta spin's unit: same points and basis values
         UnitD B = U;
         B.W = (char *)c->beta.W + (size_t)lane * c->w_lane;
@@ -549,12 +539,6 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
         }
         pend[k].push_back(stage(sub, subm, kind == 0 ? K_DENS : K_POT, kTiles[t], k));
       }
-    {  // split-BF16 operand planes of the basis values, unless already cached
-      std::vector<UnitD> su;
-      std::vector<int> sm;
-      if (precision == PGXC_PREC_BF16X3) pick(0, [](const pgxc_ctx *c) { return c->run_split != 0; }, su, sm);
-      pend[k].push_back(stage(su, sm, K_SPLIT, 0, k));
-    }
     {  // spin functional over the alpha units of open-shell molecules
       std::vector<UnitD> fu;
       std::vector<int> fm;
@@ -677,9 +661,8 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     S.wmat = mk(pend[k][2]);
     for (int li = 0; li < kLists; ++li) S.dens[li] = mk(pend[k][3 + li]);
     for (int li = 0; li < kLists; ++li) S.pot[li] = mk(pend[k][3 + kLists + li]);
-    S.split = mk(pend[k][3 + 2 * kLists]);
-    S.func2 = mk(pend[k][4 + 2 * kLists]);
-    S.wmat2 = mk(pend[k][5 + 2 * kLists]);
+    S.func2 = mk(pend[k][3 + 2 * kLists]);
+    S.wmat2 = mk(pend[k][4 + 2 * kLists]);
     S.nunits = pend[k][0].nu;
   }
   plan->nlanes = plan_lanes;
@@ -751,7 +734,6 @@ static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent
     hipEvent_t *E = ev ? ev->data() + kEvStep + 5 * k : nullptr;
     if (E) (void)hipEventRecord(E[0], t);
     launch_basis<T>(t, gga, S.basis);
-    if (x3) launch_split(t, S.split);
     if (E) (void)hipEventRecord(E[1], t);
     for (int li = 0; li < kLists; ++li) {
       const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;
@@ -802,14 +784,13 @@ static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent
 static std::mutex g_cache_mu;
 static std::map<int, size_t> g_cache_bytes;
 
-// bytes of basis values per (function, point): 4 planes, plus for FP32 the
-// transposed split-BF16 planes of the values (2 x 2 bytes)
-static size_t phi_elem_bytes(size_t elem) { return elem == 8 ? 32 : 20; }
+// bytes of basis values per (function, point): value and gradient planes
+static size_t phi_elem_bytes(size_t elem) { return 4 * elem; }
 
 // Test switch PGXC_POISON_BASIS=1: basis buffers, when they get their first
 // contents (allocation, or the zeroing of the lane buffers at a change of
 // precision), are filled with NaN bytes wherever a build writes basis values
-// (rows of the real functions; the transposed split planes whole), so a stage
+// (rows of the real functions), so a stage
 // reading a block the basis stage left unwritten turns the results into NaN.
 static hipError_t poison_basis(const pgxc_ctx *c, void *base, size_t elem, size_t per_buf,
                                size_t nbuf, hipStream_t s) {
@@ -821,7 +802,6 @@ static hipError_t poison_basis(const pgxc_ctx *c, void *base, size_t elem, size_
     char *b = (char *)base + k * per_buf;
     for (int pl = 0; pl < 4 && e == hipSuccess; ++pl)
       e = hipMemsetAsync(b + pl * plane * elem, 0xff, rows * elem, s);
-    if (elem == 4 && e == hipSuccess) e = hipMemsetAsync(b + 16 * plane, 0xff, 4 * plane, s);
   }
   return e;
 }
@@ -896,14 +876,10 @@ static int ensure_work(pgxc_ctx *c, size_t elem, int precision, hipStream_t s, b
   const size_t stride = c->phic[e] ? need_phi : 0;
   // an LDA pass writes only the value planes: a GGA build after it evaluates again
   const int skip = c->phic[e] && c->phic_valid[e] && (c->phic_grad[e] || !grad);
-  const int rs = precision == PGXC_PREC_BF16X3 && !(c->phic[1] && c->split_valid);
-  if (base != c->phi_base || stride != c->phi_stride || skip != c->skip_basis ||
-      rs != c->run_split)
-    moved = true;
+  if (base != c->phi_base || stride != c->phi_stride || skip != c->skip_basis) moved = true;
   c->phi_base = base;
   c->phi_stride = stride;
   c->skip_basis = skip;
-  c->run_split = rs;
   if (moved) {  // their unit descriptors hold the old pointers
     free_plan(c->plan);
     free_plan(c->plan_e);
@@ -918,7 +894,6 @@ static void mark_basis_cached(pgxc_ctx *c, size_t elem, int precision, bool grad
     c->phic_valid[e] = 1;
     if (grad) c->phic_grad[e] = 1;
   }
-  if (c->phic[1] && precision == PGXC_PREC_BF16X3) c->split_valid = 1;
 }
 
 static int validate_build(pgxc_ctx *c, uint32_t functional, int32_t precision, int &incr) {
@@ -1171,8 +1146,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   // runs of the chunks the potential contraction reads there (wneed);
   // gradients on the tile's runs (density epilogue; the W pass reads the runs
   // not cut off on the group, a subset).  The basis stage writes exactly
-  // these blocks and the split those of the values; no stage reads any other
-  // block of a basis-value buffer.
+  // these blocks; no stage reads any other block of a basis-value buffer.
   {
     std::vector<uint32_t> need((size_t)ngr * 2 * nw8, 0u), tw((size_t)nw8);
     for (long t = 0; t < ntp; ++t) {
@@ -1199,9 +1173,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
         amask[(size_t)d.atom * nw8 + r / 32] |= 1u << (r % 32);
     }
     std::vector<uint32_t> bi, bfl, bm((size_t)nw8);
-    std::vector<uint64_t> si;
     c->h_bi_off.assign(1, 0);
-    c->h_si_off.assign(1, 0);
     const long gpb = kThreads / kGroup;  // groups per basis block
     for (int k = 0; k < c->nchunks; ++k) {
       const long g0 = (long)k * c->Mc / kGroup;
@@ -1235,24 +1207,10 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
         }
       }
       c->h_bi_off.push_back((long)bi.size());
-      for (long pb = 0; pb < c->Mc / 64; ++pb) {
-        const long g = g0 + 2 * pb;
-        for (int fb = 0; fb < (c->NP + 63) / 64; ++fb) {
-          // the eight runs 8 fb .. 8 fb + 7 lie in one word (runs past the
-          // last function have no bits)
-          const int w = 8 * fb / 32, sh = (8 * fb) % 32;
-          const uint64_t m0 = (need[(size_t)g * 2 * nw8 + w] >> sh) & 0xffu;
-          const uint64_t m1 = (need[(size_t)(g + 1) * 2 * nw8 + w] >> sh) & 0xffu;
-          // the item's runs read on each of its two groups ride along
-          if (m0 | m1) si.push_back((m0 | m1 << 8) << 32 | (uint64_t)(pb << 16 | fb));
-        }
-      }
-      c->h_si_off.push_back((long)si.size());
     }
     if ((rc = upload(c, &c->bneed, need))) return rc;
     if ((rc = upload(c, &c->bi, bi))) return rc;
     if ((rc = upload(c, &c->bfl, bfl))) return rc;
-    if ((rc = upload(c, &c->si, si))) return rc;
     if (getenv("PGXC_DEBUG")) {  // sizes of the screened basis work
       long live = 0, nv = 0, ng = 0;
       for (long g = 0; g < ngr; ++g)
@@ -1262,9 +1220,8 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
           ng += __builtin_popcount(need[(size_t)g * 2 * nw8 + nw8 + w]);
         }
       fprintf(stderr, "pgxc: %ld groups x %d runs; blocks live %ld, values read %ld, gradients read %ld; "
-                      "basis items %zu (of %ld), split items %zu (of %ld)\n",
-              ngr, n8, live, nv, ng, bi.size(), (long)c->nchunks * (c->Mc / kThreads) * c->n_atoms,
-              si.size(), (long)c->nchunks * (c->Mc / 64) * ((c->NP + 63) / 64));
+                      "basis items %zu (of %ld)\n",
+              ngr, n8, live, nv, ng, bi.size(), (long)c->nchunks * (c->Mc / kThreads) * c->n_atoms);
     }
   }
   // work items per chunk
@@ -2058,7 +2015,7 @@ int pgxc_destroy(pgxc_ctx *c) {
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
                   c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->su_off, c->su,
-                  c->gcf_off, c->gcf, c->di, c->pi, c->bneed, c->bi, c->bfl, c->si, c->beta.P, c->beta.Pref, c->beta.D,
+                  c->gcf_off, c->gcf, c->di, c->pi, c->bneed, c->bi, c->bfl, c->beta.P, c->beta.Pref, c->beta.D,
                   c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
                   c->beta.Vstate, c->beta.Vout, c->eterm};
   for (void *b : bufs) dfree(b);
@@ -2155,14 +2112,13 @@ int pgxc_batch_build(pgxc_batch *b, uint32_t functional, int32_t precision, int3
   for (int m = 0; m < nm; ++m) {
     pgxc_ctx *c = b->mols[m];
     const void *phi0 = c->phi_base, *w0 = c->W, *wb0 = c->beta.W;
-    const int skip0 = c->skip_basis, split0 = c->run_split;
+    const int skip0 = c->skip_basis;
     int rc = ensure_work(c, elem, precision, s, gga);
     if (rc) {
       b->err = "molecule " + std::to_string(m) + ": " + c->err;
       return rc;
     }
-    moved = moved || phi0 != c->phi_base || w0 != c->W || wb0 != c->beta.W || skip0 != c->skip_basis ||
-            split0 != c->run_split;
+    moved = moved || phi0 != c->phi_base || w0 != c->W || wb0 != c->beta.W || skip0 != c->skip_basis;
   }
   if (moved) free_plan(b->plan);
   int rc = get_plan(b->mols, functional, precision, incr, b->prio, b->order_version, b->sched,

@@ -44,7 +44,6 @@ struct UnitD {
   const ShellD *shells;
   const double *exps, *norms, *contr;
   void *phi;        // [4][NP][Mc] values, d/dx, d/dy, d/dz (T)
-  void *phit;       // split-BF16 builds: the values as bf16 hi / lo planes, transposed [2][Mc][NP]
   void *W;          // [NP][Mc] (T); split-BF16 builds: bf16 hi / lo planes [2][NP][Mc]
   const void *D;    // [NP][NP] density operand (T); split-BF16: transposed hi / lo planes
   double *part;     // [NP/tf][4][Mc] density partials
@@ -82,13 +81,9 @@ struct UnitD {
   // the runs whose values, then the runs whose gradients a stage reads there.
   // bitems: basis items (256-point block << 16 | atom) touching them, with
   // bflags: 8 words per item, bit 8 j + g of words 0-3 (4-7): the values
-  // (gradients) of the atom's shell j are read on group g of the block;
-  // sitems: split items (64-point block << 16 | 64-function block), the
-  // item's eight runs read on its first group in bits 32-39, on its second
-  // in bits 40-47.
+  // (gradients) of the atom's shell j are read on group g of the block.
   const uint32_t *bneed, *bitems, *bflags;
-  const uint64_t *sitems;
-  long nbi, nsi;
+  long nbi;
   int bscreen, pad_b;
   long Mc, npad;
   int n_atoms, NP, ntiles, nsplit, incremental;
@@ -144,8 +139,6 @@ void launch_energy(hipStream_t s, bool gga, bool spin, const StageList &L);
 // coef, coef2) from one read of the basis values
 template <typename T>
 void launch_wmat(hipStream_t s, bool grad, bool split, bool two, const StageList &L);
-// split-BF16 operand planes of the basis values (item = 32 functions x 64 points)
-void launch_split(hipStream_t s, const StageList &L);
 template <typename T>
 void launch_potential(hipStream_t s, int tf, int var, const StageList &L);
 void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L);
@@ -166,7 +159,6 @@ void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, lon
 inline long items_basis(long Mc, int n_atoms) { return (Mc / kThreads) * n_atoms; }
 inline long items_functional(long Mc) { return Mc / kThreads; }
 inline long items_wmat(long Mc, int NP) { return (Mc / (4 * kThreads)) * (NP / 16); }
-inline long items_split(long Mc, int NP) { return (Mc / 64) * ((NP + 63) / 64); }
 // point splits of the potential contraction are whole multiples of this
 constexpr int kSplitPts = 2048;
 

@@ -4,11 +4,11 @@
 //   hi = bf16(x), lo = bf16(x - hi)
 // and a product is formed from three v_mfma_f32_16x16x32_bf16 passes,
 //   a*b ~ a_lo*b_hi + a_hi*b_lo + a_hi*b_hi,   FP32 accumulation,
-// dropping a_lo*b_lo (relative 2^-16).  The build stores its operands already
-// split, k-contiguous (the transposed basis values once per geometry by
-// k_split, W by k_wmat, D^T by k_prep), so staging is a copy of 16-byte
-// vectors into LDS; the potential's basis-value operand is read in FP32 and
-// split while it is staged.
+// dropping a_lo*b_lo (relative 2^-16).  W (k_wmat) and D^T (k_prep) are
+// stored split and k-contiguous, so their staging is a copy of 16-byte
+// vectors into LDS; the basis values are read in FP32 (as stored, points
+// contiguous) and split while they are staged, k-contiguous for the
+// potential and transposed for the density.
 // Same tiles and epilogues as the FP32/FP64 contractions (xc_kernels.hip);
 // K advances 32 per MFMA.
 #include <type_traits>
@@ -167,7 +167,75 @@ struct PFOperand {
   }
 };
 
-// OA / OB: operand staging (PSOperand<R, NT>, rows set by init).  kofs(kt,
+// Staging of an FP32 operand stored with its rows contiguous ([k][row]: the
+// basis values, points contiguous) for an MFMA that takes it k-contiguous:
+// per stage R rows x 4 k runs; a thread takes one (row, k run) pair, loads
+// the run's 8 values of its row (8 loads of 4 bytes, k apart by ld elements;
+// a wave's 64 threads hold 64 consecutive rows of one run, so each load
+// reads 256 contiguous bytes), splits them with split_bf16 and writes 16
+// bytes of hi and 16 of lo: the LDS image PSOperand writes from planes that
+// were split and transposed beforehand, and the same bits.  The row pointers
+// are set once per item; kofs(kt, c) gives the first k of run c of step kt
+// (or -1 for zeros), and the loads run unconditionally as in PSOperand.
+template <int R, int NT = kThreads>
+struct PTOperand {
+  static constexpr int NR = R * (kXBK / 8);  // (row, k run) pairs per stage
+  static constexpr int PER = NR / NT;
+  static_assert(PER * NT == NR, "whole runs per thread");
+  const float *row[PER];  // plane + row offset of pair e, nullptr: row absent
+  const float *dummy;
+  long ld;
+  float v[2][PER][8];
+  unsigned ok[2];
+
+  template <class RowF>
+  __device__ __forceinline__ void init(const float *plane, long ld_, const RowF &rowf) {
+    dummy = plane;
+    ld = ld_;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const long o = rowf((int)((threadIdx.x + NT * e) % R));
+      row[e] = o >= 0 ? plane + o : nullptr;
+    }
+  }
+  template <int SL, class KOfs>
+  __device__ __forceinline__ void load(const KOfs &kofs, int kt, int ksteps) {
+    unsigned m = 0;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int c = (int)((threadIdx.x + NT * e) / R);
+      const long k0 = kt < ksteps ? kofs(kt, c) : -1;
+      const bool g = row[e] != nullptr && k0 >= 0;
+      const float *p = g ? row[e] + k0 * ld : dummy;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[SL][e][j] = gld(p + j * ld);
+      m |= (unsigned)g << e;
+    }
+    ok[SL] = m;
+  }
+  template <int SL>
+  __device__ __forceinline__ void store(__bf16 *hi, __bf16 *lo) const {
+    typedef uint16_t u8v __attribute__((ext_vector_type(8)));
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int r = (int)((threadIdx.x + NT * e) % R), c = (int)((threadIdx.x + NT * e) / R);
+      const bool g = (ok[SL] >> e) & 1u;
+      u8v h, l;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        __bf16 hh, ll;
+        split_bf16(g ? v[SL][e][j] : 0.0f, hh, ll);
+        h[j] = __builtin_bit_cast(uint16_t, hh);
+        l[j] = __builtin_bit_cast(uint16_t, ll);
+      }
+      *(u8v *)(hi + r * kXBKP + 8 * c) = h;
+      *(u8v *)(lo + r * kXBKP + 8 * c) = l;
+    }
+  }
+};
+
+// OA / OB: operand staging (PSOperand, PFOperand or PTOperand, rows set by
+// init).  kofs(kt,
 // c): element offset of k run c (8 k) of step kt, or -1 for zeros.  The NT
 // threads form NT / 64 waves in a (NT / 64 / WGN) x WGN grid, each with a
 // (BM / rows) x (BN / WGN) tile of 16x16 blocks; acc holds one wave's blocks.
@@ -287,12 +355,13 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(2)
     lds_list(U.tl + off, c8, kl);
     const uint16_t *crun = kl + ch * U.CH;
     const int ncr = min(2 * U.CH, c8 - ch * U.CH), nslots = ncr > U.CH ? 2 : 1;
-    // A(p, k) = Phi^T[m0 + p][f(k)], B(k, n) = D^T[col(n)][f(k)]
-    const __bf16 *th = (const __bf16 *)U.phit, *dh = (const __bf16 *)U.D;
-    const long np2 = NP * NP, pn = U.Mc * NP;
-    PSOperand<BM, NT> opa;
+    // A(p, k) = Phi[f(k)][m0 + p] (FP32 values, split while staged),
+    // B(k, n) = D^T[col(n)][f(k)]
+    const __bf16 *dh = (const __bf16 *)U.D;
+    const long np2 = NP * NP;
+    PTOperand<BM, NT> opa;
     PSOperand<BN2, NT> opb;
-    opa.init({th, th + pn}, [&](int m) -> long { return (m0 + m) * NP; });
+    opa.init((const float *)U.phi, U.Mc, [&](int m) -> long { return m0 + m; });
     opb.init({dh, dh + np2}, [&](int n) -> long {
       const long c = run_fn(crun, ncr, n);
       return c < 0 ? -1 : c * NP;
@@ -377,83 +446,6 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
     item_log(S, it, t0);
     L = walk_take(S, L, ahead, &slot);
   }
-}
-
-// ------------------------------------------------------------------ split
-// item = (64 functions, 64 points) of a unit: the FP32 basis values become
-// bf16 hi / lo planes transposed, [p][n] (A operand of the density
-// contraction; the potential contraction splits the values as it stages
-// them), through an LDS tile so reads and writes run along their contiguous
-// dimension, 16 bytes per store (a row segment of 64 functions is 128 bytes).
-// Screened units walk the items touching blocks whose values a contraction
-// reads, and split only those (run of 8 functions x 32-point group) blocks.
-__global__ __launch_bounds__(kThreads) void k_split(const StageList S) {
-  typedef uint16_t u8v __attribute__((ext_vector_type(8)));
-  __shared__ float t[64][65];
-  const int tid = threadIdx.x;
-  for (long L = blockIdx.x; L < S.total; L += gridDim.x) {
-    const int u = find_unit_x3(S.base, S.nu, L);
-    const UnitD &U = S.units[u];
-    const long local = L - S.base[u];
-    const int nfbk = (U.NP + 63) / 64;
-    long n0, p0;
-    uint32_t lm = 0xffffu;  // bit 8 g + run: the run's values are read on group g
-    if (U.bscreen) {
-      const uint64_t e = U.sitems[local];
-      n0 = (long)(e & 0xffffu) * 64;
-      p0 = (long)((e >> 16) & 0xffffu) * 64;
-      lm = (uint32_t)(e >> 32);
-    } else {
-      n0 = (local % nfbk) * 64;
-      p0 = (local / nfbk) * 64;
-    }
-    const long Mc = U.Mc, NP = U.NP, pn = NP * Mc;
-    const int nfr = (int)(NP - n0 < 64 ? NP - n0 : 64);  // NP is a multiple of 32
-    const float *phi = (const float *)U.phi;
-    uint16_t *th = (uint16_t *)U.phit;
-    // whether the block of function n0 + r and point p0 + c is read
-    auto live = [&](int r, int c) -> bool { return (lm >> (8 * (c / kGroup) + r / kRun)) & 1u; };
-    {  // function row r, points c0 .. c0 + 15 (one group)
-      const int r = tid >> 2, c0 = (tid & 3) * 16;
-      if (r < nfr && live(r, c0)) {
-        const long q = (n0 + r) * Mc + p0 + c0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const f4 v = gld((const f4 *)(phi + q + 4 * j));
-#pragma unroll
-          for (int e = 0; e < 4; ++e) t[r][c0 + 4 * j + e] = v[e];
-        }
-      }
-    }
-    __syncthreads();
-    {  // point row pr, functions f0 .. f0 + 15 as two runs of 8
-      const int pr = tid >> 2, f0 = (tid & 3) * 16;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int fa = f0 + 8 * h;
-        if (fa >= nfr || !live(fa, pr)) continue;
-        u8v hi, lo;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          __bf16 hh, ll;
-          split_bf16(t[fa + e][pr], hh, ll);
-          hi[e] = __builtin_bit_cast(uint16_t, hh);
-          lo[e] = __builtin_bit_cast(uint16_t, ll);
-        }
-        const long q = (p0 + pr) * NP + n0 + fa;
-        gst((u8v *)(th + q), hi);
-        gst((u8v *)(th + pn + q), lo);
-      }
-    }
-    __syncthreads();
-  }
-}
-
-void launch_split(hipStream_t s, const StageList &L) {
-  if (!L.total) return;
-  const long cap = 256L * 8;
-  const unsigned g = (unsigned)(L.total < cap ? L.total : cap);
-  hipLaunchKernelGGL(k_split, dim3(g), dim3(kThreads), 0, s, L);
 }
 
 void launch_potential_bf16x3(hipStream_t s, int tf, const StageList &L) {
