@@ -58,6 +58,7 @@ struct pgxc_ctx {
   std::string err;
   // sizes
   int n_atoms = 0, n_shells = 0, nbas = 0, NP = 0, veclen = 128, tf = 128;
+  int lmax = 0;  // largest angular momentum of the basis
   int var[2] = {0, 1};  // contraction variant for FP64 and for FP32 builds
   long npts = 0, npad = 0, Mc = 0;
   int nchunks = 0, nsplit = 1, ntiles = 0;
@@ -378,6 +379,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     long total = 0;
     bool has_order = false, has_queues = false, has_log = false;
     size_t frow = 0;  // first feature row of the list's items
+    int lmax = 0;     // basis lists: the largest angular momentum of their molecules
   };
   std::vector<double> flops;  // counted flops per feature row
   size_t plan_ctr_off_tmp = 0;
@@ -511,6 +513,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
           bm.push_back(umol[i]);
         }
       pend[k].push_back(stage(bu, bm, K_BASIS, 0, k));
+      for (int m : bm) pend[k].back().lmax = std::max(pend[k].back().lmax, mols[m]->lmax);
     }
     {
       std::vector<UnitD> fu;
@@ -650,6 +653,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
     L.ctr = P.has_queues ? (int *)(B + P.ctr_off) : nullptr;
     L.qbeg = P.has_queues ? (const long *)(B + P.qbeg_off) : nullptr;
     L.tlog = P.has_log ? (uint32_t *)(B + P.tlog_off) : nullptr;
+    L.lmax = P.lmax;
     if (P.has_log) plan->logs.push_back({L.tlog, P.total});
     return L;
   };
@@ -1342,6 +1346,7 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
     }
     at[d.atom].shell_end = s + 1;
     at[d.atom].lmax = std::max(at[d.atom].lmax, d.l);
+    c->lmax = std::max(c->lmax, d.l);
     d.exp_off = (int)eoff;
     d.contr_off = (int)coff;
     d.func_off = (int)foff;

@@ -121,6 +121,7 @@ struct StageList {
   int *ctr;
   const long *qbeg;
   uint32_t *tlog;
+  int lmax;  // basis lists: the largest angular momentum of their atoms
 };
 
 // ---- launchers (kernels live in xc_kernels.hip / xc_bf16x3.hip) -----------

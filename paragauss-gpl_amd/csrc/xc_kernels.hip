@@ -559,7 +559,7 @@ __device__ __forceinline__ long item_at(const int *__restrict__ order, long L, l
 // waits on no global load.  Screened units (bscreen) walk their list of the
 // (block, atom) pairs whose values a later stage reads, and write only the
 // shells read on each point's group; dense units write every shell.
-template <bool GRAD, typename T>
+template <bool GRAD, typename T, int LMAX>
 __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ units, int nu,
                                                     const long *__restrict__ base,
                                                     const int *__restrict__ order, long total) {
@@ -608,12 +608,14 @@ __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ un
     T *phi = (T *)U.phi;
     const long cs = (long)U.NP * U.Mc;
     const int grp = tid / kGroup;
+    // the kernel holds the registers of its largest case, so it is built
+    // for the largest angular momentum of the list (at.lmax <= LMAX)
     switch (at.lmax) {
       case 0: basis_atom<0, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-      case 1: basis_atom<1, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-      case 2: basis_atom<2, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-      case 3: basis_atom<3, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-      default: basis_atom<4, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+      case 1: if constexpr (LMAX >= 1) basis_atom<1, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+      case 2: if constexpr (LMAX >= 2) basis_atom<2, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+      case 3: if constexpr (LMAX >= 3) basis_atom<3, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+      default: if constexpr (LMAX >= 4) basis_atom<4, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
     }
   }
 }
@@ -1360,14 +1362,24 @@ unsigned contraction_grid(const StageList &L, int per_cu) {
   return (unsigned)g;
 }
 
+template <typename T, int LMAX>
+static void basis_l(hipStream_t s, bool grad, const StageList &L) {
+  dim3 g(persistent_grid(L.total, 8));
+  if (grad)
+    hipLaunchKernelGGL((k_basis<true, T, LMAX>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  else
+    hipLaunchKernelGGL((k_basis<false, T, LMAX>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
 template <typename T>
 void launch_basis(hipStream_t s, bool grad, const StageList &L) {
   if (!L.total) return;
-  dim3 g(persistent_grid(L.total, 8));
-  if (grad)
-    hipLaunchKernelGGL((k_basis<true, T>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
-  else
-    hipLaunchKernelGGL((k_basis<false, T>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  switch (L.lmax) {
+    case 0: case 1: basis_l<T, 1>(s, grad, L); break;
+    case 2: basis_l<T, 2>(s, grad, L); break;
+    case 3: basis_l<T, 3>(s, grad, L); break;
+    default: basis_l<T, 4>(s, grad, L); break;
+  }
 }
 
 template <typename T, int BN, int VAR>
