@@ -1,19 +1,25 @@
 #!/usr/bin/env python3
 """XC-build throughput on one MI355X (grid points per second).
 
-Workload: the (H2O)32 cluster (96 atoms, 1056 basis functions, ~1.9 M
+Workload: the (H2O)32 cluster (96 atoms, 1056 basis functions, 1.96 M
 Becke-partitioned grid points, 75 radial x 302 Lebedev per atom, 160
-occupied orbitals, PBE).  A step is one full XC build over the whole grid.
-  value      : mixed-precision build run as an incremental difference-density
-               build (each step a new density, ||dP||_F = 1e-4 from the last),
-               inputs resident in HBM; device time of the build.  The
-               fastest of the FP32-MFMA and split-BF16 forms whose E_xc stays
-               within 1e-6 Eh of the FP64 build is reported.
-  fp64_*     : the plain FP64 GPU build of the same workload.
-  dE_vs_fp64 : E_xc after the chain of mixed incremental builds minus E_xc of
-               a full FP64 build of the final density (the FP64 build
-               matches the CPU oracle to 1e-9 Eh in tests/test_gpu_parity.py);
-               dV_vs_fp64 the same for max |V_xc|.
+occupied orbitals, PBE).  A step is one full XC build over the whole grid,
+the basis values and gradients evaluated in it (as the reference evaluates
+them in every build).
+  value        : mixed-precision build run as an incremental difference-density
+                 build (each step a new density, ||dP||_F = 1e-4 from the
+                 last), inputs resident in HBM; device time of the build.  The
+                 fastest of the FP32-MFMA and split-BF16 forms whose E_xc
+                 stays within 1e-6 Eh of the FP64 build is reported.
+  fp64_*       : the plain FP64 GPU build of the same workload.
+  dE_vs_fp64   : E_xc after the chain of mixed incremental builds minus E_xc
+                 of a full FP64 build of the final density; dV_vs_fp64 the
+                 same for max |V_xc|.
+  cpu_baseline : the whole FP64 build of the CPU oracle on the host's cores;
+                 its result also checks the GPU FP64 build at this size
+                 (gpu_fp64_vs_cpu).
+  warm_cache   : the same builds with the basis values kept in HBM across
+                 builds of the geometry.
 Prints one JSON line.
 """
 import argparse

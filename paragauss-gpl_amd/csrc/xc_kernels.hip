@@ -318,16 +318,19 @@ __device__ __forceinline__ void put_fn(long f0, int nf, T r0, T r1, T x, T y, T 
                                        const T *Cx, const T *Cy, const T *Cz, T *phi, long Mc,
                                        long cs, int p, bool wg) {
   constexpr int o = L * L;
+  // global (not flat) stores: a flat store also counts against the LDS wait
+  // counter, so every wait for the staged shell data would wait for the
+  // stores issued before it
 #pragma unroll
   for (int m = 0; m < 2 * L + 1; ++m) {
     const long q = (f0 + (long)m * nf) * Mc + p;
     const T sv = C[o + m];
-    phi[q] = sv * r0;
+    gst(phi + q, sv * r0);
     if (GRAD && wg) {
       const T hv = sv * r1;
-      phi[cs + q] = x * hv + Cx[o + m] * r0;
-      phi[2 * cs + q] = y * hv + Cy[o + m] * r0;
-      phi[3 * cs + q] = z * hv + Cz[o + m] * r0;
+      gst(phi + cs + q, x * hv + Cx[o + m] * r0);
+      gst(phi + 2 * cs + q, y * hv + Cy[o + m] * r0);
+      gst(phi + 3 * cs + q, z * hv + Cz[o + m] * r0);
     }
   }
 }
@@ -586,14 +589,14 @@ __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ un
     const int e0 = at.exp_begin, c0 = at.contr_begin;
     const long p = blk * kThreads + tid;
     // the point's centre offsets in FP64, loads in flight across the staging
-    const double gx = U.gx[p], gy = U.gy[p], gz = U.gz[p];
+    const double gx = gld(U.gx + p), gy = gld(U.gy + p), gz = gld(U.gz + p);
     __syncthreads();  // the previous item's reads of A are done
     for (int i = tid; i < nsh; i += kThreads) A.sh[i] = U.shells[s0 + i];
     for (int i = tid; i < at.n_exp; i += kThreads) {
-      A.ex[i] = (T)U.exps[e0 + i];
-      A.nm[i] = (T)U.norms[e0 + i];
+      A.ex[i] = (T)gld(U.exps + e0 + i);
+      A.nm[i] = (T)gld(U.norms + e0 + i);
     }
-    for (int i = tid; i < at.n_contr; i += kThreads) A.ct[i] = (T)U.contr[c0 + i];
+    for (int i = tid; i < at.n_contr; i += kThreads) A.ct[i] = (T)gld(U.contr + c0 + i);
     if (tid < nsh * (kThreads / kGroup)) {  // flag of (shell tid / 8, group tid % 8)
       uint32_t f = GRAD ? 3u : 1u;
       if (U.bscreen) {
