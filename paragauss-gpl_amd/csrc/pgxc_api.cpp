@@ -45,6 +45,19 @@ void dfree(void *p) {
   if (p) (void)hipFree(p);
 }
 
+// Device buffers freed when the scope ends, on every return path.
+struct ScopedDev {
+  std::vector<void *> bufs;
+  ~ScopedDev() {
+    for (void *b : bufs) dfree(b);
+  }
+  template <typename T>
+  T *keep(T *p) {
+    bufs.push_back((void *)p);
+    return p;
+  }
+};
+
 }  // namespace
 
 struct pgxc_ctx {
@@ -122,13 +135,13 @@ struct pgxc_ctx {
   double sum_bn = 0, sum_n2 = 0;  // sums over 128-point tiles of B n_b and n_b^2 (screened)
   // host and device copies: per tile run lists, per split run unions and
   // [group][chunk] flags, per chunk density and potential items
-  std::vector<int> h_tl_off, h_su_off;
-  std::vector<uint16_t> h_tl, h_su;
+  std::vector<int> h_tl_off, h_su_off, h_tf_off;
+  std::vector<uint16_t> h_tl, h_su, h_tfl;
   std::vector<long> h_gcf_off, h_di_off, h_pi_off;
   std::vector<uint8_t> h_gcf;
   std::vector<uint32_t> h_di, h_pi;
-  int *tl_off = nullptr, *su_off = nullptr;
-  uint16_t *tl = nullptr, *su = nullptr;
+  int *tl_off = nullptr, *su_off = nullptr, *tf_off = nullptr;
+  uint16_t *tl = nullptr, *su = nullptr, *tfl = nullptr;
   long *gcf_off = nullptr;
   uint8_t *gcf = nullptr;
   uint32_t *di = nullptr, *pi = nullptr;
@@ -292,7 +305,8 @@ static void unit_item_features(const pgxc_ctx *c, int step, ItemKind kind, int p
       const uint32_t e = c->h_di[(size_t)i];
       const long T = T0 + (e >> 8);
       const int pr = (int)(e & 255), c8 = c->h_tl_off[(size_t)T + 1] - c->h_tl_off[(size_t)T];
-      const double cols = (double)kRun * std::min(2 * CH, c8 - 2 * pr * CH);
+      const int nf = c->h_tf_off[(size_t)T + 1] - c->h_tf_off[(size_t)T];
+      const double cols = (double)std::min(2 * tf, nf - 2 * pr * tf);
       push(0, (double)c8 * kRun, c->NP, true, 2.0 * kScreenTile * cols * c8 * kRun);
     }
   } else {
@@ -460,6 +474,8 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.npi = c->h_pi_off[k + 1] - c->h_pi_off[k];
       U.tl_off = c->tl_off + p0 / kScreenTile;
       U.tl = c->tl;
+      U.tf_off = c->tf_off + p0 / kScreenTile;
+      U.tfl = c->tfl;
       U.su_off = c->su_off + (long)k * c->nsplit;
       U.su = c->su;
       U.gcf_off = c->gcf_off + (long)k * c->nsplit;
@@ -1075,14 +1091,21 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   if ((rc = upload(c, &c->sshells, ss))) return rc;
   if ((rc = dalloc(c, &c->nbt, ntp))) return rc;
   if ((rc = dalloc(c, &c->sig8, (size_t)ngr * nw8))) return rc;
+  const int nws = (c->n_shells + 31) / 32;
+  ScopedDev tmp;
+  uint32_t *d_smark = nullptr;
+  HIPCHK(c, hipMalloc((void **)&d_smark, sizeof(uint32_t) * (size_t)ntp * nws));
+  tmp.keep(d_smark);
   launch_screen(c->stream, ntp, c->grid, c->grid + c->npad, c->grid + 2 * c->npad, c->n_atoms,
-                c->atoms, c->n_shells, c->sshells, n8, c->nbt, c->sig8);
+                c->atoms, c->n_shells, c->sshells, n8, c->nbt, c->sig8, d_smark);
   HIPCHK(c, hipGetLastError());
   std::vector<int> nb((size_t)ntp);
-  std::vector<uint32_t> sig((size_t)ngr * nw8);
+  std::vector<uint32_t> sig((size_t)ngr * nw8), smark((size_t)ntp * nws);
   HIPCHK(c, hipMemcpyAsync(nb.data(), c->nbt, sizeof(int) * ntp, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(sig.data(), c->sig8, sizeof(uint32_t) * sig.size(), hipMemcpyDeviceToHost,
                            c->stream));
+  HIPCHK(c, hipMemcpyAsync(smark.data(), d_smark, sizeof(uint32_t) * smark.size(),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (on) {
     double f = 0.0;
@@ -1102,6 +1125,25 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
     for (long g = 0; g < ngr; ++g)
       for (int r = 0; r < n8; ++r) sig[(size_t)g * nw8 + r / 32] |= 1u << (r % 32);
     HIPCHK(c, hipMemcpy(c->sig8, sig.data(), sizeof(uint32_t) * sig.size(), hipMemcpyHostToDevice));
+    for (long t = 0; t < ntp; ++t)
+      for (int si = 0; si < c->n_shells; ++si) smark[(size_t)t * nws + si / 32] |= 1u << (si % 32);
+  }
+  // per 128-point tile: the functions of its marked shells, ascending (the
+  // density contraction's output columns)
+  {
+    c->h_tf_off.assign(1, 0);
+    c->h_tfl.clear();
+    std::vector<uint32_t> fm((size_t)(c->NP + 31) / 32);
+    for (long t = 0; t < ntp; ++t) {
+      std::fill(fm.begin(), fm.end(), 0u);
+      for (int si = 0; si < c->n_shells; ++si)
+        if ((smark[(size_t)t * nws + si / 32] >> (si % 32)) & 1u)
+          for (int f = ss[(size_t)si].func_off; f < ss[(size_t)si].func_off + ss[(size_t)si].nfunc; ++f)
+            fm[(size_t)f / 32] |= 1u << (f % 32);
+      for (int f = 0; f < c->NP; ++f)
+        if ((fm[(size_t)f / 32] >> (f % 32)) & 1u) c->h_tfl.push_back((uint16_t)f);
+      c->h_tf_off.push_back((int)c->h_tfl.size());
+    }
   }
   auto bit = [&](long g, int r) { return (sig[(size_t)g * nw8 + r / 32] >> (r % 32)) & 1u; };
   // per 128-point tile: its runs (union over its four groups)
@@ -1237,9 +1279,10 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   for (int k = 0; k < c->nchunks; ++k) {
     for (long t = 0; t < tpc; ++t) {
       const long T = k * tpc + t;
-      const int c8 = c->h_tl_off[(size_t)T + 1] - c->h_tl_off[(size_t)T];
-      // density items: pairs of column chunks of the tile's runs
-      for (int pr = 0; pr < (c8 + 2 * CH - 1) / (2 * CH); ++pr) c->h_di.push_back((uint32_t)(t << 8 | pr));
+      const int nf = c->h_tf_off[(size_t)T + 1] - c->h_tf_off[(size_t)T];
+      // density items: pairs of column chunks (tf functions) of the tile's functions
+      for (int pr = 0; pr < (nf + 2 * c->tf - 1) / (2 * c->tf); ++pr)
+        c->h_di.push_back((uint32_t)(t << 8 | pr));
     }
     c->h_di_off.push_back((long)c->h_di.size());
     for (int sl = 0; sl < c->nsplit; ++sl) {
@@ -1253,6 +1296,8 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   }
   if ((rc = upload(c, &c->tl_off, c->h_tl_off))) return rc;
   if ((rc = upload(c, &c->tl, c->h_tl))) return rc;
+  if ((rc = upload(c, &c->tf_off, c->h_tf_off))) return rc;
+  if ((rc = upload(c, &c->tfl, c->h_tfl))) return rc;
   if ((rc = upload(c, &c->su_off, c->h_su_off))) return rc;
   if ((rc = upload(c, &c->su, c->h_su))) return rc;
   if ((rc = upload(c, &c->gcf_off, c->h_gcf_off))) return rc;
@@ -1298,6 +1343,8 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   c->cache_off = opt ? opt->no_basis_cache : 0;
   if (opt && opt->variant_fp64 >= 0) c->var[0] = opt->variant_fp64;
   if (opt && opt->variant_fp32 >= 0) c->var[1] = opt->variant_fp32;
+  if (n_shells > kMaxShells)
+    return fail(c, PGXC_ERR_UNSUPPORTED, "pgxc_create: more than " + std::to_string(kMaxShells) + " shells");
   c->n_atoms = n_atoms;
   c->n_shells = n_shells;
   c->npts = npts;
@@ -1575,17 +1622,6 @@ extern "C" int pgxc_set_density(pgxc_ctx *c, int32_t ispin, int32_t nbas, const 
 }
 
 // Temporary device buffers of a call, freed when the call returns on any path.
-struct ScopedDev {
-  std::vector<void *> bufs;
-  ~ScopedDev() {
-    for (void *b : bufs) dfree(b);
-  }
-  template <typename T>
-  T *keep(T *p) {
-    bufs.push_back((void *)p);
-    return p;
-  }
-};
 
 extern "C" int pgxc_set_orbitals(pgxc_ctx *c, int32_t ispin, int32_t nbas, int32_t nocc,
                                  const double *C, const double *occ, int32_t mode) {
@@ -2019,7 +2055,7 @@ int pgxc_destroy(pgxc_ctx *c) {
   void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
-                  c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->su_off, c->su,
+                  c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->tf_off, c->tfl, c->su_off, c->su,
                   c->gcf_off, c->gcf, c->di, c->pi, c->bneed, c->bi, c->bfl, c->beta.P, c->beta.Pref, c->beta.D,
                   c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
                   c->beta.Vstate, c->beta.Vout, c->eterm};

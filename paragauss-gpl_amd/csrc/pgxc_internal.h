@@ -62,17 +62,19 @@ struct UnitD {
   // when screening is off).  ditems / pitems: the unit's density items
   // (tile << 8 | pair of column chunks) and potential items (split << 20 |
   // row chunk << 10 | column chunk).  tl_off / tl: per 128-point tile of the unit, its
-  // ascending run list (tl_off offset to the unit's first tile, values index
-  // tl).  su_off / su: per point split of the unit, the union of its groups'
-  // runs; a chunk is CH = tile / 8 consecutive entries of a list.  gcf_off /
+  // ascending run list, the density's K (tl_off offset to the unit's first
+  // tile, values index tl); tf_off / tfl: the same for its functions (those
+  // of the shells not cut off on the tile), the density's output columns in
+  // chunks of tile functions.  su_off / su: per point split of the unit, the
+  // union of its groups' runs; a chunk is CH = tile / 8 consecutive entries.  gcf_off /
   // gcf: per split, [group][chunk] flags, whether the chunk can be nonzero on
   // the group.  sig8: per 32-point group (from the unit's first), nw8 words
   // of run flags, the runs not cut off on the group.  wbits: the same shape,
   // the runs the potential contraction reads on the group (W is written
   // there, 0 where cut off).
   const uint32_t *ditems, *pitems;
-  const int *tl_off, *su_off;
-  const uint16_t *tl, *su;
+  const int *tl_off, *su_off, *tf_off;
+  const uint16_t *tl, *su, *tfl;
   const long *gcf_off;
   const uint8_t *gcf;
   const uint32_t *sig8, *wbits;
@@ -175,6 +177,7 @@ constexpr int kGroup = 32;
 constexpr int kRun = 8;
 constexpr int kScreenTile = 128;       // point tile of the density contraction
 constexpr int kMaxRuns = 1024;         // runs of 8 functions per molecule (NP <= 8192)
+constexpr int kMaxShells = 8192;       // shells per molecule
 constexpr int kMaxSplitGroups = 2048;  // point groups per potential split
 constexpr double kCutoff = 50.0 * (1.0 + 1.0e-6);  // margin: a superset in every precision
 struct ScreenShell {
@@ -183,7 +186,8 @@ struct ScreenShell {
 };
 void launch_screen(hipStream_t s, long ntiles_pts, const double *gx, const double *gy,
                    const double *gz, int n_atoms, const AtomD *atoms, int n_shells,
-                   const ScreenShell *shells, int n8, int *nb, uint32_t *sig8);
+                   const ScreenShell *shells, int n8, int *nb, uint32_t *sig8,
+                   uint32_t *smark);
 
 // P = C diag(occ) C^T into an NP x NP padded array
 void launch_form_density(hipStream_t s, const double *C, const double *occ, int n,

@@ -340,6 +340,7 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(2)
   static_assert(sizeof(double) * WGN * 4 * BM <= sizeof(__bf16) * 2 * XStage<BM, BN2>::n, "alias");
   double(&red)[WGN][4][BM] = *reinterpret_cast<double(*)[WGN][4][BM]>(lds);
   __shared__ uint16_t kl[kMaxRuns];
+  __shared__ uint16_t cl[BN2];
   __shared__ long slot;
   static_assert(kXBK == 4 * kRun, "four runs per K step");
   for (long L = walk_next(S, -1, &slot); L < S.total;) {
@@ -351,10 +352,11 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(2)
     const int tile = (int)(e >> 8), ch = 2 * (int)(e & 255);
     const long m0 = (long)tile * BM, NP = U.NP;
     const int off = U.tl_off[tile], c8 = U.tl_off[tile + 1] - off;
-    __syncthreads();  // the previous item's readers of kl are done
+    const int foff = U.tf_off[tile] + ch * BN;
+    const int ncol = min(BN2, U.tf_off[tile + 1] - foff), nslots = ncol > BN ? 2 : 1;
+    __syncthreads();  // the previous item's readers of kl and cl are done
+    for (int i = threadIdx.x; i < ncol; i += NT) cl[i] = gld(U.tfl + foff + i);
     lds_list(U.tl + off, c8, kl);
-    const uint16_t *crun = kl + ch * U.CH;
-    const int ncr = min(2 * U.CH, c8 - ch * U.CH), nslots = ncr > U.CH ? 2 : 1;
     // A(p, k) = Phi[f(k)][m0 + p] (FP32 values, split while staged),
     // B(k, n) = D^T[col(n)][f(k)]
     const __bf16 *dh = (const __bf16 *)U.D;
@@ -362,17 +364,14 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(2)
     PTOperand<BM, NT> opa;
     PSOperand<BN2, NT> opb;
     opa.init((const float *)U.phi, U.Mc, [&](int m) -> long { return m0 + m; });
-    opb.init({dh, dh + np2}, [&](int n) -> long {
-      const long c = run_fn(crun, ncr, n);
-      return c < 0 ? -1 : c * NP;
-    });
+    opb.init({dh, dh + np2}, [&](int n) -> long { return n < ncol ? (long)cl[n] * NP : -1; });
     auto kofs = [&](int kt, int c) -> long { return run_fn(kl, c8, kt * kXBK + 8 * c); };
     f4 acc[BM / 2 / 16][BN2 / WGN / 16];
     gemm_tile_bf16x3<BM, BN2, NT, WGN>(opa, opb, (c8 + 3) / 4, kofs, lds, acc);
     const int ahead = walk_ahead(S);
     constexpr int IB = BN2 / WGN / 16 > 3 ? 1 : 2;  // row blocks per round: within 256 registers
     density_epilogue<float, f4, BM, BN2, GRAD, IB, NT, WGN>(acc, (const float *)U.phi, U.Mc,
-                                                           U.NP, m0, crun, ncr, ch, U.part, red,
+                                                           U.NP, m0, cl, ncol, ch, U.part, red,
                                                            nslots);
     item_log(S, it, t0);
     L = walk_take(S, L, ahead, &slot);

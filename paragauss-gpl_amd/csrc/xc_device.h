@@ -214,7 +214,7 @@ __device__ __forceinline__ double group_reduce(double (&v)[M], int lane, int &in
 // contributions rho += sum_n Phi(p,n) X(p,n), grad rho += 2 sum_n dPhi(p,n)
 // X(p,n), summed over the chunk's columns in a fixed order (j, then the
 // 16-lane group reduction) and written to part[ch][c][p].  Column n of the
-// chunk is function run_fn(crun, ncr, n) (absent columns add nothing).  With
+// chunk is function cfn[n] for n < ncol (absent columns add nothing).  With
 // the FP32 result placement a lane holds four consecutive points of one
 // column, read as one 16-byte load per component; the FP64 placement reads
 // point by point.  A lane's columns are looked up once; the loads of IB row
@@ -228,7 +228,7 @@ template <typename T, typename ACC, int BM, int BN, bool GRAD, int IB = 1, int N
           int WGN = 2>
 __device__ __forceinline__ void density_epilogue(
     const ACC (&acc)[BM / (NT / 64 / WGN) / 16][BN / WGN / 16], const T *__restrict__ phi,
-    long Mc, int NP, long m0, const uint16_t *crun, int ncr, int ch, double *__restrict__ part,
+    long Mc, int NP, long m0, const uint16_t *cfn, int ncol, int ch, double *__restrict__ part,
     double (&red)[WGN][4][BM], int nslots = 1) {
   constexpr int WGM = NT / 64 / WGN, WM = BM / WGM, WN = BN / WGN, TM = WM / 16, TN = WN / 16;
   constexpr int NC = GRAD ? 4 : 1, SLOTS = WGN / 2;
@@ -241,9 +241,9 @@ __device__ __forceinline__ void density_epilogue(
   bool live[TN];
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const long n = run_fn(crun, ncr, wn * WN + j * 16 + mf_col(lane));
-    live[j] = n >= 0;
-    col[j] = live[j] ? n * Mc : 0;
+    const int e = wn * WN + j * 16 + mf_col(lane);
+    live[j] = e < ncol;
+    col[j] = live[j] ? (long)cfn[min(e, ncol - 1)] * Mc : 0;
   }
 #pragma unroll
   for (int i0 = 0; i0 < TM; i0 += IB) {

@@ -62,16 +62,20 @@ constexpr int lds_elems() { return ST * R * (KC ? kc_row<T, BK>() : BK); }
 // Operand addressing (gathered rows and K runs): aaddr(i, kt, kk) is the
 // element offset in A of the vector that starts at tile row / column i and
 // local k index kk of K step kt (vectors run along the contiguous dimension),
-// or -1 for a vector of zeros; baddr likewise for B.
+// or -1 for a vector of zeros; baddr likewise for B.  Each operand is kept
+// in LDS in its own layout ([row][k] padded when k-contiguous, else
+// [k][row]); with both k-contiguous the MFMAs take E k per 16-byte read,
+// otherwise one element per lane and k.
 template <typename T, int BM, int BN, int BK, int ST, bool AKC, bool BKC, class AAddr,
           class BAddr>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__restrict__ B,
                                           int ksteps, AAddr aaddr, BAddr baddr, T *As, T *Bs,
                                           typename MF<T>::acc (&acc)[BM / 32][BN / 32]) {
-  static_assert(AKC == BKC, "operands share one LDS layout");
-  constexpr bool KC = AKC;
+  constexpr bool KC = AKC && BKC;
   constexpr int E = kc_elems<T>(), BKP = kc_row<T, BK>();
   static_assert(!KC || BK % (4 * E) == 0, "k chunk");
+  // elements of one LDS stage of each operand
+  constexpr int SA = AKC ? BM * BKP : BK * BM, SB = BKC ? BN * BKP : BK * BN;
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int MAXV = 16 / (int)sizeof(T);
   constexpr int VA = pick_vec(BM * BK, AKC ? BK : BM, MAXV);
@@ -140,43 +144,33 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
 #pragma unroll
     for (int e = 0; e < LB; ++e)
       if (!((okb >> e) & 1u)) rb[e] = typename VB_t::type{};
-    if constexpr (KC) {
-      T *as = As + buf * BM * BKP, *bs = Bs + buf * BN * BKP;
-#pragma unroll
-      for (int e = 0; e < LA; ++e) {
-        const int idx = tid + kThreads * e;
-        const int mm = idx / (BK / VA), kv = idx % (BK / VA);
-        *(typename VA_t::type *)(as + mm * BKP + kv * VA) = ra[e];
-      }
-#pragma unroll
-      for (int e = 0; e < LB; ++e) {
-        const int idx = tid + kThreads * e;
-        const int nn = idx / (BK / VB), kv = idx % (BK / VB);
-        *(typename VB_t::type *)(bs + nn * BKP + kv * VB) = rb[e];
-      }
-      return;
-    }
-    T *as = As + buf * BK * BM, *bs = Bs + buf * BK * BN;
+    T *as = As + buf * SA, *bs = Bs + buf * SB;
 #pragma unroll
     for (int e = 0; e < LA; ++e) {
       const int idx = tid + kThreads * e;
+      if constexpr (AKC) {  // the vector as loaded, row-major [row][k]
+        const int mm = idx / (BK / VA), kv = idx % (BK / VA);
+        *(typename VA_t::type *)(as + mm * BKP + kv * VA) = ra[e];
+      } else {
 #pragma unroll
-      for (int t = 0; t < VA; ++t) {
-        int mm, kk;
-        if (AKC) { mm = idx / (BK / VA); kk = (idx % (BK / VA)) * VA + t; }
-        else { mm = (idx % (BM / VA)) * VA + t; kk = idx / (BM / VA); }
-        as[kk * BM + mm] = VA_t::get(ra[e], t);
+        for (int t = 0; t < VA; ++t) {
+          const int mm = (idx % (BM / VA)) * VA + t, kk = idx / (BM / VA);
+          as[kk * BM + mm] = VA_t::get(ra[e], t);
+        }
       }
     }
 #pragma unroll
     for (int e = 0; e < LB; ++e) {
       const int idx = tid + kThreads * e;
+      if constexpr (BKC) {
+        const int nn = idx / (BK / VB), kv = idx % (BK / VB);
+        *(typename VB_t::type *)(bs + nn * BKP + kv * VB) = rb[e];
+      } else {
 #pragma unroll
-      for (int t = 0; t < VB; ++t) {
-        int nn, kk;
-        if (BKC) { nn = idx / (BK / VB); kk = (idx % (BK / VB)) * VB + t; }
-        else { nn = (idx % (BN / VB)) * VB + t; kk = idx / (BN / VB); }
-        bs[kk * BN + nn] = VB_t::get(rb[e], t);
+        for (int t = 0; t < VB; ++t) {
+          const int nn = (idx % (BN / VB)) * VB + t, kk = idx / (BN / VB);
+          bs[kk * BN + nn] = VB_t::get(rb[e], t);
+        }
       }
     }
   };
@@ -203,15 +197,21 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
       }
       return;
     }
-    const T *as = As + buf * BK * BM, *bs = Bs + buf * BK * BN;
+    const T *as = As + buf * SA, *bs = Bs + buf * SB;
 #pragma unroll
     for (int k4 = 0; k4 < BK / 4; ++k4) {
       const int kr = k4 * 4 + (lane >> 4);
       T a[TM], b[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) a[i] = as[kr * BM + wm * WM + i * 16 + (lane & 15)];
+      for (int i = 0; i < TM; ++i) {
+        const int r = wm * WM + i * 16 + (lane & 15);
+        a[i] = AKC ? as[r * BKP + kr] : as[kr * BM + r];
+      }
 #pragma unroll
-      for (int j = 0; j < TN; ++j) b[j] = bs[kr * BN + wn * WN + j * 16 + (lane & 15)];
+      for (int j = 0; j < TN; ++j) {
+        const int r = wn * WN + j * 16 + (lane & 15);
+        b[j] = BKC ? bs[r * BKP + kr] : bs[kr * BN + r];
+      }
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -639,10 +639,11 @@ template <typename T, int BN, int VAR, bool GRAD>
 __global__ __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void k_density(const StageList S) {
   constexpr int BM = kDBM, BK = Var<VAR>::BK, ST = Var<VAR>::ST, TM = BM / 32, TN = BN / 32;
-  __shared__ T As[ST * BK * BM];
-  __shared__ T Bs[ST * BK * BN];
+  __shared__ T As[lds_elems<T, BM, BK, ST, false>()];
+  __shared__ __attribute__((aligned(16))) T Bs[lds_elems<T, BN, BK, ST, true>()];
   __shared__ double red[2][4][BM];
   __shared__ uint16_t kl[kMaxRuns];
+  __shared__ uint16_t cl[2 * BN];
   __shared__ long slot;
   for (long L = walk_next(S, -1, &slot); L < S.total;) {
     const uint64_t t0 = S.tlog ? wall_clock64() : 0;
@@ -653,30 +654,33 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void k_density(cons
     const int tile = (int)(e >> 8), pair = (int)(e & 255);
     const long m0 = (long)tile * BM, Mc = U.Mc, NP = U.NP;
     const int off = U.tl_off[tile], c8 = U.tl_off[tile + 1] - off;
-    __syncthreads();  // the previous item's readers of kl are done
+    const int foff = U.tf_off[tile] + 2 * pair * BN, nf = U.tf_off[tile + 1] - foff;
+    __syncthreads();  // the previous item's readers of kl and cl are done
+    for (int i = threadIdx.x; i < min(2 * BN, nf); i += kThreads) cl[i] = gld(U.tfl + foff + i);
     lds_list(U.tl + off, c8, kl);
     const int ksteps = (c8 * kRun + BK - 1) / BK;
     const T *phi = (const T *)U.phi, *D = (const T *)U.D;
     int ahead = 0;
     // the item's two column chunks in turn (the second may be absent)
-    for (int ch = 2 * pair; ch < 2 * pair + 2 && ch * U.CH < c8; ++ch) {
-      const uint16_t *crun = kl + ch * U.CH;
-      const int ncr = min(U.CH, c8 - ch * U.CH);
-      // A(m, k) = Phi[f(k)][m0 + m], B(k, n) = D[f(k)][col(n)]: K and the
-      // columns run over the tile's gathered runs
+    for (int h = 0; h < 2 && h * BN < nf; ++h) {
+      const uint16_t *cfn = cl + h * BN;
+      const int ncol = min(BN, nf - h * BN);
+      // A(m, k) = Phi[f(k)][m0 + m], B(k, n) = D[f(k)][col(n)] = D[col(n)][f(k)]
+      // (D symmetric, so B is k-contiguous along runs): K runs over the
+      // tile's gathered runs, the columns over its functions
       auto aaddr = [&](int m, int kt, int kk) -> long {
         const long f = run_fn(kl, c8, kt * BK + kk);
         return f < 0 ? -1 : f * Mc + m;
       };
       auto baddr = [&](int n, int kt, int kk) -> long {
-        const long f = run_fn(kl, c8, kt * BK + kk), c = run_fn(crun, ncr, n);
-        return (f < 0 || c < 0) ? -1 : f * NP + c;
+        const long f = run_fn(kl, c8, kt * BK + kk);
+        return (f < 0 || n >= ncol) ? -1 : (long)cfn[min(n, ncol - 1)] * NP + f;
       };
       typename MF<T>::acc acc[TM][TN];
-      gemm_tile<T, BM, BN, BK, ST, false, false>(phi + m0, D, ksteps, aaddr, baddr, As, Bs, acc);
-      if (ch == 2 * pair + 1 || (ch + 1) * U.CH >= c8) ahead = walk_ahead(S);  // the last chunk
-      density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, Mc, U.NP, m0, crun, ncr,
-                                                            ch, U.part, red);
+      gemm_tile<T, BM, BN, BK, ST, false, true>(phi + m0, D, ksteps, aaddr, baddr, As, Bs, acc);
+      if (h == 1 || (h + 1) * BN >= nf) ahead = walk_ahead(S);  // the last chunk
+      density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, Mc, U.NP, m0, cfn, ncol,
+                                                            2 * pair + h, U.part, red);
     }
     item_log(S, it, t0);
     L = walk_take(S, L, ahead, &slot);
@@ -702,7 +706,7 @@ __global__ __launch_bounds__(kThreads) void k_functional(const UnitD *__restrict
     double rho = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
     // partial sums of the density items of this point's tile, one per chunk
     const int tile = (int)(p / kScreenTile);
-    const int nch = (U.tl_off[tile + 1] - U.tl_off[tile] + U.CH - 1) / U.CH;
+    const int tf = U.CH * kRun, nch = (U.tf_off[tile + 1] - U.tf_off[tile] + tf - 1) / tf;
     for (int t = 0; t < nch; ++t) {
       rho += part[((long)t * 4 + 0) * Mc + p];
       if (GGA) {
@@ -942,7 +946,7 @@ __global__ __launch_bounds__(kThreads) void k_functional_spin(const UnitD *__res
     double *coef[2] = {U.coef, U.coef2};
     double rho[2], g[2][3];
     const int tile = (int)(p / kScreenTile);
-    const int nch = (U.tl_off[tile + 1] - U.tl_off[tile] + U.CH - 1) / U.CH;
+    const int tf = U.CH * kRun, nch = (U.tf_off[tile + 1] - U.tf_off[tile] + tf - 1) / tf;
     for (int sp = 0; sp < 2; ++sp) {
       double r = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
       for (int t = 0; t < nch; ++t) {
@@ -1059,7 +1063,7 @@ __global__ __launch_bounds__(kThreads) void k_energy(const UnitD *__restrict__ u
     const long blk = it - base[u];
     const long p = blk * kThreads + tid, Mc = U.Mc;
     const int tile = (int)(p / kScreenTile);
-    const int nch = (U.tl_off[tile + 1] - U.tl_off[tile] + U.CH - 1) / U.CH;
+    const int tf = U.CH * kRun, nch = (U.tf_off[tile + 1] - U.tf_off[tile] + tf - 1) / tf;
     double rho[2] = {0.0, 0.0}, g[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     for (int sp = 0; sp < (SPIN ? 2 : 1); ++sp) {
       const double *part = sp ? U.part2 : U.part;
@@ -1540,22 +1544,27 @@ PGXC_INST(float)
 // group to each atom (FP64, reduced over the group's 32 lanes), then per
 // shell and group alpha_min r^2_min < cut-off marks the shell's runs of 8
 // functions.  Outputs per group: the run flags (nw8 words, sig8); per tile:
-// n_b, the functions of the shells marked on any of its groups.
+// n_b, the functions of the shells marked on any of its groups, and those
+// shells (smark, one bit per shell).
 __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *__restrict__ gx,
                                                      const double *__restrict__ gy,
                                                      const double *__restrict__ gz, int n_atoms,
                                                      const AtomD *__restrict__ atoms, int n_shells,
                                                      const ScreenShell *__restrict__ shells,
-                                                     int n8, int *nb, uint32_t *sig8) {
+                                                     int n8, int *nb, uint32_t *sig8,
+                                                     uint32_t *smark) {
   constexpr int AC = 512;  // atoms per pass
   constexpr int W8 = kMaxRuns / 32;
+  constexpr int WS = kMaxShells / 32;
   __shared__ double r2g[4][AC];
   __shared__ unsigned bits[4][W8];
+  __shared__ unsigned sb[WS];
   __shared__ int nbs;
   const int tid = threadIdx.x;
-  const int nw8 = (n8 + 31) / 32;
+  const int nw8 = (n8 + 31) / 32, nws = (n_shells + 31) / 32;
   for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
     for (int i = tid; i < 4 * W8; i += kThreads) (&bits[0][0])[i] = 0u;
+    for (int i = tid; i < WS; i += kThreads) sb[i] = 0u;
     if (tid == 0) nbs = 0;
     const long p = t * kScreenTile + (tid & 127);
     const double x = gx[p], y = gy[p], z = gz[p];
@@ -1583,7 +1592,10 @@ __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *
             for (int b = b0; b <= b1; ++b) atomicOr(&bits[q][b >> 5], 1u << (b & 31));
           }
         }
-        if (any) atomicAdd(&nbs, S.nfunc);
+        if (any) {
+          atomicAdd(&nbs, S.nfunc);
+          atomicOr(&sb[si >> 5], 1u << (si & 31));
+        }
       }
     }
     __syncthreads();
@@ -1592,16 +1604,17 @@ __global__ __launch_bounds__(kThreads) void k_screen(long ntiles, const double *
       const int q = i / nw8, w = i % nw8;
       sig8[(t * 4 + q) * nw8 + w] = bits[q][w];
     }
+    for (int i = tid; i < nws; i += kThreads) smark[t * nws + i] = sb[i];
     __syncthreads();
   }
 }
 
 void launch_screen(hipStream_t s, long ntiles_pts, const double *gx, const double *gy,
                    const double *gz, int n_atoms, const AtomD *atoms, int n_shells,
-                   const ScreenShell *shells, int n8, int *nb, uint32_t *sig8) {
+                   const ScreenShell *shells, int n8, int *nb, uint32_t *sig8, uint32_t *smark) {
   if (ntiles_pts <= 0) return;
   hipLaunchKernelGGL(k_screen, dim3(persistent_grid(ntiles_pts, 8)), dim3(kThreads), 0, s,
-                     ntiles_pts, gx, gy, gz, n_atoms, atoms, n_shells, shells, n8, nb, sig8);
+                     ntiles_pts, gx, gy, gz, n_atoms, atoms, n_shells, shells, n8, nb, sig8, smark);
 }
 
 // ------------------------------------------------------------ small kernels
