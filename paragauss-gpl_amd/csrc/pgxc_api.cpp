@@ -284,7 +284,7 @@ long unit_items(const UnitD &U, ItemKind k, int tf) {
 // kdense: the dense K length.
 static void unit_item_features(const pgxc_ctx *c, int step, ItemKind kind, int precision,
                                std::vector<double> &feat, std::vector<double> &flops) {
-  const int tf = c->tf, CH = c->CH;
+  const int tf = c->tf;
   const int var = precision == PGXC_PREC_FP64 ? c->var[0] : precision == PGXC_PREC_FP32 ? c->var[1] : 0;
   auto push = [&](int kd, double kact, double kdense, bool active, double fl) {
     const double f[PGXC_NFEATURES] = {(double)kd,
@@ -315,12 +315,12 @@ static void unit_item_features(const pgxc_ctx *c, int step, ItemKind kind, int p
       const uint32_t e = c->h_pi[(size_t)i];
       const long S = (long)step * c->nsplit + (e >> 20);
       const int ci = (int)((e >> 10) & 1023), cj = (int)(e & 1023);
-      const int nu = c->h_su_off[(size_t)S + 1] - c->h_su_off[(size_t)S], nch = (nu + CH - 1) / CH;
+      const int nu = c->h_su_off[(size_t)S + 1] - c->h_su_off[(size_t)S], nch = (nu + tf - 1) / tf;
       const uint8_t *fl = c->h_gcf.data() + c->h_gcf_off[(size_t)S];
       long cnt = 0;
       for (long g = 0; g < G; ++g) cnt += (fl[g * nch + ci] & fl[g * nch + cj]) != 0;
-      const double rows = (double)kRun * std::min(CH, nu - ci * CH);
-      const double cols = (double)kRun * std::min(CH, nu - cj * CH);
+      const double rows = (double)std::min(tf, nu - ci * tf);
+      const double cols = (double)std::min(tf, nu - cj * tf);
       push(1, (double)cnt * kGroup, (double)spts, cnt > 0, 2.0 * rows * cols * cnt * kGroup);
     }
   }
@@ -1073,7 +1073,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   c->CH = c->tf / kRun;
   c->screen = on;
   c->nw8 = (c->n8 + 31) / 32;
-  const int n8 = c->n8, nw8 = c->nw8, CH = c->CH;
+  const int n8 = c->n8, nw8 = c->nw8;
   const long G = c->Mc / c->nsplit / kGroup;
   if (n8 > kMaxRuns) return fail(c, PGXC_ERR_UNSUPPORTED, "more than 8192 basis functions (padded)");
   if (G > kMaxSplitGroups) return fail(c, PGXC_ERR_UNSUPPORTED, "point split above 65536 points");
@@ -1155,36 +1155,51 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
         c->h_tl.push_back((uint16_t)r);
     c->h_tl_off.push_back((int)c->h_tl.size());
   }
-  // per split: union of its groups' runs, [group][chunk] flags
+  // per split: the union of its tiles' functions (the potential's rows and
+  // columns, in chunks of tf functions), and [group][chunk] flags: whether a
+  // run holding a function of the chunk is live on the group
   const long NS = (long)c->nchunks * c->nsplit;
   std::vector<uint32_t> wneed((size_t)ngr * nw8, 0u);
   c->h_su_off.assign(1, 0);
   c->h_su.clear();
   c->h_gcf_off.clear();
   c->h_gcf.clear();
-  for (long S = 0; S < NS; ++S) {
-    const long g0 = S * G;
-    const size_t u0 = c->h_su.size();
-    for (int r = 0; r < n8; ++r) {
-      uint32_t any = 0;
-      for (long g = g0; g < g0 + G && !any; ++g) any = bit(g, r);
-      if (any) c->h_su.push_back((uint16_t)r);
-    }
-    c->h_su_off.push_back((int)c->h_su.size());
-    const int nu = (int)(c->h_su.size() - u0), nch = (nu + CH - 1) / CH;
-    c->h_gcf_off.push_back((long)c->h_gcf.size());
-    for (long g = g0; g < g0 + G; ++g)
+  {
+    const int tf = c->tf;
+    std::vector<uint32_t> fm((size_t)(c->NP + 31) / 32);
+    std::vector<int> cr;  // the distinct runs of one chunk
+    for (long S = 0; S < NS; ++S) {
+      const long g0 = S * G;
+      std::fill(fm.begin(), fm.end(), 0u);
+      for (long t = g0 / 4; t < (g0 + G) / 4; ++t)
+        for (int i = c->h_tf_off[(size_t)t]; i < c->h_tf_off[(size_t)t + 1]; ++i) {
+          const int f = c->h_tfl[(size_t)i];
+          fm[(size_t)f / 32] |= 1u << (f % 32);
+        }
+      const size_t u0 = c->h_su.size();
+      for (int f = 0; f < c->NP; ++f)
+        if ((fm[(size_t)f / 32] >> (f % 32)) & 1u) c->h_su.push_back((uint16_t)f);
+      c->h_su_off.push_back((int)c->h_su.size());
+      const int nu = (int)(c->h_su.size() - u0), nch = (nu + tf - 1) / tf;
+      c->h_gcf_off.push_back((long)c->h_gcf.size());
+      const size_t gf0 = c->h_gcf.size();
+      c->h_gcf.resize(gf0 + (size_t)G * nch, 0);
       for (int ch = 0; ch < nch; ++ch) {
-        uint8_t f = 0;
-        for (int e = ch * CH; e < std::min(nu, (ch + 1) * CH) && !f; ++e) f = (uint8_t)bit(g, c->h_su[u0 + e]);
-        c->h_gcf.push_back(f);
-        // the potential contraction reads W on this group for every run of a flagged chunk
-        if (f)
-          for (int e = ch * CH; e < std::min(nu, (ch + 1) * CH); ++e) {
-            const int r = c->h_su[u0 + e];
-            wneed[(size_t)g * nw8 + r / 32] |= 1u << (r % 32);
-          }
+        cr.clear();
+        for (int e = ch * tf; e < std::min(nu, (ch + 1) * tf); ++e) {
+          const int r = c->h_su[u0 + e] / kRun;
+          if (cr.empty() || cr.back() != r) cr.push_back(r);
+        }
+        for (long g = g0; g < g0 + G; ++g) {
+          uint8_t f = 0;
+          for (size_t i = 0; i < cr.size() && !f; ++i) f = (uint8_t)bit(g, cr[i]);
+          c->h_gcf[gf0 + (size_t)(g - g0) * nch + ch] = f;
+          // the potential contraction reads W on this group for every function of a flagged chunk
+          if (f)
+            for (int r : cr) wneed[(size_t)g * nw8 + r / 32] |= 1u << (r % 32);
+        }
       }
+    }
   }
   if ((rc = upload(c, &c->wbits, wneed))) return rc;
   // Blocks of the basis values some stage reads, per 32-point group: values
@@ -1287,7 +1302,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
     c->h_di_off.push_back((long)c->h_di.size());
     for (int sl = 0; sl < c->nsplit; ++sl) {
       const long S = (long)k * c->nsplit + sl;
-      const int nu = c->h_su_off[(size_t)S + 1] - c->h_su_off[(size_t)S], nch = (nu + CH - 1) / CH;
+      const int nu = c->h_su_off[(size_t)S + 1] - c->h_su_off[(size_t)S], nch = (nu + c->tf - 1) / c->tf;
       for (int ci = 0; ci < nch; ++ci)
         for (int cj = 0; cj < nch; ++cj)
           c->h_pi.push_back((uint32_t)sl << 20 | (uint32_t)ci << 10 | (uint32_t)cj);

@@ -58,20 +58,22 @@ struct UnitD {
   void *W2;         // and the beta spin's W, written by the same W pass
   double *eterm;    // [Mc/256][5] block sums of the energy terms and the charge
   double *Hpart;    // [nsplit][NP][NP] potential partials of the molecule
-  // Work of the contractions, over gathered runs of 8 functions (all runs
-  // when screening is off).  ditems / pitems: the unit's density items
-  // (tile << 8 | pair of column chunks) and potential items (split << 20 |
-  // row chunk << 10 | column chunk).  tl_off / tl: per 128-point tile of the unit, its
-  // ascending run list, the density's K (tl_off offset to the unit's first
-  // tile, values index tl); tf_off / tfl: the same for its functions (those
-  // of the shells not cut off on the tile), the density's output columns in
-  // chunks of tile functions.  su_off / su: per point split of the unit, the
-  // union of its groups' runs; a chunk is CH = tile / 8 consecutive entries.  gcf_off /
-  // gcf: per split, [group][chunk] flags, whether the chunk can be nonzero on
-  // the group.  sig8: per 32-point group (from the unit's first), nw8 words
-  // of run flags, the runs not cut off on the group.  wbits: the same shape,
-  // the runs the potential contraction reads on the group (W is written
-  // there, 0 where cut off).
+  // Work of the contractions, over gathered runs of 8 functions and gathered
+  // functions (all of them when screening is off).  ditems / pitems: the
+  // unit's density items (tile << 8 | pair of column chunks) and potential
+  // items (split << 20 | row chunk << 10 | column chunk).  tl_off / tl: per
+  // 128-point tile of the unit, its ascending run list, the density's K
+  // (tl_off offset to the unit's first tile, values index tl); tf_off / tfl:
+  // the same for its functions (those of the shells not cut off on the
+  // tile), the density's output columns in chunks of tile functions.
+  // su_off / su: per point split of the unit, the union of its tiles'
+  // functions, the potential's rows and columns in chunks of tile functions.
+  // gcf_off / gcf: per split, [group][chunk] flags, whether the chunk can be
+  // nonzero on the group (a run holding one of its functions is live there).
+  // sig8: per 32-point group (from the unit's first), nw8 words of run
+  // flags, the runs not cut off on the group.  wbits: the same shape, the
+  // runs the potential contraction reads on the group (W is written there,
+  // 0 where cut off).
   const uint32_t *ditems, *pitems;
   const int *tl_off, *su_off, *tf_off;
   const uint16_t *tl, *su, *tfl;

@@ -401,7 +401,7 @@ void launch_density_bf16x3(hipStream_t s, bool grad, int tf, const StageList &L)
 template <int BT>
 __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S) {
   __shared__ __attribute__((aligned(16))) __bf16 lds[2 * XStage<BT, BT>::n];
-  __shared__ uint16_t ul[kMaxRuns];
+  __shared__ uint16_t ul[2 * BT];  // the functions of the row chunk, then of the column chunk
   __shared__ uint16_t gl[kMaxSplitGroups];
   __shared__ int wtot[4];
   __shared__ long slot;
@@ -414,28 +414,24 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
     const uint32_t e = U.pitems[it - S.base[u]];
     const int sl = (int)(e >> 20), ci = (int)((e >> 10) & 1023), cj = (int)(e & 1023);
     const long Mc = U.Mc;
-    const int so = U.su_off[sl], nu = U.su_off[sl + 1] - so, CH = U.CH;
-    const int nch = (nu + CH - 1) / CH, G = (int)(Mc / U.nsplit) / kGroup;
+    const int so = U.su_off[sl], nu = U.su_off[sl + 1] - so;
+    const int nch = (nu + BT - 1) / BT, G = (int)(Mc / U.nsplit) / kGroup;
     const long kb = (long)sl * G * kGroup;
+    const int nri = min(BT, nu - ci * BT), nrj = min(BT, nu - cj * BT);
     __syncthreads();  // the previous item's readers of ul / gl are done
-    lds_list(U.su + so, nu, ul);
+    for (int i = threadIdx.x; i < 2 * BT; i += kThreads)
+      if (i < BT ? i < nri : i - BT < nrj) ul[i] = gld(U.su + so + (i < BT ? ci * BT + i : cj * BT + i - BT));
+    __syncthreads();
     const int ng = potential_groups(U.gcf + U.gcf_off[sl], G, nch, ci, cj, gl, wtot);
-    const uint16_t *ri = ul + ci * CH, *rj = ul + cj * CH;
-    const int nri = min(CH, nu - ci * CH), nrj = min(CH, nu - cj * CH);
+    const uint16_t *ri = ul, *rj = ul + BT;
     const __bf16 *wh = (const __bf16 *)U.W;
     const long pn = (long)U.NP * Mc;
     // A(m, k) = W[f_i(m)][kb + p(k)] (stored split), B(k, n) =
     // Phi[f_j(n)][kb + p(k)] (FP32 values, split while staged)
     PSOperand<BT> opa;
     PFOperand<BT> opb;
-    opa.init({wh, wh + pn}, [&](int m) -> long {
-      const long f = run_fn(ri, nri, m);
-      return f < 0 ? -1 : f * Mc;
-    });
-    opb.init((const float *)U.phi, [&](int n) -> long {
-      const long f = run_fn(rj, nrj, n);
-      return f < 0 ? -1 : f * Mc;
-    });
+    opa.init({wh, wh + pn}, [&](int m) -> long { return m < nri ? (long)ri[m] * Mc : -1; });
+    opb.init((const float *)U.phi, [&](int n) -> long { return n < nrj ? (long)rj[n] * Mc : -1; });
     auto kofs = [&](int kt, int c) -> long { return kb + (long)gl[kt] * kGroup + 8 * c; };
     f4 acc[BT / 32][BT / 32];
     gemm_tile_bf16x3<BT, BT, kThreads, 2>(opa, opb, ng, kofs, lds, acc);

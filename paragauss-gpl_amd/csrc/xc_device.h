@@ -309,25 +309,28 @@ __device__ __forceinline__ void density_epilogue(
 }
 
 // Epilogue of H = W^T Phi for one gathered row chunk x column chunk: the
-// tile is added into the split's FP64 slot at the functions of its rows and
-// columns (slots are zeroed at the start of a build).
+// tile is added into the split's FP64 slot at the functions of its rows
+// (rfn[m], m < nr) and columns (cfn[n], n < nc) (slots are zeroed at the
+// start of a build).
 template <typename T, typename ACC, int BM, int BN>
 __device__ __forceinline__ void potential_epilogue(const ACC (&acc)[BM / 32][BN / 32], double *H,
-                                                   int NP, const uint16_t *rrun, int nrr,
-                                                   const uint16_t *crun, int ncr) {
+                                                   int NP, const uint16_t *rfn, int nr,
+                                                   const uint16_t *cfn, int nc) {
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const long col = run_fn(crun, ncr, wn * WN + j * 16 + mf_col(lane));
-    if (col < 0) continue;
+    const int n = wn * WN + j * 16 + mf_col(lane);
+    if (n >= nc) continue;
+    const long col = cfn[n];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long row = run_fn(rrun, nrr, wm * WM + i * 16 + MF<T>::row(lane, r));
-        if (row < 0) continue;
+        const int m = wm * WM + i * 16 + MF<T>::row(lane, r);
+        if (m >= nr) continue;
+        const long row = rfn[m];
         const long q = row * NP + col;
         gst(H + q, gld(H + q) + (double)acc[i][j][r]);
       }

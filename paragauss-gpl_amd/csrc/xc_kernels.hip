@@ -1218,7 +1218,7 @@ __global__ __launch_bounds__(kThreads) void k_potential(const StageList S) {
   constexpr int R = kGroup / BK;  // K steps per 32-point group
   __shared__ __attribute__((aligned(16))) T As[lds_elems<T, BT, BK, ST, true>()];
   __shared__ __attribute__((aligned(16))) T Bs[lds_elems<T, BT, BK, ST, true>()];
-  __shared__ uint16_t ul[kMaxRuns];
+  __shared__ uint16_t ul[2 * BT];  // the functions of the row chunk, then of the column chunk
   __shared__ uint16_t gl[kMaxSplitGroups];
   __shared__ int wtot[4];
   __shared__ long slot;
@@ -1230,24 +1230,24 @@ __global__ __launch_bounds__(kThreads) void k_potential(const StageList S) {
     const uint32_t e = U.pitems[it - S.base[u]];
     const int sl = (int)(e >> 20), ci = (int)((e >> 10) & 1023), cj = (int)(e & 1023);
     const long Mc = U.Mc;
-    const int so = U.su_off[sl], nu = U.su_off[sl + 1] - so, CH = U.CH;
-    const int nch = (nu + CH - 1) / CH, G = (int)(Mc / U.nsplit) / kGroup;
+    const int so = U.su_off[sl], nu = U.su_off[sl + 1] - so;
+    const int nch = (nu + BT - 1) / BT, G = (int)(Mc / U.nsplit) / kGroup;
     const long kb = (long)sl * G * kGroup;  // first point of the split
+    const int nri = min(BT, nu - ci * BT), nrj = min(BT, nu - cj * BT);
     __syncthreads();  // the previous item's readers of ul / gl are done
-    lds_list(U.su + so, nu, ul);
+    for (int i = threadIdx.x; i < 2 * BT; i += kThreads)
+      if (i < BT ? i < nri : i - BT < nrj) ul[i] = gld(U.su + so + (i < BT ? ci * BT + i : cj * BT + i - BT));
+    __syncthreads();
     const int ng = potential_groups(U.gcf + U.gcf_off[sl], G, nch, ci, cj, gl, wtot);
-    const uint16_t *ri = ul + ci * CH, *rj = ul + cj * CH;
-    const int nri = min(CH, nu - ci * CH), nrj = min(CH, nu - cj * CH);
+    const uint16_t *ri = ul, *rj = ul + BT;
     // A(m, k) = W[f_i(m)][kb + p(k)], B(k, n) = Phi[f_j(n)][kb + p(k)]: rows from
     // the split's runs, K over the groups where both chunks can be nonzero
     auto kpt = [&](int kt, int kk) -> long { return (long)gl[kt / R] * kGroup + (kt % R) * BK + kk; };
     auto aaddr = [&](int m, int kt, int kk) -> long {
-      const long f = run_fn(ri, nri, m);
-      return f < 0 ? -1 : f * Mc + kb + kpt(kt, kk);
+      return m < nri ? (long)ri[min(m, nri - 1)] * Mc + kb + kpt(kt, kk) : -1;
     };
     auto baddr = [&](int n, int kt, int kk) -> long {
-      const long f = run_fn(rj, nrj, n);
-      return f < 0 ? -1 : f * Mc + kb + kpt(kt, kk);
+      return n < nrj ? (long)rj[min(n, nrj - 1)] * Mc + kb + kpt(kt, kk) : -1;
     };
     const T *W = (const T *)U.W, *phi = (const T *)U.phi;
     typename MF<T>::acc acc[TT][TT];
