@@ -142,6 +142,8 @@ struct pgxc_ctx {
   std::vector<uint32_t> h_di, h_pi;
   int *tl_off = nullptr, *su_off = nullptr, *tf_off = nullptr;
   uint16_t *tl = nullptr, *su = nullptr, *tfl = nullptr;
+  uint32_t *tfb = nullptr;  // per tile, its functions as a bitmap of nwf words
+  int nwf = 0;
   long *gcf_off = nullptr;
   uint8_t *gcf = nullptr;
   uint32_t *di = nullptr, *pi = nullptr;
@@ -476,6 +478,8 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.tl = c->tl;
       U.tf_off = c->tf_off + p0 / kScreenTile;
       U.tfl = c->tfl;
+      U.tfb = c->tfb + (p0 / kScreenTile) * c->nwf;
+      U.nwf = c->nwf;
       U.su_off = c->su_off + (long)k * c->nsplit;
       U.su = c->su;
       U.gcf_off = c->gcf_off + (long)k * c->nsplit;
@@ -1130,12 +1134,14 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   }
   // per 128-point tile: the functions of its marked shells, ascending (the
   // density contraction's output columns)
+  // (and as a bitmap of nwf words per tile)
+  c->nwf = (c->NP + 31) / 32;
+  std::vector<uint32_t> tfb((size_t)ntp * c->nwf, 0u);
   {
     c->h_tf_off.assign(1, 0);
     c->h_tfl.clear();
-    std::vector<uint32_t> fm((size_t)(c->NP + 31) / 32);
     for (long t = 0; t < ntp; ++t) {
-      std::fill(fm.begin(), fm.end(), 0u);
+      uint32_t *fm = tfb.data() + (size_t)t * c->nwf;
       for (int si = 0; si < c->n_shells; ++si)
         if ((smark[(size_t)t * nws + si / 32] >> (si % 32)) & 1u)
           for (int f = ss[(size_t)si].func_off; f < ss[(size_t)si].func_off + ss[(size_t)si].nfunc; ++f)
@@ -1145,6 +1151,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
       c->h_tf_off.push_back((int)c->h_tfl.size());
     }
   }
+  if ((rc = upload(c, &c->tfb, tfb))) return rc;
   auto bit = [&](long g, int r) { return (sig[(size_t)g * nw8 + r / 32] >> (r % 32)) & 1u; };
   // per 128-point tile: its runs (union over its four groups)
   c->h_tl_off.assign(1, 0);
@@ -1254,11 +1261,14 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
             const int nf = (2 * d.l + 1) * (d.n_unc + d.n_contr);
             for (long gg = 0; gg < gpb; ++gg) {
               const uint32_t *nd = need.data() + (size_t)(g0 + b * gpb + gg) * 2 * nw8;
-              uint32_t v = 0, gr = 0;
-              for (int r = d.func_off / kRun; r <= (d.func_off + nf - 1) / kRun; ++r) {
+              uint32_t v = 0;
+              for (int r = d.func_off / kRun; r <= (d.func_off + nf - 1) / kRun; ++r)
                 v |= nd[r >> 5] >> (r & 31);
-                gr |= nd[nw8 + (r >> 5)] >> (r & 31);
-              }
+              // gradients: read only for the tile's functions (the density
+              // epilogue's columns, and the W pass's live functions)
+              const long tg = (g0 + b * gpb + gg) / 4;
+              const int sg = ssh[(size_t)a].first + j;
+              const uint32_t gr = smark[(size_t)tg * nws + sg / 32] >> (sg % 32);
               const int bit = 8 * j + (int)gg;
               fw[bit >> 5] |= (v & 1u) << (bit & 31);
               fw[4 + (bit >> 5)] |= (gr & 1u) << (bit & 31);
@@ -2070,7 +2080,7 @@ int pgxc_destroy(pgxc_ctx *c) {
   void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
-                  c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->tf_off, c->tfl, c->su_off, c->su,
+                  c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->tf_off, c->tfl, c->tfb, c->su_off, c->su,
                   c->gcf_off, c->gcf, c->di, c->pi, c->bneed, c->bi, c->bfl, c->beta.P, c->beta.Pref, c->beta.D,
                   c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
                   c->beta.Vstate, c->beta.Vout, c->eterm};

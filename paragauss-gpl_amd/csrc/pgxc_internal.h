@@ -77,6 +77,8 @@ struct UnitD {
   const uint32_t *ditems, *pitems;
   const int *tl_off, *su_off, *tf_off;
   const uint16_t *tl, *su, *tfl;
+  const uint32_t *tfb;  // the tiles' functions as bitmaps, nwf words per tile
+  int nwf, pad_f;
   const long *gcf_off;
   const uint8_t *gcf;
   const uint32_t *sig8, *wbits;

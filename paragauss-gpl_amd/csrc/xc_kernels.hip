@@ -1118,7 +1118,9 @@ __global__ __launch_bounds__(kThreads) void k_point_f(long Mc, long npad, uint32
 // 32-point group per thread: W = a Phi + b . grad Phi.  SPLIT (split-BF16
 // builds, T = float): W stored as bf16 hi / lo planes.  A run of 8 functions
 // cut off on the group is 0 without reading Phi, and is not written at all
-// where the potential contraction never reads it.
+// where the potential contraction never reads it; inside a live run, a
+// function outside the tile's function list (its shell cut off on the whole
+// tile) is 0 without reading Phi either.
 template <typename T, bool GRAD, bool SPLIT, bool TWO>
 __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ units, int nu,
                                                    const long *__restrict__ base,
@@ -1159,6 +1161,7 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
       }
     };
     const uint32_t *sg = U.sig8 + (p / kGroup) * U.nw8, *wn = U.wbits + (p / kGroup) * U.nw8;
+    const uint32_t *fb = U.tfb + (p / kScreenTile) * U.nwf;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {  // the item's two runs of 8 functions
       const long r = f0 / kRun + h, fr = r * kRun;
@@ -1185,15 +1188,18 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
             b2[sp][j] = gld(cf[sp] + 3 * Mc + p + j);
           }
         }
+      const uint32_t fm = gld(fb + (fr >> 5)) >> (fr & 31);  // the run's functions in the tile's list
 #pragma unroll 4
       for (int k = 0; k < kRun; ++k) {
         const long q = (fr + k) * Mc + p;
-        const t4 v0 = gld((const t4 *)(phi + q));
-        t4 v1 = {}, v2 = {}, v3 = {};
-        if (GRAD) {
-          v1 = gld((const t4 *)(phi + cs + q));
-          v2 = gld((const t4 *)(phi + 2 * cs + q));
-          v3 = gld((const t4 *)(phi + 3 * cs + q));
+        t4 v0 = {}, v1 = {}, v2 = {}, v3 = {};
+        if ((fm >> k) & 1u) {
+          v0 = gld((const t4 *)(phi + q));
+          if (GRAD) {
+            v1 = gld((const t4 *)(phi + cs + q));
+            v2 = gld((const t4 *)(phi + 2 * cs + q));
+            v3 = gld((const t4 *)(phi + 3 * cs + q));
+          }
         }
 #pragma unroll
         for (int sp = 0; sp < NS; ++sp) {
