@@ -670,10 +670,12 @@ def main():
     # gradients of the live functions written once, and the grid read
     bytes_basis = 32.0 * npts + 16.0 * stats["bn"]
     ach_b = bytes_basis / (st["basis"] * 1e-3) / 1e12
-    roof_basis = {"bound": "hbm", "kernels": "k_basis<true,float>",
+    # the basis kernel is built for the largest angular momentum of the basis
+    kbasis = "k_basis<true,float,%d>" % max(1, max(sh.l for sh in b.shells))
+    roof_basis = {"bound": "hbm", "kernels": kbasis,
                   "stage_ms_one_lane": st["basis"], "bytes_per_build_algorithmic": bytes_basis,
                   "achieved": ach_b, "peak": 8.0, "unit": "TB/s", "frac": ach_b / 8.0,
-                  "traffic": profiled_traffic("k_basis<true,float>")}
+                  "traffic": profiled_traffic(kbasis)}
     roof["timed"] = "one-lane builds with the basis evaluated (%d launches)" % (nlaunch * layout["chunks"])
     roof["traffic_unit"] = ("bytes per dispatch (one grid chunk), rocprofv3 FETCH_SIZE + WRITE_SIZE "
                             "scaled by a copy kernel")
