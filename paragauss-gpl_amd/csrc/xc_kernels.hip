@@ -346,7 +346,13 @@ struct AtomLds {
   uint8_t fl[kAtomShells][kThreads / kGroup];
 };
 
-// ex, nm: the shell's primitives; ct: its n_exp x n_contr coefficients
+// ex, nm: the shell's primitives; ct: its n_exp x n_contr coefficients.
+// Primitives go in blocks of kPB: the block's exponents, norms and
+// coefficients are read from LDS together (one wait, not one per read), its
+// radial factors formed without a branch per primitive (a block whose
+// primitives are all cut off on the wave's points skips the exponentials),
+// then added into the contractions in primitive order.
+constexpr int kPB = 4;
 template <int L, bool GRAD, typename T>
 __device__ __forceinline__ void shell_emit(const ShellD &sh, const T *ex, const T *nm, const T *ct,
                                            T x, T y, T z, T r2, const T *C, const T *Cx,
@@ -356,21 +362,44 @@ __device__ __forceinline__ void shell_emit(const ShellD &sh, const T *ex, const 
   T rc0[kMaxContr], rc1[kMaxContr];
 #pragma unroll
   for (int k = 0; k < kMaxContr; ++k) rc0[k] = rc1[k] = T(0);
-  for (int e = 0; e < ne; ++e) {
-    const T alpha = ex[e];
-    const T ar2 = alpha * r2;
-    T r0 = T(0), r1 = T(0);
-    if (ar2 < T(50)) {  // primitive cut-off of radial1()
-      r0 = nm[e] * exp(-ar2);
-      r1 = (T(-2) * alpha) * r0;
+  for (int e0 = 0; e0 < ne; e0 += kPB) {
+    T al[kPB], ar2[kPB], nv[kPB], r0[kPB], r1[kPB];
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < kPB; ++i) {
+      const int e = min(e0 + i, ne - 1);  // past the shell: a repeat, never used
+      al[i] = ex[e];
+      nv[i] = nm[e];
+      ar2[i] = al[i] * r2;
+      any = any || (e0 + i < ne && ar2[i] < T(50));
     }
-    if (e < nu) put_fn<L, GRAD>(sh.func_off + e, nf, r0, r1, x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p, wg);
+    if (__ballot(any)) {
+#pragma unroll
+      for (int i = 0; i < kPB; ++i) {
+        const T v = nv[i] * exp(-ar2[i]);
+        r0[i] = (e0 + i < ne && ar2[i] < T(50)) ? v : T(0);  // primitive cut-off of radial1()
+        r1[i] = (T(-2) * al[i]) * r0[i];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < kPB; ++i) r0[i] = r1[i] = T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < kPB; ++i)
+      if (e0 + i < nu)
+        put_fn<L, GRAD>(sh.func_off + e0 + i, nf, r0[i], r1[i], x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p, wg);
 #pragma unroll
     for (int k = 0; k < kMaxContr; ++k)
       if (k < nc) {
-        const T c = ct[k * ne + e];
-        rc0[k] += r0 * c;
-        rc1[k] += r1 * c;
+        T c[kPB];
+#pragma unroll
+        for (int i = 0; i < kPB; ++i) c[i] = ct[k * ne + min(e0 + i, ne - 1)];
+#pragma unroll
+        for (int i = 0; i < kPB; ++i)
+          if (e0 + i < ne) {
+            rc0[k] += r0[i] * c[i];
+            rc1[k] += r1[i] * c[i];
+          }
       }
   }
 #pragma unroll
