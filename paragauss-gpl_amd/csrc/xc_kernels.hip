@@ -1247,8 +1247,10 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
 
 // --------------------------------------------------------------- potential
 // item = (point split, function tile pair) of H[split] (+)= W^T Phi
+// Variants 0 and 1 keep one LDS stage set under 64 KB: built for two waves
+// per SIMD, two workgroups share a CU and hide each other's waits.
 template <typename T, int BT, int VAR>
-__global__ __launch_bounds__(kThreads) void k_potential(const StageList S) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(VAR == 2 ? 1 : 2))) void k_potential(const StageList S) {
   constexpr int BK = Var<VAR>::BK, ST = Var<VAR>::ST, TT = BT / 32;
   constexpr int R = kGroup / BK;  // K steps per 32-point group
   __shared__ __attribute__((aligned(16))) T As[lds_elems<T, BT, BK, ST, true>()];
