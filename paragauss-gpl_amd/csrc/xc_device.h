@@ -225,7 +225,7 @@ __device__ __forceinline__ double group_reduce(double (&v)[M], int lane, int &in
 // make one partial slot: slot ch + s sums wave columns 2s and 2s + 1, and
 // the first nslots slots are written.
 template <typename T, typename ACC, int BM, int BN, bool GRAD, int IB = 1, int NT = kThreads,
-          int WGN = 2>
+          int WGN = 2, bool PW = false>
 __device__ __forceinline__ void density_epilogue(
     const ACC (&acc)[BM / (NT / 64 / WGN) / 16][BN / WGN / 16], const T *__restrict__ phi,
     long Mc, int NP, long m0, const uint16_t *cfn, int ncol, int ch, double *__restrict__ part,
@@ -245,6 +245,48 @@ __device__ __forceinline__ void density_epilogue(
     live[j] = e < ncol;
     col[j] = live[j] ? (long)cfn[min(e, ncol - 1)] * Mc : 0;
   }
+  if constexpr (PW) {
+    // one component plane at a time (a quarter of the registers of the loads;
+    // each sum is formed in the same order)
+    static_assert(!QUAD, "point-by-point placement");
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      double s[NC][4];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        T v[TN][4];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const long q0 = col[j] + m0 + wm * WM + i * 16;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) v[j][r] = gld(phi + c * cs + q0 + MF<T>::row(lane, r));
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s[c][r] = 0.0;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          if (!live[j]) continue;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double xv = (double)acc[i][j][r];
+            if (c == 0) s[0][r] += (double)v[j][r] * xv;
+            else s[c][r] += 2.0 * (double)v[j][r] * xv;
+          }
+        }
+      }
+      double w[NC * 4];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w[c * 4 + r] = s[c][r];
+      int idx;
+      const double tot = group_reduce<NC * 4>(w, lane, idx);
+      if (NC == 4 || (lane & 3) == 0) {
+        const int c = idx / 4, r = idx % 4;
+        red[wn][c][wm * WM + i * 16 + MF<T>::row(lane, r)] = tot;
+      }
+    }
+  } else {
 #pragma unroll
   for (int i0 = 0; i0 < TM; i0 += IB) {
     T v[IB][TN][NC][4];
@@ -298,6 +340,7 @@ __device__ __forceinline__ void density_epilogue(
         red[wn][c][wm * WM + i * 16 + MF<T>::row(lane, r)] = tot;
       }
     }
+  }
   }
   __syncthreads();
   for (int idx = tid; idx < SLOTS * NC * BM; idx += NT) {

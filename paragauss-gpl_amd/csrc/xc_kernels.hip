@@ -666,7 +666,7 @@ template <> struct Var<2> { static constexpr int BK = 32, ST = 2; };
 // function tile
 template <typename T, int BN, int VAR, bool GRAD>
 __global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void k_density(const StageList S) {
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 || VAR != 2 ? 2 : 1))) void k_density(const StageList S) {
   constexpr int BM = kDBM, BK = Var<VAR>::BK, ST = Var<VAR>::ST, TM = BM / 32, TN = BN / 32;
   __shared__ T As[lds_elems<T, BM, BK, ST, false>()];
   __shared__ __attribute__((aligned(16))) T Bs[lds_elems<T, BN, BK, ST, true>()];
@@ -708,8 +708,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void k_density(cons
       typename MF<T>::acc acc[TM][TN];
       gemm_tile<T, BM, BN, BK, ST, false, true>(phi + m0, D, ksteps, aaddr, baddr, As, Bs, acc);
       if (h == 1 || (h + 1) * BN >= nf) ahead = walk_ahead(S);  // the last chunk
-      density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD>(acc, phi, Mc, U.NP, m0, cfn, ncol,
-                                                            2 * pair + h, U.part, red);
+      // FP64 variants 0 and 1 run two waves per SIMD: their epilogue reads
+      // one component plane at a time to fit the registers
+      constexpr bool PW = sizeof(T) == 8 && VAR != 2;
+      density_epilogue<T, typename MF<T>::acc, BM, BN, GRAD, 1, kThreads, 2, PW>(
+          acc, phi, Mc, U.NP, m0, cfn, ncol, 2 * pair + h, U.part, red);
     }
     item_log(S, it, t0);
     L = walk_take(S, L, ahead, &slot);
