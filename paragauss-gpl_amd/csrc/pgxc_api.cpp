@@ -22,7 +22,6 @@
 #include <vector>
 
 #include "../../include/pgxc.h"
-#include <rocblas/rocblas.h>
 
 #include "pgxc_internal.h"
 
@@ -144,6 +143,10 @@ struct pgxc_ctx {
   uint16_t *tl = nullptr, *su = nullptr, *tfl = nullptr;
   uint32_t *tfb = nullptr;  // per tile, its functions as a bitmap of nwf words
   int nwf = 0;
+  // nuclear gradient work space (first gradient on): Hessian planes and
+  // products of one chunk, per-(tile, function) sums
+  double *ghess = nullptr, *gXY = nullptr, *gpart = nullptr;
+  long gpart_n = 0;
   long *gcf_off = nullptr;
   uint8_t *gcf = nullptr;
   uint32_t *di = nullptr, *pi = nullptr;
@@ -1792,28 +1795,30 @@ static int gradient_impl(pgxc_ctx *c, double *grad, double *pm) {
   // every block of the values: the products below read all functions on all points
   if ((rc = get_plan(mols, functional, PGXC_PREC_FP64, full, {}, -1, c->sched, gp, c, 0, 1)))
     return rc;
-  const long Mc = c->Mc, NP = c->NP, cs = NP * Mc;
-  const int nseg = (int)(Mc / kGradSeg);
-  double *hess = nullptr, *XY = nullptr, *part = nullptr;
-  rocblas_handle hb = nullptr;
-  std::vector<double> gfn(3 * (size_t)NP, 0.0), hp(3 * (size_t)NP * nseg);
+  const long Mc = c->Mc, NP = c->NP, cs = NP * Mc, tpc = Mc / kScreenTile;
   auto done = [&](int code) {
-    dfree(hess);
-    dfree(XY);
-    dfree(part);
-    if (hb) rocblas_destroy_handle(hb);
     free_plan(gp);
     return code;
   };
-  if (hipMalloc((void **)&hess, 6 * (size_t)cs * 8) != hipSuccess ||
-      hipMalloc((void **)&XY, 4 * (size_t)cs * 8) != hipSuccess ||
-      hipMalloc((void **)&part, hp.size() * 8) != hipSuccess)
+  // the chunk's Hessian planes, products and per-(tile, function) sums stay
+  // with the context for later gradients
+  long nlist = 1;  // the most list entries of one chunk
+  for (int k = 0; k < c->nchunks; ++k)
+    nlist = std::max<long>(nlist, c->h_tf_off[(size_t)(k + 1) * tpc] - c->h_tf_off[(size_t)k * tpc]);
+  if (!c->ghess && hipMalloc((void **)&c->ghess, 6 * (size_t)cs * 8) != hipSuccess) c->ghess = nullptr;
+  if (!c->gXY && hipMalloc((void **)&c->gXY, 4 * (size_t)cs * 8) != hipSuccess) c->gXY = nullptr;
+  if (c->gpart_n < 3 * nlist) {
+    dfree(c->gpart);
+    c->gpart_n = 0;
+    if (hipMalloc((void **)&c->gpart, 3 * (size_t)nlist * 8) == hipSuccess) c->gpart_n = 3 * nlist;
+    else c->gpart = nullptr;
+  }
+  if (!c->ghess || !c->gXY || !c->gpart) {
+    (void)hipGetLastError();
     return done(fail(c, PGXC_ERR_NOMEM, "pgxc_gradient: no device memory for the chunk's Hessian and products"));
-  if (hipMemsetAsync(hess, 0, 6 * (size_t)cs * 8, s) != hipSuccess)  // padding functions stay zero
-    return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: hipMemsetAsync"));
-  if (rocblas_create_handle(&hb) != rocblas_status_success || rocblas_set_stream(hb, s) != rocblas_status_success)
-    return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: rocBLAS handle"));
-  const double one = 1.0, zero = 0.0;
+  }
+  double *hess = c->ghess, *XY = c->gXY, *part = c->gpart;
+  std::vector<double> gfn(3 * (size_t)NP, 0.0), hp(3 * (size_t)nlist);
   for (int k = 0; k < c->nchunks; ++k) {
     const long p0 = (long)k * Mc;
     const int lane = k % c->nlanes;
@@ -1822,28 +1827,26 @@ static int gradient_impl(pgxc_ctx *c, double *grad, double *pm) {
     launch_basis<double>(s, true, gp->steps[k].basis);
     launch_basis_hess(s, c->grid + p0, c->grid + c->npad + p0, c->grid + 2 * c->npad + p0, c->atoms,
                       c->n_atoms, c->shells, c->exps, c->norms, c->contr, (int)NP, Mc, hess);
+    const long l0 = c->h_tf_off[(size_t)k * tpc], nl = c->h_tf_off[(size_t)(k + 1) * tpc] - l0;
     for (int sp = 0; sp < c->ispin; ++sp) {
-      // X = Phi P, Y_c = (d/dc Phi) P: Phi planes as Mc x NP column-major
-      // matrices; P of the last build (P_ref), symmetric
+      // X = Phi P, Y_c = (d/dc Phi) P with P of the last build (P_ref),
+      // over the density items of the chunk, for each tile's functions
       const double *P = sp ? c->beta.Pref : c->Pref;
-      for (int pl = 0; pl < (gga ? 4 : 1); ++pl)
-        if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, (rocblas_int)Mc,
-                          (rocblas_int)NP, (rocblas_int)NP, &one, phi + pl * cs, (rocblas_int)Mc, P,
-                          (rocblas_int)NP, &zero, XY + pl * cs, (rocblas_int)Mc) != rocblas_status_success)
-          return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: rocblas_dgemm"));
+      for (int li = 0; li < kLists; ++li)
+        launch_xy(s, kTiles[li / kNumVariants], li % kNumVariants, gp->steps[k].dens[li], P, XY,
+                  gga ? 4 : 1, c->ispin, sp);
       const double *st = (sp ? c->beta.state : c->state) + p0;
       const long np = c->npad;
-      launch_grad_fn(s, gga, (int)NP, Mc, XY, phi, hess, st + S_A * np, st + S_BX * np,
-                     st + S_BY * np, st + S_BZ * np, part);
+      launch_grad_fn(s, gga, tpc, c->tf_off + p0 / kScreenTile, c->tfl, (int)NP, Mc, XY, phi, hess,
+                     st + S_A * np, st + S_BX * np, st + S_BY * np, st + S_BZ * np, part);
       if (pm)
-        launch_grad_pt(s, gga, (int)NP, Mc, XY, phi, hess, st + S_A * np, st + S_BX * np,
-                       st + S_BY * np, st + S_BZ * np, pm + p0, np);
-      if (hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess)
+        launch_grad_pt(s, gga, c->tf_off + p0 / kScreenTile, c->tfl, (int)NP, Mc, XY, phi, hess,
+                       st + S_A * np, st + S_BX * np, st + S_BY * np, st + S_BZ * np, pm + p0, np);
+      if (nl > 0 && (hipMemcpyAsync(hp.data(), part, (size_t)nl * 3 * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                     hipStreamSynchronize(s) != hipSuccess))
         return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: kernel or copy failed"));
-      for (long i = 0; i < NP; ++i)
-        for (int sg = 0; sg < nseg; ++sg)
-          for (int d = 0; d < 3; ++d) gfn[3 * i + d] += hp[(i * nseg + sg) * 3 + d];
+      for (long e = 0; e < nl; ++e)  // tiles in order, each tile's functions in order
+        for (int d = 0; d < 3; ++d) gfn[3 * (size_t)c->h_tfl[(size_t)(l0 + e)] + d] += hp[(size_t)e * 3 + d];
     }
   }
   if (hipGetLastError() != hipSuccess) return done(fail(c, PGXC_ERR_HIP, "pgxc_gradient: launch failed"));
@@ -2081,7 +2084,7 @@ int pgxc_destroy(pgxc_ctx *c) {
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
                   c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->tf_off, c->tfl, c->tfb, c->su_off, c->su,
-                  c->gcf_off, c->gcf, c->di, c->pi, c->bneed, c->bi, c->bfl, c->beta.P, c->beta.Pref, c->beta.D,
+                  c->gcf_off, c->gcf, c->di, c->pi, c->bneed, c->bi, c->bfl, c->ghess, c->gXY, c->gpart, c->beta.P, c->beta.Pref, c->beta.D,
                   c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
                   c->beta.Vstate, c->beta.Vout, c->eterm};
   for (void *b : bufs) dfree(b);

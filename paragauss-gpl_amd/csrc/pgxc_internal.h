@@ -199,24 +199,29 @@ void launch_form_density(hipStream_t s, const double *C, const double *occ, int 
 
 
 // Nuclear gradient, basis-centre term: the basis Hessian planes [6][NP][Mc]
-// (xx, xy, xz, yy, yz, zz) of one chunk, and per function i and segment of
-// kGradSeg points the sums of grad phi_i (2 a X_i + b . Y_i) + X_i (b . grad)
-// grad phi_i into part[i][seg][3] (XY = [4][NP][Mc]: X = Phi P, Y_c)
-constexpr int kGradSeg = 2048;
+// (xx, xy, xz, yy, yz, zz) of one chunk, and per 128-point tile and function
+// of its list the sums of grad phi_i (2 a X_i + b . Y_i) + X_i (b . grad)
+// grad phi_i into part[list position][3] (XY = [4][NP][Mc]: X = Phi P, Y_c)
 void launch_basis_hess(hipStream_t s, const double *gx, const double *gy, const double *gz,
                        const AtomD *atoms, int n_atoms, const ShellD *shells, const double *exps,
                        const double *norms, const double *contr, int NP, long Mc, double *hess);
-void launch_grad_fn(hipStream_t s, bool gga, int NP, long Mc, const double *XY,
-                    const double *phi, const double *hess, const double *a, const double *bx,
-                    const double *by, const double *bz, double *part);
+// the gradient's products X = Phi P, Y_c = (d/dc Phi) P over a density list's
+// items (tile x function chunks), written for the tiles' functions only
+void launch_xy(hipStream_t s, int tf, int var, const StageList &L, const double *P, double *XY,
+               int nplanes, int nspin, int spin);
+void launch_grad_fn(hipStream_t s, bool gga, long ntiles, const int *tf_off, const uint16_t *tfl,
+                    int NP, long Mc, const double *XY, const double *phi, const double *hess,
+                    const double *a, const double *bx, const double *by, const double *bz,
+                    double *part);
 // Weight-derivative and point-motion terms (the grid moves with the atoms):
 // pm[d * pstride + p] += 2 sum_i t_d(i, p) (w grad f at each point); f per
 // point from the state (st2: the beta spin's, spin builds); the Becke weight
 // derivatives times w f into part[blk][atom][3] (q: [na][Mc], rm: [3][Mc]
 // scratch; owner -1 for padding points)
-void launch_grad_pt(hipStream_t s, bool gga, int NP, long Mc, const double *XY, const double *phi,
-                    const double *hess, const double *a, const double *bx, const double *by,
-                    const double *bz, double *pm, long pstride);
+void launch_grad_pt(hipStream_t s, bool gga, const int *tf_off, const uint16_t *tfl, int NP,
+                    long Mc, const double *XY, const double *phi, const double *hess,
+                    const double *a, const double *bx, const double *by, const double *bz,
+                    double *pm, long pstride);
 void launch_point_f(hipStream_t s, bool gga, bool spin, uint32_t mask, long Mc, long npad,
                     const double *st, const double *st2, double *fp);
 void launch_becke_grad(hipStream_t s, int na, const AtomD *atoms, const double *inv_dist, long Mc,

@@ -210,13 +210,19 @@ __device__ __forceinline__ void grad_bracket(const double *__restrict__ XY,
   }
 }
 
-// item = (function i, segment of kGradSeg points): the segment's sum of
+// item = one 128-point tile of the chunk: for each function i of its list
+// (tfl[tf_off[t] ..], the functions not cut off on the tile; the others add
+// exact zeros) the tile's sum of
 //   t_d = grad_d phi_i (2 a X_i + b . Y_i) + X_i sum_c b_c H_dc phi_i
-// into part[i][seg][3].  XY = [4][NP][Mc]: X, then Y_x, Y_y, Y_z; phi the
-// chunk's [4][NP][Mc] basis values; a, b the build's per-point coefficients.
+// into part[(list position - pbase) * 3 + d], over its points in a fixed
+// order.  Two functions at a time, one per half of the workgroup (a point
+// per thread).  XY = [4][NP][Mc]: X, then Y_x, Y_y, Y_z (written for the
+// listed functions); phi the chunk's [4][NP][Mc] basis values; a, b the
+// build's per-point coefficients.
 template <bool GGA>
-__global__ __launch_bounds__(kThreads) void k_grad_fn(int NP, long Mc, int nseg,
-                                                      const double *__restrict__ XY,
+__global__ __launch_bounds__(kThreads) void k_grad_fn(long ntiles, const int *__restrict__ tf_off,
+                                                      const uint16_t *__restrict__ tfl, int NP,
+                                                      long Mc, const double *__restrict__ XY,
                                                       const double *__restrict__ phi,
                                                       const double *__restrict__ hess,
                                                       const double *__restrict__ a,
@@ -224,25 +230,30 @@ __global__ __launch_bounds__(kThreads) void k_grad_fn(int NP, long Mc, int nseg,
                                                       const double *__restrict__ by,
                                                       const double *__restrict__ bz,
                                                       double *__restrict__ part) {
+  static_assert(kThreads == 2 * kScreenTile, "two halves of one tile's points");
   __shared__ double sh[3][kThreads];
-  const long cs = (long)NP * Mc, total = (long)NP * nseg;
-  const int tid = threadIdx.x;
-  for (long it = blockIdx.x; it < total; it += gridDim.x) {
-    const long i = it / nseg, seg = it % nseg;
-    double t[3] = {0.0, 0.0, 0.0};
-    for (long p = seg * kGradSeg + tid; p < (seg + 1) * kGradSeg; p += kThreads) {
-      const double b[3] = {GGA ? bx[p] : 0.0, GGA ? by[p] : 0.0, GGA ? bz[p] : 0.0};
-      grad_bracket<GGA>(XY, phi, hess, cs, i * Mc + p, 2.0 * a[p], b, t);
-    }
-    for (int d = 0; d < 3; ++d) sh[d][tid] = t[d];
-    __syncthreads();
-    for (int st = kThreads / 2; st > 0; st >>= 1) {
-      if (tid < st)
-        for (int d = 0; d < 3; ++d) sh[d][tid] += sh[d][tid + st];
+  const long cs = (long)NP * Mc, pbase = tf_off[0];
+  const int tid = threadIdx.x, half = tid / kScreenTile, pt = tid % kScreenTile;
+  for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long f0 = tf_off[t];
+    const int nf = tf_off[t + 1] - (int)f0;
+    const long p = t * kScreenTile + pt;
+    const double a2 = 2.0 * a[p];
+    const double b[3] = {GGA ? bx[p] : 0.0, GGA ? by[p] : 0.0, GGA ? bz[p] : 0.0};
+    for (int e0 = 0; e0 < nf; e0 += 2) {
+      const int e = e0 + half;
+      double tt[3] = {0.0, 0.0, 0.0};
+      if (e < nf) grad_bracket<GGA>(XY, phi, hess, cs, (long)tfl[f0 + e] * Mc + p, a2, b, tt);
+      for (int d = 0; d < 3; ++d) sh[d][tid] = tt[d];
+      __syncthreads();
+      for (int st = kScreenTile / 2; st > 0; st >>= 1) {
+        if (pt < st)
+          for (int d = 0; d < 3; ++d) sh[d][tid] += sh[d][tid + st];
+        __syncthreads();
+      }
+      if (pt < 3 && e < nf) part[(f0 + e - pbase) * 3 + pt] = sh[pt][half * kScreenTile];
       __syncthreads();
     }
-    if (tid < 3) part[(i * nseg + seg) * 3 + tid] = sh[tid][0];
-    __syncthreads();
   }
 }
 
@@ -250,7 +261,9 @@ __global__ __launch_bounds__(kThreads) void k_grad_fn(int NP, long Mc, int nseg,
 // as k_grad_fn summed over functions instead of points: w grad f at the point
 // (the motion of the point with its atom)
 template <bool GGA>
-__global__ __launch_bounds__(kThreads) void k_grad_pt(int NP, long Mc, const double *__restrict__ XY,
+__global__ __launch_bounds__(kThreads) void k_grad_pt(const int *__restrict__ tf_off,
+                                                      const uint16_t *__restrict__ tfl, int NP,
+                                                      long Mc, const double *__restrict__ XY,
                                                       const double *__restrict__ phi,
                                                       const double *__restrict__ hess,
                                                       const double *__restrict__ a,
@@ -268,7 +281,9 @@ __global__ __launch_bounds__(kThreads) void k_grad_pt(int NP, long Mc, const dou
       b[2] = bz[p];
     }
     double t[3] = {0.0, 0.0, 0.0};
-    for (int i = 0; i < NP; ++i) grad_bracket<GGA>(XY, phi, hess, cs, (long)i * Mc + p, a2, b, t);
+    const long tile = p / kScreenTile;
+    for (int e = tf_off[tile]; e < tf_off[tile + 1]; ++e)  // the functions not cut off here
+      grad_bracket<GGA>(XY, phi, hess, cs, (long)tfl[e] * Mc + p, a2, b, t);
     for (int d = 0; d < 3; ++d) pm[d * pstride + p] += 2.0 * t[d];
   }
 }
@@ -432,33 +447,34 @@ void launch_basis_hess(hipStream_t s, const double *gx, const double *gy, const 
                      shells, exps, norms, contr, NP, Mc, hess);
 }
 
-void launch_grad_fn(hipStream_t s, bool gga, int NP, long Mc, const double *XY,
-                    const double *phi, const double *hess, const double *a, const double *bx,
-                    const double *by, const double *bz, double *part) {
-  const int nseg = (int)(Mc / kGradSeg);
-  const long total = (long)NP * nseg;
-  const unsigned g = (unsigned)std::min<long>(total, 256L * 8);
+void launch_grad_fn(hipStream_t s, bool gga, long ntiles, const int *tf_off, const uint16_t *tfl,
+                    int NP, long Mc, const double *XY, const double *phi, const double *hess,
+                    const double *a, const double *bx, const double *by, const double *bz,
+                    double *part) {
+  if (ntiles <= 0) return;
+  const unsigned g = (unsigned)std::min<long>(ntiles, 256L * 8);
   if (gga)
-    hipLaunchKernelGGL(k_grad_fn<true>, dim3(g), dim3(kThreads), 0, s, NP, Mc, nseg, XY, phi,
-                       hess, a, bx, by, bz, part);
+    hipLaunchKernelGGL(k_grad_fn<true>, dim3(g), dim3(kThreads), 0, s, ntiles, tf_off, tfl, NP, Mc,
+                       XY, phi, hess, a, bx, by, bz, part);
   else
-    hipLaunchKernelGGL(k_grad_fn<false>, dim3(g), dim3(kThreads), 0, s, NP, Mc, nseg, XY, phi,
-                       hess, a, bx, by, bz, part);
+    hipLaunchKernelGGL(k_grad_fn<false>, dim3(g), dim3(kThreads), 0, s, ntiles, tf_off, tfl, NP, Mc,
+                       XY, phi, hess, a, bx, by, bz, part);
 }
 
 }  // namespace pgxc
 
 namespace pgxc {
-void launch_grad_pt(hipStream_t s, bool gga, int NP, long Mc, const double *XY, const double *phi,
-                    const double *hess, const double *a, const double *bx, const double *by,
-                    const double *bz, double *pm, long pstride) {
+void launch_grad_pt(hipStream_t s, bool gga, const int *tf_off, const uint16_t *tfl, int NP,
+                    long Mc, const double *XY, const double *phi, const double *hess,
+                    const double *a, const double *bx, const double *by, const double *bz,
+                    double *pm, long pstride) {
   const unsigned g = (unsigned)std::min<long>((Mc + kThreads - 1) / kThreads, 256L * 8);
   if (gga)
-    hipLaunchKernelGGL(k_grad_pt<true>, dim3(g), dim3(kThreads), 0, s, NP, Mc, XY, phi, hess, a,
-                       bx, by, bz, pm, pstride);
+    hipLaunchKernelGGL(k_grad_pt<true>, dim3(g), dim3(kThreads), 0, s, tf_off, tfl, NP, Mc, XY,
+                       phi, hess, a, bx, by, bz, pm, pstride);
   else
-    hipLaunchKernelGGL(k_grad_pt<false>, dim3(g), dim3(kThreads), 0, s, NP, Mc, XY, phi, hess, a,
-                       bx, by, bz, pm, pstride);
+    hipLaunchKernelGGL(k_grad_pt<false>, dim3(g), dim3(kThreads), 0, s, tf_off, tfl, NP, Mc, XY,
+                       phi, hess, a, bx, by, bz, pm, pstride);
 }
 
 void launch_becke_grad(hipStream_t s, int na, const AtomD *atoms, const double *inv_dist, long Mc,

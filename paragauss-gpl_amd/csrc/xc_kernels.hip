@@ -1429,6 +1429,96 @@ void launch_basis(hipStream_t s, bool grad, const StageList &L) {
   }
 }
 
+// ---------------------------------------------------- gradient products
+// The products of the nuclear gradient, X = Phi P and Y_c = (d/dc Phi) P
+// (P the density matrix of the last build), over the density contraction's
+// items: 128-point tile x pair of chunks of the tile's functions, K over the
+// tile's runs, the same GEMM with P for D.  Written to XY[plane][f][p] for
+// the tile's functions f only: the gradient's per-point terms take X_f, Y_f
+// where phi_f is not cut off.  nplanes: 1 (X, LDA) or 4.  Units u with
+// u % nspin == spin only (a plan lists a molecule's spins as consecutive
+// units; their items are the same tiles).
+template <int BN, int VAR>
+__global__ __launch_bounds__(kThreads) void k_xy(const StageList S, const double *__restrict__ P,
+                                                 double *__restrict__ XY, int nplanes, int nspin,
+                                                 int spin) {
+  typedef double T;
+  constexpr int BM = kDBM, BK = Var<VAR>::BK, ST = Var<VAR>::ST, TM = BM / 32, TN = BN / 32;
+  __shared__ T As[lds_elems<T, BM, BK, ST, false>()];
+  __shared__ __attribute__((aligned(16))) T Bs[lds_elems<T, BN, BK, ST, true>()];
+  __shared__ uint16_t kl[kMaxRuns];
+  __shared__ uint16_t cl[2 * BN];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wm = wave >> 1, wn = wave & 1;
+  for (long L = blockIdx.x; L < S.total; L += gridDim.x) {
+    const long it = item_at(S.order, L, S.total);
+    const int u = find_unit(S.base, S.nu, it);
+    if (u % nspin != spin) continue;  // uniform over the workgroup
+    const UnitD &U = S.units[u];
+    const uint32_t e = U.ditems[it - S.base[u]];
+    const int tile = (int)(e >> 8), pair = (int)(e & 255);
+    const long m0 = (long)tile * BM, Mc = U.Mc, NP = U.NP, cs = NP * Mc;
+    const int off = U.tl_off[tile], c8 = U.tl_off[tile + 1] - off;
+    const int foff = U.tf_off[tile] + 2 * pair * BN, nf = U.tf_off[tile + 1] - foff;
+    __syncthreads();  // the previous item's readers of kl and cl are done
+    for (int i = threadIdx.x; i < min(2 * BN, nf); i += kThreads) cl[i] = gld(U.tfl + foff + i);
+    lds_list(U.tl + off, c8, kl);
+    const int ksteps = (c8 * kRun + BK - 1) / BK;
+    const T *phi = (const T *)U.phi;
+    for (int h = 0; h < 2 && h * BN < nf; ++h) {
+      const uint16_t *cfn = cl + h * BN;
+      const int ncol = min(BN, nf - h * BN);
+      auto baddr = [&](int n, int kt, int kk) -> long {
+        const long f = run_fn(kl, c8, kt * BK + kk);
+        return (f < 0 || n >= ncol) ? -1 : (long)cfn[min(n, ncol - 1)] * NP + f;
+      };
+      for (int c = 0; c < nplanes; ++c) {
+        auto aaddr = [&](int m, int kt, int kk) -> long {
+          const long f = run_fn(kl, c8, kt * BK + kk);
+          return f < 0 ? -1 : c * cs + f * Mc + m;
+        };
+        typename MF<T>::acc acc[TM][TN];
+        gemm_tile<T, BM, BN, BK, ST, false, true>(phi + m0, P, ksteps, aaddr, baddr, As, Bs, acc);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const int n = wn * (BN / 2) + j * 16 + mf_col(lane);
+          if (n >= ncol) continue;
+          double *xy = XY + c * cs + (long)cfn[n] * Mc + m0;
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) gst(xy + wm * (BM / 2) + i * 16 + MF<T>::row(lane, r), acc[i][j][r]);
+        }
+      }
+    }
+  }
+}
+
+template <int BN>
+static void xy_bn(hipStream_t s, int var, const StageList &L, const double *P, double *XY, int np,
+                  int ns, int sp) {
+  dim3 g(persistent_grid(L.total, 2));
+  switch (var) {
+    case 1: hipLaunchKernelGGL((k_xy<BN, 1>), g, dim3(kThreads), 0, s, L, P, XY, np, ns, sp); break;
+    case 2: hipLaunchKernelGGL((k_xy<BN, 2>), g, dim3(kThreads), 0, s, L, P, XY, np, ns, sp); break;
+    default: hipLaunchKernelGGL((k_xy<BN, 0>), g, dim3(kThreads), 0, s, L, P, XY, np, ns, sp); break;
+  }
+}
+
+void launch_xy(hipStream_t s, int tf, int var, const StageList &L, const double *P, double *XY,
+               int nplanes, int nspin, int spin) {
+  if (!L.total) return;
+  StageList W = L;  // a plain static walk (no queues, no schedule order, no log)
+  W.ctr = nullptr;
+  W.qbeg = nullptr;
+  W.order = nullptr;
+  W.tlog = nullptr;
+  switch (tf) {
+    case 64: xy_bn<64>(s, var, W, P, XY, nplanes, nspin, spin); break;
+    case 96: xy_bn<96>(s, var, W, P, XY, nplanes, nspin, spin); break;
+    default: xy_bn<128>(s, var, W, P, XY, nplanes, nspin, spin); break;
+  }
+}
+
 template <typename T, int BN, int VAR>
 static void density_bn(hipStream_t s, bool grad, const StageList &L) {
   dim3 g(contraction_grid(L, 2));
