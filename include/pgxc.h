@@ -203,7 +203,10 @@ int pgxc_energy(pgxc_ctx *ctx, uint32_t functional, int32_t precision, double *t
  * receives dE_xc/dR (written, not added; the reference stores the negative).
  * Evaluated at the density matrix and potential coefficients of the last
  * build on the context, in FP64 whatever that build's precision: basis
- * values and Hessians per chunk, X = Phi P and (d/dc Phi) P as DGEMMs.
+ * values and Hessians per chunk, X = Phi P and (d/dc Phi) P on the screened
+ * tiles.  The first call allocates the work space of one chunk (Hessian
+ * planes and products, 80 bytes per function and point of a chunk) and the
+ * context keeps it.
  */
 int pgxc_gradient(pgxc_ctx *ctx, double *grad);
 
