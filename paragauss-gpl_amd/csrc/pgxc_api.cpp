@@ -151,9 +151,9 @@ struct pgxc_ctx {
   uint8_t *gcf = nullptr;
   uint32_t *di = nullptr, *pi = nullptr;
   // blocks of the basis values some stage reads (see setup_screen): per
-  // group the value and gradient run words, per chunk the basis items
-  // (256-point block << 16 | atom) that touch them and their write flags
-  uint32_t *bneed = nullptr, *bi = nullptr, *bfl = nullptr;
+  // chunk the basis items (256-point block << 16 | atom) that touch them and
+  // their write flags
+  uint32_t *bi = nullptr, *bfl = nullptr;
   std::vector<long> h_bi_off;
   std::vector<int> fpos;          // internal position of each caller function
   std::vector<int> fatom;         // atom of each internal function position
@@ -490,7 +490,6 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.sig8 = c->sig8 + (p0 / kGroup) * c->nw8;
       U.wbits = c->wbits + (p0 / kGroup) * c->nw8;
       U.bscreen = dense_basis ? 0 : 1;  // the basis stage writes the blocks read later
-      U.bneed = c->bneed + (p0 / kGroup) * 2 * c->nw8;
       U.bitems = c->bi + c->h_bi_off[k];
       U.bflags = c->bfl + 8 * c->h_bi_off[k];
       U.nbi = c->h_bi_off[k + 1] - c->h_bi_off[k];
@@ -1282,7 +1281,6 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
       }
       c->h_bi_off.push_back((long)bi.size());
     }
-    if ((rc = upload(c, &c->bneed, need))) return rc;
     if ((rc = upload(c, &c->bi, bi))) return rc;
     if ((rc = upload(c, &c->bfl, bfl))) return rc;
     if (getenv("PGXC_DEBUG")) {  // sizes of the screened basis work
@@ -2084,7 +2082,7 @@ int pgxc_destroy(pgxc_ctx *c) {
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
                   c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->tf_off, c->tfl, c->tfb, c->su_off, c->su,
-                  c->gcf_off, c->gcf, c->di, c->pi, c->bneed, c->bi, c->bfl, c->ghess, c->gXY, c->gpart, c->beta.P, c->beta.Pref, c->beta.D,
+                  c->gcf_off, c->gcf, c->di, c->pi, c->bi, c->bfl, c->ghess, c->gXY, c->gpart, c->beta.P, c->beta.Pref, c->beta.D,
                   c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
                   c->beta.Vstate, c->beta.Vout, c->eterm};
   for (void *b : bufs) dfree(b);

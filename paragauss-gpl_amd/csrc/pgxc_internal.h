@@ -83,12 +83,11 @@ struct UnitD {
   const uint8_t *gcf;
   const uint32_t *sig8, *wbits;
   // Basis stage over the blocks the other stages read (bscreen = 1; 0: every
-  // block).  bneed: per 32-point group (from the unit's first), 2 nw8 words:
-  // the runs whose values, then the runs whose gradients a stage reads there.
-  // bitems: basis items (256-point block << 16 | atom) touching them, with
-  // bflags: 8 words per item, bit 8 j + g of words 0-3 (4-7): the values
-  // (gradients) of the atom's shell j are read on group g of the block.
-  const uint32_t *bneed, *bitems, *bflags;
+  // block).  bitems: basis items (256-point block << 16 | atom) that write
+  // some block, with bflags: 8 words per item, bit 8 j + g of words 0-3
+  // (4-7): the values (gradients) of the atom's shell j are read on group g
+  // of the block.
+  const uint32_t *bitems, *bflags;
   long nbi;
   int bscreen, pad_b;
   long Mc, npad;
