@@ -60,8 +60,8 @@ extern "C" {
 /* Schedules of the two contraction stages (pgxc_set_schedule,
  * pgxc_batch_set_schedule).  They change which workgroup computes which work
  * item and when, never a result bit. */
-#define PGXC_SCHED_STATIC 0   /* fixed round-robin walk (the default)         */
-#define PGXC_SCHED_ANALYTIC 1 /* queues balanced by counted flops per item     */
+#define PGXC_SCHED_STATIC 0   /* fixed round-robin walk                        */
+#define PGXC_SCHED_ANALYTIC 1 /* queues balanced by counted flops (the default) */
 #define PGXC_SCHED_MODEL 2    /* queues balanced by the cost model's prediction */
 
 /* Features of one work item, in this order (pgxc_item_log, cost model):

@@ -35,7 +35,7 @@ long round_up(long a, long b) { return (a + b - 1) / b * b; }
 
 // schedule of the contraction stages (PGXC_SCHED_*) and item logging
 struct Sched {
-  int mode = PGXC_SCHED_STATIC;
+  int mode = PGXC_SCHED_ANALYTIC;  // no slower than the static walk in any measurement
   pgxc_costmodel *model = nullptr;
   int log = 0;
 };
