@@ -364,8 +364,7 @@ def bench_open_shell(args):
     tu = None if args.no_tune else autotune.tune(b, g, C, occ, mixed=BF16X3, log=log)
     xc = XCBuilder(b, g, tile=tu["tile"] if tu else 0,
                    variants=(tu["variant_fp64"], tu["variant_fp32"]) if tu else (-1, -1))
-    from pgxc import _lib
-    xc.set_schedule(_lib.SCHED_ANALYTIC, None)  # the headline's schedule
+    headline_schedule(xc)
 
     def timed(prec, incremental_chain=None):
         out = []
@@ -448,6 +447,15 @@ def bench_post_scf(args):
                       **out}), flush=True)
 
 
+def headline_schedule(xc):
+    """The headline's contraction schedule: queues ordered by the shipped cost
+    model, or by counted flops where no model ships."""
+    from pgxc import _lib
+    from pgxc.costmodel import default_model
+    model = default_model()
+    xc.set_schedule(_lib.SCHED_MODEL if model is not None else _lib.SCHED_ANALYTIC, model)
+
+
 def bench_profile(args):
     """The kernels the headline's roofline is taken from, alone, for rocprofv3
     (kernel trace and traffic counters): a copy kernel of known size (counter
@@ -458,8 +466,9 @@ def bench_profile(args):
     copy_calibration(1024)
     els, b, g, C, occ = workload("h2o32")
     for cache in (False, True):
-        xc = XCBuilder(b, g, tile=96, variants=(2, 1), basis_cache=cache)
+        xc = XCBuilder(b, g, tile=96, variants=(1, 1), basis_cache=cache)
         xc.set_lanes(1)
+        headline_schedule(xc)
         xc.set_orbitals(C, occ)
         for prec in (FP64, BF16X3):
             for i in range(args.steps):

@@ -79,6 +79,10 @@ struct pgxc_ctx {
   double *grid = nullptr;  // [4][npad] x, y, z, w
   AtomD *atoms = nullptr;
   ShellD *shells = nullptr;
+  RangeD *ranges = nullptr;  // atom ranges of the basis stage
+  int *ratoms = nullptr;     // atoms in shell order
+  int n_ranges = 0;
+  std::vector<RangeD> h_ranges;
   double *exps = nullptr, *norms = nullptr, *contr = nullptr;
   // device: density (NP x NP); D holds the operand in the build's precision
   double *P = nullptr, *Pref = nullptr;
@@ -273,7 +277,7 @@ enum ItemKind { K_BASIS, K_FUNC, K_WMAT, K_DENS, K_POT };
 
 long unit_items(const UnitD &U, ItemKind k, int tf) {
   switch (k) {
-    case K_BASIS: return U.bscreen ? U.nbi : items_basis(U.Mc, U.n_atoms);
+    case K_BASIS: return U.bscreen ? U.nbi : items_basis(U.Mc, U.n_ranges);
     case K_FUNC: return items_functional(U.Mc);
     case K_WMAT: return items_wmat(U.Mc, U.NP);
     case K_DENS: return U.ndi;
@@ -450,6 +454,9 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.gw = c->grid + 3 * c->npad + p0;
       U.atoms = c->atoms;
       U.shells = c->shells;
+      U.ranges = c->ranges;
+      U.ratoms = c->ratoms;
+      U.n_ranges = c->n_ranges;
       U.exps = c->exps;
       U.norms = c->norms;
       U.contr = c->contr;
@@ -491,7 +498,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.wbits = c->wbits + (p0 / kGroup) * c->nw8;
       U.bscreen = dense_basis ? 0 : 1;  // the basis stage writes the blocks read later
       U.bitems = c->bi + c->h_bi_off[k];
-      U.bflags = c->bfl + 8 * c->h_bi_off[k];
+      U.bflags = c->bfl + 16 * c->h_bi_off[k];
       U.nbi = c->h_bi_off[k + 1] - c->h_bi_off[k];
       if (c->ispin == 2) {  // the beta spin's unit: same points and basis values
         UnitD B = U;
@@ -1229,18 +1236,17 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
           need[(size_t)g * 2 * nw8 + nw8 + w] = tw[(size_t)w];
         }
     }
-    // each atom's contiguous shell range (first, count)
-    std::vector<std::pair<int, int>> ssh((size_t)c->n_atoms, {0, 0});
-    for (int i = (int)sh.size() - 1; i >= 0; --i) {
-      ssh[(size_t)sh[(size_t)i].atom].first = i;
-      ++ssh[(size_t)sh[(size_t)i].atom].second;
-    }
-    // runs holding each atom's functions
-    std::vector<uint32_t> amask((size_t)c->n_atoms * nw8, 0u);
-    for (const ShellD &d : sh) {
-      const int nf = (2 * d.l + 1) * (d.n_unc + d.n_contr);
-      for (int r = d.func_off / kRun; r <= (d.func_off + nf - 1) / kRun; ++r)
-        amask[(size_t)d.atom * nw8 + r / 32] |= 1u << (r % 32);
+    // runs holding each atom range's functions
+    const int nr = c->n_ranges;
+    std::vector<uint32_t> rmask((size_t)nr * nw8, 0u);
+    for (int ri = 0; ri < nr; ++ri) {
+      const RangeD &R = c->h_ranges[(size_t)ri];
+      for (int si = R.shell_begin; si < R.shell_begin + R.n_shells; ++si) {
+        const ShellD &d = sh[(size_t)si];
+        const int nf = (2 * d.l + 1) * (d.n_unc + d.n_contr);
+        for (int r = d.func_off / kRun; r <= (d.func_off + nf - 1) / kRun; ++r)
+          rmask[(size_t)ri * nw8 + r / 32] |= 1u << (r % 32);
+      }
     }
     std::vector<uint32_t> bi, bfl, bm((size_t)nw8);
     c->h_bi_off.assign(1, 0);
@@ -1251,15 +1257,17 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
         std::fill(bm.begin(), bm.end(), 0u);
         for (long g = g0 + b * gpb; g < g0 + (b + 1) * gpb; ++g)
           for (int w = 0; w < nw8; ++w) bm[(size_t)w] |= need[(size_t)g * 2 * nw8 + w];
-        for (int a = 0; a < c->n_atoms; ++a) {
+        for (int ri = 0; ri < nr; ++ri) {
           uint32_t any = 0;
-          for (int w = 0; w < nw8 && !any; ++w) any = bm[(size_t)w] & amask[(size_t)a * nw8 + w];
+          for (int w = 0; w < nw8 && !any; ++w) any = bm[(size_t)w] & rmask[(size_t)ri * nw8 + w];
           if (!any) continue;
-          bi.push_back((uint32_t)(b << 16 | a));
-          // per (shell j of the atom, group g of the block): values, gradients read
-          uint32_t fw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          for (int j = 0; j < ssh[(size_t)a].second; ++j) {
-            const ShellD &d = sh[(size_t)(ssh[(size_t)a].first + j)];
+          bi.push_back((uint32_t)(b << 16 | ri));
+          // per (shell j of the range, group g of the block): values, gradients read
+          uint32_t fw[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+          const RangeD &R = c->h_ranges[(size_t)ri];
+          for (int j = 0; j < R.n_shells; ++j) {
+            const int sg = R.shell_begin + j;
+            const ShellD &d = sh[(size_t)sg];
             const int nf = (2 * d.l + 1) * (d.n_unc + d.n_contr);
             for (long gg = 0; gg < gpb; ++gg) {
               const uint32_t *nd = need.data() + (size_t)(g0 + b * gpb + gg) * 2 * nw8;
@@ -1269,14 +1277,13 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
               // gradients: read only for the tile's functions (the density
               // epilogue's columns, and the W pass's live functions)
               const long tg = (g0 + b * gpb + gg) / 4;
-              const int sg = ssh[(size_t)a].first + j;
               const uint32_t gr = smark[(size_t)tg * nws + sg / 32] >> (sg % 32);
               const int bit = 8 * j + (int)gg;
               fw[bit >> 5] |= (v & 1u) << (bit & 31);
-              fw[4 + (bit >> 5)] |= (gr & 1u) << (bit & 31);
+              fw[8 + (bit >> 5)] |= (gr & 1u) << (bit & 31);
             }
           }
-          bfl.insert(bfl.end(), fw, fw + 8);
+          bfl.insert(bfl.end(), fw, fw + 16);
         }
       }
       c->h_bi_off.push_back((long)bi.size());
@@ -1293,7 +1300,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
         }
       fprintf(stderr, "pgxc: %ld groups x %d runs; blocks live %ld, values read %ld, gradients read %ld; "
                       "basis items %zu (of %ld)\n",
-              ngr, n8, live, nv, ng, bi.size(), (long)c->nchunks * (c->Mc / kThreads) * c->n_atoms);
+              ngr, n8, live, nv, ng, bi.size(), (long)c->nchunks * (c->Mc / kThreads) * c->n_ranges);
     }
   }
   // work items per chunk
@@ -1495,10 +1502,41 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
   HIPCHK(c, hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   HIPCHK(c, hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
+  // atom ranges of the basis stage: consecutive atoms in shell order, as
+  // many as the kernel's LDS image holds
+  std::vector<int> ratoms;
+  for (int a = 0; a < n_atoms; ++a)
+    if (at[a].shell_end > at[a].shell_begin) ratoms.push_back(a);
+  std::sort(ratoms.begin(), ratoms.end(),
+            [&](int p, int q) { return at[p].shell_begin < at[q].shell_begin; });
+  c->h_ranges.clear();
+  for (size_t i = 0; i < ratoms.size(); ++i) {
+    const AtomD &a = at[ratoms[i]];
+    const int ns = a.shell_end - a.shell_begin;
+    if (c->h_ranges.empty() || c->h_ranges.back().n_atoms == kRangeAtoms ||
+        c->h_ranges.back().n_shells + ns > kRangeShells ||
+        c->h_ranges.back().n_exp + a.n_exp > kRangePrims ||
+        c->h_ranges.back().n_contr + a.n_contr > kRangeContr) {
+      RangeD r{};
+      r.atom_begin = (int)i;
+      r.shell_begin = a.shell_begin;
+      r.exp_begin = a.exp_begin;
+      r.contr_begin = a.contr_begin;
+      c->h_ranges.push_back(r);
+    }
+    RangeD &r = c->h_ranges.back();
+    ++r.n_atoms;
+    r.n_shells += ns;
+    r.n_exp += a.n_exp;
+    r.n_contr += a.n_contr;
+  }
+  c->n_ranges = (int)c->h_ranges.size();
   int rc;
   if ((rc = dalloc(c, &c->grid, 4 * (size_t)c->npad))) return rc;
   if ((rc = dalloc(c, &c->atoms, n_atoms))) return rc;
   if ((rc = dalloc(c, &c->shells, n_shells))) return rc;
+  if ((rc = dalloc(c, &c->ranges, std::max<size_t>(c->h_ranges.size(), 1)))) return rc;
+  if ((rc = dalloc(c, &c->ratoms, std::max<size_t>(ratoms.size(), 1)))) return rc;
   if ((rc = dalloc(c, &c->exps, eoff))) return rc;
   if ((rc = dalloc(c, &c->norms, eoff))) return rc;
   if ((rc = dalloc(c, &c->contr, std::max<long>(coff, 1)))) return rc;
@@ -1526,6 +1564,11 @@ int pgxc_create_opt(pgxc_ctx **out, int32_t n_atoms, const double *atom_xyz, int
   }
   HIPCHK(c, hipMemcpy(c->atoms, at.data(), sizeof(AtomD) * n_atoms, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->shells, sh.data(), sizeof(ShellD) * n_shells, hipMemcpyHostToDevice));
+  if (!ratoms.empty()) {
+    HIPCHK(c, hipMemcpy(c->ranges, c->h_ranges.data(), sizeof(RangeD) * c->h_ranges.size(),
+                        hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->ratoms, ratoms.data(), sizeof(int) * ratoms.size(), hipMemcpyHostToDevice));
+  }
   if ((rc = dalloc(c, &c->d_fpos, c->nbas))) return rc;
   HIPCHK(c, hipMemcpy(c->d_fpos, c->fpos.data(), sizeof(int) * c->nbas, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->exps, exps, 8 * eoff, hipMemcpyHostToDevice));
@@ -2078,7 +2121,7 @@ int pgxc_destroy(pgxc_ctx *c) {
     std::lock_guard<std::mutex> lk(g_cache_mu);
     g_cache_bytes[c->device] -= c->phic_bytes[0] + c->phic_bytes[1];
   }
-  void *bufs[] = {c->grid, c->atoms, c->shells, c->exps, c->norms, c->contr, c->P, c->Pref,
+  void *bufs[] = {c->grid, c->atoms, c->shells, c->ranges, c->ratoms, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
                   c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->tf_off, c->tfl, c->tfb, c->su_off, c->su,

@@ -12,9 +12,14 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kLMax = 4;      // highest angular momentum of a shell
 constexpr int kMaxContr = 8;  // contracted functions per shell
-// per atom, what the basis kernel stages in LDS: shells, primitives and
-// contraction coefficients
+// per atom at most: shells, primitives and contraction coefficients
 constexpr int kAtomShells = 16, kAtomPrims = 256, kAtomContr = 1024;
+// A basis item evaluates one 256-point block against a range of atoms,
+// consecutive in shell order, whose basis data the kernel stages in LDS
+// together: at most kRangeAtoms atoms, kRangeShells shells, kRangePrims
+// primitives and kRangeContr contraction coefficients (an atom of the
+// per-atom maximum fits alone).
+constexpr int kRangeAtoms = 8, kRangeShells = 32, kRangePrims = 256, kRangeContr = 1024;
 constexpr int kThreads = 256; // 4 wave64 per workgroup everywhere
 constexpr double kFar = 1.0e6;  // coordinate of padding points (all exps cut off)
 
@@ -32,6 +37,14 @@ struct AtomD {
   int exp_begin, n_exp, contr_begin, n_contr;
 };
 
+// A range of atoms for the basis stage: ratoms[atom_begin .. +n_atoms) are
+// its atoms, in shell order; their shells (shell_begin, n_shells), primitives
+// and contraction coefficients are contiguous.
+struct RangeD {
+  int atom_begin, n_atoms, shell_begin, n_shells;
+  int exp_begin, n_exp, contr_begin, n_contr;
+};
+
 // Per-point state kept across builds (incremental mode), SoA of length npad.
 enum StateSlot { S_RHO = 0, S_GX, S_GY, S_GZ, S_A, S_BX, S_BY, S_BZ, S_NSLOT };
 
@@ -43,6 +56,8 @@ struct UnitD {
   const AtomD *atoms;
   const ShellD *shells;
   const double *exps, *norms, *contr;
+  const RangeD *ranges;  // the basis stage's atom ranges
+  const int *ratoms;     // atoms in shell order (the ranges index it)
   void *phi;        // [4][NP][Mc] values, d/dx, d/dy, d/dz (T)
   void *W;          // [NP][Mc] (T); split-BF16 builds: bf16 hi / lo planes [2][NP][Mc]
   const void *D;    // [NP][NP] density operand (T); split-BF16: transposed hi / lo planes
@@ -83,15 +98,15 @@ struct UnitD {
   const uint8_t *gcf;
   const uint32_t *sig8, *wbits;
   // Basis stage over the blocks the other stages read (bscreen = 1; 0: every
-  // block).  bitems: basis items (256-point block << 16 | atom) that write
-  // some block, with bflags: 8 words per item, bit 8 j + g of words 0-3
-  // (4-7): the values (gradients) of the atom's shell j are read on group g
-  // of the block.
+  // block).  bitems: basis items (256-point block << 16 | atom range) that
+  // write some block, with bflags: 16 words per item, bit 8 j + g of words
+  // 0-7 (8-15): the values (gradients) of the range's shell j are read on
+  // group g of the block.
   const uint32_t *bitems, *bflags;
   long nbi;
   int bscreen, pad_b;
   long Mc, npad;
-  int n_atoms, NP, ntiles, nsplit, incremental;
+  int n_atoms, n_ranges, NP, ntiles, nsplit, incremental, pad_n;
   uint32_t mask;
   int CH, nw8;
   long ndi, npi;  // density / potential items of the unit
@@ -162,7 +177,7 @@ void launch_prep(hipStream_t s, const PackD *mols, int nm, const long *base, lon
                  int mode, bool finish);
 
 // item counts per unit, shared by the planner and the kernels' decoding
-inline long items_basis(long Mc, int n_atoms) { return (Mc / kThreads) * n_atoms; }
+inline long items_basis(long Mc, int n_ranges) { return (Mc / kThreads) * n_ranges; }
 inline long items_functional(long Mc) { return Mc / kThreads; }
 inline long items_wmat(long Mc, int NP) { return (Mc / (4 * kThreads)) * (NP / 16); }
 // point splits of the potential contraction are whole multiples of this

@@ -335,15 +335,18 @@ __device__ __forceinline__ void put_fn(long f0, int nf, T r0, T r1, T x, T y, T 
   }
 }
 
-// One atom's basis data staged in LDS for a basis item: its shells, their
-// exponents, norms and contraction coefficients in the evaluation
-// precision, and per (shell, 32-point group of the item) what to write:
-// bit 0 the values, bit 1 the gradients.
+// A range of atoms' basis data staged in LDS for a basis item: the shells,
+// their exponents, norms and contraction coefficients in the evaluation
+// precision, per atom its centre, shell range (relative to the range's
+// first shell) and largest l, and per (shell, 32-point group of the item)
+// what to write: bit 0 the values, bit 1 the gradients.
 template <typename T>
-struct AtomLds {
-  ShellD sh[kAtomShells];
-  T ex[kAtomPrims], nm[kAtomPrims], ct[kAtomContr];
-  uint8_t fl[kAtomShells][kThreads / kGroup];
+struct RangeLds {
+  ShellD sh[kRangeShells];
+  T ex[kRangePrims], nm[kRangePrims], ct[kRangeContr];
+  double ax[kRangeAtoms], ay[kRangeAtoms], az[kRangeAtoms];
+  int s0[kRangeAtoms], s1[kRangeAtoms], lm[kRangeAtoms];
+  uint8_t fl[kRangeShells][kThreads / kGroup];
 };
 
 // ex, nm: the shell's primitives; ct: its n_exp x n_contr coefficients.
@@ -408,17 +411,17 @@ __device__ __forceinline__ void shell_emit(const ShellD &sh, const T *ex, const 
       put_fn<L, GRAD>(sh.func_off + nu + k, nf, rc0[k], rc1[k], x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p, wg);
 }
 
-// The atom's shells at one point, from the LDS image A (nsh shells, first
-// primitive e0 and coefficient c0 of the atom); grp: the point's group in
-// the item.
+// One atom's shells sb .. se-1 of the LDS image A at one point (e0, c0: the
+// range's first primitive and coefficient); grp: the point's group in the
+// item.
 template <int LA, bool GRAD, typename T>
-__device__ __forceinline__ void basis_atom(const AtomLds<T> &A, int nsh, int e0, int c0, T x, T y,
-                                           T z, T *phi, long Mc, long cs, int p, int grp) {
+__device__ __forceinline__ void basis_atom(const RangeLds<T> &A, int sb, int se, int e0, int c0,
+                                           T x, T y, T z, T *phi, long Mc, long cs, int p, int grp) {
   constexpr int NLM = (LA + 1) * (LA + 1);
   T C[NLM], Cx[NLM], Cy[NLM], Cz[NLM];
   sh_eval<LA, T>(x, y, z, C, Cx, Cy, Cz);
   const T r2 = x * x + y * y + z * z;
-  for (int s = 0; s < nsh; ++s) {
+  for (int s = sb; s < se; ++s) {
     const int f = A.fl[s][grp];
     if (!(f & 1)) continue;  // no stage reads the shell's values on this group
     const bool wg = GRAD && (f & 2);
@@ -585,17 +588,18 @@ __device__ __forceinline__ long item_at(const int *__restrict__ order, long L, l
 }
 
 // ------------------------------------------------------------------- basis
-// item = (256-point block, atom) of a unit; one thread per (point, atom).
-// The workgroup first stages the atom's shells, primitives and coefficients
-// and the per-(shell, group) write flags in LDS, so the evaluation itself
-// waits on no global load.  Screened units (bscreen) walk their list of the
-// (block, atom) pairs whose values a later stage reads, and write only the
+// item = (256-point block, range of atoms) of a unit; one thread per point,
+// walking the range's atoms.  The workgroup first stages the range's shells,
+// primitives, coefficients and atom centres and the per-(shell, group) write
+// flags in LDS, all loads in flight together, so the evaluation itself waits
+// on no global load.  Screened units (bscreen) walk their list of the
+// (block, range) pairs whose values a later stage reads, and write only the
 // shells read on each point's group; dense units write every shell.
 template <bool GRAD, typename T, int LMAX>
 __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ units, int nu,
                                                     const long *__restrict__ base,
                                                     const int *__restrict__ order, long total) {
-  __shared__ AtomLds<T> A;
+  __shared__ RangeLds<T> A;
   const int tid = threadIdx.x;
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
     const long it = item_at(order, L, total);
@@ -603,51 +607,64 @@ __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ un
     const UnitD &U = units[u];
     const long local = it - base[u];
     long blk;
-    int atom;
+    int rng;
     if (U.bscreen) {
       const uint32_t e = U.bitems[local];
       blk = e >> 16;
-      atom = (int)(e & 0xffffu);
+      rng = (int)(e & 0xffffu);
     } else {
-      atom = (int)(local % U.n_atoms);
-      blk = local / U.n_atoms;
+      rng = (int)(local % U.n_ranges);
+      blk = local / U.n_ranges;
     }
-    const AtomD at = U.atoms[atom];
-    const int s0 = at.shell_begin, nsh = at.shell_end - s0;
-    if (nsh <= 0) continue;  // an atom without shells (uniform over the workgroup)
-    const int e0 = at.exp_begin, c0 = at.contr_begin;
+    const RangeD R = U.ranges[rng];
+    const int e0 = R.exp_begin, c0 = R.contr_begin;
     const long p = blk * kThreads + tid;
-    // the point's centre offsets in FP64, loads in flight across the staging
+    // the point's coordinates in FP64, loads in flight across the staging
     const double gx = gld(U.gx + p), gy = gld(U.gy + p), gz = gld(U.gz + p);
     __syncthreads();  // the previous item's reads of A are done
-    for (int i = tid; i < nsh; i += kThreads) A.sh[i] = U.shells[s0 + i];
-    for (int i = tid; i < at.n_exp; i += kThreads) {
+    for (int i = tid; i < R.n_shells; i += kThreads) A.sh[i] = U.shells[R.shell_begin + i];
+    for (int i = tid; i < R.n_exp; i += kThreads) {
       A.ex[i] = (T)gld(U.exps + e0 + i);
       A.nm[i] = (T)gld(U.norms + e0 + i);
     }
-    for (int i = tid; i < at.n_contr; i += kThreads) A.ct[i] = (T)gld(U.contr + c0 + i);
-    if (tid < nsh * (kThreads / kGroup)) {  // flag of (shell tid / 8, group tid % 8)
+    for (int i = tid; i < R.n_contr; i += kThreads) A.ct[i] = (T)gld(U.contr + c0 + i);
+    if (tid < R.n_atoms) {
+      const AtomD at = U.atoms[gld(U.ratoms + R.atom_begin + tid)];
+      A.ax[tid] = at.x;
+      A.ay[tid] = at.y;
+      A.az[tid] = at.z;
+      A.s0[tid] = at.shell_begin - R.shell_begin;
+      A.s1[tid] = at.shell_end - R.shell_begin;
+      A.lm[tid] = at.lmax;
+    }
+    if (tid < R.n_shells * (kThreads / kGroup)) {  // flag of (shell tid / 8, group tid % 8)
       uint32_t f = GRAD ? 3u : 1u;
       if (U.bscreen) {
-        const uint32_t *fw = U.bflags + 8 * local;
+        const uint32_t *fw = U.bflags + 16 * local;
         f = ((fw[tid >> 5] >> (tid & 31)) & 1u) |
-            (GRAD ? ((fw[4 + (tid >> 5)] >> (tid & 31)) & 1u) << 1 : 0u);
+            (GRAD ? ((fw[8 + (tid >> 5)] >> (tid & 31)) & 1u) << 1 : 0u);
       }
       A.fl[tid / (kThreads / kGroup)][tid % (kThreads / kGroup)] = (uint8_t)f;
     }
     __syncthreads();
-    const T x = (T)(gx - at.x), y = (T)(gy - at.y), z = (T)(gz - at.z);
     T *phi = (T *)U.phi;
     const long cs = (long)U.NP * U.Mc;
     const int grp = tid / kGroup;
-    // the kernel holds the registers of its largest case, so it is built
-    // for the largest angular momentum of the list (at.lmax <= LMAX)
-    switch (at.lmax) {
-      case 0: basis_atom<0, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-      case 1: if constexpr (LMAX >= 1) basis_atom<1, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-      case 2: if constexpr (LMAX >= 2) basis_atom<2, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-      case 3: if constexpr (LMAX >= 3) basis_atom<3, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-      default: if constexpr (LMAX >= 4) basis_atom<4, GRAD>(A, nsh, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+    for (int a = 0; a < R.n_atoms; ++a) {
+      const int sb = A.s0[a], se = A.s1[a];
+      bool any = false;
+      for (int s = sb; s < se; ++s) any = any || (A.fl[s][grp] & 1);
+      if (!any) continue;  // no stage reads this atom's values on the point's group
+      const T x = (T)(gx - A.ax[a]), y = (T)(gy - A.ay[a]), z = (T)(gz - A.az[a]);
+      // the kernel holds the registers of its largest case, so it is built
+      // for the largest angular momentum of the list (every atom's lmax <= LMAX)
+      switch (A.lm[a]) {
+        case 0: basis_atom<0, GRAD>(A, sb, se, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+        case 1: if constexpr (LMAX >= 1) basis_atom<1, GRAD>(A, sb, se, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+        case 2: if constexpr (LMAX >= 2) basis_atom<2, GRAD>(A, sb, se, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+        case 3: if constexpr (LMAX >= 3) basis_atom<3, GRAD>(A, sb, se, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+        default: if constexpr (LMAX >= 4) basis_atom<4, GRAD>(A, sb, se, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+      }
     }
   }
 }
