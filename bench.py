@@ -667,14 +667,34 @@ def main():
     else:
         v = layout["variant_fp32"]
         kname = "k_%s<float,%d,%d%s>" % (dom, tf, v, ",true" if dom == "density" else "")
-    roof = {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": peak,
+    mfma = {"achieved": achieved, "peak": peak,
             "unit": "TFLOP/s" + (" (FP32-equivalent: BF16 peak / 3 passes)" if best == "bf16x3" else ""),
-            "frac": achieved / peak, "traffic": profiled_traffic(kname),
+            "frac": achieved / peak,
             "achieved_dense_equivalent": 2.0 * npts * n * n / (st[dom] * 1e-3) / 1e12,
             "achieved_executed": executed[best][dom] / (st[dom] * 1e-3) / 1e12,
             "flops_per_build_executed": executed[best],
             "flops_per_build_algorithmic": stats["flops"],
             "flops_per_build_dense": 4.0 * npts * n * n + 16.0 * npts * n}
+    # Algorithmic bytes of the contraction (per batch b of B_b points and n_b
+    # live functions): the density reads the P sub-block (8 n_b^2, FP64) and
+    # the stored values and gradients (4 planes of 4-byte elements,
+    # 16 B_b n_b); the potential writes the V partial (8 n_b^2) and reads the
+    # stored values (4 B_b n_b).  Against these bytes the contraction's
+    # arithmetic intensity sits below the ridge of its MFMA type (peak
+    # flops / 8 TB/s), which makes HBM its roofline bound.
+    abytes = (8.0 * stats["n2"] + (16.0 if dom == "density" else 4.0) * stats["bn"])
+    intensity = flops / abytes
+    ridge = peak / 8.0
+    if intensity < ridge:
+        ach_bw = abytes / (st[dom] * 1e-3) / 1e12
+        roof = {"bound": "hbm", "kernel": kname, "achieved": ach_bw, "peak": 8.0, "unit": "TB/s",
+                "frac": ach_bw / 8.0, "traffic": profiled_traffic(kname)}
+    else:
+        roof = {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": peak,
+                "unit": mfma["unit"], "frac": achieved / peak, "traffic": profiled_traffic(kname)}
+    roof.update({"algorithmic_bytes_per_launch": abytes / layout["chunks"],
+                 "algorithmic_flops_per_launch": flops / layout["chunks"],
+                 "intensity_flop_per_byte": intensity, "ridge_flop_per_byte": ridge, "mfma": mfma})
     # basis stage (k_basis) against the HBM roof: FP32 values and
     # gradients of the live functions written once, and the grid read
     bytes_basis = 32.0 * npts + 16.0 * stats["bn"]
@@ -688,11 +708,11 @@ def main():
     roof["timed"] = "one-lane builds with the basis evaluated (%d launches)" % (nlaunch * layout["chunks"])
     roof["traffic_unit"] = ("bytes per dispatch (one grid chunk), rocprofv3 FETCH_SIZE + WRITE_SIZE "
                             "scaled by a copy kernel")
-    if roof["traffic"]:
-        per_ms = st[dom] / layout["chunks"]
-        hbm = roof["traffic"] / (per_ms * 1e-3) / 1e12
-        roof["hbm"] = {"achieved": hbm, "peak": 8.0, "unit": "TB/s", "frac": hbm / 8.0,
-                       "dispatch_ms": per_ms}
+    roof["dispatch_ms"] = st[dom] / layout["chunks"]
+    if roof["traffic"]:  # measured bytes (rocprofv3 counters) at the live dispatch time
+        hbm = roof["traffic"] / (roof["dispatch_ms"] * 1e-3) / 1e12
+        roof["traffic_rate"] = {"achieved": hbm, "peak": 8.0, "unit": "TB/s", "frac": hbm / 8.0,
+                                "traffic_over_algorithmic": roof["traffic"] / roof["algorithmic_bytes_per_launch"]}
     # the warm-cache build's dominant contraction, for comparison with the
     # kernel work of the previous measurements
     domw = "density" if st1w["density"] >= st1w["potential"] else "potential"

@@ -183,6 +183,47 @@ def test_spdfg_basis_matches_oracle(functional, mask):
               q_tol=1e-9 + 5e-7)
 
 
+def test_basis_atom_ranges_at_their_limits():
+    """The basis stage stages consecutive atoms together, up to 8 atoms,
+    32 shells, 256 primitives and 1024 contraction coefficients per range.
+    Atoms built to close ranges on each limit, several exactly at it: N
+    (4 shells of 64 primitives, 256 primitives: a range of its own), three C
+    (128 primitives and 512 coefficients each: two fill the coefficient
+    limit), two O of 16 shells each (the per-atom maximum; with the third C
+    and the first O a range reaches 18 shells, the second O would pass 32),
+    then nine H (the 8-atom limit).  FP64 against the oracle at its bar,
+    FP32 and split-BF16 full builds within theirs."""
+    rng = np.random.default_rng(17)
+
+    def shell(a, l, ne, nu, nc):
+        ex = np.sort(np.exp(rng.uniform(np.log(0.1), np.log(500.0), ne)))
+        return basis.Shell(a, l, ex, nu, rng.normal(size=(ne, nc)) if nc else np.zeros((ne, 0)))
+
+    els = ["N", "C", "C", "C", "O", "O"] + ["H"] * 9
+    sites = np.array([(i, j, k) for i in range(3) for j in range(3) for k in range(2)], dtype=float)
+    xyz = 2.6 * sites[:len(els)] + rng.uniform(-0.2, 0.2, size=(len(els), 3))
+    b = basis.Basis(xyz)
+    for a, el in enumerate(els):
+        if el == "N":
+            b.shells += [shell(a, l, 64, 1, 3) for l in (0, 1, 0, 1)]
+        elif el == "C":
+            b.shells += [shell(a, l, 64, 1, 4) for l in (0, 1)]
+        elif el == "O":
+            b.shells += [shell(a, s % 3, 1, 1, 0) for s in range(16)]
+        else:
+            b.shells += [shell(a, 0, 3, 1, 1), shell(a, 1, 1, 1, 0)]
+    g = grid.molecular_grid(els, xyz, nrad=30, nang=194)
+    C, occ = systems.random_orbitals(b.nbas, 12, seed=18)
+    ref = oracle.xc_build(b, g, oracle.MASK_PBE, C=C, occ=occ, nthreads=NT)
+    vmax = np.max(np.abs(ref[0]))
+    with XCBuilder(b, g, chunk_points=16384) as xc:
+        xc.set_orbitals(C, occ)
+        check(xc.build(PBE, FP64), ref)
+        for prec, rel in ((FP32, 2e-6), (BF16X3, 5e-5)):
+            check(xc.build(PBE, prec), ref, e_tol=rel * abs(ref[1]), v_tol=10 * rel * vmax,
+                  q_tol=rel * abs(ref[2]))
+
+
 def test_becke_gpu_matches_numpy():
     els, xyz = systems.water_cluster(2, seed=1)
     g_np = grid.molecular_grid(els, xyz, nrad=30, nang=302, partition="numpy")
