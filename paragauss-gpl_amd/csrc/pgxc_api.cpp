@@ -147,6 +147,11 @@ struct pgxc_ctx {
   uint16_t *tl = nullptr, *su = nullptr, *tfl = nullptr;
   uint32_t *tfb = nullptr;  // per tile, its functions as a bitmap of nwf words
   int nwf = 0;
+  // per entry of a split's function list: bit t set when the function is
+  // in the list of the split's tile t (every bit when a split has more than
+  // 32 tiles)
+  std::vector<uint32_t> h_sumask;
+  uint32_t *sumask = nullptr;
   // nuclear gradient work space (first gradient on): Hessian planes and
   // products of one chunk, per-(tile, function) sums
   double *ghess = nullptr, *gXY = nullptr, *gpart = nullptr;
@@ -492,6 +497,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.nwf = c->nwf;
       U.su_off = c->su_off + (long)k * c->nsplit;
       U.su = c->su;
+      U.sumask = c->sumask;
       U.gcf_off = c->gcf_off + (long)k * c->nsplit;
       U.gcf = c->gcf;
       U.sig8 = c->sig8 + (p0 / kGroup) * c->nw8;
@@ -1178,6 +1184,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   std::vector<uint32_t> wneed((size_t)ngr * nw8, 0u);
   c->h_su_off.assign(1, 0);
   c->h_su.clear();
+  c->h_sumask.clear();
   c->h_gcf_off.clear();
   c->h_gcf.clear();
   {
@@ -1196,6 +1203,18 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
       for (int f = 0; f < c->NP; ++f)
         if ((fm[(size_t)f / 32] >> (f % 32)) & 1u) c->h_su.push_back((uint16_t)f);
       c->h_su_off.push_back((int)c->h_su.size());
+      // per entry: the split's tiles whose list holds the function
+      const long st0 = g0 / 4, nst = G / 4;
+      for (size_t e = u0; e < c->h_su.size(); ++e) {
+        const int f = c->h_su[e];
+        uint32_t m = 0;
+        if (nst > 32)
+          m = ~0u;
+        else
+          for (long t = 0; t < nst; ++t)
+            m |= ((tfb[(size_t)(st0 + t) * c->nwf + (size_t)f / 32] >> (f % 32)) & 1u) << t;
+        c->h_sumask.push_back(m);
+      }
       const int nu = (int)(c->h_su.size() - u0), nch = (nu + tf - 1) / tf;
       c->h_gcf_off.push_back((long)c->h_gcf.size());
       const size_t gf0 = c->h_gcf.size();
@@ -1301,6 +1320,32 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
       fprintf(stderr, "pgxc: %ld groups x %d runs; blocks live %ld, values read %ld, gradients read %ld; "
                       "basis items %zu (of %ld)\n",
               ngr, n8, live, nv, ng, bi.size(), (long)c->nchunks * (c->Mc / kThreads) * c->n_ranges);
+      // elements (function x point): basis values and gradients written,
+      // W written (runs live or read by the potential) and of those the
+      // ones in the tile's list
+      double ev = 0, eg = 0, ew = 0, ewl = 0;
+      for (size_t i = 0; i < bi.size(); ++i) {
+        const RangeD &R = c->h_ranges[bi[i] & 0xffffu];
+        const uint32_t *fw = bfl.data() + 16 * i;
+        for (int j = 0; j < R.n_shells; ++j) {
+          const ShellD &d = sh[(size_t)(R.shell_begin + j)];
+          const int nf = (2 * d.l + 1) * (d.n_unc + d.n_contr);
+          for (int gg = 0; gg < 8; ++gg) {
+            const int bit = 8 * j + gg;
+            ev += ((fw[bit >> 5] >> (bit & 31)) & 1u) * nf * 32.0;
+            eg += ((fw[8 + (bit >> 5)] >> (bit & 31)) & 1u) * nf * 32.0;
+          }
+        }
+      }
+      for (long g = 0; g < ngr; ++g)
+        for (int r = 0; r < n8; ++r)
+          if (((sig[(size_t)g * nw8 + r / 32] | wneed[(size_t)g * nw8 + r / 32]) >> (r % 32)) & 1u) {
+            ew += kRun * 32.0;
+            for (int f = r * kRun; f < std::min(c->NP, (r + 1) * kRun); ++f)
+              ewl += ((tfb[(size_t)(g / 4) * c->nwf + f / 32] >> (f % 32)) & 1u) * 32.0;
+          }
+      fprintf(stderr, "pgxc: elements per build: basis values %.4g, gradients %.4g; W %.4g, of them in "
+                      "the tiles' lists %.4g\n", ev, eg, ew, ewl);
     }
   }
   // work items per chunk
@@ -1333,6 +1378,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   if ((rc = upload(c, &c->tfl, c->h_tfl))) return rc;
   if ((rc = upload(c, &c->su_off, c->h_su_off))) return rc;
   if ((rc = upload(c, &c->su, c->h_su))) return rc;
+  if ((rc = upload(c, &c->sumask, c->h_sumask))) return rc;
   if ((rc = upload(c, &c->gcf_off, c->h_gcf_off))) return rc;
   if ((rc = upload(c, &c->gcf, c->h_gcf))) return rc;
   if ((rc = upload(c, &c->di, c->h_di))) return rc;
@@ -2124,7 +2170,7 @@ int pgxc_destroy(pgxc_ctx *c) {
   void *bufs[] = {c->grid, c->atoms, c->shells, c->ranges, c->ratoms, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
-                  c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->tf_off, c->tfl, c->tfb, c->su_off, c->su,
+                  c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->tf_off, c->tfl, c->tfb, c->su_off, c->su, c->sumask,
                   c->gcf_off, c->gcf, c->di, c->pi, c->bi, c->bfl, c->ghess, c->gXY, c->gpart, c->beta.P, c->beta.Pref, c->beta.D,
                   c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
                   c->beta.Vstate, c->beta.Vout, c->eterm};

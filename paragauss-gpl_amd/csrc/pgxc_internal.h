@@ -93,6 +93,9 @@ struct UnitD {
   const int *tl_off, *su_off, *tf_off;
   const uint16_t *tl, *su, *tfl;
   const uint32_t *tfb;  // the tiles' functions as bitmaps, nwf words per tile
+  // per entry of su: bit t set when its function is in the list of the
+  // split's tile t, so 0 there (every bit when a split has more than 32 tiles)
+  const uint32_t *sumask;
   int nwf, pad_f;
   const long *gcf_off;
   const uint8_t *gcf;

@@ -58,6 +58,8 @@ struct PSOperand {
   static_assert(NT % 4 == 0, "k run = threadIdx.x & 3");
   const __bf16 *row[PER];  // plane + row offset of vector e, nullptr: row absent
   const __bf16 *dummy;
+  unsigned tm[PER];  // vector e's row: bit t set when it can be nonzero on tile t
+  long kbase;        // k offset of tile 0
   u32x4 v[2][PER];
   unsigned ok[2];
 
@@ -67,16 +69,24 @@ struct PSOperand {
     r = rem >> 2;
     c = rem & 3;
   }
-  // rowf(r): element offset of row r (k = 0), or -1
+  // rowf(r): element offset of row r (k = 0), or -1; maskf(r): the 128-k
+  // tiles from k = kb on where row r can be nonzero (bit t: tile t, at most
+  // 32 tiles); a row's steps on other tiles stage zeros without reading it
   template <class RowF>
   __device__ __forceinline__ void init(const Src &S, const RowF &rowf) {
+    init(S, rowf, [](int) { return ~0u; }, 0);
+  }
+  template <class RowF, class MaskF>
+  __device__ __forceinline__ void init(const Src &S, const RowF &rowf, const MaskF &maskf, long kb) {
     dummy = S.hi;
+    kbase = kb;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       int plane, r, c;
       place(threadIdx.x + NT * e, plane, r, c);
       const long o = rowf(r);
       row[e] = o >= 0 ? (plane ? S.lo : S.hi) + o : nullptr;
+      tm[e] = maskf(r);
     }
   }
   // kofs(kt, c): element offset of k run c of step kt, or -1; steps past
@@ -84,10 +94,11 @@ struct PSOperand {
   template <int SL, class KOfs>
   __device__ __forceinline__ void load(const KOfs &kofs, int kt, int ksteps) {
     const long ko = kt < ksteps ? kofs(kt, (int)(threadIdx.x & 3)) : -1;
+    const int tb = (int)((ko - kbase) >> 7) & 31;
     unsigned m = 0;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const bool g = row[e] != nullptr && ko >= 0;
+      const bool g = row[e] != nullptr && ko >= 0 && ((tm[e] >> tb) & 1u);
       v[SL][e] = *(gvec)(g ? row[e] + ko : dummy);
       m |= (unsigned)g << e;
     }
@@ -119,16 +130,26 @@ struct PFOperand {
   static_assert(NT % 8 == 0, "k quad = threadIdx.x & 7");
   const float *row[PER];  // plane + row offset of quad e, nullptr: row absent
   const float *dummy;
+  unsigned tm[PER];  // as PSOperand
+  long kbase;
   f4 v[2][PER];
   unsigned ok[2];
 
   template <class RowF>
   __device__ __forceinline__ void init(const float *plane, const RowF &rowf) {
+    init(plane, rowf, [](int) { return ~0u; }, 0);
+  }
+  template <class RowF, class MaskF>
+  __device__ __forceinline__ void init(const float *plane, const RowF &rowf, const MaskF &maskf,
+                                       long kb) {
     dummy = plane;
+    kbase = kb;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const long o = rowf((int)((threadIdx.x + NT * e) >> 3));
+      const int r = (int)((threadIdx.x + NT * e) >> 3);
+      const long o = rowf(r);
       row[e] = o >= 0 ? plane + o : nullptr;
+      tm[e] = maskf(r);
     }
   }
   template <int SL, class KOfs>
@@ -136,10 +157,11 @@ struct PFOperand {
     const int q = (int)(threadIdx.x & 7);
     const long kr = kt < ksteps ? kofs(kt, q >> 1) : -1;
     const long ko = kr >= 0 ? kr + 4 * (q & 1) : -1;
+    const int tb = (int)((ko - kbase) >> 7) & 31;
     unsigned m = 0;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const bool g = row[e] != nullptr && ko >= 0;
+      const bool g = row[e] != nullptr && ko >= 0 && ((tm[e] >> tb) & 1u);
       v[SL][e] = gld((const f4 *)(g ? row[e] + ko : dummy));
       m |= (unsigned)g << e;
     }
@@ -430,8 +452,13 @@ __global__ __launch_bounds__(kThreads) void k_potential_bf16x3(const StageList S
     // Phi[f_j(n)][kb + p(k)] (FP32 values, split while staged)
     PSOperand<BT> opa;
     PFOperand<BT> opb;
-    opa.init({wh, wh + pn}, [&](int m) -> long { return m < nri ? (long)ri[m] * Mc : -1; });
-    opb.init((const float *)U.phi, [&](int n) -> long { return n < nrj ? (long)rj[n] * Mc : -1; });
+    // rows and columns skip the tiles whose list lacks their function (W and
+    // Phi are 0 there)
+    const uint32_t *tmi = U.sumask + so + ci * BT, *tmj = U.sumask + so + cj * BT;
+    opa.init({wh, wh + pn}, [&](int m) -> long { return m < nri ? (long)ri[m] * Mc : -1; },
+             [&](int m) -> unsigned { return m < nri ? gld(tmi + m) : 0u; }, kb);
+    opb.init((const float *)U.phi, [&](int n) -> long { return n < nrj ? (long)rj[n] * Mc : -1; },
+             [&](int n) -> unsigned { return n < nrj ? gld(tmj + n) : 0u; }, kb);
     auto kofs = [&](int kt, int c) -> long { return kb + (long)gl[kt] * kGroup + 8 * c; };
     f4 acc[BT / 32][BT / 32];
     gemm_tile_bf16x3<BT, BT, kThreads, 2>(opa, opb, ng, kofs, lds, acc);

@@ -1211,17 +1211,24 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
     };
     const uint32_t *sg = U.sig8 + (p / kGroup) * U.nw8, *wn = U.wbits + (p / kGroup) * U.nw8;
     const uint32_t *fb = U.tfb + (p / kScreenTile) * U.nwf;
+    // split-BF16 W is read only by the split-BF16 potential, which skips the
+    // functions outside a tile's list (when a split spans at most 32 tiles):
+    // no zeros are written for them
+    const bool skip = SPLIT && Mc / U.nsplit <= 32 * kScreenTile;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {  // the item's two runs of 8 functions
       const long r = f0 / kRun + h, fr = r * kRun;
       const uint32_t bit = 1u << (r & 31);
+      const uint32_t fm = gld(fb + (fr >> 5)) >> (fr & 31);  // the run's functions in the tile's list
+      const uint32_t wm = skip ? fm : 0xffu;                 // the run's functions written
       if (!(gld(sg + (r >> 5)) & bit)) {
         if (gld(wn + (r >> 5)) & bit) {
           const double z[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
           for (int k = 0; k < kRun; ++k)
+            if ((wm >> k) & 1u)
 #pragma unroll
-            for (int sp = 0; sp < NS; ++sp) put(sp, (fr + k) * Mc + p, z);
+              for (int sp = 0; sp < NS; ++sp) put(sp, (fr + k) * Mc + p, z);
         }
         continue;
       }
@@ -1237,9 +1244,9 @@ __global__ __launch_bounds__(kThreads) void k_wmat(const UnitD *__restrict__ uni
             b2[sp][j] = gld(cf[sp] + 3 * Mc + p + j);
           }
         }
-      const uint32_t fm = gld(fb + (fr >> 5)) >> (fr & 31);  // the run's functions in the tile's list
 #pragma unroll 4
       for (int k = 0; k < kRun; ++k) {
+        if (!((wm >> k) & 1u)) continue;
         const long q = (fr + k) * Mc + p;
         t4 v0 = {}, v1 = {}, v2 = {}, v3 = {};
         if ((fm >> k) & 1u) {
