@@ -140,11 +140,13 @@ struct pgxc_ctx {
   // [group][chunk] flags, per chunk density and potential items
   std::vector<int> h_tl_off, h_su_off, h_tf_off;
   std::vector<uint16_t> h_tl, h_su, h_tfl;
+  std::vector<uint8_t> h_tlm;
   std::vector<long> h_gcf_off, h_di_off, h_pi_off;
   std::vector<uint8_t> h_gcf;
   std::vector<uint32_t> h_di, h_pi;
   int *tl_off = nullptr, *su_off = nullptr, *tf_off = nullptr;
   uint16_t *tl = nullptr, *su = nullptr, *tfl = nullptr;
+  uint8_t *tlm = nullptr;
   uint32_t *tfb = nullptr;  // per tile, its functions as a bitmap of nwf words
   int nwf = 0;
   // per entry of a split's function list: bit t set when the function is
@@ -494,6 +496,7 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       U.tf_off = c->tf_off + p0 / kScreenTile;
       U.tfl = c->tfl;
       U.tfb = c->tfb + (p0 / kScreenTile) * c->nwf;
+      U.tlm = c->tlm;
       U.nwf = c->nwf;
       U.su_off = c->su_off + (long)k * c->nsplit;
       U.su = c->su;
@@ -1168,13 +1171,17 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   }
   if ((rc = upload(c, &c->tfb, tfb))) return rc;
   auto bit = [&](long g, int r) { return (sig[(size_t)g * nw8 + r / 32] >> (r % 32)) & 1u; };
-  // per 128-point tile: its runs (union over its four groups)
+  // per 128-point tile: its runs (union over its four groups), and per run
+  // its functions in the tile's list
   c->h_tl_off.assign(1, 0);
   c->h_tl.clear();
+  c->h_tlm.clear();
   for (long t = 0; t < ntp; ++t) {
     for (int r = 0; r < n8; ++r)
-      if (bit(4 * t, r) | bit(4 * t + 1, r) | bit(4 * t + 2, r) | bit(4 * t + 3, r))
+      if (bit(4 * t, r) | bit(4 * t + 1, r) | bit(4 * t + 2, r) | bit(4 * t + 3, r)) {
         c->h_tl.push_back((uint16_t)r);
+        c->h_tlm.push_back((uint8_t)(tfb[(size_t)t * c->nwf + (size_t)r / 4] >> (8 * (r % 4))));
+      }
     c->h_tl_off.push_back((int)c->h_tl.size());
   }
   // per split: the union of its tiles' functions (the potential's rows and
@@ -1268,6 +1275,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
       }
     }
     std::vector<uint32_t> bi, bfl, bm((size_t)nw8);
+    const bool masked = G / 4 <= 32;
     c->h_bi_off.assign(1, 0);
     const long gpb = kThreads / kGroup;  // groups per basis block
     for (int k = 0; k < c->nchunks; ++k) {
@@ -1280,7 +1288,6 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
           uint32_t any = 0;
           for (int w = 0; w < nw8 && !any; ++w) any = bm[(size_t)w] & rmask[(size_t)ri * nw8 + w];
           if (!any) continue;
-          bi.push_back((uint32_t)(b << 16 | ri));
           // per (shell j of the range, group g of the block): values, gradients read
           uint32_t fw[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
           const RangeD &R = c->h_ranges[(size_t)ri];
@@ -1298,10 +1305,16 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
               const long tg = (g0 + b * gpb + gg) / 4;
               const uint32_t gr = smark[(size_t)tg * nws + sg / 32] >> (sg % 32);
               const int bit = 8 * j + (int)gg;
-              fw[bit >> 5] |= (v & 1u) << (bit & 31);
+              // values: with the contractions skipping the functions outside
+              // a tile's list (always in the density; in the potentials when a
+              // split spans at most 32 tiles), only those of the shells marked
+              // on the tile
+              fw[bit >> 5] |= ((masked ? gr : v) & 1u) << (bit & 31);
               fw[8 + (bit >> 5)] |= (gr & 1u) << (bit & 31);
             }
           }
+          if (!(fw[0] | fw[1] | fw[2] | fw[3] | fw[4] | fw[5] | fw[6] | fw[7])) continue;
+          bi.push_back((uint32_t)(b << 16 | ri));
           bfl.insert(bfl.end(), fw, fw + 16);
         }
       }
@@ -1344,8 +1357,22 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
             for (int f = r * kRun; f < std::min(c->NP, (r + 1) * kRun); ++f)
               ewl += ((tfb[(size_t)(g / 4) * c->nwf + f / 32] >> (f % 32)) & 1u) * 32.0;
           }
-      fprintf(stderr, "pgxc: elements per build: basis values %.4g, gradients %.4g; W %.4g, of them in "
-                      "the tiles' lists %.4g\n", ev, eg, ew, ewl);
+      // values if only the tiles' runs were read (no potential chunks)
+      double evt = 0;
+      for (long g = 0; g < ngr; ++g) {
+        const long t = g / 4;
+        std::vector<uint32_t> tw2((size_t)nw8, 0u);
+        for (int i = c->h_tl_off[(size_t)t]; i < c->h_tl_off[(size_t)t + 1]; ++i)
+          tw2[c->h_tl[(size_t)i] >> 5] |= 1u << (c->h_tl[(size_t)i] & 31);
+        for (const ShellD &d : sh) {
+          const int nf = (2 * d.l + 1) * (d.n_unc + d.n_contr);
+          uint32_t v = 0;
+          for (int r = d.func_off / kRun; r <= (d.func_off + nf - 1) / kRun; ++r) v |= tw2[r >> 5] >> (r & 31);
+          evt += (v & 1u) * nf * 32.0;
+        }
+      }
+      fprintf(stderr, "pgxc: elements per build: basis values %.4g (tile runs only %.4g), gradients %.4g; "
+                      "W %.4g, of them in the tiles' lists %.4g\n", ev, evt, eg, ew, ewl);
     }
   }
   // work items per chunk
@@ -1374,6 +1401,7 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
   }
   if ((rc = upload(c, &c->tl_off, c->h_tl_off))) return rc;
   if ((rc = upload(c, &c->tl, c->h_tl))) return rc;
+  if ((rc = upload(c, &c->tlm, c->h_tlm))) return rc;
   if ((rc = upload(c, &c->tf_off, c->h_tf_off))) return rc;
   if ((rc = upload(c, &c->tfl, c->h_tfl))) return rc;
   if ((rc = upload(c, &c->su_off, c->h_su_off))) return rc;
@@ -2170,7 +2198,7 @@ int pgxc_destroy(pgxc_ctx *c) {
   void *bufs[] = {c->grid, c->atoms, c->shells, c->ranges, c->ratoms, c->exps, c->norms, c->contr, c->P, c->Pref,
                   c->D, c->phi, c->W, c->part, c->coef, c->Hpart, c->state,
                   c->Vstate, c->Vout, c->epart, c->nbt, c->sig8, c->sshells, c->d_fpos,
-                  c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->tf_off, c->tfl, c->tfb, c->su_off, c->su, c->sumask,
+                  c->phic[0], c->phic[1], c->wbits, c->tl_off, c->tl, c->tf_off, c->tfl, c->tfb, c->tlm, c->su_off, c->su, c->sumask,
                   c->gcf_off, c->gcf, c->di, c->pi, c->bi, c->bfl, c->ghess, c->gXY, c->gpart, c->beta.P, c->beta.Pref, c->beta.D,
                   c->beta.W, c->beta.part, c->beta.coef, c->beta.Hpart, c->beta.state,
                   c->beta.Vstate, c->beta.Vout, c->eterm};

@@ -93,6 +93,10 @@ struct UnitD {
   const int *tl_off, *su_off, *tf_off;
   const uint16_t *tl, *su, *tfl;
   const uint32_t *tfb;  // the tiles' functions as bitmaps, nwf words per tile
+  // per entry of tl: its run's functions in the tile's list (bit j: function
+  // 8 r + j); the density contraction stages zeros for the others, which the
+  // basis stage does not write
+  const uint8_t *tlm;
   // per entry of su: bit t set when its function is in the list of the
   // split's tile t, so 0 there (every bit when a split has more than 32 tiles)
   const uint32_t *sumask;

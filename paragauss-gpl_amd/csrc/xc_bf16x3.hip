@@ -204,16 +204,20 @@ struct PTOperand {
   static constexpr int NR = R * (kXBK / 8);  // (row, k run) pairs per stage
   static constexpr int PER = NR / NT;
   static_assert(PER * NT == NR, "whole runs per thread");
+  static_assert(PER <= 4, "8 function flags per pair in one word");
   const float *row[PER];  // plane + row offset of pair e, nullptr: row absent
   const float *dummy;
+  const uint8_t *kmk;     // per k run of the list: its k read (bit j: k 8 c + j); nullptr: all
   long ld;
   float v[2][PER][8];
-  unsigned ok[2];
+  unsigned ok[2];         // 8 bits per pair: the k loaded
 
   template <class RowF>
-  __device__ __forceinline__ void init(const float *plane, long ld_, const RowF &rowf) {
+  __device__ __forceinline__ void init(const float *plane, long ld_, const RowF &rowf,
+                                       const uint8_t *kmask = nullptr) {
     dummy = plane;
     ld = ld_;
+    kmk = kmask;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       const long o = rowf((int)((threadIdx.x + NT * e) % R));
@@ -228,10 +232,11 @@ struct PTOperand {
       const int c = (int)((threadIdx.x + NT * e) / R);
       const long k0 = kt < ksteps ? kofs(kt, c) : -1;
       const bool g = row[e] != nullptr && k0 >= 0;
+      const unsigned jm = g ? (kmk ? (unsigned)kmk[kt * (kXBK / 8) + c] : 0xffu) : 0u;
       const float *p = g ? row[e] + k0 * ld : dummy;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[SL][e][j] = gld(p + j * ld);
-      m |= (unsigned)g << e;
+      for (int j = 0; j < 8; ++j) v[SL][e][j] = gld(((jm >> j) & 1u) ? p + j * ld : dummy);
+      m |= jm << (8 * e);
     }
     ok[SL] = m;
   }
@@ -241,12 +246,12 @@ struct PTOperand {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       const int r = (int)((threadIdx.x + NT * e) % R), c = (int)((threadIdx.x + NT * e) / R);
-      const bool g = (ok[SL] >> e) & 1u;
+      const unsigned jm = ok[SL] >> (8 * e);
       u8v h, l;
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         __bf16 hh, ll;
-        split_bf16(g ? v[SL][e][j] : 0.0f, hh, ll);
+        split_bf16(((jm >> j) & 1u) ? v[SL][e][j] : 0.0f, hh, ll);
         h[j] = __builtin_bit_cast(uint16_t, hh);
         l[j] = __builtin_bit_cast(uint16_t, ll);
       }
@@ -362,6 +367,7 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(2)
   static_assert(sizeof(double) * WGN * 4 * BM <= sizeof(__bf16) * 2 * XStage<BM, BN2>::n, "alias");
   double(&red)[WGN][4][BM] = *reinterpret_cast<double(*)[WGN][4][BM]>(lds);
   __shared__ uint16_t kl[kMaxRuns];
+  __shared__ uint8_t km[kMaxRuns];  // per run of kl: its functions in the tile's list
   __shared__ uint16_t cl[BN2];
   __shared__ long slot;
   static_assert(kXBK == 4 * kRun, "four runs per K step");
@@ -378,14 +384,16 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(2)
     const int ncol = min(BN2, U.tf_off[tile + 1] - foff), nslots = ncol > BN ? 2 : 1;
     __syncthreads();  // the previous item's readers of kl and cl are done
     for (int i = threadIdx.x; i < ncol; i += NT) cl[i] = gld(U.tfl + foff + i);
+    for (int i = threadIdx.x; i < c8; i += NT) km[i] = gld(U.tlm + off + i);
     lds_list(U.tl + off, c8, kl);
-    // A(p, k) = Phi[f(k)][m0 + p] (FP32 values, split while staged),
-    // B(k, n) = D^T[col(n)][f(k)]
+    // A(p, k) = Phi[f(k)][m0 + p] (FP32 values, split while staged; 0 for the
+    // functions of the runs outside the tile's list, which the basis stage
+    // leaves unwritten), B(k, n) = D^T[col(n)][f(k)]
     const __bf16 *dh = (const __bf16 *)U.D;
     const long np2 = NP * NP;
     PTOperand<BM, NT> opa;
     PSOperand<BN2, NT> opb;
-    opa.init((const float *)U.phi, U.Mc, [&](int m) -> long { return m0 + m; });
+    opa.init((const float *)U.phi, U.Mc, [&](int m) -> long { return m0 + m; }, km);
     opb.init({dh, dh + np2}, [&](int n) -> long { return n < ncol ? (long)cl[n] * NP : -1; });
     auto kofs = [&](int kt, int c) -> long { return run_fn(kl, c8, kt * kXBK + 8 * c); };
     f4 acc[BM / 2 / 16][BN2 / WGN / 16];

@@ -689,6 +689,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 || VAR != 2 ? 2 : 1))) void k_
   __shared__ __attribute__((aligned(16))) T Bs[lds_elems<T, BN, BK, ST, true>()];
   __shared__ double red[2][4][BM];
   __shared__ uint16_t kl[kMaxRuns];
+  __shared__ uint8_t km[kMaxRuns];  // per run of kl: its functions in the tile's list
   __shared__ uint16_t cl[2 * BN];
   __shared__ long slot;
   for (long L = walk_next(S, -1, &slot); L < S.total;) {
@@ -703,6 +704,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 || VAR != 2 ? 2 : 1))) void k_
     const int foff = U.tf_off[tile] + 2 * pair * BN, nf = U.tf_off[tile + 1] - foff;
     __syncthreads();  // the previous item's readers of kl and cl are done
     for (int i = threadIdx.x; i < min(2 * BN, nf); i += kThreads) cl[i] = gld(U.tfl + foff + i);
+    for (int i = threadIdx.x; i < c8; i += kThreads) km[i] = gld(U.tlm + off + i);
     lds_list(U.tl + off, c8, kl);
     const int ksteps = (c8 * kRun + BK - 1) / BK;
     const T *phi = (const T *)U.phi, *D = (const T *)U.D;
@@ -713,10 +715,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 || VAR != 2 ? 2 : 1))) void k_
       const int ncol = min(BN, nf - h * BN);
       // A(m, k) = Phi[f(k)][m0 + m], B(k, n) = D[f(k)][col(n)] = D[col(n)][f(k)]
       // (D symmetric, so B is k-contiguous along runs): K runs over the
-      // tile's gathered runs, the columns over its functions
+      // tile's gathered runs, the columns over its functions; A is 0 for the
+      // functions of the runs outside the tile's list (left unwritten by the
+      // basis stage)
       auto aaddr = [&](int m, int kt, int kk) -> long {
-        const long f = run_fn(kl, c8, kt * BK + kk);
-        return f < 0 ? -1 : f * Mc + m;
+        const int k = kt * BK + kk;
+        const long f = run_fn(kl, c8, k);
+        return (f < 0 || !((km[k >> 3] >> (k & 7)) & 1u)) ? -1 : f * Mc + m;
       };
       auto baddr = [&](int n, int kt, int kk) -> long {
         const long f = run_fn(kl, c8, kt * BK + kk);
@@ -1283,6 +1288,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(VAR ==
   __shared__ __attribute__((aligned(16))) T As[lds_elems<T, BT, BK, ST, true>()];
   __shared__ __attribute__((aligned(16))) T Bs[lds_elems<T, BT, BK, ST, true>()];
   __shared__ uint16_t ul[2 * BT];  // the functions of the row chunk, then of the column chunk
+  __shared__ uint32_t cm[BT];      // per column: the split's tiles whose list holds its function
   __shared__ uint16_t gl[kMaxSplitGroups];
   __shared__ int wtot[4];
   __shared__ long slot;
@@ -1301,17 +1307,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(VAR ==
     __syncthreads();  // the previous item's readers of ul / gl are done
     for (int i = threadIdx.x; i < 2 * BT; i += kThreads)
       if (i < BT ? i < nri : i - BT < nrj) ul[i] = gld(U.su + so + (i < BT ? ci * BT + i : cj * BT + i - BT));
+    for (int i = threadIdx.x; i < nrj; i += kThreads) cm[i] = gld(U.sumask + so + cj * BT + i);
     __syncthreads();
     const int ng = potential_groups(U.gcf + U.gcf_off[sl], G, nch, ci, cj, gl, wtot);
     const uint16_t *ri = ul, *rj = ul + BT;
     // A(m, k) = W[f_i(m)][kb + p(k)], B(k, n) = Phi[f_j(n)][kb + p(k)]: rows from
-    // the split's runs, K over the groups where both chunks can be nonzero
+    // the split's functions, K over the groups where both chunks can be
+    // nonzero; B is 0 on the tiles whose list lacks the column's function
+    // (left unwritten by the basis stage)
     auto kpt = [&](int kt, int kk) -> long { return (long)gl[kt / R] * kGroup + (kt % R) * BK + kk; };
     auto aaddr = [&](int m, int kt, int kk) -> long {
       return m < nri ? (long)ri[min(m, nri - 1)] * Mc + kb + kpt(kt, kk) : -1;
     };
     auto baddr = [&](int n, int kt, int kk) -> long {
-      return n < nrj ? (long)rj[min(n, nrj - 1)] * Mc + kb + kpt(kt, kk) : -1;
+      const long k = kpt(kt, kk);
+      return (n < nrj && ((cm[min(n, nrj - 1)] >> ((int)(k >> 7) & 31)) & 1u))
+                 ? (long)rj[min(n, nrj - 1)] * Mc + kb + k : -1;
     };
     const T *W = (const T *)U.W, *phi = (const T *)U.phi;
     typename MF<T>::acc acc[TT][TT];

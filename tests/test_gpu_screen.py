@@ -254,6 +254,33 @@ def odd_molecule():
     return b, g, C, occ
 
 
+def test_unwritten_basis_blocks_with_wide_point_splits(monkeypatch):
+    """A chunk above 256 Ki points makes a point split of the potential span
+    more than 32 tiles, past what its per-function tile masks hold: the
+    potentials then read every function of their chunks, and the basis and W
+    passes write the zeros of the functions outside a tile's list again.
+    Buffers poisoned with NaN bytes, all three precisions against the oracle."""
+    monkeypatch.setenv("PGXC_POISON_BASIS", "1")
+    els, xyz = systems.water_cluster(5, seed=3)  # 306 530 points: 48-tile splits in one chunk
+    b = basis.seeded_basis(els, xyz, seed=4, shell_set="medium")
+    g = grid.molecular_grid(els, xyz, nrad=75, nang=302)
+    C, occ = systems.random_orbitals(b.nbas, 25, seed=5)
+    hr, er, qr = oracle.xc_build(b, g, oracle.MASK_PBE, C=C, occ=occ, nthreads=NT)
+    with XCBuilder(b, g, chunk_points=524288) as xc:
+        lay = xc.layout()
+        assert lay["chunks"] == 1 and lay["chunk_points"] // lay["nsplit"] > 32 * 128
+        xc.set_orbitals(C, occ)
+        for prec, rel in ((FP64, 0.0), (FP32, 2e-6), (BF16X3, 5e-5)):
+            ham, e, q = xc.build(PBE, prec)
+            dv = np.max(np.abs(ham - hr))
+            print("split %d points prec=%d dE=%.2e dV=%.2e"
+                  % (lay["chunk_points"] // lay["nsplit"], prec, e - er, dv), flush=True)
+            assert np.all(np.isfinite(ham)) and np.isfinite(e) and np.isfinite(q)
+            assert abs(e - er) <= 1e-9 + rel * abs(er)
+            assert dv <= 1e-10 + 10 * rel * np.max(np.abs(hr))
+            assert abs(q - qr) <= 1e-9 + rel * abs(qr)
+
+
 @pytest.mark.parametrize("system", ["dimer", "odd"])
 @pytest.mark.parametrize("cache", [False, True])
 def test_unwritten_basis_blocks_are_never_read(dimer, system, cache, monkeypatch):
