@@ -162,8 +162,8 @@ struct pgxc_ctx {
   uint8_t *gcf = nullptr;
   uint32_t *di = nullptr, *pi = nullptr;
   // blocks of the basis values some stage reads (see setup_screen): per
-  // chunk the basis items (256-point block << 16 | atom) that touch them and
-  // their write flags
+  // chunk the basis items (256-point block << 16 | atom range) that touch
+  // them and their write flags
   uint32_t *bi = nullptr, *bfl = nullptr;
   std::vector<long> h_bi_off;
   std::vector<int> fpos;          // internal position of each caller function
@@ -1244,12 +1244,14 @@ static int setup_screen(pgxc_ctx *c, const std::vector<ShellD> &sh, const double
     }
   }
   if ((rc = upload(c, &c->wbits, wneed))) return rc;
-  // Blocks of the basis values some stage reads, per 32-point group: values
-  // on the runs of its tile's list (density operand and epilogue) and on the
-  // runs of the chunks the potential contraction reads there (wneed);
-  // gradients on the tile's runs (density epilogue; the W pass reads the runs
-  // not cut off on the group, a subset).  The basis stage writes exactly
-  // these blocks; no stage reads any other block of a basis-value buffer.
+  // What the basis stage writes, per (shell, 32-point group): values and
+  // gradients of the shells marked on the group's tile, the functions of the
+  // tile's list (the density's operand and epilogue, the W pass and the
+  // potentials read no others: they stage zeros for the rest).  When a point
+  // split spans more than 32 tiles, the potentials' tile masks are off, and
+  // values are also written on the runs of the tile (the density's K) and of
+  // the chunks the potentials read there (wneed).  No stage reads any other
+  // block of a basis-value buffer.
   {
     std::vector<uint32_t> need((size_t)ngr * 2 * nw8, 0u), tw((size_t)nw8);
     for (long t = 0; t < ntp; ++t) {
