@@ -1033,12 +1033,14 @@ static void morton_order(const double *xyz, long n, long stride, long step,
 }
 
 // Internal order of the basis functions: shells by reach class (the radius
-// sqrt(50 / alpha_min) beyond which the shell is cut off; three classes split
-// at the tertiles, longest reach first), then by their atom's place on a
-// Morton curve, then in caller order.  Functions that can be nonzero on a
-// compact group of points then fill few 32-function blocks.  A shell keeps
-// its functions together and in caller order.  Sets each shell's internal
-// func_off and fpos[caller function] = internal position.
+// sqrt(50 / alpha_min) beyond which the shell is cut off, in steps of 10 % on
+// a log scale, longest reach first), then by their atom's place on a Morton
+// curve, then in caller order.  Shells of one element share their exponents,
+// so a class holds the like shells of neighbouring atoms: the functions that
+// can be nonzero on a compact group of points fill whole runs of 8 and few
+// 32-function blocks.  A shell keeps its functions together and in caller
+// order.  Sets each shell's internal func_off and fpos[caller function] =
+// internal position.
 static void function_order(std::vector<ShellD> &sh, const double *exps, const double *atom_xyz,
                            int n_atoms, std::vector<int> &fpos) {
   const size_t ns = sh.size();
@@ -1048,16 +1050,13 @@ static void function_order(std::vector<ShellD> &sh, const double *exps, const do
     for (int e = 1; e < sh[i].n_exp; ++e) amin = std::min(amin, exps[sh[i].exp_off + e]);
     reach[i] = std::sqrt(50.0 / amin);
   }
-  std::vector<double> sorted = reach;
-  std::sort(sorted.begin(), sorted.end());
-  const double t1 = sorted[ns / 3], t2 = sorted[(2 * ns) / 3];
   std::vector<long> aperm;
   morton_order(atom_xyz, n_atoms, 1, 3, aperm);
   std::vector<int> arank(n_atoms);
   for (int r = 0; r < n_atoms; ++r) arank[aperm[r]] = r;
   std::vector<int> idx(ns);
   std::iota(idx.begin(), idx.end(), 0);
-  auto cls = [&](int i) { return reach[i] >= t2 ? 0 : reach[i] >= t1 ? 1 : 2; };
+  auto cls = [&](int i) { return -(int)std::floor(std::log(reach[i]) / std::log(1.1)); };
   std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
     if (cls(a) != cls(b)) return cls(a) < cls(b);
     return arank[sh[a].atom] < arank[sh[b].atom];
