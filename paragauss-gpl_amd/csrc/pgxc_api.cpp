@@ -280,11 +280,12 @@ void make_order(const std::vector<long> &base, const std::vector<double> &prio,
   }
 }
 
-enum ItemKind { K_BASIS, K_FUNC, K_WMAT, K_DENS, K_POT };
+enum ItemKind { K_BASIS, K_FUNC, K_WMAT, K_DENS, K_POT, K_WBASIS };
 
 long unit_items(const UnitD &U, ItemKind k, int tf) {
   switch (k) {
-    case K_BASIS: return U.bscreen ? U.nbi : items_basis(U.Mc, U.n_ranges);
+    case K_BASIS:
+    case K_WBASIS: return U.bscreen ? U.nbi : items_basis(U.Mc, U.n_ranges);
     case K_FUNC: return items_functional(U.Mc);
     case K_WMAT: return items_wmat(U.Mc, U.NP);
     case K_DENS: return U.ndi;
@@ -559,11 +560,14 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       pick(0, [](const pgxc_ctx *c) { return c->ispin == 1; }, fu, fm);
       pend[k].push_back(stage(fu, fm, K_FUNC, 0, k));
     }
+    // W: split-BF16 builds evaluate the basis again over the basis items
+    const ItemKind wkind = precision == PGXC_PREC_BF16X3 ? K_WBASIS : K_WMAT;
     {  // closed-shell units; the open-shell ones in wmat2 below
       std::vector<UnitD> wu;
       std::vector<int> wm;
       pick(0, [](const pgxc_ctx *c) { return c->ispin == 1; }, wu, wm);
-      pend[k].push_back(stage(wu, wm, K_WMAT, 0, k));
+      pend[k].push_back(stage(wu, wm, wkind, 0, k));
+      for (int m : wm) pend[k].back().lmax = std::max(pend[k].back().lmax, mols[m]->lmax);
     }
     const int cls = precision == PGXC_PREC_FP64 ? 0 : 1;
     for (int kind = 0; kind < 2; ++kind)
@@ -585,7 +589,8 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       std::vector<int> fm;
       pick(0, [](const pgxc_ctx *c) { return c->ispin == 2; }, fu, fm);
       pend[k].push_back(stage(fu, fm, K_FUNC, 0, k));
-      pend[k].push_back(stage(fu, fm, K_WMAT, 0, k));  // both spins' W per alpha unit
+      pend[k].push_back(stage(fu, fm, wkind, 0, k));  // both spins' W per alpha unit
+      for (int m : fm) pend[k].back().lmax = std::max(pend[k].back().lmax, mols[m]->lmax);
     }
   }
   // queue bounds: each list's positions (natural order, so neighbouring items
@@ -795,8 +800,13 @@ static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent
     }
     launch_functional(t, gga, S.func);
     launch_functional_spin(t, gga, S.func2);
-    launch_wmat<T>(t, gga, x3, false, S.wmat);
-    launch_wmat<T>(t, gga, x3, true, S.wmat2);
+    if (x3) {
+      launch_wbasis(t, gga, false, S.wmat);
+      launch_wbasis(t, gga, true, S.wmat2);
+    } else {
+      launch_wmat<T>(t, gga, false, false, S.wmat);
+      launch_wmat<T>(t, gga, false, true, S.wmat2);
+    }
     if (E) (void)hipEventRecord(E[3], t);
     for (int li = 0; li < kLists; ++li) {
       const int tf = kTiles[li / kNumVariants], v = li % kNumVariants;

@@ -154,6 +154,9 @@ struct StageList {
 // ---- launchers (kernels live in xc_kernels.hip / xc_bf16x3.hip) -----------
 template <typename T>
 void launch_basis(hipStream_t s, bool grad, const StageList &L);
+// split-BF16 W over the basis items, the basis evaluated again (two: both
+// spins of open-shell units)
+void launch_wbasis(hipStream_t s, bool grad, bool two, const StageList &L);
 // tf = function tile (64, 96 or 128) shared by every unit of the list;
 // var = contraction variant: 0 (K step 16, 2 LDS stages), 1 (32, 1), 2 (32, 2)
 constexpr int kNumVariants = 3;

@@ -313,27 +313,51 @@ __device__ __forceinline__ void sh_eval(R x, R y, R z, R *C, R *Cx, R *Cy, R *Cz
 // ------------------------------------------------------------------- basis
 // Evaluation in the output precision T: FP64 for the plain build, FP32 for
 // the mixed builds (whose low-precision error is proportional to ||dP||).
+// Value and gradient of function m of a shell of angular momentum L at one
+// point (r0, r1: the contracted radial factor and its derivative factor);
+// every consumer of basis values forms them with this one expression.
 template <int L, bool GRAD, typename T>
-__device__ __forceinline__ void put_fn(long f0, int nf, T r0, T r1, T x, T y, T z, const T *C,
-                                       const T *Cx, const T *Cy, const T *Cz, T *phi, long Mc,
-                                       long cs, int p, bool wg) {
+__device__ __forceinline__ void fn_values(int m, T r0, T r1, T x, T y, T z, const T *C,
+                                          const T *Cx, const T *Cy, const T *Cz, T &v, T &gx,
+                                          T &gy, T &gz) {
   constexpr int o = L * L;
-  // global (not flat) stores: a flat store also counts against the LDS wait
-  // counter, so every wait for the staged shell data would wait for the
-  // stores issued before it
-#pragma unroll
-  for (int m = 0; m < 2 * L + 1; ++m) {
-    const long q = (f0 + (long)m * nf) * Mc + p;
-    const T sv = C[o + m];
-    gst(phi + q, sv * r0);
-    if (GRAD && wg) {
-      const T hv = sv * r1;
-      gst(phi + cs + q, x * hv + Cx[o + m] * r0);
-      gst(phi + 2 * cs + q, y * hv + Cy[o + m] * r0);
-      gst(phi + 3 * cs + q, z * hv + Cz[o + m] * r0);
-    }
+  const T sv = C[o + m];
+  v = sv * r0;
+  if (GRAD) {
+    const T hv = sv * r1;
+    gx = x * hv + Cx[o + m] * r0;
+    gy = y * hv + Cy[o + m] * r0;
+    gz = z * hv + Cz[o + m] * r0;
   }
 }
+
+// Emitter of the basis stage: stores a shell's values (and gradients, when
+// wg) at point p of the [4][NP][Mc] planes.  Global (not flat) stores: a flat
+// store also counts against the LDS wait counter, so every wait for the
+// staged shell data would wait for the stores issued before it.
+template <typename T>
+struct StorePhi {
+  T *phi;
+  long Mc, cs;
+  int p;
+  template <int L, bool GRAD>
+  __device__ __forceinline__ void emit(long f0, int nf, T r0, T r1, T x, T y, T z, const T *C,
+                                       const T *Cx, const T *Cy, const T *Cz, bool wg) const {
+#pragma unroll
+    for (int m = 0; m < 2 * L + 1; ++m) {
+      const long q = (f0 + (long)m * nf) * Mc + p;
+      T v, gx = T(0), gy = T(0), gz = T(0);
+      fn_values<L, false>(m, r0, r1, x, y, z, C, Cx, Cy, Cz, v, gx, gy, gz);
+      gst(phi + q, v);
+      if (GRAD && wg) {
+        fn_values<L, true>(m, r0, r1, x, y, z, C, Cx, Cy, Cz, v, gx, gy, gz);
+        gst(phi + cs + q, gx);
+        gst(phi + 2 * cs + q, gy);
+        gst(phi + 3 * cs + q, gz);
+      }
+    }
+  }
+};
 
 // A range of atoms' basis data staged in LDS for a basis item: the shells,
 // their exponents, norms and contraction coefficients in the evaluation
@@ -356,11 +380,10 @@ struct RangeLds {
 // primitives are all cut off on the wave's points skips the exponentials),
 // then added into the contractions in primitive order.
 constexpr int kPB = 4;
-template <int L, bool GRAD, typename T>
+template <int L, bool GRAD, typename T, class Put>
 __device__ __forceinline__ void shell_emit(const ShellD &sh, const T *ex, const T *nm, const T *ct,
                                            T x, T y, T z, T r2, const T *C, const T *Cx,
-                                           const T *Cy, const T *Cz, T *phi, long Mc, long cs,
-                                           int p, bool wg) {
+                                           const T *Cy, const T *Cz, const Put &put, bool wg) {
   const int ne = sh.n_exp, nu = sh.n_unc, nc = sh.n_contr, nf = nu + nc;
   T rc0[kMaxContr], rc1[kMaxContr];
 #pragma unroll
@@ -390,7 +413,7 @@ __device__ __forceinline__ void shell_emit(const ShellD &sh, const T *ex, const 
 #pragma unroll
     for (int i = 0; i < kPB; ++i)
       if (e0 + i < nu)
-        put_fn<L, GRAD>(sh.func_off + e0 + i, nf, r0[i], r1[i], x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p, wg);
+        put.template emit<L, GRAD>(sh.func_off + e0 + i, nf, r0[i], r1[i], x, y, z, C, Cx, Cy, Cz, wg);
 #pragma unroll
     for (int k = 0; k < kMaxContr; ++k)
       if (k < nc) {
@@ -408,15 +431,15 @@ __device__ __forceinline__ void shell_emit(const ShellD &sh, const T *ex, const 
 #pragma unroll
   for (int k = 0; k < kMaxContr; ++k)
     if (k < nc)
-      put_fn<L, GRAD>(sh.func_off + nu + k, nf, rc0[k], rc1[k], x, y, z, C, Cx, Cy, Cz, phi, Mc, cs, p, wg);
+      put.template emit<L, GRAD>(sh.func_off + nu + k, nf, rc0[k], rc1[k], x, y, z, C, Cx, Cy, Cz, wg);
 }
 
 // One atom's shells sb .. se-1 of the LDS image A at one point (e0, c0: the
 // range's first primitive and coefficient); grp: the point's group in the
 // item.
-template <int LA, bool GRAD, typename T>
+template <int LA, bool GRAD, bool ALLG = false, typename T, class Put>
 __device__ __forceinline__ void basis_atom(const RangeLds<T> &A, int sb, int se, int e0, int c0,
-                                           T x, T y, T z, T *phi, long Mc, long cs, int p, int grp) {
+                                           T x, T y, T z, const Put &put, int grp) {
   constexpr int NLM = (LA + 1) * (LA + 1);
   T C[NLM], Cx[NLM], Cy[NLM], Cz[NLM];
   sh_eval<LA, T>(x, y, z, C, Cx, Cy, Cz);
@@ -424,16 +447,16 @@ __device__ __forceinline__ void basis_atom(const RangeLds<T> &A, int sb, int se,
   for (int s = sb; s < se; ++s) {
     const int f = A.fl[s][grp];
     if (!(f & 1)) continue;  // no stage reads the shell's values on this group
-    const bool wg = GRAD && (f & 2);
+    const bool wg = GRAD && (ALLG || (f & 2));  // ALLG: gradients wherever values are formed
     const ShellD sh = A.sh[s];
     const T *ex = A.ex + (sh.exp_off - e0), *nm = A.nm + (sh.exp_off - e0);
     const T *ct = A.ct + (sh.contr_off - c0);
     switch (sh.l) {
-      case 0: shell_emit<0, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
-      case 1: if constexpr (LA >= 1) shell_emit<1, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
-      case 2: if constexpr (LA >= 2) shell_emit<2, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
-      case 3: if constexpr (LA >= 3) shell_emit<3, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
-      case 4: if constexpr (LA >= 4) shell_emit<4, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, phi, Mc, cs, p, wg); break;
+      case 0: shell_emit<0, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, put, wg); break;
+      case 1: if constexpr (LA >= 1) shell_emit<1, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, put, wg); break;
+      case 2: if constexpr (LA >= 2) shell_emit<2, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, put, wg); break;
+      case 3: if constexpr (LA >= 3) shell_emit<3, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, put, wg); break;
+      case 4: if constexpr (LA >= 4) shell_emit<4, GRAD>(sh, ex, nm, ct, x, y, z, r2, C, Cx, Cy, Cz, put, wg); break;
     }
   }
 }
@@ -595,77 +618,173 @@ __device__ __forceinline__ long item_at(const int *__restrict__ order, long L, l
 // on no global load.  Screened units (bscreen) walk their list of the
 // (block, range) pairs whose values a later stage reads, and write only the
 // shells read on each point's group; dense units write every shell.
+// A basis item staged: its first point, the range's first primitive and
+// coefficient, its atom count, and the point's coordinates.
+struct BasisItem {
+  long p;
+  int e0, c0, n_atoms;
+  double gx, gy, gz;
+};
+
+// Stages basis item `local` of unit U in A (every thread calls it; starts
+// and ends with a barrier).
+template <bool GRAD, typename T>
+__device__ __forceinline__ BasisItem stage_basis_item(const UnitD &U, long local, RangeLds<T> &A) {
+  const int tid = threadIdx.x;
+  long blk;
+  int rng;
+  if (U.bscreen) {
+    const uint32_t e = U.bitems[local];
+    blk = e >> 16;
+    rng = (int)(e & 0xffffu);
+  } else {
+    rng = (int)(local % U.n_ranges);
+    blk = local / U.n_ranges;
+  }
+  const RangeD R = U.ranges[rng];
+  BasisItem it;
+  it.e0 = R.exp_begin;
+  it.c0 = R.contr_begin;
+  it.n_atoms = R.n_atoms;
+  it.p = blk * kThreads + tid;
+  // the point's coordinates in FP64, loads in flight across the staging
+  it.gx = gld(U.gx + it.p);
+  it.gy = gld(U.gy + it.p);
+  it.gz = gld(U.gz + it.p);
+  __syncthreads();  // the previous item's reads of A are done
+  for (int i = tid; i < R.n_shells; i += kThreads) A.sh[i] = U.shells[R.shell_begin + i];
+  for (int i = tid; i < R.n_exp; i += kThreads) {
+    A.ex[i] = (T)gld(U.exps + it.e0 + i);
+    A.nm[i] = (T)gld(U.norms + it.e0 + i);
+  }
+  for (int i = tid; i < R.n_contr; i += kThreads) A.ct[i] = (T)gld(U.contr + it.c0 + i);
+  if (tid < R.n_atoms) {
+    const AtomD at = U.atoms[gld(U.ratoms + R.atom_begin + tid)];
+    A.ax[tid] = at.x;
+    A.ay[tid] = at.y;
+    A.az[tid] = at.z;
+    A.s0[tid] = at.shell_begin - R.shell_begin;
+    A.s1[tid] = at.shell_end - R.shell_begin;
+    A.lm[tid] = at.lmax;
+  }
+  if (tid < R.n_shells * (kThreads / kGroup)) {  // flag of (shell tid / 8, group tid % 8)
+    uint32_t f = GRAD ? 3u : 1u;
+    if (U.bscreen) {
+      const uint32_t *fw = U.bflags + 16 * local;
+      f = ((fw[tid >> 5] >> (tid & 31)) & 1u) |
+          (GRAD ? ((fw[8 + (tid >> 5)] >> (tid & 31)) & 1u) << 1 : 0u);
+    }
+    A.fl[tid / (kThreads / kGroup)][tid % (kThreads / kGroup)] = (uint8_t)f;
+  }
+  __syncthreads();
+  return it;
+}
+
+// The staged range's atoms in turn at the thread's point, each shell written
+// on the point's group handed to put (ALLG: with its gradients wherever its
+// values are).
+template <bool GRAD, bool ALLG, int LMAX, typename T, class Put>
+__device__ __forceinline__ void eval_basis_item(const RangeLds<T> &A, const BasisItem &it,
+                                                const Put &put) {
+  const int grp = threadIdx.x / kGroup;
+  for (int a = 0; a < it.n_atoms; ++a) {
+    const int sb = A.s0[a], se = A.s1[a];
+    bool any = false;
+    for (int s = sb; s < se; ++s) any = any || (A.fl[s][grp] & 1);
+    if (!any) continue;  // no stage reads this atom's values on the point's group
+    const T x = (T)(it.gx - A.ax[a]), y = (T)(it.gy - A.ay[a]), z = (T)(it.gz - A.az[a]);
+    // the kernel holds the registers of its largest case, so it is built
+    // for the largest angular momentum of the list (every atom's lmax <= LMAX)
+    switch (A.lm[a]) {
+      case 0: basis_atom<0, GRAD, ALLG>(A, sb, se, it.e0, it.c0, x, y, z, put, grp); break;
+      case 1: if constexpr (LMAX >= 1) basis_atom<1, GRAD, ALLG>(A, sb, se, it.e0, it.c0, x, y, z, put, grp); break;
+      case 2: if constexpr (LMAX >= 2) basis_atom<2, GRAD, ALLG>(A, sb, se, it.e0, it.c0, x, y, z, put, grp); break;
+      case 3: if constexpr (LMAX >= 3) basis_atom<3, GRAD, ALLG>(A, sb, se, it.e0, it.c0, x, y, z, put, grp); break;
+      default: if constexpr (LMAX >= 4) basis_atom<4, GRAD, ALLG>(A, sb, se, it.e0, it.c0, x, y, z, put, grp); break;
+    }
+  }
+}
+
 template <bool GRAD, typename T, int LMAX>
 __global__ __launch_bounds__(kThreads) void k_basis(const UnitD *__restrict__ units, int nu,
                                                     const long *__restrict__ base,
                                                     const int *__restrict__ order, long total) {
   __shared__ RangeLds<T> A;
-  const int tid = threadIdx.x;
   for (long L = blockIdx.x; L < total; L += gridDim.x) {
-    const long it = item_at(order, L, total);
-    const int u = find_unit(base, nu, it);
+    const long item = item_at(order, L, total);
+    const int u = find_unit(base, nu, item);
     const UnitD &U = units[u];
-    const long local = it - base[u];
-    long blk;
-    int rng;
-    if (U.bscreen) {
-      const uint32_t e = U.bitems[local];
-      blk = e >> 16;
-      rng = (int)(e & 0xffffu);
-    } else {
-      rng = (int)(local % U.n_ranges);
-      blk = local / U.n_ranges;
-    }
-    const RangeD R = U.ranges[rng];
-    const int e0 = R.exp_begin, c0 = R.contr_begin;
-    const long p = blk * kThreads + tid;
-    // the point's coordinates in FP64, loads in flight across the staging
-    const double gx = gld(U.gx + p), gy = gld(U.gy + p), gz = gld(U.gz + p);
-    __syncthreads();  // the previous item's reads of A are done
-    for (int i = tid; i < R.n_shells; i += kThreads) A.sh[i] = U.shells[R.shell_begin + i];
-    for (int i = tid; i < R.n_exp; i += kThreads) {
-      A.ex[i] = (T)gld(U.exps + e0 + i);
-      A.nm[i] = (T)gld(U.norms + e0 + i);
-    }
-    for (int i = tid; i < R.n_contr; i += kThreads) A.ct[i] = (T)gld(U.contr + c0 + i);
-    if (tid < R.n_atoms) {
-      const AtomD at = U.atoms[gld(U.ratoms + R.atom_begin + tid)];
-      A.ax[tid] = at.x;
-      A.ay[tid] = at.y;
-      A.az[tid] = at.z;
-      A.s0[tid] = at.shell_begin - R.shell_begin;
-      A.s1[tid] = at.shell_end - R.shell_begin;
-      A.lm[tid] = at.lmax;
-    }
-    if (tid < R.n_shells * (kThreads / kGroup)) {  // flag of (shell tid / 8, group tid % 8)
-      uint32_t f = GRAD ? 3u : 1u;
-      if (U.bscreen) {
-        const uint32_t *fw = U.bflags + 16 * local;
-        f = ((fw[tid >> 5] >> (tid & 31)) & 1u) |
-            (GRAD ? ((fw[8 + (tid >> 5)] >> (tid & 31)) & 1u) << 1 : 0u);
-      }
-      A.fl[tid / (kThreads / kGroup)][tid % (kThreads / kGroup)] = (uint8_t)f;
-    }
-    __syncthreads();
-    T *phi = (T *)U.phi;
-    const long cs = (long)U.NP * U.Mc;
-    const int grp = tid / kGroup;
-    for (int a = 0; a < R.n_atoms; ++a) {
-      const int sb = A.s0[a], se = A.s1[a];
-      bool any = false;
-      for (int s = sb; s < se; ++s) any = any || (A.fl[s][grp] & 1);
-      if (!any) continue;  // no stage reads this atom's values on the point's group
-      const T x = (T)(gx - A.ax[a]), y = (T)(gy - A.ay[a]), z = (T)(gz - A.az[a]);
-      // the kernel holds the registers of its largest case, so it is built
-      // for the largest angular momentum of the list (every atom's lmax <= LMAX)
-      switch (A.lm[a]) {
-        case 0: basis_atom<0, GRAD>(A, sb, se, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-        case 1: if constexpr (LMAX >= 1) basis_atom<1, GRAD>(A, sb, se, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-        case 2: if constexpr (LMAX >= 2) basis_atom<2, GRAD>(A, sb, se, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-        case 3: if constexpr (LMAX >= 3) basis_atom<3, GRAD>(A, sb, se, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
-        default: if constexpr (LMAX >= 4) basis_atom<4, GRAD>(A, sb, se, e0, c0, x, y, z, phi, U.Mc, cs, p, grp); break;
+    const BasisItem it = stage_basis_item<GRAD>(U, item - base[u], A);
+    const StorePhi<T> put{(T *)U.phi, U.Mc, (long)U.NP * U.Mc, (int)it.p};
+    eval_basis_item<GRAD, false, LMAX>(A, it, put);
+  }
+}
+
+// Emitter of the split-BF16 W pass: W = a Phi + b . grad Phi of a shell's
+// functions at point p (FP64 from the FP32 values, as a pass reading stored
+// values forms it), stored as bf16 hi / lo planes for NS spins.
+template <int NS>
+struct StoreWSplit {
+  uint16_t *wh[2], *wl[2];
+  long Mc;
+  int p;
+  double a[2], b0[2], b1[2], b2[2];
+  template <int L, bool GRAD>
+  __device__ __forceinline__ void emit(long f0, int nf, float r0, float r1, float x, float y,
+                                       float z, const float *C, const float *Cx, const float *Cy,
+                                       const float *Cz, bool) const {
+#pragma unroll
+    for (int m = 0; m < 2 * L + 1; ++m) {
+      const long q = (f0 + (long)m * nf) * Mc + p;
+      float v, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+      fn_values<L, GRAD>(m, r0, r1, x, y, z, C, Cx, Cy, Cz, v, gx, gy, gz);
+#pragma unroll
+      for (int sp = 0; sp < NS; ++sp) {
+        const double w = GRAD ? a[sp] * (double)v + b0[sp] * (double)gx + b1[sp] * (double)gy +
+                                    b2[sp] * (double)gz
+                              : a[sp] * (double)v;
+        __bf16 h, l;
+        split_bf16((float)w, h, l);
+        gst(wh[sp] + q, __builtin_bit_cast(uint16_t, h));
+        gst(wl[sp] + q, __builtin_bit_cast(uint16_t, l));
       }
     }
+  }
+};
+
+// Split-BF16 W pass over the basis items: the basis functions evaluated again
+// (FP32, the same expressions as the basis stage, so the same values) instead
+// of read back, W written for the shells whose values the basis stage writes
+// (TWO: both spins of an open-shell unit).  The split-BF16 potential reads W
+// only for the functions of a tile's list (with wide point splits, also the
+// zeros of the other functions of its chunks, which are written here too).
+template <bool GRAD, bool TWO, int LMAX>
+__global__ __launch_bounds__(kThreads) void k_wbasis(const UnitD *__restrict__ units, int nu,
+                                                     const long *__restrict__ base,
+                                                     const int *__restrict__ order, long total) {
+  __shared__ RangeLds<float> A;
+  constexpr int NS = TWO ? 2 : 1;
+  for (long L = blockIdx.x; L < total; L += gridDim.x) {
+    const long item = item_at(order, L, total);
+    const int u = find_unit(base, nu, item);
+    const UnitD &U = units[u];
+    const BasisItem it = stage_basis_item<GRAD>(U, item - base[u], A);
+    const long Mc = U.Mc, cs = (long)U.NP * Mc;
+    StoreWSplit<NS> put;
+    put.Mc = Mc;
+    put.p = (int)it.p;
+#pragma unroll
+    for (int sp = 0; sp < NS; ++sp) {
+      uint16_t *w = (uint16_t *)(sp ? U.W2 : U.W);
+      const double *cf = sp ? U.coef2 : U.coef;
+      put.wh[sp] = w;
+      put.wl[sp] = w + cs;
+      put.a[sp] = gld(cf + it.p);
+      put.b0[sp] = GRAD ? gld(cf + Mc + it.p) : 0.0;
+      put.b1[sp] = GRAD ? gld(cf + 2 * Mc + it.p) : 0.0;
+      put.b2[sp] = GRAD ? gld(cf + 3 * Mc + it.p) : 0.0;
+    }
+    eval_basis_item<GRAD, true, LMAX>(A, it, put);
   }
 }
 
@@ -1451,6 +1570,29 @@ static void basis_l(hipStream_t s, bool grad, const StageList &L) {
     hipLaunchKernelGGL((k_basis<true, T, LMAX>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
   else
     hipLaunchKernelGGL((k_basis<false, T, LMAX>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
+template <int LMAX>
+static void wbasis_l(hipStream_t s, bool grad, bool two, const StageList &L) {
+  dim3 g(persistent_grid(L.total, 8));
+  if (grad && two)
+    hipLaunchKernelGGL((k_wbasis<true, true, LMAX>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  else if (grad)
+    hipLaunchKernelGGL((k_wbasis<true, false, LMAX>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  else if (two)
+    hipLaunchKernelGGL((k_wbasis<false, true, LMAX>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+  else
+    hipLaunchKernelGGL((k_wbasis<false, false, LMAX>), g, dim3(kThreads), 0, s, L.units, L.nu, L.base, L.order, L.total);
+}
+
+void launch_wbasis(hipStream_t s, bool grad, bool two, const StageList &L) {
+  if (!L.total) return;
+  switch (L.lmax) {
+    case 0: case 1: wbasis_l<1>(s, grad, two, L); break;
+    case 2: wbasis_l<2>(s, grad, two, L); break;
+    case 3: wbasis_l<3>(s, grad, two, L); break;
+    default: wbasis_l<4>(s, grad, two, L); break;
+  }
 }
 
 template <typename T>
