@@ -448,7 +448,16 @@ __device__ __forceinline__ void basis_atom(const RangeLds<T> &A, int sb, int se,
     const int f = A.fl[s][grp];
     if (!(f & 1)) continue;  // no stage reads the shell's values on this group
     const bool wg = GRAD && (ALLG || (f & 2));  // ALLG: gradients wherever values are formed
-    const ShellD sh = A.sh[s];
+    // every lane reads the same shell: its fields are uniform over the wave,
+    // so the loops over its primitives and functions run on scalar counters
+    ShellD sh = A.sh[s];
+    sh.l = __builtin_amdgcn_readfirstlane(sh.l);
+    sh.n_exp = __builtin_amdgcn_readfirstlane(sh.n_exp);
+    sh.n_unc = __builtin_amdgcn_readfirstlane(sh.n_unc);
+    sh.n_contr = __builtin_amdgcn_readfirstlane(sh.n_contr);
+    sh.exp_off = __builtin_amdgcn_readfirstlane(sh.exp_off);
+    sh.contr_off = __builtin_amdgcn_readfirstlane(sh.contr_off);
+    sh.func_off = __builtin_amdgcn_readfirstlane(sh.func_off);
     const T *ex = A.ex + (sh.exp_off - e0), *nm = A.nm + (sh.exp_off - e0);
     const T *ct = A.ct + (sh.contr_off - c0);
     switch (sh.l) {
@@ -687,15 +696,16 @@ template <bool GRAD, bool ALLG, int LMAX, typename T, class Put>
 __device__ __forceinline__ void eval_basis_item(const RangeLds<T> &A, const BasisItem &it,
                                                 const Put &put) {
   const int grp = threadIdx.x / kGroup;
-  for (int a = 0; a < it.n_atoms; ++a) {
-    const int sb = A.s0[a], se = A.s1[a];
+  const int na = __builtin_amdgcn_readfirstlane(it.n_atoms);
+  for (int a = 0; a < na; ++a) {
+    const int sb = __builtin_amdgcn_readfirstlane(A.s0[a]), se = __builtin_amdgcn_readfirstlane(A.s1[a]);
     bool any = false;
     for (int s = sb; s < se; ++s) any = any || (A.fl[s][grp] & 1);
     if (!any) continue;  // no stage reads this atom's values on the point's group
     const T x = (T)(it.gx - A.ax[a]), y = (T)(it.gy - A.ay[a]), z = (T)(it.gz - A.az[a]);
     // the kernel holds the registers of its largest case, so it is built
     // for the largest angular momentum of the list (every atom's lmax <= LMAX)
-    switch (A.lm[a]) {
+    switch (__builtin_amdgcn_readfirstlane(A.lm[a])) {
       case 0: basis_atom<0, GRAD, ALLG>(A, sb, se, it.e0, it.c0, x, y, z, put, grp); break;
       case 1: if constexpr (LMAX >= 1) basis_atom<1, GRAD, ALLG>(A, sb, se, it.e0, it.c0, x, y, z, put, grp); break;
       case 2: if constexpr (LMAX >= 2) basis_atom<2, GRAD, ALLG>(A, sb, se, it.e0, it.c0, x, y, z, put, grp); break;
