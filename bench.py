@@ -20,6 +20,12 @@ them in every build).
                  (gpu_fp64_vs_cpu).
   warm_cache   : the same builds with the basis values kept in HBM across
                  builds of the geometry.
+  roofline     : the density contraction (the larger contraction) timed on
+                 one lane: its algorithmic bytes per launch (P sub-block in
+                 FP64, values and gradients in FP32) over the launch time,
+                 against 8 TB/s when its intensity (flops per algorithmic
+                 byte) is below the ridge of its MFMA type, else its flops
+                 against the MFMA peak; the other rate is reported beside it.
 Prints one JSON line.
 """
 import argparse
