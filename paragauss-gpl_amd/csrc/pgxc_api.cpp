@@ -224,7 +224,14 @@ struct Plan {
   long model_version = -1;
   int *d_ctr = nullptr;  // queue counters of every contraction list
   long nctr = 0;
-  std::vector<std::pair<int *, long>> hzero;  // H partials, cleared at the start
+  // H partials, cleared at the start: (buffer, ints, lane); lane 1's on the
+  // second stream, overlapping the first lane's first stages
+  struct HZero {
+    int *p;
+    long n;
+    int lane;
+  };
+  std::vector<HZero> hzero;
   // item log: features of every contraction item (row-major), and per list
   // the device log and the list's first feature row
   std::vector<double> feat;
@@ -664,7 +671,9 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       Q.nsplit = c->nsplit * lanes_of(c);  // the fold sums the partials of every lane
       Q.incremental = incr[m];
       packs.push_back(Q);
-      plan->hzero.push_back({(int *)Q.Hpart, 2L * lanes_of(c) * c->nsplit * c->NP * c->NP});
+      for (int l = 0; l < lanes_of(c); ++l)
+        plan->hzero.push_back({(int *)(Q.Hpart + (size_t)l * c->nsplit * c->NP * c->NP),
+                               2L * c->nsplit * c->NP * c->NP, l});
     }
   }
   const int npack = (int)packs.size();
@@ -767,13 +776,18 @@ static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent
   const bool x3 = pl.precision == PGXC_PREC_BF16X3;
   const bool two = pl.nlanes > 1 && pl.steps.size() > 1;
   if (pl.nctr) launch_zero_ints(s, pl.d_ctr, pl.nctr);
+  // potential items add into H: each lane's partials cleared on its stream
   if (!pl.energy)
-    for (auto &z : pl.hzero) launch_zero_ints(s, z.first, z.second);  // potential items add into H
+    for (auto &z : pl.hzero)
+      if (z.lane == 0 || !two) launch_zero_ints(s, z.p, z.n);
   launch_prep(s, pl.d_pack, pl.nmol, pl.d_prep_base, pl.prep_total,
               x3 ? 2 : std::is_same<T, double>::value ? 0 : 1, false);
   if (two) {
     (void)hipEventRecord(fork, s);
     (void)hipStreamWaitEvent(s2, fork, 0);
+    if (!pl.energy)
+      for (auto &z : pl.hzero)
+        if (z.lane != 0) launch_zero_ints(s2, z.p, z.n);
   }
   for (size_t k = 0; k < pl.steps.size(); ++k) {
     const StepPlan &S = pl.steps[k];
