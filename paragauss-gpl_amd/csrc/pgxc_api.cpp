@@ -220,6 +220,7 @@ struct Plan {
   long order_version = -1;
   int sched_mode = 0, log = 0;
   int energy = 0;  // energy terms only: no W, potential, fold or packing
+  int wbasis = 0;  // split-BF16 W from the basis evaluated again (k_wbasis)
   const pgxc_costmodel *model = nullptr;
   long model_version = -1;
   int *d_ctr = nullptr;  // queue counters of every contraction list
@@ -568,7 +569,11 @@ static int get_plan(std::vector<pgxc_ctx *> &mols, uint32_t functional, int prec
       pend[k].push_back(stage(fu, fm, K_FUNC, 0, k));
     }
     // W: split-BF16 builds evaluate the basis again over the basis items
-    const ItemKind wkind = precision == PGXC_PREC_BF16X3 ? K_WBASIS : K_WMAT;
+    // (test switch PGXC_W_READ=1: they read the stored values back instead,
+    // the same bits)
+    const bool wb = precision == PGXC_PREC_BF16X3 && !getenv("PGXC_W_READ");
+    plan->wbasis = wb ? 1 : 0;
+    const ItemKind wkind = wb ? K_WBASIS : K_WMAT;
     {  // closed-shell units; the open-shell ones in wmat2 below
       std::vector<UnitD> wu;
       std::vector<int> wm;
@@ -814,12 +819,12 @@ static void launch_plan(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent
     }
     launch_functional(t, gga, S.func);
     launch_functional_spin(t, gga, S.func2);
-    if (x3) {
+    if (pl.wbasis) {
       launch_wbasis(t, gga, false, S.wmat);
       launch_wbasis(t, gga, true, S.wmat2);
     } else {
-      launch_wmat<T>(t, gga, false, false, S.wmat);
-      launch_wmat<T>(t, gga, false, true, S.wmat2);
+      launch_wmat<T>(t, gga, x3, false, S.wmat);
+      launch_wmat<T>(t, gga, x3, true, S.wmat2);
     }
     if (E) (void)hipEventRecord(E[3], t);
     for (int li = 0; li < kLists; ++li) {

@@ -254,6 +254,33 @@ def odd_molecule():
     return b, g, C, occ
 
 
+@pytest.mark.parametrize("functional", [PBE, LDA])
+def test_split_w_from_basis_evaluation_equals_reading(functional, monkeypatch):
+    """The split-BF16 W pass evaluates the basis functions again instead of
+    reading the stored values and gradients back; the test switch
+    PGXC_W_READ=1 makes a context's plans read them.  Incremental split-BF16
+    builds on several chunks and both lanes: the same bits either way."""
+    els, xyz = systems.water_cluster(4, seed=3)
+    b = basis.seeded_basis(els, xyz, seed=4, shell_set="medium")
+    g = grid.molecular_grid(els, xyz, nrad=40, nang=302, partition="gpu")
+    C, occ = systems.random_orbitals(b.nbas, 20, seed=5)
+    C1 = systems.perturbed_orbitals(C, 1e-3, seed=6)
+    out = {}
+    for read in (False, True):
+        if read:
+            monkeypatch.setenv("PGXC_W_READ", "1")
+        else:
+            monkeypatch.delenv("PGXC_W_READ", raising=False)
+        with XCBuilder(b, g, chunk_points=32768) as xc:
+            assert xc.layout()["chunks"] > 2
+            xc.set_orbitals(C, occ)
+            xc.build(functional, FP64)
+            xc.set_orbitals(C1, occ, incremental=True)
+            out[read] = xc.build(functional, BF16X3)
+    assert np.array_equal(out[False][0], out[True][0])
+    assert out[False][1] == out[True][1] and out[False][2] == out[True][2]
+
+
 def test_unwritten_basis_blocks_with_wide_point_splits(monkeypatch):
     """A chunk above 256 Ki points makes a point split of the potential span
     more than 32 tiles, past what its per-function tile masks hold: the

@@ -105,6 +105,23 @@ def test_open_shell_split_bf16(case):
           q_tol=rel * abs(r[2]))
 
 
+def test_open_shell_split_w_from_basis_evaluation_equals_reading(case, monkeypatch):
+    """Both spins' split-BF16 W from one evaluation of the basis (the default)
+    or from the stored values read back (PGXC_W_READ=1): the same bits."""
+    b, g, C, occ = case
+    out = {}
+    for read in (False, True):
+        if read:
+            monkeypatch.setenv("PGXC_W_READ", "1")
+        else:
+            monkeypatch.delenv("PGXC_W_READ", raising=False)
+        with XCBuilder(b, g) as xc:
+            xc.set_orbitals(C, occ)
+            out[read] = xc.build(PBE, BF16X3)
+    assert np.array_equal(out[False][0], out[True][0])
+    assert out[False][1] == out[True][1] and out[False][2] == out[True][2]
+
+
 def test_open_shell_incremental_and_restart(case):
     """A full build, then an incremental one with changed spin densities:
     the FP64 bar against the oracle's full build of the new densities; a
