@@ -87,8 +87,9 @@ struct UnitD {
   // nonzero on the group (a run holding one of its functions is live there).
   // sig8: per 32-point group (from the unit's first), nw8 words of run
   // flags, the runs not cut off on the group.  wbits: the same shape, the
-  // runs the potential contraction reads on the group (W is written there,
-  // 0 where cut off).
+  // runs the potential contraction reads on the group (the FP64/FP32 W pass
+  // writes W there, 0 where cut off; the split-BF16 W is written for the
+  // tile's functions, the only ones its potential reads).
   const uint32_t *ditems, *pitems;
   const int *tl_off, *su_off, *tf_off;
   const uint16_t *tl, *su, *tfl;
