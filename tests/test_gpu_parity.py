@@ -145,6 +145,44 @@ def test_incremental_mixed_error_tracks_dP(h2o, norm, prec):
     check(inc, ref, e_tol=1e-9 + k * norm, v_tol=1e-10 + k * norm, q_tol=1e-9 + k * norm)
 
 
+@pytest.mark.parametrize("prec", [FP32, BF16X3])
+def test_long_incremental_chain_and_refresh(h2o, prec):
+    """A long SCF on incremental mixed builds: 30 steps, each a new density
+    ||dP||_F = 1e-3 from the last.  Every step adds its increment's own
+    low-precision error, so against a full FP64 build of the same density the
+    error stays within the single-step bar of the summed ||dP||_F (the
+    refresh budget a caller keeps); a full FP64 build (the refresh) resets
+    the state, after which one incremental step is back at the single-step
+    bar."""
+    b, g, C, occ = h2o
+    k = 1e-3 if prec == FP32 else 5e-3  # as test_incremental_mixed_error_tracks_dP
+    norm = 1e-3
+    with XCBuilder(b, g) as xc, XCBuilder(b, g) as full:
+        xc.set_orbitals(C, occ)
+        xc.build(PBE, FP64)
+        cur, total, worst = C, 0.0, 0.0
+        for step in range(30):
+            cur = systems.perturbed_orbitals(cur, norm, seed=100 + step)
+            total += norm
+            xc.set_orbitals(cur, occ, incremental=True)
+            ham, e, q = xc.build(PBE, prec)
+            full.set_orbitals(cur, occ)
+            hr, er, qr = full.build(PBE, FP64)
+            de, dv = abs(e - er), np.max(np.abs(ham - hr))
+            assert de <= 1e-9 + k * total and dv <= 1e-10 + k * total
+            worst = max(worst, de / total)
+        print("prec=%d: worst |dE| / summed ||dP||_F over 30 steps %.2e" % (prec, worst), flush=True)
+        xc.set_orbitals(cur, occ)  # refresh: a full FP64 build of the current density
+        xc.build(PBE, FP64)
+        nxt = systems.perturbed_orbitals(cur, norm, seed=999)
+        xc.set_orbitals(nxt, occ, incremental=True)
+        ham, e, q = xc.build(PBE, prec)
+        full.set_orbitals(nxt, occ)
+        hr, er, qr = full.build(PBE, FP64)
+        assert abs(e - er) <= 1e-9 + k * norm
+        assert np.max(np.abs(ham - hr)) <= 1e-10 + k * norm
+
+
 def test_water_cluster_medium_basis_fp64():
     els, xyz = systems.water_cluster(4, seed=3)
     b = basis.seeded_basis(els, xyz, seed=4, shell_set="medium")
